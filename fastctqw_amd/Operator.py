@@ -1,0 +1,179 @@
+"""Walk operator U(t) = expm(-i H t) (mirror of FastCTQW/Operator.py).
+
+Conventions kept from the reference:
+* H = -gamma*A or -gamma*(D - A) (Operator.py:49-53), A_exp = -1j*H*t
+  (:55-56), dtype complex64 for float32/complex64 data and complex128 for
+  float64/complex128 (:27-32), TypeError otherwise;
+* ValueError for a non-square / non-symmetric A (:80-90) and a non-Hermitian
+  H (:92-103);
+* device "cpu" -> scipy.linalg.expm (:74-76).
+
+Deliberate differences (drop-in supersets):
+* device "cuda" (also "gpu", "hip") runs the gfx950 engine with the engine
+  class matching the operator dtype; the reference always picked
+  ExpMatComplex64 (:40), truncating complex128 input;
+* on the GPU, A_exp is uploaded once, U stays in device memory, and
+  ``getOperator`` copies it to the host only when asked (:165-166);
+  ``applyTo`` computes U psi on the device with FP64 accumulation.
+"""
+from __future__ import annotations
+
+import numpy as np
+from numpy.typing import NDArray
+
+_GPU_DEVICES = ("cuda", "gpu", "hip")
+
+
+def _expm_cpu(a):
+    from scipy.linalg import expm
+
+    return expm(a)
+
+
+class Operator:
+    def __init__(self, data, gamma: float = 1, laplacian: bool = False, device: str = "cpu",
+                 dtype=np.complex64) -> None:
+        self._expMator = None
+        self._expmMat = None
+        self._on_device = False
+        if isinstance(data, np.ndarray):
+            if not self._is_adjacency_matrix(data):
+                raise ValueError("A is not a adj matrix")
+            self._n = data.shape[0]
+            self._gamma = gamma
+            self._laplacian = laplacian
+            self._device = device
+            if data.dtype in (np.float32, np.complex64):
+                self._dtype = np.complex64
+            elif data.dtype in (np.float64, np.complex128):
+                self._dtype = np.complex128
+            else:
+                raise TypeError(f"A.dtype is error : {data.dtype}")
+            self._hamiltonian = self._buildHamiltonian(data, gamma)
+            if not self._is_hermitian(self._hamiltonian):
+                raise ValueError("_hamiltonian is not a hamiltonian matrix")
+            self._expMator = self._initExpMator(self._n, self._dtype)
+        elif isinstance(data, (int, np.integer)):
+            self._n = int(data)
+            self._device = device
+            self._dtype = np.dtype(dtype).type
+            self._expMator = self._initExpMator(self._n, self._dtype)
+        else:
+            raise TypeError(f"data must be an ndarray or int, got {type(data)!r}")
+
+    def free(self):
+        if self._expMator is not None:
+            self._expMator.free()
+
+    def _buildHamiltonian(self, A: np.ndarray, gamma: float) -> NDArray:
+        if self._laplacian:
+            return -gamma * np.asarray(np.diag(A.sum(axis=1)) - A, dtype=self._dtype)
+        return -gamma * np.asarray(A, self._dtype)
+
+    def _buildUnitory(self, time):
+        return np.asarray(-1j * self._hamiltonian * time, dtype=self._dtype)
+
+    def _initExpMator(self, n: int, dtype=np.complex64):
+        if self._device == "cpu":
+            return None
+        if self._device in _GPU_DEVICES:
+            from .fastexpm import ExpMatComplex64, ExpMatComplex128
+
+            if dtype == np.complex64:
+                return ExpMatComplex64(n)
+            if dtype == np.complex128:
+                return ExpMatComplex128(n)
+            raise TypeError(f"self._phaseFactor.dtype: {dtype} is error")
+        raise ValueError(f"unknown device {self._device!r}")
+
+    def buildExpmOperator(self, time: float) -> None:
+        """U = e^{-iHt}."""
+        if self._device == "cpu":
+            self._expmMat = _expm_cpu(self._buildUnitory(time))
+            self._on_device = False
+        else:
+            self._expMator.upload(self._buildUnitory(time))
+            self._expMator.run_resident()
+            self._expmMat = None
+            self._on_device = True
+
+    def applyTo(self, psi0: np.ndarray) -> np.ndarray:
+        """U psi0 in the operator dtype; on the GPU with FP64 accumulation."""
+        if self._on_device:
+            return self._expMator.apply(psi0).astype(self._dtype, copy=False)
+        return self.getOperator() @ psi0
+
+    def lastRunStats(self) -> dict:
+        return self._expMator.stats() if self._expMator is not None else {}
+
+    def _is_adjacency_matrix(self, A: np.ndarray) -> bool:
+        if A.ndim != 2:
+            print(f"Error: Matrix is not 2D. Dimensions: {A.ndim}")
+            return False
+        if A.shape[0] != A.shape[1]:
+            print(f"Error: Matrix is not square. Shape: {A.shape}")
+            return False
+        if not np.array_equal(A, A.T):
+            print("Error: Matrix is not symmetric (expected for undirected graph).")
+            return False
+        return True
+
+    def _is_hermitian(self, hamiltonian) -> bool:
+        return np.allclose(hamiltonian, hamiltonian.conj().T)
+
+    @classmethod
+    def from_networkx_graph(cls, nx_graph, gamma: float = 1, laplacian: bool = False,
+                            device: str = "cpu") -> "Operator":
+        import networkx as nx
+
+        return cls(nx.to_numpy_array(nx_graph), gamma, laplacian, device)
+
+    @classmethod
+    def from_numpy_array(cls, A: np.ndarray, gamma: float = 1, laplacian: bool = False,
+                         device: str = "cpu") -> "Operator":
+        return cls(A, gamma, laplacian, device)
+
+    @classmethod
+    def from_num_nodes(cls, n: int, dtype=np.complex64, device: str = "cpu"):
+        return cls(n, device=device, dtype=dtype)
+
+    def reset_numpy_array(self, A: np.ndarray, gamma: float = 1, laplacian: bool = False):
+        if A.dtype in (np.float32, np.complex64):
+            dtype = np.complex64
+        elif A.dtype in (np.float64, np.complex128):
+            dtype = np.complex128
+        else:
+            raise TypeError(f"A.dtype is error : {A.dtype}")
+        if not self._is_adjacency_matrix(A):
+            raise ValueError("A is not a adj matrix")
+        changed = dtype != getattr(self, "_dtype", None) or A.shape[0] != self._n
+        self._dtype = dtype
+        self._n = A.shape[0]
+        self._gamma = gamma
+        self._laplacian = laplacian
+        self._hamiltonian = self._buildHamiltonian(A, gamma)
+        if not self._is_hermitian(self._hamiltonian):
+            raise ValueError("_hamiltonian is not a hamiltonian matrix")
+        if self._device != "cpu" and (self._expMator is None or changed):
+            if self._expMator is not None:
+                self._expMator.free()
+            self._expMator = self._initExpMator(self._n, self._dtype)
+        self._expmMat = None
+        self._on_device = False
+
+    def reset_networkx_graph(self, nx_graph, gamma: float = 1, laplacian: bool = False):
+        import networkx as nx
+
+        self.reset_numpy_array(nx.to_numpy_array(nx_graph), gamma, laplacian)
+
+    def setHamiltonian(self, hamiltonian: NDArray):
+        self._hamiltonian = hamiltonian
+
+    def setexpmOperator(self, newexpmMat):
+        self._expmMat = newexpmMat
+        self._on_device = False
+
+    def getOperator(self):
+        if self._expmMat is None and self._on_device:
+            self._expmMat = self._expMator.download()
+        return self._expmMat

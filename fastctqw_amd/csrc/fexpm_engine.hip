@@ -1,0 +1,872 @@
+// Scaling-and-squaring Pade matrix exponential on one MI355X (gfx950).
+//
+// Pipeline of one run (stage names follow the reference's declared buffers
+// d_A, d_A2, d_A4, d_A6, d_u*, d_v*, d_mu, d_nrmA,
+// include/MatrixExpCalculator.h:52-61):
+//   1. analysis (FP64): trace -> mu, exact 1-norms of A and A - mu I,
+//      structure flags (purely imaginary + exactly symmetric), non-finite;
+//   2. degree m and scaling s from the 1-norm (Higham 2005, Alg. 2.3);
+//   3. A <- (A - mu I) / 2^s in planar working precision W;
+//   4. Pade products A2, A4, A6 and U, V on MFMA;
+//   5. FP64 LU with partial pivoting of [V - U | V + U], back substitution;
+//   6. s squarings on MFMA;
+//   7. U = e^mu X written in the caller's layout.
+//
+// CTQW inputs A = -i t H (H real symmetric) are detected in step 1 and take
+// a structured path: A = i G with G real symmetric, every Pade product is a
+// real symmetric product (half the tiles), and every squaring of the
+// complex-symmetric X = Xr + i Xi is X X^T computed by the 3M rule with
+// three half products.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/fastexpm.h"
+#include "fexpm_gemm.hpp"
+#include "fexpm_kernels.hpp"
+
+namespace fexpm {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define FX_HIP(x)                                                                                  \
+  do {                                                                                             \
+    hipError_t e_ = (x);                                                                           \
+    if (e_ != hipSuccess)                                                                          \
+      throw ::fexpm::Error(e_ == hipErrorOutOfMemory ? FEXPM_ENOMEM : FEXPM_EHIP,                  \
+                           std::string(#x) + ": " + hipGetErrorString(e_));                        \
+  } while (0)
+
+#define FX_LAUNCH() FX_HIP(hipGetLastError())
+
+// theta_m (double precision) and Pade coefficients b_j, Higham 2005.
+static const int kDegrees[5] = {3, 5, 7, 9, 13};
+static const double kTheta[5] = {1.495585217958292e-2, 2.539398330063230e-1, 9.504178996162932e-1,
+                                 2.097847961257068e0, 5.371920351148152e0};
+
+static double pade_b(int m, int j) {
+  // b_j = (2m-j)! / (j! (m-j)!), exact in double for m <= 13
+  long double num = 1.0L;
+  for (int k = m - j + 1; k <= 2 * m - j; ++k) num *= (long double)k;  // (2m-j)!/(m-j)!
+  long double den = 1.0L;
+  for (int k = 2; k <= j; ++k) den *= (long double)k;
+  return (double)(num / den);
+}
+
+static void choose_degree(double norm, int& m, int& s) {
+  for (int q = 0; q < 4; ++q) {
+    if (norm <= kTheta[q]) {
+      m = kDegrees[q];
+      s = 0;
+      return;
+    }
+  }
+  m = 13;
+  s = std::max(0, (int)std::ceil(std::log2(norm / kTheta[4])));
+  while (norm / std::ldexp(1.0, s) > kTheta[4]) ++s;
+}
+
+static inline int64_t round_up(int64_t x, int64_t r) { return (x + r - 1) / r * r; }
+
+// ---------------------------------------------------------------------------
+template <typename T> static T* dalloc(size_t count) {
+  void* p = nullptr;
+  if (count == 0) count = 1;
+  FX_HIP(hipMalloc(&p, count * sizeof(T)));
+  FX_HIP(hipMemset(p, 0, count * sizeof(T)));
+  return static_cast<T*>(p);
+}
+static void dfree(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+template <typename W> struct Plane2 {
+  W* r = nullptr;
+  W* i = nullptr;
+};
+
+struct PipelineBase {
+  virtual ~PipelineBase() {}
+  virtual void run(const void* d_a, void* d_u, hipStream_t st, fexpm_stats& stats, int opt_structure) = 0;
+  virtual void apply(const double2* d_psi0, int nstates, double2* d_out, hipStream_t st) = 0;
+  virtual bool has_result() const = 0;
+};
+
+template <typename W> struct Pipeline : PipelineBase {
+  int n, dtype;
+  int64_t np, ld;        // padded order / leading dimension of W planes
+  int64_t ldl;           // leading dimension of the FP64 system [M | B]
+  Plane2<W> a, p2, p4, p6, t1, t2, u, v, x0, x1;
+  double* lr = nullptr;  // FP64 system, np x 2np
+  double* li = nullptr;
+  double* tr = nullptr;  // transposed solved block row (2np x 64)
+  double* ti = nullptr;
+  int* ipiv = nullptr;
+  PanelCand* cand = nullptr;  // 2 x gmax
+  double* top = nullptr;      // 2 x 2*LU_NB
+  int gmax = 0;
+  double* colpart = nullptr;
+  double* off = nullptr;
+  double2* diag = nullptr;
+  int* flags = nullptr;
+  Scalars* scal = nullptr;
+  hipEvent_t ev[8];
+  bool events = false;
+  // result of the last run (for apply)
+  Plane2<W> res;
+  double ph_re = 1.0, ph_im = 0.0;
+  bool have_result = false;
+  double flops = 0.0;
+
+  Pipeline(int n_, int dtype_) : n(n_), dtype(dtype_) {
+    np = round_up(std::max(n, 1), 128);
+    ld = np;
+    ldl = 2 * np;
+    Plane2<W>* all[] = {&a, &p2, &p4, &p6, &t1, &t2, &u, &v, &x0, &x1};
+    const size_t cnt = (size_t)np * (size_t)ld;
+    for (Plane2<W>* p : all) {
+      p->r = dalloc<W>(cnt);
+      p->i = dalloc<W>(cnt);
+    }
+    lr = dalloc<double>((size_t)np * ldl);
+    li = dalloc<double>((size_t)np * ldl);
+    tr = dalloc<double>((size_t)2 * np * LU_NB);
+    ti = dalloc<double>((size_t)2 * np * LU_NB);
+    ipiv = dalloc<int>(np);
+    gmax = (int)((np + PANEL_R - 1) / PANEL_R);
+    cand = dalloc<PanelCand>((size_t)2 * gmax);
+    top = dalloc<double>(4 * LU_NB);
+    const int nt = (n + 31) / 32;
+    colpart = dalloc<double>((size_t)std::max(nt, 1) * std::max(n, 1));
+    off = dalloc<double>(std::max(n, 1));
+    diag = dalloc<double2>(std::max(n, 1));
+    flags = dalloc<int>(4);
+    scal = dalloc<Scalars>(1);
+    for (int k = 0; k < 8; ++k) FX_HIP(hipEventCreate(&ev[k]));
+    events = true;
+  }
+
+  ~Pipeline() override {
+    Plane2<W>* all[] = {&a, &p2, &p4, &p6, &t1, &t2, &u, &v, &x0, &x1};
+    for (Plane2<W>* p : all) {
+      dfree(p->r);
+      dfree(p->i);
+    }
+    dfree(lr);
+    dfree(li);
+    dfree(tr);
+    dfree(ti);
+    dfree(ipiv);
+    dfree(cand);
+    dfree(top);
+    dfree(colpart);
+    dfree(off);
+    dfree(diag);
+    dfree(flags);
+    dfree(scal);
+    if (events)
+      for (int k = 0; k < 8; ++k) (void)hipEventDestroy(ev[k]);
+  }
+
+  bool has_result() const override { return have_result; }
+
+  // ---- launch helpers ----
+  template <int MODE, bool SYM>
+  void gemm(const GemmArgs<W>& g, hipStream_t st) {
+    using C = GemmCfg<W>;
+    constexpr int NT = (C::BM / C::WM) * (C::BN / C::WN) * 64;
+    double tiles;
+    if (SYM) {
+      const int T = (g.m + C::BM - 1) / C::BM;
+      const int nb = T * (T + 1) / 2;
+      gemm_nt_kernel<W, MODE, SYM><<<dim3(nb), dim3(NT), 0, st>>>(g);
+      tiles = nb;
+    } else {
+      dim3 grid((g.n + C::BN - 1) / C::BN, (g.m + C::BM - 1) / C::BM);
+      gemm_nt_kernel<W, MODE, SYM><<<grid, dim3(NT), 0, st>>>(g);
+      tiles = (double)grid.x * grid.y;
+    }
+    FX_LAUNCH();
+    const double mult = MODE == MODE_REAL ? 1.0 : (MODE == MODE_C4M ? 4.0 : 3.0);
+    flops += mult * 2.0 * tiles * C::BM * C::BN * (double)g.k;
+  }
+
+  static GemmArgs<W> args(const Plane2<W>& A, const Plane2<W>& B, int64_t lda, int64_t ldb, int m, int nn, int k) {
+    GemmArgs<W> g;
+    std::memset(&g, 0, sizeof(g));
+    g.ar = A.r;
+    g.ai = A.i;
+    g.lda = lda;
+    g.br = B.r;
+    g.bi = B.i;
+    g.ldb = ldb;
+    g.m = m;
+    g.n = nn;
+    g.k = k;
+    g.epi.alpha = 1.0;
+    return g;
+  }
+
+  void set_out(GemmArgs<W>& g, const Plane2<W>& C) {
+    g.epi.cr = C.r;
+    g.epi.ci = C.i;
+    g.epi.ldc = ld;
+    g.epi.ld_add = ld;
+  }
+
+  void lincomb(const Plane2<W>& out, bool cplx, std::initializer_list<std::pair<double, const Plane2<W>*>> terms,
+               double dg, hipStream_t st) {
+    Lincomb<W> L;
+    std::memset(&L, 0, sizeof(L));
+    L.out_r = out.r;
+    L.out_i = cplx ? out.i : nullptr;
+    int k = 0;
+    for (auto& t : terms) {
+      L.c[k] = t.first;
+      L.in_r[k] = t.second->r;
+      L.in_i[k] = t.second->i;
+      ++k;
+    }
+    L.nin = k;
+    L.diag = dg;
+    dim3 grid((n + 255) / 256, n);
+    lincomb_kernel<W><<<grid, 256, 0, st>>>(L, n, ld);
+    FX_LAUNCH();
+  }
+
+  void transpose(const Plane2<W>& in, const Plane2<W>& out, bool cplx, hipStream_t st) {
+    dim3 grid((n + 31) / 32, (n + 31) / 32);
+    transpose_kernel<W><<<grid, 256, 0, st>>>(in.r, cplx ? in.i : nullptr, out.r, cplx ? out.i : nullptr, n, ld);
+    FX_LAUNCH();
+  }
+
+  // NT product with symmetric (half-tile) output, real planes.
+  void sym_real(const Plane2<W>& A, const Plane2<W>& B, const Plane2<W>& C, hipStream_t st,
+                std::initializer_list<std::pair<double, const Plane2<W>*>> add = {}, double dg = 0.0) {
+    GemmArgs<W> g = args(A, B, ld, ld, n, n, (int)np);
+    set_out(g, C);
+    int k = 0;
+    for (auto& t : add) {
+      g.epi.addc[k] = t.first;
+      g.epi.add_r[k] = t.second->r;
+      g.epi.add_i[k] = t.second->i;
+      ++k;
+    }
+    g.epi.nadd = k;
+    g.epi.diag = dg;
+    gemm<MODE_REAL, true>(g, st);
+  }
+
+  // General complex product C = A * B (B^T supplied as BT), 3M.
+  void cplx_mul(const Plane2<W>& A, const Plane2<W>& BT, const Plane2<W>& C, hipStream_t st,
+                std::initializer_list<std::pair<double, const Plane2<W>*>> add = {}, double dg = 0.0) {
+    GemmArgs<W> g = args(A, BT, ld, ld, n, n, (int)np);
+    set_out(g, C);
+    int k = 0;
+    for (auto& t : add) {
+      g.epi.addc[k] = t.first;
+      g.epi.add_r[k] = t.second->r;
+      g.epi.add_i[k] = t.second->i;
+      ++k;
+    }
+    g.epi.nadd = k;
+    g.epi.diag = dg;
+    gemm<MODE_C4M, false>(g, st);
+  }
+
+  // ---- Pade stage ----
+  // Structured: A = i C, C real symmetric in a.r; A^{2k} = (-1)^k C^{2k}.
+  // Produces V in v.r and U = i Uq with Uq in u.r.
+  void pade_structured(int m, hipStream_t st) {
+    double b[14];
+    for (int j = 0; j <= m; ++j) b[j] = pade_b(m, j);
+    sym_real(a, a, p2, st);  // C2
+    if (m >= 5) sym_real(p2, p2, p4, st);
+    if (m >= 7) sym_real(p4, p2, p6, st);
+    if (m == 13) {
+      // Z1 = b13 C6 - b11 C4 + b9 C2, Z2 = b12 C6 - b10 C4 + b8 C2
+      lincomb(t1, false, {{b[13], &p6}, {-b[11], &p4}, {b[9], &p2}}, 0.0, st);
+      lincomb(t2, false, {{b[12], &p6}, {-b[10], &p4}, {b[8], &p2}}, 0.0, st);
+      // Wb = C6 Z1 - b7 C6 + b5 C4 - b3 C2 + b1 I   (into x0)
+      sym_real(p6, t1, x0, st, {{-b[7], &p6}, {b[5], &p4}, {-b[3], &p2}}, b[1]);
+      // V = C6 Z2 - b6 C6 + b4 C4 - b2 C2 + b0 I
+      sym_real(p6, t2, v, st, {{-b[6], &p6}, {b[4], &p4}, {-b[2], &p2}}, b[0]);
+      // Uq = C Wb
+      sym_real(a, x0, u, st);
+      return;
+    }
+    if (m == 9) sym_real(p6, p2, x1, st);  // C8 in x1
+    // odd = sum_k b_{2k+1} (-1)^k C^{2k}, even = sum_k b_{2k} (-1)^k C^{2k}
+    if (m == 3) {
+      lincomb(t1, false, {{-b[3], &p2}}, b[1], st);
+      lincomb(v, false, {{-b[2], &p2}}, b[0], st);
+    } else if (m == 5) {
+      lincomb(t1, false, {{b[5], &p4}, {-b[3], &p2}}, b[1], st);
+      lincomb(v, false, {{b[4], &p4}, {-b[2], &p2}}, b[0], st);
+    } else if (m == 7) {
+      lincomb(t1, false, {{-b[7], &p6}, {b[5], &p4}, {-b[3], &p2}}, b[1], st);
+      lincomb(v, false, {{-b[6], &p6}, {b[4], &p4}, {-b[2], &p2}}, b[0], st);
+    } else {  // 9
+      lincomb(t1, false, {{b[9], &x1}, {-b[7], &p6}, {b[5], &p4}, {-b[3], &p2}}, b[1], st);
+      lincomb(v, false, {{b[8], &x1}, {-b[6], &p6}, {b[4], &p4}, {-b[2], &p2}}, b[0], st);
+    }
+    sym_real(a, t1, u, st);
+  }
+
+  // General complex A in a (planar).  Produces U in u, V in v.
+  void pade_general(int m, hipStream_t st) {
+    double b[14];
+    for (int j = 0; j <= m; ++j) b[j] = pade_b(m, j);
+    transpose(a, t1, true, st);      // A^T
+    cplx_mul(a, t1, p2, st);         // A2
+    transpose(p2, t2, true, st);     // A2^T
+    if (m >= 5) cplx_mul(p2, t2, p4, st);  // A4
+    if (m >= 7) cplx_mul(p4, t2, p6, st);  // A6
+    if (m == 13) {
+      lincomb(t1, true, {{b[13], &p6}, {b[11], &p4}, {b[9], &p2}}, 0.0, st);
+      transpose(t1, x0, true, st);
+      cplx_mul(p6, x0, x1, st, {{b[7], &p6}, {b[5], &p4}, {b[3], &p2}}, b[1]);  // bracket of U
+      lincomb(t1, true, {{b[12], &p6}, {b[10], &p4}, {b[8], &p2}}, 0.0, st);
+      transpose(t1, x0, true, st);
+      cplx_mul(p6, x0, v, st, {{b[6], &p6}, {b[4], &p4}, {b[2], &p2}}, b[0]);
+      transpose(x1, x0, true, st);
+      cplx_mul(a, x0, u, st);
+      return;
+    }
+    if (m == 9) cplx_mul(p6, t2, x1, st);  // A8
+    if (m == 3) {
+      lincomb(t1, true, {{b[3], &p2}}, b[1], st);
+      lincomb(v, true, {{b[2], &p2}}, b[0], st);
+    } else if (m == 5) {
+      lincomb(t1, true, {{b[5], &p4}, {b[3], &p2}}, b[1], st);
+      lincomb(v, true, {{b[4], &p4}, {b[2], &p2}}, b[0], st);
+    } else if (m == 7) {
+      lincomb(t1, true, {{b[7], &p6}, {b[5], &p4}, {b[3], &p2}}, b[1], st);
+      lincomb(v, true, {{b[6], &p6}, {b[4], &p4}, {b[2], &p2}}, b[0], st);
+    } else {
+      lincomb(t1, true, {{b[9], &x1}, {b[7], &p6}, {b[5], &p4}, {b[3], &p2}}, b[1], st);
+      lincomb(v, true, {{b[8], &x1}, {b[6], &p6}, {b[4], &p4}, {b[2], &p2}}, b[0], st);
+    }
+    transpose(t1, x0, true, st);
+    cplx_mul(a, x0, u, st);
+  }
+
+  // ---- FP64 solve of (V - U) X = (V + U) ----
+  void solve(bool structured, hipStream_t st) {
+    const int inp = (int)np;
+    {
+      dim3 grid((unsigned)((2 * np + 255) / 256), (unsigned)np);
+      form_system<W><<<grid, 256, 0, st>>>(u.r, u.i, v.r, v.i, ld, n, inp, lr, li, ldl, structured ? 1 : 0);
+      FX_LAUNCH();
+    }
+    const int nb = LU_NB;
+    const int nblk = inp / nb;
+    for (int kb = 0; kb < nblk; ++kb) {
+      const int k0 = kb * nb;
+      const int G = (inp - k0 + PANEL_R - 1) / PANEL_R;
+      lu_panel_init<<<G, 256, 0, st>>>(lr, li, ldl, inp, k0, cand, top);
+      FX_LAUNCH();
+      for (int j = 0; j < nb; ++j) {
+        const int par = j & 1;
+        lu_panel_step<<<G, 256, 0, st>>>(lr, li, ldl, inp, k0, j, G, cand + par * gmax, cand + (par ^ 1) * gmax,
+                                         top + par * 2 * LU_NB, top + (par ^ 1) * 2 * LU_NB, ipiv, scal);
+        FX_LAUNCH();
+      }
+      const int c0 = k0 + nb, c1 = 2 * inp;
+      lu_laswp<<<(c1 - c0 + 255) / 256, 256, 0, st>>>(lr, li, ldl, k0, nb, ipiv, c0, c1);
+      FX_LAUNCH();
+      lu_trsm_block_row<false><<<(c1 - c0 + 63) / 64, 64, 0, st>>>(lr, li, ldl, k0, c0, c1, tr, ti);
+      FX_LAUNCH();
+      if (k0 + nb < inp) {
+        GemmArgs<double> g;
+        std::memset(&g, 0, sizeof(g));
+        g.ar = lr + (int64_t)(k0 + nb) * ldl + k0;
+        g.ai = li + (int64_t)(k0 + nb) * ldl + k0;
+        g.lda = ldl;
+        g.br = tr;
+        g.bi = ti;
+        g.ldb = nb;
+        g.m = inp - k0 - nb;
+        g.n = c1 - c0;
+        g.k = nb;
+        g.epi.cr = lr + (int64_t)(k0 + nb) * ldl + (k0 + nb);
+        g.epi.ci = li + (int64_t)(k0 + nb) * ldl + (k0 + nb);
+        g.epi.ldc = ldl;
+        g.epi.alpha = -1.0;
+        g.epi.beta = 1.0;
+        lu_gemm(g, st);
+      }
+    }
+    for (int ib = nblk - 1; ib >= 0; --ib) {
+      const int r0 = ib * nb;
+      lu_trsm_block_row<true><<<(inp + 63) / 64, 64, 0, st>>>(lr, li, ldl, r0, inp, 2 * inp, tr, ti);
+      FX_LAUNCH();
+      if (r0 > 0) {
+        GemmArgs<double> g;
+        std::memset(&g, 0, sizeof(g));
+        g.ar = lr + r0;
+        g.ai = li + r0;
+        g.lda = ldl;
+        g.br = tr;
+        g.bi = ti;
+        g.ldb = nb;
+        g.m = r0;
+        g.n = inp;
+        g.k = nb;
+        g.epi.cr = lr + inp;
+        g.epi.ci = li + inp;
+        g.epi.ldc = ldl;
+        g.epi.alpha = -1.0;
+        g.epi.beta = 1.0;
+        lu_gemm(g, st);
+      }
+    }
+    {
+      dim3 grid((n + 31) / 32, (n + 31) / 32);
+      solution_to_work<W><<<grid, 256, 0, st>>>(lr + inp, li + inp, ldl, x0.r, x0.i, ld, n, structured ? 1 : 0);
+      FX_LAUNCH();
+    }
+  }
+
+  void lu_gemm(const GemmArgs<double>& g, hipStream_t st) {
+    using C = GemmCfg<double>;
+    constexpr int NT = (C::BM / C::WM) * (C::BN / C::WN) * 64;
+    dim3 grid((g.n + C::BN - 1) / C::BN, (g.m + C::BM - 1) / C::BM);
+    gemm_nt_kernel<double, MODE_C4M, false><<<grid, dim3(NT), 0, st>>>(g);
+    FX_LAUNCH();
+    flops += 8.0 * (double)grid.x * grid.y * C::BM * C::BN * (double)g.k;
+  }
+
+  // ---- squarings ----
+  Plane2<W>* square(int s, bool structured, hipStream_t st) {
+    Plane2<W>* cur = &x0;
+    Plane2<W>* nxt = &x1;
+    for (int q = 0; q < s; ++q) {
+      if (structured) {
+        GemmArgs<W> g = args(*cur, *cur, ld, ld, n, n, (int)np);
+        set_out(g, *nxt);
+        gemm<MODE_C3M, true>(g, st);
+      } else {
+        transpose(*cur, t1, true, st);
+        GemmArgs<W> g = args(*cur, t1, ld, ld, n, n, (int)np);
+        set_out(g, *nxt);
+        gemm<MODE_C3M, false>(g, st);
+      }
+      std::swap(cur, nxt);
+    }
+    return cur;
+  }
+
+  template <typename Tin, typename Tout>
+  void run_typed(const Tin* d_a, Tout* d_u, hipStream_t st, fexpm_stats& S, int opt_structure) {
+    flops = 0.0;
+    have_result = false;
+    FX_HIP(hipEventRecord(ev[0], st));
+    const int nt = (n + 31) / 32;
+    FX_HIP(hipMemsetAsync(flags, 0, sizeof(int), st));
+    analyze_tiles<Tin><<<nt * (nt + 1) / 2, 256, 0, st>>>(d_a, n, nt, colpart, diag, flags);
+    FX_LAUNCH();
+    reduce_colpart<<<(n + 255) / 256, 256, 0, st>>>(colpart, n, nt, off);
+    FX_LAUNCH();
+    finalize_norms<<<1, 1024, 0, st>>>(off, diag, n, flags, scal);
+    FX_LAUNCH();
+    Scalars hs;
+    FX_HIP(hipMemcpyAsync(&hs, scal, sizeof(Scalars), hipMemcpyDeviceToHost, st));
+    FX_HIP(hipStreamSynchronize(st));
+    FX_HIP(hipEventRecord(ev[1], st));
+
+    const bool is_complex = (dtype == FEXPM_COMPLEX64 || dtype == FEXPM_COMPLEX128);
+    S.nonfinite = (hs.flags & FLAG_NONFINITE) ? 1 : 0;
+    S.norm_plain = hs.norm_plain;
+    if (S.nonfinite || !std::isfinite(hs.norm_plain)) {
+      S.nonfinite = 1;
+      const int64_t cnt = (int64_t)n * n;
+      fill_nan<Tout><<<(unsigned)((cnt + 255) / 256), 256, 0, st>>>(d_u, cnt);
+      FX_LAUNCH();
+      FX_HIP(hipStreamSynchronize(st));
+      S.m = 0;
+      S.s = 0;
+      S.structured = 0;
+      S.shifted = 0;
+      S.mu_re = S.mu_im = 0.0;
+      S.norm = hs.norm_plain;
+      S.ms_total = S.ms_analyze = S.ms_pade = S.ms_solve = S.ms_square = S.ms_output = 0.0;
+      S.flops_issued = 0.0;
+      return;
+    }
+    const bool structured =
+        is_complex && opt_structure && !(hs.flags & (FLAG_NONZERO_RE | FLAG_ASYM));
+    const bool shifted = hs.norm_shift <= hs.norm_plain;
+    const double mu_re = shifted ? hs.mu_re : 0.0, mu_im = shifted ? hs.mu_im : 0.0;
+    const double norm = shifted ? hs.norm_shift : hs.norm_plain;
+    int m, s;
+    choose_degree(norm, m, s);
+    const double scale = std::ldexp(1.0, -s);
+    {
+      dim3 grid((n + 255) / 256, n);
+      prepare_planar<Tin, W><<<grid, 256, 0, st>>>(d_a, n, a.r, a.i, ld, mu_re, mu_im, scale, structured ? 1 : 0);
+      FX_LAUNCH();
+    }
+    if (structured)
+      pade_structured(m, st);
+    else
+      pade_general(m, st);
+    FX_HIP(hipEventRecord(ev[2], st));
+    solve(structured, st);
+    FX_HIP(hipEventRecord(ev[3], st));
+    Plane2<W>* X = square(s, structured, st);
+    FX_HIP(hipEventRecord(ev[4], st));
+    // phase e^mu
+    const double er = std::exp(mu_re);
+    ph_re = er * std::cos(mu_im);
+    ph_im = er * std::sin(mu_im);
+    {
+      dim3 grid((n + 255) / 256, n);
+      write_output<W, Tout><<<grid, 256, 0, st>>>(X->r, X->i, ld, n, ph_re, ph_im, d_u);
+      FX_LAUNCH();
+    }
+    FX_HIP(hipEventRecord(ev[5], st));
+    FX_HIP(hipEventSynchronize(ev[5]));
+    Scalars hs2;
+    FX_HIP(hipMemcpy(&hs2, scal, sizeof(Scalars), hipMemcpyDeviceToHost));
+    res = *X;
+    have_result = true;
+    float t01, t12, t23, t34, t45;
+    FX_HIP(hipEventElapsedTime(&t01, ev[0], ev[1]));
+    FX_HIP(hipEventElapsedTime(&t12, ev[1], ev[2]));
+    FX_HIP(hipEventElapsedTime(&t23, ev[2], ev[3]));
+    FX_HIP(hipEventElapsedTime(&t34, ev[3], ev[4]));
+    FX_HIP(hipEventElapsedTime(&t45, ev[4], ev[5]));
+    S.m = m;
+    S.s = s;
+    S.structured = structured ? 1 : 0;
+    S.shifted = shifted ? 1 : 0;
+    S.mu_re = mu_re;
+    S.mu_im = mu_im;
+    S.norm = norm;
+    S.ms_analyze = t01;
+    S.ms_pade = t12;
+    S.ms_solve = t23;
+    S.ms_square = t34;
+    S.ms_output = t45;
+    S.ms_total = t01 + t12 + t23 + t34 + t45;
+    S.flops_issued = flops;
+    if (hs2.info != 0) throw Error(FEXPM_ESINGULAR, "Pade denominator is exactly singular");
+  }
+
+  void run(const void* d_a, void* d_u, hipStream_t st, fexpm_stats& S, int opt_structure) override {
+    switch (dtype) {
+      case FEXPM_FLOAT32:
+        run_typed<float, float>((const float*)d_a, (float*)d_u, st, S, opt_structure);
+        break;
+      case FEXPM_FLOAT64:
+        run_typed<double, double>((const double*)d_a, (double*)d_u, st, S, opt_structure);
+        break;
+      case FEXPM_COMPLEX64:
+        run_typed<float2, float2>((const float2*)d_a, (float2*)d_u, st, S, opt_structure);
+        break;
+      default:
+        run_typed<double2, double2>((const double2*)d_a, (double2*)d_u, st, S, opt_structure);
+        break;
+    }
+  }
+
+  void apply(const double2* d_psi0, int nstates, double2* d_out, hipStream_t st) override {
+    apply_states<W><<<(n + 3) / 4, 256, 0, st>>>(res.r, res.i, ld, n, d_psi0, nstates, ph_re, ph_im, d_out);
+    FX_LAUNCH();
+  }
+};
+
+}  // namespace fexpm
+
+// ===========================================================================
+// C ABI
+using namespace fexpm;
+
+struct fexpm_engine {
+  int64_t n = 0;
+  int dtype = 0;
+  bool freed = false;
+  std::mutex mtx;
+  std::string err;
+  fexpm_stats stats;
+  hipStream_t stream = nullptr;
+  std::unique_ptr<PipelineBase> pipe;
+  void* d_in = nullptr;
+  void* d_out = nullptr;
+  size_t esize = 0;
+  int opt_structure = 1;
+  int opt_precision = 0;
+};
+
+static size_t elem_size(int dtype) {
+  switch (dtype) {
+    case FEXPM_FLOAT32: return 4;
+    case FEXPM_FLOAT64: return 8;
+    case FEXPM_COMPLEX64: return 8;
+    case FEXPM_COMPLEX128: return 16;
+  }
+  return 0;
+}
+
+template <typename F> static int guarded(fexpm_engine* e, F&& f) {
+  try {
+    f();
+    return FEXPM_OK;
+  } catch (const Error& x) {
+    if (e) e->err = x.what();
+    return x.code;
+  } catch (const std::bad_alloc&) {
+    if (e) e->err = "host allocation failed";
+    return FEXPM_ENOMEM;
+  } catch (const std::exception& x) {
+    if (e) e->err = x.what();
+    return FEXPM_EHIP;
+  }
+}
+
+static void check_device() {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count < 1) throw Error(FEXPM_ENODEVICE, "no HIP device available");
+}
+
+static void ensure_io(fexpm_engine* e) {
+  const size_t bytes = (size_t)e->n * (size_t)e->n * e->esize;
+  if (!e->d_in) {
+    FX_HIP(hipMalloc(&e->d_in, std::max<size_t>(bytes, 16)));
+  }
+  if (!e->d_out) {
+    FX_HIP(hipMalloc(&e->d_out, std::max<size_t>(bytes, 16)));
+  }
+}
+
+static void release(fexpm_engine* e) {
+  e->pipe.reset();
+  if (e->d_in) (void)hipFree(e->d_in);
+  if (e->d_out) (void)hipFree(e->d_out);
+  e->d_in = e->d_out = nullptr;
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  e->stream = nullptr;
+}
+
+extern "C" {
+
+int fexpm_create(int64_t n, int dtype, fexpm_engine** out) {
+  if (!out) return FEXPM_EINVAL;
+  *out = nullptr;
+  if (n < 0 || n > (int64_t)1 << 20 || elem_size(dtype) == 0) return FEXPM_EINVAL;
+  fexpm_engine* e = new (std::nothrow) fexpm_engine();
+  if (!e) return FEXPM_ENOMEM;
+  e->n = n;
+  e->dtype = dtype;
+  e->esize = elem_size(dtype);
+  std::memset(&e->stats, 0, sizeof(e->stats));
+  e->stats.n = n;
+  e->stats.dtype = dtype;
+  const int rc = guarded(e, [&] {
+    check_device();
+    FX_HIP(hipStreamCreate(&e->stream));
+    if (n > 0) {
+      if (dtype == FEXPM_FLOAT32 || dtype == FEXPM_COMPLEX64)
+        e->pipe.reset(new Pipeline<float>((int)n, dtype));
+      else
+        e->pipe.reset(new Pipeline<double>((int)n, dtype));
+    }
+  });
+  if (rc != FEXPM_OK) {
+    // keep the handle so that the caller can read the message, then destroy
+    std::string msg = e->err;
+    release(e);
+    delete e;
+    (void)msg;
+    return rc;
+  }
+  *out = e;
+  return FEXPM_OK;
+}
+
+int fexpm_free(fexpm_engine* e) {
+  if (!e) return FEXPM_EINVAL;
+  std::lock_guard<std::mutex> lk(e->mtx);
+  if (e->freed) return FEXPM_OK;
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  release(e);
+  e->freed = true;
+  return FEXPM_OK;
+}
+
+int fexpm_destroy(fexpm_engine* e) {
+  if (!e) return FEXPM_EINVAL;
+  fexpm_free(e);
+  delete e;
+  return FEXPM_OK;
+}
+
+int64_t fexpm_get_n(const fexpm_engine* e) { return e ? e->n : -1; }
+
+static int run_device_locked(fexpm_engine* e, const void* d_a, void* d_u, hipStream_t st) {
+  if (e->freed) {
+    e->err = "engine memory has been freed";
+    return FEXPM_EFREED;
+  }
+  if (e->n == 0) return FEXPM_OK;
+  return guarded(e, [&] {
+    e->stats.n = e->n;
+    e->stats.dtype = e->dtype;
+    e->pipe->run(d_a, d_u, st ? st : e->stream, e->stats, e->opt_structure);
+  });
+}
+
+int fexpm_run_device(fexpm_engine* e, const void* d_a, void* d_u, void* stream) {
+  if (!e || (e->n > 0 && (!d_a || !d_u))) return FEXPM_EINVAL;
+  std::lock_guard<std::mutex> lk(e->mtx);
+  return run_device_locked(e, d_a, d_u, (hipStream_t)stream);
+}
+
+int fexpm_upload(fexpm_engine* e, const void* a) {
+  if (!e || (e->n > 0 && !a)) return FEXPM_EINVAL;
+  std::lock_guard<std::mutex> lk(e->mtx);
+  if (e->freed) return FEXPM_EFREED;
+  if (e->n == 0) return FEXPM_OK;
+  return guarded(e, [&] {
+    ensure_io(e);
+    FX_HIP(hipMemcpy(e->d_in, a, (size_t)e->n * e->n * e->esize, hipMemcpyHostToDevice));
+  });
+}
+
+int fexpm_run_resident(fexpm_engine* e) {
+  if (!e) return FEXPM_EINVAL;
+  std::lock_guard<std::mutex> lk(e->mtx);
+  if (e->freed) return FEXPM_EFREED;
+  if (e->n == 0) return FEXPM_OK;
+  if (!e->d_in) {
+    e->err = "no resident input: call fexpm_upload first";
+    return FEXPM_EINVAL;
+  }
+  return run_device_locked(e, e->d_in, e->d_out, e->stream);
+}
+
+int fexpm_download(fexpm_engine* e, void* u) {
+  if (!e || (e->n > 0 && !u)) return FEXPM_EINVAL;
+  std::lock_guard<std::mutex> lk(e->mtx);
+  if (e->freed) return FEXPM_EFREED;
+  if (e->n == 0) return FEXPM_OK;
+  if (!e->d_out) return FEXPM_EINVAL;
+  return guarded(e, [&] {
+    FX_HIP(hipMemcpy(u, e->d_out, (size_t)e->n * e->n * e->esize, hipMemcpyDeviceToHost));
+  });
+}
+
+int fexpm_run(fexpm_engine* e, const void* a, void* u) {
+  if (!e || (e->n > 0 && (!a || !u))) return FEXPM_EINVAL;
+  std::lock_guard<std::mutex> lk(e->mtx);
+  if (e->freed) {
+    e->err = "engine memory has been freed";
+    return FEXPM_EFREED;
+  }
+  if (e->n == 0) return FEXPM_OK;
+  int rc = guarded(e, [&] {
+    ensure_io(e);
+    FX_HIP(hipMemcpyAsync(e->d_in, a, (size_t)e->n * e->n * e->esize, hipMemcpyHostToDevice, e->stream));
+  });
+  if (rc != FEXPM_OK) return rc;
+  rc = run_device_locked(e, e->d_in, e->d_out, e->stream);
+  if (rc != FEXPM_OK) return rc;
+  return guarded(e, [&] {
+    FX_HIP(hipMemcpyAsync(u, e->d_out, (size_t)e->n * e->n * e->esize, hipMemcpyDeviceToHost, e->stream));
+    FX_HIP(hipStreamSynchronize(e->stream));
+  });
+}
+
+int fexpm_apply(fexpm_engine* e, const void* psi0, int64_t nstates, void* psi) {
+  if (!e || nstates < 0 || (nstates > 0 && (!psi0 || !psi))) return FEXPM_EINVAL;
+  std::lock_guard<std::mutex> lk(e->mtx);
+  if (e->freed) return FEXPM_EFREED;
+  if (e->n == 0 || nstates == 0) return FEXPM_OK;
+  if (!e->pipe->has_result()) {
+    e->err = "no result to apply: run first";
+    return FEXPM_EINVAL;
+  }
+  return guarded(e, [&] {
+    const size_t bytes = (size_t)nstates * e->n * sizeof(double2);
+    void* din = nullptr;
+    void* dout = nullptr;
+    FX_HIP(hipMalloc(&din, bytes));
+    hipError_t er = hipMalloc(&dout, bytes);
+    if (er != hipSuccess) {
+      (void)hipFree(din);
+      FX_HIP(er);
+    }
+    try {
+      FX_HIP(hipMemcpyAsync(din, psi0, bytes, hipMemcpyHostToDevice, e->stream));
+      e->pipe->apply((const double2*)din, (int)nstates, (double2*)dout, e->stream);
+      FX_HIP(hipMemcpyAsync(psi, dout, bytes, hipMemcpyDeviceToHost, e->stream));
+      FX_HIP(hipStreamSynchronize(e->stream));
+    } catch (...) {
+      (void)hipFree(din);
+      (void)hipFree(dout);
+      throw;
+    }
+    (void)hipFree(din);
+    (void)hipFree(dout);
+  });
+}
+
+int fexpm_set_option(fexpm_engine* e, int key, int64_t value) {
+  if (!e) return FEXPM_EINVAL;
+  std::lock_guard<std::mutex> lk(e->mtx);
+  switch (key) {
+    case FEXPM_OPT_PRECISION:
+      if (value != FEXPM_PREC_NATIVE) {
+        e->err = "unsupported precision mode";
+        return FEXPM_EINVAL;
+      }
+      e->opt_precision = (int)value;
+      return FEXPM_OK;
+    case FEXPM_OPT_STRUCTURE:
+      e->opt_structure = value ? 1 : 0;
+      return FEXPM_OK;
+  }
+  e->err = "unknown option";
+  return FEXPM_EINVAL;
+}
+
+int fexpm_get_stats(const fexpm_engine* e, fexpm_stats* out) {
+  if (!e || !out) return FEXPM_EINVAL;
+  *out = e->stats;
+  return FEXPM_OK;
+}
+
+const char* fexpm_last_error(const fexpm_engine* e) { return e ? e->err.c_str() : "null engine"; }
+
+const char* fexpm_status_string(int status) {
+  switch (status) {
+    case FEXPM_OK: return "ok";
+    case FEXPM_EINVAL: return "invalid argument";
+    case FEXPM_ENOMEM: return "out of device memory";
+    case FEXPM_EHIP: return "HIP runtime error";
+    case FEXPM_ESINGULAR: return "singular Pade denominator";
+    case FEXPM_EFREED: return "engine memory has been freed";
+    case FEXPM_ENODEVICE: return "no HIP device";
+  }
+  return "unknown status";
+}
+
+double fexpm_gpu_memory_gb(void) {
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) return -1.0;
+  return (double)fr / (1024.0 * 1024.0 * 1024.0);
+}
+
+}  // extern "C"

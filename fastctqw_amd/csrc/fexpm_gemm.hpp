@@ -1,0 +1,316 @@
+// MFMA GEMM family for gfx950 (CDNA4), row-major "NT" layout:
+//
+//     C[i][j] = sum_k A[i][k] * B[j][k]
+//
+// Both operands are read along contiguous rows (k fastest), so a product
+// P*Q of row-major matrices is NT(P, Q^T); symmetric right operands need no
+// transpose at all.  One kernel template covers every dense contraction of
+// the scaling-and-squaring pipeline:
+//
+//   MODE_REAL  one real accumulator               (structured Pade products)
+//   MODE_C4M   complex planar, 4 real products    (LU / solve updates, FP64)
+//   MODE_C3M   complex planar, 3 real products    (general squarings)
+//              T1 = Ar Br^T, T2 = Ai Bi^T, T3 = (Ar+Ai)(Br+Bi)^T
+//              Re = T1 - T2, Im = T3 - T1 - T2
+//   SYM        only lower-triangular output tiles are computed; each value
+//              is written at (i,j) and (j,i) by the same lane, so the result
+//              is exactly symmetric.  Used when the product is symmetric in
+//              exact arithmetic (polynomials in one symmetric matrix, and
+//              X X^T for complex-symmetric X: the 3M squaring of
+//              X = Xr + i Xi with Xr, Xi symmetric costs 3 half products).
+//
+// Tiles: a workgroup computes BM x BN outputs with (BM/WM)*(BN/WN) waves of
+// 64 lanes; every wave owns a WM x WN block made of 16x16 MFMA blocks
+// (v_mfma_{f32,f64}_16x16x4).  Operand tiles of BK=16 columns are staged
+// global -> registers -> LDS with double buffering; rows keep k contiguous
+// so one 16-byte LDS read feeds four MFMA k-steps: lane l of a 16x16x4 MFMA
+// supplies A[l%16][k] with k = 4*(l/16) + step, and the same permutation of
+// k is used for B, so the products pair correctly.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fexpm {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct AccT;
+template <> struct AccT<float> { using type = f32x4; };
+template <> struct AccT<double> { using type = f64x4; };
+
+template <typename T> struct Vec16;
+template <> struct Vec16<float> { using type = float4; };
+template <> struct Vec16<double> { using type = double2; };
+
+__device__ __forceinline__ f32x4 mfma16(float a, float b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f64x4 mfma16(double a, double b, f64x4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+enum GemmMode { MODE_REAL = 0, MODE_C4M = 1, MODE_C3M = 2 };
+
+// Epilogue: C = alpha*acc + beta*C_old + sum_t addc[t]*ADD_t + diag*I.
+// ADD_t are planar (re, im) matrices sharing ld_add; for MODE_REAL only the
+// re planes are used.  diag is added to the real part only.
+template <typename T> struct Epilogue {
+  T* cr;
+  T* ci;
+  int64_t ldc;
+  double alpha;
+  double beta;
+  int nadd;
+  const T* add_r[3];
+  const T* add_i[3];
+  double addc[3];
+  int64_t ld_add;
+  double diag;
+};
+
+template <typename T> struct GemmArgs {
+  const T* ar;
+  const T* ai;
+  int64_t lda;
+  const T* br;
+  const T* bi;
+  int64_t ldb;
+  int m, n, k;  // k must be a multiple of 16; operand columns beyond the
+                // logical k are zero in storage
+  Epilogue<T> epi;
+};
+
+template <typename T> struct GemmCfg;
+template <> struct GemmCfg<float> {
+  static constexpr int BM = 128, BN = 128, BK = 16, PAD = 4, WM = 64, WN = 32;
+};
+template <> struct GemmCfg<double> {
+  static constexpr int BM = 64, BN = 64, BK = 16, PAD = 2, WM = 32, WN = 32;
+};
+
+// Staging of a ROWS x BK tile through registers, NT threads.
+template <typename T, int ROWS, int BK, int PAD, int NT> struct TileStage {
+  using V = typename Vec16<T>::type;
+  static constexpr int VEC = 16 / (int)sizeof(T);
+  static constexpr int CPR = BK / VEC;
+  static constexpr int RPP = NT / CPR;
+  static constexpr int PASSES = (ROWS + RPP - 1) / RPP;
+  V r[PASSES];
+
+  __device__ __forceinline__ void load(const T* __restrict__ p, int64_t ld, int nvalid, int k0) {
+    const int t = threadIdx.x, c = t % CPR, rr = t / CPR;
+#pragma unroll
+    for (int q = 0; q < PASSES; ++q) {
+      const int lr = rr + q * RPP;
+      if (lr < ROWS && lr < nvalid)
+        r[q] = *reinterpret_cast<const V*>(p + (int64_t)lr * ld + k0 + c * VEC);
+      else
+        r[q] = V{};
+    }
+  }
+  __device__ __forceinline__ void store(T* lds) const {
+    const int t = threadIdx.x, c = t % CPR, rr = t / CPR;
+#pragma unroll
+    for (int q = 0; q < PASSES; ++q) {
+      const int lr = rr + q * RPP;
+      if (lr < ROWS) *reinterpret_cast<V*>(lds + lr * (BK + PAD) + c * VEC) = r[q];
+    }
+  }
+};
+
+// Four consecutive k values of one row of an LDS tile.
+template <int BK, int PAD>
+__device__ __forceinline__ void frag4(const float* lds, int row, int kq, float out[4]) {
+  const float4 v = *reinterpret_cast<const float4*>(lds + row * (BK + PAD) + kq);
+  out[0] = v.x; out[1] = v.y; out[2] = v.z; out[3] = v.w;
+}
+template <int BK, int PAD>
+__device__ __forceinline__ void frag4(const double* lds, int row, int kq, double out[4]) {
+  const double2 v0 = *reinterpret_cast<const double2*>(lds + row * (BK + PAD) + kq);
+  const double2 v1 = *reinterpret_cast<const double2*>(lds + row * (BK + PAD) + kq + 2);
+  out[0] = v0.x; out[1] = v0.y; out[2] = v1.x; out[3] = v1.y;
+}
+
+// Lower-triangular tile index: b -> (ti, tj), ti >= tj, row-major over ti.
+__device__ __forceinline__ void tri_tile(int b, int& ti, int& tj) {
+  int i = (int)((sqrtf(8.0f * (float)b + 1.0f) - 1.0f) * 0.5f);
+  while ((i + 1) * (i + 2) / 2 <= b) ++i;
+  while (i * (i + 1) / 2 > b) --i;
+  ti = i;
+  tj = b - i * (i + 1) / 2;
+}
+
+// Spread consecutive tiles over the 8 XCDs so that workgroups that run on
+// one XCD (dispatched round-robin: block b on XCD b % 8) work on
+// neighbouring tiles and share their operand rows in that XCD's L2.
+__device__ __forceinline__ int xcd_remap(int b, int nblocks) {
+  constexpr int NX = 8;
+  const int per = nblocks / NX;
+  const int full = per * NX;
+  if (b >= full) return b;
+  return (b % NX) * per + b / NX;
+}
+
+template <typename T, int MODE, bool SYM>
+__global__ __launch_bounds__((GemmCfg<T>::BM / GemmCfg<T>::WM) * (GemmCfg<T>::BN / GemmCfg<T>::WN) * 64)
+void gemm_nt_kernel(GemmArgs<T> g) {
+  using C = GemmCfg<T>;
+  constexpr int BM = C::BM, BN = C::BN, BK = C::BK, PAD = C::PAD, WM = C::WM, WN = C::WN;
+  constexpr int WAVES_N = BN / WN;
+  constexpr int NT = (BM / WM) * WAVES_N * 64;
+  constexpr int FI = WM / 16, FJ = WN / 16;
+  constexpr int NPL = (MODE == MODE_REAL) ? 1 : 2;
+  constexpr int NACC = (MODE == MODE_REAL) ? 1 : (MODE == MODE_C4M ? 2 : 3);
+  constexpr int TA = BM * (BK + PAD), TB = BN * (BK + PAD);
+  using Acc = typename AccT<T>::type;
+
+  __shared__ __attribute__((aligned(16))) T smem[2][NPL * (TA + TB)];
+
+  int tm, tn;
+  if (SYM) {
+    const int b = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    tri_tile(b, tm, tn);
+  } else {
+    const int nbn = (int)gridDim.x;
+    const int lin = xcd_remap((int)(blockIdx.y * gridDim.x + blockIdx.x), (int)(gridDim.x * gridDim.y));
+    tm = lin / nbn;
+    tn = lin % nbn;
+  }
+  const int row0 = tm * BM, col0 = tn * BN;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wr = (wave / WAVES_N) * WM, wc = (wave % WAVES_N) * WN;
+  const int kq = 4 * (lane >> 4);
+  const int l16 = lane & 15;
+
+  Acc acc[NACC][FI][FJ];
+#pragma unroll
+  for (int a = 0; a < NACC; ++a)
+#pragma unroll
+    for (int i = 0; i < FI; ++i)
+#pragma unroll
+      for (int j = 0; j < FJ; ++j) acc[a][i][j] = Acc{};
+
+  const T* pa[2] = {g.ar + (int64_t)row0 * g.lda, (NPL == 2) ? g.ai + (int64_t)row0 * g.lda : nullptr};
+  const T* pb[2] = {g.br + (int64_t)col0 * g.ldb, (NPL == 2) ? g.bi + (int64_t)col0 * g.ldb : nullptr};
+  const int mvalid = g.m - row0, nvalid = g.n - col0;
+
+  TileStage<T, BM, BK, PAD, NT> sa[NPL];
+  TileStage<T, BN, BK, PAD, NT> sb[NPL];
+
+  const int KT = g.k / BK;
+#pragma unroll
+  for (int p = 0; p < NPL; ++p) {
+    sa[p].load(pa[p], g.lda, mvalid, 0);
+    sb[p].load(pb[p], g.ldb, nvalid, 0);
+  }
+#pragma unroll
+  for (int p = 0; p < NPL; ++p) {
+    sa[p].store(&smem[0][p * (TA + TB)]);
+    sb[p].store(&smem[0][p * (TA + TB) + TA]);
+  }
+  __syncthreads();
+
+  for (int kt = 0; kt < KT; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < KT) {
+#pragma unroll
+      for (int p = 0; p < NPL; ++p) {
+        sa[p].load(pa[p], g.lda, mvalid, (kt + 1) * BK);
+        sb[p].load(pb[p], g.ldb, nvalid, (kt + 1) * BK);
+      }
+    }
+    T fa[NPL][FI][4], fb[NPL][FJ][4];
+#pragma unroll
+    for (int p = 0; p < NPL; ++p) {
+      const T* la = &smem[cur][p * (TA + TB)];
+      const T* lb = la + TA;
+#pragma unroll
+      for (int i = 0; i < FI; ++i) frag4<BK, PAD>(la, wr + 16 * i + l16, kq, fa[p][i]);
+#pragma unroll
+      for (int j = 0; j < FJ; ++j) frag4<BK, PAD>(lb, wc + 16 * j + l16, kq, fb[p][j]);
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+#pragma unroll
+      for (int i = 0; i < FI; ++i) {
+#pragma unroll
+        for (int j = 0; j < FJ; ++j) {
+          if (MODE == MODE_REAL) {
+            acc[0][i][j] = mfma16(fa[0][i][s], fb[0][j][s], acc[0][i][j]);
+          } else if (MODE == MODE_C4M) {
+            acc[0][i][j] = mfma16(fa[0][i][s], fb[0][j][s], acc[0][i][j]);
+            acc[0][i][j] = mfma16(-fa[1][i][s], fb[1][j][s], acc[0][i][j]);
+            acc[1][i][j] = mfma16(fa[0][i][s], fb[1][j][s], acc[1][i][j]);
+            acc[1][i][j] = mfma16(fa[1][i][s], fb[0][j][s], acc[1][i][j]);
+          } else {
+            acc[0][i][j] = mfma16(fa[0][i][s], fb[0][j][s], acc[0][i][j]);
+            acc[1][i][j] = mfma16(fa[1][i][s], fb[1][j][s], acc[1][i][j]);
+            acc[2][i][j] = mfma16(fa[0][i][s] + fa[1][i][s], fb[0][j][s] + fb[1][j][s], acc[2][i][j]);
+          }
+        }
+      }
+    }
+    if (kt + 1 < KT) {
+      const int nxt = cur ^ 1;
+#pragma unroll
+      for (int p = 0; p < NPL; ++p) {
+        sa[p].store(&smem[nxt][p * (TA + TB)]);
+        sb[p].store(&smem[nxt][p * (TA + TB) + TA]);
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---- epilogue ----
+  const Epilogue<T>& e = g.epi;
+  const bool diag_tile = SYM && (tm == tn);
+#pragma unroll
+  for (int i = 0; i < FI; ++i) {
+#pragma unroll
+    for (int j = 0; j < FJ; ++j) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = row0 + wr + 16 * i + 4 * (lane >> 4) + r;
+        const int col = col0 + wc + 16 * j + l16;
+        if (row >= g.m || col >= g.n) continue;
+        if (diag_tile && row < col) continue;
+        double vr, vi = 0.0;
+        if (MODE == MODE_REAL) {
+          vr = (double)acc[0][i][j][r];
+        } else if (MODE == MODE_C4M) {
+          vr = (double)acc[0][i][j][r];
+          vi = (double)acc[1][i][j][r];
+        } else {
+          const T t1 = acc[0][i][j][r], t2 = acc[1][i][j][r], t3 = acc[2][i][j][r];
+          vr = (double)(t1 - t2);
+          vi = (double)(t3 - t1 - t2);
+        }
+        const int64_t o = (int64_t)row * e.ldc + col;
+        T outr = (T)(e.alpha * vr);
+        T outi = (T)(e.alpha * vi);
+        if (e.beta != 0.0) {
+          outr += (T)e.beta * e.cr[o];
+          if (NPL == 2) outi += (T)e.beta * e.ci[o];
+        }
+        if (e.nadd > 0) {
+          const int64_t oa = (int64_t)row * e.ld_add + col;
+          for (int t = 0; t < e.nadd; ++t) {
+            outr += (T)e.addc[t] * e.add_r[t][oa];
+            if (NPL == 2) outi += (T)e.addc[t] * e.add_i[t][oa];
+          }
+        }
+        if (row == col) outr += (T)e.diag;
+        e.cr[o] = outr;
+        if (NPL == 2) e.ci[o] = outi;
+        if (SYM && row != col) {
+          const int64_t om = (int64_t)col * e.ldc + row;
+          e.cr[om] = outr;
+          if (NPL == 2) e.ci[om] = outi;
+        }
+      }
+    }
+  }
+}
+
+}  // namespace fexpm

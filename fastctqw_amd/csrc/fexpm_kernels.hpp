@@ -1,0 +1,742 @@
+// Bandwidth-bound kernels of the expm pipeline (analysis, conversion,
+// linear combinations, transposes, output) and the FP64 blocked LU with
+// partial pivoting used for the Pade denominator solve (V - U) X = (V + U).
+//
+// Matrices inside the engine are "planar": separate row-major re and im
+// planes with a padded leading dimension.  Padding columns are zero and are
+// never written, so GEMM k-loops may run over the padded extent.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <math.h>
+
+namespace fexpm {
+
+// ---------------------------------------------------------------------------
+// Input element access: complex64 / complex128 (interleaved) or real.
+template <typename Tin> struct InElem;
+template <> struct InElem<float2> {
+  static __device__ __forceinline__ double re(float2 v) { return v.x; }
+  static __device__ __forceinline__ double im(float2 v) { return v.y; }
+};
+template <> struct InElem<double2> {
+  static __device__ __forceinline__ double re(double2 v) { return v.x; }
+  static __device__ __forceinline__ double im(double2 v) { return v.y; }
+};
+template <> struct InElem<float> {
+  static __device__ __forceinline__ double re(float v) { return v; }
+  static __device__ __forceinline__ double im(float) { return 0.0; }
+};
+template <> struct InElem<double> {
+  static __device__ __forceinline__ double re(double v) { return v; }
+  static __device__ __forceinline__ double im(double) { return 0.0; }
+};
+
+enum AnalyzeFlags { FLAG_NONZERO_RE = 1, FLAG_ASYM = 2, FLAG_NONFINITE = 4 };
+
+// One 32x32 tile pair (I, J), I >= J, per workgroup of 256 threads:
+//  * exact symmetry test a_ij == a_ji (both parts), nonzero real parts,
+//    non-finite entries -> atomic OR into flags;
+//  * per-row-block partial column sums of |a_ij| over off-diagonal entries,
+//    colpart[rowblock * n + col], each entry written exactly once, so the
+//    later reduction over row blocks is deterministic;
+//  * diagonal entries (re, im) into diag[].
+template <typename Tin>
+__global__ __launch_bounds__(256) void analyze_tiles(const Tin* __restrict__ a, int n, int nt,
+                                                     double* __restrict__ colpart,
+                                                     double2* __restrict__ diag, int* flags) {
+  __shared__ double2 t0[32][33];
+  __shared__ double2 t1[32][33];
+  __shared__ int sflag;
+  int b = blockIdx.x;
+  int I, J;
+  {
+    int i = (int)((sqrtf(8.0f * (float)b + 1.0f) - 1.0f) * 0.5f);
+    while ((i + 1) * (i + 2) / 2 <= b) ++i;
+    while (i * (i + 1) / 2 > b) --i;
+    I = i;
+    J = b - i * (i + 1) / 2;
+  }
+  if (I >= nt) return;
+  if (threadIdx.x == 0) sflag = 0;
+  __syncthreads();
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  int fl = 0;
+  for (int r = ty; r < 32; r += 8) {
+    const int gi = I * 32 + r, gj = J * 32 + tx;
+    double2 v = make_double2(0.0, 0.0);
+    if (gi < n && gj < n) {
+      const Tin e = a[(int64_t)gi * n + gj];
+      v = make_double2(InElem<Tin>::re(e), InElem<Tin>::im(e));
+    }
+    t0[r][tx] = v;
+    const int hi = J * 32 + r, hj = I * 32 + tx;
+    double2 w = make_double2(0.0, 0.0);
+    if (hi < n && hj < n) {
+      const Tin e = a[(int64_t)hi * n + hj];
+      w = make_double2(InElem<Tin>::re(e), InElem<Tin>::im(e));
+    }
+    t1[r][tx] = w;
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int gi = I * 32 + r, gj = J * 32 + tx;
+    if (gi < n && gj < n) {
+      const double2 v = t0[r][tx];
+      const double2 w = t1[tx][r];  // element (gj, gi)
+      if (v.x != w.x || v.y != w.y) fl |= FLAG_ASYM;
+      if (v.x != 0.0 || w.x != 0.0) fl |= FLAG_NONZERO_RE;
+      if (!isfinite(v.x) || !isfinite(v.y) || !isfinite(w.x) || !isfinite(w.y)) fl |= FLAG_NONFINITE;
+    }
+  }
+  if (fl) atomicOr(&sflag, fl);
+  // column sums: tile (I,J) -> rowblock I, columns of block J;
+  // tile (J,I) (if I != J) -> rowblock J, columns of block I.
+  if (threadIdx.x < 32) {
+    const int c = threadIdx.x;
+    double s = 0.0;
+    const int gj = J * 32 + c;
+    for (int r = 0; r < 32; ++r) {
+      const int gi = I * 32 + r;
+      if (gi == gj) continue;
+      const double2 v = t0[r][c];
+      s += sqrt(v.x * v.x + v.y * v.y);
+    }
+    if (gj < n) colpart[(int64_t)I * n + gj] = s;
+    if (I == J) {
+      if (gj < n) diag[gj] = t0[c][c];
+    }
+  } else if (threadIdx.x < 64 && I != J) {
+    const int c = threadIdx.x - 32;
+    double s = 0.0;
+    const int gj = I * 32 + c;
+    for (int r = 0; r < 32; ++r) {
+      const double2 v = t1[r][c];
+      s += sqrt(v.x * v.x + v.y * v.y);
+    }
+    if (gj < n) colpart[(int64_t)J * n + gj] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && sflag) atomicOr(flags, sflag);
+}
+
+// off[j] = sum over row blocks of colpart (fixed order -> deterministic).
+__global__ void reduce_colpart(const double* __restrict__ colpart, int n, int nt, double* __restrict__ off) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  double s = 0.0;
+  for (int b = 0; b < nt; ++b) s += colpart[(int64_t)b * n + j];
+  off[j] = s;
+}
+
+// Scalars produced on the device and read back by the host once per run.
+struct Scalars {
+  double tr_re, tr_im;   // trace
+  double mu_re, mu_im;   // tr / n
+  double norm_plain;     // ||A||_1
+  double norm_shift;     // ||A - mu I||_1
+  int flags;
+  int info;              // LU: nonzero if an exactly zero pivot was met
+};
+
+// Single workgroup of 1024 threads: trace, mu and both 1-norms.  The trace
+// is summed in a fixed tree order.
+__global__ __launch_bounds__(1024) void finalize_norms(const double* __restrict__ off,
+                                                       const double2* __restrict__ diag, int n,
+                                                       const int* flags, Scalars* out) {
+  __shared__ double sr[1024], si[1024], s0[1024], s1[1024];
+  const int t = threadIdx.x;
+  double ar = 0.0, ai = 0.0;
+  for (int j = t; j < n; j += 1024) {
+    ar += diag[j].x;
+    ai += diag[j].y;
+  }
+  sr[t] = ar;
+  si[t] = ai;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if (t < w) {
+      sr[t] += sr[t + w];
+      si[t] += si[t + w];
+    }
+    __syncthreads();
+  }
+  const double tr_re = sr[0], tr_im = si[0];
+  const double mu_re = tr_re / n, mu_im = tr_im / n;
+  double m0 = 0.0, m1 = 0.0;
+  for (int j = t; j < n; j += 1024) {
+    const double2 d = diag[j];
+    const double c0 = off[j] + sqrt(d.x * d.x + d.y * d.y);
+    const double dr = d.x - mu_re, di = d.y - mu_im;
+    const double c1 = off[j] + sqrt(dr * dr + di * di);
+    m0 = fmax(m0, c0);
+    m1 = fmax(m1, c1);
+    if (c0 != c0) m0 = c0;  // propagate NaN
+    if (c1 != c1) m1 = c1;
+  }
+  __syncthreads();
+  s0[t] = m0;
+  s1[t] = m1;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if (t < w) {
+      const double x0 = s0[t + w], x1 = s1[t + w];
+      s0[t] = (x0 != x0) ? x0 : fmax(s0[t], x0);
+      s1[t] = (x1 != x1) ? x1 : fmax(s1[t], x1);
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    out->tr_re = tr_re;
+    out->tr_im = tr_im;
+    out->mu_re = mu_re;
+    out->mu_im = mu_im;
+    out->norm_plain = s0[0];
+    out->norm_shift = s1[0];
+    out->flags = *flags;
+    out->info = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Input -> planar working matrix: w = (A - mu I) * scale.
+// structured: A = i*G with G real symmetric; only wr = (G - Im(mu) I)*scale.
+template <typename Tin, typename W>
+__global__ void prepare_planar(const Tin* __restrict__ a, int n, W* __restrict__ wr, W* __restrict__ wi,
+                               int64_t ld, double mu_re, double mu_im, double scale, int structured) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (j >= n) return;
+  const Tin e = a[(int64_t)i * n + j];
+  double re = InElem<Tin>::re(e), im = InElem<Tin>::im(e);
+  if (i == j) {
+    re -= mu_re;
+    im -= mu_im;
+  }
+  if (structured) {
+    wr[(int64_t)i * ld + j] = (W)(im * scale);
+  } else {
+    wr[(int64_t)i * ld + j] = (W)(re * scale);
+    wi[(int64_t)i * ld + j] = (W)(im * scale);
+  }
+}
+
+// out = sum_t c_t P_t + diag*I on an n x n planar region (im plane optional).
+template <typename W> struct Lincomb {
+  W* out_r;
+  W* out_i;
+  const W* in_r[4];
+  const W* in_i[4];
+  double c[4];
+  int nin;
+  double diag;
+};
+
+template <typename W>
+__global__ void lincomb_kernel(Lincomb<W> L, int n, int64_t ld) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (j >= n) return;
+  const int64_t o = (int64_t)i * ld + j;
+  W sr = (i == j) ? (W)L.diag : (W)0;
+  W si = (W)0;
+  for (int t = 0; t < L.nin; ++t) {
+    sr += (W)L.c[t] * L.in_r[t][o];
+    if (L.out_i) si += (W)L.c[t] * L.in_i[t][o];
+  }
+  L.out_r[o] = sr;
+  if (L.out_i) L.out_i[o] = si;
+}
+
+// out = in^T on an n x n planar region, 32x32 LDS tiles (im plane optional).
+template <typename W>
+__global__ __launch_bounds__(256) void transpose_kernel(const W* __restrict__ ir, const W* __restrict__ ii,
+                                                        W* __restrict__ orr, W* __restrict__ oi, int n,
+                                                        int64_t ld) {
+  __shared__ W tr[32][33];
+  __shared__ W ti[32][33];
+  const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int r = ty; r < 32; r += 8) {
+    const int i = by + r, j = bx + tx;
+    if (i < n && j < n) {
+      tr[r][tx] = ir[(int64_t)i * ld + j];
+      if (ii) ti[r][tx] = ii[(int64_t)i * ld + j];
+    }
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int i = bx + r, j = by + tx;  // output (i, j) = input (j, i)
+    if (i < n && j < n) {
+      orr[(int64_t)i * ld + j] = tr[tx][r];
+      if (ii) oi[(int64_t)i * ld + j] = ti[tx][r];
+    }
+  }
+}
+
+// Augmented FP64 system [V - U | V + U] over the padded np x 2np extent.
+// Padding rows/columns of the left block are the identity, of the right
+// block zero, so the padded LU reduces to the unpadded one.
+// structured: V real, U = i*Uq (Uq real) -> V - U = V - i Uq, V + U = V + i Uq.
+template <typename W>
+__global__ void form_system(const W* __restrict__ ur, const W* __restrict__ ui, const W* __restrict__ vr,
+                            const W* __restrict__ vi, int64_t ldw, int n, int np, double* __restrict__ lr,
+                            double* __restrict__ li, int64_t ldl, int structured) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;  // 0 .. 2np
+  const int i = blockIdx.y;
+  if (j >= 2 * np) return;
+  const int jj = j < np ? j : j - np;
+  const bool left = j < np;
+  double re = 0.0, im = 0.0;
+  if (i < n && jj < n) {
+    const int64_t o = (int64_t)i * ldw + jj;
+    if (structured) {
+      const double v = (double)vr[o], u = (double)ur[o];
+      re = v;
+      im = left ? -u : u;
+    } else {
+      const double v0 = (double)vr[o], v1 = (double)vi[o];
+      const double u0 = (double)ur[o], u1 = (double)ui[o];
+      re = left ? v0 - u0 : v0 + u0;
+      im = left ? v1 - u1 : v1 + u1;
+    }
+  } else if (left && i == jj) {
+    re = 1.0;
+  }
+  lr[(int64_t)i * ldl + j] = re;
+  li[(int64_t)i * ldl + j] = im;
+}
+
+// X (FP64, right block of the system) -> working precision, optionally
+// symmetrised: x = (X + X^T)/2.
+template <typename W>
+__global__ __launch_bounds__(256) void solution_to_work(const double* __restrict__ xr, const double* __restrict__ xi,
+                                                        int64_t ldx, W* __restrict__ wr, W* __restrict__ wi,
+                                                        int64_t ldw, int n, int symmetrize) {
+  __shared__ double tr[32][33];
+  __shared__ double ti[32][33];
+  const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  if (symmetrize) {
+    for (int r = ty; r < 32; r += 8) {  // load transposed block (bx.., by..)
+      const int i = bx + r, j = by + tx;
+      if (i < n && j < n) {
+        tr[r][tx] = xr[(int64_t)i * ldx + j];
+        ti[r][tx] = xi[(int64_t)i * ldx + j];
+      }
+    }
+    __syncthreads();
+  }
+  for (int r = ty; r < 32; r += 8) {
+    const int i = by + r, j = bx + tx;
+    if (i < n && j < n) {
+      double vr = xr[(int64_t)i * ldx + j], vi = xi[(int64_t)i * ldx + j];
+      if (symmetrize) {
+        vr = 0.5 * (vr + tr[tx][r]);
+        vi = 0.5 * (vi + ti[tx][r]);
+      }
+      wr[(int64_t)i * ldw + j] = (W)vr;
+      wi[(int64_t)i * ldw + j] = (W)vi;
+    }
+  }
+}
+
+// Output: out[i][j] = e^mu * (xr + i xi), interleaved complex or real.
+template <typename Tout> struct OutElem;
+template <> struct OutElem<float2> {
+  static __device__ __forceinline__ float2 make(double re, double im) { return make_float2((float)re, (float)im); }
+};
+template <> struct OutElem<double2> {
+  static __device__ __forceinline__ double2 make(double re, double im) { return make_double2(re, im); }
+};
+template <> struct OutElem<float> {
+  static __device__ __forceinline__ float make(double re, double) { return (float)re; }
+};
+template <> struct OutElem<double> {
+  static __device__ __forceinline__ double make(double re, double) { return re; }
+};
+
+template <typename W, typename Tout>
+__global__ void write_output(const W* __restrict__ xr, const W* __restrict__ xi, int64_t ld, int n,
+                             double ph_re, double ph_im, Tout* __restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (j >= n) return;
+  const int64_t o = (int64_t)i * ld + j;
+  const double a = (double)xr[o], b = (double)xi[o];
+  out[(int64_t)i * n + j] = OutElem<Tout>::make(ph_re * a - ph_im * b, ph_re * b + ph_im * a);
+}
+
+// Fill an n x n output with NaN (non-finite input).
+template <typename Tout>
+__global__ void fill_nan(Tout* out, int64_t count) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < count) out[k] = OutElem<Tout>::make(__builtin_nan(""), __builtin_nan(""));
+}
+
+// ---------------------------------------------------------------------------
+// FP64 blocked LU with partial pivoting on the augmented planar system
+// S = [M | B] (np rows, 2np columns, ld), panel width LU_NB.
+//
+// Panel factorisation runs one launch per column.  Workgroup w owns rows
+// [k0 + w*PANEL_R, k0 + (w+1)*PANEL_R) of the panel and writes only those
+// rows, so no two workgroups write the same address.  Launch j:
+//   1. every workgroup reduces the candidates published by launch j-1
+//      (its local argmax of |re|+|im| in column k0+j and that row's panel
+//      values) to the global pivot p (ties -> smaller row index);
+//   2. the owner of row c = k0+j stores the pivot row there, the owner of
+//      row p stores the old row c (published as "top") there;
+//   3. every row below c is scaled and rank-1 updated within the panel;
+//   4. the local argmax of column c+1 and the new row c+1 are published for
+//      launch j+1 (buffers alternate by parity of j).
+constexpr int LU_NB = 64;
+constexpr int PANEL_R = 64;
+
+struct PanelCand {
+  double val;
+  int idx;
+  int pad;
+  double re[LU_NB];
+  double im[LU_NB];
+};
+
+__device__ __forceinline__ double cabs1(double r, double i) { return fabs(r) + fabs(i); }
+
+// Workgroup argmax over (val, idx): larger val wins, ties -> smaller idx.
+__device__ __forceinline__ void argmax_merge(double& v, int& k, double v2, int k2) {
+  if (v2 > v || (v2 == v && k2 >= 0 && (k < 0 || k2 < k))) {
+    v = v2;
+    k = k2;
+  }
+}
+
+__global__ __launch_bounds__(256) void lu_panel_init(const double* __restrict__ sr, const double* __restrict__ si,
+                                                     int64_t ld, int np, int k0, PanelCand* cand,
+                                                     double* top) {
+  __shared__ double sv[256];
+  __shared__ int sk[256];
+  const int w = blockIdx.x;
+  const int r0 = k0 + w * PANEL_R;
+  const int t = threadIdx.x;
+  double v = -1.0;
+  int k = -1;
+  if (t < PANEL_R) {
+    const int i = r0 + t;
+    if (i < np) {
+      v = cabs1(sr[(int64_t)i * ld + k0], si[(int64_t)i * ld + k0]);
+      k = i;
+      if (!(v >= 0.0)) v = -1.0, k = -1;  // NaN never wins
+    }
+  }
+  sv[t] = v;
+  sk[t] = k;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) {
+      double a = sv[t];
+      int ka = sk[t];
+      argmax_merge(a, ka, sv[t + s], sk[t + s]);
+      sv[t] = a;
+      sk[t] = ka;
+    }
+    __syncthreads();
+  }
+  const int best = sk[0];
+  if (t == 0) {
+    cand[w].val = sv[0];
+    cand[w].idx = best;
+  }
+  if (best >= 0 && t < LU_NB) {
+    cand[w].re[t] = sr[(int64_t)best * ld + k0 + t];
+    cand[w].im[t] = si[(int64_t)best * ld + k0 + t];
+  }
+  if (r0 <= k0 && k0 < r0 + PANEL_R && t < LU_NB) {
+    top[t] = sr[(int64_t)k0 * ld + k0 + t];
+    top[LU_NB + t] = si[(int64_t)k0 * ld + k0 + t];
+  }
+}
+
+__global__ __launch_bounds__(256) void lu_panel_step(double* __restrict__ sr, double* __restrict__ si, int64_t ld,
+                                                     int np, int k0, int j, int G, const PanelCand* __restrict__ cin,
+                                                     PanelCand* __restrict__ cout, const double* __restrict__ tin,
+                                                     double* __restrict__ tout, int* __restrict__ ipiv,
+                                                     Scalars* sc) {
+  __shared__ double rr[PANEL_R][LU_NB + 1];
+  __shared__ double ri[PANEL_R][LU_NB + 1];
+  __shared__ double pr[LU_NB], pi[LU_NB];
+  __shared__ double sv[256];
+  __shared__ int sk[256];
+  __shared__ int swin;
+  const int t = threadIdx.x;
+  const int w = blockIdx.x;
+  const int c = k0 + j;
+  const int r0 = k0 + w * PANEL_R;
+
+  // 1. global pivot: reduce (val, row, publishing workgroup) triplets
+  __shared__ int sw[256];
+  {
+    double v = -1.0;
+    int k = -1, q0 = -1;
+    for (int q = t; q < G; q += 256) {
+      const double v2 = cin[q].val;
+      const int k2 = cin[q].idx;
+      if (k2 >= 0 && (v2 > v || (v2 == v && (k < 0 || k2 < k)))) {
+        v = v2;
+        k = k2;
+        q0 = q;
+      }
+    }
+    sv[t] = v;
+    sk[t] = k;
+    sw[t] = q0;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (t < s) {
+        const double v2 = sv[t + s];
+        const int k2 = sk[t + s];
+        if (k2 >= 0 && (v2 > sv[t] || (v2 == sv[t] && (sk[t] < 0 || k2 < sk[t])))) {
+          sv[t] = v2;
+          sk[t] = k2;
+          sw[t] = sw[t + s];
+        }
+      }
+      __syncthreads();
+    }
+    if (t == 0) swin = sw[0];
+    __syncthreads();
+  }
+  const int p = sk[0] >= 0 ? sk[0] : c;
+  const double pval = sv[0];
+  const int win = swin;
+  if (t < LU_NB) {
+    if (win >= 0) {
+      pr[t] = cin[win].re[t];
+      pi[t] = cin[win].im[t];
+    } else {
+      pr[t] = sr[(int64_t)c * ld + k0 + t];
+      pi[t] = si[(int64_t)c * ld + k0 + t];
+    }
+  }
+  if (w == 0 && t == 0) {
+    ipiv[c] = p;
+    if (!(pval > 0.0)) atomicOr(&sc->info, 1);
+  }
+  __syncthreads();
+
+  // 2./3. my rows
+  const int lo = max(r0, c), hi = min(r0 + PANEL_R, np);
+  if (lo < hi) {
+    // load rows lo..hi-1 (entire panel width)
+    for (int e = t; e < (hi - lo) * LU_NB; e += 256) {
+      const int lr = e / LU_NB, col = e % LU_NB;
+      const int i = lo + lr;
+      double vr, vi;
+      if (i == c) {
+        vr = pr[col];
+        vi = pi[col];
+      } else if (i == p) {
+        vr = tin[col];
+        vi = tin[LU_NB + col];
+      } else {
+        vr = sr[(int64_t)i * ld + k0 + col];
+        vi = si[(int64_t)i * ld + k0 + col];
+      }
+      rr[i - r0][col] = vr;
+      ri[i - r0][col] = vi;
+    }
+    __syncthreads();
+    // reciprocal of the pivot
+    const double dr = pr[j], di = pi[j];
+    const double den = dr * dr + di * di;
+    const bool ok = den > 0.0;
+    const double qr = ok ? dr / den : 0.0, qi = ok ? -di / den : 0.0;
+    // thread -> (row, column group of 16); column j is left untouched
+    // until every thread of the row has read it
+    const int lr = t >> 2, cg = t & 3;
+    const int i = r0 + lr;
+    const bool upd = i > c && i < hi;
+    if (upd) {
+      const double ar = rr[lr][j], ai = ri[lr][j];
+      const double lre = ar * qr - ai * qi, lim = ar * qi + ai * qr;
+      for (int col = cg * 16; col < cg * 16 + 16; ++col) {
+        if (col > j) {
+          rr[lr][col] -= lre * pr[col] - lim * pi[col];
+          ri[lr][col] -= lre * pi[col] + lim * pr[col];
+        }
+      }
+    }
+    __syncthreads();
+    if (upd && cg == 0) {
+      const double ar = rr[lr][j], ai = ri[lr][j];
+      rr[lr][j] = ar * qr - ai * qi;
+      ri[lr][j] = ar * qi + ai * qr;
+    }
+    __syncthreads();
+    // write back
+    for (int e = t; e < (hi - lo) * LU_NB; e += 256) {
+      const int q = e / LU_NB, col = e % LU_NB;
+      const int ii = lo + q;
+      sr[(int64_t)ii * ld + k0 + col] = rr[ii - r0][col];
+      si[(int64_t)ii * ld + k0 + col] = ri[ii - r0][col];
+    }
+  }
+  // 4. candidates for column j+1
+  if (j + 1 < LU_NB) {
+    const int jn = j + 1, cn = c + 1;
+    double v = -1.0;
+    int k = -1;
+    if (t < PANEL_R) {
+      const int ii = r0 + t;
+      if (ii >= cn && ii < np && ii >= lo && ii < hi) {
+        v = cabs1(rr[t][jn], ri[t][jn]);
+        k = ii;
+        if (!(v >= 0.0)) v = -1.0, k = -1;
+      }
+    }
+    __syncthreads();
+    sv[t] = v;
+    sk[t] = k;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (t < s) {
+        double a = sv[t];
+        int ka = sk[t];
+        argmax_merge(a, ka, sv[t + s], sk[t + s]);
+        sv[t] = a;
+        sk[t] = ka;
+      }
+      __syncthreads();
+    }
+    const int best = sk[0];
+    if (t == 0) {
+      cout[w].val = sv[0];
+      cout[w].idx = best;
+    }
+    if (best >= 0 && t < LU_NB) {
+      cout[w].re[t] = rr[best - r0][t];
+      cout[w].im[t] = ri[best - r0][t];
+    }
+    if (r0 <= cn && cn < r0 + PANEL_R && cn < np && t < LU_NB) {
+      tout[t] = rr[cn - r0][t];
+      tout[LU_NB + t] = ri[cn - r0][t];
+    }
+  }
+}
+
+// Apply the panel's row interchanges to columns [c0, c1).
+__global__ void lu_laswp(double* __restrict__ sr, double* __restrict__ si, int64_t ld, int k0, int nb,
+                         const int* __restrict__ ipiv, int c0, int c1) {
+  const int col = c0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= c1) return;
+  for (int q = 0; q < nb; ++q) {
+    const int r = k0 + q, p = ipiv[r];
+    if (p != r) {
+      const int64_t a = (int64_t)r * ld + col, b = (int64_t)p * ld + col;
+      const double xr = sr[a], xi = si[a];
+      sr[a] = sr[b];
+      si[a] = si[b];
+      sr[b] = xr;
+      si[b] = xi;
+    }
+  }
+}
+
+// Block-row triangular solve, 64 columns per workgroup of 64 threads:
+//   lower (unit):   S[k0:k0+64, cols] = L11^{-1} S[k0:k0+64, cols]
+//   upper (non-unit): S[k0:k0+64, cols] = U11^{-1} S[k0:k0+64, cols]
+// The solved block is also written transposed (row = column index - c0,
+// 64 k-values) to tr/ti (ld 64) as the right operand of the following GEMM.
+template <bool UPPER>
+__global__ __launch_bounds__(64) void lu_trsm_block_row(double* __restrict__ sr, double* __restrict__ si, int64_t ld,
+                                                        int k0, int c0, int c1, double* __restrict__ tr,
+                                                        double* __restrict__ ti) {
+  __shared__ double lr[LU_NB][LU_NB + 1];
+  __shared__ double li[LU_NB][LU_NB + 1];
+  __shared__ double xr[LU_NB][64 + 1];
+  __shared__ double xi[LU_NB][64 + 1];
+  __shared__ double dr[LU_NB], di[LU_NB];
+  const int t = threadIdx.x;
+  const int cb = c0 + blockIdx.x * 64;
+  for (int e = t; e < LU_NB * LU_NB; e += 64) {
+    const int r = e / LU_NB, q = e % LU_NB;
+    lr[r][q] = sr[(int64_t)(k0 + r) * ld + k0 + q];
+    li[r][q] = si[(int64_t)(k0 + r) * ld + k0 + q];
+  }
+  const int col = cb + t;
+  const bool active = col < c1;
+  for (int r = 0; r < LU_NB; ++r) {
+    xr[r][t] = active ? sr[(int64_t)(k0 + r) * ld + col] : 0.0;
+    xi[r][t] = active ? si[(int64_t)(k0 + r) * ld + col] : 0.0;
+  }
+  __syncthreads();
+  if (UPPER) {
+    if (t < LU_NB) {
+      const double a = lr[t][t], b = li[t][t];
+      const double den = a * a + b * b;
+      dr[t] = a / den;
+      di[t] = -b / den;
+    }
+    __syncthreads();
+    for (int r = LU_NB - 1; r >= 0; --r) {
+      double ar = xr[r][t], ai = xi[r][t];
+      for (int q = r + 1; q < LU_NB; ++q) {
+        const double ur = lr[r][q], ui = li[r][q];
+        const double br = xr[q][t], bi = xi[q][t];
+        ar -= ur * br - ui * bi;
+        ai -= ur * bi + ui * br;
+      }
+      const double zr = ar * dr[r] - ai * di[r], zi = ar * di[r] + ai * dr[r];
+      xr[r][t] = zr;
+      xi[r][t] = zi;
+    }
+  } else {
+    for (int r = 1; r < LU_NB; ++r) {
+      double ar = xr[r][t], ai = xi[r][t];
+      for (int q = 0; q < r; ++q) {
+        const double ur = lr[r][q], ui = li[r][q];
+        const double br = xr[q][t], bi = xi[q][t];
+        ar -= ur * br - ui * bi;
+        ai -= ur * bi + ui * br;
+      }
+      xr[r][t] = ar;
+      xi[r][t] = ai;
+    }
+  }
+  if (active) {
+    for (int r = 0; r < LU_NB; ++r) {
+      sr[(int64_t)(k0 + r) * ld + col] = xr[r][t];
+      si[(int64_t)(k0 + r) * ld + col] = xi[r][t];
+      tr[(int64_t)(col - c0) * LU_NB + r] = xr[r][t];
+      ti[(int64_t)(col - c0) * LU_NB + r] = xi[r][t];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// psi_out[s][i] = e^mu sum_j X[i][j] psi0[s][j], FP64 accumulation; one
+// wavefront per row, states looped (nstates small).
+template <typename W>
+__global__ __launch_bounds__(256) void apply_states(const W* __restrict__ xr, const W* __restrict__ xi, int64_t ld,
+                                                    int n, const double2* __restrict__ psi0, int nstates,
+                                                    double ph_re, double ph_im, double2* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= n) return;
+  for (int s = 0; s < nstates; ++s) {
+    const double2* p = psi0 + (int64_t)s * n;
+    double ar = 0.0, ai = 0.0;
+    for (int j = lane; j < n; j += 64) {
+      const double a = (double)xr[(int64_t)row * ld + j], b = (double)xi[(int64_t)row * ld + j];
+      const double2 v = p[j];
+      ar += a * v.x - b * v.y;
+      ai += a * v.y + b * v.x;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      ar += __shfl_xor(ar, off);
+      ai += __shfl_xor(ai, off);
+    }
+    if (lane == 0) out[(int64_t)s * n + row] = make_double2(ph_re * ar - ph_im * ai, ph_re * ai + ph_im * ar);
+  }
+}
+
+}  // namespace fexpm

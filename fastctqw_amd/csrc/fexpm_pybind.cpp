@@ -1,0 +1,206 @@
+// pybind11 module `_fastexpm_core`: the same four classes, constructor,
+// method and keyword names as the reference binding (src/main.cu:8-64):
+//   ExpMatFloat32 / ExpMatFloat64 / ExpMatComplex64 / ExpMatComplex128
+//   __init__(n), run(arr_a) -> ndarray, free()
+// Every call goes through the C ABI declared in include/fastexpm.h; no HIP
+// or device type appears here.  The GIL is released while the device works
+// (the engine serialises calls on one instance itself).
+//
+// Extra methods beyond the reference (used by Operator/Ctqw and the tests):
+//   upload(arr_a), run_resident(), download(), run_device(ptr_a, ptr_u),
+//   apply(psi0) -> psi (U of the last run times states, FP64 accumulation),
+//   stats() -> dict, set_option(key, value), n (property).
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+
+#include <complex>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/fastexpm.h"
+
+namespace py = pybind11;
+
+static void raise_status(int rc, const fexpm_engine* e) {
+  if (rc == FEXPM_OK) return;
+  std::string msg = std::string("fastexpm: ") + fexpm_status_string(rc);
+  if (e) {
+    const char* d = fexpm_last_error(e);
+    if (d && *d) msg += std::string(": ") + d;
+  }
+  switch (rc) {
+    case FEXPM_ENOMEM:
+      throw std::bad_alloc();
+    case FEXPM_EINVAL:
+      throw std::invalid_argument(msg);
+    default:
+      throw std::runtime_error(msg);
+  }
+}
+
+template <typename T, int DT> class ExpMat {
+ public:
+  explicit ExpMat(int64_t n) : n_(n) {
+    if (n < 0) throw std::invalid_argument("fastexpm: n must be non-negative");
+    fexpm_engine* e = nullptr;
+    int rc = fexpm_create(n, DT, &e);
+    raise_status(rc, nullptr);
+    e_ = e;
+  }
+  ~ExpMat() {
+    if (e_) fexpm_destroy(e_);
+  }
+  ExpMat(const ExpMat&) = delete;
+  ExpMat& operator=(const ExpMat&) = delete;
+
+  py::array_t<T> run(py::array_t<T, py::array::c_style | py::array::forcecast> a) {
+    check_shape(a);
+    py::array_t<T> out({n_, n_});
+    const T* src = a.data();
+    T* dst = out.mutable_data();
+    int rc;
+    {
+      py::gil_scoped_release nogil;
+      rc = fexpm_run(e_, src, dst);
+    }
+    raise_status(rc, e_);
+    return out;
+  }
+
+  void upload(py::array_t<T, py::array::c_style | py::array::forcecast> a) {
+    check_shape(a);
+    const T* src = a.data();
+    int rc;
+    {
+      py::gil_scoped_release nogil;
+      rc = fexpm_upload(e_, src);
+    }
+    raise_status(rc, e_);
+  }
+
+  void run_resident() {
+    int rc;
+    {
+      py::gil_scoped_release nogil;
+      rc = fexpm_run_resident(e_);
+    }
+    raise_status(rc, e_);
+  }
+
+  py::array_t<T> download() {
+    py::array_t<T> out({n_, n_});
+    T* dst = out.mutable_data();
+    int rc;
+    {
+      py::gil_scoped_release nogil;
+      rc = fexpm_download(e_, dst);
+    }
+    raise_status(rc, e_);
+    return out;
+  }
+
+  void run_device(uintptr_t d_a, uintptr_t d_u, uintptr_t stream) {
+    int rc;
+    {
+      py::gil_scoped_release nogil;
+      rc = fexpm_run_device(e_, reinterpret_cast<const void*>(d_a), reinterpret_cast<void*>(d_u),
+                            reinterpret_cast<void*>(stream));
+    }
+    raise_status(rc, e_);
+  }
+
+  py::array_t<std::complex<double>> apply(
+      py::array_t<std::complex<double>, py::array::c_style | py::array::forcecast> psi0) {
+    int64_t ns;
+    if (psi0.ndim() == 1) {
+      if (psi0.shape(0) != n_) throw std::invalid_argument("fastexpm.apply: state length must be n");
+      ns = 1;
+    } else if (psi0.ndim() == 2) {
+      if (psi0.shape(1) != n_) throw std::invalid_argument("fastexpm.apply: states must have shape (k, n)");
+      ns = psi0.shape(0);
+    } else {
+      throw std::invalid_argument("fastexpm.apply: expected a 1-D or 2-D array");
+    }
+    std::vector<ssize_t> shape(psi0.shape(), psi0.shape() + psi0.ndim());
+    py::array_t<std::complex<double>> out(shape);
+    const void* src = psi0.data();
+    void* dst = out.mutable_data();
+    int rc;
+    {
+      py::gil_scoped_release nogil;
+      rc = fexpm_apply(e_, src, ns, dst);
+    }
+    raise_status(rc, e_);
+    return out;
+  }
+
+  void free() {
+    int rc = fexpm_free(e_);
+    raise_status(rc, e_);
+  }
+
+  void set_option(int key, int64_t value) { raise_status(fexpm_set_option(e_, key, value), e_); }
+
+  py::dict stats() const {
+    fexpm_stats s;
+    raise_status(fexpm_get_stats(e_, &s), e_);
+    py::dict d;
+    d["n"] = s.n;
+    d["dtype"] = s.dtype;
+    d["m"] = s.m;
+    d["s"] = s.s;
+    d["structured"] = (bool)s.structured;
+    d["shifted"] = (bool)s.shifted;
+    d["nonfinite"] = (bool)s.nonfinite;
+    d["mu"] = std::complex<double>(s.mu_re, s.mu_im);
+    d["norm"] = s.norm;
+    d["norm_plain"] = s.norm_plain;
+    d["ms_total"] = s.ms_total;
+    d["ms_analyze"] = s.ms_analyze;
+    d["ms_pade"] = s.ms_pade;
+    d["ms_solve"] = s.ms_solve;
+    d["ms_square"] = s.ms_square;
+    d["ms_output"] = s.ms_output;
+    d["flops_issued"] = s.flops_issued;
+    return d;
+  }
+
+  int64_t n() const { return n_; }
+
+ private:
+  template <typename A> void check_shape(const A& a) const {
+    if (a.ndim() != 2 || a.shape(0) != n_ || a.shape(1) != n_)
+      throw std::invalid_argument("fastexpm: input matrix must have shape (n, n)");
+  }
+  int64_t n_;
+  fexpm_engine* e_ = nullptr;
+};
+
+template <typename T, int DT> static void bind(py::module_& m, const char* name) {
+  using C = ExpMat<T, DT>;
+  py::class_<C>(m, name)
+      .def(py::init<int64_t>(), py::arg("n"),
+           "Initializes the MatrixExpCalculator for n x n matrices, allocating GPU memory.")
+      .def("run", &C::run, py::arg("arr_a"),
+           "Performs matrix exponential calculation on the given NumPy array.")
+      .def("free", &C::free, "Free memory")
+      .def("upload", &C::upload, py::arg("arr_a"))
+      .def("run_resident", &C::run_resident)
+      .def("download", &C::download)
+      .def("run_device", &C::run_device, py::arg("d_a"), py::arg("d_u"), py::arg("stream") = 0)
+      .def("apply", &C::apply, py::arg("psi0"))
+      .def("set_option", &C::set_option, py::arg("key"), py::arg("value"))
+      .def("stats", &C::stats)
+      .def_property_readonly("n", &C::n);
+}
+
+PYBIND11_MODULE(_fastexpm_core, m) {
+  m.doc() = "MI355X matrix exponential engine (C ABI: include/fastexpm.h)";
+  bind<float, FEXPM_FLOAT32>(m, "ExpMatFloat32");
+  bind<double, FEXPM_FLOAT64>(m, "ExpMatFloat64");
+  bind<std::complex<float>, FEXPM_COMPLEX64>(m, "ExpMatComplex64");
+  bind<std::complex<double>, FEXPM_COMPLEX128>(m, "ExpMatComplex128");
+  m.attr("OPT_PRECISION") = (int)FEXPM_OPT_PRECISION;
+  m.attr("OPT_STRUCTURE") = (int)FEXPM_OPT_STRUCTURE;
+  m.def("gpu_memory_gb", &fexpm_gpu_memory_gb);
+}

@@ -1,0 +1,94 @@
+"""Python wrappers over the pybind11 module ``_fastexpm_core``.
+
+Behaviour follows the reference wrappers (FastCTQW/fastexpm/_wrapper.py:9-130):
+``run`` raises TypeError for a non-ndarray (:92) and ValueError for a shape
+other than (N, N) (:94), returns ``backend.run(A).reshape(N, N)`` (:96);
+``__exit__`` frees and turns a failing ``free`` into a warning without
+swallowing the original exception (:79-88).  ``free`` is idempotent.
+
+The native module is required: there is no CPU fallback.  If it has not been
+built, importing this package raises ImportError naming the build command.
+"""
+from __future__ import annotations
+
+import warnings
+
+import numpy as np
+
+try:
+    from . import _fastexpm_core
+except ImportError as exc:  # pragma: no cover - exercised only without a build
+    raise ImportError(
+        "fastexpm native module is not built: run `python -m fastctqw_amd.build` "
+        f"(gfx950 HIP library + pybind11 layer). Original error: {exc}"
+    ) from exc
+
+
+class _ExpMatBase:
+    _core_cls = None
+
+    def __init__(self, N: int):
+        self.N = N
+        self._backend = self._core_cls(N)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, exc_type, exc_val, exc_tb):
+        try:
+            self.free()
+        except Exception as cleanup_error:
+            warnings.warn(f"[fastexpm] Failed to free resources in __exit__: {cleanup_error}")
+        return False
+
+    def run(self, A) -> np.ndarray:
+        if not isinstance(A, np.ndarray):
+            raise TypeError("Input must be a NumPy array.")
+        if A.shape != (self.N, self.N):
+            raise ValueError(f"Input matrix must have shape ({self.N}, {self.N}).")
+        return self._backend.run(A).reshape(self.N, self.N)
+
+    def free(self):
+        self._backend.free()
+
+    # ---- device-resident extensions (no counterpart in the reference) ----
+    def upload(self, A: np.ndarray) -> None:
+        """Copy A to the engine's device buffer."""
+        if not isinstance(A, np.ndarray):
+            raise TypeError("Input must be a NumPy array.")
+        if A.shape != (self.N, self.N):
+            raise ValueError(f"Input matrix must have shape ({self.N}, {self.N}).")
+        self._backend.upload(A)
+
+    def run_resident(self) -> None:
+        """U = expm(A) for the uploaded A; U stays on the device."""
+        self._backend.run_resident()
+
+    def download(self) -> np.ndarray:
+        return self._backend.download()
+
+    def apply(self, psi0) -> np.ndarray:
+        """psi = U psi0 (complex128) for the U of the last run, FP64 accumulation."""
+        return self._backend.apply(np.asarray(psi0, dtype=np.complex128))
+
+    def stats(self) -> dict:
+        return self._backend.stats()
+
+    def set_structure_detection(self, enabled: bool) -> None:
+        self._backend.set_option(_fastexpm_core.OPT_STRUCTURE, 1 if enabled else 0)
+
+
+class ExpMatFloat32(_ExpMatBase):
+    _core_cls = _fastexpm_core.ExpMatFloat32
+
+
+class ExpMatFloat64(_ExpMatBase):
+    _core_cls = _fastexpm_core.ExpMatFloat64
+
+
+class ExpMatComplex64(_ExpMatBase):
+    _core_cls = _fastexpm_core.ExpMatComplex64
+
+
+class ExpMatComplex128(_ExpMatBase):
+    _core_cls = _fastexpm_core.ExpMatComplex128

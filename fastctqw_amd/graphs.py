@@ -1,0 +1,180 @@
+"""Seeded, reproducible graph and state generators for CTQW workloads.
+
+The reference densifies networkx graphs (``Ctqw.from_networkx_graph``,
+FastCTQW/Ctqw.py:46-57) and builds ``H = -gamma*A`` or ``-gamma*(D - A)``
+(FastCTQW/Operator.py:49-53).  Its own tests draw graphs from unseeded
+networkx generators (tests/test_fastexpm_accuracy.py:110,137), so nothing
+they produce is reproducible.  This module gives every workload a seed:
+
+* Erdos-Renyi edges, edge weights and random states come from a
+  counter-based hash (splitmix64 of (seed, stream, i, j)), so an N = 10^4
+  graph is reproducible from three integers without shipping 800 MB, and the
+  same bits can be produced by a device kernel.
+* Random d-regular graphs use the Steger-Wormald pairing procedure driven by
+  numpy's PCG64 stream (bit-stable across numpy releases).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+_MIX1 = np.uint64(0xBF58476D1CE4E5B9)
+_MIX2 = np.uint64(0x94D049BB133111EB)
+
+
+def splitmix64(x):
+    """Vectorised splitmix64 finaliser on uint64 arrays (wrapping arithmetic)."""
+    x = np.asarray(x, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = x + _GOLDEN
+        z = (z ^ (z >> np.uint64(30))) * _MIX1
+        z = (z ^ (z >> np.uint64(27))) * _MIX2
+        return z ^ (z >> np.uint64(31))
+
+
+def _stream_key(seed: int, stream: int) -> np.uint64:
+    with np.errstate(over="ignore"):
+        k = splitmix64(np.uint64(seed & 0xFFFFFFFFFFFFFFFF))
+        return splitmix64(k ^ np.uint64(stream & 0xFFFFFFFFFFFFFFFF))
+
+
+def uniform01(seed: int, stream: int, counters) -> np.ndarray:
+    """U[0,1) doubles from 53 hash bits of (seed, stream, counter)."""
+    key = _stream_key(seed, stream)
+    with np.errstate(over="ignore"):
+        h = splitmix64(np.asarray(counters, dtype=np.uint64) ^ key)
+    return (h >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+# Hash streams: one per quantity so that edges and weights are independent.
+STREAM_EDGE = 1
+STREAM_WEIGHT = 2
+STREAM_STATE_RE = 3
+STREAM_STATE_IM = 4
+
+
+def _pair_counter(i, j, n):
+    return i.astype(np.uint64) * np.uint64(n) + j.astype(np.uint64)
+
+
+def erdos_renyi(n: int, p: float, seed: int, weighted: bool = False,
+                dtype=np.float64) -> np.ndarray:
+    """Dense symmetric adjacency of G(n, p); edge (i<j) iff u(seed,i,j) < p.
+
+    Weighted graphs carry w_ij = 1 - u'(seed,i,j) in (0, 1].  Rows are
+    produced in blocks so that n = 10^4 stays within a few hundred MB.
+    """
+    a = np.zeros((n, n), dtype=dtype)
+    blk = max(1, min(n, (1 << 24) // max(n, 1)))
+    cols = np.arange(n, dtype=np.uint64)
+    for r0 in range(0, n, blk):
+        r1 = min(n, r0 + blk)
+        rows = np.arange(r0, r1, dtype=np.uint64)[:, None]
+        ctr = rows * np.uint64(n) + cols[None, :]
+        mask = uniform01(seed, STREAM_EDGE, ctr) < p
+        mask &= cols[None, :] > rows  # strict upper triangle
+        if weighted:
+            w = 1.0 - uniform01(seed, STREAM_WEIGHT, ctr)
+            a[r0:r1] = np.where(mask, w, 0.0)
+        else:
+            a[r0:r1] = mask
+    # mirror the strict upper triangle into the lower one
+    a += a.T
+    return a
+
+
+def random_regular(n: int, d: int, seed: int, weighted: bool = False,
+                   dtype=np.float64, max_restarts: int = 1000) -> np.ndarray:
+    """Dense adjacency of a random simple d-regular graph (Steger-Wormald).
+
+    Points (vertex, stub) are paired one random pair at a time; a pair is
+    accepted when it joins two distinct vertices not yet adjacent.  When no
+    suitable pair is left the procedure restarts with the same stream.
+    """
+    if (n * d) % 2 != 0 or d >= n or d < 0:
+        raise ValueError(f"no simple {d}-regular graph on {n} vertices")
+    rng = np.random.Generator(np.random.PCG64(seed))
+    for _ in range(max_restarts):
+        edges = _sw_attempt(n, d, rng)
+        if edges is not None:
+            break
+    else:  # pragma: no cover - astronomically unlikely for d <= 8
+        raise RuntimeError("random_regular: too many restarts")
+    a = np.zeros((n, n), dtype=dtype)
+    if not edges:
+        return a
+    e = np.asarray(edges, dtype=np.int64)
+    if weighted:
+        lo = np.minimum(e[:, 0], e[:, 1]).astype(np.uint64)
+        hi = np.maximum(e[:, 0], e[:, 1]).astype(np.uint64)
+        w = 1.0 - uniform01(seed, STREAM_WEIGHT, lo * np.uint64(n) + hi)
+    else:
+        w = 1.0
+    a[e[:, 0], e[:, 1]] = w
+    a[e[:, 1], e[:, 0]] = w
+    return a
+
+
+def _sw_attempt(n, d, rng):
+    points = np.repeat(np.arange(n), d)
+    adj = [set() for _ in range(n)]
+    edges = []
+    pts = list(points)
+    while pts:
+        # try a bounded number of random pairs before declaring a dead end
+        ok = False
+        m = len(pts)
+        for _ in range(64):
+            i, j = rng.integers(0, m, size=2)
+            if i == j:
+                continue
+            u, v = pts[i], pts[j]
+            if u != v and v not in adj[u]:
+                ok = True
+                break
+        if not ok:
+            # exhaustive search for any suitable pair among the remaining points
+            found = None
+            for i in range(m):
+                for j in range(i + 1, m):
+                    u, v = pts[i], pts[j]
+                    if u != v and v not in adj[u]:
+                        found = (i, j)
+                        break
+                if found:
+                    break
+            if found is None:
+                return None
+            i, j = found
+            u, v = pts[i], pts[j]
+        adj[u].add(v)
+        adj[v].add(u)
+        edges.append((u, v))
+        for k in sorted((i, j), reverse=True):
+            pts[k] = pts[-1]
+            pts.pop()
+    return edges
+
+
+def laplacian(a: np.ndarray) -> np.ndarray:
+    """L = D - A with D = diag(row sums) (FastCTQW/Operator.py:51)."""
+    return np.diag(a.sum(axis=1)) - a
+
+
+def random_state(n: int, seed: int, index: int = 0) -> np.ndarray:
+    """Normalised complex128 state with Re, Im ~ N(0,1) (Box-Muller on hash bits)."""
+    base = np.uint64(index) * np.uint64(2 * n) if index else np.uint64(0)
+    k = np.arange(2 * n, dtype=np.uint64) + base
+    u1 = uniform01(seed, STREAM_STATE_RE, k)
+    u2 = uniform01(seed, STREAM_STATE_IM, k)
+    r = np.sqrt(-2.0 * np.log1p(-u1))
+    g = r * np.cos(2.0 * np.pi * u2)
+    psi = g[:n] + 1j * g[n:]
+    return psi / np.linalg.norm(psi)
+
+
+def c2_laplacian(n: int = 10000, p: float = 0.5, seed: int = 2,
+                 weighted: bool = False, dtype=np.float64) -> np.ndarray:
+    """Laplacian of the seeded G(n, p) graph used for the N = 10^4 walk."""
+    return laplacian(erdos_renyi(n, p, seed, weighted=weighted, dtype=dtype))

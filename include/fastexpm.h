@@ -1,0 +1,131 @@
+/*
+ * fastexpm -- C ABI of the MI355X (gfx950) matrix-exponential engine.
+ *
+ * This is the drop-in boundary behind FastCTQW's `fastexpm` Python classes.
+ * In the reference, the pybind11 module `_fastexpm_core` (src/main.cu:8-64)
+ * binds `MatrixExpCalculator<T>` (include/MatrixExpCalculator.h:17-99),
+ * whose bodies live in a prebuilt library that is absent from the reference
+ * checkout (CMakeLists.txt:23-25).  Here the engine is a plain C ABI
+ * (`extern "C"`, plain pointers and sizes, no torch or pybind types); the
+ * module `_fastexpm_core` is a thin pybind11 layer over it.
+ *
+ * Status codes: every entry point returns FEXPM_OK (0) or a negative code;
+ * fexpm_last_error() gives the message of the last failure on an engine.
+ * An engine serialises its own calls (one mutex per engine); separate
+ * engines are independent.
+ */
+#ifndef FASTEXPM_H
+#define FASTEXPM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fexpm_engine fexpm_engine;
+
+/* Element types, matching the reference's four bound classes
+ * ExpMatFloat32 / ExpMatFloat64 / ExpMatComplex64 / ExpMatComplex128
+ * (src/main.cu:12, 25, 38, 51). */
+enum fexpm_dtype {
+  FEXPM_FLOAT32 = 0,
+  FEXPM_FLOAT64 = 1,
+  FEXPM_COMPLEX64 = 2,
+  FEXPM_COMPLEX128 = 3
+};
+
+enum fexpm_status {
+  FEXPM_OK = 0,
+  FEXPM_EINVAL = -1,     /* bad argument (size, dtype, null pointer, option) */
+  FEXPM_ENOMEM = -2,     /* device allocation failed */
+  FEXPM_EHIP = -3,       /* HIP runtime error */
+  FEXPM_ESINGULAR = -4,  /* exactly singular Pade denominator */
+  FEXPM_EFREED = -5,     /* engine memory was released by fexpm_free */
+  FEXPM_ENODEVICE = -6   /* no usable gfx950 device */
+};
+
+/* Precision of the Pade products and squarings (option FEXPM_OPT_PRECISION). */
+enum fexpm_precision {
+  FEXPM_PREC_NATIVE = 0 /* FP32 MFMA for complex64/float32, FP64 for 128/64 */
+};
+
+enum fexpm_option {
+  FEXPM_OPT_PRECISION = 1,   /* enum fexpm_precision */
+  FEXPM_OPT_STRUCTURE = 2    /* 1 = detect purely-imaginary symmetric A (default), 0 = general path */
+};
+
+/* Per-run report (filled by fexpm_get_stats after a run). */
+typedef struct fexpm_stats {
+  int64_t n;
+  int32_t dtype;
+  int32_t m;            /* Pade degree (3, 5, 7, 9 or 13) */
+  int32_t s;            /* number of squarings */
+  int32_t structured;   /* 1: A = i*G with G real symmetric (CTQW form) */
+  int32_t shifted;      /* 1: trace shift mu applied */
+  int32_t nonfinite;    /* 1: input had NaN/Inf, output is NaN */
+  double mu_re, mu_im;  /* trace shift tr(A)/n */
+  double norm;          /* 1-norm used to choose m and s */
+  double norm_plain;    /* ||A||_1 */
+  double ms_total;      /* device time of the whole run (ms) */
+  double ms_analyze;    /* norm / shift / structure analysis */
+  double ms_pade;       /* scaling + Pade products + U/V assembly */
+  double ms_solve;      /* FP64 LU + solve */
+  double ms_square;     /* s squarings */
+  double ms_output;     /* phase + write of U */
+  double flops_issued;  /* real flops issued by the MFMA kernels this run */
+} fexpm_stats;
+
+/* Create an engine for n x n matrices of the given dtype and allocate its
+ * device scratch (replaces MatrixExpCalculator<T>(size_t n),
+ * include/MatrixExpCalculator.h:37,65,93; bound at src/main.cu:14). */
+int fexpm_create(int64_t n, int dtype, fexpm_engine** out);
+
+/* Release device memory; idempotent.  Later runs fail with FEXPM_EFREED
+ * (replaces MatrixExpCalculator<T>::free, include/MatrixExpCalculator.h:42,70,98;
+ * bound at src/main.cu:21). */
+int fexpm_free(fexpm_engine* e);
+
+/* Free everything and destroy the handle (replaces ~MatrixExpCalculator). */
+int fexpm_destroy(fexpm_engine* e);
+
+/* Matrix order (replaces MatrixExpCalculator<T>::getN,
+ * include/MatrixExpCalculator.h:41,69,97). */
+int64_t fexpm_get_n(const fexpm_engine* e);
+
+/* U = expm(A) from host memory to host memory: `a` and `u` are row-major
+ * n x n arrays of the engine's dtype (complex = interleaved re, im).  `a` is
+ * not modified (replaces MatrixExpCalculator<T>::run,
+ * include/MatrixExpCalculator.h:40,68,96; bound at src/main.cu:17). */
+int fexpm_run(fexpm_engine* e, const void* a, void* u);
+
+/* Same computation on device memory: `d_a`, `d_u` are device pointers to
+ * row-major n x n arrays of the engine's dtype; `stream` is a hipStream_t
+ * (NULL = the engine's own stream).  No host transfer of matrix data. */
+int fexpm_run_device(fexpm_engine* e, const void* d_a, void* d_u, void* stream);
+
+/* Engine-resident input/output for callers without device memory of their
+ * own: upload A once, run any number of times, download U. */
+int fexpm_upload(fexpm_engine* e, const void* a);
+int fexpm_run_resident(fexpm_engine* e);
+int fexpm_download(fexpm_engine* e, void* u);
+
+/* psi[k] = U psi0[k] for k < nstates with the U of the last run, FP64
+ * accumulation on the device; psi0 and psi are host complex128 arrays of
+ * nstates x n (replaces the host product at FastCTQW/Ctqw.py:94). */
+int fexpm_apply(fexpm_engine* e, const void* psi0, int64_t nstates, void* psi);
+
+int fexpm_set_option(fexpm_engine* e, int key, int64_t value);
+int fexpm_get_stats(const fexpm_engine* e, fexpm_stats* out);
+const char* fexpm_last_error(const fexpm_engine* e);
+const char* fexpm_status_string(int status);
+
+/* Free device memory in GiB (replaces get_current_gpu_memory_gb,
+ * include/MatrixExpCalculator.h:101). Returns a negative value on error. */
+double fexpm_gpu_memory_gb(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FASTEXPM_H */

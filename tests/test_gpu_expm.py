@@ -1,0 +1,269 @@
+"""GPU parity of the fastexpm engine against the FP64 oracle.
+
+Every case prints its name before touching the GPU so that a failure can be
+attributed.  Inputs are seeded.  Tolerances (relative Frobenius error of U
+against the oracle) are stated per precision:
+
+* complex128 / float64 (FP64 everywhere): 1e-11
+* complex64 / float32 (FP32 Pade products and squarings, FP64 solve):
+  5e-6 * 2^(s/2), the random-walk growth of FP32 rounding over s squarings
+  of a unitary-scaled matrix, floored at 2e-6.
+
+The degree m and scaling s chosen on the device must equal the oracle's.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from fastctqw_amd.graphs import erdos_renyi, laplacian, random_regular
+from oracle.closed import complete_graph, cycle_graph, expm_complete, expm_cycle
+from oracle.eig import eig_propagator, taylor_expm
+from oracle.metrics import infidelity, rel_fro, unitarity_defect
+from oracle.padeexpm import pade_expm
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(f"[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def fx():
+    from fastctqw_amd import fastexpm
+
+    return fastexpm
+
+
+def _engine(fx, dtype, n):
+    return {
+        np.complex64: fx.ExpMatComplex64,
+        np.complex128: fx.ExpMatComplex128,
+        np.float32: fx.ExpMatFloat32,
+        np.float64: fx.ExpMatFloat64,
+    }[dtype](n)
+
+
+def _tol(dtype, s):
+    if dtype in (np.complex128, np.float64):
+        return 1e-11
+    return max(2e-6, 5e-6 * 2.0 ** (s / 2.0))
+
+
+def _check(fx, a, dtype, name, ref=None):
+    n = a.shape[0]
+    _say(f"{name} n={n} dtype={np.dtype(dtype).name}")
+    a = np.ascontiguousarray(a.astype(dtype))
+    st = pade_expm(a.astype(np.complex128), stages=True)
+    with _engine(fx, dtype, n) as e:
+        u = e.run(a)
+        stats = e.stats()
+    assert u.shape == (n, n) and u.dtype == np.dtype(dtype)
+    assert stats["m"] == st["m"], (stats, st["m"], st["s"])
+    assert stats["s"] == st["s"], (stats, st["m"], st["s"])
+    target = st["E"] if ref is None else ref
+    err = rel_fro(target, u)
+    tol = _tol(dtype, st["s"])
+    print(f"  m={stats['m']} s={stats['s']} structured={stats['structured']} relF={err:.3e} tol={tol:.1e}",
+          flush=True)
+    assert err <= tol, err
+    return u, stats
+
+
+# ---------------------------------------------------------------------------
+# CTQW form A = -i t H, H real symmetric (structured path)
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+@pytest.mark.parametrize("n", [1, 2, 3, 17, 64, 127, 128, 129, 300])
+def test_ctqw_laplacian_er(fx, dtype, n):
+    L = laplacian(erdos_renyi(n, 0.3, seed=100 + n, weighted=True))
+    t = 2.5
+    ref = eig_propagator(L, t)
+    u, stats = _check(fx, -1j * t * L, dtype, "ER-laplacian", ref=ref)
+    assert stats["structured"]
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+@pytest.mark.parametrize("t", [0.001, 0.05, 0.3, 1.0, 4.5, 20.0])
+def test_ctqw_degrees(fx, dtype, t):
+    """Small t exercises Pade degrees 3..9 without scaling."""
+    a = random_regular(96, 4, seed=7).astype(np.float64)
+    ref = eig_propagator(-a, t)
+    _check(fx, 1j * t * a, dtype, f"R4-adjacency t={t}", ref=ref)
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_c1_four_regular_walk(fx, dtype):
+    """4-regular graph, N=1000, t=4.5: U and psi against the eigen oracle."""
+    n, t = 1000, 4.5
+    a = random_regular(n, 4, seed=1)
+    H = a  # configuration "H = adjacency"; A_exp = -i t H
+    ref = eig_propagator(H, t)
+    u, stats = _check(fx, -1j * t * H, dtype, "C1 4-regular N=1000", ref=ref)
+    assert stats["structured"]
+    # uniform superposition is an eigenvector: psi(t) = e^{-i 4 t} psi0
+    psi0 = np.full(n, 1 / np.sqrt(n))
+    psi = u.astype(np.complex128) @ psi0
+    assert infidelity(np.exp(-4j * t) * psi0, psi) < 1e-10
+    rng = np.random.default_rng(5)
+    phi = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    phi /= np.linalg.norm(phi)
+    inf = infidelity(ref @ phi, u.astype(np.complex128) @ phi)
+    print(f"  random-state infidelity {inf:.3e}", flush=True)
+    assert inf < 1e-9
+
+
+def test_closed_forms(fx):
+    for n in (5, 64, 200):
+        a = 1j * 0.7 * complete_graph(n)
+        _check(fx, a, np.complex128, "complete", ref=expm_complete(n, 0.7))
+        _check(fx, 1j * 1.3 * cycle_graph(n), np.complex128, "cycle", ref=expm_cycle(n, 1.3))
+
+
+# ---------------------------------------------------------------------------
+# Reference test inputs (tests/test_fastexpm_accuracy.py:83-159):
+# A = 1j * adjacency for complete, ER(0.05) and 4-regular graphs, checked
+# against the reference's own Taylor oracle restated.
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+@pytest.mark.parametrize("n", [16, 64, 256])
+@pytest.mark.parametrize("family", ["complete", "er", "r4"])
+def test_reference_accuracy_inputs(fx, dtype, n, family):
+    if family == "complete":
+        adj = complete_graph(n)
+    elif family == "er":
+        adj = erdos_renyi(n, 0.05, seed=n)
+    else:
+        adj = random_regular(n, 4, seed=n)
+    a = 1j * adj
+    _, stats = _check(fx, a, dtype, f"ref-{family}", ref=taylor_expm(a, m=100))
+    assert stats["structured"]
+
+
+# ---------------------------------------------------------------------------
+# General complex A (not purely imaginary / not symmetric)
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+@pytest.mark.parametrize("n", [1, 5, 64, 129, 250])
+def test_general_complex(fx, dtype, n):
+    rng = np.random.default_rng(n)
+    a = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) * (3.0 / np.sqrt(n))
+    _, stats = _check(fx, a, dtype, "general-complex")
+    assert not stats["structured"]
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("n", [4, 100])
+def test_real_dtypes(fx, dtype, n):
+    rng = np.random.default_rng(11 + n)
+    a = rng.standard_normal((n, n)) * (2.0 / np.sqrt(n))
+    _check(fx, a, dtype, "real")
+
+
+def test_reference_perf_inputs_complex(fx):
+    """Perf-test inputs (tests/test_fastexpm_perf.py:14-18): Adj + 1j*rand."""
+    rng = np.random.default_rng(0)
+    n = 48
+    a = complete_graph(n) + 1j * rng.random((n, n))
+    _check(fx, a, np.complex128, "perf-complete-general")
+
+
+# ---------------------------------------------------------------------------
+# Invariants at a size the oracle does not reach cheaply
+
+def test_group_law_and_unitarity(fx):
+    n = 700
+    L = laplacian(erdos_renyi(n, 0.5, seed=3))
+    _say("group law n=700")
+    with fx.ExpMatComplex128(n) as e:
+        u1 = e.run(-1j * 1.0 * L)
+        u2 = e.run(-1j * 2.0 * L)
+        u3 = e.run(-1j * 3.0 * L)
+    assert rel_fro(u3, u1 @ u2) < 1e-11
+    assert unitarity_defect(u3) < 1e-12
+    ones = np.ones(n)
+    assert np.allclose(u3 @ ones, ones, atol=1e-10)  # Laplacian: U 1 = 1
+
+
+# ---------------------------------------------------------------------------
+# Edge cases and API behaviour
+
+def test_zero_matrix_and_nonfinite(fx):
+    _say("zero / nonfinite")
+    with fx.ExpMatComplex64(10) as e:
+        u = e.run(np.zeros((10, 10), dtype=np.complex64))
+        assert np.array_equal(u, np.eye(10, dtype=np.complex64))
+        bad = np.zeros((10, 10), dtype=np.complex64)
+        bad[3, 4] = np.nan
+        u = e.run(bad)
+        assert np.isnan(u).all()
+        assert e.stats()["nonfinite"]
+
+
+def test_free_idempotent_and_errors(fx):
+    _say("free / errors")
+    e = fx.ExpMatComplex64(8)
+    with pytest.raises(TypeError):
+        e.run([[0.0]])
+    with pytest.raises(ValueError):
+        e.run(np.zeros((7, 7), dtype=np.complex64))
+    e.free()
+    e.free()
+    with pytest.raises(RuntimeError):
+        e.run(np.zeros((8, 8), dtype=np.complex64))
+
+
+def test_input_not_modified_and_reuse(fx):
+    _say("reuse")
+    n = 40
+    a = (-1j * 0.9 * laplacian(erdos_renyi(n, 0.4, seed=9))).astype(np.complex64)
+    keep = a.copy()
+    with fx.ExpMatComplex64(n) as e:
+        u1 = e.run(a)
+        u2 = e.run(a)
+    assert np.array_equal(a, keep)
+    assert np.array_equal(u1, u2)
+
+
+def test_device_resident_and_apply(fx):
+    _say("resident + apply")
+    n = 333
+    L = laplacian(erdos_renyi(n, 0.2, seed=4, weighted=True))
+    t = 3.0
+    a = (-1j * t * L).astype(np.complex64)
+    with fx.ExpMatComplex64(n) as e:
+        e.upload(a)
+        e.run_resident()
+        u = e.download()
+        psi0 = np.zeros((2, n), dtype=np.complex128)
+        psi0[0, 0] = 1.0
+        psi0[1] = 1 / np.sqrt(n)
+        psi = e.apply(psi0)
+    ref = eig_propagator(L, t)
+    assert rel_fro(ref, u) < _tol(np.complex64, 10)
+    assert infidelity(ref[:, 0], psi[0]) < 1e-9
+    assert np.allclose(psi[0], u[:, 0].astype(np.complex128), atol=1e-6)
+
+
+def test_ctqw_drop_in(fx):
+    """README usage (README.md:61-72) with device='cuda'."""
+    _say("Ctqw drop-in")
+    from FastCTQW.Ctqw import Ctqw
+    from FastCTQW.State import State
+
+    n = 120
+    adj = random_regular(n, 4, seed=3).astype(np.float32)
+    psi0 = State.getSingleNodeState(n, 0)
+    w = Ctqw.from_numpy_array(adj, psi0, device="cuda")
+    w.runWalk(time=4.5)
+    psi = w.getFinalState().getState()
+    # reference convention: H = -gamma*A, A_exp = -1j*H*t = +1j*t*A
+    ref = eig_propagator(-adj.astype(np.float64), 4.5)[:, 0]
+    assert infidelity(ref, psi) < 1e-9
+    u = w.getOperator().getOperator()
+    assert u.shape == (n, n)
+    w_cpu = Ctqw.from_numpy_array(adj, psi0, device="cpu")
+    w_cpu.runWalk(time=4.5)
+    assert infidelity(w_cpu.getFinalState().getState(), psi) < 1e-9
