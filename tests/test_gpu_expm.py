@@ -75,33 +75,33 @@ def _check(fx, a, dtype, name, ref=None):
 # ---------------------------------------------------------------------------
 # CTQW form A = -i t H, H real symmetric (structured path)
 
-@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+@pytest.mark.parametrize("dt", [np.complex64, np.complex128])
 @pytest.mark.parametrize("n", [1, 2, 3, 17, 64, 127, 128, 129, 300])
-def test_ctqw_laplacian_er(fx, dtype, n):
+def test_ctqw_laplacian_er(fx, dt, n):
     L = laplacian(erdos_renyi(n, 0.3, seed=100 + n, weighted=True))
     t = 2.5
     ref = eig_propagator(L, t)
-    u, stats = _check(fx, -1j * t * L, dtype, "ER-laplacian", ref=ref)
+    u, stats = _check(fx, -1j * t * L, dt, "ER-laplacian", ref=ref)
     assert stats["structured"]
 
 
-@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+@pytest.mark.parametrize("dt", [np.complex64, np.complex128])
 @pytest.mark.parametrize("t", [0.001, 0.05, 0.3, 1.0, 4.5, 20.0])
-def test_ctqw_degrees(fx, dtype, t):
+def test_ctqw_degrees(fx, dt, t):
     """Small t exercises Pade degrees 3..9 without scaling."""
     a = random_regular(96, 4, seed=7).astype(np.float64)
     ref = eig_propagator(-a, t)
-    _check(fx, 1j * t * a, dtype, f"R4-adjacency t={t}", ref=ref)
+    _check(fx, 1j * t * a, dt, f"R4-adjacency t={t}", ref=ref)
 
 
-@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
-def test_c1_four_regular_walk(fx, dtype):
+@pytest.mark.parametrize("dt", [np.complex64, np.complex128])
+def test_c1_four_regular_walk(fx, dt):
     """4-regular graph, N=1000, t=4.5: U and psi against the eigen oracle."""
     n, t = 1000, 4.5
     a = random_regular(n, 4, seed=1)
     H = a  # configuration "H = adjacency"; A_exp = -i t H
     ref = eig_propagator(H, t)
-    u, stats = _check(fx, -1j * t * H, dtype, "C1 4-regular N=1000", ref=ref)
+    u, stats = _check(fx, -1j * t * H, dt, "C1 4-regular N=1000", ref=ref)
     assert stats["structured"]
     # uniform superposition is an eigenvector: psi(t) = e^{-i 4 t} psi0
     psi0 = np.full(n, 1 / np.sqrt(n))
@@ -127,10 +127,10 @@ def test_closed_forms(fx):
 # A = 1j * adjacency for complete, ER(0.05) and 4-regular graphs, checked
 # against the reference's own Taylor oracle restated.
 
-@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+@pytest.mark.parametrize("dt", [np.complex64, np.complex128])
 @pytest.mark.parametrize("n", [16, 64, 256])
 @pytest.mark.parametrize("family", ["complete", "er", "r4"])
-def test_reference_accuracy_inputs(fx, dtype, n, family):
+def test_reference_accuracy_inputs(fx, dt, n, family):
     if family == "complete":
         adj = complete_graph(n)
     elif family == "er":
@@ -138,28 +138,28 @@ def test_reference_accuracy_inputs(fx, dtype, n, family):
     else:
         adj = random_regular(n, 4, seed=n)
     a = 1j * adj
-    _, stats = _check(fx, a, dtype, f"ref-{family}", ref=taylor_expm(a, m=100))
+    _, stats = _check(fx, a, dt, f"ref-{family}", ref=taylor_expm(a, m=100))
     assert stats["structured"]
 
 
 # ---------------------------------------------------------------------------
 # General complex A (not purely imaginary / not symmetric)
 
-@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+@pytest.mark.parametrize("dt", [np.complex64, np.complex128])
 @pytest.mark.parametrize("n", [1, 5, 64, 129, 250])
-def test_general_complex(fx, dtype, n):
+def test_general_complex(fx, dt, n):
     rng = np.random.default_rng(n)
     a = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) * (3.0 / np.sqrt(n))
-    _, stats = _check(fx, a, dtype, "general-complex")
+    _, stats = _check(fx, a, dt, "general-complex")
     assert not stats["structured"]
 
 
-@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
 @pytest.mark.parametrize("n", [4, 100])
-def test_real_dtypes(fx, dtype, n):
+def test_real_dts(fx, dt, n):
     rng = np.random.default_rng(11 + n)
     a = rng.standard_normal((n, n)) * (2.0 / np.sqrt(n))
-    _check(fx, a, dtype, "real")
+    _check(fx, a, dt, "real")
 
 
 def test_reference_perf_inputs_complex(fx):

@@ -1,0 +1,207 @@
+"""Host-side tests (no GPU): generators, the C ABI library's exports, the
+Python wrappers' validation and the CPU walk path of Ctqw/Operator/State."""
+import ctypes
+import os
+import re
+import warnings
+
+import numpy as np
+import pytest
+
+from fastctqw_amd.graphs import (erdos_renyi, laplacian, random_regular, random_state, splitmix64,
+                                 uniform01)
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "fastctqw_amd", "fastexpm", "libfastexpm_hip.so")
+HEADER = os.path.join(ROOT, "include", "fastexpm.h")
+
+
+# --------------------------------------------------------------------------- generators
+
+def test_splitmix64_known_values():
+    # splitmix64 finaliser of 0 and 1 (first outputs of the reference
+    # generator seeded with 0 are the finaliser of 0x9E3779B97F4A7C15*k)
+    assert int(splitmix64(np.uint64(0))) == 0xE220A8397B1DCDAF
+    assert int(splitmix64(np.uint64(0x9E3779B97F4A7C15))) == 0x6E789E6AA1B965F4
+
+
+def test_uniform_range_and_determinism():
+    u = uniform01(3, 1, np.arange(100000, dtype=np.uint64))
+    assert u.min() >= 0 and u.max() < 1
+    assert abs(u.mean() - 0.5) < 0.01
+    assert np.array_equal(u, uniform01(3, 1, np.arange(100000, dtype=np.uint64)))
+    assert not np.array_equal(u, uniform01(4, 1, np.arange(100000, dtype=np.uint64)))
+
+
+def test_erdos_renyi_properties():
+    a = erdos_renyi(400, 0.3, seed=2)
+    assert np.array_equal(a, a.T) and np.all(np.diag(a) == 0)
+    assert set(np.unique(a)) <= {0.0, 1.0}
+    dens = a.sum() / (400 * 399)
+    assert abs(dens - 0.3) < 0.01
+    assert np.array_equal(a, erdos_renyi(400, 0.3, seed=2))
+    w = erdos_renyi(200, 0.5, seed=3, weighted=True)
+    assert np.array_equal(w, w.T) and w.max() <= 1.0 and w[w > 0].min() > 0
+    assert np.array_equal(w > 0, erdos_renyi(200, 0.5, seed=3) > 0)
+
+
+@pytest.mark.parametrize("n,d", [(10, 3), (64, 4), (500, 8), (1000, 4)])
+def test_random_regular(n, d):
+    a = random_regular(n, d, seed=n + d)
+    assert np.array_equal(a, a.T) and np.all(np.diag(a) == 0)
+    assert np.all(a.sum(axis=1) == d)
+    assert np.array_equal(a, random_regular(n, d, seed=n + d))
+
+
+def test_random_regular_rejects_impossible():
+    with pytest.raises(ValueError):
+        random_regular(5, 3, seed=0)
+
+
+def test_laplacian_and_state():
+    a = erdos_renyi(50, 0.4, seed=1, weighted=True)
+    L = laplacian(a)
+    assert np.allclose(L @ np.ones(50), 0)
+    psi = random_state(50, seed=55)
+    assert abs(np.linalg.norm(psi) - 1) < 1e-14
+    assert not np.allclose(psi, random_state(50, seed=55, index=1))
+
+
+# --------------------------------------------------------------------------- C ABI
+
+def _declared_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|double|const char\*)\s+(fexpm_\w+)\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(LIB):
+        pytest.skip("HIP library not built")
+    syms = _declared_symbols()
+    assert len(syms) >= 14
+    lib = ctypes.CDLL(LIB)
+    for s in syms:
+        assert hasattr(lib, s), s
+
+
+def test_library_no_device_and_argument_errors():
+    if not os.path.exists(LIB):
+        pytest.skip("HIP library not built")
+    lib = ctypes.CDLL(LIB)
+    lib.fexpm_status_string.restype = ctypes.c_char_p
+    assert lib.fexpm_status_string(0) == b"ok"
+    assert lib.fexpm_status_string(-4) == b"singular Pade denominator"
+    h = ctypes.c_void_p()
+    # invalid dtype / size are rejected before any device call
+    assert lib.fexpm_create(ctypes.c_int64(4), 9, ctypes.byref(h)) == -1
+    assert lib.fexpm_create(ctypes.c_int64(-1), 2, ctypes.byref(h)) == -1
+    assert lib.fexpm_create(ctypes.c_int64(4), 2, None) == -1
+    assert lib.fexpm_free(None) == -1
+    lib.fexpm_get_n.restype = ctypes.c_int64
+    assert lib.fexpm_get_n(None) == -1
+
+
+# --------------------------------------------------------------------------- wrappers
+
+class _FakeBackend:
+    def __init__(self, n):
+        self.n = n
+        self.freed = 0
+
+    def run(self, a):
+        return np.eye(self.n, dtype=a.dtype).ravel()
+
+    def free(self):
+        self.freed += 1
+        raise RuntimeError("boom")
+
+
+def _fake_wrapper(monkeypatch):
+    pytest.importorskip("fastctqw_amd.fastexpm._fastexpm_core")
+    from fastctqw_amd.fastexpm import _wrapper
+
+    class W(_wrapper._ExpMatBase):
+        _core_cls = _FakeBackend
+
+    return W
+
+
+def test_wrapper_validation_and_exit(monkeypatch):
+    W = _fake_wrapper(monkeypatch)
+    w = W(3)
+    with pytest.raises(TypeError):
+        w.run([[1.0]])
+    with pytest.raises(ValueError):
+        w.run(np.zeros((2, 2)))
+    assert w.run(np.zeros((3, 3))).shape == (3, 3)  # reshape of a flat return (_wrapper.py:96)
+    with warnings.catch_warnings(record=True) as rec:
+        warnings.simplefilter("always")
+        with pytest.raises(KeyError):
+            with W(2):
+                raise KeyError("original")
+    assert any("Failed to free" in str(r.message) for r in rec)
+
+
+# --------------------------------------------------------------------------- CPU walk path
+
+def test_ctqw_cpu_path_matches_oracle():
+    from fastctqw_amd.Ctqw import Ctqw
+    from fastctqw_amd.State import State
+    from oracle.eig import eig_propagator
+    from oracle.metrics import infidelity
+
+    n = 30
+    adj = random_regular(n, 4, seed=9).astype(np.float64)
+    w = Ctqw.from_numpy_array(adj, State.getUniformSuperposition(n), gamma=0.5, laplacian=True)
+    w.runWalk(time=2.0)
+    psi = w.getFinalState().getState()
+    Lg = laplacian(adj)
+    ref = eig_propagator(-0.5 * Lg, 2.0) @ np.full(n, 1 / np.sqrt(n))
+    assert infidelity(ref, psi) < 1e-12
+    assert w.getOperator().getOperator().dtype == np.complex128
+
+
+def test_operator_errors_and_dtype_map():
+    from fastctqw_amd.Operator import Operator
+
+    with pytest.raises(ValueError):
+        Operator(np.array([[0, 1], [0, 0]], dtype=np.float32))
+    with pytest.raises(TypeError):
+        Operator(np.zeros((3, 3), dtype=np.int32))
+    op = Operator(np.zeros((3, 3), dtype=np.float32))
+    assert op._dtype == np.complex64
+    op = Operator(np.zeros((3, 3), dtype=np.float64))
+    assert op._dtype == np.complex128
+
+
+def test_state_semantics():
+    from fastctqw_amd.State import State
+
+    assert State.getUniformSuperposition(4).getState().dtype == np.complex64
+    s = State.getLocalSuperposition(5, [1, 3]).getState()
+    assert np.allclose(np.abs(s) ** 2, [0, 0.5, 0, 0.5, 0])
+    with pytest.raises(ValueError):
+        State(np.zeros((2, 2)))
+    with warnings.catch_warnings(record=True) as rec:
+        warnings.simplefilter("always")
+        State(np.array([1.0, 1.0]))
+    assert rec
+
+
+def test_directed_graph_rejected():
+    nx = pytest.importorskip("networkx")
+    from fastctqw_amd.Ctqw import Ctqw
+    from fastctqw_amd.State import State
+
+    with pytest.raises(TypeError):
+        Ctqw.from_networkx_graph(nx.DiGraph([(0, 1)]), State.getSingleNodeState(2, 0))
+
+
+def test_product_does_not_import_oracle():
+    import subprocess
+    import sys
+
+    code = ("import sys; import fastctqw_amd, fastctqw_amd.Ctqw, fastctqw_amd.graphs;"
+            "import FastCTQW; print(any(m == 'oracle' or m.startswith('oracle.') for m in sys.modules))")
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, cwd=ROOT)
+    assert out.stdout.strip() == "False", out.stderr

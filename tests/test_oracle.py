@@ -1,0 +1,197 @@
+"""Pinning of the CPU oracle (runs without a GPU).
+
+Each oracle function is checked against something that does not share its
+code: a library routine (scipy.linalg.expm, LAPACK eigh), arbitrary
+precision (mpmath), closed forms, invariants, and committed golden vectors.
+Planted mistakes (a perturbed Pade coefficient, a wrong sign of mu, s off by
+one) are shown to fail these checks.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg as sl
+
+from fastctqw_amd.graphs import erdos_renyi, laplacian, random_regular
+from oracle import padeexpm as P
+from oracle.closed import (complete_graph, cycle_graph, expm_complete, expm_cycle, expm_hypercube,
+                           hypercube_graph)
+from oracle.eig import eig_propagator, eig_state, taylor_expm
+from oracle.metrics import infidelity, raw_infidelity, rel_fro, unitarity_defect
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "expm_golden.npz")
+
+
+def _rational(m, x, coeffs=None):
+    b = coeffs if coeffs is not None else P.PADE_B[m]
+    p = sum(b[j] * x ** j for j in range(m + 1))
+    q = sum(b[j] * (-x) ** j for j in range(m + 1))
+    return p / q
+
+
+@pytest.mark.parametrize("m", [3, 5, 7, 9, 13])
+def test_pade_backward_error_at_theta(m):
+    """Higham 2005: theta_m is where the relative backward error
+    |h_m(x)|/|x| of r_m, h_m(x) = log(e^-x r_m(x)), reaches u = 2^-53.
+    Evaluated in mpmath so rounding does not interfere."""
+    mpmath = pytest.importorskip("mpmath")
+    mpmath.mp.dps = 60
+    th = mpmath.mpf(P.THETA[m])
+    b = [mpmath.mpf(int(c)) for c in P.PADE_B[m]]
+    r = sum(b[j] * th ** j for j in range(m + 1)) / sum(b[j] * (-th) ** j for j in range(m + 1))
+    err = abs(mpmath.log(r * mpmath.e ** (-th))) / th
+    assert err < 1.05 * 2.0 ** -53, float(err)
+    assert err > 2.0 ** -57, float(err)  # theta is the largest x with error ~u, not far below
+
+
+def test_pade_coefficients_are_the_diagonal_pade_approximant():
+    """r_m(x) - e^x = O(x^(2m+1)): the first 2m Taylor terms of r_m match."""
+    mpmath = pytest.importorskip("mpmath")
+    mpmath.mp.dps = 80
+    for m in (3, 5, 7, 9, 13):
+        x = mpmath.mpf("1e-3")
+        r = sum(mpmath.mpf(int(c)) * x ** j for j, c in enumerate(P.PADE_B[m])) / \
+            sum(mpmath.mpf(int(c)) * (-x) ** j for j, c in enumerate(P.PADE_B[m]))
+        # error ~ c x^(2m+1) with c tiny; must be far below x^(2m)
+        assert abs(r - mpmath.e ** x) < x ** (2 * m)
+
+
+def test_planted_coefficient_error_is_caught():
+    mpmath = pytest.importorskip("mpmath")
+    mpmath.mp.dps = 60
+    b = list(P.PADE_B[13])
+    b[7] *= 1.0 + 1e-9
+    th = mpmath.mpf(P.THETA[13])
+    r = sum(mpmath.mpf(b[j]) * th ** j for j in range(14)) / sum(mpmath.mpf(b[j]) * (-th) ** j for j in range(14))
+    assert abs(r * mpmath.e ** (-th) - 1) > 2.0 ** -40
+
+
+def test_choose_degree_boundaries():
+    for q, m in enumerate((3, 5, 7, 9)):
+        assert P.choose_degree(P.THETA[m]) == (m, 0)
+        assert P.choose_degree(np.nextafter(P.THETA[m], 0)) == (m, 0)
+    assert P.choose_degree(np.nextafter(P.THETA[9], 10)) == (13, 0)
+    assert P.choose_degree(P.THETA[13]) == (13, 0)
+    assert P.choose_degree(np.nextafter(P.THETA[13], 100)) == (13, 1)
+    assert P.choose_degree(2 * P.THETA[13]) == (13, 1)
+    assert P.choose_degree(np.nextafter(2 * P.THETA[13], 100)) == (13, 2)
+    assert P.choose_degree(0.0) == (3, 0)
+    # 2^-s * norm <= theta_13 < 2^-(s-1) * norm
+    for norm in np.geomspace(6.0, 1e6, 50):
+        m, s = P.choose_degree(norm)
+        assert norm / 2 ** s <= P.THETA[13] < norm / 2 ** (s - 1)
+
+
+def test_trace_shift_rule():
+    L = laplacian(erdos_renyi(50, 0.5, seed=1))
+    a = -1j * 2.0 * L
+    mu, sh = P.trace_shift(a)
+    assert mu == pytest.approx(-1j * 2.0 * np.trace(L) / 50)
+    assert P.onenorm(sh) < P.onenorm(a)
+    # a matrix whose shift would increase the norm keeps mu = 0
+    b = np.diag([10.0, -10.0]).astype(complex)
+    b[0, 1] = 0.5
+    mu2, sh2 = P.trace_shift(b + 1.0 * np.eye(2))
+    assert P.onenorm(sh2) <= P.onenorm(b + np.eye(2))
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 40, 150])
+def test_oracle_vs_scipy_general(n):
+    rng = np.random.default_rng(n)
+    a = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) * (4.0 / np.sqrt(n))
+    assert rel_fro(sl.expm(a), P.pade_expm(a)) < 1e-12
+
+
+@pytest.mark.parametrize("t", [1e-4, 0.03, 0.5, 2.0, 4.5, 20.0])
+def test_oracle_vs_eig_ctqw(t):
+    L = laplacian(erdos_renyi(120, 0.3, seed=3, weighted=True))
+    ref = eig_propagator(L, t)
+    assert rel_fro(ref, P.pade_expm(-1j * t * L)) < 1e-12
+    assert rel_fro(ref, sl.expm(-1j * t * L)) < 1e-12
+
+
+def test_planted_mu_sign_and_s_errors_are_caught():
+    L = laplacian(erdos_renyi(60, 0.5, seed=8))
+    a = -1j * 4.5 * L
+    ref = eig_propagator(L, 4.5)
+    st = P.pade_expm(a, stages=True)
+    wrong_mu = np.exp(-st["mu"]) * np.linalg.matrix_power(st["X"], 2 ** st["s"])
+    assert rel_fro(ref, wrong_mu) > 1e-3
+    x_short = st["X"]
+    for _ in range(st["s"] - 1):
+        x_short = x_short @ x_short
+    assert rel_fro(ref, np.exp(st["mu"]) * x_short) > 1e-3
+
+
+def test_closed_forms_against_eig_and_scipy():
+    for n in (4, 9, 33):
+        assert rel_fro(expm_complete(n, 0.8), sl.expm(1j * 0.8 * complete_graph(n))) < 1e-13
+        assert rel_fro(expm_cycle(n, 1.1), eig_propagator(-cycle_graph(n), 1.1)) < 1e-12
+    for d in (2, 5):
+        assert rel_fro(expm_hypercube(d, 0.6), eig_propagator(-hypercube_graph(d), 0.6)) < 1e-12
+
+
+def test_oracle_closed_forms():
+    for n in (5, 64):
+        assert rel_fro(expm_complete(n, 0.7), P.pade_expm(1j * 0.7 * complete_graph(n))) < 1e-13
+        assert rel_fro(expm_cycle(n, 1.3), P.pade_expm(1j * 1.3 * cycle_graph(n))) < 1e-12
+    assert rel_fro(expm_hypercube(6, 1.3), P.pade_expm(1j * 1.3 * hypercube_graph(6))) < 1e-12
+    d = np.array([0.3 + 1j, -2.0, 5.0j, 7.0])
+    assert rel_fro(np.diag(np.exp(d)), P.pade_expm(np.diag(d))) < 1e-13
+    assert np.array_equal(P.pade_expm(np.zeros((3, 3))), np.eye(3))
+    assert P.pade_expm(np.array([[2.0 + 1j]]))[0, 0] == pytest.approx(np.exp(2.0 + 1j), rel=1e-15)
+
+
+def test_oracle_mpmath_tiny():
+    mpmath = pytest.importorskip("mpmath")
+    mpmath.mp.dps = 40
+    rng = np.random.default_rng(1)
+    for n in (3, 6, 12):
+        a = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) * 1.5
+        ref = np.array(mpmath.expm(mpmath.matrix(a.tolist())).tolist(), dtype=np.complex128)
+        assert rel_fro(ref, P.pade_expm(a)) < 1e-13
+
+
+def test_reference_taylor_oracle_restatement():
+    """The reference test's Taylor oracle agrees with the eigen oracle on its
+    own input families (tests/test_fastexpm_accuracy.py:83-159)."""
+    for adj in (complete_graph(32), erdos_renyi(64, 0.05, seed=2), random_regular(64, 4, seed=2)):
+        a = 1j * adj
+        assert rel_fro(eig_propagator(-adj, 1.0), taylor_expm(a, m=100)) < 1e-9
+
+
+def test_invariants():
+    L = laplacian(erdos_renyi(80, 0.4, seed=5))
+    u1, u2, u3 = (P.pade_expm(-1j * t * L) for t in (0.7, 1.9, 2.6))
+    assert rel_fro(u3, u1 @ u2) < 1e-12
+    assert unitarity_defect(u3) < 1e-13
+    assert np.allclose(u3 @ np.ones(80), np.ones(80))
+    assert rel_fro(u1, u1.T) < 1e-13  # complex symmetric
+
+
+def test_eig_state_and_metrics():
+    L = laplacian(erdos_renyi(70, 0.3, seed=6))
+    psi0 = np.zeros(70, complex)
+    psi0[3] = 1
+    assert np.allclose(eig_state(L, 1.7, psi0), eig_propagator(L, 1.7) @ psi0)
+    v = np.array([1, 1j, 0]) / np.sqrt(2)
+    assert infidelity(v, 1j * v) == pytest.approx(0.0, abs=1e-15)
+    assert infidelity(v, np.array([1, 0, 0])) == pytest.approx(0.5)
+    assert raw_infidelity(v, 1.1 * v) < 0  # norm errors show up unnormalised
+
+
+def test_golden_vectors():
+    """Committed fixtures (tests/golden/make_golden.py, mpmath 40 digits)."""
+    g = np.load(GOLDEN, allow_pickle=False)
+    names = sorted({k[:-2] for k in g.files if k.endswith("_A")})
+    assert names
+    for name in names:
+        a, u = g[name + "_A"], g[name + "_U"]
+        assert rel_fro(u, P.pade_expm(a)) < 1e-13, name
+        assert rel_fro(u, sl.expm(a)) < 1e-12, name
+
+
+def test_work_model():
+    assert P.work_flops(10000, 13) == pytest.approx((8 * 19 + 32 / 3) * 1e12)
+    assert math.isclose(P.work_flops(1000, 2, 13), (8 * 8 + 32 / 3) * 1e9)
