@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "../../include/fastexpm.h"
+#include "../../include/fastexpm_testing.h"
 #include "fexpm_gemm.hpp"
 #include "fexpm_kernels.hpp"
 
@@ -101,7 +102,91 @@ struct PipelineBase {
   virtual void run(const void* d_a, void* d_u, hipStream_t st, fexpm_stats& stats, int opt_structure) = 0;
   virtual void apply(const double2* d_psi0, int nstates, double2* d_out, hipStream_t st) = 0;
   virtual bool has_result() const = 0;
+  virtual void debug_plane(int which, double* re, double* im) = 0;
+  int max_squarings = -1;
 };
+
+// ---------------------------------------------------------------------------
+// Blocked right-looking LU with partial pivoting of the augmented FP64
+// system S = [M | B] (np x 2np, leading dimension ldl; np a multiple of
+// LU_NB), followed by block back substitution.  On return the right block
+// holds X = M^{-1} B.  Returns the real flops issued by its MFMA GEMMs.
+static double lu_gemm(const GemmArgs<double>& g, hipStream_t st) {
+  using C = GemmCfg<double>;
+  constexpr int NT = (C::BM / C::WM) * (C::BN / C::WN) * 64;
+  dim3 grid((g.n + C::BN - 1) / C::BN, (g.m + C::BM - 1) / C::BM);
+  gemm_nt_kernel<double, MODE_C4M, false><<<grid, dim3(NT), 0, st>>>(g);
+  FX_LAUNCH();
+  return 8.0 * (double)grid.x * grid.y * C::BM * C::BN * (double)g.k;
+}
+
+static double lu_factor_solve(double* lr, double* li, int64_t ldl, int inp, double* tr, double* ti, int* ipiv,
+                              PanelCand* cand, int gmax, double* top, Scalars* scal, hipStream_t st) {
+  double flops = 0.0;
+  const int nb = LU_NB;
+  const int nblk = inp / nb;
+  for (int kb = 0; kb < nblk; ++kb) {
+    const int k0 = kb * nb;
+    const int G = (inp - k0 + PANEL_R - 1) / PANEL_R;
+    lu_panel_init<<<G, 256, 0, st>>>(lr, li, ldl, inp, k0, cand, top);
+    FX_LAUNCH();
+    for (int j = 0; j < nb; ++j) {
+      const int par = j & 1;
+      lu_panel_step<<<G, 256, 0, st>>>(lr, li, ldl, inp, k0, j, G, cand + par * gmax, cand + (par ^ 1) * gmax,
+                                       top + par * 2 * LU_NB, top + (par ^ 1) * 2 * LU_NB, ipiv, scal);
+      FX_LAUNCH();
+    }
+    const int c0 = k0 + nb, c1 = 2 * inp;
+    lu_laswp<<<(c1 - c0 + 255) / 256, 256, 0, st>>>(lr, li, ldl, k0, nb, ipiv, c0, c1);
+    FX_LAUNCH();
+    lu_trsm_block_row<false><<<(c1 - c0 + 63) / 64, 64, 0, st>>>(lr, li, ldl, k0, c0, c1, tr, ti);
+    FX_LAUNCH();
+    if (k0 + nb < inp) {
+      GemmArgs<double> g;
+      std::memset(&g, 0, sizeof(g));
+      g.ar = lr + (int64_t)(k0 + nb) * ldl + k0;
+      g.ai = li + (int64_t)(k0 + nb) * ldl + k0;
+      g.lda = ldl;
+      g.br = tr;
+      g.bi = ti;
+      g.ldb = nb;
+      g.m = inp - k0 - nb;
+      g.n = c1 - c0;
+      g.k = nb;
+      g.epi.cr = lr + (int64_t)(k0 + nb) * ldl + (k0 + nb);
+      g.epi.ci = li + (int64_t)(k0 + nb) * ldl + (k0 + nb);
+      g.epi.ldc = ldl;
+      g.epi.alpha = -1.0;
+      g.epi.beta = 1.0;
+      flops += lu_gemm(g, st);
+    }
+  }
+  for (int ib = nblk - 1; ib >= 0; --ib) {
+    const int r0 = ib * nb;
+    lu_trsm_block_row<true><<<(inp + 63) / 64, 64, 0, st>>>(lr, li, ldl, r0, inp, 2 * inp, tr, ti);
+    FX_LAUNCH();
+    if (r0 > 0) {
+      GemmArgs<double> g;
+      std::memset(&g, 0, sizeof(g));
+      g.ar = lr + r0;
+      g.ai = li + r0;
+      g.lda = ldl;
+      g.br = tr;
+      g.bi = ti;
+      g.ldb = nb;
+      g.m = r0;
+      g.n = inp;
+      g.k = nb;
+      g.epi.cr = lr + inp;
+      g.epi.ci = li + inp;
+      g.epi.ldc = ldl;
+      g.epi.alpha = -1.0;
+      g.epi.beta = 1.0;
+      flops += lu_gemm(g, st);
+    }
+  }
+  return flops;
+}
 
 template <typename W> struct Pipeline : PipelineBase {
   int n, dtype;
@@ -370,82 +455,12 @@ template <typename W> struct Pipeline : PipelineBase {
       form_system<W><<<grid, 256, 0, st>>>(u.r, u.i, v.r, v.i, ld, n, inp, lr, li, ldl, structured ? 1 : 0);
       FX_LAUNCH();
     }
-    const int nb = LU_NB;
-    const int nblk = inp / nb;
-    for (int kb = 0; kb < nblk; ++kb) {
-      const int k0 = kb * nb;
-      const int G = (inp - k0 + PANEL_R - 1) / PANEL_R;
-      lu_panel_init<<<G, 256, 0, st>>>(lr, li, ldl, inp, k0, cand, top);
-      FX_LAUNCH();
-      for (int j = 0; j < nb; ++j) {
-        const int par = j & 1;
-        lu_panel_step<<<G, 256, 0, st>>>(lr, li, ldl, inp, k0, j, G, cand + par * gmax, cand + (par ^ 1) * gmax,
-                                         top + par * 2 * LU_NB, top + (par ^ 1) * 2 * LU_NB, ipiv, scal);
-        FX_LAUNCH();
-      }
-      const int c0 = k0 + nb, c1 = 2 * inp;
-      lu_laswp<<<(c1 - c0 + 255) / 256, 256, 0, st>>>(lr, li, ldl, k0, nb, ipiv, c0, c1);
-      FX_LAUNCH();
-      lu_trsm_block_row<false><<<(c1 - c0 + 63) / 64, 64, 0, st>>>(lr, li, ldl, k0, c0, c1, tr, ti);
-      FX_LAUNCH();
-      if (k0 + nb < inp) {
-        GemmArgs<double> g;
-        std::memset(&g, 0, sizeof(g));
-        g.ar = lr + (int64_t)(k0 + nb) * ldl + k0;
-        g.ai = li + (int64_t)(k0 + nb) * ldl + k0;
-        g.lda = ldl;
-        g.br = tr;
-        g.bi = ti;
-        g.ldb = nb;
-        g.m = inp - k0 - nb;
-        g.n = c1 - c0;
-        g.k = nb;
-        g.epi.cr = lr + (int64_t)(k0 + nb) * ldl + (k0 + nb);
-        g.epi.ci = li + (int64_t)(k0 + nb) * ldl + (k0 + nb);
-        g.epi.ldc = ldl;
-        g.epi.alpha = -1.0;
-        g.epi.beta = 1.0;
-        lu_gemm(g, st);
-      }
-    }
-    for (int ib = nblk - 1; ib >= 0; --ib) {
-      const int r0 = ib * nb;
-      lu_trsm_block_row<true><<<(inp + 63) / 64, 64, 0, st>>>(lr, li, ldl, r0, inp, 2 * inp, tr, ti);
-      FX_LAUNCH();
-      if (r0 > 0) {
-        GemmArgs<double> g;
-        std::memset(&g, 0, sizeof(g));
-        g.ar = lr + r0;
-        g.ai = li + r0;
-        g.lda = ldl;
-        g.br = tr;
-        g.bi = ti;
-        g.ldb = nb;
-        g.m = r0;
-        g.n = inp;
-        g.k = nb;
-        g.epi.cr = lr + inp;
-        g.epi.ci = li + inp;
-        g.epi.ldc = ldl;
-        g.epi.alpha = -1.0;
-        g.epi.beta = 1.0;
-        lu_gemm(g, st);
-      }
-    }
+    flops += lu_factor_solve(lr, li, ldl, inp, tr, ti, ipiv, cand, gmax, top, scal, st);
     {
       dim3 grid((n + 31) / 32, (n + 31) / 32);
       solution_to_work<W><<<grid, 256, 0, st>>>(lr + inp, li + inp, ldl, x0.r, x0.i, ld, n, structured ? 1 : 0);
       FX_LAUNCH();
     }
-  }
-
-  void lu_gemm(const GemmArgs<double>& g, hipStream_t st) {
-    using C = GemmCfg<double>;
-    constexpr int NT = (C::BM / C::WM) * (C::BN / C::WN) * 64;
-    dim3 grid((g.n + C::BN - 1) / C::BN, (g.m + C::BM - 1) / C::BM);
-    gemm_nt_kernel<double, MODE_C4M, false><<<grid, dim3(NT), 0, st>>>(g);
-    FX_LAUNCH();
-    flops += 8.0 * (double)grid.x * grid.y * C::BM * C::BN * (double)g.k;
   }
 
   // ---- squarings ----
@@ -525,7 +540,8 @@ template <typename W> struct Pipeline : PipelineBase {
     FX_HIP(hipEventRecord(ev[2], st));
     solve(structured, st);
     FX_HIP(hipEventRecord(ev[3], st));
-    Plane2<W>* X = square(s, structured, st);
+    const int s_do = max_squarings >= 0 ? std::min(s, max_squarings) : s;
+    Plane2<W>* X = square(s_do, structured, st);
     FX_HIP(hipEventRecord(ev[4], st));
     // phase e^mu
     const double er = std::exp(mu_re);
@@ -582,6 +598,19 @@ template <typename W> struct Pipeline : PipelineBase {
     }
   }
 
+  void debug_plane(int which, double* re, double* im) override {
+    const Plane2<W>* planes[] = {&a, &p2, &p4, &p6, &t1, &t2, &u, &v, &x0, &x1, &res};
+    if (which < 0 || which > 10) throw Error(FEXPM_EINVAL, "unknown plane");
+    const Plane2<W>& P = *planes[which];
+    if (!P.r) throw Error(FEXPM_EINVAL, "plane not available");
+    std::vector<W> buf((size_t)n * n);
+    FX_HIP(hipDeviceSynchronize());
+    FX_HIP(hipMemcpy2D(buf.data(), n * sizeof(W), P.r, ld * sizeof(W), n * sizeof(W), n, hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < buf.size(); ++k) re[k] = (double)buf[k];
+    FX_HIP(hipMemcpy2D(buf.data(), n * sizeof(W), P.i, ld * sizeof(W), n * sizeof(W), n, hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < buf.size(); ++k) im[k] = (double)buf[k];
+  }
+
   void apply(const double2* d_psi0, int nstates, double2* d_out, hipStream_t st) override {
     apply_states<W><<<(n + 3) / 4, 256, 0, st>>>(res.r, res.i, ld, n, d_psi0, nstates, ph_re, ph_im, d_out);
     FX_LAUNCH();
@@ -620,18 +649,23 @@ static size_t elem_size(int dtype) {
   return 0;
 }
 
+// Message of the last failure of a call that has no engine (creation,
+// testing entry points), per calling thread.
+static thread_local std::string g_last_error;
+
 template <typename F> static int guarded(fexpm_engine* e, F&& f) {
+  std::string& err = e ? e->err : g_last_error;
   try {
     f();
     return FEXPM_OK;
   } catch (const Error& x) {
-    if (e) e->err = x.what();
+    err = x.what();
     return x.code;
   } catch (const std::bad_alloc&) {
-    if (e) e->err = "host allocation failed";
+    err = "host allocation failed";
     return FEXPM_ENOMEM;
   } catch (const std::exception& x) {
-    if (e) e->err = x.what();
+    err = x.what();
     return FEXPM_EHIP;
   }
 }
@@ -685,11 +719,9 @@ int fexpm_create(int64_t n, int dtype, fexpm_engine** out) {
     }
   });
   if (rc != FEXPM_OK) {
-    // keep the handle so that the caller can read the message, then destroy
-    std::string msg = e->err;
+    g_last_error = e->err;  // readable through fexpm_last_error(NULL)
     release(e);
     delete e;
-    (void)msg;
     return rc;
   }
   *out = e;
@@ -837,6 +869,9 @@ int fexpm_set_option(fexpm_engine* e, int key, int64_t value) {
     case FEXPM_OPT_STRUCTURE:
       e->opt_structure = value ? 1 : 0;
       return FEXPM_OK;
+    case FEXPM_OPT_MAX_SQUARINGS:
+      if (e->pipe) e->pipe->max_squarings = (int)value;
+      return FEXPM_OK;
   }
   e->err = "unknown option";
   return FEXPM_EINVAL;
@@ -848,7 +883,7 @@ int fexpm_get_stats(const fexpm_engine* e, fexpm_stats* out) {
   return FEXPM_OK;
 }
 
-const char* fexpm_last_error(const fexpm_engine* e) { return e ? e->err.c_str() : "null engine"; }
+const char* fexpm_last_error(const fexpm_engine* e) { return e ? e->err.c_str() : g_last_error.c_str(); }
 
 const char* fexpm_status_string(int status) {
   switch (status) {
@@ -867,6 +902,149 @@ double fexpm_gpu_memory_gb(void) {
   size_t fr = 0, tot = 0;
   if (hipMemGetInfo(&fr, &tot) != hipSuccess) return -1.0;
   return (double)fr / (1024.0 * 1024.0 * 1024.0);
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Testing ABI (include/fastexpm_testing.h)
+namespace {
+
+template <typename W> struct DevPlane {
+  W* p = nullptr;
+  explicit DevPlane(size_t cnt) { FX_HIP(hipMalloc(&p, std::max<size_t>(cnt, 1) * sizeof(W))); FX_HIP(hipMemset(p, 0, std::max<size_t>(cnt, 1) * sizeof(W))); }
+  ~DevPlane() { if (p) (void)hipFree(p); }
+  void put(const double* h, int rows, int cols, int64_t ld) {
+    if (!h) return;
+    std::vector<W> b((size_t)rows * ld, (W)0);
+    for (int i = 0; i < rows; ++i)
+      for (int j = 0; j < cols; ++j) b[(size_t)i * ld + j] = (W)h[(size_t)i * cols + j];
+    FX_HIP(hipMemcpy(p, b.data(), b.size() * sizeof(W), hipMemcpyHostToDevice));
+  }
+  void get(double* h, int rows, int cols, int64_t ld) {
+    if (!h) return;
+    std::vector<W> b((size_t)rows * ld);
+    FX_HIP(hipMemcpy(b.data(), p, b.size() * sizeof(W), hipMemcpyDeviceToHost));
+    for (int i = 0; i < rows; ++i)
+      for (int j = 0; j < cols; ++j) h[(size_t)i * cols + j] = (double)b[(size_t)i * ld + j];
+  }
+};
+
+template <typename W, int MODE, bool SYM>
+void test_gemm_t(int m, int n, int k, const double* ar, const double* ai, const double* br, const double* bi,
+                 double* cr, double* ci) {
+  const int64_t kp = round_up(std::max(k, 1), 16);
+  const int64_t ldc = std::max(n, 1);
+  DevPlane<W> dar((size_t)m * kp), dai((size_t)m * kp), dbr((size_t)n * kp), dbi((size_t)n * kp);
+  DevPlane<W> dcr((size_t)m * ldc), dci((size_t)m * ldc);
+  dar.put(ar, m, k, kp);
+  dai.put(ai, m, k, kp);
+  dbr.put(br, n, k, kp);
+  dbi.put(bi, n, k, kp);
+  GemmArgs<W> g;
+  std::memset(&g, 0, sizeof(g));
+  g.ar = dar.p; g.ai = dai.p; g.lda = kp;
+  g.br = dbr.p; g.bi = dbi.p; g.ldb = kp;
+  g.m = m; g.n = n; g.k = (int)kp;
+  g.epi.cr = dcr.p; g.epi.ci = dci.p; g.epi.ldc = ldc; g.epi.alpha = 1.0; g.epi.ld_add = ldc;
+  using C = GemmCfg<W>;
+  constexpr int NT = (C::BM / C::WM) * (C::BN / C::WN) * 64;
+  if (SYM) {
+    const int T = (m + C::BM - 1) / C::BM;
+    gemm_nt_kernel<W, MODE, true><<<dim3(T * (T + 1) / 2), dim3(NT)>>>(g);
+  } else {
+    dim3 grid((n + C::BN - 1) / C::BN, (m + C::BM - 1) / C::BM);
+    gemm_nt_kernel<W, MODE, false><<<grid, dim3(NT)>>>(g);
+  }
+  FX_LAUNCH();
+  FX_HIP(hipDeviceSynchronize());
+  dcr.get(cr, m, n, ldc);
+  if (MODE != MODE_REAL) dci.get(ci, m, n, ldc);
+}
+
+template <typename W>
+void test_gemm_w(int mode, int sym, int m, int n, int k, const double* ar, const double* ai, const double* br,
+                 const double* bi, double* cr, double* ci) {
+  if (mode == MODE_REAL && !sym) test_gemm_t<W, MODE_REAL, false>(m, n, k, ar, ai, br, bi, cr, ci);
+  else if (mode == MODE_REAL) test_gemm_t<W, MODE_REAL, true>(m, n, k, ar, ai, br, bi, cr, ci);
+  else if (mode == MODE_C4M && !sym) test_gemm_t<W, MODE_C4M, false>(m, n, k, ar, ai, br, bi, cr, ci);
+  else if (mode == MODE_C3M && !sym) test_gemm_t<W, MODE_C3M, false>(m, n, k, ar, ai, br, bi, cr, ci);
+  else if (mode == MODE_C3M) test_gemm_t<W, MODE_C3M, true>(m, n, k, ar, ai, br, bi, cr, ci);
+  else throw Error(FEXPM_EINVAL, "unsupported gemm variant");
+}
+
+}  // namespace
+
+extern "C" {
+
+int fexpm_test_gemm(int wtype, int mode, int sym, int m, int n, int k, const double* ar, const double* ai,
+                    const double* br, const double* bi, double* cr, double* ci) {
+  if (m <= 0 || n <= 0 || k <= 0 || !ar || !br || !cr || (wtype != 0 && wtype != 1)) return FEXPM_EINVAL;
+  if (mode != MODE_REAL && (!ai || !bi || !ci)) return FEXPM_EINVAL;
+  if (sym && m != n) return FEXPM_EINVAL;
+  return guarded(nullptr, [&] {
+    check_device();
+    if (wtype == 0) test_gemm_w<float>(mode, sym, m, n, k, ar, ai, br, bi, cr, ci);
+    else test_gemm_w<double>(mode, sym, m, n, k, ar, ai, br, bi, cr, ci);
+  });
+}
+
+int fexpm_test_lu_solve(int n, const double* mr, const double* mi, const double* br, const double* bi,
+                        double* xr, double* xi, int* info) {
+  if (n <= 0 || !mr || !mi || !br || !bi || !xr || !xi) return FEXPM_EINVAL;
+  return guarded(nullptr, [&] {
+    check_device();
+    const int np = (int)round_up(n, 128);
+    const int64_t ldl = 2 * (int64_t)np;
+    std::vector<double> hr((size_t)np * ldl, 0.0), hi((size_t)np * ldl, 0.0);
+    for (int i = 0; i < np; ++i) {
+      for (int j = 0; j < np; ++j) {
+        if (i < n && j < n) {
+          hr[(size_t)i * ldl + j] = mr[(size_t)i * n + j];
+          hi[(size_t)i * ldl + j] = mi[(size_t)i * n + j];
+          hr[(size_t)i * ldl + np + j] = br[(size_t)i * n + j];
+          hi[(size_t)i * ldl + np + j] = bi[(size_t)i * n + j];
+        } else if (i == j) {
+          hr[(size_t)i * ldl + j] = 1.0;
+        }
+      }
+    }
+    DevPlane<double> lr((size_t)np * ldl), li((size_t)np * ldl), tr((size_t)2 * np * LU_NB),
+        ti((size_t)2 * np * LU_NB), top(4 * LU_NB);
+    int* ipiv = nullptr;
+    PanelCand* cand = nullptr;
+    Scalars* scal = nullptr;
+    const int gmax = (np + PANEL_R - 1) / PANEL_R;
+    FX_HIP(hipMalloc(&ipiv, np * sizeof(int)));
+    FX_HIP(hipMalloc(&cand, 2 * gmax * sizeof(PanelCand)));
+    FX_HIP(hipMalloc(&scal, sizeof(Scalars)));
+    FX_HIP(hipMemset(scal, 0, sizeof(Scalars)));
+    FX_HIP(hipMemcpy(lr.p, hr.data(), hr.size() * 8, hipMemcpyHostToDevice));
+    FX_HIP(hipMemcpy(li.p, hi.data(), hi.size() * 8, hipMemcpyHostToDevice));
+    lu_factor_solve(lr.p, li.p, ldl, np, tr.p, ti.p, ipiv, cand, gmax, top.p, scal, nullptr);
+    FX_HIP(hipDeviceSynchronize());
+    FX_HIP(hipMemcpy(hr.data(), lr.p, hr.size() * 8, hipMemcpyDeviceToHost));
+    FX_HIP(hipMemcpy(hi.data(), li.p, hi.size() * 8, hipMemcpyDeviceToHost));
+    Scalars hs;
+    FX_HIP(hipMemcpy(&hs, scal, sizeof(Scalars), hipMemcpyDeviceToHost));
+    (void)hipFree(ipiv);
+    (void)hipFree(cand);
+    (void)hipFree(scal);
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) {
+        xr[(size_t)i * n + j] = hr[(size_t)i * ldl + np + j];
+        xi[(size_t)i * n + j] = hi[(size_t)i * ldl + np + j];
+      }
+    if (info) *info = hs.info;
+  });
+}
+
+int fexpm_debug_plane(fexpm_engine* e, int which, double* re, double* im) {
+  if (!e || !re || !im) return FEXPM_EINVAL;
+  std::lock_guard<std::mutex> lk(e->mtx);
+  if (e->freed) return FEXPM_EFREED;
+  if (e->n == 0) return FEXPM_OK;
+  return guarded(e, [&] { e->pipe->debug_plane(which, re, im); });
 }
 
 }  // extern "C"

@@ -10,7 +10,9 @@
  * module `_fastexpm_core` is a thin pybind11 layer over it.
  *
  * Status codes: every entry point returns FEXPM_OK (0) or a negative code;
- * fexpm_last_error() gives the message of the last failure on an engine.
+ * fexpm_last_error(e) gives the message of the last failure on engine e,
+ * fexpm_last_error(NULL) that of the calling thread's last failed call
+ * without an engine (fexpm_create, testing entry points).
  * An engine serialises its own calls (one mutex per engine); separate
  * engines are independent.
  */

@@ -1,0 +1,59 @@
+/*
+ * fastexpm testing ABI -- component entry points for parity tests.
+ *
+ * These have no counterpart in the reference (whose engine body is absent,
+ * CMakeLists.txt:23-25); they expose single kernels and internal stages of
+ * the engine so that tests can pin each stage against the CPU oracle:
+ *   - fexpm_test_gemm: one launch of the MFMA GEMM family on host data;
+ *   - fexpm_test_lu_solve: the FP64 blocked LU + solve on host data;
+ *   - fexpm_debug_plane: copy an internal working plane after a run;
+ *   - option FEXPM_OPT_MAX_SQUARINGS: perform at most that many squarings
+ *     (-1 = all), so that the Pade value X can be inspected.
+ * All arrays are row-major, FP64 on the host side.
+ */
+#ifndef FASTEXPM_TESTING_H
+#define FASTEXPM_TESTING_H
+
+#include <stdint.h>
+
+#include "fastexpm.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { FEXPM_OPT_MAX_SQUARINGS = 100 };
+
+/* Internal planes readable by fexpm_debug_plane (working precision, n x n). */
+enum fexpm_plane {
+  FEXPM_PLANE_A = 0,   /* (A - mu I) / 2^s; structured: its imaginary part G */
+  FEXPM_PLANE_A2 = 1,  /* A^2 (structured: G^2) */
+  FEXPM_PLANE_A4 = 2,
+  FEXPM_PLANE_A6 = 3,
+  FEXPM_PLANE_T1 = 4,
+  FEXPM_PLANE_T2 = 5,
+  FEXPM_PLANE_U = 6,   /* Pade odd part U (structured: U = i * plane) */
+  FEXPM_PLANE_V = 7,   /* Pade even part V */
+  FEXPM_PLANE_X0 = 8,
+  FEXPM_PLANE_X1 = 9,
+  FEXPM_PLANE_RESULT = 10 /* X after the squarings, before the phase e^mu */
+};
+
+/* C = A B^T (mode 0 real, 1 complex 4M, 2 complex 3M), sym != 0: symmetric
+ * half-tile output (requires m == n).  wtype 0 = FP32 MFMA, 1 = FP64 MFMA.
+ * A is m x k, B is n x k, C is m x n; imaginary pointers unused for mode 0. */
+int fexpm_test_gemm(int wtype, int mode, int sym, int m, int n, int k, const double* ar, const double* ai,
+                    const double* br, const double* bi, double* cr, double* ci);
+
+/* X = M^{-1} B for complex n x n M and B (planar re/im); *info != 0 if an
+ * exactly zero pivot was met. */
+int fexpm_test_lu_solve(int n, const double* mr, const double* mi, const double* br, const double* bi,
+                        double* xr, double* xi, int* info);
+
+int fexpm_debug_plane(fexpm_engine* e, int which, double* re, double* im);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

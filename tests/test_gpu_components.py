@@ -1,0 +1,179 @@
+"""Component parity on the GPU through the testing C ABI
+(include/fastexpm_testing.h): one MFMA GEMM launch, the FP64 LU solve, and
+the engine's internal Pade stages, each against an FP64 numpy reference.
+
+Tolerances: FP32 MFMA products 2e-6 relative (Frobenius) for K <= 512, FP64
+products 1e-13, LU solve 1e-12 (well-conditioned systems).
+"""
+import ctypes
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from fastctqw_amd.graphs import erdos_renyi, laplacian
+from oracle.metrics import rel_fro
+from oracle.padeexpm import PADE_B, pade_expm
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "fastctqw_amd", "fastexpm", "libfastexpm_hip.so")
+
+_D = ctypes.POINTER(ctypes.c_double)
+
+
+def _say(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def lib():
+    lib = ctypes.CDLL(LIB)
+    lib.fexpm_last_error.restype = ctypes.c_char_p
+    return lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_D)
+
+
+def _gemm(lib, wtype, mode, sym, a, b):
+    m, k = a.shape
+    n = b.shape[0]
+    ar = np.ascontiguousarray(a.real, dtype=np.float64)
+    br = np.ascontiguousarray(b.real, dtype=np.float64)
+    ai = np.ascontiguousarray(a.imag, dtype=np.float64) if mode else None
+    bi = np.ascontiguousarray(b.imag, dtype=np.float64) if mode else None
+    cr = np.zeros((m, n))
+    ci = np.zeros((m, n)) if mode else None
+    rc = lib.fexpm_test_gemm(wtype, mode, sym, m, n, k, _p(ar), _p(ai), _p(br), _p(bi), _p(cr), _p(ci))
+    assert rc == 0, lib.fexpm_last_error(None)
+    return cr if mode == 0 else cr + 1j * ci
+
+
+GEMM_SHAPES = [(16, 16, 16), (64, 64, 32), (128, 128, 64), (200, 130, 48), (257, 300, 80), (17, 5, 7)]
+
+
+@pytest.mark.parametrize("wtype", [0, 1])
+@pytest.mark.parametrize("shape", GEMM_SHAPES)
+def test_gemm_real(lib, wtype, shape):
+    m, n, k = shape
+    _say(f"gemm real w={wtype} {shape}")
+    rng = np.random.default_rng(m * 7 + n)
+    a, b = rng.standard_normal((m, k)), rng.standard_normal((n, k))
+    c = _gemm(lib, wtype, 0, 0, a, b)
+    tol = 2e-6 if wtype == 0 else 1e-13
+    assert rel_fro(a @ b.T, c) < tol
+
+
+@pytest.mark.parametrize("wtype", [0, 1])
+@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("shape", GEMM_SHAPES)
+def test_gemm_complex(lib, wtype, mode, shape):
+    m, n, k = shape
+    _say(f"gemm complex w={wtype} mode={mode} {shape}")
+    rng = np.random.default_rng(m * 3 + n + mode)
+    a = rng.standard_normal((m, k)) + 1j * rng.standard_normal((m, k))
+    b = rng.standard_normal((n, k)) + 1j * rng.standard_normal((n, k))
+    c = _gemm(lib, wtype, mode, 0, a, b)
+    tol = 4e-6 if wtype == 0 else 1e-13
+    assert rel_fro(a @ b.T, c) < tol
+
+
+@pytest.mark.parametrize("wtype", [0, 1])
+@pytest.mark.parametrize("n", [16, 100, 128, 129, 300])
+def test_gemm_symmetric(lib, wtype, n):
+    _say(f"gemm sym w={wtype} n={n}")
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal((n, n))
+    c = _gemm(lib, wtype, 0, 1, x, x)
+    tol = 2e-6 if wtype == 0 else 1e-13
+    assert rel_fro(x @ x.T, c) < tol
+    assert np.array_equal(c, c.T)
+    z = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    z = z + z.T
+    c = _gemm(lib, wtype, 2, 1, z, z)
+    assert rel_fro(z @ z, c) < 2 * tol
+    assert np.array_equal(c, c.T)
+
+
+def _solve(lib, m, b):
+    n = m.shape[0]
+    mr, mi = np.ascontiguousarray(m.real), np.ascontiguousarray(m.imag)
+    br, bi = np.ascontiguousarray(b.real), np.ascontiguousarray(b.imag)
+    xr, xi = np.zeros((n, n)), np.zeros((n, n))
+    info = ctypes.c_int(0)
+    rc = lib.fexpm_test_lu_solve(n, _p(mr), _p(mi), _p(br), _p(bi), _p(xr), _p(xi), ctypes.byref(info))
+    assert rc == 0, lib.fexpm_last_error(None)
+    return xr + 1j * xi, info.value
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 63, 64, 65, 100, 200, 333])
+def test_lu_solve(lib, n):
+    _say(f"lu n={n}")
+    rng = np.random.default_rng(n)
+    m = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + 2 * np.sqrt(n) * np.eye(n)
+    b = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    x, info = _solve(lib, m, b)
+    assert info == 0
+    assert rel_fro(np.linalg.solve(m, b), x) < 1e-12
+
+
+def test_lu_needs_pivoting(lib):
+    _say("lu pivoting")
+    n = 150
+    rng = np.random.default_rng(1)
+    perm = rng.permutation(n)
+    m = np.zeros((n, n), complex)
+    m[np.arange(n), perm] = 1.0 + 0.5j  # zero diagonal: fails without row interchanges
+    m += 1e-3 * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    b = rng.standard_normal((n, n)) + 0j
+    x, info = _solve(lib, m, b)
+    assert info == 0
+    assert rel_fro(np.linalg.solve(m, b), x) < 1e-11
+
+
+def test_lu_singular_flag(lib):
+    _say("lu singular")
+    n = 20
+    m = np.zeros((n, n), complex)
+    m[:, :10] = np.eye(n)[:, :10]
+    _, info = _solve(lib, m, np.eye(n, dtype=complex))
+    assert info != 0
+
+
+# ---------------------------------------------------------------------------
+# Engine stages (structured path) against the FP64 stage restatement
+
+def _planes(lib_fx, e, which, n):
+    re, im = np.zeros((n, n)), np.zeros((n, n))
+    lib = ctypes.CDLL(LIB)
+    return re, im
+
+
+@pytest.mark.parametrize("dt", [np.complex64, np.complex128])
+def test_structured_stages(lib, dt):
+    from fastctqw_amd.fastexpm import ExpMatComplex64, ExpMatComplex128
+
+    n, t = 200, 3.0
+    _say(f"stages n={n} {np.dtype(dt).name}")
+    L = laplacian(erdos_renyi(n, 0.5, seed=2))
+    a = (-1j * t * L).astype(dt)
+    st = pade_expm(a.astype(np.complex128), stages=True)
+    e = (ExpMatComplex64 if dt == np.complex64 else ExpMatComplex128)(n)
+    e._backend.set_option(100, 0)  # no squarings: output = e^mu X
+    u0 = e.run(a)
+    s = e.stats()
+    assert s["structured"] and s["m"] == st["m"] and s["s"] == st["s"]
+    tol = 1e-5 if dt == np.complex64 else 1e-12
+    # FP64 C (imag part of the shifted, scaled A)
+    c_ref = np.imag(a.astype(np.complex128) - st["mu"] * np.eye(n)) / 2 ** st["s"]
+    assert abs(s["mu"] - st["mu"]) <= 1e-12 * abs(st["mu"])
+    x_ref = np.exp(st["mu"]) * st["X"]
+    err = rel_fro(x_ref, u0)
+    print(f"  X relF={err:.3e}", flush=True)
+    assert err < tol
+    del c_ref
+    e.free()
