@@ -25,7 +25,8 @@
 // global -> registers -> LDS with double buffering; rows keep k contiguous
 // so one 16-byte LDS read feeds four MFMA k-steps: lane l of a 16x16x4 MFMA
 // supplies A[l%16][k] with k = 4*(l/16) + step, and the same permutation of
-// k is used for B, so the products pair correctly.
+// k is used for B, so the products pair correctly.  Accumulator element r
+// of lane l holds column l%16 and row acc_row<T>(l, r) of the 16x16 block.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -51,6 +52,13 @@ __device__ __forceinline__ f64x4 mfma16(double a, double b, f64x4 c) {
 }
 
 enum GemmMode { MODE_REAL = 0, MODE_C4M = 1, MODE_C3M = 2 };
+
+// Row (within a 16x16 block) of accumulator element r of a lane; the column
+// is lane % 16.  The FP32 16x16x4 MFMA keeps four consecutive rows per lane,
+// the FP64 one interleaves them with stride 4.
+template <typename T> __device__ __forceinline__ int acc_row(int lane, int r);
+template <> __device__ __forceinline__ int acc_row<float>(int lane, int r) { return 4 * (lane >> 4) + r; }
+template <> __device__ __forceinline__ int acc_row<double>(int lane, int r) { return (lane >> 4) + 4 * r; }
 
 // Epilogue: C = alpha*acc + beta*C_old + sum_t addc[t]*ADD_t + diag*I.
 // ADD_t are planar (re, im) matrices sharing ld_add; for MODE_REAL only the
@@ -271,7 +279,7 @@ void gemm_nt_kernel(GemmArgs<T> g) {
     for (int j = 0; j < FJ; ++j) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = row0 + wr + 16 * i + 4 * (lane >> 4) + r;
+        const int row = row0 + wr + 16 * i + acc_row<T>(lane, r);
         const int col = col0 + wc + 16 * j + l16;
         if (row >= g.m || col >= g.n) continue;
         if (diag_tile && row < col) continue;
