@@ -131,15 +131,23 @@ def test_closed_forms(fx):
 @pytest.mark.parametrize("n", [16, 64, 256])
 @pytest.mark.parametrize("family", ["complete", "er", "r4"])
 def test_reference_accuracy_inputs(fx, dt, n, family):
+    """Against the exact closed form (complete graph) or the eigen oracle,
+    and against the reference's Taylor oracle, whose own FP64 error grows
+    with its ceil(log2 ||A||_1) squarings (1e-9 allowed for FP64 inputs)."""
     if family == "complete":
         adj = complete_graph(n)
+        exact = expm_complete(n, 1.0)
     elif family == "er":
         adj = erdos_renyi(n, 0.05, seed=n)
+        exact = eig_propagator(-adj, 1.0)
     else:
         adj = random_regular(n, 4, seed=n)
+        exact = eig_propagator(-adj, 1.0)
     a = 1j * adj
-    _, stats = _check(fx, a, dt, f"ref-{family}", ref=taylor_expm(a, m=100))
+    u, stats = _check(fx, a, dt, f"ref-{family}", ref=exact)
     assert stats["structured"]
+    tay = taylor_expm(a, m=100)
+    assert rel_fro(tay, u) <= max(1e-9, _tol(dt, stats["s"]))
 
 
 # ---------------------------------------------------------------------------
