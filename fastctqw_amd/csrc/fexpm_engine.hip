@@ -204,6 +204,7 @@ template <typename W> struct Pipeline : PipelineBase {
   double* colpart = nullptr;
   double* off = nullptr;
   double2* diag = nullptr;
+  double2* dsq[2] = {nullptr, nullptr};  // split-form diagonals (squarings)
   int* flags = nullptr;
   Scalars* scal = nullptr;
   hipEvent_t ev[8];
@@ -236,6 +237,8 @@ template <typename W> struct Pipeline : PipelineBase {
     colpart = dalloc<double>((size_t)std::max(nt, 1) * std::max(n, 1));
     off = dalloc<double>(std::max(n, 1));
     diag = dalloc<double2>(std::max(n, 1));
+    dsq[0] = dalloc<double2>(std::max(n, 1));
+    dsq[1] = dalloc<double2>(std::max(n, 1));
     flags = dalloc<int>(4);
     scal = dalloc<Scalars>(1);
     for (int k = 0; k < 8; ++k) FX_HIP(hipEventCreate(&ev[k]));
@@ -258,6 +261,8 @@ template <typename W> struct Pipeline : PipelineBase {
     dfree(colpart);
     dfree(off);
     dfree(diag);
+    dfree(dsq[0]);
+    dfree(dsq[1]);
     dfree(flags);
     dfree(scal);
     if (events)
@@ -458,28 +463,39 @@ template <typename W> struct Pipeline : PipelineBase {
     flops += lu_factor_solve(lr, li, ldl, inp, tr, ti, ipiv, cand, gmax, top, scal, st);
     {
       dim3 grid((n + 31) / 32, (n + 31) / 32);
-      solution_to_work<W><<<grid, 256, 0, st>>>(lr + inp, li + inp, ldl, x0.r, x0.i, ld, n, structured ? 1 : 0);
+      solution_to_work<W><<<grid, 256, 0, st>>>(lr + inp, li + inp, ldl, x0.r, x0.i, ld, n, structured ? 1 : 0,
+                                                dsq[0]);
       FX_LAUNCH();
     }
   }
 
   // ---- squarings ----
+  // X = diag(d) + Y on entry (x0 holds Y, dsq[0] holds d); each squaring
+  // keeps the split form; the diagonal is merged back at the end.
   Plane2<W>* square(int s, bool structured, hipStream_t st) {
     Plane2<W>* cur = &x0;
     Plane2<W>* nxt = &x1;
+    int dc = 0;
     for (int q = 0; q < s; ++q) {
+      GemmArgs<W> g;
       if (structured) {
-        GemmArgs<W> g = args(*cur, *cur, ld, ld, n, n, (int)np);
-        set_out(g, *nxt);
-        gemm<MODE_C3M, true>(g, st);
+        g = args(*cur, *cur, ld, ld, n, n, (int)np);
       } else {
         transpose(*cur, t1, true, st);
-        GemmArgs<W> g = args(*cur, t1, ld, ld, n, n, (int)np);
-        set_out(g, *nxt);
-        gemm<MODE_C3M, false>(g, st);
+        g = args(*cur, t1, ld, ld, n, n, (int)np);
       }
+      set_out(g, *nxt);
+      g.epi.d_in = dsq[dc];
+      g.epi.d_out = dsq[dc ^ 1];
+      if (structured)
+        gemm<MODE_C3M, true>(g, st);
+      else
+        gemm<MODE_C3M, false>(g, st);
       std::swap(cur, nxt);
+      dc ^= 1;
     }
+    merge_diagonal<W><<<(n + 255) / 256, 256, 0, st>>>(cur->r, cur->i, ld, n, dsq[dc]);
+    FX_LAUNCH();
     return cur;
   }
 

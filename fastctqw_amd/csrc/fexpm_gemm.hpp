@@ -75,6 +75,13 @@ template <typename T> struct Epilogue {
   double addc[3];
   int64_t ld_add;
   double diag;
+  // Diagonal-split squaring (MODE_C3M only): the operand is Y = X - diag(d)
+  // (zero diagonal, the A planes), d in FP64.  Then
+  //   X^2 = diag(d^2 + (YY)_ii) + [(d_i + d_j) Y_ij + (YY)_ij]_{i != j},
+  // so the MFMA never sums products of an O(1) diagonal with O(eps) terms.
+  // Writes d_out and leaves zeros on the output diagonal.
+  const double2* d_in;
+  double2* d_out;
 };
 
 template <typename T> struct GemmArgs {
@@ -295,6 +302,21 @@ void gemm_nt_kernel(GemmArgs<T> g) {
           vi = (double)(t3 - t1 - t2);
         }
         const int64_t o = (int64_t)row * e.ldc + col;
+        if (MODE == MODE_C3M && e.d_in) {
+          if (row == col) {
+            const double2 dd = e.d_in[row];
+            e.d_out[row] = make_double2(dd.x * dd.x - dd.y * dd.y + vr, 2.0 * dd.x * dd.y + vi);
+            vr = 0.0;
+            vi = 0.0;
+          } else {
+            const double2 di = e.d_in[row], dj = e.d_in[col];
+            const double sr = di.x + dj.x, si = di.y + dj.y;
+            const int64_t oy = (int64_t)row * g.lda + col;
+            const double yr = (double)g.ar[oy], yi = (double)g.ai[oy];
+            vr += sr * yr - si * yi;
+            vi += sr * yi + si * yr;
+          }
+        }
         T outr = (T)(e.alpha * vr);
         T outi = (T)(e.alpha * vi);
         if (e.beta != 0.0) {

@@ -308,11 +308,12 @@ __global__ void form_system(const W* __restrict__ ur, const W* __restrict__ ui, 
 }
 
 // X (FP64, right block of the system) -> working precision, optionally
-// symmetrised: x = (X + X^T)/2.
+// symmetrised: x = (X + X^T)/2.  The diagonal goes to d (FP64) and the
+// planes keep a zero diagonal (split form X = diag(d) + Y for squaring).
 template <typename W>
 __global__ __launch_bounds__(256) void solution_to_work(const double* __restrict__ xr, const double* __restrict__ xi,
                                                         int64_t ldx, W* __restrict__ wr, W* __restrict__ wi,
-                                                        int64_t ldw, int n, int symmetrize) {
+                                                        int64_t ldw, int n, int symmetrize, double2* __restrict__ d) {
   __shared__ double tr[32][33];
   __shared__ double ti[32][33];
   const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
@@ -335,10 +336,25 @@ __global__ __launch_bounds__(256) void solution_to_work(const double* __restrict
         vr = 0.5 * (vr + tr[tx][r]);
         vi = 0.5 * (vi + ti[tx][r]);
       }
+      if (i == j) {
+        d[i] = make_double2(vr, vi);
+        vr = 0.0;
+        vi = 0.0;
+      }
       wr[(int64_t)i * ldw + j] = (W)vr;
       wi[(int64_t)i * ldw + j] = (W)vi;
     }
   }
+}
+
+// Split form back to a plain matrix: X_ii = d_i (rounded to W).
+template <typename W>
+__global__ void merge_diagonal(W* __restrict__ wr, W* __restrict__ wi, int64_t ldw, int n,
+                               const double2* __restrict__ d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  wr[(int64_t)i * ldw + i] = (W)d[i].x;
+  wi[(int64_t)i * ldw + i] = (W)d[i].y;
 }
 
 // Output: out[i][j] = e^mu * (xr + i xi), interleaved complex or real.
