@@ -73,6 +73,36 @@ def cpu_baseline(n_target, s_target, n_sample, t, seed, budget_s=20.0):
     }
 
 
+def _eigh_reference(n, p, seed, t, out_path):
+    """Child process: FP64 eigendecomposition of the same Laplacian with
+    torch.linalg.eigh on the GPU; writes psi_ref for e_0 and for a seeded
+    random state, plus (lambda, V) checks.  Runs in its own process so that
+    torch's HIP runtime never shares a process with the engine."""
+    import torch
+
+    from fastctqw_amd.graphs import c2_laplacian, random_state
+
+    L = torch.from_numpy(c2_laplacian(n, p, seed, dtype=np.float64)).cuda()
+    t0 = time.perf_counter()
+    lam, V = torch.linalg.eigh(L)
+    torch.cuda.synchronize()
+    t_eig = time.perf_counter() - t0
+    res = torch.linalg.norm(L @ V - V * lam[None, :]) / torch.linalg.norm(L)
+    ph = torch.exp(-1j * t * lam.to(torch.complex128))
+    Vc = V.to(torch.complex128)
+    phi = torch.from_numpy(random_state(n, seed=55)).cuda()
+    e0 = torch.zeros(n, dtype=torch.complex128, device="cuda")
+    e0[0] = 1.0
+    refs = [Vc @ (ph * (Vc.T @ v)) for v in (e0, phi)]
+    np.savez(out_path, psi_e0=refs[0].cpu().numpy(), psi_rand=refs[1].cpu().numpy(),
+             eig_residual=float(res), eig_seconds=t_eig)
+
+
+def _infidelity(a, b):
+    num = abs(np.vdot(a, b)) ** 2
+    return float(max(0.0, 1.0 - num / (np.vdot(a, a).real * np.vdot(b, b).real)))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,11 +115,40 @@ def main():
     ap.add_argument("--dtype", choices=["complex64", "complex128"], default="complex64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-n", type=int, default=2000)
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the FP64 eigendecomposition reference (torch.linalg.eigh, child process)")
+    ap.add_argument("--eigh-ref", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
+
+    if args.eigh_ref:
+        _eigh_reference(args.n, args.p, args.seed, args.t, args.eigh_ref)
+        return
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}), flush=True)
         sys.exit(2)
+
+    # FP64 reference first, in a separate process, before this process
+    # touches the GPU (torch and the engine never share a process)
+    ref, ref_error = None, None
+    if not args.no_accuracy:
+        import subprocess
+        import tempfile
+
+        ref_path = os.path.join(tempfile.gettempdir(), f"fexpm_ref_{os.getpid()}.npz")
+        print("[bench] FP64 eigh reference (child process) ...", file=sys.stderr, flush=True)
+        try:
+            cp = subprocess.run([sys.executable, os.path.abspath(__file__), "--eigh-ref", ref_path,
+                                 "--n", str(args.n), "--p", str(args.p), "--seed", str(args.seed),
+                                 "--t", str(args.t)], timeout=900)
+            if cp.returncode == 0 and os.path.exists(ref_path):
+                ref = dict(np.load(ref_path))
+                os.remove(ref_path)
+            else:
+                ref_error = f"eigh reference failed (exit code {cp.returncode})"
+        except subprocess.TimeoutExpired:
+            ref_error = "eigh reference timed out"
+        print(f"[bench] reference done: {ref_error or 'ok'}", file=sys.stderr, flush=True)
 
     from fastctqw_amd.fastexpm import ExpMatComplex64, ExpMatComplex128
     from fastctqw_amd.graphs import c2_laplacian
@@ -126,14 +185,31 @@ def main():
     issued = 3 * 2.0 * (T * (T + 1) // 2) * 128 * 128 * np_pad if st["structured"] else 3 * 2.0 * np_pad ** 3
     issued_tf = issued / (sq_ms * 1e-3) / 1e12
 
-    # cheap accuracy invariants on the device result
+    # accuracy: invariants and, unless disabled, infidelity against an FP64
+    # eigendecomposition of the same H computed in a child process
+    from fastctqw_amd.graphs import random_state
+
     u = eng.download()
     ones = np.ones(n, dtype=np.complex128)
     u1 = u.astype(np.complex128) @ ones
     lap_defect = float(np.linalg.norm(u1 - ones) / np.linalg.norm(ones))
     cols = u[:, :64].astype(np.complex128)
     unit_defect = float(np.linalg.norm(cols.conj().T @ cols - np.eye(64)) / 8.0)
-    eng.free()
+    psi_e0 = u[:, 0].astype(np.complex128)
+    psi_rand = eng.apply(random_state(n, seed=55))  # FP64 accumulation on the device
+    del u
+    acc = {"laplacian_U1_defect": lap_defect, "unitarity_defect_64cols": unit_defect}
+    if ref is not None:
+        acc.update({
+            "oracle": "FP64 eigendecomposition (torch.linalg.eigh on the GPU), psi = V e^{-it lam} V^T psi0",
+            "eig_residual": float(ref["eig_residual"]), "eig_seconds": float(ref["eig_seconds"]),
+            "infidelity_e0": _infidelity(ref["psi_e0"], psi_e0),
+            "infidelity_random_state": _infidelity(ref["psi_rand"], psi_rand),
+            "rel_err_e0": float(np.linalg.norm(psi_e0 - ref["psi_e0"]) / np.linalg.norm(ref["psi_e0"])),
+            "raw_infidelity_e0": float(1.0 - abs(np.vdot(ref["psi_e0"], psi_e0)) ** 2),
+        })
+    elif ref_error:
+        acc["oracle_error"] = ref_error
 
     out = {
         "metric": "s per N=10k complex expm",
@@ -159,11 +235,12 @@ def main():
                      "issued_tflops": issued_tf, "issued_frac": issued_tf / peak,
                      "note": "achieved = 8 N^3 algorithmic flops per squaring (4M complex convention) / "
                              "avg launch; issued = real MFMA flops of the 3M half-tile kernel"},
-        "accuracy": {"laplacian_U1_defect": lap_defect, "unitarity_defect_64cols": unit_defect},
+        "accuracy": acc,
     }
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(n, s, args.cpu_sample_n, args.t, args.seed)
+    eng.free()
     print(json.dumps(out), flush=True)
 
 
