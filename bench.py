@@ -73,22 +73,27 @@ def cpu_baseline(n_target, s_target, n_sample, t, seed, budget_s=20.0):
     }
 
 
-def _eigh_reference(n, p, seed, t, out_path):
-    """Child process: FP64 eigendecomposition of the same Laplacian with
-    torch.linalg.eigh on the GPU; writes psi_ref for e_0 and for a seeded
-    random state, plus (lambda, V) checks.  Runs in its own process so that
-    torch's HIP runtime never shares a process with the engine."""
+def _eigh_reference(n, p, seed, t, out_path, dtype="complex64"):
+    """Child process: FP64 eigendecomposition of the matrix the engine
+    actually receives.  The engine's input is A = -i*t*L rounded to
+    ``dtype``; A = i*G with G = Im(A) real symmetric (exact in FP64), so
+    expm(A) = V diag(e^{i lam}) V^T with (lam, V) = eigh(G), computed with
+    torch.linalg.eigh on the GPU in FP64.  Writes psi_ref for e_0 and for a
+    seeded random state.  Runs in its own process so that torch's HIP
+    runtime never shares a process with the engine."""
     import torch
 
     from fastctqw_amd.graphs import c2_laplacian, random_state
 
-    L = torch.from_numpy(c2_laplacian(n, p, seed, dtype=np.float64)).cuda()
+    a = (-1j * t * c2_laplacian(n, p, seed, dtype=np.float64)).astype(dtype)
+    G = torch.from_numpy(np.ascontiguousarray(a.imag, dtype=np.float64)).cuda()
+    del a
     t0 = time.perf_counter()
-    lam, V = torch.linalg.eigh(L)
+    lam, V = torch.linalg.eigh(G)
     torch.cuda.synchronize()
     t_eig = time.perf_counter() - t0
-    res = torch.linalg.norm(L @ V - V * lam[None, :]) / torch.linalg.norm(L)
-    ph = torch.exp(-1j * t * lam.to(torch.complex128))
+    res = torch.linalg.norm(G @ V - V * lam[None, :]) / torch.linalg.norm(G)
+    ph = torch.exp(1j * lam.to(torch.complex128))
     Vc = V.to(torch.complex128)
     phi = torch.from_numpy(random_state(n, seed=55)).cuda()
     e0 = torch.zeros(n, dtype=torch.complex128, device="cuda")
@@ -121,7 +126,7 @@ def main():
     args = ap.parse_args()
 
     if args.eigh_ref:
-        _eigh_reference(args.n, args.p, args.seed, args.t, args.eigh_ref)
+        _eigh_reference(args.n, args.p, args.seed, args.t, args.eigh_ref, args.dtype)
         return
 
     if args.gpus != 1:
@@ -140,7 +145,7 @@ def main():
         try:
             cp = subprocess.run([sys.executable, os.path.abspath(__file__), "--eigh-ref", ref_path,
                                  "--n", str(args.n), "--p", str(args.p), "--seed", str(args.seed),
-                                 "--t", str(args.t)], timeout=900)
+                                 "--t", str(args.t), "--dtype", args.dtype], timeout=900)
             if cp.returncode == 0 and os.path.exists(ref_path):
                 ref = dict(np.load(ref_path))
                 os.remove(ref_path)
@@ -201,7 +206,8 @@ def main():
     acc = {"laplacian_U1_defect": lap_defect, "unitarity_defect_64cols": unit_defect}
     if ref is not None:
         acc.update({
-            "oracle": "FP64 eigendecomposition (torch.linalg.eigh on the GPU), psi = V e^{-it lam} V^T psi0",
+            "oracle": "FP64 eigendecomposition of G = Im(A) for the same rounded input A = iG "
+                      "(torch.linalg.eigh on the GPU), psi = V e^{i lam} V^T psi0",
             "eig_residual": float(ref["eig_residual"]), "eig_seconds": float(ref["eig_seconds"]),
             "infidelity_e0": _infidelity(ref["psi_e0"], psi_e0),
             "infidelity_random_state": _infidelity(ref["psi_rand"], psi_rand),

@@ -27,6 +27,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/fastexpm.h"
@@ -104,6 +105,9 @@ struct PipelineBase {
   virtual bool has_result() const = 0;
   virtual void debug_plane(int which, double* re, double* im) = 0;
   int max_squarings = -1;
+  int kblock = 8;          // FP32 squarings: k-tiles (of 16) per accumulation block, 0 = off
+  int fp64_squarings = 0;  // FP32 engines: leading squarings done in FP64
+  int pade_fp64 = 1;       // FP32 engines: Pade + solve in FP64 (companion FP64 pipeline)
 };
 
 // ---------------------------------------------------------------------------
@@ -272,7 +276,7 @@ template <typename W> struct Pipeline : PipelineBase {
   bool has_result() const override { return have_result; }
 
   // ---- launch helpers ----
-  template <int MODE, bool SYM>
+  template <int MODE, bool SYM, bool KBLK = false>
   void gemm(const GemmArgs<W>& g, hipStream_t st) {
     using C = GemmCfg<W>;
     constexpr int NT = (C::BM / C::WM) * (C::BN / C::WN) * 64;
@@ -280,11 +284,11 @@ template <typename W> struct Pipeline : PipelineBase {
     if (SYM) {
       const int T = (g.m + C::BM - 1) / C::BM;
       const int nb = T * (T + 1) / 2;
-      gemm_nt_kernel<W, MODE, SYM><<<dim3(nb), dim3(NT), 0, st>>>(g);
+      gemm_nt_kernel<W, MODE, SYM, KBLK><<<dim3(nb), dim3(NT), 0, st>>>(g);
       tiles = nb;
     } else {
       dim3 grid((g.n + C::BN - 1) / C::BN, (g.m + C::BM - 1) / C::BM);
-      gemm_nt_kernel<W, MODE, SYM><<<grid, dim3(NT), 0, st>>>(g);
+      gemm_nt_kernel<W, MODE, SYM, KBLK><<<grid, dim3(NT), 0, st>>>(g);
       tiles = (double)grid.x * grid.y;
     }
     FX_LAUNCH();
@@ -453,7 +457,10 @@ template <typename W> struct Pipeline : PipelineBase {
   }
 
   // ---- FP64 solve of (V - U) X = (V + U) ----
-  void solve(bool structured, hipStream_t st) {
+  // Returns the index of the split-form diagonal buffer holding diag(X) and
+  // leaves Y = X - diag(X) in x0.  With q64 > 0 (structured path of an FP32
+  // engine) the first q64 squarings run in FP64 on the free LU buffer first.
+  int solve(bool structured, int q64, hipStream_t st) {
     const int inp = (int)np;
     {
       dim3 grid((unsigned)((2 * np + 255) / 256), (unsigned)np);
@@ -461,21 +468,58 @@ template <typename W> struct Pipeline : PipelineBase {
       FX_LAUNCH();
     }
     flops += lu_factor_solve(lr, li, ldl, inp, tr, ti, ipiv, cand, gmax, top, scal, st);
-    {
+    if (q64 <= 0) {
       dim3 grid((n + 31) / 32, (n + 31) / 32);
       solution_to_work<W><<<grid, 256, 0, st>>>(lr + inp, li + inp, ldl, x0.r, x0.i, ld, n, structured ? 1 : 0,
                                                 dsq[0]);
       FX_LAUNCH();
+      return 0;
     }
+    const int kpad = (int)round_up(n, 16);
+    {
+      dim3 grid((kpad + 31) / 32, (n + 31) / 32);
+      split_fp64<<<grid, 256, 0, st>>>(lr + inp, li + inp, ldl, lr, li, n, kpad, 1, dsq[0]);
+      FX_LAUNCH();
+    }
+    double* cr = lr;
+    double* ci = li;
+    double* nr = lr + inp;
+    double* ni = li + inp;
+    int dc = 0;
+    for (int q = 0; q < q64; ++q) {
+      GemmArgs<double> g;
+      std::memset(&g, 0, sizeof(g));
+      g.ar = cr; g.ai = ci; g.lda = ldl;
+      g.br = cr; g.bi = ci; g.ldb = ldl;
+      g.m = n; g.n = n; g.k = kpad;
+      g.epi.cr = nr; g.epi.ci = ni; g.epi.ldc = ldl; g.epi.alpha = 1.0;
+      g.epi.d_in = dsq[dc]; g.epi.d_out = dsq[dc ^ 1];
+      using C = GemmCfg<double>;
+      constexpr int NT = (C::BM / C::WM) * (C::BN / C::WN) * 64;
+      const int T = (n + C::BM - 1) / C::BM;
+      gemm_nt_kernel<double, MODE_C3M, true><<<dim3(T * (T + 1) / 2), dim3(NT), 0, st>>>(g);
+      FX_LAUNCH();
+      flops += 3.0 * 2.0 * (T * (T + 1) / 2) * C::BM * C::BN * (double)kpad;
+      std::swap(cr, nr);
+      std::swap(ci, ni);
+      dc ^= 1;
+    }
+    {
+      dim3 grid((n + 255) / 256, n);
+      fp64_to_work<W><<<grid, 256, 0, st>>>(cr, ci, ldl, x0.r, x0.i, ld, n);
+      FX_LAUNCH();
+    }
+    return dc;
   }
 
   // ---- squarings ----
   // X = diag(d) + Y on entry (x0 holds Y, dsq[0] holds d); each squaring
   // keeps the split form; the diagonal is merged back at the end.
-  Plane2<W>* square(int s, bool structured, hipStream_t st) {
+  // s squarings of the split form (Y in x0, d in dsq[dc]); returns the plane
+  // holding the last Y and the index of its diagonal buffer.
+  std::pair<Plane2<W>*, int> square_split(int s, bool structured, int dc, hipStream_t st) {
     Plane2<W>* cur = &x0;
     Plane2<W>* nxt = &x1;
-    int dc = 0;
     for (int q = 0; q < s; ++q) {
       GemmArgs<W> g;
       if (structured) {
@@ -487,17 +531,50 @@ template <typename W> struct Pipeline : PipelineBase {
       set_out(g, *nxt);
       g.epi.d_in = dsq[dc];
       g.epi.d_out = dsq[dc ^ 1];
-      if (structured)
-        gemm<MODE_C3M, true>(g, st);
-      else
-        gemm<MODE_C3M, false>(g, st);
+      g.kblock = kblock;
+      const bool blk = kblock > 0 && std::is_same<W, float>::value;
+      if (structured) {
+        if (blk) gemm<MODE_C3M, true, true>(g, st);
+        else gemm<MODE_C3M, true>(g, st);
+      } else {
+        if (blk) gemm<MODE_C3M, false, true>(g, st);
+        else gemm<MODE_C3M, false>(g, st);
+      }
       std::swap(cur, nxt);
       dc ^= 1;
     }
-    merge_diagonal<W><<<(n + 255) / 256, 256, 0, st>>>(cur->r, cur->i, ld, n, dsq[dc]);
-    FX_LAUNCH();
-    return cur;
+    return {cur, dc};
   }
+
+  Plane2<W>* square(int s, bool structured, int dc, hipStream_t st) {
+    auto r = square_split(s, structured, dc, st);
+    merge_diagonal<W><<<(n + 255) / 256, 256, 0, st>>>(r.first->r, r.first->i, ld, n, dsq[r.second]);
+    FX_LAUNCH();
+    return r.first;
+  }
+
+  // Stages 3-4 in this pipeline's precision: scaled shifted planar A and the
+  // Pade U, V.
+  template <typename Tin>
+  void front_pade(const Tin* d_a, bool structured, int m, double mu_re, double mu_im, double scale,
+                  hipStream_t st) {
+    dim3 grid((n + 255) / 256, n);
+    prepare_planar<Tin, W><<<grid, 256, 0, st>>>(d_a, n, a.r, a.i, ld, mu_re, mu_im, scale, structured ? 1 : 0);
+    FX_LAUNCH();
+    if (structured)
+      pade_structured(m, st);
+    else
+      pade_general(m, st);
+  }
+
+  // FP64 companion of an FP32 engine: Pade, solve and leading squarings in
+  // FP64 (allocated on first use).
+  std::unique_ptr<Pipeline<double>> hi;
+  Pipeline<double>* companion() {
+    if (!hi) hi.reset(new Pipeline<double>(n, FEXPM_COMPLEX128));
+    return hi.get();
+  }
+  Scalars* companion_scal() { return hi ? hi->scal : scal; }
 
   template <typename Tin, typename Tout>
   void run_typed(const Tin* d_a, Tout* d_u, hipStream_t st, fexpm_stats& S, int opt_structure) {
@@ -544,20 +621,32 @@ template <typename W> struct Pipeline : PipelineBase {
     int m, s;
     choose_degree(norm, m, s);
     const double scale = std::ldexp(1.0, -s);
-    {
-      dim3 grid((n + 255) / 256, n);
-      prepare_planar<Tin, W><<<grid, 256, 0, st>>>(d_a, n, a.r, a.i, ld, mu_re, mu_im, scale, structured ? 1 : 0);
-      FX_LAUNCH();
-    }
-    if (structured)
-      pade_structured(m, st);
-    else
-      pade_general(m, st);
-    FX_HIP(hipEventRecord(ev[2], st));
-    solve(structured, st);
-    FX_HIP(hipEventRecord(ev[3], st));
     const int s_do = max_squarings >= 0 ? std::min(s, max_squarings) : s;
-    Plane2<W>* X = square(s_do, structured, st);
+    const bool hybrid = std::is_same<W, float>::value && pade_fp64;
+    int q64 = 0, dc = 0;
+    if (hybrid) {
+      // FP64 Pade + solve (+ leading squarings), then FP32 squarings
+      Pipeline<double>* h = companion();
+      h->flops = 0.0;
+      h->front_pade<Tin>(d_a, structured, m, mu_re, mu_im, scale, st);
+      FX_HIP(hipEventRecord(ev[2], st));
+      q64 = std::min(s_do, fp64_squarings);
+      const int dh = h->solve(structured, 0, st);
+      auto r = h->square_split(q64, structured, dh, st);
+      dim3 grid((n + 255) / 256, n);
+      fp64_to_work<W><<<grid, 256, 0, st>>>(r.first->r, r.first->i, h->ld, x0.r, x0.i, ld, n);
+      FX_LAUNCH();
+      FX_HIP(hipMemcpyAsync(dsq[0], h->dsq[r.second], (size_t)n * sizeof(double2), hipMemcpyDeviceToDevice, st));
+      flops += h->flops;
+      dc = 0;
+    } else {
+      front_pade<Tin>(d_a, structured, m, mu_re, mu_im, scale, st);
+      FX_HIP(hipEventRecord(ev[2], st));
+      q64 = (std::is_same<W, float>::value && structured) ? std::min(s_do, fp64_squarings) : 0;
+      dc = solve(structured, q64, st);
+    }
+    FX_HIP(hipEventRecord(ev[3], st));
+    Plane2<W>* X = square(s_do - q64, structured, dc, st);
     FX_HIP(hipEventRecord(ev[4], st));
     // phase e^mu
     const double er = std::exp(mu_re);
@@ -571,7 +660,7 @@ template <typename W> struct Pipeline : PipelineBase {
     FX_HIP(hipEventRecord(ev[5], st));
     FX_HIP(hipEventSynchronize(ev[5]));
     Scalars hs2;
-    FX_HIP(hipMemcpy(&hs2, scal, sizeof(Scalars), hipMemcpyDeviceToHost));
+    FX_HIP(hipMemcpy(&hs2, hybrid ? companion_scal() : scal, sizeof(Scalars), hipMemcpyDeviceToHost));
     res = *X;
     have_result = true;
     float t01, t12, t23, t34, t45;
@@ -594,6 +683,9 @@ template <typename W> struct Pipeline : PipelineBase {
     S.ms_output = t45;
     S.ms_total = t01 + t12 + t23 + t34 + t45;
     S.flops_issued = flops;
+    S.s_fp64 = q64;
+    S.pade_fp64 = hybrid ? 1 : 0;
+    S.kblock = kblock;
     if (hs2.info != 0) throw Error(FEXPM_ESINGULAR, "Pade denominator is exactly singular");
   }
 
@@ -887,6 +979,23 @@ int fexpm_set_option(fexpm_engine* e, int key, int64_t value) {
       return FEXPM_OK;
     case FEXPM_OPT_MAX_SQUARINGS:
       if (e->pipe) e->pipe->max_squarings = (int)value;
+      return FEXPM_OK;
+    case FEXPM_OPT_KBLOCK:
+      if (value < 0 || value > 1024) {
+        e->err = "kblock must be in [0, 1024]";
+        return FEXPM_EINVAL;
+      }
+      if (e->pipe) e->pipe->kblock = (int)value;
+      return FEXPM_OK;
+    case FEXPM_OPT_PADE_FP64:
+      if (e->pipe) e->pipe->pade_fp64 = value ? 1 : 0;
+      return FEXPM_OK;
+    case FEXPM_OPT_FP64_SQUARINGS:
+      if (value < 0 || value > 64) {
+        e->err = "fp64 squarings must be in [0, 64]";
+        return FEXPM_EINVAL;
+      }
+      if (e->pipe) e->pipe->fp64_squarings = (int)value;
       return FEXPM_OK;
   }
   e->err = "unknown option";

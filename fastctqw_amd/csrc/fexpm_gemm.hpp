@@ -93,6 +93,8 @@ template <typename T> struct GemmArgs {
   int64_t ldb;
   int m, n, k;  // k must be a multiple of 16; operand columns beyond the
                 // logical k are zero in storage
+  int kblock;   // KBLK kernels: flush the MFMA accumulators every kblock
+                // k-tiles of 16 into the outer sums (>= 1)
   Epilogue<T> epi;
 };
 
@@ -167,7 +169,13 @@ __device__ __forceinline__ int xcd_remap(int b, int nblocks) {
   return (b % NX) * per + b / NX;
 }
 
-template <typename T, int MODE, bool SYM>
+// KBLK (MODE_C3M only): blocked accumulation.  The MFMA adds every product
+// into its FP32 accumulator one at a time, so a long sequential sum loses
+// terms smaller than half an ulp of the running sum.  With KBLK the three
+// accumulators cover kblock*16 values of k at a time and are then folded
+// into outer Re = T1 - T2 and Im = T3 - T1 - T2 sums, which bounds the
+// length of every sequential MFMA sum.
+template <typename T, int MODE, bool SYM, bool KBLK = false>
 __global__ __launch_bounds__((GemmCfg<T>::BM / GemmCfg<T>::WM) * (GemmCfg<T>::BN / GemmCfg<T>::WN) * 64)
 void gemm_nt_kernel(GemmArgs<T> g) {
   using C = GemmCfg<T>;
@@ -205,6 +213,15 @@ void gemm_nt_kernel(GemmArgs<T> g) {
     for (int i = 0; i < FI; ++i)
 #pragma unroll
       for (int j = 0; j < FJ; ++j) acc[a][i][j] = Acc{};
+  static_assert(!KBLK || MODE == MODE_C3M, "blocked accumulation is for the 3M kernel");
+  constexpr int NOUT = KBLK ? 2 : 1;
+  Acc outer[NOUT][FI][FJ];
+#pragma unroll
+  for (int a = 0; a < NOUT; ++a)
+#pragma unroll
+    for (int i = 0; i < FI; ++i)
+#pragma unroll
+      for (int j = 0; j < FJ; ++j) outer[a][i][j] = Acc{};
 
   const T* pa[2] = {g.ar + (int64_t)row0 * g.lda, (NPL == 2) ? g.ai + (int64_t)row0 * g.lda : nullptr};
   const T* pb[2] = {g.br + (int64_t)col0 * g.ldb, (NPL == 2) ? g.bi + (int64_t)col0 * g.ldb : nullptr};
@@ -266,6 +283,19 @@ void gemm_nt_kernel(GemmArgs<T> g) {
         }
       }
     }
+    if (KBLK && ((kt + 1) % g.kblock == 0 || kt + 1 == KT)) {
+#pragma unroll
+      for (int i = 0; i < FI; ++i) {
+#pragma unroll
+        for (int j = 0; j < FJ; ++j) {
+          outer[0][i][j] += acc[0][i][j] - acc[1][i][j];
+          outer[1][i][j] += acc[2][i][j] - acc[0][i][j] - acc[1][i][j];
+          acc[0][i][j] = Acc{};
+          acc[1][i][j] = Acc{};
+          acc[2][i][j] = Acc{};
+        }
+      }
+    }
     if (kt + 1 < KT) {
       const int nxt = cur ^ 1;
 #pragma unroll
@@ -296,6 +326,9 @@ void gemm_nt_kernel(GemmArgs<T> g) {
         } else if (MODE == MODE_C4M) {
           vr = (double)acc[0][i][j][r];
           vi = (double)acc[1][i][j][r];
+        } else if (KBLK) {
+          vr = (double)outer[0][i][j][r];
+          vi = (double)outer[1][i][j][r];
         } else {
           const T t1 = acc[0][i][j][r], t2 = acc[1][i][j][r], t3 = acc[2][i][j][r];
           vr = (double)(t1 - t2);

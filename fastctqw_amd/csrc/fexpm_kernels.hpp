@@ -347,6 +347,59 @@ __global__ __launch_bounds__(256) void solution_to_work(const double* __restrict
   }
 }
 
+// FP64 solution (right block) -> split form in FP64 for FP64 squarings:
+// dst = sym(X) with zero diagonal (columns up to kpad, zeros beyond n),
+// d = diag(X).
+__global__ __launch_bounds__(256) void split_fp64(const double* __restrict__ xr, const double* __restrict__ xi,
+                                                  int64_t ld, double* __restrict__ yr, double* __restrict__ yi, int n,
+                                                  int kpad, int symmetrize, double2* __restrict__ d) {
+  __shared__ double tr[32][33];
+  __shared__ double ti[32][33];
+  const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  if (symmetrize) {
+    for (int r = ty; r < 32; r += 8) {
+      const int i = bx + r, j = by + tx;
+      if (i < n && j < n) {
+        tr[r][tx] = xr[(int64_t)i * ld + j];
+        ti[r][tx] = xi[(int64_t)i * ld + j];
+      }
+    }
+    __syncthreads();
+  }
+  for (int r = ty; r < 32; r += 8) {
+    const int i = by + r, j = bx + tx;
+    if (i >= n || j >= kpad) continue;
+    double vr = 0.0, vi = 0.0;
+    if (j < n) {
+      vr = xr[(int64_t)i * ld + j];
+      vi = xi[(int64_t)i * ld + j];
+      if (symmetrize) {
+        vr = 0.5 * (vr + tr[tx][r]);
+        vi = 0.5 * (vi + ti[tx][r]);
+      }
+      if (i == j) {
+        d[i] = make_double2(vr, vi);
+        vr = 0.0;
+        vi = 0.0;
+      }
+    }
+    yr[(int64_t)i * ld + j] = vr;
+    yi[(int64_t)i * ld + j] = vi;
+  }
+}
+
+// FP64 planes (split form, zero diagonal) -> working precision planes.
+template <typename W>
+__global__ void fp64_to_work(const double* __restrict__ xr, const double* __restrict__ xi, int64_t ldx,
+                             W* __restrict__ wr, W* __restrict__ wi, int64_t ldw, int n) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (j >= n) return;
+  wr[(int64_t)i * ldw + j] = (W)xr[(int64_t)i * ldx + j];
+  wi[(int64_t)i * ldw + j] = (W)xi[(int64_t)i * ldx + j];
+}
+
 // Split form back to a plain matrix: X_ii = d_i (rounded to W).
 template <typename W>
 __global__ void merge_diagonal(W* __restrict__ wr, W* __restrict__ wi, int64_t ldw, int n,

@@ -18,6 +18,7 @@
 #include <string>
 
 #include "../../include/fastexpm.h"
+#include "../../include/fastexpm_testing.h"
 
 namespace py = pybind11;
 
@@ -162,10 +163,26 @@ template <typename T, int DT> class ExpMat {
     d["ms_square"] = s.ms_square;
     d["ms_output"] = s.ms_output;
     d["flops_issued"] = s.flops_issued;
+    d["s_fp64"] = s.s_fp64;
+    d["kblock"] = s.kblock;
+    d["pade_fp64"] = (bool)s.pade_fp64;
     return d;
   }
 
   int64_t n() const { return n_; }
+
+  // Internal working plane after a run (testing ABI, include/fastexpm_testing.h).
+  py::array_t<std::complex<double>> debug_plane(int which) {
+    py::array_t<double> re({n_, n_}), im({n_, n_});
+    raise_status(fexpm_debug_plane(e_, which, re.mutable_data(), im.mutable_data()), e_);
+    py::array_t<std::complex<double>> out({n_, n_});
+    auto o = out.mutable_unchecked<2>();
+    auto r = re.unchecked<2>();
+    auto i = im.unchecked<2>();
+    for (ssize_t a = 0; a < n_; ++a)
+      for (ssize_t b = 0; b < n_; ++b) o(a, b) = std::complex<double>(r(a, b), i(a, b));
+    return out;
+  }
 
  private:
   template <typename A> void check_shape(const A& a) const {
@@ -191,6 +208,7 @@ template <typename T, int DT> static void bind(py::module_& m, const char* name)
       .def("apply", &C::apply, py::arg("psi0"))
       .def("set_option", &C::set_option, py::arg("key"), py::arg("value"))
       .def("stats", &C::stats)
+      .def("debug_plane", &C::debug_plane, py::arg("which"))
       .def_property_readonly("n", &C::n);
 }
 
@@ -202,5 +220,9 @@ PYBIND11_MODULE(_fastexpm_core, m) {
   bind<std::complex<double>, FEXPM_COMPLEX128>(m, "ExpMatComplex128");
   m.attr("OPT_PRECISION") = (int)FEXPM_OPT_PRECISION;
   m.attr("OPT_STRUCTURE") = (int)FEXPM_OPT_STRUCTURE;
+  m.attr("OPT_KBLOCK") = (int)FEXPM_OPT_KBLOCK;
+  m.attr("OPT_FP64_SQUARINGS") = (int)FEXPM_OPT_FP64_SQUARINGS;
+  m.attr("OPT_PADE_FP64") = (int)FEXPM_OPT_PADE_FP64;
+  m.attr("OPT_MAX_SQUARINGS") = (int)FEXPM_OPT_MAX_SQUARINGS;
   m.def("gpu_memory_gb", &fexpm_gpu_memory_gb);
 }

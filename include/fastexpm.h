@@ -53,8 +53,13 @@ enum fexpm_precision {
 };
 
 enum fexpm_option {
-  FEXPM_OPT_PRECISION = 1,   /* enum fexpm_precision */
-  FEXPM_OPT_STRUCTURE = 2    /* 1 = detect purely-imaginary symmetric A (default), 0 = general path */
+  FEXPM_OPT_PRECISION = 1,       /* enum fexpm_precision */
+  FEXPM_OPT_STRUCTURE = 2,       /* 1 = detect purely-imaginary symmetric A (default), 0 = general path */
+  FEXPM_OPT_KBLOCK = 3,          /* FP32 squarings: MFMA accumulation block in units of 16 k
+                                    (default 8; 0 = one block over all k) */
+  FEXPM_OPT_FP64_SQUARINGS = 4,  /* FP32 engines: number of leading squarings run in FP64 (default 0) */
+  FEXPM_OPT_PADE_FP64 = 5        /* FP32 engines: 1 = Pade products and solve in FP64 (default),
+                                    0 = Pade products in FP32 */
 };
 
 /* Per-run report (filled by fexpm_get_stats after a run). */
@@ -76,6 +81,9 @@ typedef struct fexpm_stats {
   double ms_square;     /* s squarings */
   double ms_output;     /* phase + write of U */
   double flops_issued;  /* real flops issued by the MFMA kernels this run */
+  int32_t s_fp64;       /* squarings performed in FP64 by an FP32 engine */
+  int32_t kblock;       /* accumulation block of the FP32 squarings (16-k tiles) */
+  int32_t pade_fp64;    /* 1: Pade products and solve ran in FP64 */
 } fexpm_stats;
 
 /* Create an engine for n x n matrices of the given dtype and allocate its
