@@ -111,83 +111,117 @@ struct PipelineBase {
 };
 
 // ---------------------------------------------------------------------------
-// Blocked right-looking LU with partial pivoting of the augmented FP64
-// system S = [M | B] (np x 2np, leading dimension ldl; np a multiple of
-// LU_NB), followed by block back substitution.  On return the right block
-// holds X = M^{-1} B.  Returns the real flops issued by its MFMA GEMMs.
+// Two-level blocked right-looking LU with partial pivoting of the augmented
+// FP64 system S = [M | B] (np x 2np, leading dimension ldl; np a multiple of
+// 128), followed by block back substitution.  On return the right block
+// holds X = M^{-1} B.
+//
+// Outer blocks of LU_OUTER columns are factored with 64-wide inner panels
+// whose updates stay inside the outer panel; the rows they interchange are
+// also interchanged in the outer panel's earlier L columns, so that the
+// outer L21 is consistent when it is used.  Then the outer block row
+// U12 = L11^{-1} A12 is formed slab by slab and the trailing matrix,
+// including the right-hand side, is updated by one complex GEMM with
+// K = LU_OUTER (3M on FP64 MFMA).  The forward substitution thus happens
+// inside the factorisation.  Back substitution uses the same two levels.
+// temp (tr/ti) holds 2np x LU_OUTER values per plane.  Returns the real
+// flops issued by the MFMA GEMMs.
+constexpr int LU_OUTER = 256;
+
 static double lu_gemm(const GemmArgs<double>& g, hipStream_t st) {
   using C = GemmCfg<double>;
   constexpr int NT = (C::BM / C::WM) * (C::BN / C::WN) * 64;
+  if (g.m <= 0 || g.n <= 0) return 0.0;
   dim3 grid((g.n + C::BN - 1) / C::BN, (g.m + C::BM - 1) / C::BM);
-  gemm_nt_kernel<double, MODE_C4M, false><<<grid, dim3(NT), 0, st>>>(g);
+  gemm_nt_kernel<double, MODE_C3M, false><<<grid, dim3(NT), 0, st>>>(g);
   FX_LAUNCH();
-  return 8.0 * (double)grid.x * grid.y * C::BM * C::BN * (double)g.k;
+  return 3.0 * 2.0 * (double)grid.x * grid.y * C::BM * C::BN * (double)g.k;
+}
+
+// C[rows, cols] -= A[rows, k] * B, with B given transposed (bt: ncols x k, ldb).
+static double lu_update(double* lr, double* li, int64_t ldl, int r0, int nrows, int kcol0, int k, int c0,
+                        int ncols, const double* btr, const double* bti, int ldb, hipStream_t st) {
+  GemmArgs<double> g;
+  std::memset(&g, 0, sizeof(g));
+  g.ar = lr + (int64_t)r0 * ldl + kcol0;
+  g.ai = li + (int64_t)r0 * ldl + kcol0;
+  g.lda = ldl;
+  g.br = btr;
+  g.bi = bti;
+  g.ldb = ldb;
+  g.m = nrows;
+  g.n = ncols;
+  g.k = k;
+  g.epi.cr = lr + (int64_t)r0 * ldl + c0;
+  g.epi.ci = li + (int64_t)r0 * ldl + c0;
+  g.epi.ldc = ldl;
+  g.epi.alpha = -1.0;
+  g.epi.beta = 1.0;
+  return lu_gemm(g, st);
+}
+
+static void lu_swap_cols(double* lr, double* li, int64_t ldl, int k0, const int* ipiv, int c0, int c1,
+                         hipStream_t st) {
+  if (c1 <= c0) return;
+  lu_laswp<<<(c1 - c0 + 255) / 256, 256, 0, st>>>(lr, li, ldl, k0, LU_NB, ipiv, c0, c1);
+  FX_LAUNCH();
 }
 
 static double lu_factor_solve(double* lr, double* li, int64_t ldl, int inp, double* tr, double* ti, int* ipiv,
                               PanelCand* cand, int gmax, double* top, Scalars* scal, hipStream_t st) {
   double flops = 0.0;
   const int nb = LU_NB;
-  const int nblk = inp / nb;
-  for (int kb = 0; kb < nblk; ++kb) {
-    const int k0 = kb * nb;
-    const int G = (inp - k0 + PANEL_R - 1) / PANEL_R;
-    lu_panel_init<<<G, 256, 0, st>>>(lr, li, ldl, inp, k0, cand, top);
-    FX_LAUNCH();
-    for (int j = 0; j < nb; ++j) {
-      const int par = j & 1;
-      lu_panel_step<<<G, 256, 0, st>>>(lr, li, ldl, inp, k0, j, G, cand + par * gmax, cand + (par ^ 1) * gmax,
-                                       top + par * 2 * LU_NB, top + (par ^ 1) * 2 * LU_NB, ipiv, scal);
+  const int ncol = 2 * inp;
+  for (int K0 = 0; K0 < inp; K0 += LU_OUTER) {
+    const int nbo = std::min(LU_OUTER, inp - K0);
+    const int K1 = K0 + nbo;
+    // --- outer panel S[K0:, K0:K1] with 64-wide inner panels ---
+    for (int k0 = K0; k0 < K1; k0 += nb) {
+      const int G = (inp - k0 + PANEL_R - 1) / PANEL_R;
+      lu_panel_init<<<G, 256, 0, st>>>(lr, li, ldl, inp, k0, cand, top);
       FX_LAUNCH();
+      for (int j = 0; j < nb; ++j) {
+        const int par = j & 1;
+        lu_panel_step<<<G, 256, 0, st>>>(lr, li, ldl, inp, k0, j, G, cand + par * gmax, cand + (par ^ 1) * gmax,
+                                         top + par * 2 * LU_NB, top + (par ^ 1) * 2 * LU_NB, ipiv, scal);
+        FX_LAUNCH();
+      }
+      lu_swap_cols(lr, li, ldl, k0, ipiv, K0, k0, st);          // earlier L columns of this outer panel
+      lu_swap_cols(lr, li, ldl, k0, ipiv, k0 + nb, K1, st);     // rest of the outer panel
+      if (k0 + nb < K1) {
+        lu_trsm_block_row<false><<<(K1 - k0 - nb + 63) / 64, 64, 0, st>>>(lr, li, ldl, k0, k0 + nb, K1, tr, ti,
+                                                                          nb, 0);
+        FX_LAUNCH();
+        flops += lu_update(lr, li, ldl, k0 + nb, inp - k0 - nb, k0, nb, k0 + nb, K1 - k0 - nb, tr, ti, nb, st);
+      }
     }
-    const int c0 = k0 + nb, c1 = 2 * inp;
-    lu_laswp<<<(c1 - c0 + 255) / 256, 256, 0, st>>>(lr, li, ldl, k0, nb, ipiv, c0, c1);
-    FX_LAUNCH();
-    lu_trsm_block_row<false><<<(c1 - c0 + 63) / 64, 64, 0, st>>>(lr, li, ldl, k0, c0, c1, tr, ti);
-    FX_LAUNCH();
-    if (k0 + nb < inp) {
-      GemmArgs<double> g;
-      std::memset(&g, 0, sizeof(g));
-      g.ar = lr + (int64_t)(k0 + nb) * ldl + k0;
-      g.ai = li + (int64_t)(k0 + nb) * ldl + k0;
-      g.lda = ldl;
-      g.br = tr;
-      g.bi = ti;
-      g.ldb = nb;
-      g.m = inp - k0 - nb;
-      g.n = c1 - c0;
-      g.k = nb;
-      g.epi.cr = lr + (int64_t)(k0 + nb) * ldl + (k0 + nb);
-      g.epi.ci = li + (int64_t)(k0 + nb) * ldl + (k0 + nb);
-      g.epi.ldc = ldl;
-      g.epi.alpha = -1.0;
-      g.epi.beta = 1.0;
-      flops += lu_gemm(g, st);
+    if (K1 >= ncol) continue;
+    // --- interchanges of the whole outer panel on the columns to its right ---
+    for (int k0 = K0; k0 < K1; k0 += nb) lu_swap_cols(lr, li, ldl, k0, ipiv, K1, ncol, st);
+    // --- block row U12 = L11^{-1} A12, slab by slab; transposed into tr (ld LU_OUTER) ---
+    for (int r = K0; r < K1; r += nb) {
+      lu_trsm_block_row<false><<<(ncol - K1 + 63) / 64, 64, 0, st>>>(lr, li, ldl, r, K1, ncol, tr, ti, LU_OUTER,
+                                                                     r - K0);
+      FX_LAUNCH();
+      if (r + nb < K1)
+        flops += lu_update(lr, li, ldl, r + nb, K1 - r - nb, r, nb, K1, ncol - K1, tr + (r - K0), ti + (r - K0),
+                           LU_OUTER, st);
     }
+    // --- trailing update with K = nbo ---
+    if (K1 < inp) flops += lu_update(lr, li, ldl, K1, inp - K1, K0, nbo, K1, ncol - K1, tr, ti, LU_OUTER, st);
   }
-  for (int ib = nblk - 1; ib >= 0; --ib) {
-    const int r0 = ib * nb;
-    lu_trsm_block_row<true><<<(inp + 63) / 64, 64, 0, st>>>(lr, li, ldl, r0, inp, 2 * inp, tr, ti);
-    FX_LAUNCH();
-    if (r0 > 0) {
-      GemmArgs<double> g;
-      std::memset(&g, 0, sizeof(g));
-      g.ar = lr + r0;
-      g.ai = li + r0;
-      g.lda = ldl;
-      g.br = tr;
-      g.bi = ti;
-      g.ldb = nb;
-      g.m = r0;
-      g.n = inp;
-      g.k = nb;
-      g.epi.cr = lr + inp;
-      g.epi.ci = li + inp;
-      g.epi.ldc = ldl;
-      g.epi.alpha = -1.0;
-      g.epi.beta = 1.0;
-      flops += lu_gemm(g, st);
+  // --- back substitution U X = Y on the right block ---
+  const int nouter = (inp + LU_OUTER - 1) / LU_OUTER;
+  for (int ob = nouter - 1; ob >= 0; --ob) {
+    const int I0 = ob * LU_OUTER;
+    const int nbo = std::min(LU_OUTER, inp - I0);
+    for (int r = I0 + nbo - nb; r >= I0; r -= nb) {
+      lu_trsm_block_row<true><<<(inp + 63) / 64, 64, 0, st>>>(lr, li, ldl, r, inp, ncol, tr, ti, LU_OUTER, r - I0);
+      FX_LAUNCH();
+      if (r > I0)
+        flops += lu_update(lr, li, ldl, I0, r - I0, r, nb, inp, inp, tr + (r - I0), ti + (r - I0), LU_OUTER, st);
     }
+    if (I0 > 0) flops += lu_update(lr, li, ldl, 0, I0, I0, nbo, inp, inp, tr, ti, LU_OUTER, st);
   }
   return flops;
 }
@@ -199,7 +233,7 @@ template <typename W> struct Pipeline : PipelineBase {
   Plane2<W> a, p2, p4, p6, t1, t2, u, v, x0, x1;
   double* lr = nullptr;  // FP64 system, np x 2np
   double* li = nullptr;
-  double* tr = nullptr;  // transposed solved block row (2np x 64)
+  double* tr = nullptr;  // transposed solved block rows (2np x LU_OUTER)
   double* ti = nullptr;
   int* ipiv = nullptr;
   PanelCand* cand = nullptr;  // 2 x gmax
@@ -231,8 +265,8 @@ template <typename W> struct Pipeline : PipelineBase {
     }
     lr = dalloc<double>((size_t)np * ldl);
     li = dalloc<double>((size_t)np * ldl);
-    tr = dalloc<double>((size_t)2 * np * LU_NB);
-    ti = dalloc<double>((size_t)2 * np * LU_NB);
+    tr = dalloc<double>((size_t)2 * np * LU_OUTER);
+    ti = dalloc<double>((size_t)2 * np * LU_OUTER);
     ipiv = dalloc<int>(np);
     gmax = (int)((np + PANEL_R - 1) / PANEL_R);
     cand = dalloc<PanelCand>((size_t)2 * gmax);
@@ -1134,8 +1168,8 @@ int fexpm_test_lu_solve(int n, const double* mr, const double* mi, const double*
         }
       }
     }
-    DevPlane<double> lr((size_t)np * ldl), li((size_t)np * ldl), tr((size_t)2 * np * LU_NB),
-        ti((size_t)2 * np * LU_NB), top(4 * LU_NB);
+    DevPlane<double> lr((size_t)np * ldl), li((size_t)np * ldl), tr((size_t)2 * np * LU_OUTER),
+        ti((size_t)2 * np * LU_OUTER), top(4 * LU_NB);
     int* ipiv = nullptr;
     PanelCand* cand = nullptr;
     Scalars* scal = nullptr;

@@ -714,11 +714,12 @@ __global__ void lu_laswp(double* __restrict__ sr, double* __restrict__ si, int64
 //   lower (unit):   S[k0:k0+64, cols] = L11^{-1} S[k0:k0+64, cols]
 //   upper (non-unit): S[k0:k0+64, cols] = U11^{-1} S[k0:k0+64, cols]
 // The solved block is also written transposed (row = column index - c0,
-// 64 k-values) to tr/ti (ld 64) as the right operand of the following GEMM.
+// 64 k-values) to tr/ti as the right operand of the following GEMM.
+// The transposed copy goes to tr[(col - c0) * ldt + toff + r].
 template <bool UPPER>
 __global__ __launch_bounds__(64) void lu_trsm_block_row(double* __restrict__ sr, double* __restrict__ si, int64_t ld,
                                                         int k0, int c0, int c1, double* __restrict__ tr,
-                                                        double* __restrict__ ti) {
+                                                        double* __restrict__ ti, int ldt, int toff) {
   __shared__ double lr[LU_NB][LU_NB + 1];
   __shared__ double li[LU_NB][LU_NB + 1];
   __shared__ double xr[LU_NB][64 + 1];
@@ -775,8 +776,8 @@ __global__ __launch_bounds__(64) void lu_trsm_block_row(double* __restrict__ sr,
     for (int r = 0; r < LU_NB; ++r) {
       sr[(int64_t)(k0 + r) * ld + col] = xr[r][t];
       si[(int64_t)(k0 + r) * ld + col] = xi[r][t];
-      tr[(int64_t)(col - c0) * LU_NB + r] = xr[r][t];
-      ti[(int64_t)(col - c0) * LU_NB + r] = xi[r][t];
+      tr[(int64_t)(col - c0) * ldt + toff + r] = xr[r][t];
+      ti[(int64_t)(col - c0) * ldt + toff + r] = xi[r][t];
     }
   }
 }

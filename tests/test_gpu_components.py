@@ -110,7 +110,7 @@ def _solve(lib, m, b):
     return xr + 1j * xi, info.value
 
 
-@pytest.mark.parametrize("n", [1, 2, 5, 63, 64, 65, 100, 200, 333])
+@pytest.mark.parametrize("n", [1, 2, 5, 63, 64, 65, 100, 200, 333, 640, 777])
 def test_lu_solve(lib, n):
     _say(f"lu n={n}")
     rng = np.random.default_rng(n)
