@@ -122,7 +122,8 @@ struct PipelineBase {
 // outer L21 is consistent when it is used.  Then the outer block row
 // U12 = L11^{-1} A12 is formed slab by slab and the trailing matrix,
 // including the right-hand side, is updated by one complex GEMM with
-// K = LU_OUTER (3M on FP64 MFMA).  The forward substitution thus happens
+// K = LU_OUTER (4M on FP64 MFMA; measured faster than 3M here because the
+// 3M variant needs more registers per wave).  The forward substitution thus happens
 // inside the factorisation.  Back substitution uses the same two levels.
 // temp (tr/ti) holds 2np x LU_OUTER values per plane.  Returns the real
 // flops issued by the MFMA GEMMs.
@@ -133,9 +134,9 @@ static double lu_gemm(const GemmArgs<double>& g, hipStream_t st) {
   constexpr int NT = (C::BM / C::WM) * (C::BN / C::WN) * 64;
   if (g.m <= 0 || g.n <= 0) return 0.0;
   dim3 grid((g.n + C::BN - 1) / C::BN, (g.m + C::BM - 1) / C::BM);
-  gemm_nt_kernel<double, MODE_C3M, false><<<grid, dim3(NT), 0, st>>>(g);
+  gemm_nt_kernel<double, MODE_C4M, false><<<grid, dim3(NT), 0, st>>>(g);
   FX_LAUNCH();
-  return 3.0 * 2.0 * (double)grid.x * grid.y * C::BM * C::BN * (double)g.k;
+  return 4.0 * 2.0 * (double)grid.x * grid.y * C::BM * C::BN * (double)g.k;
 }
 
 // C[rows, cols] -= A[rows, k] * B, with B given transposed (bt: ncols x k, ldb).
@@ -1207,3 +1208,98 @@ int fexpm_debug_plane(fexpm_engine* e, int which, double* re, double* im) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Device-side GEMM timing (testing ABI): average launch time over reps
+// launches on device-resident operands filled with a deterministic pattern.
+namespace {
+
+template <typename W>
+__global__ void fill_pattern(W* p, size_t cnt, uint32_t seed) {
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= cnt) return;
+  uint32_t h = (uint32_t)(k * 2654435761u) ^ seed;
+  h ^= h >> 15;
+  h *= 2246822519u;
+  h ^= h >> 13;
+  p[k] = (W)((double)(h & 0xFFFFFF) / 16777216.0 - 0.5);
+}
+
+template <typename W, int MODE, bool SYM, bool KBLK, class CFG>
+double time_gemm_t(int m, int n, int k, int reps) {
+  const int64_t kp = round_up(std::max(k, 1), 16);
+  const size_t ca = (size_t)m * kp, cb = (size_t)n * kp, cc = (size_t)m * n;
+  W *ar, *ai, *br, *bi, *cr, *ci;
+  FX_HIP(hipMalloc(&ar, ca * sizeof(W)));
+  FX_HIP(hipMalloc(&ai, ca * sizeof(W)));
+  FX_HIP(hipMalloc(&br, cb * sizeof(W)));
+  FX_HIP(hipMalloc(&bi, cb * sizeof(W)));
+  FX_HIP(hipMalloc(&cr, cc * sizeof(W)));
+  FX_HIP(hipMalloc(&ci, cc * sizeof(W)));
+  fill_pattern<W><<<(unsigned)((ca + 255) / 256), 256>>>(ar, ca, 1u);
+  fill_pattern<W><<<(unsigned)((ca + 255) / 256), 256>>>(ai, ca, 2u);
+  fill_pattern<W><<<(unsigned)((cb + 255) / 256), 256>>>(br, cb, 3u);
+  fill_pattern<W><<<(unsigned)((cb + 255) / 256), 256>>>(bi, cb, 4u);
+  fill_pattern<W><<<(unsigned)((cc + 255) / 256), 256>>>(cr, cc, 5u);
+  fill_pattern<W><<<(unsigned)((cc + 255) / 256), 256>>>(ci, cc, 6u);
+  GemmArgs<W> g;
+  std::memset(&g, 0, sizeof(g));
+  g.ar = ar; g.ai = ai; g.lda = kp; g.br = br; g.bi = bi; g.ldb = kp;
+  g.m = m; g.n = n; g.k = (int)kp; g.kblock = 8;
+  g.epi.cr = cr; g.epi.ci = ci; g.epi.ldc = n; g.epi.alpha = -1.0; g.epi.beta = SYM ? 0.0 : 1.0;
+  dim3 grid, block(CFG::NT);
+  if (SYM) {
+    const int T = (m + CFG::BM - 1) / CFG::BM;
+    grid = dim3(T * (T + 1) / 2);
+  } else {
+    grid = dim3((n + CFG::BN - 1) / CFG::BN, (m + CFG::BM - 1) / CFG::BM);
+  }
+  hipEvent_t e0, e1;
+  FX_HIP(hipEventCreate(&e0));
+  FX_HIP(hipEventCreate(&e1));
+  gemm_nt_kernel<W, MODE, SYM, KBLK, CFG><<<grid, block>>>(g);  // warm-up
+  FX_LAUNCH();
+  FX_HIP(hipEventRecord(e0));
+  for (int r = 0; r < reps; ++r) gemm_nt_kernel<W, MODE, SYM, KBLK, CFG><<<grid, block>>>(g);
+  FX_HIP(hipEventRecord(e1));
+  FX_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  FX_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  for (W* p : {ar, ai, br, bi, cr, ci}) (void)hipFree(p);
+  return (double)ms / std::max(reps, 1);
+}
+
+}  // namespace
+
+extern "C" int fexpm_test_gemm_time(int wtype, int mode, int sym, int cfg, int m, int n, int k, int reps,
+                                    double* ms) {
+  if (!ms || m <= 0 || n <= 0 || k <= 0 || reps <= 0 || (sym && m != n)) return FEXPM_EINVAL;
+  return guarded(nullptr, [&] {
+    check_device();
+    double t = -1.0;
+    if (wtype == 0) {
+      if (mode == 2 && sym) t = time_gemm_t<float, MODE_C3M, true, true, GemmCfg<float>>(m, n, k, reps);
+      else if (mode == 2) t = time_gemm_t<float, MODE_C3M, false, true, GemmCfg<float>>(m, n, k, reps);
+      else if (mode == 0 && sym) t = time_gemm_t<float, MODE_REAL, true, false, GemmCfg<float>>(m, n, k, reps);
+      else if (mode == 1) t = time_gemm_t<float, MODE_C4M, false, false, GemmCfg<float>>(m, n, k, reps);
+    } else {
+#define FX_TIME_D(CF)                                                                              \
+  do {                                                                                             \
+    if (mode == 2 && sym) t = time_gemm_t<double, MODE_C3M, true, false, CF>(m, n, k, reps);       \
+    else if (mode == 2) t = time_gemm_t<double, MODE_C3M, false, false, CF>(m, n, k, reps);       \
+    else if (mode == 1) t = time_gemm_t<double, MODE_C4M, false, false, CF>(m, n, k, reps);       \
+    else if (mode == 0 && sym) t = time_gemm_t<double, MODE_REAL, true, false, CF>(m, n, k, reps); \
+    else if (mode == 0) t = time_gemm_t<double, MODE_REAL, false, false, CF>(m, n, k, reps);      \
+  } while (0)
+      if (cfg == 0) FX_TIME_D(DCfg4w);
+      else if (cfg == 1) FX_TIME_D(DCfg8w);
+      else if (cfg == 2) FX_TIME_D(DCfgTall);
+      else if (cfg == 3) FX_TIME_D(DCfgWide);
+#undef FX_TIME_D
+    }
+    if (t < 0) throw Error(FEXPM_EINVAL, "unsupported gemm timing variant");
+    *ms = t;
+  });
+}

@@ -98,13 +98,23 @@ template <typename T> struct GemmArgs {
   Epilogue<T> epi;
 };
 
+// Tile configuration: workgroup tile BM x BN, k-tile BK, LDS row padding
+// PAD, wave tile WM x WN; NT threads.
+template <int BM_, int BN_, int BK_, int PAD_, int WM_, int WN_> struct TileCfg {
+  static constexpr int BM = BM_, BN = BN_, BK = BK_, PAD = PAD_, WM = WM_, WN = WN_;
+  static constexpr int NT = (BM / WM) * (BN / WN) * 64;
+};
 template <typename T> struct GemmCfg;
-template <> struct GemmCfg<float> {
-  static constexpr int BM = 128, BN = 128, BK = 16, PAD = 4, WM = 64, WN = 32;
-};
-template <> struct GemmCfg<double> {
-  static constexpr int BM = 64, BN = 64, BK = 16, PAD = 2, WM = 32, WN = 32;
-};
+template <> struct GemmCfg<float> : TileCfg<128, 128, 16, 4, 64, 32> {};
+// FP64: 8 waves of 32x16 keep every variant within 256 registers (two
+// waves per SIMD); measured 63 TFLOP/s (4M, K = 256) and 61 TFLOP/s (real
+// symmetric, K = 10^4) on MI355X versus 50 / 57 for 4 waves of 32x32.
+template <> struct GemmCfg<double> : TileCfg<64, 64, 16, 2, 32, 16> {};
+// Alternative FP64 tilings (selected per call site, see fexpm_test_gemm_time).
+using DCfg4w = TileCfg<64, 64, 16, 2, 32, 32>;     // 4 waves, 32x32 wave tiles
+using DCfg8w = TileCfg<64, 64, 16, 2, 32, 16>;     // 8 waves, 32x16 wave tiles
+using DCfgTall = TileCfg<128, 64, 16, 2, 32, 32>;  // 8 waves, 128x64 workgroup tile
+using DCfgWide = TileCfg<64, 128, 16, 2, 32, 32>;  // 8 waves, 64x128 workgroup tile
 
 // Staging of a ROWS x BK tile through registers, NT threads.
 template <typename T, int ROWS, int BK, int PAD, int NT> struct TileStage {
@@ -175,10 +185,10 @@ __device__ __forceinline__ int xcd_remap(int b, int nblocks) {
 // accumulators cover kblock*16 values of k at a time and are then folded
 // into outer Re = T1 - T2 and Im = T3 - T1 - T2 sums, which bounds the
 // length of every sequential MFMA sum.
-template <typename T, int MODE, bool SYM, bool KBLK = false>
-__global__ __launch_bounds__((GemmCfg<T>::BM / GemmCfg<T>::WM) * (GemmCfg<T>::BN / GemmCfg<T>::WN) * 64)
+template <typename T, int MODE, bool SYM, bool KBLK = false, class CFG = GemmCfg<T>>
+__global__ __launch_bounds__(CFG::NT)
 void gemm_nt_kernel(GemmArgs<T> g) {
-  using C = GemmCfg<T>;
+  using C = CFG;
   constexpr int BM = C::BM, BN = C::BN, BK = C::BK, PAD = C::PAD, WM = C::WM, WN = C::WN;
   constexpr int WAVES_N = BN / WN;
   constexpr int NT = (BM / WM) * WAVES_N * 64;

@@ -52,6 +52,12 @@ int fexpm_test_lu_solve(int n, const double* mr, const double* mi, const double*
 
 int fexpm_debug_plane(fexpm_engine* e, int which, double* re, double* im);
 
+/* Average device time (ms) of one GEMM launch over reps launches on
+ * device-resident operands (m x k, n x k -> m x n); cfg selects an FP64
+ * tiling (0 4-wave 32x32, 1 8-wave 32x16 = default, 2 128x64, 3 64x128;
+ * 2 and 3 are not square and are invalid with sym). */
+int fexpm_test_gemm_time(int wtype, int mode, int sym, int cfg, int m, int n, int k, int reps, double* ms);
+
 #ifdef __cplusplus
 }
 #endif
