@@ -129,6 +129,25 @@ struct PipelineBase {
 // flops issued by the MFMA GEMMs.
 constexpr int LU_OUTER = 256;
 
+// Panel factorisation: one cooperative launch per 64-wide panel (default)
+// or one launch per column.
+static int g_panel_coop = 1;
+
+// Largest cooperative grid of lu_panel_coop that the device can hold.
+static int panel_coop_capacity() {
+  static int cap = -1;
+  if (cap < 0) {
+    int dev = 0, per_cu = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)lu_panel_coop, 256, 0) != hipSuccess)
+      cap = 0;
+    else
+      cap = per_cu * prop.multiProcessorCount;
+  }
+  return cap;
+}
+
 static double lu_gemm(const GemmArgs<double>& g, hipStream_t st) {
   using C = GemmCfg<double>;
   constexpr int NT = (C::BM / C::WM) * (C::BN / C::WN) * 64;
@@ -168,8 +187,23 @@ static void lu_swap_cols(double* lr, double* li, int64_t ldl, int k0, const int*
   FX_LAUNCH();
 }
 
+// S[k0:k0+64, c0:c1] <- T^{-1} S[k0:k0+64, c0:c1] with T the unit lower
+// (UPPER = false) or upper (UPPER = true) diagonal block at k0; transposed
+// copy to tr/ti.  inv holds 2 x 64 x 64 doubles of scratch.
+template <bool UPPER>
+static void lu_block_row_solve(double* lr, double* li, int64_t ldl, int k0, int c0, int c1, double* tr, double* ti,
+                               int ldt, int toff, double* inv, hipStream_t st) {
+  if (c1 <= c0) return;
+  lu_invert_diag_block<UPPER><<<1, 64, 0, st>>>(lr, li, ldl, k0, inv, inv + LU_NB * LU_NB);
+  FX_LAUNCH();
+  lu_apply_inverse<<<(c1 - c0 + 63) / 64, 256, 0, st>>>(lr, li, ldl, k0, c0, c1, inv, inv + LU_NB * LU_NB, tr, ti,
+                                                        ldt, toff);
+  FX_LAUNCH();
+}
+
 static double lu_factor_solve(double* lr, double* li, int64_t ldl, int inp, double* tr, double* ti, int* ipiv,
-                              PanelCand* cand, int gmax, double* top, Scalars* scal, hipStream_t st) {
+                              PanelCand* cand, int gmax, double* top, Scalars* scal, double* inv,
+                              hipStream_t st) {
   double flops = 0.0;
   const int nb = LU_NB;
   const int ncol = 2 * inp;
@@ -179,20 +213,24 @@ static double lu_factor_solve(double* lr, double* li, int64_t ldl, int inp, doub
     // --- outer panel S[K0:, K0:K1] with 64-wide inner panels ---
     for (int k0 = K0; k0 < K1; k0 += nb) {
       const int G = (inp - k0 + PANEL_R - 1) / PANEL_R;
-      lu_panel_init<<<G, 256, 0, st>>>(lr, li, ldl, inp, k0, cand, top);
-      FX_LAUNCH();
-      for (int j = 0; j < nb; ++j) {
-        const int par = j & 1;
-        lu_panel_step<<<G, 256, 0, st>>>(lr, li, ldl, inp, k0, j, G, cand + par * gmax, cand + (par ^ 1) * gmax,
-                                         top + par * 2 * LU_NB, top + (par ^ 1) * 2 * LU_NB, ipiv, scal);
+      if (g_panel_coop && G <= panel_coop_capacity()) {
+        void* kargs[] = {&lr, &li, (void*)&ldl, &inp, (void*)&k0, (void*)&G, &cand, &top, &ipiv, &scal};
+        FX_HIP(hipLaunchCooperativeKernel((const void*)lu_panel_coop, dim3(G), dim3(256), kargs, 0, st));
+      } else {
+        lu_panel_init<<<G, 256, 0, st>>>(lr, li, ldl, inp, k0, cand, top);
         FX_LAUNCH();
+        for (int j = 0; j < nb; ++j) {
+          const int par = j & 1;
+          lu_panel_step<<<G, 256, 0, st>>>(lr, li, ldl, inp, k0, j, G, cand + par * gmax,
+                                           cand + (par ^ 1) * gmax, top + par * 2 * LU_NB,
+                                           top + (par ^ 1) * 2 * LU_NB, ipiv, scal);
+          FX_LAUNCH();
+        }
       }
       lu_swap_cols(lr, li, ldl, k0, ipiv, K0, k0, st);          // earlier L columns of this outer panel
       lu_swap_cols(lr, li, ldl, k0, ipiv, k0 + nb, K1, st);     // rest of the outer panel
       if (k0 + nb < K1) {
-        lu_trsm_block_row<false><<<(K1 - k0 - nb + 63) / 64, 64, 0, st>>>(lr, li, ldl, k0, k0 + nb, K1, tr, ti,
-                                                                          nb, 0);
-        FX_LAUNCH();
+        lu_block_row_solve<false>(lr, li, ldl, k0, k0 + nb, K1, tr, ti, nb, 0, inv, st);
         flops += lu_update(lr, li, ldl, k0 + nb, inp - k0 - nb, k0, nb, k0 + nb, K1 - k0 - nb, tr, ti, nb, st);
       }
     }
@@ -201,9 +239,7 @@ static double lu_factor_solve(double* lr, double* li, int64_t ldl, int inp, doub
     for (int k0 = K0; k0 < K1; k0 += nb) lu_swap_cols(lr, li, ldl, k0, ipiv, K1, ncol, st);
     // --- block row U12 = L11^{-1} A12, slab by slab; transposed into tr (ld LU_OUTER) ---
     for (int r = K0; r < K1; r += nb) {
-      lu_trsm_block_row<false><<<(ncol - K1 + 63) / 64, 64, 0, st>>>(lr, li, ldl, r, K1, ncol, tr, ti, LU_OUTER,
-                                                                     r - K0);
-      FX_LAUNCH();
+      lu_block_row_solve<false>(lr, li, ldl, r, K1, ncol, tr, ti, LU_OUTER, r - K0, inv, st);
       if (r + nb < K1)
         flops += lu_update(lr, li, ldl, r + nb, K1 - r - nb, r, nb, K1, ncol - K1, tr + (r - K0), ti + (r - K0),
                            LU_OUTER, st);
@@ -217,8 +253,7 @@ static double lu_factor_solve(double* lr, double* li, int64_t ldl, int inp, doub
     const int I0 = ob * LU_OUTER;
     const int nbo = std::min(LU_OUTER, inp - I0);
     for (int r = I0 + nbo - nb; r >= I0; r -= nb) {
-      lu_trsm_block_row<true><<<(inp + 63) / 64, 64, 0, st>>>(lr, li, ldl, r, inp, ncol, tr, ti, LU_OUTER, r - I0);
-      FX_LAUNCH();
+      lu_block_row_solve<true>(lr, li, ldl, r, inp, ncol, tr, ti, LU_OUTER, r - I0, inv, st);
       if (r > I0)
         flops += lu_update(lr, li, ldl, I0, r - I0, r, nb, inp, inp, tr + (r - I0), ti + (r - I0), LU_OUTER, st);
     }
@@ -239,6 +274,7 @@ template <typename W> struct Pipeline : PipelineBase {
   int* ipiv = nullptr;
   PanelCand* cand = nullptr;  // 2 x gmax
   double* top = nullptr;      // 2 x 2*LU_NB
+  double* inv = nullptr;      // 2 x 64 x 64 triangular-block inverse
   int gmax = 0;
   double* colpart = nullptr;
   double* off = nullptr;
@@ -272,6 +308,7 @@ template <typename W> struct Pipeline : PipelineBase {
     gmax = (int)((np + PANEL_R - 1) / PANEL_R);
     cand = dalloc<PanelCand>((size_t)2 * gmax);
     top = dalloc<double>(4 * LU_NB);
+    inv = dalloc<double>(2 * LU_NB * LU_NB);
     const int nt = (n + 31) / 32;
     colpart = dalloc<double>((size_t)std::max(nt, 1) * std::max(n, 1));
     off = dalloc<double>(std::max(n, 1));
@@ -297,6 +334,7 @@ template <typename W> struct Pipeline : PipelineBase {
     dfree(ipiv);
     dfree(cand);
     dfree(top);
+    dfree(inv);
     dfree(colpart);
     dfree(off);
     dfree(diag);
@@ -502,7 +540,7 @@ template <typename W> struct Pipeline : PipelineBase {
       form_system<W><<<grid, 256, 0, st>>>(u.r, u.i, v.r, v.i, ld, n, inp, lr, li, ldl, structured ? 1 : 0);
       FX_LAUNCH();
     }
-    flops += lu_factor_solve(lr, li, ldl, inp, tr, ti, ipiv, cand, gmax, top, scal, st);
+    flops += lu_factor_solve(lr, li, ldl, inp, tr, ti, ipiv, cand, gmax, top, scal, inv, st);
     if (q64 <= 0) {
       dim3 grid((n + 31) / 32, (n + 31) / 32);
       solution_to_work<W><<<grid, 256, 0, st>>>(lr + inp, li + inp, ldl, x0.r, x0.i, ld, n, structured ? 1 : 0,
@@ -1022,6 +1060,9 @@ int fexpm_set_option(fexpm_engine* e, int key, int64_t value) {
       }
       if (e->pipe) e->pipe->kblock = (int)value;
       return FEXPM_OK;
+    case FEXPM_OPT_PANEL_COOP:
+      g_panel_coop = value ? 1 : 0;
+      return FEXPM_OK;
     case FEXPM_OPT_PADE_FP64:
       if (e->pipe) e->pipe->pade_fp64 = value ? 1 : 0;
       return FEXPM_OK;
@@ -1181,7 +1222,8 @@ int fexpm_test_lu_solve(int n, const double* mr, const double* mi, const double*
     FX_HIP(hipMemset(scal, 0, sizeof(Scalars)));
     FX_HIP(hipMemcpy(lr.p, hr.data(), hr.size() * 8, hipMemcpyHostToDevice));
     FX_HIP(hipMemcpy(li.p, hi.data(), hi.size() * 8, hipMemcpyHostToDevice));
-    lu_factor_solve(lr.p, li.p, ldl, np, tr.p, ti.p, ipiv, cand, gmax, top.p, scal, nullptr);
+    DevPlane<double> inv(2 * LU_NB * LU_NB);
+    lu_factor_solve(lr.p, li.p, ldl, np, tr.p, ti.p, ipiv, cand, gmax, top.p, scal, inv.p, nullptr);
     FX_HIP(hipDeviceSynchronize());
     FX_HIP(hipMemcpy(hr.data(), lr.p, hr.size() * 8, hipMemcpyDeviceToHost));
     FX_HIP(hipMemcpy(hi.data(), li.p, hi.size() * 8, hipMemcpyDeviceToHost));

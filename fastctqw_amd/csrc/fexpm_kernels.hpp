@@ -6,6 +6,7 @@
 // planes with a padded leading dimension.  Padding columns are zero and are
 // never written, so GEMM k-loops may run over the padded extent.
 #pragma once
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <math.h>
@@ -692,6 +693,186 @@ __global__ __launch_bounds__(256) void lu_panel_step(double* __restrict__ sr, do
   }
 }
 
+// Whole-panel factorisation in one cooperative launch (G workgroups, all
+// co-resident by the cooperative launch contract).  Workgroup w keeps rows
+// [k0 + 64w, k0 + 64w + 64) of the 64-wide panel in LDS for all 64 columns
+// and exchanges, per column, its argmax candidate (value, row, row values)
+// and -- for the owner of row c -- row c itself through global buffers,
+// separated by one grid barrier.  Buffers alternate by column parity, so a
+// workgroup running one column ahead never overwrites what another still
+// reads.  Exchanged values are stored and loaded with agent-scope atomics,
+// which keeps them coherent across the XCDs' separate L2 caches.
+__device__ __forceinline__ void st_agent(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double ld_agent(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int ld_agent(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ __launch_bounds__(256) void lu_panel_coop(double* __restrict__ sr, double* __restrict__ si, int64_t ld,
+                                                     int np, int k0, int G, PanelCand* cand, double* top,
+                                                     int* __restrict__ ipiv, Scalars* sc) {
+  namespace cg = cooperative_groups;
+  cg::grid_group grid = cg::this_grid();
+  __shared__ double rr[PANEL_R][LU_NB + 1];
+  __shared__ double ri[PANEL_R][LU_NB + 1];
+  __shared__ double pr[LU_NB], pi[LU_NB];
+  __shared__ double sv[256];
+  __shared__ int sk[256];
+  __shared__ int sw[256];
+  const int t = threadIdx.x;
+  const int w = blockIdx.x;
+  const int r0 = k0 + w * PANEL_R;
+  const int hi = min(r0 + PANEL_R, np);
+  const int nrow = max(hi - r0, 0);
+  for (int e = t; e < nrow * LU_NB; e += 256) {
+    const int lr = e / LU_NB, col = e % LU_NB;
+    rr[lr][col] = sr[(int64_t)(r0 + lr) * ld + k0 + col];
+    ri[lr][col] = si[(int64_t)(r0 + lr) * ld + k0 + col];
+  }
+  __syncthreads();
+  for (int j = 0; j < LU_NB; ++j) {
+    const int c = k0 + j;
+    const int par = j & 1;
+    PanelCand* cb = cand + par * G;
+    double* tb = top + par * 2 * LU_NB;
+    // 1. local candidate of column j among rows >= c
+    {
+      double v = -1.0;
+      int k = -1;
+      if (t < nrow) {
+        const int i = r0 + t;
+        if (i >= c) {
+          v = cabs1(rr[t][j], ri[t][j]);
+          k = i;
+          if (!(v >= 0.0)) v = -1.0, k = -1;
+        }
+      }
+      sv[t] = v;
+      sk[t] = k;
+      __syncthreads();
+      for (int s = 128; s > 0; s >>= 1) {
+        if (t < s) {
+          double a = sv[t];
+          int ka = sk[t];
+          argmax_merge(a, ka, sv[t + s], sk[t + s]);
+          sv[t] = a;
+          sk[t] = ka;
+        }
+        __syncthreads();
+      }
+      const int best = sk[0];
+      if (t == 0) {
+        st_agent(&cb[w].val, sv[0]);
+        st_agent(&cb[w].idx, best);
+      }
+      if (best >= 0 && t < LU_NB) {
+        st_agent(&cb[w].re[t], rr[best - r0][t]);
+        st_agent(&cb[w].im[t], ri[best - r0][t]);
+      }
+      if (c >= r0 && c < hi && t < LU_NB) {
+        st_agent(&tb[t], rr[c - r0][t]);
+        st_agent(&tb[LU_NB + t], ri[c - r0][t]);
+      }
+    }
+    __threadfence();
+    grid.sync();
+    // 2. global pivot
+    {
+      double v = -1.0;
+      int k = -1, q0 = -1;
+      for (int q = t; q < G; q += 256) {
+        const double v2 = ld_agent(&cb[q].val);
+        const int k2 = ld_agent(&cb[q].idx);
+        if (k2 >= 0 && (v2 > v || (v2 == v && (k < 0 || k2 < k)))) {
+          v = v2;
+          k = k2;
+          q0 = q;
+        }
+      }
+      sv[t] = v;
+      sk[t] = k;
+      sw[t] = q0;
+      __syncthreads();
+      for (int s = 128; s > 0; s >>= 1) {
+        if (t < s) {
+          const double v2 = sv[t + s];
+          const int k2 = sk[t + s];
+          if (k2 >= 0 && (v2 > sv[t] || (v2 == sv[t] && (sk[t] < 0 || k2 < sk[t])))) {
+            sv[t] = v2;
+            sk[t] = k2;
+            sw[t] = sw[t + s];
+          }
+        }
+        __syncthreads();
+      }
+    }
+    const int p = sk[0] >= 0 ? sk[0] : c;
+    const double pval = sv[0];
+    const int win = sw[0];
+    if (t < LU_NB) {
+      if (win >= 0) {
+        pr[t] = ld_agent(&cb[win].re[t]);
+        pi[t] = ld_agent(&cb[win].im[t]);
+      } else {
+        pr[t] = ld_agent(&tb[t]);
+        pi[t] = ld_agent(&tb[LU_NB + t]);
+      }
+    }
+    if (w == 0 && t == 0) {
+      ipiv[c] = p;
+      if (!(pval > 0.0)) atomicOr(&sc->info, 1);
+    }
+    __syncthreads();
+    // 3. interchange rows c and p inside LDS (each side by its owner)
+    if (p != c && p >= r0 && p < hi && t < LU_NB) {
+      rr[p - r0][t] = ld_agent(&tb[t]);
+      ri[p - r0][t] = ld_agent(&tb[LU_NB + t]);
+    }
+    if (c >= r0 && c < hi && t < LU_NB) {
+      rr[c - r0][t] = pr[t];
+      ri[c - r0][t] = pi[t];
+    }
+    __syncthreads();
+    // 4. scale and rank-1 update of my rows below c
+    const double dr = pr[j], di = pi[j];
+    const double den = dr * dr + di * di;
+    const bool ok = den > 0.0;
+    const double qr = ok ? dr / den : 0.0, qi = ok ? -di / den : 0.0;
+    const int lr = t >> 2, cg4 = t & 3;
+    const int i = r0 + lr;
+    const bool upd = lr < nrow && i > c;
+    if (upd) {
+      const double ar = rr[lr][j], ai = ri[lr][j];
+      const double lre = ar * qr - ai * qi, lim = ar * qi + ai * qr;
+      for (int col = cg4 * 16; col < cg4 * 16 + 16; ++col) {
+        if (col > j) {
+          rr[lr][col] -= lre * pr[col] - lim * pi[col];
+          ri[lr][col] -= lre * pi[col] + lim * pr[col];
+        }
+      }
+    }
+    __syncthreads();
+    if (upd && cg4 == 0) {
+      const double ar = rr[lr][j], ai = ri[lr][j];
+      rr[lr][j] = ar * qr - ai * qi;
+      ri[lr][j] = ar * qi + ai * qr;
+    }
+    __syncthreads();
+  }
+  for (int e = t; e < nrow * LU_NB; e += 256) {
+    const int lr = e / LU_NB, col = e % LU_NB;
+    sr[(int64_t)(r0 + lr) * ld + k0 + col] = rr[lr][col];
+    si[(int64_t)(r0 + lr) * ld + k0 + col] = ri[lr][col];
+  }
+}
+
 // Apply the panel's row interchanges to columns [c0, c1).
 __global__ void lu_laswp(double* __restrict__ sr, double* __restrict__ si, int64_t ld, int k0, int nb,
                          const int* __restrict__ ipiv, int c0, int c1) {
@@ -778,6 +959,128 @@ __global__ __launch_bounds__(64) void lu_trsm_block_row(double* __restrict__ sr,
       si[(int64_t)(k0 + r) * ld + col] = xi[r][t];
       tr[(int64_t)(col - c0) * ldt + toff + r] = xr[r][t];
       ti[(int64_t)(col - c0) * ldt + toff + r] = xi[r][t];
+    }
+  }
+}
+
+// Inverse of the 64x64 diagonal block at (k0, k0): unit lower (L) or
+// non-unit upper (U) triangle, one workgroup of 64 threads, thread t
+// solving for column t of the inverse.  Output: inv (64 x 64 complex,
+// planar, row-major, ld 64).
+template <bool UPPER>
+__global__ __launch_bounds__(64) void lu_invert_diag_block(const double* __restrict__ sr,
+                                                           const double* __restrict__ si, int64_t ld, int k0,
+                                                           double* __restrict__ invr, double* __restrict__ invi) {
+  __shared__ double lr[LU_NB][LU_NB + 1];
+  __shared__ double li[LU_NB][LU_NB + 1];
+  __shared__ double xr[LU_NB][LU_NB + 1];
+  __shared__ double xi[LU_NB][LU_NB + 1];
+  const int t = threadIdx.x;
+  for (int e = t; e < LU_NB * LU_NB; e += 64) {
+    const int r = e / LU_NB, q = e % LU_NB;
+    lr[r][q] = sr[(int64_t)(k0 + r) * ld + k0 + q];
+    li[r][q] = si[(int64_t)(k0 + r) * ld + k0 + q];
+  }
+  for (int r = 0; r < LU_NB; ++r) {
+    xr[r][t] = (r == t) ? 1.0 : 0.0;
+    xi[r][t] = 0.0;
+  }
+  __syncthreads();
+  if (UPPER) {
+    for (int r = LU_NB - 1; r >= 0; --r) {
+      double ar = xr[r][t], ai = xi[r][t];
+      for (int q = r + 1; q < LU_NB; ++q) {
+        const double ur = lr[r][q], ui = li[r][q], br = xr[q][t], bi = xi[q][t];
+        ar -= ur * br - ui * bi;
+        ai -= ur * bi + ui * br;
+      }
+      const double dr = lr[r][r], di = li[r][r];
+      const double den = dr * dr + di * di;
+      const double qr = dr / den, qi = -di / den;
+      xr[r][t] = ar * qr - ai * qi;
+      xi[r][t] = ar * qi + ai * qr;
+    }
+  } else {
+    for (int r = 1; r < LU_NB; ++r) {
+      double ar = xr[r][t], ai = xi[r][t];
+      for (int q = 0; q < r; ++q) {
+        const double ur = lr[r][q], ui = li[r][q], br = xr[q][t], bi = xi[q][t];
+        ar -= ur * br - ui * bi;
+        ai -= ur * bi + ui * br;
+      }
+      xr[r][t] = ar;
+      xi[r][t] = ai;
+    }
+  }
+  for (int r = 0; r < LU_NB; ++r) {
+    invr[r * LU_NB + t] = xr[r][t];
+    invi[r * LU_NB + t] = xi[r][t];
+  }
+}
+
+// Block-row solve by an explicit triangular inverse: for the 64 rows at k0
+// and columns [c0, c1), S[k0:k0+64, col] <- inv * S[k0:k0+64, col], and the
+// result transposed to tr[(col - c0) * ldt + toff + r].  256 threads per
+// workgroup handle 64 columns: thread (c = t % 64, g = t / 64) forms rows
+// 16g..16g+15 of column c; the transposed copy is written from LDS with
+// consecutive lanes on consecutive r.
+__global__ __launch_bounds__(256) void lu_apply_inverse(double* __restrict__ sr, double* __restrict__ si, int64_t ld,
+                                                        int k0, int c0, int c1, const double* __restrict__ invr,
+                                                        const double* __restrict__ invi, double* __restrict__ tr,
+                                                        double* __restrict__ ti, int ldt, int toff) {
+  __shared__ double ir[LU_NB][LU_NB + 1];
+  __shared__ double ii[LU_NB][LU_NB + 1];
+  __shared__ double br[LU_NB][64 + 1];
+  __shared__ double bi[LU_NB][64 + 1];
+  const int t = threadIdx.x;
+  const int cb = c0 + blockIdx.x * 64;
+  for (int e = t; e < LU_NB * LU_NB; e += 256) {
+    const int r = e / LU_NB, q = e % LU_NB;
+    ir[r][q] = invr[e];
+    ii[r][q] = invi[e];
+  }
+  for (int e = t; e < LU_NB * 64; e += 256) {
+    const int r = e / 64, c = e % 64;
+    const int col = cb + c;
+    const bool ok = col < c1;
+    br[r][c] = ok ? sr[(int64_t)(k0 + r) * ld + col] : 0.0;
+    bi[r][c] = ok ? si[(int64_t)(k0 + r) * ld + col] : 0.0;
+  }
+  __syncthreads();
+  const int c = t & 63, g = t >> 6;
+  double ar[16], ai[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) ar[k] = ai[k] = 0.0;
+  for (int q = 0; q < LU_NB; ++q) {
+    const double xr = br[q][c], xi = bi[q][c];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const double vr = ir[16 * g + k][q], vi = ii[16 * g + k][q];
+      ar[k] += vr * xr - vi * xi;
+      ai[k] += vr * xi + vi * xr;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    br[16 * g + k][c] = ar[k];
+    bi[16 * g + k][c] = ai[k];
+  }
+  __syncthreads();
+  for (int e = t; e < LU_NB * 64; e += 256) {
+    const int r = e / 64, cc = e % 64;
+    const int col = cb + cc;
+    if (col < c1) {
+      sr[(int64_t)(k0 + r) * ld + col] = br[r][cc];
+      si[(int64_t)(k0 + r) * ld + col] = bi[r][cc];
+    }
+  }
+  for (int e = t; e < LU_NB * 64; e += 256) {
+    const int cc = e / 64, r = e % 64;
+    const int col = cb + cc;
+    if (col < c1) {
+      tr[(int64_t)(col - c0) * ldt + toff + r] = br[r][cc];
+      ti[(int64_t)(col - c0) * ldt + toff + r] = bi[r][cc];
     }
   }
 }
