@@ -194,7 +194,7 @@ template <bool UPPER>
 static void lu_block_row_solve(double* lr, double* li, int64_t ldl, int k0, int c0, int c1, double* tr, double* ti,
                                int ldt, int toff, double* inv, hipStream_t st) {
   if (c1 <= c0) return;
-  lu_invert_diag_block<UPPER><<<1, 64, 0, st>>>(lr, li, ldl, k0, inv, inv + LU_NB * LU_NB);
+  lu_invert_diag_block<UPPER><<<1, 256, 0, st>>>(lr, li, ldl, k0, inv, inv + LU_NB * LU_NB);
   FX_LAUNCH();
   lu_apply_inverse<<<(c1 - c0 + 63) / 64, 256, 0, st>>>(lr, li, ldl, k0, c0, c1, inv, inv + LU_NB * LU_NB, tr, ti,
                                                         ldt, toff);
@@ -203,8 +203,10 @@ static void lu_block_row_solve(double* lr, double* li, int64_t ldl, int k0, int 
 
 static double lu_factor_solve(double* lr, double* li, int64_t ldl, int inp, double* tr, double* ti, int* ipiv,
                               PanelCand* cand, int gmax, double* top, Scalars* scal, double* inv,
-                              hipStream_t st) {
+                              unsigned* ctrs, hipStream_t st) {
   double flops = 0.0;
+  // one zeroed arrival counter per panel for the cooperative panel kernel
+  FX_HIP(hipMemsetAsync(ctrs, 0, (size_t)(inp / LU_NB) * sizeof(unsigned), st));
   const int nb = LU_NB;
   const int ncol = 2 * inp;
   for (int K0 = 0; K0 < inp; K0 += LU_OUTER) {
@@ -214,7 +216,8 @@ static double lu_factor_solve(double* lr, double* li, int64_t ldl, int inp, doub
     for (int k0 = K0; k0 < K1; k0 += nb) {
       const int G = (inp - k0 + PANEL_R - 1) / PANEL_R;
       if (g_panel_coop && G <= panel_coop_capacity()) {
-        void* kargs[] = {&lr, &li, (void*)&ldl, &inp, (void*)&k0, (void*)&G, &cand, &top, &ipiv, &scal};
+        unsigned* ctr = ctrs + k0 / LU_NB;
+        void* kargs[] = {&lr, &li, (void*)&ldl, &inp, (void*)&k0, (void*)&G, &cand, &top, &ipiv, &scal, &ctr};
         FX_HIP(hipLaunchCooperativeKernel((const void*)lu_panel_coop, dim3(G), dim3(256), kargs, 0, st));
       } else {
         lu_panel_init<<<G, 256, 0, st>>>(lr, li, ldl, inp, k0, cand, top);
@@ -275,6 +278,7 @@ template <typename W> struct Pipeline : PipelineBase {
   PanelCand* cand = nullptr;  // 2 x gmax
   double* top = nullptr;      // 2 x 2*LU_NB
   double* inv = nullptr;      // 2 x 64 x 64 triangular-block inverse
+  unsigned* ctrs = nullptr;   // grid-barrier counters, one per panel
   int gmax = 0;
   double* colpart = nullptr;
   double* off = nullptr;
@@ -309,6 +313,7 @@ template <typename W> struct Pipeline : PipelineBase {
     cand = dalloc<PanelCand>((size_t)2 * gmax);
     top = dalloc<double>(4 * LU_NB);
     inv = dalloc<double>(2 * LU_NB * LU_NB);
+    ctrs = dalloc<unsigned>(np / LU_NB);
     const int nt = (n + 31) / 32;
     colpart = dalloc<double>((size_t)std::max(nt, 1) * std::max(n, 1));
     off = dalloc<double>(std::max(n, 1));
@@ -335,6 +340,7 @@ template <typename W> struct Pipeline : PipelineBase {
     dfree(cand);
     dfree(top);
     dfree(inv);
+    dfree(ctrs);
     dfree(colpart);
     dfree(off);
     dfree(diag);
@@ -540,7 +546,7 @@ template <typename W> struct Pipeline : PipelineBase {
       form_system<W><<<grid, 256, 0, st>>>(u.r, u.i, v.r, v.i, ld, n, inp, lr, li, ldl, structured ? 1 : 0);
       FX_LAUNCH();
     }
-    flops += lu_factor_solve(lr, li, ldl, inp, tr, ti, ipiv, cand, gmax, top, scal, inv, st);
+    flops += lu_factor_solve(lr, li, ldl, inp, tr, ti, ipiv, cand, gmax, top, scal, inv, ctrs, st);
     if (q64 <= 0) {
       dim3 grid((n + 31) / 32, (n + 31) / 32);
       solution_to_work<W><<<grid, 256, 0, st>>>(lr + inp, li + inp, ldl, x0.r, x0.i, ld, n, structured ? 1 : 0,
@@ -759,6 +765,7 @@ template <typename W> struct Pipeline : PipelineBase {
     S.s_fp64 = q64;
     S.pade_fp64 = hybrid ? 1 : 0;
     S.kblock = kblock;
+    if (hs2.info & 2) throw Error(FEXPM_EHIP, "LU panel grid barrier timed out");
     if (hs2.info != 0) throw Error(FEXPM_ESINGULAR, "Pade denominator is exactly singular");
   }
 
@@ -1223,7 +1230,11 @@ int fexpm_test_lu_solve(int n, const double* mr, const double* mi, const double*
     FX_HIP(hipMemcpy(lr.p, hr.data(), hr.size() * 8, hipMemcpyHostToDevice));
     FX_HIP(hipMemcpy(li.p, hi.data(), hi.size() * 8, hipMemcpyHostToDevice));
     DevPlane<double> inv(2 * LU_NB * LU_NB);
-    lu_factor_solve(lr.p, li.p, ldl, np, tr.p, ti.p, ipiv, cand, gmax, top.p, scal, inv.p, nullptr);
+    unsigned* ctrs = nullptr;
+    FX_HIP(hipMalloc(&ctrs, (np / LU_NB) * sizeof(unsigned)));
+    lu_factor_solve(lr.p, li.p, ldl, np, tr.p, ti.p, ipiv, cand, gmax, top.p, scal, inv.p, ctrs, nullptr);
+    FX_HIP(hipDeviceSynchronize());
+    (void)hipFree(ctrs);
     FX_HIP(hipDeviceSynchronize());
     FX_HIP(hipMemcpy(hr.data(), lr.p, hr.size() * 8, hipMemcpyDeviceToHost));
     FX_HIP(hipMemcpy(hi.data(), li.p, hi.size() * 8, hipMemcpyDeviceToHost));

@@ -694,13 +694,13 @@ __global__ __launch_bounds__(256) void lu_panel_step(double* __restrict__ sr, do
 }
 
 // Whole-panel factorisation in one cooperative launch (G workgroups, all
-// co-resident by the cooperative launch contract).  Workgroup w keeps rows
+// co-resident by the cooperative launch contract; ctr is a zeroed counter).  Workgroup w keeps rows
 // [k0 + 64w, k0 + 64w + 64) of the 64-wide panel in LDS for all 64 columns
 // and exchanges, per column, its argmax candidate (value, row, row values)
 // and -- for the owner of row c -- row c itself through global buffers,
 // separated by one grid barrier.  Buffers alternate by column parity, so a
 // workgroup running one column ahead never overwrites what another still
-// reads.  Exchanged values are stored and loaded with agent-scope atomics,
+// reads (a workgroup can be at most one barrier ahead of any other).  Exchanged values are stored and loaded with agent-scope atomics,
 // which keeps them coherent across the XCDs' separate L2 caches.
 __device__ __forceinline__ void st_agent(double* p, double v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -715,11 +715,31 @@ __device__ __forceinline__ int ld_agent(const int* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Grid barrier over G co-resident workgroups: one arrival counter that is
+// never reset within a launch (barrier b waits for G * (b + 1) arrivals).
+// The wait is bounded: after about 2^24 polls it records an error in
+// sc->info (bit 1) and returns, so a launch can never hang.
+__device__ __forceinline__ void grid_barrier(unsigned* ctr, unsigned target, Scalars* sc) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned polls = 0;
+    while (__hip_atomic_load(ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      __builtin_amdgcn_s_sleep(1);
+      if (++polls > (1u << 24)) {
+        atomicOr(&sc->info, 2);
+        break;
+      }
+    }
+    __threadfence();
+  }
+  __syncthreads();
+}
+
 __global__ __launch_bounds__(256) void lu_panel_coop(double* __restrict__ sr, double* __restrict__ si, int64_t ld,
                                                      int np, int k0, int G, PanelCand* cand, double* top,
-                                                     int* __restrict__ ipiv, Scalars* sc) {
-  namespace cg = cooperative_groups;
-  cg::grid_group grid = cg::this_grid();
+                                                     int* __restrict__ ipiv, Scalars* sc, unsigned* ctr) {
   __shared__ double rr[PANEL_R][LU_NB + 1];
   __shared__ double ri[PANEL_R][LU_NB + 1];
   __shared__ double pr[LU_NB], pi[LU_NB];
@@ -781,8 +801,7 @@ __global__ __launch_bounds__(256) void lu_panel_coop(double* __restrict__ sr, do
         st_agent(&tb[LU_NB + t], ri[c - r0][t]);
       }
     }
-    __threadfence();
-    grid.sync();
+    grid_barrier(ctr, (unsigned)G * (unsigned)(j + 1), sc);
     // 2. global pivot
     {
       double v = -1.0;
@@ -964,57 +983,67 @@ __global__ __launch_bounds__(64) void lu_trsm_block_row(double* __restrict__ sr,
 }
 
 // Inverse of the 64x64 diagonal block at (k0, k0): unit lower (L) or
-// non-unit upper (U) triangle, one workgroup of 64 threads, thread t
-// solving for column t of the inverse.  Output: inv (64 x 64 complex,
-// planar, row-major, ld 64).
+// non-unit upper (U) triangle.  Right-looking elimination on X = I with 256
+// threads: step q finalises row q of X and updates the remaining rows in
+// parallel (64 synchronised steps).  Output: inv (64 x 64 complex, planar,
+// row-major, ld 64).
 template <bool UPPER>
-__global__ __launch_bounds__(64) void lu_invert_diag_block(const double* __restrict__ sr,
-                                                           const double* __restrict__ si, int64_t ld, int k0,
-                                                           double* __restrict__ invr, double* __restrict__ invi) {
+__global__ __launch_bounds__(256) void lu_invert_diag_block(const double* __restrict__ sr,
+                                                            const double* __restrict__ si, int64_t ld, int k0,
+                                                            double* __restrict__ invr, double* __restrict__ invi) {
   __shared__ double lr[LU_NB][LU_NB + 1];
   __shared__ double li[LU_NB][LU_NB + 1];
   __shared__ double xr[LU_NB][LU_NB + 1];
   __shared__ double xi[LU_NB][LU_NB + 1];
   const int t = threadIdx.x;
-  for (int e = t; e < LU_NB * LU_NB; e += 64) {
+  for (int e = t; e < LU_NB * LU_NB; e += 256) {
     const int r = e / LU_NB, q = e % LU_NB;
     lr[r][q] = sr[(int64_t)(k0 + r) * ld + k0 + q];
     li[r][q] = si[(int64_t)(k0 + r) * ld + k0 + q];
-  }
-  for (int r = 0; r < LU_NB; ++r) {
-    xr[r][t] = (r == t) ? 1.0 : 0.0;
-    xi[r][t] = 0.0;
+    xr[r][q] = (r == q) ? 1.0 : 0.0;
+    xi[r][q] = 0.0;
   }
   __syncthreads();
   if (UPPER) {
-    for (int r = LU_NB - 1; r >= 0; --r) {
-      double ar = xr[r][t], ai = xi[r][t];
-      for (int q = r + 1; q < LU_NB; ++q) {
-        const double ur = lr[r][q], ui = li[r][q], br = xr[q][t], bi = xi[q][t];
-        ar -= ur * br - ui * bi;
-        ai -= ur * bi + ui * br;
+    for (int q = LU_NB - 1; q >= 0; --q) {
+      // row q of X: divide by U[q][q] (its columns c >= q are the nonzero ones)
+      if (t < LU_NB && t >= q) {
+        const double dr = lr[q][q], di = li[q][q];
+        const double den = dr * dr + di * di;
+        const double qr = dr / den, qi = -di / den;
+        const double ar = xr[q][t], ai = xi[q][t];
+        xr[q][t] = ar * qr - ai * qi;
+        xi[q][t] = ar * qi + ai * qr;
       }
-      const double dr = lr[r][r], di = li[r][r];
-      const double den = dr * dr + di * di;
-      const double qr = dr / den, qi = -di / den;
-      xr[r][t] = ar * qr - ai * qi;
-      xi[r][t] = ar * qi + ai * qr;
+      __syncthreads();
+      // rows r < q: X[r][c] -= U[r][q] X[q][c] for c >= q
+      for (int e = t; e < q * LU_NB; e += 256) {
+        const int r = e / LU_NB, c = e % LU_NB;
+        if (c >= q) {
+          const double ur = lr[r][q], ui = li[r][q], br = xr[q][c], bi = xi[q][c];
+          xr[r][c] -= ur * br - ui * bi;
+          xi[r][c] -= ur * bi + ui * br;
+        }
+      }
+      __syncthreads();
     }
   } else {
-    for (int r = 1; r < LU_NB; ++r) {
-      double ar = xr[r][t], ai = xi[r][t];
-      for (int q = 0; q < r; ++q) {
-        const double ur = lr[r][q], ui = li[r][q], br = xr[q][t], bi = xi[q][t];
-        ar -= ur * br - ui * bi;
-        ai -= ur * bi + ui * br;
+    for (int q = 0; q < LU_NB - 1; ++q) {
+      // rows r > q: X[r][c] -= L[r][q] X[q][c] for c <= q
+      for (int e = t; e < (LU_NB - 1 - q) * LU_NB; e += 256) {
+        const int r = q + 1 + e / LU_NB, c = e % LU_NB;
+        if (c <= q) {
+          const double ur = lr[r][q], ui = li[r][q], br = xr[q][c], bi = xi[q][c];
+          xr[r][c] -= ur * br - ui * bi;
+          xi[r][c] -= ur * bi + ui * br;
+        }
       }
-      xr[r][t] = ar;
-      xi[r][t] = ai;
+      __syncthreads();
     }
   }
-  for (int r = 0; r < LU_NB; ++r) {
-    invr[r * LU_NB + t] = xr[r][t];
-    invi[r * LU_NB + t] = xi[r][t];
+  for (int e = t; e < LU_NB * LU_NB; e += 256) {
+    invr[e] = xr[e / LU_NB][e % LU_NB];
+    invi[e] = xi[e / LU_NB][e % LU_NB];
   }
 }
 
