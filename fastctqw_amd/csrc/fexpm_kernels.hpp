@@ -762,8 +762,8 @@ __global__ __launch_bounds__(256) void lu_panel_coop(double* __restrict__ sr, do
     const int par = j & 1;
     PanelCand* cb = cand + par * G;
     double* tb = top + par * 2 * LU_NB;
-    // 1. local candidate of column j among rows >= c
-    {
+    // 1. local candidate of column j among rows >= c (wave 0, shuffles)
+    if (t < 64) {
       double v = -1.0;
       int k = -1;
       if (t < nrow) {
@@ -774,39 +774,31 @@ __global__ __launch_bounds__(256) void lu_panel_coop(double* __restrict__ sr, do
           if (!(v >= 0.0)) v = -1.0, k = -1;
         }
       }
-      sv[t] = v;
-      sk[t] = k;
-      __syncthreads();
-      for (int s = 128; s > 0; s >>= 1) {
-        if (t < s) {
-          double a = sv[t];
-          int ka = sk[t];
-          argmax_merge(a, ka, sv[t + s], sk[t + s]);
-          sv[t] = a;
-          sk[t] = ka;
-        }
-        __syncthreads();
+      for (int off = 32; off > 0; off >>= 1) {
+        const double v2 = __shfl_xor(v, off);
+        const int k2 = __shfl_xor(k, off);
+        argmax_merge(v, k, v2, k2);
       }
-      const int best = sk[0];
+      const int best = k;  // identical in every lane after the butterfly
       if (t == 0) {
-        st_agent(&cb[w].val, sv[0]);
+        st_agent(&cb[w].val, v);
         st_agent(&cb[w].idx, best);
       }
-      if (best >= 0 && t < LU_NB) {
+      if (best >= 0) {
         st_agent(&cb[w].re[t], rr[best - r0][t]);
         st_agent(&cb[w].im[t], ri[best - r0][t]);
       }
-      if (c >= r0 && c < hi && t < LU_NB) {
+      if (c >= r0 && c < hi) {
         st_agent(&tb[t], rr[c - r0][t]);
         st_agent(&tb[LU_NB + t], ri[c - r0][t]);
       }
     }
     grid_barrier(ctr, (unsigned)G * (unsigned)(j + 1), sc);
-    // 2. global pivot
-    {
+    // 2. global pivot (wave 0, shuffles; ties -> lower row, then lower workgroup)
+    if (t < 64) {
       double v = -1.0;
       int k = -1, q0 = -1;
-      for (int q = t; q < G; q += 256) {
+      for (int q = t; q < G; q += 64) {
         const double v2 = ld_agent(&cb[q].val);
         const int k2 = ld_agent(&cb[q].idx);
         if (k2 >= 0 && (v2 > v || (v2 == v && (k < 0 || k2 < k)))) {
@@ -815,23 +807,23 @@ __global__ __launch_bounds__(256) void lu_panel_coop(double* __restrict__ sr, do
           q0 = q;
         }
       }
-      sv[t] = v;
-      sk[t] = k;
-      sw[t] = q0;
-      __syncthreads();
-      for (int s = 128; s > 0; s >>= 1) {
-        if (t < s) {
-          const double v2 = sv[t + s];
-          const int k2 = sk[t + s];
-          if (k2 >= 0 && (v2 > sv[t] || (v2 == sv[t] && (sk[t] < 0 || k2 < sk[t])))) {
-            sv[t] = v2;
-            sk[t] = k2;
-            sw[t] = sw[t + s];
-          }
+      for (int off = 32; off > 0; off >>= 1) {
+        const double v2 = __shfl_xor(v, off);
+        const int k2 = __shfl_xor(k, off);
+        const int q2 = __shfl_xor(q0, off);
+        if (k2 >= 0 && (v2 > v || (v2 == v && (k < 0 || k2 < k)))) {
+          v = v2;
+          k = k2;
+          q0 = q2;
         }
-        __syncthreads();
+      }
+      if (t == 0) {
+        sv[0] = v;
+        sk[0] = k;
+        sw[0] = q0;
       }
     }
+    __syncthreads();
     const int p = sk[0] >= 0 ? sk[0] : c;
     const double pval = sv[0];
     const int win = sw[0];
