@@ -102,9 +102,11 @@ struct PipelineBase {
   virtual ~PipelineBase() {}
   virtual void run(const void* d_a, void* d_u, hipStream_t st, fexpm_stats& stats, int opt_structure) = 0;
   virtual void apply(const double2* d_psi0, int nstates, double2* d_out, hipStream_t st) = 0;
+  virtual void propagate(const double2* d_psi0, int nstates, int nsteps, double2* d_out, hipStream_t st) = 0;
   virtual bool has_result() const = 0;
   virtual void debug_plane(int which, double* re, double* im) = 0;
   int max_squarings = -1;
+  double last_prop_ms = 0.0;  // device time of the last propagate()
   int kblock = 8;          // FP32 squarings: k-tiles (of 16) per accumulation block, 0 = off
   int fp64_squarings = 0;  // FP32 engines: leading squarings done in FP64
   int pade_fp64 = 1;       // FP32 engines: Pade + solve in FP64 (companion FP64 pipeline)
@@ -346,6 +348,11 @@ template <typename W> struct Pipeline : PipelineBase {
     dfree(diag);
     dfree(dsq[0]);
     dfree(dsq[1]);
+    for (auto& P : ps) {
+      dfree(P.r);
+      dfree(P.i);
+    }
+    dfree(norm0);
     dfree(flags);
     dfree(scal);
     if (events)
@@ -742,6 +749,7 @@ template <typename W> struct Pipeline : PipelineBase {
     FX_HIP(hipMemcpy(&hs2, hybrid ? companion_scal() : scal, sizeof(Scalars), hipMemcpyDeviceToHost));
     res = *X;
     have_result = true;
+    last_structured = structured;
     float t01, t12, t23, t34, t45;
     FX_HIP(hipEventElapsedTime(&t01, ev[0], ev[1]));
     FX_HIP(hipEventElapsedTime(&t12, ev[1], ev[2]));
@@ -797,6 +805,61 @@ template <typename W> struct Pipeline : PipelineBase {
     for (size_t k = 0; k < buf.size(); ++k) re[k] = (double)buf[k];
     FX_HIP(hipMemcpy2D(buf.data(), n * sizeof(W), P.i, ld * sizeof(W), n * sizeof(W), n, hipMemcpyDeviceToHost));
     for (size_t k = 0; k < buf.size(); ++k) im[k] = (double)buf[k];
+  }
+
+  // psi_k = U^k psi0 for k = 1..nsteps and all states at once: Psi (states
+  // as rows) <- Psi X^T by the NT GEMM with B = rows of X (no transpose),
+  // then refine_states applies the FP64 norm refinement (unitary CTQW
+  // propagators) and the phase e^{k mu}.  d_out: nsteps x nstates x n.
+  Plane2<W> ps[2];
+  double* norm0 = nullptr;
+  int ps_rows = 0;
+  bool last_structured = false;
+
+  void propagate(const double2* d_psi0, int nstates, int nsteps, double2* d_out, hipStream_t st) override {
+    if (nstates > ps_rows) {
+      for (auto& P : ps) {
+        dfree(P.r);
+        dfree(P.i);
+        P.r = P.i = nullptr;
+      }
+      dfree(norm0);
+      norm0 = nullptr;
+      ps_rows = (int)round_up(nstates, 128);
+      for (auto& P : ps) {
+        P.r = dalloc<W>((size_t)ps_rows * ld);
+        P.i = dalloc<W>((size_t)ps_rows * ld);
+      }
+      norm0 = dalloc<double>(ps_rows);
+    }
+    FX_HIP(hipEventRecord(ev[6], st));
+    states_to_planes<W><<<nstates, 256, 0, st>>>(d_psi0, n, ps[0].r, ps[0].i, ld, norm0);
+    FX_LAUNCH();
+    int cur = 0;
+    double mre = std::log(std::max(std::hypot(ph_re, ph_im), 1e-300)), mim = std::atan2(ph_im, ph_re);
+    for (int k = 1; k <= nsteps; ++k) {
+      GemmArgs<W> g = args(ps[cur], res, ld, ld, nstates, n, (int)np);
+      g.epi.cr = ps[cur ^ 1].r;
+      g.epi.ci = ps[cur ^ 1].i;
+      g.epi.ldc = ld;
+      g.epi.ld_add = ld;
+      g.kblock = kblock > 0 ? kblock : 1 << 20;
+      if (std::is_same<W, float>::value)
+        gemm<MODE_C3M, false, true>(g, st);
+      else
+        gemm<MODE_C3M, false>(g, st);
+      cur ^= 1;
+      const double er = std::exp(k * mre);
+      refine_states<W><<<nstates, 256, 0, st>>>(ps[cur].r, ps[cur].i, ld, n, norm0, last_structured ? 1 : 0,
+                                                er * std::cos(k * mim), er * std::sin(k * mim),
+                                                d_out + (int64_t)(k - 1) * nstates * n);
+      FX_LAUNCH();
+    }
+    FX_HIP(hipEventRecord(ev[7], st));
+    FX_HIP(hipEventSynchronize(ev[7]));
+    float ms = 0.f;
+    FX_HIP(hipEventElapsedTime(&ms, ev[6], ev[7]));
+    last_prop_ms = ms;
   }
 
   void apply(const double2* d_psi0, int nstates, double2* d_out, hipStream_t st) override {
@@ -1043,6 +1106,41 @@ int fexpm_apply(fexpm_engine* e, const void* psi0, int64_t nstates, void* psi) {
   });
 }
 
+int fexpm_propagate(fexpm_engine* e, const void* psi0, int64_t nstates, int64_t nsteps, void* psi) {
+  if (!e || nstates < 0 || nsteps < 0 || (nstates > 0 && nsteps > 0 && (!psi0 || !psi))) return FEXPM_EINVAL;
+  std::lock_guard<std::mutex> lk(e->mtx);
+  if (e->freed) return FEXPM_EFREED;
+  if (e->n == 0 || nstates == 0 || nsteps == 0) return FEXPM_OK;
+  if (!e->pipe->has_result()) {
+    e->err = "no propagator: run first";
+    return FEXPM_EINVAL;
+  }
+  return guarded(e, [&] {
+    const size_t in_bytes = (size_t)nstates * e->n * sizeof(double2);
+    const size_t out_bytes = in_bytes * (size_t)nsteps;
+    void* din = nullptr;
+    void* dout = nullptr;
+    FX_HIP(hipMalloc(&din, in_bytes));
+    hipError_t er = hipMalloc(&dout, out_bytes);
+    if (er != hipSuccess) {
+      (void)hipFree(din);
+      FX_HIP(er);
+    }
+    try {
+      FX_HIP(hipMemcpyAsync(din, psi0, in_bytes, hipMemcpyHostToDevice, e->stream));
+      e->pipe->propagate((const double2*)din, (int)nstates, (int)nsteps, (double2*)dout, e->stream);
+      FX_HIP(hipMemcpyAsync(psi, dout, out_bytes, hipMemcpyDeviceToHost, e->stream));
+      FX_HIP(hipStreamSynchronize(e->stream));
+    } catch (...) {
+      (void)hipFree(din);
+      (void)hipFree(dout);
+      throw;
+    }
+    (void)hipFree(din);
+    (void)hipFree(dout);
+  });
+}
+
 int fexpm_set_option(fexpm_engine* e, int key, int64_t value) {
   if (!e) return FEXPM_EINVAL;
   std::lock_guard<std::mutex> lk(e->mtx);
@@ -1088,6 +1186,7 @@ int fexpm_set_option(fexpm_engine* e, int key, int64_t value) {
 int fexpm_get_stats(const fexpm_engine* e, fexpm_stats* out) {
   if (!e || !out) return FEXPM_EINVAL;
   *out = e->stats;
+  out->ms_propagate = e->pipe ? e->pipe->last_prop_ms : 0.0;
   return FEXPM_OK;
 }
 

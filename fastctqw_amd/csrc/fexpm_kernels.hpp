@@ -1107,6 +1107,64 @@ __global__ __launch_bounds__(256) void lu_apply_inverse(double* __restrict__ sr,
 }
 
 // ---------------------------------------------------------------------------
+// Propagation of many states: Psi (states as rows, planar working
+// precision) <-> complex128 host layout, with an FP64 refinement per step.
+
+// Psi planes <- psi0 (nstates x n complex128); norm0[s] = ||psi0_s||_2.
+template <typename W>
+__global__ __launch_bounds__(256) void states_to_planes(const double2* __restrict__ psi0, int n, W* __restrict__ pr,
+                                                        W* __restrict__ pi, int64_t ld, double* __restrict__ norm0) {
+  __shared__ double red[256];
+  const int s = blockIdx.x, t = threadIdx.x;
+  double acc = 0.0;
+  for (int j = t; j < n; j += 256) {
+    const double2 v = psi0[(int64_t)s * n + j];
+    pr[(int64_t)s * ld + j] = (W)v.x;
+    pi[(int64_t)s * ld + j] = (W)v.y;
+    acc += v.x * v.x + v.y * v.y;
+  }
+  red[t] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) red[t] += red[t + w];
+    __syncthreads();
+  }
+  if (t == 0) norm0[s] = sqrt(red[0]);
+}
+
+// One refinement per step: FP64 norm of each propagated state; for unitary
+// propagators (renorm != 0) the state is rescaled to its initial norm in
+// place; out[s][j] = e^{k mu} * state (complex128).
+template <typename W>
+__global__ __launch_bounds__(256) void refine_states(W* __restrict__ pr, W* __restrict__ pi, int64_t ld, int n,
+                                                     const double* __restrict__ norm0, int renorm, double ph_re,
+                                                     double ph_im, double2* __restrict__ out) {
+  __shared__ double red[256];
+  const int s = blockIdx.x, t = threadIdx.x;
+  double acc = 0.0;
+  for (int j = t; j < n; j += 256) {
+    const double a = (double)pr[(int64_t)s * ld + j], b = (double)pi[(int64_t)s * ld + j];
+    acc += a * a + b * b;
+  }
+  red[t] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) red[t] += red[t + w];
+    __syncthreads();
+  }
+  const double nrm = sqrt(red[0]);
+  const double f = (renorm && nrm > 0.0) ? norm0[s] / nrm : 1.0;
+  for (int j = t; j < n; j += 256) {
+    const double a = f * (double)pr[(int64_t)s * ld + j], b = f * (double)pi[(int64_t)s * ld + j];
+    if (renorm) {
+      pr[(int64_t)s * ld + j] = (W)a;
+      pi[(int64_t)s * ld + j] = (W)b;
+    }
+    out[(int64_t)s * n + j] = make_double2(ph_re * a - ph_im * b, ph_re * b + ph_im * a);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // psi_out[s][i] = e^mu sum_j X[i][j] psi0[s][j], FP64 accumulation; one
 // wavefront per row, states looped (nstates small).
 template <typename W>

@@ -135,6 +135,24 @@ template <typename T, int DT> class ExpMat {
     return out;
   }
 
+  py::array_t<std::complex<double>> propagate(
+      py::array_t<std::complex<double>, py::array::c_style | py::array::forcecast> psi0, int64_t nsteps) {
+    if (psi0.ndim() != 2 || psi0.shape(1) != n_)
+      throw std::invalid_argument("fastexpm.propagate: states must have shape (k, n)");
+    if (nsteps < 0) throw std::invalid_argument("fastexpm.propagate: nsteps must be >= 0");
+    const int64_t ns = psi0.shape(0);
+    py::array_t<std::complex<double>> out({nsteps, ns, n_});
+    const void* src = psi0.data();
+    void* dst = out.mutable_data();
+    int rc;
+    {
+      py::gil_scoped_release nogil;
+      rc = fexpm_propagate(e_, src, ns, nsteps, dst);
+    }
+    raise_status(rc, e_);
+    return out;
+  }
+
   void free() {
     int rc = fexpm_free(e_);
     raise_status(rc, e_);
@@ -166,6 +184,7 @@ template <typename T, int DT> class ExpMat {
     d["s_fp64"] = s.s_fp64;
     d["kblock"] = s.kblock;
     d["pade_fp64"] = (bool)s.pade_fp64;
+    d["ms_propagate"] = s.ms_propagate;
     return d;
   }
 
@@ -206,6 +225,7 @@ template <typename T, int DT> static void bind(py::module_& m, const char* name)
       .def("download", &C::download)
       .def("run_device", &C::run_device, py::arg("d_a"), py::arg("d_u"), py::arg("stream") = 0)
       .def("apply", &C::apply, py::arg("psi0"))
+      .def("propagate", &C::propagate, py::arg("psi0"), py::arg("nsteps"))
       .def("set_option", &C::set_option, py::arg("key"), py::arg("value"))
       .def("stats", &C::stats)
       .def("debug_plane", &C::debug_plane, py::arg("which"))

@@ -71,6 +71,12 @@ class _ExpMatBase:
         """psi = U psi0 (complex128) for the U of the last run, FP64 accumulation."""
         return self._backend.apply(np.asarray(psi0, dtype=np.complex128))
 
+    def propagate(self, psi0, nsteps: int) -> np.ndarray:
+        """psi[k-1] = U^k psi0 for k = 1..nsteps (complex128, shape
+        (nsteps, nstates, N)) with the U of the last run."""
+        psi0 = np.atleast_2d(np.asarray(psi0, dtype=np.complex128))
+        return self._backend.propagate(psi0, int(nsteps))
+
     def stats(self) -> dict:
         return self._backend.stats()
 

@@ -86,6 +86,7 @@ typedef struct fexpm_stats {
   int32_t s_fp64;       /* squarings performed in FP64 by an FP32 engine */
   int32_t kblock;       /* accumulation block of the FP32 squarings (16-k tiles) */
   int32_t pade_fp64;    /* 1: Pade products and solve ran in FP64 */
+  double ms_propagate;  /* device time of the last fexpm_propagate (ms) */
 } fexpm_stats;
 
 /* Create an engine for n x n matrices of the given dtype and allocate its
@@ -126,6 +127,15 @@ int fexpm_download(fexpm_engine* e, void* u);
  * accumulation on the device; psi0 and psi are host complex128 arrays of
  * nstates x n (replaces the host product at FastCTQW/Ctqw.py:94). */
 int fexpm_apply(fexpm_engine* e, const void* psi0, int64_t nstates, void* psi);
+
+/* Propagation of many states by repeated application of the U of the last
+ * run: psi[k][s] = U^k psi0[s], k = 1..nsteps, s < nstates (host complex128,
+ * psi0: nstates x n, psi: nsteps x nstates x n).  Each step is one MFMA
+ * GEMM over all states followed by an FP64 refinement (norm restored for
+ * CTQW propagators, phase e^{k mu}).  With U = expm(-i dt H) this gives the
+ * walk at t_k = k dt.  No counterpart in the reference (which re-runs the
+ * whole expm per time, FastCTQW/Ctqw.py:90-94). */
+int fexpm_propagate(fexpm_engine* e, const void* psi0, int64_t nstates, int64_t nsteps, void* psi);
 
 int fexpm_set_option(fexpm_engine* e, int key, int64_t value);
 int fexpm_get_stats(const fexpm_engine* e, fexpm_stats* out);

@@ -255,6 +255,35 @@ def test_device_resident_and_apply(fx):
     assert np.allclose(psi[0], u[:, 0].astype(np.complex128), atol=1e-6)
 
 
+@pytest.mark.parametrize("dt", [np.complex64, np.complex128])
+def test_propagate_states(fx, dt):
+    """Many times x many states (the C5 form): psi_k = U(dt)^k psi0 against
+    the eigen oracle of the same rounded input, every k and every state."""
+    from fastctqw_amd.graphs import random_state
+
+    n, h, nsteps, ns = 300, 0.5, 6, 5
+    _say(f"propagate n={n} steps={nsteps} states={ns} {np.dtype(dt).name}")
+    L = laplacian(erdos_renyi(n, 0.3, seed=5, weighted=True))
+    a = (-1j * h * L).astype(dt)
+    G = np.ascontiguousarray(a.imag, dtype=np.float64)
+    lam, V = np.linalg.eigh(G)
+    psi0 = np.stack([random_state(n, seed=55, index=k) for k in range(ns)])
+    with _engine(fx, dt, n) as e:
+        e.run(a)
+        out = e.propagate(psi0, nsteps)
+        assert e.stats()["ms_propagate"] > 0
+    assert out.shape == (nsteps, ns, n)
+    tol = 1e-9 if dt == np.complex64 else 1e-20
+    worst = 0.0
+    for k in range(1, nsteps + 1):
+        ref = (V * np.exp(1j * k * lam)[None, :]) @ (V.T @ psi0.T)
+        for sidx in range(ns):
+            worst = max(worst, infidelity(ref[:, sidx], out[k - 1, sidx]))
+            assert abs(np.linalg.norm(out[k - 1, sidx]) - 1.0) < 1e-12
+    print(f"  worst infidelity {worst:.2e}", flush=True)
+    assert worst < tol
+
+
 def test_ctqw_drop_in(fx):
     """README usage (README.md:61-72) with device='cuda'."""
     _say("Ctqw drop-in")
