@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -105,11 +106,13 @@ struct PipelineBase {
   virtual void propagate(const double2* d_psi0, int nstates, int nsteps, double2* d_out, hipStream_t st) = 0;
   virtual bool has_result() const = 0;
   virtual void debug_plane(int which, double* re, double* im) = 0;
+  virtual void set_size(int n_new, hipStream_t st) = 0;
   int max_squarings = -1;
   double last_prop_ms = 0.0;  // device time of the last propagate()
   int kblock = 8;          // FP32 squarings: k-tiles (of 16) per accumulation block, 0 = off
   int fp64_squarings = 0;  // FP32 engines: leading squarings done in FP64
   int pade_fp64 = 1;       // FP32 engines: Pade + solve in FP64 (companion FP64 pipeline)
+  int panel_coop = 1;      // LU panels by one cooperative launch (0: one launch per column)
 };
 
 // ---------------------------------------------------------------------------
@@ -205,7 +208,7 @@ static void lu_block_row_solve(double* lr, double* li, int64_t ldl, int k0, int 
 
 static double lu_factor_solve(double* lr, double* li, int64_t ldl, int inp, double* tr, double* ti, int* ipiv,
                               PanelCand* cand, int gmax, double* top, Scalars* scal, double* inv,
-                              unsigned* ctrs, hipStream_t st) {
+                              unsigned* ctrs, hipStream_t st, bool coop = true) {
   double flops = 0.0;
   // one zeroed arrival counter per panel for the cooperative panel kernel
   FX_HIP(hipMemsetAsync(ctrs, 0, (size_t)(inp / LU_NB) * sizeof(unsigned), st));
@@ -217,7 +220,7 @@ static double lu_factor_solve(double* lr, double* li, int64_t ldl, int inp, doub
     // --- outer panel S[K0:, K0:K1] with 64-wide inner panels ---
     for (int k0 = K0; k0 < K1; k0 += nb) {
       const int G = (inp - k0 + PANEL_R - 1) / PANEL_R;
-      if (g_panel_coop && G <= panel_coop_capacity()) {
+      if (coop && g_panel_coop && G <= panel_coop_capacity()) {
         unsigned* ctr = ctrs + k0 / LU_NB;
         void* kargs[] = {&lr, &li, (void*)&ldl, &inp, (void*)&k0, (void*)&G, &cand, &top, &ipiv, &scal, &ctr};
         FX_HIP(hipLaunchCooperativeKernel((const void*)lu_panel_coop, dim3(G), dim3(256), kargs, 0, st));
@@ -268,8 +271,13 @@ static double lu_factor_solve(double* lr, double* li, int64_t ldl, int inp, doub
 }
 
 template <typename W> struct Pipeline : PipelineBase {
+  // Buffers are sized for the capacity ncap; a run works on the current
+  // order n <= ncap (set_size), np = round_up(n, 128).  Leading dimensions
+  // are those of the capacity.
   int n, dtype;
-  int64_t np, ld;        // padded order / leading dimension of W planes
+  int ncap;
+  int64_t np, ld;        // current padded order / leading dimension of W planes
+  int64_t npcap;
   int64_t ldl;           // leading dimension of the FP64 system [M | B]
   Plane2<W> a, p2, p4, p6, t1, t2, u, v, x0, x1;
   double* lr = nullptr;  // FP64 system, np x 2np
@@ -296,26 +304,27 @@ template <typename W> struct Pipeline : PipelineBase {
   bool have_result = false;
   double flops = 0.0;
 
-  Pipeline(int n_, int dtype_) : n(n_), dtype(dtype_) {
-    np = round_up(std::max(n, 1), 128);
-    ld = np;
-    ldl = 2 * np;
+  Pipeline(int n_, int dtype_) : n(n_), dtype(dtype_), ncap(n_) {
+    npcap = round_up(std::max(ncap, 1), 128);
+    np = npcap;
+    ld = npcap;
+    ldl = 2 * npcap;
     Plane2<W>* all[] = {&a, &p2, &p4, &p6, &t1, &t2, &u, &v, &x0, &x1};
-    const size_t cnt = (size_t)np * (size_t)ld;
+    const size_t cnt = (size_t)npcap * (size_t)ld;
     for (Plane2<W>* p : all) {
       p->r = dalloc<W>(cnt);
       p->i = dalloc<W>(cnt);
     }
-    lr = dalloc<double>((size_t)np * ldl);
-    li = dalloc<double>((size_t)np * ldl);
-    tr = dalloc<double>((size_t)2 * np * LU_OUTER);
-    ti = dalloc<double>((size_t)2 * np * LU_OUTER);
-    ipiv = dalloc<int>(np);
-    gmax = (int)((np + PANEL_R - 1) / PANEL_R);
+    lr = dalloc<double>((size_t)npcap * ldl);
+    li = dalloc<double>((size_t)npcap * ldl);
+    tr = dalloc<double>((size_t)2 * npcap * LU_OUTER);
+    ti = dalloc<double>((size_t)2 * npcap * LU_OUTER);
+    ipiv = dalloc<int>(npcap);
+    gmax = (int)((npcap + PANEL_R - 1) / PANEL_R);
     cand = dalloc<PanelCand>((size_t)2 * gmax);
     top = dalloc<double>(4 * LU_NB);
     inv = dalloc<double>(2 * LU_NB * LU_NB);
-    ctrs = dalloc<unsigned>(np / LU_NB);
+    ctrs = dalloc<unsigned>(npcap / LU_NB);
     const int nt = (n + 31) / 32;
     colpart = dalloc<double>((size_t)std::max(nt, 1) * std::max(n, 1));
     off = dalloc<double>(std::max(n, 1));
@@ -360,6 +369,28 @@ template <typename W> struct Pipeline : PipelineBase {
   }
 
   bool has_result() const override { return have_result; }
+
+  // Switch to order n_new <= ncap.  The working planes are cleared so that
+  // columns beyond the new order are zero again (GEMM k-loops rely on it).
+  void set_size(int n_new, hipStream_t st) override {
+    if (n_new < 0 || n_new > ncap) throw Error(FEXPM_EINVAL, "order exceeds the engine capacity");
+    if (n_new == n) return;
+    n = n_new;
+    np = round_up(std::max(n, 1), 128);
+    Plane2<W>* all[] = {&a, &p2, &p4, &p6, &t1, &t2, &u, &v, &x0, &x1};
+    const size_t bytes = (size_t)npcap * (size_t)ld * sizeof(W);
+    for (Plane2<W>* p : all) {
+      FX_HIP(hipMemsetAsync(p->r, 0, bytes, st));
+      FX_HIP(hipMemsetAsync(p->i, 0, bytes, st));
+    }
+    for (auto& P : ps)
+      if (P.r) {
+        FX_HIP(hipMemsetAsync(P.r, 0, (size_t)ps_rows * ld * sizeof(W), st));
+        FX_HIP(hipMemsetAsync(P.i, 0, (size_t)ps_rows * ld * sizeof(W), st));
+      }
+    have_result = false;
+    if (hi) hi->set_size(n_new, st);
+  }
 
   // ---- launch helpers ----
   template <int MODE, bool SYM, bool KBLK = false>
@@ -553,7 +584,7 @@ template <typename W> struct Pipeline : PipelineBase {
       form_system<W><<<grid, 256, 0, st>>>(u.r, u.i, v.r, v.i, ld, n, inp, lr, li, ldl, structured ? 1 : 0);
       FX_LAUNCH();
     }
-    flops += lu_factor_solve(lr, li, ldl, inp, tr, ti, ipiv, cand, gmax, top, scal, inv, ctrs, st);
+    flops += lu_factor_solve(lr, li, ldl, inp, tr, ti, ipiv, cand, gmax, top, scal, inv, ctrs, st, panel_coop != 0);
     if (q64 <= 0) {
       dim3 grid((n + 31) / 32, (n + 31) / 32);
       solution_to_work<W><<<grid, 256, 0, st>>>(lr + inp, li + inp, ldl, x0.r, x0.i, ld, n, structured ? 1 : 0,
@@ -657,7 +688,12 @@ template <typename W> struct Pipeline : PipelineBase {
   // FP64 (allocated on first use).
   std::unique_ptr<Pipeline<double>> hi;
   Pipeline<double>* companion() {
-    if (!hi) hi.reset(new Pipeline<double>(n, FEXPM_COMPLEX128));
+    if (!hi) {
+      hi.reset(new Pipeline<double>(ncap, FEXPM_COMPLEX128));
+      hi->panel_coop = panel_coop;
+      hi->set_size(n, nullptr);
+      FX_HIP(hipDeviceSynchronize());
+    }
     return hi.get();
   }
   Scalars* companion_scal() { return hi ? hi->scal : scal; }
@@ -714,6 +750,7 @@ template <typename W> struct Pipeline : PipelineBase {
       // FP64 Pade + solve (+ leading squarings), then FP32 squarings
       Pipeline<double>* h = companion();
       h->flops = 0.0;
+      h->panel_coop = panel_coop;
       h->front_pade<Tin>(d_a, structured, m, mu_re, mu_im, scale, st);
       FX_HIP(hipEventRecord(ev[2], st));
       q64 = std::min(s_do, fp64_squarings);
@@ -1166,7 +1203,7 @@ int fexpm_set_option(fexpm_engine* e, int key, int64_t value) {
       if (e->pipe) e->pipe->kblock = (int)value;
       return FEXPM_OK;
     case FEXPM_OPT_PANEL_COOP:
-      g_panel_coop = value ? 1 : 0;
+      if (e->pipe) e->pipe->panel_coop = value ? 1 : 0;
       return FEXPM_OK;
     case FEXPM_OPT_PADE_FP64:
       if (e->pipe) e->pipe->pade_fp64 = value ? 1 : 0;
@@ -1210,6 +1247,226 @@ double fexpm_gpu_memory_gb(void) {
   if (hipMemGetInfo(&fr, &tot) != hipSuccess) return -1.0;
   return (double)fr / (1024.0 * 1024.0 * 1024.0);
 }
+
+}  // extern "C"
+
+// ===========================================================================
+// Batches of independent expms of uneven order (many walks).
+#include <thread>
+
+struct fexpm_batch {
+  int dtype = 0;
+  int64_t nmax = 0;
+  int lanes = 1;
+  size_t esize = 0;
+  std::mutex mtx;
+  std::string err;
+  std::vector<std::unique_ptr<PipelineBase>> pipes;
+  std::vector<hipStream_t> streams;
+  std::vector<int64_t> n;       // orders of the uploaded batch
+  std::vector<size_t> off;      // element offsets of each matrix in the packed buffers
+  std::vector<size_t> voff;     // element offsets of each state
+  void* d_in = nullptr;
+  void* d_out = nullptr;
+  size_t cap_elems = 0;
+  double2* d_psi0 = nullptr;
+  double2* d_psi = nullptr;
+  size_t cap_states = 0;
+  std::vector<fexpm_stats> stats;
+  double last_ms = 0.0;
+};
+
+static void batch_release(fexpm_batch* b) {
+  b->pipes.clear();
+  for (hipStream_t s : b->streams) (void)hipStreamDestroy(s);
+  b->streams.clear();
+  if (b->d_in) (void)hipFree(b->d_in);
+  if (b->d_out) (void)hipFree(b->d_out);
+  if (b->d_psi0) (void)hipFree(b->d_psi0);
+  if (b->d_psi) (void)hipFree(b->d_psi);
+  b->d_in = b->d_out = nullptr;
+  b->d_psi0 = b->d_psi = nullptr;
+}
+
+template <typename F> static int guarded_b(fexpm_batch* b, F&& f) {
+  try {
+    f();
+    return FEXPM_OK;
+  } catch (const Error& x) {
+    b->err = x.what();
+    return x.code;
+  } catch (const std::exception& x) {
+    b->err = x.what();
+    return FEXPM_EHIP;
+  }
+}
+
+extern "C" {
+
+int fexpm_batch_create(int dtype, int64_t nmax, int lanes, fexpm_batch** out) {
+  if (!out || nmax <= 0 || nmax > (int64_t)1 << 16 || lanes < 1 || lanes > 16 || elem_size(dtype) == 0)
+    return FEXPM_EINVAL;
+  *out = nullptr;
+  fexpm_batch* b = new (std::nothrow) fexpm_batch();
+  if (!b) return FEXPM_ENOMEM;
+  b->dtype = dtype;
+  b->nmax = nmax;
+  b->lanes = lanes;
+  b->esize = elem_size(dtype);
+  int rc = guarded_b(b, [&] {
+    check_device();
+    for (int l = 0; l < lanes; ++l) {
+      hipStream_t st;
+      FX_HIP(hipStreamCreate(&st));
+      b->streams.push_back(st);
+      PipelineBase* p;
+      if (dtype == FEXPM_FLOAT32 || dtype == FEXPM_COMPLEX64)
+        p = new Pipeline<float>((int)nmax, dtype);
+      else
+        p = new Pipeline<double>((int)nmax, dtype);
+      p->panel_coop = 0;  // lanes run concurrently: no cooperative launches
+      b->pipes.emplace_back(p);
+    }
+  });
+  if (rc != FEXPM_OK) {
+    g_last_error = b->err;
+    batch_release(b);
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return FEXPM_OK;
+}
+
+int fexpm_batch_destroy(fexpm_batch* b) {
+  if (!b) return FEXPM_EINVAL;
+  batch_release(b);
+  delete b;
+  return FEXPM_OK;
+}
+
+const char* fexpm_batch_last_error(const fexpm_batch* b) { return b ? b->err.c_str() : g_last_error.c_str(); }
+
+int fexpm_batch_upload(fexpm_batch* b, int64_t count, const int64_t* n, const void* a) {
+  if (!b || count < 0 || (count > 0 && (!n || !a))) return FEXPM_EINVAL;
+  std::lock_guard<std::mutex> lk(b->mtx);
+  for (int64_t i = 0; i < count; ++i)
+    if (n[i] < 1 || n[i] > b->nmax) {
+      b->err = "order outside [1, nmax]";
+      return FEXPM_EINVAL;
+    }
+  return guarded_b(b, [&] {
+    b->n.assign(n, n + count);
+    b->off.resize(count);
+    b->voff.resize(count);
+    size_t tot = 0, vt = 0;
+    for (int64_t i = 0; i < count; ++i) {
+      b->off[i] = tot;
+      b->voff[i] = vt;
+      tot += (size_t)n[i] * (size_t)n[i];
+      vt += (size_t)n[i];
+    }
+    if (tot > b->cap_elems) {
+      if (b->d_in) (void)hipFree(b->d_in);
+      if (b->d_out) (void)hipFree(b->d_out);
+      b->d_in = b->d_out = nullptr;
+      FX_HIP(hipMalloc(&b->d_in, tot * b->esize));
+      FX_HIP(hipMalloc(&b->d_out, tot * b->esize));
+      b->cap_elems = tot;
+    }
+    if (vt > b->cap_states) {
+      if (b->d_psi0) (void)hipFree(b->d_psi0);
+      if (b->d_psi) (void)hipFree(b->d_psi);
+      b->d_psi0 = b->d_psi = nullptr;
+      FX_HIP(hipMalloc(&b->d_psi0, vt * sizeof(double2)));
+      FX_HIP(hipMalloc(&b->d_psi, vt * sizeof(double2)));
+      b->cap_states = vt;
+    }
+    if (tot) FX_HIP(hipMemcpy(b->d_in, a, tot * b->esize, hipMemcpyHostToDevice));
+    b->stats.assign(count, fexpm_stats());
+  });
+}
+
+// U_i = expm(A_i) for the uploaded batch on the device; with psi0 (packed
+// complex128 host vectors, length n_i each) also psi_i = U_i psi0_i, copied
+// to psi.  Walks are assigned to lanes by decreasing n^3 (greedy least
+// loaded); every lane runs its walks on its own stream from its own host
+// thread, so small walks of different lanes overlap on the device.
+int fexpm_batch_run_resident(fexpm_batch* b, const void* psi0, void* psi) {
+  if (!b || (psi0 && !psi)) return FEXPM_EINVAL;
+  std::lock_guard<std::mutex> lk(b->mtx);
+  const int64_t count = (int64_t)b->n.size();
+  if (count == 0) return FEXPM_OK;
+  return guarded_b(b, [&] {
+    size_t vt = b->voff.back() + (size_t)b->n.back();
+    if (psi0) FX_HIP(hipMemcpy(b->d_psi0, psi0, vt * sizeof(double2), hipMemcpyHostToDevice));
+    std::vector<int64_t> order(count);
+    for (int64_t i = 0; i < count; ++i) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](int64_t x, int64_t y) { return b->n[x] > b->n[y]; });
+    std::vector<std::vector<int64_t>> assign(b->lanes);
+    std::vector<double> load(b->lanes, 0.0);
+    for (int64_t i : order) {
+      int best = 0;
+      for (int l = 1; l < b->lanes; ++l)
+        if (load[l] < load[best]) best = l;
+      assign[best].push_back(i);
+      load[best] += (double)b->n[i] * b->n[i] * b->n[i];
+    }
+    std::vector<std::string> errs(b->lanes);
+    std::vector<int> codes(b->lanes, FEXPM_OK);
+    auto t0 = std::chrono::steady_clock::now();
+    auto worker = [&](int l) {
+      try {
+        FX_HIP(hipSetDevice(0));
+        PipelineBase* p = b->pipes[l].get();
+        hipStream_t st = b->streams[l];
+        for (int64_t i : assign[l]) {
+          p->set_size((int)b->n[i], st);
+          fexpm_stats S;
+          std::memset(&S, 0, sizeof(S));
+          S.n = b->n[i];
+          S.dtype = b->dtype;
+          p->run((const char*)b->d_in + b->off[i] * b->esize, (char*)b->d_out + b->off[i] * b->esize, st, S, 1);
+          if (psi0) p->apply(b->d_psi0 + b->voff[i], 1, b->d_psi + b->voff[i], st);
+          b->stats[i] = S;
+        }
+        FX_HIP(hipStreamSynchronize(st));
+      } catch (const Error& x) {
+        errs[l] = x.what();
+        codes[l] = x.code;
+      } catch (const std::exception& x) {
+        errs[l] = x.what();
+        codes[l] = FEXPM_EHIP;
+      }
+    };
+    std::vector<std::thread> th;
+    for (int l = 1; l < b->lanes; ++l) th.emplace_back(worker, l);
+    worker(0);
+    for (auto& x : th) x.join();
+    b->last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    for (int l = 0; l < b->lanes; ++l)
+      if (codes[l] != FEXPM_OK) throw Error(codes[l], "lane " + std::to_string(l) + ": " + errs[l]);
+    if (psi0) FX_HIP(hipMemcpy(psi, b->d_psi, vt * sizeof(double2), hipMemcpyDeviceToHost));
+  });
+}
+
+int fexpm_batch_download(fexpm_batch* b, int64_t i, void* u) {
+  if (!b || !u) return FEXPM_EINVAL;
+  std::lock_guard<std::mutex> lk(b->mtx);
+  if (i < 0 || i >= (int64_t)b->n.size()) return FEXPM_EINVAL;
+  return guarded_b(b, [&] {
+    FX_HIP(hipMemcpy(u, (const char*)b->d_out + b->off[i] * b->esize, (size_t)b->n[i] * b->n[i] * b->esize,
+                     hipMemcpyDeviceToHost));
+  });
+}
+
+int fexpm_batch_get_stats(const fexpm_batch* b, int64_t i, fexpm_stats* out) {
+  if (!b || !out || i < 0 || i >= (int64_t)b->stats.size()) return FEXPM_EINVAL;
+  *out = b->stats[i];
+  return FEXPM_OK;
+}
+
+double fexpm_batch_last_ms(const fexpm_batch* b) { return b ? b->last_ms : -1.0; }
 
 }  // extern "C"
 

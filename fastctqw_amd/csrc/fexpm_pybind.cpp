@@ -14,6 +14,8 @@
 #include <pybind11/pybind11.h>
 
 #include <complex>
+#include <cstring>
+#include <vector>
 #include <stdexcept>
 #include <string>
 
@@ -212,6 +214,116 @@ template <typename T, int DT> class ExpMat {
   fexpm_engine* e_ = nullptr;
 };
 
+// Batch of independent expms of uneven order (C ABI fexpm_batch_*).
+class ExpMatBatch {
+ public:
+  ExpMatBatch(const std::string& dtype, int64_t nmax, int lanes) {
+    if (dtype == "complex64") dt_ = FEXPM_COMPLEX64, es_ = 8;
+    else if (dtype == "complex128") dt_ = FEXPM_COMPLEX128, es_ = 16;
+    else throw std::invalid_argument("ExpMatBatch: dtype must be complex64 or complex128");
+    fexpm_batch* b = nullptr;
+    int rc = fexpm_batch_create(dt_, nmax, lanes, &b);
+    if (rc != FEXPM_OK)
+      throw std::runtime_error(std::string("fastexpm batch: ") + fexpm_status_string(rc) + ": " +
+                               fexpm_batch_last_error(nullptr));
+    b_ = b;
+  }
+  ~ExpMatBatch() {
+    if (b_) fexpm_batch_destroy(b_);
+  }
+  void check(int rc) {
+    if (rc != FEXPM_OK)
+      throw std::runtime_error(std::string("fastexpm batch: ") + fexpm_status_string(rc) + ": " +
+                               fexpm_batch_last_error(b_));
+  }
+  // mats: list of square complex arrays (cast to the batch dtype)
+  void upload(py::list mats) {
+    n_.clear();
+    std::vector<char> host;
+    for (auto h : mats) {
+      py::array a = py::array::ensure(h, py::array::c_style | py::array::forcecast);
+      if (a.ndim() != 2 || a.shape(0) != a.shape(1)) throw std::invalid_argument("batch: square matrices expected");
+      py::array c = dt_ == FEXPM_COMPLEX64
+                        ? (py::array)py::array_t<std::complex<float>, py::array::c_style | py::array::forcecast>(a)
+                        : (py::array)py::array_t<std::complex<double>, py::array::c_style | py::array::forcecast>(a);
+      const size_t off = host.size();
+      host.resize(off + (size_t)c.nbytes());
+      std::memcpy(host.data() + off, c.data(), (size_t)c.nbytes());
+      n_.push_back(a.shape(0));
+    }
+    int rc;
+    {
+      py::gil_scoped_release nogil;
+      rc = fexpm_batch_upload(b_, (int64_t)n_.size(), n_.data(), host.data());
+    }
+    check(rc);
+  }
+  // Runs the uploaded batch; with psi0 (list of vectors) returns the list of psi.
+  py::object run_resident(py::object psi0) {
+    if (psi0.is_none()) {
+      int rc;
+      {
+        py::gil_scoped_release nogil;
+        rc = fexpm_batch_run_resident(b_, nullptr, nullptr);
+      }
+      check(rc);
+      return py::none();
+    }
+    py::list lst = psi0;
+    if ((size_t)py::len(lst) != n_.size()) throw std::invalid_argument("batch: one state per matrix expected");
+    size_t tot = 0;
+    for (int64_t v : n_) tot += (size_t)v;
+    std::vector<std::complex<double>> in(tot), out(tot);
+    size_t off = 0;
+    for (size_t i = 0; i < n_.size(); ++i) {
+      auto v = py::array_t<std::complex<double>, py::array::c_style | py::array::forcecast>(lst[i]);
+      if (v.ndim() != 1 || v.shape(0) != n_[i]) throw std::invalid_argument("batch: state length must match n_i");
+      std::memcpy(in.data() + off, v.data(), (size_t)n_[i] * sizeof(std::complex<double>));
+      off += (size_t)n_[i];
+    }
+    int rc;
+    {
+      py::gil_scoped_release nogil;
+      rc = fexpm_batch_run_resident(b_, in.data(), out.data());
+    }
+    check(rc);
+    py::list res;
+    off = 0;
+    for (size_t i = 0; i < n_.size(); ++i) {
+      py::array_t<std::complex<double>> v(n_[i]);
+      std::memcpy(v.mutable_data(), out.data() + off, (size_t)n_[i] * sizeof(std::complex<double>));
+      res.append(v);
+      off += (size_t)n_[i];
+    }
+    return res;
+  }
+  py::array download(int64_t i) {
+    if (i < 0 || i >= (int64_t)n_.size()) throw std::invalid_argument("batch: index out of range");
+    py::array out = dt_ == FEXPM_COMPLEX64 ? (py::array)py::array_t<std::complex<float>>({n_[i], n_[i]})
+                                           : (py::array)py::array_t<std::complex<double>>({n_[i], n_[i]});
+    check(fexpm_batch_download(b_, i, out.mutable_data()));
+    return out;
+  }
+  py::dict stats(int64_t i) {
+    fexpm_stats s;
+    check(fexpm_batch_get_stats(b_, i, &s));
+    py::dict d;
+    d["n"] = s.n;
+    d["m"] = s.m;
+    d["s"] = s.s;
+    d["structured"] = (bool)s.structured;
+    d["ms_total"] = s.ms_total;
+    return d;
+  }
+  double last_ms() const { return fexpm_batch_last_ms(b_); }
+
+ private:
+  fexpm_batch* b_ = nullptr;
+  int dt_ = 0;
+  size_t es_ = 0;
+  std::vector<int64_t> n_;
+};
+
 template <typename T, int DT> static void bind(py::module_& m, const char* name) {
   using C = ExpMat<T, DT>;
   py::class_<C>(m, name)
@@ -244,5 +356,12 @@ PYBIND11_MODULE(_fastexpm_core, m) {
   m.attr("OPT_FP64_SQUARINGS") = (int)FEXPM_OPT_FP64_SQUARINGS;
   m.attr("OPT_PADE_FP64") = (int)FEXPM_OPT_PADE_FP64;
   m.attr("OPT_MAX_SQUARINGS") = (int)FEXPM_OPT_MAX_SQUARINGS;
+  py::class_<ExpMatBatch>(m, "ExpMatBatch")
+      .def(py::init<const std::string&, int64_t, int>(), py::arg("dtype"), py::arg("nmax"), py::arg("lanes") = 4)
+      .def("upload", &ExpMatBatch::upload, py::arg("mats"))
+      .def("run_resident", &ExpMatBatch::run_resident, py::arg("psi0") = py::none())
+      .def("download", &ExpMatBatch::download, py::arg("i"))
+      .def("stats", &ExpMatBatch::stats, py::arg("i"))
+      .def("last_ms", &ExpMatBatch::last_ms);
   m.def("gpu_memory_gb", &fexpm_gpu_memory_gb);
 }

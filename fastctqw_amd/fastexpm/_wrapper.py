@@ -98,3 +98,36 @@ class ExpMatComplex64(_ExpMatBase):
 
 class ExpMatComplex128(_ExpMatBase):
     _core_cls = _fastexpm_core.ExpMatComplex128
+
+
+class ExpMatBatch:
+    """Many independent exponentials of uneven order on the GPU (no
+    counterpart in the reference).  ``run(mats, psi0=None)`` returns the list
+    of U (and of psi = U psi0 when states are given)."""
+
+    def __init__(self, nmax: int, dtype=np.complex64, lanes: int = 4):
+        self.nmax = nmax
+        name = np.dtype(dtype).name
+        self._backend = _fastexpm_core.ExpMatBatch(name, int(nmax), int(lanes))
+
+    def upload(self, mats):
+        self._backend.upload([np.asarray(m) for m in mats])
+        self._count = len(mats)
+
+    def run_resident(self, psi0=None):
+        return self._backend.run_resident(None if psi0 is None else [np.asarray(v) for v in psi0])
+
+    def download(self, i: int) -> np.ndarray:
+        return self._backend.download(int(i))
+
+    def run(self, mats, psi0=None):
+        self.upload(mats)
+        psi = self.run_resident(psi0)
+        us = [self.download(i) for i in range(len(mats))]
+        return us if psi is None else (us, psi)
+
+    def stats(self, i: int) -> dict:
+        return self._backend.stats(int(i))
+
+    def last_ms(self) -> float:
+        return self._backend.last_ms()

@@ -178,3 +178,33 @@ def c2_laplacian(n: int = 10000, p: float = 0.5, seed: int = 2,
                  weighted: bool = False, dtype=np.float64) -> np.ndarray:
     """Laplacian of the seeded G(n, p) graph used for the N = 10^4 walk."""
     return laplacian(erdos_renyi(n, p, seed, weighted=weighted, dtype=dtype))
+
+
+def c4_walks(count: int = 256, seed: int = 4000, nmin: int = 64, nmax: int = 2048):
+    """Seeded batch of uneven CTQW instances (the many-walks workload).
+
+    Instance i draws from PCG64(seed + i): N ~ U{nmin..nmax}; even i: a random
+    d-regular graph (d ~ U{3..8}, N bumped by one when N*d is odd), odd i:
+    G(N, p) with p ~ U[0.05, 0.5]; edge weights uniform(0, 1]; H = adjacency
+    or Laplacian at random; t ~ U[0.1, 20].  Returns a list of dicts with
+    keys n, kind, laplacian, t, H (float64) and A = -1j*t*H (complex128).
+    """
+    out = []
+    for i in range(count):
+        rng = np.random.Generator(np.random.PCG64(seed + i))
+        n = int(rng.integers(nmin, nmax + 1))
+        lap = bool(rng.integers(0, 2))
+        t = float(rng.uniform(0.1, 20.0))
+        if i % 2 == 0:
+            d = int(rng.integers(3, 9))
+            if (n * d) % 2:
+                n += 1 if n < nmax else -1
+            adj = random_regular(n, d, seed=seed + i, weighted=True)
+            kind = f"regular(d={d})"
+        else:
+            p = float(rng.uniform(0.05, 0.5))
+            adj = erdos_renyi(n, p, seed=seed + i, weighted=True)
+            kind = f"er(p={p:.3f})"
+        h = laplacian(adj) if lap else adj
+        out.append({"n": n, "kind": kind, "laplacian": lap, "t": t, "H": h, "A": -1j * t * h})
+    return out

@@ -142,6 +142,25 @@ int fexpm_get_stats(const fexpm_engine* e, fexpm_stats* out);
 const char* fexpm_last_error(const fexpm_engine* e);
 const char* fexpm_status_string(int status);
 
+/* ---- Batches of independent expms of uneven order (many walks) ----
+ * No counterpart in the reference (which runs one expm per Operator).  A
+ * batch owns `lanes` pipelines of capacity nmax, each with its own stream;
+ * walks are assigned to lanes by decreasing n^3 and each lane runs on its
+ * own host thread, so small walks overlap on the device. */
+typedef struct fexpm_batch fexpm_batch;
+int fexpm_batch_create(int dtype, int64_t nmax, int lanes, fexpm_batch** out);
+int fexpm_batch_destroy(fexpm_batch* b);
+const char* fexpm_batch_last_error(const fexpm_batch* b);
+/* `count` row-major matrices of orders n[i] (engine dtype), packed back to
+ * back in host memory, copied to the device. */
+int fexpm_batch_upload(fexpm_batch* b, int64_t count, const int64_t* n, const void* a);
+/* U_i = expm(A_i) for every uploaded matrix (U stays on the device); with
+ * psi0 != NULL also psi_i = U_i psi0_i (packed complex128, length n_i each). */
+int fexpm_batch_run_resident(fexpm_batch* b, const void* psi0, void* psi);
+int fexpm_batch_download(fexpm_batch* b, int64_t i, void* u);
+int fexpm_batch_get_stats(const fexpm_batch* b, int64_t i, fexpm_stats* out);
+double fexpm_batch_last_ms(const fexpm_batch* b);  /* wall time of the last batch run */
+
 /* Free device memory in GiB (replaces get_current_gpu_memory_gb,
  * include/MatrixExpCalculator.h:101). Returns a negative value on error. */
 double fexpm_gpu_memory_gb(void);

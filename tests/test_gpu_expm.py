@@ -284,6 +284,32 @@ def test_propagate_states(fx, dt):
     assert worst < tol
 
 
+def test_batch_uneven_walks(fx):
+    """Many walks of uneven order, graph family and time in one batch (the
+    C4 form, small orders): every U and psi against the eigen oracle of the
+    same rounded input; lanes reuse one pipeline across orders."""
+    from fastctqw_amd.graphs import c4_walks
+
+    walks = c4_walks(count=10, seed=4100, nmin=16, nmax=300)
+    _say(f"batch of {len(walks)} walks, n in {[w['n'] for w in walks]}")
+    b = fx.ExpMatBatch(nmax=300, dtype=np.complex64, lanes=3)
+    mats = [w["A"].astype(np.complex64) for w in walks]
+    psi0 = [np.full(w["n"], 1 / np.sqrt(w["n"]), dtype=complex) for w in walks]
+    us, psis = b.run(mats, psi0)
+    for i, (w, u, psi) in enumerate(zip(walks, us, psis)):
+        G = np.ascontiguousarray(mats[i].imag, dtype=np.float64)
+        lam, V = np.linalg.eigh(G)
+        ref = (V * np.exp(1j * lam)[None, :]) @ V.T
+        st = pade_expm(mats[i].astype(np.complex128), stages=True)
+        assert b.stats(i)["s"] == st["s"] and b.stats(i)["m"] == st["m"]
+        err = rel_fro(ref, u)
+        assert err <= _tol(np.complex64, st["s"]), (i, w["kind"], err)
+        assert infidelity(ref @ psi0[i], psi) < 1e-9
+    # an order above the capacity is rejected
+    with pytest.raises(RuntimeError):
+        b.upload([np.zeros((301, 301), np.complex64)])
+
+
 def test_ctqw_drop_in(fx):
     """README usage (README.md:61-72) with device='cuda'."""
     _say("Ctqw drop-in")
