@@ -184,6 +184,171 @@ def run_c5(args):
     print(json.dumps(res), flush=True)
 
 
+C4_COUNT, C4_SEED = 256, 4000
+C4_CHECK_U = (0, 1, 2, 3, 64, 65, 128, 129)  # walks whose full U is compared with U_ref
+
+
+def _c4_states(walks_n, first=0):
+    """Uniform superposition and a seeded random state for walks first, first+1, ..."""
+    from fastctqw_amd.graphs import random_state
+
+    uni = [np.full(n, 1.0 / np.sqrt(n), dtype=np.complex128) for n in walks_n]
+    rnd = [random_state(n, seed=44, index=first + i) for i, n in enumerate(walks_n)]
+    return uni, rnd
+
+
+def _c4_reference(out_path, count):
+    """Child process: for each C4 walk, FP64 eigendecomposition of
+    G = Im(A) of the complex64 input (torch on the GPU), psi_ref for the
+    uniform and a random state, and U_ref for a few walks."""
+    import torch
+
+    from fastctqw_amd.graphs import iter_c4_walks
+
+    res = {}
+    ns = []
+    for i, w in enumerate(iter_c4_walks(count, C4_SEED)):
+        a = w["A"].astype(np.complex64)
+        ns.append(w["n"])
+        G = torch.from_numpy(np.ascontiguousarray(a.imag, dtype=np.float64)).cuda()
+        lam, V = torch.linalg.eigh(G)
+        Vc = V.to(torch.complex128)
+        ph = torch.exp(1j * lam.to(torch.complex128))
+        uni, rnd = _c4_states([w["n"]], first=i)
+        for tag, v in (("uni", uni[0]), ("rnd", rnd[0])):
+            x = torch.from_numpy(v).cuda()
+            res[f"{tag}{i}"] = (Vc @ (ph * (Vc.T @ x))).cpu().numpy()
+        if i in C4_CHECK_U:
+            res[f"U{i}"] = ((Vc * ph[None, :]) @ Vc.T).cpu().numpy()
+    res["n"] = np.asarray(ns)
+    np.savez(out_path, **res)
+
+
+def c4_cpu_baseline(mats, s_list, budget_s=20.0):
+    """scipy.linalg.expm + U@psi0 (FastCTQW/Operator.py:76, Ctqw.py:94) on
+    the first walks of the batch until the budget is spent; the batch time
+    is extrapolated by the work model summed over all walks."""
+    import scipy.linalg as sl
+    from threadpoolctl import threadpool_limits
+
+    cores = min(16, os.cpu_count() or 1)
+    done, spent = 0, 0.0
+    with threadpool_limits(limits=cores):
+        sl.expm(mats[0])
+        for a in mats:
+            t0 = time.perf_counter()
+            u = sl.expm(a)
+            _ = u @ np.full(a.shape[0], 1.0 / np.sqrt(a.shape[0]), dtype=a.dtype)
+            spent += time.perf_counter() - t0
+            done += 1
+            if spent > budget_s:
+                break
+    w_all = sum(work_flops(a.shape[0], s) for a, s in zip(mats, s_list))
+    w_done = sum(work_flops(a.shape[0], s) for a, s in zip(mats[:done], s_list[:done]))
+    batch_s = spent * w_all / w_done
+    return {
+        "value": len(mats) / batch_s, "unit": "walks/s", "cores": cores, "kind": "port",
+        "sample": (f"scipy.linalg.expm + U@psi0 (complex64) on the first {done} of {len(mats)} walks: "
+                   f"{spent:.2f} s, extrapolated by sum W(N_i, s_i) to {batch_s:.1f} s per batch"),
+    }
+
+
+def run_c4(args):
+    """256 independent walks of uneven order, graph, H and t: U_i and
+    psi_i = U_i psi0 (uniform superposition) for every walk."""
+    import subprocess
+    import tempfile
+
+    from fastctqw_amd.graphs import iter_c4_walks
+
+    count = args.c4_count
+    ref_path = os.path.join(tempfile.gettempdir(), f"fexpm_c4_{os.getpid()}.npz")
+    child = None
+    if not args.no_accuracy:
+        # reference runs on the GPU in a child while this process generates
+        # the inputs on the host (this process touches the GPU afterwards)
+        child = subprocess.Popen([sys.executable, os.path.abspath(__file__), "--c4-ref", ref_path,
+                                  "--c4-count", str(count)])
+    t_gen = time.perf_counter()
+    mats, kinds = [], []
+    for w in iter_c4_walks(count, C4_SEED):
+        mats.append(w["A"].astype(np.complex64))
+        kinds.append((w["kind"], w["laplacian"], w["t"]))
+    ns = [a.shape[0] for a in mats]
+    print(f"[bench] {count} walks generated in {time.perf_counter() - t_gen:.1f} s, "
+          f"sum N^3 = {sum(float(n) ** 3 for n in ns):.3e}", file=sys.stderr, flush=True)
+    ref, ref_error = None, None
+    if child is not None:
+        try:
+            rc = child.wait(timeout=900)
+        except subprocess.TimeoutExpired:
+            child.kill()
+            rc = "timeout"
+        if rc == 0 and os.path.exists(ref_path):
+            ref = dict(np.load(ref_path))
+            os.remove(ref_path)
+        else:
+            ref_error = f"reference failed ({rc})"
+        print(f"[bench] reference done: {ref_error or 'ok'}", file=sys.stderr, flush=True)
+
+    from fastctqw_amd.fastexpm import ExpMatBatch
+
+    uni, rnd = _c4_states(ns)
+    b = ExpMatBatch(nmax=max(ns), dtype=np.complex64, lanes=args.lanes)
+    b.upload(mats)
+    for _ in range(args.warmup):
+        b.run_resident(uni)
+        print(f"[bench] warmup {b.last_ms():.1f} ms", file=sys.stderr, flush=True)
+    ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        psi = b.run_resident(uni)
+        ms.append(b.last_ms())
+    wall = time.perf_counter() - t0
+    st = [b.stats(i) for i in range(count)]
+    dev = float(np.median(ms))
+    s_list = [x["s"] for x in st]
+    flops = sum(work_flops(n, s) for n, s in zip(ns, s_list))
+    res = {
+        "metric": f"batched walks/s ({count} uneven walks, N in [64, 2048])",
+        "value": count / (dev / 1000.0), "unit": "walks/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": dev, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c64 (fp32 mfma products, fp64 pade/solve, fp64 psi)",
+        "data": "synthetic: seeded d-regular / ER graphs, weights U(0,1], adjacency or Laplacian, t in [0.1,20]",
+        "config": {"workload": f"C4: {count} walks, U_i=expm(-i t_i H_i), psi_i=U_i psi0 (uniform)",
+                   "lanes": args.lanes, "sum_n3": float(sum(float(n) ** 3 for n in ns)),
+                   "s_histogram": {str(k): s_list.count(k) for k in sorted(set(s_list))},
+                   "structured_walks": int(sum(x["structured"] for x in st)),
+                   "model_tflops": flops / (dev * 1e-3) / 1e12,
+                   "wall_ms_per_step_incl_psi_copies": 1000.0 * wall / args.steps},
+    }
+    if ref is not None:
+        psi_r = b.run_resident(rnd)
+        inf_u = [_infidelity(ref[f"uni{i}"], psi[i]) for i in range(count)]
+        inf_r = [_infidelity(ref[f"rnd{i}"], psi_r[i]) for i in range(count)]
+        relf = {}
+        for i in C4_CHECK_U:
+            if i < count:
+                u = b.download(i).astype(np.complex128)
+                relf[str(i)] = float(np.linalg.norm(u - ref[f"U{i}"]) / np.linalg.norm(ref[f"U{i}"]))
+        worst = int(np.argmax(inf_r))
+        res["accuracy"] = {
+            "oracle": "FP64 eigendecomposition of G = Im(A) per walk (same rounded complex64 input)",
+            "worst_infidelity_uniform": float(max(inf_u)),
+            "worst_infidelity_random_state": float(max(inf_r)),
+            "median_infidelity_random_state": float(np.median(inf_r)),
+            "worst_walk": {"i": worst, "n": ns[worst], "kind": kinds[worst][0], "laplacian": kinds[worst][1],
+                           "t": kinds[worst][2], "s": s_list[worst]},
+            "U_relF": relf,
+        }
+    elif ref_error:
+        res["accuracy"] = {"oracle_error": ref_error}
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
+        res["cpu_baseline"] = c4_cpu_baseline(mats, s_list)
+    print(json.dumps(res), flush=True)
+
+
 def _infidelity(a, b):
     num = abs(np.vdot(a, b)) ** 2
     return float(max(0.0, 1.0 - num / (np.vdot(a, a).real * np.vdot(b, b).real)))
@@ -205,9 +370,23 @@ def main():
                     help="skip the FP64 eigendecomposition reference (torch.linalg.eigh, child process)")
     ap.add_argument("--eigh-ref", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--c5-ref", default=None, help=argparse.SUPPRESS)
-    ap.add_argument("--config", choices=["c2", "c5"], default="c2",
-                    help="c2: N=10^4 single expm (default); c5: 32 times x 128 states, N=8192")
+    ap.add_argument("--c4-ref", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--c4-count", type=int, default=C4_COUNT)
+    ap.add_argument("--lanes", type=int, default=4, help="C4: concurrent lanes (streams + host threads)")
+    ap.add_argument("--config", choices=["c2", "c4", "c5"], default="c2",
+                    help="c2: N=10^4 single expm (default); c4: 256 uneven walks; "
+                         "c5: 32 times x 128 states, N=8192")
     args = ap.parse_args()
+
+    if args.c4_ref:
+        _c4_reference(args.c4_ref, args.c4_count)
+        return
+    if args.config == "c4":
+        if args.gpus != 1:
+            print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}), flush=True)
+            sys.exit(2)
+        run_c4(args)
+        return
 
     if args.c5_ref:
         _c5_reference(args.n, 0.3, 5, 0.5, 128, 32, args.c5_ref)

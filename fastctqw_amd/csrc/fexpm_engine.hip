@@ -113,6 +113,7 @@ struct PipelineBase {
   int fp64_squarings = 0;  // FP32 engines: leading squarings done in FP64
   int pade_fp64 = 1;       // FP32 engines: Pade + solve in FP64 (companion FP64 pipeline)
   int panel_coop = 1;      // LU panels by one cooperative launch (0: one launch per column)
+  int spd_solve = 1;       // structured inputs: Cholesky solve (0: pivoted LU as for general A)
 };
 
 // ---------------------------------------------------------------------------
@@ -161,6 +162,47 @@ static double lu_gemm(const GemmArgs<double>& g, hipStream_t st) {
   gemm_nt_kernel<double, MODE_C4M, false><<<grid, dim3(NT), 0, st>>>(g);
   FX_LAUNCH();
   return 4.0 * 2.0 * (double)grid.x * grid.y * C::BM * C::BN * (double)g.k;
+}
+
+// FP64 product with the default tile shape; returns the real flops issued.
+template <int MODE, bool SYM>
+static double gemm64(const GemmArgs<double>& g, hipStream_t st) {
+  using C = GemmCfg<double>;
+  constexpr int NT = (C::BM / C::WM) * (C::BN / C::WN) * 64;
+  if (g.m <= 0 || g.n <= 0) return 0.0;
+  double tiles;
+  if (SYM) {
+    const int T = (g.m + C::BM - 1) / C::BM;
+    tiles = (double)(T * (T + 1) / 2);
+    gemm_nt_kernel<double, MODE, true><<<dim3(T * (T + 1) / 2), dim3(NT), 0, st>>>(g);
+  } else {
+    dim3 grid((g.n + C::BN - 1) / C::BN, (g.m + C::BM - 1) / C::BM);
+    tiles = (double)grid.x * grid.y;
+    gemm_nt_kernel<double, MODE, false><<<grid, dim3(NT), 0, st>>>(g);
+  }
+  FX_LAUNCH();
+  const double mult = MODE == MODE_REAL ? 1.0 : (MODE == MODE_C4M ? 4.0 : 3.0);
+  return mult * 2.0 * tiles * C::BM * C::BN * (double)g.k;
+}
+
+// Real FP64 NT product C = alpha A B^T + beta C on plain pointers.
+static double dgemm_real(const double* a, int64_t lda, const double* b, int64_t ldb, double* c, int64_t ldc, int m,
+                         int n, int k, double alpha, double beta, bool sym, hipStream_t st) {
+  GemmArgs<double> g;
+  std::memset(&g, 0, sizeof(g));
+  g.ar = a;
+  g.lda = lda;
+  g.br = b;
+  g.ldb = ldb;
+  g.m = m;
+  g.n = n;
+  g.k = k;
+  g.epi.cr = c;
+  g.epi.ldc = ldc;
+  g.epi.ld_add = ldc;
+  g.epi.alpha = alpha;
+  g.epi.beta = beta;
+  return sym ? gemm64<MODE_REAL, true>(g, st) : gemm64<MODE_REAL, false>(g, st);
 }
 
 // C[rows, cols] -= A[rows, k] * B, with B given transposed (bt: ncols x k, ldb).
@@ -315,8 +357,10 @@ template <typename W> struct Pipeline : PipelineBase {
       p->r = dalloc<W>(cnt);
       p->i = dalloc<W>(cnt);
     }
-    lr = dalloc<double>((size_t)npcap * ldl);
-    li = dalloc<double>((size_t)npcap * ldl);
+    // lr and li are one allocation (li = lr + npcap*ldl): the structured
+    // solve uses the 4 npcap^2 doubles as one contiguous work area.
+    lr = dalloc<double>((size_t)2 * npcap * ldl);
+    li = lr + (size_t)npcap * ldl;
     tr = dalloc<double>((size_t)2 * npcap * LU_OUTER);
     ti = dalloc<double>((size_t)2 * npcap * LU_OUTER);
     ipiv = dalloc<int>(npcap);
@@ -344,7 +388,6 @@ template <typename W> struct Pipeline : PipelineBase {
       dfree(p->i);
     }
     dfree(lr);
-    dfree(li);
     dfree(tr);
     dfree(ti);
     dfree(ipiv);
@@ -578,6 +621,8 @@ template <typename W> struct Pipeline : PipelineBase {
   // leaves Y = X - diag(X) in x0.  With q64 > 0 (structured path of an FP32
   // engine) the first q64 squarings run in FP64 on the free LU buffer first.
   int solve(bool structured, int q64, hipStream_t st) {
+    FX_HIP(hipMemsetAsync(&scal->info, 0, sizeof(int), st));
+    if (structured && spd_solve) return solve_spd(q64, st);
     const int inp = (int)np;
     {
       dim3 grid((unsigned)((2 * np + 255) / 256), (unsigned)np);
@@ -585,23 +630,123 @@ template <typename W> struct Pipeline : PipelineBase {
       FX_LAUNCH();
     }
     flops += lu_factor_solve(lr, li, ldl, inp, tr, ti, ipiv, cand, gmax, top, scal, inv, ctrs, st, panel_coop != 0);
+    if (q64 <= 0) return solution_tail(lr + inp, li + inp, ldl, lr, li, structured ? 1 : 0, 0, st);
+    return solution_tail(lr + inp, li + inp, ldl, lr, li, 1, q64, st);
+  }
+
+  // Structured solve by Cholesky (A = iC; V in v.r, U = i Uq with Uq in u.r).
+  // M = V - i Uq is normal, S = M^H M = V^2 + Uq^2 is symmetric positive
+  // definite, and with S = L L^T, Z = (V + i Uq) L^{-T}:
+  //     X = M^{-1} M^H = (V - U)^{-1} (V + U) = Z Z^T.
+  // Work area (leading dimension np, contiguous from lr): rows [0, np) Re Z,
+  // [np, 2np) Im Z, [2np, 3np) S -> L, [3np, 4np) V V^T; X = Z Z^T is
+  // written over S and V V^T.  Blocked right-looking Cholesky with 64-wide
+  // panels; the diagonal block and its inverse come from one workgroup,
+  // every other step is an FP64 MFMA product, and each panel also applies
+  // L^{-T} to the columns of Z (the stacked real and imaginary parts form a
+  // single real 2np x np right-hand side).
+  int solve_spd(int q64, hipStream_t st) {
+    const int64_t L = np;
+    const size_t P = (size_t)np * (size_t)np;
+    double* zb = lr;           // [Re Z; Im Z]
+    double* sm = lr + 2 * P;   // S, then L
+    double* vv = lr + 3 * P;   // V V^T
+    double* dinv = tr;         // inverses of the diagonal blocks, 64 x 64 each
+    const int ip = (int)np;
+    const int kpad = (int)round_up(n, 16);
+    {
+      dim3 grid((unsigned)((np + 255) / 256), (unsigned)(2 * np));
+      pack_stacked<W><<<grid, 256, 0, st>>>(v.r, u.r, ld, n, ip, zb, L);
+      FX_LAUNCH();
+    }
+    // S = Uq Uq^T + V V^T (both symmetric: products of commuting polynomials),
+    // over the padded order so that the padding rows and columns are zeros
+    flops += dgemm_real(zb, L, zb, L, vv, L, ip, ip, kpad, 1.0, 0.0, true, st);
+    {
+      GemmArgs<double> g;
+      std::memset(&g, 0, sizeof(g));
+      g.ar = zb + P;
+      g.lda = L;
+      g.br = zb + P;
+      g.ldb = L;
+      g.m = ip;
+      g.n = ip;
+      g.k = kpad;
+      g.epi.cr = sm;
+      g.epi.ldc = L;
+      g.epi.ld_add = L;
+      g.epi.alpha = 1.0;
+      g.epi.nadd = 1;
+      g.epi.addc[0] = 1.0;
+      g.epi.add_r[0] = vv;
+      flops += gemm64<MODE_REAL, true>(g, st);
+    }
+    for (int k0 = 0; k0 < ip; k0 += LU_NB) {
+      const int k1 = k0 + LU_NB;
+      const int rest = ip - k1;
+      double* dk = dinv + (size_t)(k0 / LU_NB) * LU_NB * LU_NB;
+      chol_diag<<<1, 256, 0, st>>>(sm, L, k0, n, dk, scal);
+      FX_LAUNCH();
+      double* panel = sm + (int64_t)k1 * L + k0;  // L[k1:, k0:k1]
+      if (rest > 0)
+        flops += dgemm_real(panel, L, dk, LU_NB, panel, L, rest, LU_NB, LU_NB, 1.0, 0.0, false, st);
+      // Z[:, k0:k1] <- Z[:, k0:k1] L_kk^{-T}
+      flops += dgemm_real(zb + k0, L, dk, LU_NB, zb + k0, L, 2 * ip, LU_NB, LU_NB, 1.0, 0.0, false, st);
+      if (rest > 0) {
+        // trailing S -= L21 L21^T (half tiles), Z[:, k1:] -= Z[:, k0:k1] L21^T
+        flops += dgemm_real(panel, L, panel, L, sm + (int64_t)k1 * L + k1, L, rest, rest, LU_NB, -1.0, 1.0, true,
+                            st);
+        flops += dgemm_real(zb + k0, L, panel, L, zb + k1, L, 2 * ip, rest, LU_NB, -1.0, 1.0, false, st);
+      }
+    }
+    // X = Z Z^T (3M, half tiles) over S and V V^T
+    double* xr = lr + 2 * P;
+    double* xi = lr + 3 * P;
+    {
+      GemmArgs<double> g;
+      std::memset(&g, 0, sizeof(g));
+      g.ar = zb;
+      g.ai = zb + P;
+      g.lda = L;
+      g.br = zb;
+      g.bi = zb + P;
+      g.ldb = L;
+      g.m = n;
+      g.n = n;
+      g.k = kpad;
+      g.epi.cr = xr;
+      g.epi.ci = xi;
+      g.epi.ldc = L;
+      g.epi.alpha = 1.0;
+      flops += gemm64<MODE_C3M, true>(g, st);
+    }
+    return solution_tail(xr, xi, L, zb, zb + P, 0, q64, st);
+  }
+
+  // X (FP64, xr/xi with leading dimension ldx) -> split form for the
+  // squarings: Y = X - diag(X) in x0, diag(X) in dsq[0] (index 0 returned).
+  // With q64 > 0 the first q64 squarings run in FP64 first: X is split into
+  // the free buffers wr/wi (same leading dimension), which ping-pong with
+  // xr/xi; the returned index names the diagonal buffer of the last result.
+  int solution_tail(double* xr, double* xi, int64_t ldx, double* wr, double* wi, int symmetrize, int q64,
+                    hipStream_t st) {
     if (q64 <= 0) {
       dim3 grid((n + 31) / 32, (n + 31) / 32);
-      solution_to_work<W><<<grid, 256, 0, st>>>(lr + inp, li + inp, ldl, x0.r, x0.i, ld, n, structured ? 1 : 0,
-                                                dsq[0]);
+      solution_to_work<W><<<grid, 256, 0, st>>>(xr, xi, ldx, x0.r, x0.i, ld, n, symmetrize, dsq[0]);
       FX_LAUNCH();
       return 0;
     }
+    const int64_t ldl = ldx;
     const int kpad = (int)round_up(n, 16);
     {
       dim3 grid((kpad + 31) / 32, (n + 31) / 32);
-      split_fp64<<<grid, 256, 0, st>>>(lr + inp, li + inp, ldl, lr, li, n, kpad, 1, dsq[0]);
+      split_fp64<<<grid, 256, 0, st>>>(xr, xi, ldl, wr, wi, n, kpad, symmetrize, dsq[0]);
       FX_LAUNCH();
     }
-    double* cr = lr;
-    double* ci = li;
-    double* nr = lr + inp;
-    double* ni = li + inp;
+    double* cr = wr;
+    double* ci = wi;
+    double* nr = xr;
+    double* ni = xi;
     int dc = 0;
     for (int q = 0; q < q64; ++q) {
       GemmArgs<double> g;
@@ -751,6 +896,7 @@ template <typename W> struct Pipeline : PipelineBase {
       Pipeline<double>* h = companion();
       h->flops = 0.0;
       h->panel_coop = panel_coop;
+      h->spd_solve = spd_solve;
       h->front_pade<Tin>(d_a, structured, m, mu_re, mu_im, scale, st);
       FX_HIP(hipEventRecord(ev[2], st));
       q64 = std::min(s_do, fp64_squarings);
@@ -1204,6 +1350,9 @@ int fexpm_set_option(fexpm_engine* e, int key, int64_t value) {
       return FEXPM_OK;
     case FEXPM_OPT_PANEL_COOP:
       if (e->pipe) e->pipe->panel_coop = value ? 1 : 0;
+      return FEXPM_OK;
+    case FEXPM_OPT_SPD_SOLVE:
+      if (e->pipe) e->pipe->spd_solve = value ? 1 : 0;
       return FEXPM_OK;
     case FEXPM_OPT_PADE_FP64:
       if (e->pipe) e->pipe->pade_fp64 = value ? 1 : 0;

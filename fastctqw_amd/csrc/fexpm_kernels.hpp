@@ -1191,4 +1191,92 @@ __global__ __launch_bounds__(256) void apply_states(const W* __restrict__ xr, co
   }
 }
 
+// ---------------------------------------------------------------------------
+// Structured solve (A = iC, C real symmetric).  With V and U = i Uq from the
+// real Pade products, M = V - i Uq is normal (V, Uq commute) and
+// S = M^H M = V^2 + Uq^2 is real symmetric positive definite.  For
+// S = L L^T and Z = (V + i Uq) L^{-T}:  X = M^{-1} M^H = Z Z^T.
+// The stacked real matrix [Re Z; Im Z] (2np x np) is solved in place.
+
+// big[r][j] = V[r][j] for r < np, Uq[r - np][j] for r >= np (zero padding).
+template <typename W>
+__global__ __launch_bounds__(256) void pack_stacked(const W* __restrict__ vr, const W* __restrict__ ur, int64_t ldw,
+                                                    int n, int np, double* __restrict__ big, int64_t ld) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  const int r = blockIdx.y;
+  if (j >= np) return;
+  const int i = r < np ? r : r - np;
+  const W* src = r < np ? vr : ur;
+  big[(int64_t)r * ld + j] = (i < n && j < n) ? (double)src[(int64_t)i * ldw + j] : 0.0;
+}
+
+// Cholesky factor of the 64 x 64 diagonal block at (k0, k0) of s (FP64,
+// lower triangle, in place) and its inverse into inv (64 x 64 row-major,
+// zero above the diagonal).  Rows and columns at or beyond n are padding
+// and are factored as the identity.  A pivot that is not positive sets bit
+// 0 of sc->info and is replaced by 1.
+__global__ __launch_bounds__(256) void chol_diag(double* __restrict__ s, int64_t ld, int k0, int n,
+                                                 double* __restrict__ inv, Scalars* sc) {
+  __shared__ double a[64][65];
+  __shared__ double ldg[64], rd[64];
+  const int t = threadIdx.x;
+  for (int e = t; e < 64 * 64; e += 256) {
+    const int i = e >> 6, j = e & 63;
+    double v;
+    if (k0 + i >= n || k0 + j >= n)
+      v = (i == j) ? 1.0 : 0.0;
+    else
+      v = s[(int64_t)(k0 + i) * ld + k0 + j];
+    a[i][j] = v;
+  }
+  __syncthreads();
+  for (int j = 0; j < 64; ++j) {
+    double d = a[j][j];
+    if (!(d > 0.0)) {
+      if (t == 0) atomicOr(&sc->info, 1);
+      d = 1.0;
+    }
+    d = sqrt(d);
+    const double r = 1.0 / d;
+    if (t == j) {
+      ldg[j] = d;
+      rd[j] = r;
+    }
+    if (t > j && t < 64) a[t][j] *= r;
+    __syncthreads();
+    const int m = 63 - j;
+    for (int e = t; e < m * m; e += 256) {
+      const int i = j + 1 + e / m, k = j + 1 + e % m;
+      if (k <= i) a[i][k] -= a[i][j] * a[k][j];
+    }
+    __syncthreads();
+  }
+  // factor back to s (lower triangle and diagonal)
+  for (int e = t; e < 64 * 64; e += 256) {
+    const int i = e >> 6, j = e & 63;
+    if (j <= i && k0 + i < n && k0 + j < n) s[(int64_t)(k0 + i) * ld + k0 + j] = (i == j) ? ldg[i] : a[i][j];
+  }
+  // inverse by forward substitution, column c = t / 4; the four lanes of a
+  // column split every inner product by k mod 4 and combine with shuffles.
+  const int c = t >> 2, q = t & 3;
+  double x[16];
+#pragma unroll
+  for (int kk = 0; kk < 16; ++kk) x[kk] = 0.0;
+#pragma unroll
+  for (int i = 0; i < 64; ++i) {
+    double p = 0.0;
+#pragma unroll
+    for (int kk = 0; kk < (i + 3) / 4; ++kk) {
+      const int k = 4 * kk + q;
+      if (k < i) p += a[i][k] * x[kk];
+    }
+    p += __shfl_xor(p, 1);
+    p += __shfl_xor(p, 2);
+    const double xi = ((i == c ? 1.0 : 0.0) - p) * rd[i];
+    if ((i & 3) == q) x[i >> 2] = xi;
+    if (q == 0) inv[i * 64 + c] = xi;
+  }
+}
+
+
 }  // namespace fexpm

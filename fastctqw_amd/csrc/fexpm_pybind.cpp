@@ -355,6 +355,8 @@ PYBIND11_MODULE(_fastexpm_core, m) {
   m.attr("OPT_KBLOCK") = (int)FEXPM_OPT_KBLOCK;
   m.attr("OPT_FP64_SQUARINGS") = (int)FEXPM_OPT_FP64_SQUARINGS;
   m.attr("OPT_PADE_FP64") = (int)FEXPM_OPT_PADE_FP64;
+  m.attr("OPT_PANEL_COOP") = (int)FEXPM_OPT_PANEL_COOP;
+  m.attr("OPT_SPD_SOLVE") = (int)FEXPM_OPT_SPD_SOLVE;
   m.attr("OPT_MAX_SQUARINGS") = (int)FEXPM_OPT_MAX_SQUARINGS;
   py::class_<ExpMatBatch>(m, "ExpMatBatch")
       .def(py::init<const std::string&, int64_t, int>(), py::arg("dtype"), py::arg("nmax"), py::arg("lanes") = 4)

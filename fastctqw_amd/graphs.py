@@ -189,7 +189,11 @@ def c4_walks(count: int = 256, seed: int = 4000, nmin: int = 64, nmax: int = 204
     or Laplacian at random; t ~ U[0.1, 20].  Returns a list of dicts with
     keys n, kind, laplacian, t, H (float64) and A = -1j*t*H (complex128).
     """
-    out = []
+    return list(iter_c4_walks(count, seed, nmin, nmax))
+
+
+def iter_c4_walks(count: int = 256, seed: int = 4000, nmin: int = 64, nmax: int = 2048):
+    """Generator form of :func:`c4_walks` (one instance in memory at a time)."""
     for i in range(count):
         rng = np.random.Generator(np.random.PCG64(seed + i))
         n = int(rng.integers(nmin, nmax + 1))
@@ -206,5 +210,4 @@ def c4_walks(count: int = 256, seed: int = 4000, nmin: int = 64, nmax: int = 204
             adj = erdos_renyi(n, p, seed=seed + i, weighted=True)
             kind = f"er(p={p:.3f})"
         h = laplacian(adj) if lap else adj
-        out.append({"n": n, "kind": kind, "laplacian": lap, "t": t, "H": h, "A": -1j * t * h})
-    return out
+        yield {"n": n, "kind": kind, "laplacian": lap, "t": t, "H": h, "A": -1j * t * h}

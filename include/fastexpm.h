@@ -60,8 +60,10 @@ enum fexpm_option {
   FEXPM_OPT_FP64_SQUARINGS = 4,  /* FP32 engines: number of leading squarings run in FP64 (default 0) */
   FEXPM_OPT_PADE_FP64 = 5,       /* FP32 engines: 1 = Pade products and solve in FP64 (default),
                                     0 = Pade products in FP32 */
-  FEXPM_OPT_PANEL_COOP = 6       /* LU panels: 1 = one cooperative launch per panel (default),
+  FEXPM_OPT_PANEL_COOP = 6,      /* LU panels: 1 = one cooperative launch per panel (default),
                                     0 = one launch per column (process-wide setting) */
+  FEXPM_OPT_SPD_SOLVE = 7        /* structured A = iG: 1 = real Cholesky solve of the Pade system
+                                    (default), 0 = pivoted complex LU as for general A */
 };
 
 /* Per-run report (filled by fexpm_get_stats after a run). */

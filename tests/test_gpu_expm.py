@@ -86,6 +86,31 @@ def test_ctqw_laplacian_er(fx, dt, n):
 
 
 @pytest.mark.parametrize("dt", [np.complex64, np.complex128])
+@pytest.mark.parametrize("n", [5, 63, 200, 520])
+def test_structured_cholesky_vs_lu(fx, dt, n):
+    """The structured solve (real Cholesky of S = V^2 + Uq^2, X = Z Z^T)
+    against the pivoted complex LU of the same Pade system (option
+    OPT_SPD_SOLVE = 0) and the eigen oracle."""
+    L = laplacian(erdos_renyi(n, 0.35, seed=300 + n, weighted=True))
+    t = 6.0
+    a = np.ascontiguousarray((-1j * t * L).astype(dt))
+    ref = eig_propagator(L, t)
+    out = {}
+    for spd in (1, 0):
+        _say(f"structured solve spd={spd} n={n} dtype={np.dtype(dt).name}")
+        with _engine(fx, dt, n) as e:
+            e._backend.set_option(fx._fastexpm_core.OPT_SPD_SOLVE, spd)
+            out[spd] = e.run(a)
+            st = e.stats()
+        assert st["structured"]
+        err = rel_fro(ref, out[spd])
+        print(f"  spd={spd} s={st['s']} relF={err:.3e}", flush=True)
+        assert err <= _tol(dt, st["s"])
+    diff = rel_fro(out[0], out[1])
+    assert diff <= _tol(dt, st["s"]), diff
+
+
+@pytest.mark.parametrize("dt", [np.complex64, np.complex128])
 @pytest.mark.parametrize("t", [0.001, 0.05, 0.3, 1.0, 4.5, 20.0])
 def test_ctqw_degrees(fx, dt, t):
     """Small t exercises Pade degrees 3..9 without scaling."""
