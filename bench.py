@@ -184,6 +184,134 @@ def run_c5(args):
     print(json.dumps(res), flush=True)
 
 
+C3_TIMES = (1.0, 4.5, 20.0)
+C3_PRECISIONS = ("native", "fp16x2", "bf16x3")
+C3_PROBES = 8
+
+
+def _c3_graph(n, weighted):
+    from fastctqw_amd.graphs import c2_laplacian
+
+    return c2_laplacian(n, 0.5, 2, weighted=weighted, dtype=np.float64)
+
+
+def _c3_probes(n):
+    """Gaussian probe columns Z (n x C3_PROBES): E||E Z||_F^2 = C3_PROBES ||E||_F^2."""
+    rng = np.random.Generator(np.random.PCG64(33))
+    return rng.standard_normal((n, C3_PROBES)) + 1j * rng.standard_normal((n, C3_PROBES))
+
+
+def _c3_reference(n, out_path):
+    """Child process: per graph (unweighted, weighted) and t, FP64 eigh of
+    G = Im(A) for the complex64 input A = -i t L, psi_ref for e_0 and a
+    random state, and U_ref Z for the probe columns (torch on the GPU)."""
+    import torch
+
+    from fastctqw_amd.graphs import random_state
+
+    res = {}
+    Z = torch.from_numpy(_c3_probes(n)).cuda()
+    psi_r = torch.from_numpy(random_state(n, seed=55)).cuda()
+    for weighted in (False, True):
+        L = _c3_graph(n, weighted)
+        for t in C3_TIMES:
+            a_im = (-1j * t * L).astype(np.complex64).imag
+            G = torch.from_numpy(np.ascontiguousarray(a_im, dtype=np.float64)).cuda()
+            lam, V = torch.linalg.eigh(G)
+            del G
+            Vc = V.to(torch.complex128)
+            del V
+            ph = torch.exp(1j * lam.to(torch.complex128))
+            key = f"{int(weighted)}_{t}"
+            res[f"e0_{key}"] = (Vc @ (ph * Vc[0, :])).cpu().numpy()
+            res[f"rand_{key}"] = (Vc @ (ph * (Vc.T @ psi_r))).cpu().numpy()
+            res[f"UZ_{key}"] = (Vc @ (ph[:, None] * (Vc.T @ Z))).cpu().numpy()
+            del Vc
+            torch.cuda.empty_cache()
+    np.savez(out_path, **res)
+
+
+def run_c3(args):
+    """Precision study at N=10^4: unweighted and weighted ER(N,0.5)
+    Laplacians, t in {1, 4.5, 20}, complex64; squarings in FP32 MFMA
+    (native) versus fp16x2 / bf16x3 split products with FP32 accumulation
+    (Pade products, solve and norm in FP64 in every mode).  Per case: device
+    ms per expm, infidelity of U e_0 and of U psi_rand, and ||U - U_ref||_F /
+    ||U_ref||_F estimated with Gaussian probes."""
+    import subprocess
+    import tempfile
+
+    from fastctqw_amd.fastexpm import ExpMatComplex64
+    from fastctqw_amd.graphs import random_state
+
+    n = args.n
+    ref, ref_error = None, None
+    if not args.no_accuracy:
+        path = os.path.join(tempfile.gettempdir(), f"fexpm_c3_{os.getpid()}.npz")
+        print("[bench] FP64 eigh references (child process) ...", file=sys.stderr, flush=True)
+        cp = subprocess.run([sys.executable, os.path.abspath(__file__), "--c3-ref", path, "--n", str(n)],
+                            timeout=1200)
+        if cp.returncode == 0 and os.path.exists(path):
+            ref = dict(np.load(path))
+            os.remove(path)
+        else:
+            ref_error = f"reference failed (exit code {cp.returncode})"
+    Z = _c3_probes(n)
+    psi_r = random_state(n, seed=55)
+    e0 = np.zeros(n, dtype=np.complex128)
+    e0[0] = 1.0
+    eng = ExpMatComplex64(n)
+    cases = []
+    for weighted in (False, True):
+        L = _c3_graph(n, weighted)
+        for t in C3_TIMES:
+            a = (-1j * t * L).astype(np.complex64)
+            eng.upload(a)
+            del a
+            key = f"{int(weighted)}_{t}"
+            for prec in C3_PRECISIONS:
+                eng.set_precision(prec)
+                for _ in range(args.warmup):
+                    eng.run_resident()
+                ms = []
+                for _ in range(args.steps):
+                    eng.run_resident()
+                    ms.append(eng.stats()["ms_total"])
+                st = eng.stats()
+                case = {"graph": "weighted" if weighted else "unweighted", "t": t, "precision": prec,
+                        "s": st["s"], "ms": float(np.median(ms)), "ms_square": st["ms_square"]}
+                if ref is not None:
+                    out = eng.apply(np.vstack([e0, psi_r, Z.T]))
+                    uz = out[2:].T
+                    case["infidelity_e0"] = _infidelity(ref[f"e0_{key}"], out[0])
+                    case["infidelity_random_state"] = _infidelity(ref[f"rand_{key}"], out[1])
+                    case["U_relF_probe"] = float(np.linalg.norm(uz - ref[f"UZ_{key}"]) /
+                                                 np.linalg.norm(ref[f"UZ_{key}"]))
+                cases.append(case)
+                print(f"[bench] {case}", file=sys.stderr, flush=True)
+        del L
+    eng.free()
+    head = [c for c in cases if c["graph"] == "unweighted" and c["t"] == 4.5]
+    native = next(c for c in head if c["precision"] == "native")
+    best = min(head, key=lambda c: c["ms"])
+    res = {
+        "metric": "s per N=10k complex expm, split-precision squarings (fastest mode at t=4.5, unweighted)",
+        "value": best["ms"] / 1000.0, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": best["ms"], "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": f"c64 ({best['precision']} squarings, fp64 pade/solve)",
+        "data": "synthetic: seeded ER(10000,0.5) Laplacian, unweighted and weighted U(0,1]",
+        "config": {"workload": "C3: expm(-i t L), t in {1, 4.5, 20}, precision native/fp16x2/bf16x3",
+                   "n": n, "native_ms_t4.5": native["ms"], "speedup_vs_native": native["ms"] / best["ms"]},
+        "cases": cases,
+    }
+    if ref_error:
+        res["accuracy"] = {"oracle_error": ref_error}
+    else:
+        res["accuracy"] = {"oracle": "FP64 eigendecomposition of G = Im(A) per (graph, t), same rounded input; "
+                                     f"U_relF_probe from {C3_PROBES} Gaussian probe columns"}
+    print(json.dumps(res), flush=True)
+
+
 C4_COUNT, C4_SEED = 256, 4000
 C4_CHECK_U = (0, 1, 2, 3, 64, 65, 128, 129)  # walks whose full U is compared with U_ref
 
@@ -370,13 +498,26 @@ def main():
                     help="skip the FP64 eigendecomposition reference (torch.linalg.eigh, child process)")
     ap.add_argument("--eigh-ref", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--c5-ref", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--c3-ref", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--c4-ref", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--c4-count", type=int, default=C4_COUNT)
     ap.add_argument("--lanes", type=int, default=4, help="C4: concurrent lanes (streams + host threads)")
-    ap.add_argument("--config", choices=["c2", "c4", "c5"], default="c2",
-                    help="c2: N=10^4 single expm (default); c4: 256 uneven walks; "
-                         "c5: 32 times x 128 states, N=8192")
+    ap.add_argument("--config", choices=["c2", "c3", "c4", "c5"], default="c2",
+                    help="c2: N=10^4 single expm (default); c3: precision study at N=10^4; "
+                         "c4: 256 uneven walks; c5: 32 times x 128 states, N=8192")
+    ap.add_argument("--precision", choices=list(C3_PRECISIONS), default="native",
+                    help="c2: squaring precision of the complex64 engine")
     args = ap.parse_args()
+
+    if args.c3_ref:
+        _c3_reference(args.n, args.c3_ref)
+        return
+    if args.config == "c3":
+        if args.gpus != 1:
+            print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}), flush=True)
+            sys.exit(2)
+        run_c3(args)
+        return
 
     if args.c4_ref:
         _c4_reference(args.c4_ref, args.c4_count)
@@ -441,6 +582,8 @@ def main():
     print(f"[bench] input ready in {time.perf_counter() - t_gen:.1f} s", file=sys.stderr, flush=True)
 
     eng = (ExpMatComplex64 if cdt == np.complex64 else ExpMatComplex128)(args.n)
+    if args.precision != "native":
+        eng.set_precision(args.precision)
     eng.upload(a)
     for _ in range(args.warmup):
         eng.run_resident()
@@ -503,7 +646,8 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64 (fp32 mfma, fp64 solve)" if cdt == np.complex64 else "c128 (fp64)",
+        "dtype": (f"c64 ({'fp32 mfma' if args.precision == 'native' else args.precision} squarings, fp64 solve)"
+                  if cdt == np.complex64 else "c128 (fp64)"),
         "data": "synthetic: seeded ER(N,p) Laplacian, A=-i*t*L",
         "config": {"workload": f"expm(-i*{args.t}*L), L Laplacian of ER({n},{args.p}) seed {args.seed}, "
                                f"{args.dtype}, full U resident", "n": n, "t": args.t,

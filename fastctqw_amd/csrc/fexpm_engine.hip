@@ -35,6 +35,7 @@
 #include "../../include/fastexpm_testing.h"
 #include "fexpm_gemm.hpp"
 #include "fexpm_kernels.hpp"
+#include "fexpm_split.hpp"
 
 namespace fexpm {
 
@@ -114,6 +115,7 @@ struct PipelineBase {
   int pade_fp64 = 1;       // FP32 engines: Pade + solve in FP64 (companion FP64 pipeline)
   int panel_coop = 1;      // LU panels by one cooperative launch (0: one launch per column)
   int spd_solve = 1;       // structured inputs: Cholesky solve (0: pivoted LU as for general A)
+  int precision = 0;       // enum fexpm_precision for the structured squarings of FP32 engines
 };
 
 // ---------------------------------------------------------------------------
@@ -338,6 +340,8 @@ template <typename W> struct Pipeline : PipelineBase {
   double2* dsq[2] = {nullptr, nullptr};  // split-form diagonals (squarings)
   int* flags = nullptr;
   Scalars* scal = nullptr;
+  void* sbuf = nullptr;  // k-blocked 16-bit split parts (split-precision squarings)
+  size_t sbuf_bytes = 0;
   hipEvent_t ev[8];
   bool events = false;
   // result of the last run (for apply)
@@ -407,6 +411,7 @@ template <typename W> struct Pipeline : PipelineBase {
     dfree(norm0);
     dfree(flags);
     dfree(scal);
+    if (sbuf) (void)hipFree(sbuf);
     if (events)
       for (int k = 0; k < 8; ++k) (void)hipEventDestroy(ev[k]);
   }
@@ -779,7 +784,56 @@ template <typename W> struct Pipeline : PipelineBase {
   // keeps the split form; the diagonal is merged back at the end.
   // s squarings of the split form (Y in x0, d in dsq[dc]); returns the plane
   // holding the last Y and the index of its diagonal buffer.
+  // Split-precision squarings (FP32 engines, structured path): each step
+  // splits the current Y into 16-bit parts and squares on 16-bit MFMA.
+  template <class P>
+  std::pair<Plane2<W>*, int> square_split16(int s, int dc, hipStream_t st) {
+    Plane2<W>* cur = &x0;
+    Plane2<W>* nxt = &x1;
+    const int rows = (int)np;
+    const int kpad = (int)round_up(n, 16);
+    const size_t need = (size_t)npcap * (size_t)round_up(ncap, 16) * 6 * 2;  // room for either split
+    if (!sbuf || sbuf_bytes < need) {
+      if (sbuf) FX_HIP(hipFree(sbuf));
+      sbuf = nullptr;
+      FX_HIP(hipMalloc(&sbuf, need));
+      sbuf_bytes = need;
+    }
+    const int T = (n + SQ_BM - 1) / SQ_BM;
+    const int nb = T * (T + 1) / 2;
+    for (int q = 0; q < s; ++q) {
+      dim3 sg((unsigned)((kpad / 16 + 15) / 16), (unsigned)((rows + 15) / 16));
+      split_blocked<P><<<sg, 256, 0, st>>>((const float*)cur->r, (const float*)cur->i, ld, n, n, rows, kpad,
+                                          (typename P::H*)sbuf, &scal->info);
+      FX_LAUNCH();
+      SqArgs<P> a;
+      a.a = (const typename P::H*)sbuf;
+      a.rows = rows;
+      a.yr = (const float*)cur->r;
+      a.yi = (const float*)cur->i;
+      a.ldy = ld;
+      a.cr = (float*)nxt->r;
+      a.ci = (float*)nxt->i;
+      a.ldc = ld;
+      a.d_in = dsq[dc];
+      a.d_out = dsq[dc ^ 1];
+      a.m = n;
+      a.k = kpad;
+      a.kblock = kblock > 0 ? kblock : (1 << 30);
+      sq_split_kernel<P><<<dim3(nb), dim3(sq_threads<P>()), 0, st>>>(a);
+      FX_LAUNCH();
+      flops += 4.0 * P::NPAIR * 2.0 * nb * (double)SQ_BM * SQ_BM * (double)kpad;
+      std::swap(cur, nxt);
+      dc ^= 1;
+    }
+    return {cur, dc};
+  }
+
   std::pair<Plane2<W>*, int> square_split(int s, bool structured, int dc, hipStream_t st) {
+    if (std::is_same<W, float>::value && structured && precision == FEXPM_PREC_FP16X2)
+      return square_split16<Fp16x2>(s, dc, st);
+    if (std::is_same<W, float>::value && structured && precision == FEXPM_PREC_BF16X3)
+      return square_split16<Bf16x3>(s, dc, st);
     Plane2<W>* cur = &x0;
     Plane2<W>* nxt = &x1;
     for (int q = 0; q < s; ++q) {
@@ -930,6 +984,14 @@ template <typename W> struct Pipeline : PipelineBase {
     FX_HIP(hipEventSynchronize(ev[5]));
     Scalars hs2;
     FX_HIP(hipMemcpy(&hs2, hybrid ? companion_scal() : scal, sizeof(Scalars), hipMemcpyDeviceToHost));
+    const bool split16 = std::is_same<W, float>::value && structured && precision != FEXPM_PREC_NATIVE;
+    int split_info = 0;
+    if (split16) {
+      Scalars hs3;
+      FX_HIP(hipMemcpy(&hs3, scal, sizeof(Scalars), hipMemcpyDeviceToHost));
+      split_info = hs3.info & INFO_SPLIT_RANGE;
+      hs2.info &= ~INFO_SPLIT_RANGE;
+    }
     res = *X;
     have_result = true;
     last_structured = structured;
@@ -956,6 +1018,9 @@ template <typename W> struct Pipeline : PipelineBase {
     S.s_fp64 = q64;
     S.pade_fp64 = hybrid ? 1 : 0;
     S.kblock = kblock;
+    S.precision = split16 ? precision : FEXPM_PREC_NATIVE;
+    if (split_info)
+      throw Error(FEXPM_EINVAL, "split-precision squaring: a scaled operand value left the 16-bit range");
     if (hs2.info & 2) throw Error(FEXPM_EHIP, "LU panel grid barrier timed out");
     if (hs2.info != 0) throw Error(FEXPM_ESINGULAR, "Pade denominator is exactly singular");
   }
@@ -1329,11 +1394,16 @@ int fexpm_set_option(fexpm_engine* e, int key, int64_t value) {
   std::lock_guard<std::mutex> lk(e->mtx);
   switch (key) {
     case FEXPM_OPT_PRECISION:
-      if (value != FEXPM_PREC_NATIVE) {
-        e->err = "unsupported precision mode";
+      if (value != FEXPM_PREC_NATIVE && value != FEXPM_PREC_FP16X2 && value != FEXPM_PREC_BF16X3) {
+        e->err = "unknown precision mode";
+        return FEXPM_EINVAL;
+      }
+      if (value != FEXPM_PREC_NATIVE && e->dtype != FEXPM_COMPLEX64) {
+        e->err = "split precision applies to complex64 engines";
         return FEXPM_EINVAL;
       }
       e->opt_precision = (int)value;
+      if (e->pipe) e->pipe->precision = (int)value;
       return FEXPM_OK;
     case FEXPM_OPT_STRUCTURE:
       e->opt_structure = value ? 1 : 0;
@@ -1686,19 +1756,79 @@ void test_gemm_w(int mode, int sym, int m, int n, int k, const double* ar, const
   else throw Error(FEXPM_EINVAL, "unsupported gemm variant");
 }
 
+// C = A A^T (complex, m x m from m x k) through the split-precision
+// squaring kernel with a zero diagonal split; the diagonal of the product
+// comes back through d_out.
+template <class P>
+void test_split_t(int m, int k, const double* ar, const double* ai, double* cr, double* ci) {
+  const int kpad = (int)round_up(k, 16), rows = (int)round_up(m, SQ_BM);
+  const int64_t ld = std::max(kpad, rows);
+  std::vector<float> hr((size_t)rows * ld, 0.0f), hi((size_t)rows * ld, 0.0f);
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < k; ++j) {
+      hr[(size_t)i * ld + j] = (float)ar[(size_t)i * k + j];
+      hi[(size_t)i * ld + j] = (float)ai[(size_t)i * k + j];
+    }
+  DevPlane<float> yr(hr.size()), yi(hi.size()), outr((size_t)rows * ld), outi((size_t)rows * ld);
+  DevPlane<double2> din(rows), dout(rows);
+  DevPlane<typename P::H> parts((size_t)rows * kpad * 2 * P::NQ);
+  DevPlane<int> info(1);
+  FX_HIP(hipMemcpy(yr.p, hr.data(), hr.size() * 4, hipMemcpyHostToDevice));
+  FX_HIP(hipMemcpy(yi.p, hi.data(), hi.size() * 4, hipMemcpyHostToDevice));
+  FX_HIP(hipMemset(din.p, 0, rows * sizeof(double2)));
+  FX_HIP(hipMemset(info.p, 0, sizeof(int)));
+  dim3 sg((unsigned)((kpad / 16 + 15) / 16), (unsigned)((rows + 15) / 16));
+  split_blocked<P><<<sg, 256>>>(yr.p, yi.p, ld, m, k, rows, kpad, parts.p, info.p);
+  FX_LAUNCH();
+  SqArgs<P> a;
+  a.a = parts.p;
+  a.rows = rows;
+  a.yr = yr.p;
+  a.yi = yi.p;
+  a.ldy = ld;
+  a.cr = outr.p;
+  a.ci = outi.p;
+  a.ldc = ld;
+  a.d_in = din.p;
+  a.d_out = dout.p;
+  a.m = m;
+  a.k = kpad;
+  a.kblock = 8;
+  const int T = (m + SQ_BM - 1) / SQ_BM;
+  sq_split_kernel<P><<<dim3(T * (T + 1) / 2), dim3(sq_threads<P>())>>>(a);
+  FX_LAUNCH();
+  FX_HIP(hipDeviceSynchronize());
+  int hinfo = 0;
+  FX_HIP(hipMemcpy(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost));
+  if (hinfo) throw Error(FEXPM_EINVAL, "operand outside the 16-bit range");
+  std::vector<float> orr((size_t)rows * ld), oi((size_t)rows * ld);
+  std::vector<double2> dd(rows);
+  FX_HIP(hipMemcpy(orr.data(), outr.p, orr.size() * 4, hipMemcpyDeviceToHost));
+  FX_HIP(hipMemcpy(oi.data(), outi.p, oi.size() * 4, hipMemcpyDeviceToHost));
+  FX_HIP(hipMemcpy(dd.data(), dout.p, rows * sizeof(double2), hipMemcpyDeviceToHost));
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < m; ++j) {
+      cr[(size_t)i * m + j] = i == j ? dd[i].x : (double)orr[(size_t)i * ld + j];
+      ci[(size_t)i * m + j] = i == j ? dd[i].y : (double)oi[(size_t)i * ld + j];
+    }
+}
+
 }  // namespace
 
 extern "C" {
 
 int fexpm_test_gemm(int wtype, int mode, int sym, int m, int n, int k, const double* ar, const double* ai,
                     const double* br, const double* bi, double* cr, double* ci) {
-  if (m <= 0 || n <= 0 || k <= 0 || !ar || !br || !cr || (wtype != 0 && wtype != 1)) return FEXPM_EINVAL;
+  if (m <= 0 || n <= 0 || k <= 0 || !ar || !br || !cr || wtype < 0 || wtype > 3) return FEXPM_EINVAL;
   if (mode != MODE_REAL && (!ai || !bi || !ci)) return FEXPM_EINVAL;
   if (sym && m != n) return FEXPM_EINVAL;
+  if (wtype >= 2 && (mode != MODE_C3M || !sym)) return FEXPM_EINVAL;
   return guarded(nullptr, [&] {
     check_device();
     if (wtype == 0) test_gemm_w<float>(mode, sym, m, n, k, ar, ai, br, bi, cr, ci);
-    else test_gemm_w<double>(mode, sym, m, n, k, ar, ai, br, bi, cr, ci);
+    else if (wtype == 1) test_gemm_w<double>(mode, sym, m, n, k, ar, ai, br, bi, cr, ci);
+    else if (wtype == 2) test_split_t<Fp16x2>(m, k, ar, ai, cr, ci);
+    else test_split_t<Bf16x3>(m, k, ar, ai, cr, ci);
   });
 }
 
@@ -1829,6 +1959,72 @@ double time_gemm_t(int m, int n, int k, int reps) {
   return (double)ms / std::max(reps, 1);
 }
 
+template <class P>
+__global__ void fill_pattern16(typename P::H* p, size_t cnt, uint32_t seed) {
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= cnt) return;
+  uint32_t h = (uint32_t)(k * 2654435761u) ^ seed;
+  h ^= h >> 15;
+  h *= 2246822519u;
+  h ^= h >> 13;
+  p[k] = (typename P::H)((float)(h & 0xFFFFFF) / 16777216.0f - 0.5f);
+}
+
+// Split-precision squaring kernel alone (parts already in the k-blocked layout).
+template <class P, int WM, int WN>
+double time_split_t(int m, int k, int reps) {
+  const int kpad = (int)round_up(std::max(k, 1), 16), rows = (int)round_up(m, SQ_BM);
+  const size_t cp = (size_t)rows * kpad * 2 * P::NQ, cc = (size_t)rows * rows;
+  typename P::H* parts;
+  float *yr, *yi, *cr, *ci;
+  double2 *din, *dout;
+  FX_HIP(hipMalloc(&parts, cp * sizeof(typename P::H)));
+  FX_HIP(hipMalloc(&yr, cc * sizeof(float)));
+  FX_HIP(hipMalloc(&yi, cc * sizeof(float)));
+  FX_HIP(hipMalloc(&cr, cc * sizeof(float)));
+  FX_HIP(hipMalloc(&ci, cc * sizeof(float)));
+  FX_HIP(hipMalloc(&din, rows * sizeof(double2)));
+  FX_HIP(hipMalloc(&dout, rows * sizeof(double2)));
+  fill_pattern16<P><<<(unsigned)((cp + 255) / 256), 256>>>(parts, cp, 7u);
+  fill_pattern<float><<<(unsigned)((cc + 255) / 256), 256>>>(yr, cc, 8u);
+  fill_pattern<float><<<(unsigned)((cc + 255) / 256), 256>>>(yi, cc, 9u);
+  FX_HIP(hipMemset(din, 0, rows * sizeof(double2)));
+  SqArgs<P> a;
+  a.a = parts;
+  a.rows = rows;
+  a.yr = yr;
+  a.yi = yi;
+  a.ldy = rows;
+  a.cr = cr;
+  a.ci = ci;
+  a.ldc = rows;
+  a.d_in = din;
+  a.d_out = dout;
+  a.m = m;
+  a.k = kpad;
+  a.kblock = 8;
+  const int T = (m + SQ_BM - 1) / SQ_BM;
+  dim3 grid(T * (T + 1) / 2), block(64 * WM * WN);
+  hipEvent_t e0, e1;
+  FX_HIP(hipEventCreate(&e0));
+  FX_HIP(hipEventCreate(&e1));
+  sq_split_kernel<P, WM, WN><<<grid, block>>>(a);
+  FX_LAUNCH();
+  FX_HIP(hipEventRecord(e0));
+  for (int r = 0; r < reps; ++r) sq_split_kernel<P, WM, WN><<<grid, block>>>(a);
+  FX_HIP(hipEventRecord(e1));
+  FX_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  FX_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(parts);
+  for (float* p : {yr, yi, cr, ci}) (void)hipFree(p);
+  (void)hipFree(din);
+  (void)hipFree(dout);
+  return (double)ms / std::max(reps, 1);
+}
+
 }  // namespace
 
 extern "C" int fexpm_test_gemm_time(int wtype, int mode, int sym, int cfg, int m, int n, int k, int reps,
@@ -1837,7 +2033,19 @@ extern "C" int fexpm_test_gemm_time(int wtype, int mode, int sym, int cfg, int m
   return guarded(nullptr, [&] {
     check_device();
     double t = -1.0;
-    if (wtype == 0) {
+    if (wtype >= 2) {
+      // cfg: 0 = 8 waves 2x4 (default), 1 = 4 waves 2x2, 2 = 8 waves 4x2, 3 = 16 waves 4x4 (fp16x2)
+      if (mode == 2 && sym && wtype == 2) {
+        if (cfg == 0) t = time_split_t<Fp16x2, 2, 4>(m, k, reps);
+        else if (cfg == 1) t = time_split_t<Fp16x2, 2, 2>(m, k, reps);
+        else if (cfg == 2) t = time_split_t<Fp16x2, 4, 2>(m, k, reps);
+        else if (cfg == 3) t = time_split_t<Fp16x2, 4, 4>(m, k, reps);
+      } else if (mode == 2 && sym && wtype == 3) {
+        if (cfg == 0) t = time_split_t<Bf16x3, 2, 4>(m, k, reps);
+        else if (cfg == 1) t = time_split_t<Bf16x3, 2, 2>(m, k, reps);
+        else if (cfg == 2) t = time_split_t<Bf16x3, 4, 2>(m, k, reps);
+      }
+    } else if (wtype == 0) {
       if (mode == 2 && sym) t = time_gemm_t<float, MODE_C3M, true, true, GemmCfg<float>>(m, n, k, reps);
       else if (mode == 2) t = time_gemm_t<float, MODE_C3M, false, true, GemmCfg<float>>(m, n, k, reps);
       else if (mode == 0 && sym) t = time_gemm_t<float, MODE_REAL, true, false, GemmCfg<float>>(m, n, k, reps);

@@ -186,6 +186,7 @@ template <typename T, int DT> class ExpMat {
     d["s_fp64"] = s.s_fp64;
     d["kblock"] = s.kblock;
     d["pade_fp64"] = (bool)s.pade_fp64;
+    d["precision"] = s.precision == FEXPM_PREC_FP16X2 ? "fp16x2" : (s.precision == FEXPM_PREC_BF16X3 ? "bf16x3" : "native");
     d["ms_propagate"] = s.ms_propagate;
     return d;
   }
@@ -358,6 +359,9 @@ PYBIND11_MODULE(_fastexpm_core, m) {
   m.attr("OPT_PANEL_COOP") = (int)FEXPM_OPT_PANEL_COOP;
   m.attr("OPT_SPD_SOLVE") = (int)FEXPM_OPT_SPD_SOLVE;
   m.attr("OPT_MAX_SQUARINGS") = (int)FEXPM_OPT_MAX_SQUARINGS;
+  m.attr("PREC_NATIVE") = (int)FEXPM_PREC_NATIVE;
+  m.attr("PREC_FP16X2") = (int)FEXPM_PREC_FP16X2;
+  m.attr("PREC_BF16X3") = (int)FEXPM_PREC_BF16X3;
   py::class_<ExpMatBatch>(m, "ExpMatBatch")
       .def(py::init<const std::string&, int64_t, int>(), py::arg("dtype"), py::arg("nmax"), py::arg("lanes") = 4)
       .def("upload", &ExpMatBatch::upload, py::arg("mats"))

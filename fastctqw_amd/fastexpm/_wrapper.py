@@ -83,6 +83,16 @@ class _ExpMatBase:
     def set_structure_detection(self, enabled: bool) -> None:
         self._backend.set_option(_fastexpm_core.OPT_STRUCTURE, 1 if enabled else 0)
 
+    _PRECISIONS = {"native": "PREC_NATIVE", "fp16x2": "PREC_FP16X2", "bf16x3": "PREC_BF16X3"}
+
+    def set_precision(self, mode: str) -> None:
+        """Squaring precision of a complex64 engine on structured (CTQW)
+        inputs: "native" (FP32 MFMA), "fp16x2" or "bf16x3" (split 16-bit
+        products with FP32 accumulation).  No counterpart in the reference."""
+        if mode not in self._PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(self._PRECISIONS)}")
+        self._backend.set_option(_fastexpm_core.OPT_PRECISION, getattr(_fastexpm_core, self._PRECISIONS[mode]))
+
 
 class ExpMatFloat32(_ExpMatBase):
     _core_cls = _fastexpm_core.ExpMatFloat32

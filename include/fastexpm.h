@@ -49,7 +49,11 @@ enum fexpm_status {
 
 /* Precision of the Pade products and squarings (option FEXPM_OPT_PRECISION). */
 enum fexpm_precision {
-  FEXPM_PREC_NATIVE = 0 /* FP32 MFMA for complex64/float32, FP64 for 128/64 */
+  FEXPM_PREC_NATIVE = 0, /* FP32 MFMA for complex64/float32, FP64 for 128/64 */
+  FEXPM_PREC_FP16X2 = 1, /* complex64 engines, structured squarings: fp16 hi/lo parts
+                            (scale 2^14), 3 products on 16-bit MFMA, FP32 accumulation */
+  FEXPM_PREC_BF16X3 = 2  /* complex64 engines, structured squarings: three bf16 parts,
+                            6 products on 16-bit MFMA, FP32 accumulation */
 };
 
 enum fexpm_option {
@@ -89,6 +93,7 @@ typedef struct fexpm_stats {
   int32_t kblock;       /* accumulation block of the FP32 squarings (16-k tiles) */
   int32_t pade_fp64;    /* 1: Pade products and solve ran in FP64 */
   double ms_propagate;  /* device time of the last fexpm_propagate (ms) */
+  int32_t precision;    /* enum fexpm_precision used by the squarings of this run */
 } fexpm_stats;
 
 /* Create an engine for n x n matrices of the given dtype and allocate its

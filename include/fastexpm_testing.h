@@ -40,8 +40,11 @@ enum fexpm_plane {
 };
 
 /* C = A B^T (mode 0 real, 1 complex 4M, 2 complex 3M), sym != 0: symmetric
- * half-tile output (requires m == n).  wtype 0 = FP32 MFMA, 1 = FP64 MFMA.
- * A is m x k, B is n x k, C is m x n; imaginary pointers unused for mode 0. */
+ * half-tile output (requires m == n).  wtype 0 = FP32 MFMA, 1 = FP64 MFMA,
+ * 2 = fp16x2 and 3 = bf16x3 split-precision squaring kernel (mode 2, sym,
+ * B = A, complex 4M on 16-bit MFMA; |a| must stay below 65504 / 2^14 for
+ * fp16x2).  A is m x k, B is n x k, C is m x n; imaginary pointers unused
+ * for mode 0. */
 int fexpm_test_gemm(int wtype, int mode, int sym, int m, int n, int k, const double* ar, const double* ai,
                     const double* br, const double* bi, double* cr, double* ci);
 
@@ -55,7 +58,8 @@ int fexpm_debug_plane(fexpm_engine* e, int which, double* re, double* im);
 /* Average device time (ms) of one GEMM launch over reps launches on
  * device-resident operands (m x k, n x k -> m x n); cfg selects an FP64
  * tiling (0 4-wave 32x32, 1 8-wave 32x16 = default, 2 128x64, 3 64x128;
- * 2 and 3 are not square and are invalid with sym). */
+ * 2 and 3 are not square and are invalid with sym).  wtype 2 / 3 time the
+ * split-precision squaring kernel alone (mode 2, sym). */
 int fexpm_test_gemm_time(int wtype, int mode, int sym, int cfg, int m, int n, int k, int reps, double* ms);
 
 #ifdef __cplusplus

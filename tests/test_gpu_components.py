@@ -99,6 +99,37 @@ def test_gemm_symmetric(lib, wtype, n):
     assert np.array_equal(c, c.T)
 
 
+@pytest.mark.parametrize("wtype", [2, 3])
+@pytest.mark.parametrize("shape", [(16, 16), (100, 100), (128, 128), (129, 129), (300, 300), (200, 77)])
+def test_gemm_split_squaring(lib, wtype, shape):
+    """Split-precision X X^T (wtype 2: fp16 hi/lo, 3 products; 3: three bf16
+    parts, 6 products; FP32 accumulation) for complex X with |x| <= 1, the
+    operand range of the squaring, against FP64 numpy; the FP32 3M kernel on
+    the same input is printed for comparison."""
+    m, k = shape
+    _say(f"split squaring w={wtype} {shape}")
+    rng = np.random.default_rng(m * 5 + k + wtype)
+    x = (rng.uniform(-1, 1, (m, k)) + 1j * rng.uniform(-1, 1, (m, k))) / np.sqrt(2)
+    c = _gemm(lib, wtype, 2, 1, x, x)
+    ref = x @ x.T
+    err = rel_fro(ref, c)
+    err32 = rel_fro(ref, _gemm(lib, 0, 2, 0, x, x))
+    print(f"  relF={err:.2e} (fp32 3M: {err32:.2e})", flush=True)
+    assert err < 4e-6
+    assert np.array_equal(c, c.T)
+
+
+def test_gemm_split_range_error(lib):
+    """fp16 parts hold |x| * 2^14 <= 65504: a larger operand is reported."""
+    _say("split squaring range error")
+    x = np.full((32, 32), 8.0 + 0j)
+    m = x.shape[0]
+    ar, ai = np.ascontiguousarray(x.real), np.ascontiguousarray(x.imag)
+    cr, ci = np.zeros((m, m)), np.zeros((m, m))
+    rc = lib.fexpm_test_gemm(2, 2, 1, m, m, m, _p(ar), _p(ai), _p(ar), _p(ai), _p(cr), _p(ci))
+    assert rc != 0 and b"range" in lib.fexpm_last_error(None)
+
+
 def _solve(lib, m, b):
     n = m.shape[0]
     mr, mi = np.ascontiguousarray(m.real), np.ascontiguousarray(m.imag)

@@ -110,6 +110,45 @@ def test_structured_cholesky_vs_lu(fx, dt, n):
     assert diff <= _tol(dt, st["s"]), diff
 
 
+@pytest.mark.parametrize("prec", ["fp16x2", "bf16x3"])
+@pytest.mark.parametrize("n", [64, 300, 700])
+def test_split_precision_ctqw(fx, prec, n):
+    """Structured squarings in split 16-bit precision against the eigen
+    oracle (same tolerance as FP32) and against the native FP32 engine."""
+    L = laplacian(erdos_renyi(n, 0.3, seed=500 + n, weighted=True))
+    t = 5.0
+    a = np.ascontiguousarray((-1j * t * L).astype(np.complex64))
+    ref = eig_propagator(L, t)
+    _say(f"split precision {prec} n={n}")
+    with fx.ExpMatComplex64(n) as e:
+        u_native = e.run(a)
+        e.set_precision(prec)
+        u = e.run(a)
+        st = e.stats()
+    assert st["precision"] == prec and st["structured"]
+    err, err_native = rel_fro(ref, u), rel_fro(ref, u_native)
+    print(f"  s={st['s']} relF={err:.3e} (native {err_native:.3e})", flush=True)
+    assert err <= _tol(np.complex64, st["s"])
+    rng = np.random.default_rng(n)
+    psi0 = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    psi0 /= np.linalg.norm(psi0)
+    assert infidelity(ref @ psi0, u.astype(np.complex128) @ psi0) < 1e-9
+
+
+def test_split_precision_options(fx):
+    _say("split precision options")
+    with fx.ExpMatComplex128(8) as e:
+        with pytest.raises(Exception):
+            e.set_precision("fp16x2")
+    with fx.ExpMatComplex64(8) as e:
+        with pytest.raises(ValueError):
+            e.set_precision("fp8")
+        e.set_precision("bf16x3")
+        e.set_precision("native")
+        e.run(np.zeros((8, 8), np.complex64))
+        assert e.stats()["precision"] == "native"
+
+
 @pytest.mark.parametrize("dt", [np.complex64, np.complex128])
 @pytest.mark.parametrize("t", [0.001, 0.05, 0.3, 1.0, 4.5, 20.0])
 def test_ctqw_degrees(fx, dt, t):
