@@ -507,6 +507,8 @@ def main():
                          "c4: 256 uneven walks; c5: 32 times x 128 states, N=8192")
     ap.add_argument("--precision", choices=list(C3_PRECISIONS), default="native",
                     help="c2: squaring precision of the complex64 engine")
+    ap.add_argument("--no-split", action="store_true",
+                    help="c2: skip the fp16x2 split-precision comparison run")
     args = ap.parse_args()
 
     if args.c3_ref:
@@ -662,6 +664,23 @@ def main():
                              "avg launch; issued = real MFMA flops of the 3M half-tile kernel"},
         "accuracy": acc,
     }
+    if cdt == np.complex64 and args.precision == "native" and not args.no_split:
+        # the same expm with fp16x2 split-precision squarings, same input and reference
+        eng.set_precision("fp16x2")
+        for _ in range(args.warmup):
+            eng.run_resident()
+        ms = []
+        for _ in range(args.steps):
+            eng.run_resident()
+            ms.append(eng.stats()["ms_total"])
+        sp = {"precision": "fp16x2", "ms_per_step": float(np.median(ms)),
+              "ms_square": eng.stats()["ms_square"], "speedup_vs_fp32": ms_step / float(np.median(ms))}
+        if ref is not None:
+            psi = eng.apply(np.vstack([np.eye(1, n, dtype=np.complex128), random_state(n, seed=55)[None, :]]))
+            sp["infidelity_e0"] = _infidelity(ref["psi_e0"], psi[0])
+            sp["infidelity_random_state"] = _infidelity(ref["psi_rand"], psi[1])
+        out["split_precision"] = sp
+        eng.set_precision("native")
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(n, s, args.cpu_sample_n, args.t, args.seed)

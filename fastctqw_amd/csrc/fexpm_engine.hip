@@ -136,6 +136,7 @@ struct PipelineBase {
 // temp (tr/ti) holds 2np x LU_OUTER values per plane.  Returns the real
 // flops issued by the MFMA GEMMs.
 constexpr int LU_OUTER = 256;
+constexpr int CH_OUTER = 256;  // outer block of the structured (Cholesky) solve
 
 // Panel factorisation: one cooperative launch per 64-wide panel (default)
 // or one launch per column.
@@ -686,22 +687,34 @@ template <typename W> struct Pipeline : PipelineBase {
       g.epi.add_r[0] = vv;
       flops += gemm64<MODE_REAL, true>(g, st);
     }
-    for (int k0 = 0; k0 < ip; k0 += LU_NB) {
-      const int k1 = k0 + LU_NB;
-      const int rest = ip - k1;
-      double* dk = dinv + (size_t)(k0 / LU_NB) * LU_NB * LU_NB;
-      chol_diag<<<1, 256, 0, st>>>(sm, L, k0, n, dk, scal);
-      FX_LAUNCH();
-      double* panel = sm + (int64_t)k1 * L + k0;  // L[k1:, k0:k1]
-      if (rest > 0)
-        flops += dgemm_real(panel, L, dk, LU_NB, panel, L, rest, LU_NB, LU_NB, 1.0, 0.0, false, st);
-      // Z[:, k0:k1] <- Z[:, k0:k1] L_kk^{-T}
-      flops += dgemm_real(zb + k0, L, dk, LU_NB, zb + k0, L, 2 * ip, LU_NB, LU_NB, 1.0, 0.0, false, st);
-      if (rest > 0) {
-        // trailing S -= L21 L21^T (half tiles), Z[:, k1:] -= Z[:, k0:k1] L21^T
-        flops += dgemm_real(panel, L, panel, L, sm + (int64_t)k1 * L + k1, L, rest, rest, LU_NB, -1.0, 1.0, true,
-                            st);
-        flops += dgemm_real(zb + k0, L, panel, L, zb + k1, L, 2 * ip, rest, LU_NB, -1.0, 1.0, false, st);
+    // Two levels: 64-wide inner panels update only the columns of their
+    // CH_OUTER-wide outer block; the outer block then updates the trailing
+    // S (half tiles) and the remaining columns of Z with K = CH_OUTER.
+    for (int K0 = 0; K0 < ip; K0 += CH_OUTER) {
+      const int K1 = std::min(ip, K0 + CH_OUTER);
+      for (int k0 = K0; k0 < K1; k0 += LU_NB) {
+        const int k1 = k0 + LU_NB;
+        const int rest = ip - k1;
+        double* dk = dinv + (size_t)(k0 / LU_NB) * LU_NB * LU_NB;
+        chol_diag<<<1, 256, 0, st>>>(sm, L, k0, n, dk, scal);
+        FX_LAUNCH();
+        double* panel = sm + (int64_t)k1 * L + k0;  // L[k1:, k0:k1]
+        if (rest > 0)
+          flops += dgemm_real(panel, L, dk, LU_NB, panel, L, rest, LU_NB, LU_NB, 1.0, 0.0, false, st);
+        // Z[:, k0:k1] <- Z[:, k0:k1] L_kk^{-T}
+        flops += dgemm_real(zb + k0, L, dk, LU_NB, zb + k0, L, 2 * ip, LU_NB, LU_NB, 1.0, 0.0, false, st);
+        if (k1 < K1) {
+          // S[k1:, k1:K1] -= L[k1:, k0:k1] L[k1:K1, k0:k1]^T;  Z[:, k1:K1] -= Z[:, k0:k1] L[k1:K1, k0:k1]^T
+          flops += dgemm_real(panel, L, panel, L, sm + (int64_t)k1 * L + k1, L, rest, K1 - k1, LU_NB, -1.0, 1.0,
+                              false, st);
+          flops += dgemm_real(zb + k0, L, panel, L, zb + k1, L, 2 * ip, K1 - k1, LU_NB, -1.0, 1.0, false, st);
+        }
+      }
+      if (K1 < ip) {
+        const int r2 = ip - K1, kk = K1 - K0;
+        double* p2 = sm + (int64_t)K1 * L + K0;  // L[K1:, K0:K1]
+        flops += dgemm_real(p2, L, p2, L, sm + (int64_t)K1 * L + K1, L, r2, r2, kk, -1.0, 1.0, true, st);
+        flops += dgemm_real(zb + K0, L, p2, L, zb + K1, L, 2 * ip, r2, kk, -1.0, 1.0, false, st);
       }
     }
     // X = Z Z^T (3M, half tiles) over S and V V^T

@@ -1230,27 +1230,35 @@ __global__ __launch_bounds__(256) void chol_diag(double* __restrict__ s, int64_t
     a[i][j] = v;
   }
   __syncthreads();
+  // Right-looking on the unscaled columns: step j subtracts
+  // a[i][j] a[k][j] / a[j][j] from the trailing lower triangle; the column
+  // scaling by 1/sqrt(a[j][j]) is applied once at the end, so each step
+  // needs a single barrier.  Thread (ti, tk) of a 16 x 16 grid updates
+  // rows j+1+ti, +16, ... and columns j+1+tk, +16, ... (k <= i).
+  const int ti = t >> 4, tk = t & 15;
   for (int j = 0; j < 64; ++j) {
     double d = a[j][j];
     if (!(d > 0.0)) {
       if (t == 0) atomicOr(&sc->info, 1);
       d = 1.0;
     }
-    d = sqrt(d);
-    const double r = 1.0 / d;
-    if (t == j) {
-      ldg[j] = d;
-      rd[j] = r;
+    if (t == 0) {
+      ldg[j] = sqrt(d);
+      rd[j] = 1.0 / ldg[j];
     }
-    if (t > j && t < 64) a[t][j] *= r;
-    __syncthreads();
-    const int m = 63 - j;
-    for (int e = t; e < m * m; e += 256) {
-      const int i = j + 1 + e / m, k = j + 1 + e % m;
-      if (k <= i) a[i][k] -= a[i][j] * a[k][j];
+    const double r2 = 1.0 / d;
+    for (int i = j + 1 + ti; i < 64; i += 16) {
+      const double aij = a[i][j] * r2;
+      for (int k = j + 1 + tk; k <= i; k += 16) a[i][k] -= aij * a[k][j];
     }
     __syncthreads();
   }
+  // L[i][j] = a[i][j] / sqrt(d_j) below the diagonal
+  for (int e = t; e < 64 * 64; e += 256) {
+    const int i = e >> 6, j = e & 63;
+    if (j < i) a[i][j] *= rd[j];
+  }
+  __syncthreads();
   // factor back to s (lower triangle and diagonal)
   for (int e = t; e < 64 * 64; e += 256) {
     const int i = e >> 6, j = e & 63;
