@@ -67,6 +67,7 @@ def cpu_baseline(n_target, s_target, n_sample, t, seed, budget_s=20.0):
         "unit": "s per N=10k complex expm",
         "cores": cores,
         "kind": "port",
+        "host": _host_info(cores),
         "sample": (f"scipy.linalg.expm + U@psi0, complex64 ER({n_sample},0.5) Laplacian t={t} "
                    f"(s={s_s}), median of {len(times)} = {med:.3f} s, scaled by "
                    f"W({n_target},s={s_target})/W({n_sample},s={s_s}) = {scale:.1f}"),
@@ -375,7 +376,7 @@ def c4_cpu_baseline(mats, s_list, budget_s=20.0):
     w_done = sum(work_flops(a.shape[0], s) for a, s in zip(mats[:done], s_list[:done]))
     batch_s = spent * w_all / w_done
     return {
-        "value": len(mats) / batch_s, "unit": "walks/s", "cores": cores, "kind": "port",
+        "value": len(mats) / batch_s, "unit": "walks/s", "cores": cores, "kind": "port", "host": _host_info(cores),
         "sample": (f"scipy.linalg.expm + U@psi0 (complex64) on the first {done} of {len(mats)} walks: "
                    f"{spent:.2f} s, extrapolated by sum W(N_i, s_i) to {batch_s:.1f} s per batch"),
     }
@@ -477,6 +478,68 @@ def run_c4(args):
     print(json.dumps(res), flush=True)
 
 
+def _api_timing(eng, a, args):
+    """Host-inclusive times (not the headline value): ExpMatComplex64.run on
+    a host array (PCIe both ways), and Ctqw.runWalk end to end (host H
+    build, upload, expm, device U psi0) on the same graph: adjacency A,
+    Laplacian, gamma = -1 so that H = L, psi0 = e_0."""
+    from fastctqw_amd.Ctqw import Ctqw
+    from fastctqw_amd.State import State
+    from fastctqw_amd.graphs import erdos_renyi
+
+    t0 = time.perf_counter()
+    eng.run(a)
+    api_ms = 1000.0 * (time.perf_counter() - t0)
+    adj = erdos_renyi(args.n, args.p, args.seed, dtype=np.float32)
+    walk = Ctqw.from_numpy_array(adj, State.getSingleNodeState(args.n, 0), gamma=-1.0, laplacian=True,
+                                 device="cuda")
+    del adj
+    walk.runWalk(args.t)  # warm-up (engine creation)
+    t0 = time.perf_counter()
+    walk.runWalk(args.t)
+    rw_ms = 1000.0 * (time.perf_counter() - t0)
+    walk.getOperator().free()
+    return {"run_numpy_ms": api_ms, "ctqw_runwalk_ms": rw_ms,
+            "note": "run_numpy: host array in, host U out (PCIe both ways); runWalk: host H build + upload + "
+                    "expm + U psi0 on the device"}
+
+
+def _copy_calibration(mib):
+    """A plain device copy of known size through the testing ABI; profiler
+    counters of this launch calibrate FETCH_SIZE / WRITE_SIZE."""
+    import ctypes
+
+    lib = ctypes.CDLL(os.path.join(ROOT, "fastctqw_amd", "fastexpm", "libfastexpm_hip.so"))
+    lib.fexpm_last_error.restype = ctypes.c_char_p
+    ms = ctypes.c_double(0.0)
+    nbytes = int(mib) << 20
+    rc = lib.fexpm_test_copy(ctypes.c_int64(nbytes), 3, ctypes.byref(ms))
+    if rc != 0:
+        raise RuntimeError(lib.fexpm_last_error(None).decode())
+    print(json.dumps({"copy_bytes": nbytes, "launches": 4, "ms_per_launch": ms.value,
+                      "GB_s": 2.0 * nbytes / (ms.value * 1e-3) / 1e9}), flush=True)
+
+
+def _host_info(cores):
+    import platform
+
+    import scipy
+    from threadpoolctl import threadpool_info
+
+    model = platform.processor()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    blas = [f"{d.get('internal_api')} {d.get('version')}" for d in threadpool_info() if d.get("user_api") == "blas"]
+    return {"cpu_model": model, "host_cpus": os.cpu_count(), "threads_used": cores, "numpy": np.__version__,
+            "scipy": scipy.__version__, "blas": blas}
+
+
 def _infidelity(a, b):
     num = abs(np.vdot(a, b)) ** 2
     return float(max(0.0, 1.0 - num / (np.vdot(a, a).real * np.vdot(b, b).real)))
@@ -509,8 +572,15 @@ def main():
                     help="c2: squaring precision of the complex64 engine")
     ap.add_argument("--no-split", action="store_true",
                     help="c2: skip the fp16x2 split-precision comparison run")
+    ap.add_argument("--no-api-timing", action="store_true",
+                    help="c2: skip the host-inclusive run() and Ctqw.runWalk timings")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
 
+    if args.copy_calibration:
+        _copy_calibration(args.copy_calibration)
+        return
     if args.c3_ref:
         _c3_reference(args.n, args.c3_ref)
         return
@@ -681,6 +751,8 @@ def main():
             sp["infidelity_random_state"] = _infidelity(ref["psi_rand"], psi[1])
         out["split_precision"] = sp
         eng.set_precision("native")
+    if not args.no_api_timing:
+        out["api_timing"] = _api_timing(eng, a, args)
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(n, s, args.cpu_sample_n, args.t, args.seed)

@@ -2040,6 +2040,41 @@ double time_split_t(int m, int k, int reps) {
 
 }  // namespace
 
+// Plain copy of known size (HBM traffic calibration for profiler counters).
+__global__ __launch_bounds__(256) void copy_kernel(const u32x4* __restrict__ src, u32x4* __restrict__ dst,
+                                                   size_t cnt) {
+  for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < cnt; k += (size_t)gridDim.x * 256) dst[k] = src[k];
+}
+
+extern "C" int fexpm_test_copy(int64_t bytes, int reps, double* ms) {
+  if (!ms || bytes <= 0 || bytes % 16 != 0 || reps <= 0) return FEXPM_EINVAL;
+  return guarded(nullptr, [&] {
+    check_device();
+    const size_t cnt = (size_t)bytes / 16;
+    u32x4 *src = nullptr, *dst = nullptr;
+    FX_HIP(hipMalloc(&src, (size_t)bytes));
+    FX_HIP(hipMalloc(&dst, (size_t)bytes));
+    FX_HIP(hipMemset(src, 1, (size_t)bytes));
+    const unsigned grid = (unsigned)std::min<size_t>((cnt + 255) / 256, 8192);
+    hipEvent_t e0, e1;
+    FX_HIP(hipEventCreate(&e0));
+    FX_HIP(hipEventCreate(&e1));
+    copy_kernel<<<grid, 256>>>(src, dst, cnt);
+    FX_LAUNCH();
+    FX_HIP(hipEventRecord(e0));
+    for (int r = 0; r < reps; ++r) copy_kernel<<<grid, 256>>>(src, dst, cnt);
+    FX_HIP(hipEventRecord(e1));
+    FX_HIP(hipEventSynchronize(e1));
+    float t = 0.f;
+    FX_HIP(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(src);
+    (void)hipFree(dst);
+    *ms = (double)t / reps;
+  });
+}
+
 extern "C" int fexpm_test_gemm_time(int wtype, int mode, int sym, int cfg, int m, int n, int k, int reps,
                                     double* ms) {
   if (!ms || m <= 0 || n <= 0 || k <= 0 || reps <= 0 || (sym && m != n)) return FEXPM_EINVAL;

@@ -62,6 +62,11 @@ int fexpm_debug_plane(fexpm_engine* e, int which, double* re, double* im);
  * split-precision squaring kernel alone (mode 2, sym). */
 int fexpm_test_gemm_time(int wtype, int mode, int sym, int cfg, int m, int n, int k, int reps, double* ms);
 
+/* reps + 1 launches of a plain device copy of `bytes` (multiple of 16);
+ * average ms of the last reps.  Reference traffic for the profiler's
+ * FETCH_SIZE / WRITE_SIZE counters (each launch reads and writes bytes). */
+int fexpm_test_copy(int64_t bytes, int reps, double* ms);
+
 #ifdef __cplusplus
 }
 #endif
