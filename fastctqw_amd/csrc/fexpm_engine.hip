@@ -137,6 +137,7 @@ struct PipelineBase {
 // flops issued by the MFMA GEMMs.
 constexpr int LU_OUTER = 256;
 constexpr int CH_OUTER = 256;  // outer block of the structured (Cholesky) solve
+constexpr int SQ_GROUP = 8;    // tile group of the split squaring order
 
 // Panel factorisation: one cooperative launch per 64-wide panel (default)
 // or one launch per column.
@@ -343,6 +344,8 @@ template <typename W> struct Pipeline : PipelineBase {
   Scalars* scal = nullptr;
   void* sbuf = nullptr;  // k-blocked 16-bit split parts (split-precision squarings)
   size_t sbuf_bytes = 0;
+  int* sq_order = nullptr;  // grouped lower-triangular tile order for the split squaring
+  int sq_order_T = -1;
   hipEvent_t ev[8];
   bool events = false;
   // result of the last run (for apply)
@@ -413,6 +416,7 @@ template <typename W> struct Pipeline : PipelineBase {
     dfree(flags);
     dfree(scal);
     if (sbuf) (void)hipFree(sbuf);
+    if (sq_order) (void)hipFree(sq_order);
     if (events)
       for (int k = 0; k < 8; ++k) (void)hipEventDestroy(ev[k]);
   }
@@ -814,6 +818,15 @@ template <typename W> struct Pipeline : PipelineBase {
     }
     const int T = (n + SQ_BM - 1) / SQ_BM;
     const int nb = T * (T + 1) / 2;
+    if (sq_order_T != T) {
+      const int Tc = (int)((npcap + SQ_BM - 1) / SQ_BM);
+      if (!sq_order) FX_HIP(hipMalloc(&sq_order, (size_t)Tc * (Tc + 1) / 2 * sizeof(int)));
+      std::vector<int> h((size_t)nb);
+      sym_tile_order(T, SQ_GROUP, h.data());
+      FX_HIP(hipMemcpyAsync(sq_order, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice, st));
+      FX_HIP(hipStreamSynchronize(st));
+      sq_order_T = T;
+    }
     for (int q = 0; q < s; ++q) {
       dim3 sg((unsigned)((kpad / 16 + 15) / 16), (unsigned)((rows + 15) / 16));
       split_blocked<P><<<sg, 256, 0, st>>>((const float*)cur->r, (const float*)cur->i, ld, n, n, rows, kpad,
@@ -833,6 +846,7 @@ template <typename W> struct Pipeline : PipelineBase {
       a.m = n;
       a.k = kpad;
       a.kblock = kblock > 0 ? kblock : (1 << 30);
+      a.order = sq_order;
       sq_split_kernel<P><<<dim3(nb), dim3(sq_threads<P>()), 0, st>>>(a);
       FX_LAUNCH();
       flops += 4.0 * P::NPAIR * 2.0 * nb * (double)SQ_BM * SQ_BM * (double)kpad;
@@ -1808,6 +1822,11 @@ void test_split_t(int m, int k, const double* ar, const double* ai, double* cr, 
   a.k = kpad;
   a.kblock = 8;
   const int T = (m + SQ_BM - 1) / SQ_BM;
+  std::vector<int> ho((size_t)T * (T + 1) / 2);
+  sym_tile_order(T, SQ_GROUP, ho.data());
+  DevPlane<int> order(ho.size());
+  FX_HIP(hipMemcpy(order.p, ho.data(), ho.size() * sizeof(int), hipMemcpyHostToDevice));
+  a.order = order.p;
   sq_split_kernel<P><<<dim3(T * (T + 1) / 2), dim3(sq_threads<P>())>>>(a);
   FX_LAUNCH();
   FX_HIP(hipDeviceSynchronize());
@@ -1985,7 +2004,7 @@ __global__ void fill_pattern16(typename P::H* p, size_t cnt, uint32_t seed) {
 
 // Split-precision squaring kernel alone (parts already in the k-blocked layout).
 template <class P, int WM, int WN>
-double time_split_t(int m, int k, int reps) {
+double time_split_t(int m, int k, int reps, int group) {
   const int kpad = (int)round_up(std::max(k, 1), 16), rows = (int)round_up(m, SQ_BM);
   const size_t cp = (size_t)rows * kpad * 2 * P::NQ, cc = (size_t)rows * rows;
   typename P::H* parts;
@@ -2017,6 +2036,15 @@ double time_split_t(int m, int k, int reps) {
   a.k = kpad;
   a.kblock = 8;
   const int T = (m + SQ_BM - 1) / SQ_BM;
+  int* dorder = nullptr;
+  a.order = nullptr;
+  if (group > 0) {
+    std::vector<int> ho((size_t)T * (T + 1) / 2);
+    sym_tile_order(T, group, ho.data());
+    FX_HIP(hipMalloc(&dorder, ho.size() * sizeof(int)));
+    FX_HIP(hipMemcpy(dorder, ho.data(), ho.size() * sizeof(int), hipMemcpyHostToDevice));
+    a.order = dorder;
+  }
   dim3 grid(T * (T + 1) / 2), block(64 * WM * WN);
   hipEvent_t e0, e1;
   FX_HIP(hipEventCreate(&e0));
@@ -2035,6 +2063,7 @@ double time_split_t(int m, int k, int reps) {
   for (float* p : {yr, yi, cr, ci}) (void)hipFree(p);
   (void)hipFree(din);
   (void)hipFree(dout);
+  if (dorder) (void)hipFree(dorder);
   return (double)ms / std::max(reps, 1);
 }
 
@@ -2082,16 +2111,18 @@ extern "C" int fexpm_test_gemm_time(int wtype, int mode, int sym, int cfg, int m
     check_device();
     double t = -1.0;
     if (wtype >= 2) {
-      // cfg: 0 = 8 waves 2x4 (default), 1 = 4 waves 2x2, 2 = 8 waves 4x2, 3 = 16 waves 4x4 (fp16x2)
+      // cfg % 10: 0 = 8 waves 2x4, 1 = 4 waves 2x2, 2 = 8 waves 4x2, 3 = 16 waves 4x4 (fp16x2);
+      // cfg / 10: tile group of the order table (0 = row-major triangle)
+      const int lay = cfg % 10, grp = cfg / 10;
       if (mode == 2 && sym && wtype == 2) {
-        if (cfg == 0) t = time_split_t<Fp16x2, 2, 4>(m, k, reps);
-        else if (cfg == 1) t = time_split_t<Fp16x2, 2, 2>(m, k, reps);
-        else if (cfg == 2) t = time_split_t<Fp16x2, 4, 2>(m, k, reps);
-        else if (cfg == 3) t = time_split_t<Fp16x2, 4, 4>(m, k, reps);
+        if (lay == 0) t = time_split_t<Fp16x2, 2, 4>(m, k, reps, grp);
+        else if (lay == 1) t = time_split_t<Fp16x2, 2, 2>(m, k, reps, grp);
+        else if (lay == 2) t = time_split_t<Fp16x2, 4, 2>(m, k, reps, grp);
+        else if (lay == 3) t = time_split_t<Fp16x2, 4, 4>(m, k, reps, grp);
       } else if (mode == 2 && sym && wtype == 3) {
-        if (cfg == 0) t = time_split_t<Bf16x3, 2, 4>(m, k, reps);
-        else if (cfg == 1) t = time_split_t<Bf16x3, 2, 2>(m, k, reps);
-        else if (cfg == 2) t = time_split_t<Bf16x3, 4, 2>(m, k, reps);
+        if (lay == 0) t = time_split_t<Bf16x3, 2, 4>(m, k, reps, grp);
+        else if (lay == 1) t = time_split_t<Bf16x3, 2, 2>(m, k, reps, grp);
+        else if (lay == 2) t = time_split_t<Bf16x3, 4, 2>(m, k, reps, grp);
       }
     } else if (wtype == 0) {
       if (mode == 2 && sym) t = time_gemm_t<float, MODE_C3M, true, true, GemmCfg<float>>(m, n, k, reps);

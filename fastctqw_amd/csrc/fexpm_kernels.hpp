@@ -1217,27 +1217,36 @@ __global__ __launch_bounds__(256) void pack_stacked(const W* __restrict__ vr, co
 // 0 of sc->info and is replaced by 1.
 __global__ __launch_bounds__(256) void chol_diag(double* __restrict__ s, int64_t ld, int k0, int n,
                                                  double* __restrict__ inv, Scalars* sc) {
-  __shared__ double a[64][65];
+  __shared__ double a[64][65];   // scaled factor (input of the inverse)
+  __shared__ double col[2][64];  // column j of step j (unscaled), double-buffered by parity
   __shared__ double ldg[64], rd[64];
   const int t = threadIdx.x;
-  for (int e = t; e < 64 * 64; e += 256) {
-    const int i = e >> 6, j = e & 63;
-    double v;
-    if (k0 + i >= n || k0 + j >= n)
-      v = (i == j) ? 1.0 : 0.0;
-    else
-      v = s[(int64_t)(k0 + i) * ld + k0 + j];
-    a[i][j] = v;
-  }
-  __syncthreads();
-  // Right-looking on the unscaled columns: step j subtracts
-  // a[i][j] a[k][j] / a[j][j] from the trailing lower triangle; the column
-  // scaling by 1/sqrt(a[j][j]) is applied once at the end, so each step
-  // needs a single barrier.  Thread (ti, tk) of a 16 x 16 grid updates
-  // rows j+1+ti, +16, ... and columns j+1+tk, +16, ... (k <= i).
+  // thread (ti, tk) keeps the 16 elements (ti + 16p, tk + 16q) in registers
   const int ti = t >> 4, tk = t & 15;
-  for (int j = 0; j < 64; ++j) {
-    double d = a[j][j];
+  double e[4][4];
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int i = ti + 16 * p, k = tk + 16 * q;
+      e[p][q] = (k0 + i >= n || k0 + k >= n) ? (i == k ? 1.0 : 0.0) : s[(int64_t)(k0 + i) * ld + k0 + k];
+    }
+  // Right-looking on unscaled columns: step j publishes column j (its
+  // owners hold tk == j % 16), then every element (i, k) with j < k <= i
+  // loses a_ij a_kj / a_jj; the scaling by 1/sqrt(a_jj) is applied at the
+  // end.  One barrier per step.
+#pragma unroll
+  for (int jq = 0; jq < 4; ++jq)
+#pragma unroll 1
+  for (int jb = 0; jb < 16; ++jb) {
+    const int j = 16 * jq + jb;
+    double* cb = col[j & 1];
+    if (tk == jb) {
+#pragma unroll
+      for (int p = 0; p < 4; ++p) cb[ti + 16 * p] = e[p][jq];
+    }
+    __syncthreads();
+    double d = cb[j];
     if (!(d > 0.0)) {
       if (t == 0) atomicOr(&sc->info, 1);
       d = 1.0;
@@ -1247,23 +1256,34 @@ __global__ __launch_bounds__(256) void chol_diag(double* __restrict__ s, int64_t
       rd[j] = 1.0 / ldg[j];
     }
     const double r2 = 1.0 / d;
-    for (int i = j + 1 + ti; i < 64; i += 16) {
-      const double aij = a[i][j] * r2;
-      for (int k = j + 1 + tk; k <= i; k += 16) a[i][k] -= aij * a[k][j];
+    double ci[4], ck[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      ci[p] = cb[ti + 16 * p] * r2;
+      ck[p] = cb[tk + 16 * p];
     }
-    __syncthreads();
-  }
-  // L[i][j] = a[i][j] / sqrt(d_j) below the diagonal
-  for (int e = t; e < 64 * 64; e += 256) {
-    const int i = e >> 6, j = e & 63;
-    if (j < i) a[i][j] *= rd[j];
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int i = ti + 16 * p, k = tk + 16 * q;
+        if (k > j && k <= i) e[p][q] -= ci[p] * ck[q];
+      }
   }
   __syncthreads();
-  // factor back to s (lower triangle and diagonal)
-  for (int e = t; e < 64 * 64; e += 256) {
-    const int i = e >> 6, j = e & 63;
-    if (j <= i && k0 + i < n && k0 + j < n) s[(int64_t)(k0 + i) * ld + k0 + j] = (i == j) ? ldg[i] : a[i][j];
-  }
+  // L[i][k] = a_ik / sqrt(d_k) below the diagonal, sqrt(d_i) on it
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int i = ti + 16 * p, k = tk + 16 * q;
+      double v = 0.0;
+      if (k < i) v = e[p][q] * rd[k];
+      else if (k == i) v = ldg[i];
+      a[i][k] = v;
+      if (k <= i && k0 + i < n && k0 + k < n) s[(int64_t)(k0 + i) * ld + k0 + k] = v;
+    }
+  __syncthreads();
   // inverse by forward substitution, column c = t / 4; the four lanes of a
   // column split every inner product by k mod 4 and combine with shuffles.
   const int c = t >> 2, q = t & 3;

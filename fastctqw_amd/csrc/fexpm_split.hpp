@@ -162,12 +162,33 @@ template <class P> struct SqArgs {
   const double2* d_in;
   double2* d_out;
   int m, k, kblock;
+  const int* order;        // optional tile order (tm << 16 | tn) per logical block, else row-major
 };
 
+// Lower-triangular tile order in groups of g x g tiles (row-major over the
+// groups, row-major within a group): the blocks that run together on one
+// XCD (consecutive logical indices after xcd_remap) then cover a few tile
+// rows and columns of one group, so their operand rows are shared in that
+// XCD's L2.
+inline void sym_tile_order(int T, int g, int* out) {
+  int k = 0;
+  for (int gi = 0; gi * g < T; ++gi)
+    for (int gj = 0; gj <= gi; ++gj)
+      for (int tm = gi * g; tm < (gi + 1) * g && tm < T; ++tm)
+        for (int tn = gj * g; tn < (gj + 1) * g && tn <= tm; ++tn) out[k++] = (tm << 16) | tn;
+}
+
 // LDS position (in 16-bit elements) of 8-value chunk ch (0/1) of a tile
-// row: rows of 16 values, the chunk order swapped on every other group of 8
-// rows so that 16 lanes reading 16 consecutive rows hit distinct banks.
-__device__ __forceinline__ int sq_lds(int row, int ch) { return row * 16 + 8 * (ch ^ ((row >> 3) & 1)); }
+// row: rows of 16 values (32 bytes), the chunk order swapped by bit 2 xor
+// bit 3 of the row, so that 8 lanes reading 8 consecutive rows cover the
+// 8 16-byte bank groups of 128 bytes and 16 lanes the 16 groups of 256 bytes.
+__device__ __forceinline__ int sq_lds(int row, int ch) {
+  return row * 16 + 8 * (ch ^ (((row >> 2) ^ (row >> 3)) & 1));
+}
+// Arrays (plane/part) of a stage are 32 bytes apart in bank position, so
+// that the staging stores of consecutive lanes (same row, consecutive
+// arrays) do not hit the same banks.
+constexpr int SQ_TILE_STRIDE = SQ_BM * SQ_BK + 16;
 
 template <class P, int WAVES_M = P::WAVES_M, int WAVES_N = P::WAVES_N>
 __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void sq_split_kernel(SqArgs<P> g) {
@@ -176,14 +197,21 @@ __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void sq_split_kernel(SqArgs
   constexpr int NT = 64 * WAVES_M * WAVES_N;
   constexpr int NQ = P::NQ;
   constexpr int W = 2 * NQ;                   // arrays per operand tile
-  constexpr int TILE = SQ_BM * SQ_BK;         // elements per array
+  constexpr int TILE = SQ_TILE_STRIDE;        // elements per array (with bank padding)
   constexpr int CH = SQ_BM * W * 2;           // 16-byte chunks per operand stage
   constexpr int CPT = CH / NT;                // chunks per thread per operand
   static_assert(CH % NT == 0, "stage chunks");
   __shared__ __attribute__((aligned(16))) H smem[2][2 * W * TILE];
 
   int tm, tn;
-  tri_tile(xcd_remap((int)blockIdx.x, (int)gridDim.x), tm, tn);
+  const int lb = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+  if (g.order) {
+    const int code = g.order[lb];
+    tm = code >> 16;
+    tn = code & 0xFFFF;
+  } else {
+    tri_tile(lb, tm, tn);
+  }
   const int row0 = tm * SQ_BM, col0 = tn * SQ_BM;
   constexpr int FI = SQ_BM / WAVES_M / 32, FJ = SQ_BM / WAVES_N / 32;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -203,12 +231,14 @@ __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void sq_split_kernel(SqArgs
       }
 
   // stage staging: chunk c = threadIdx.x + NT * u of an operand stage is
-  // row c / (2W), array (c % (2W)) / 2, half (c % 2) of that row
-  u32x4 sa[CPT], sb[CPT];
+  // row c / (2W), array (c % (2W)) / 2, half (c % 2) of that row.  Two
+  // register sets: the loads of k-block kt+2 are issued while kt is being
+  // multiplied, so a load has two k-steps to arrive before it is stored.
+  u32x4 sa0[CPT], sb0[CPT], sa1[CPT], sb1[CPT];
   const int64_t kstride = (int64_t)g.rows * W * 16;  // elements per k-block
   const H* baseA = g.a + (int64_t)row0 * W * 16;
   const H* baseB = g.a + (int64_t)col0 * W * 16;
-  auto load = [&](int kb) {
+  auto load = [&](u32x4* sa, u32x4* sb, int kb) {
     const u32x4* pa = reinterpret_cast<const u32x4*>(baseA + kb * kstride);
     const u32x4* pb = reinterpret_cast<const u32x4*>(baseB + kb * kstride);
 #pragma unroll
@@ -218,7 +248,7 @@ __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void sq_split_kernel(SqArgs
       sb[u] = pb[c];
     }
   };
-  auto store = [&](H* s) {
+  auto store = [&](const u32x4* sa, const u32x4* sb, H* s) {
 #pragma unroll
     for (int u = 0; u < CPT; ++u) {
       const int c = threadIdx.x + NT * u;
@@ -227,15 +257,7 @@ __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void sq_split_kernel(SqArgs
       *reinterpret_cast<u32x4*>(s + (W + w) * TILE + sq_lds(row, ch)) = sb[u];
     }
   };
-
-  const int KT = g.k / SQ_BK;
-  load(0);
-  store(smem[0]);
-  __syncthreads();
-  for (int kt = 0; kt < KT; ++kt) {
-    const int cur = kt & 1;
-    if (kt + 1 < KT) load(kt + 1);
-    const H* s = smem[cur];
+  auto compute = [&](const H* s) {
     V far_[NQ][FI], fai[NQ][FI], fan[NQ][FI], fbr[NQ][FJ], fbi[NQ][FJ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
@@ -267,6 +289,9 @@ __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void sq_split_kernel(SqArgs
           acc[1][i][j] = P::mfma(fai[qa][i], fbr[qb][j], acc[1][i][j]);
         }
     }
+  };
+  const int KT = g.k / SQ_BK;
+  auto flush = [&](int kt) {
     if ((kt + 1) % g.kblock == 0 || kt + 1 == KT) {
 #pragma unroll
       for (int p = 0; p < 2; ++p)
@@ -278,7 +303,28 @@ __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void sq_split_kernel(SqArgs
             acc[p][i][j] = f32x16{};
           }
     }
-    if (kt + 1 < KT) store(smem[cur ^ 1]);
+  };
+
+  load(sa0, sb0, 0);
+  load(sa1, sb1, min(1, KT - 1));
+  store(sa0, sb0, smem[0]);
+  __syncthreads();
+  // The loads are unconditional (k-block clamped to the last one; the tail
+  // re-reads are never stored), so the memory-counter waits before each
+  // store only cover the older register set.
+  for (int kt = 0; kt < KT; kt += 2) {
+    // even step: k-block kt in smem[0]; set 0 is free for kt + 2
+    load(sa0, sb0, min(kt + 2, KT - 1));
+    compute(smem[0]);
+    flush(kt);
+    if (kt + 1 < KT) store(sa1, sb1, smem[1]);
+    __syncthreads();
+    if (kt + 1 >= KT) break;
+    // odd step: k-block kt + 1 in smem[1]; set 1 is free for kt + 3
+    load(sa1, sb1, min(kt + 3, KT - 1));
+    compute(smem[1]);
+    flush(kt + 1);
+    if (kt + 2 < KT) store(sa0, sb0, smem[0]);
     __syncthreads();
   }
 
