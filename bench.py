@@ -478,11 +478,44 @@ def run_c4(args):
     print(json.dumps(res), flush=True)
 
 
+def _c128_run(a64, args, ref):
+    """The same expm in complex128 (FP64 MFMA throughout).  The C2 input
+    -4.5i L has entries 4.5 * integer, exact in complex64, so the complex64
+    eigen reference is the reference of this input too."""
+    from fastctqw_amd.fastexpm import ExpMatComplex128
+    from fastctqw_amd.graphs import random_state
+
+    n = a64.shape[0]
+    a = a64.astype(np.complex128)
+    exact = bool(np.array_equal(a.astype(np.complex64), a64))
+    eng = ExpMatComplex128(n)
+    eng.upload(a)
+    del a
+    for _ in range(args.warmup):
+        eng.run_resident()
+    ms = []
+    for _ in range(args.steps):
+        eng.run_resident()
+        ms.append(eng.stats()["ms_total"])
+    st = eng.stats()
+    res = {"ms_per_step": float(np.median(ms)), "s": st["s"],
+           "phases_ms": {k: st[k] for k in ("ms_pade", "ms_solve", "ms_square", "ms_total")},
+           "input_equals_complex64": exact}
+    if ref is not None and exact:
+        psi = eng.apply(np.vstack([np.eye(1, n, dtype=np.complex128), random_state(n, seed=55)[None, :]]))
+        res["infidelity_e0"] = _infidelity(ref["psi_e0"], psi[0])
+        res["infidelity_random_state"] = _infidelity(ref["psi_rand"], psi[1])
+        res["rel_err_e0"] = float(np.linalg.norm(psi[0] - ref["psi_e0"]) / np.linalg.norm(ref["psi_e0"]))
+        res["rel_err_random_state"] = float(np.linalg.norm(psi[1] - ref["psi_rand"]) / np.linalg.norm(ref["psi_rand"]))
+    eng.free()
+    return res
+
+
 def _api_timing(eng, a, args):
     """Host-inclusive times (not the headline value): ExpMatComplex64.run on
-    a host array (PCIe both ways), and Ctqw.runWalk end to end (host H
-    build, upload, expm, device U psi0) on the same graph: adjacency A,
-    Laplacian, gamma = -1 so that H = L, psi0 = e_0."""
+    a host array (PCIe both ways), and Ctqw.runWalk end to end (adjacency
+    upload, H and A_exp on the device, expm, device U psi0) on the same
+    graph: adjacency A, Laplacian, gamma = -1 so that H = L, psi0 = e_0."""
     from fastctqw_amd.Ctqw import Ctqw
     from fastctqw_amd.State import State
     from fastctqw_amd.graphs import erdos_renyi
@@ -500,8 +533,8 @@ def _api_timing(eng, a, args):
     rw_ms = 1000.0 * (time.perf_counter() - t0)
     walk.getOperator().free()
     return {"run_numpy_ms": api_ms, "ctqw_runwalk_ms": rw_ms,
-            "note": "run_numpy: host array in, host U out (PCIe both ways); runWalk: host H build + upload + "
-                    "expm + U psi0 on the device"}
+            "note": "run_numpy: host array in, host U out (PCIe both ways); runWalk: float32 adjacency upload, "
+                    "H and A_exp formed on the device, expm, U psi0 on the device"}
 
 
 def _copy_calibration(mib):
@@ -572,6 +605,7 @@ def main():
                     help="c2: squaring precision of the complex64 engine")
     ap.add_argument("--no-split", action="store_true",
                     help="c2: skip the fp16x2 split-precision comparison run")
+    ap.add_argument("--no-c128", action="store_true", help="c2: skip the complex128 run")
     ap.add_argument("--no-api-timing", action="store_true",
                     help="c2: skip the host-inclusive run() and Ctqw.runWalk timings")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -753,6 +787,8 @@ def main():
         eng.set_precision("native")
     if not args.no_api_timing:
         out["api_timing"] = _api_timing(eng, a, args)
+    if cdt == np.complex64 and not args.no_c128:
+        out["complex128"] = _c128_run(a, args, ref)
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(n, s, args.cpu_sample_n, args.t, args.seed)

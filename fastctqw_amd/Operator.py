@@ -12,9 +12,14 @@ Deliberate differences (drop-in supersets):
 * device "cuda" (also "gpu", "hip") runs the gfx950 engine with the engine
   class matching the operator dtype; the reference always picked
   ExpMatComplex64 (:40), truncating complex128 input;
-* on the GPU, A_exp is uploaded once, U stays in device memory, and
-  ``getOperator`` copies it to the host only when asked (:165-166);
-  ``applyTo`` computes U psi on the device with FP64 accumulation.
+* on the GPU, a real adjacency is sent once and H = -gamma*A or
+  -gamma*(D - A) is formed on the device in FP64 (Operator.py:49-53); each
+  ``buildExpmOperator(t)`` forms A_exp = -i t H there too, so no N x N
+  complex array is built on the host or crosses PCIe.  The host H is built
+  only if something reads ``_hamiltonian``; ``setHamiltonian`` switches back
+  to uploading the host A_exp.  U stays in device memory and ``getOperator``
+  copies it to the host only when asked (:165-166); ``applyTo`` computes
+  U psi on the device with FP64 accumulation.
 """
 from __future__ import annotations
 
@@ -36,6 +41,9 @@ class Operator:
         self._expMator = None
         self._expmMat = None
         self._on_device = False
+        self._h = None
+        self._adj = None
+        self._graph_on_device = False
         if isinstance(data, np.ndarray):
             if not self._is_adjacency_matrix(data):
                 raise ValueError("A is not a adj matrix")
@@ -49,10 +57,7 @@ class Operator:
                 self._dtype = np.complex128
             else:
                 raise TypeError(f"A.dtype is error : {data.dtype}")
-            self._hamiltonian = self._buildHamiltonian(data, gamma)
-            if not self._is_hermitian(self._hamiltonian):
-                raise ValueError("_hamiltonian is not a hamiltonian matrix")
-            self._expMator = self._initExpMator(self._n, self._dtype)
+            self._set_graph(data)
         elif isinstance(data, (int, np.integer)):
             self._n = int(data)
             self._device = device
@@ -64,6 +69,34 @@ class Operator:
     def free(self):
         if self._expMator is not None:
             self._expMator.free()
+
+    def _set_graph(self, A: np.ndarray) -> None:
+        """H for adjacency A: on the device for a real A on a GPU operator
+        (a real symmetric A gives a real symmetric, hence Hermitian, H),
+        otherwise on the host with the reference's Hermitian check."""
+        if self._device in _GPU_DEVICES and not np.iscomplexobj(A):
+            if self._expMator is None:
+                self._expMator = self._initExpMator(self._n, self._dtype)
+            self._expMator.set_graph(A, self._gamma, self._laplacian)
+            self._adj, self._h, self._graph_on_device = A, None, True
+            return
+        self._adj, self._graph_on_device = None, False
+        self._h = self._buildHamiltonian(A, self._gamma)
+        if not self._is_hermitian(self._h):
+            raise ValueError("_hamiltonian is not a hamiltonian matrix")
+        if self._expMator is None:
+            self._expMator = self._initExpMator(self._n, self._dtype)
+
+    @property
+    def _hamiltonian(self):
+        if self._h is None and self._adj is not None:
+            self._h = self._buildHamiltonian(self._adj, self._gamma)
+        return self._h
+
+    @_hamiltonian.setter
+    def _hamiltonian(self, value) -> None:
+        self._h = value
+        self._graph_on_device = False
 
     def _buildHamiltonian(self, A: np.ndarray, gamma: float) -> NDArray:
         if self._laplacian:
@@ -91,6 +124,10 @@ class Operator:
         if self._device == "cpu":
             self._expmMat = _expm_cpu(self._buildUnitory(time))
             self._on_device = False
+        elif self._graph_on_device:
+            self._expMator.run_graph(time)
+            self._expmMat = None
+            self._on_device = True
         else:
             self._expMator.upload(self._buildUnitory(time))
             self._expMator.run_resident()
@@ -151,13 +188,10 @@ class Operator:
         self._n = A.shape[0]
         self._gamma = gamma
         self._laplacian = laplacian
-        self._hamiltonian = self._buildHamiltonian(A, gamma)
-        if not self._is_hermitian(self._hamiltonian):
-            raise ValueError("_hamiltonian is not a hamiltonian matrix")
-        if self._device != "cpu" and (self._expMator is None or changed):
-            if self._expMator is not None:
-                self._expMator.free()
-            self._expMator = self._initExpMator(self._n, self._dtype)
+        if self._device != "cpu" and changed and self._expMator is not None:
+            self._expMator.free()
+            self._expMator = None
+        self._set_graph(A)
         self._expmMat = None
         self._on_device = False
 
@@ -167,7 +201,7 @@ class Operator:
         self.reset_numpy_array(nx.to_numpy_array(nx_graph), gamma, laplacian)
 
     def setHamiltonian(self, hamiltonian: NDArray):
-        self._hamiltonian = hamiltonian
+        self._hamiltonian = hamiltonian  # later walks upload -1j * H * t from the host
 
     def setexpmOperator(self, newexpmMat):
         self._expmMat = newexpmMat

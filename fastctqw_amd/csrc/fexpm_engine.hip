@@ -1163,6 +1163,8 @@ struct fexpm_engine {
   size_t esize = 0;
   int opt_structure = 1;
   int opt_precision = 0;
+  double* d_h = nullptr;  // walk Hamiltonian kept by fexpm_set_graph (FP64, n x n)
+  bool have_graph = false;
 };
 
 static size_t elem_size(int dtype) {
@@ -1216,6 +1218,9 @@ static void release(fexpm_engine* e) {
   if (e->d_in) (void)hipFree(e->d_in);
   if (e->d_out) (void)hipFree(e->d_out);
   e->d_in = e->d_out = nullptr;
+  if (e->d_h) (void)hipFree(e->d_h);
+  e->d_h = nullptr;
+  e->have_graph = false;
   if (e->stream) (void)hipStreamDestroy(e->stream);
   e->stream = nullptr;
 }
@@ -1312,6 +1317,66 @@ int fexpm_run_resident(fexpm_engine* e) {
     e->err = "no resident input: call fexpm_upload first";
     return FEXPM_EINVAL;
   }
+  return run_device_locked(e, e->d_in, e->d_out, e->stream);
+}
+
+int fexpm_set_graph(fexpm_engine* e, const void* adj, int adj_dtype, double gamma, int laplacian) {
+  if (!e || (e->n > 0 && !adj)) return FEXPM_EINVAL;
+  std::lock_guard<std::mutex> lk(e->mtx);
+  if (e->freed) return FEXPM_EFREED;
+  if (e->dtype != FEXPM_COMPLEX64 && e->dtype != FEXPM_COMPLEX128) {
+    e->err = "fexpm_set_graph needs a complex engine (A_exp = -i t H is complex)";
+    return FEXPM_EINVAL;
+  }
+  if (adj_dtype != FEXPM_FLOAT32 && adj_dtype != FEXPM_FLOAT64) {
+    e->err = "adjacency dtype must be FEXPM_FLOAT32 or FEXPM_FLOAT64";
+    return FEXPM_EINVAL;
+  }
+  if (e->n == 0) return FEXPM_OK;
+  return guarded(e, [&] {
+    const int n = (int)e->n;
+    const size_t cnt = (size_t)n * n, asz = adj_dtype == FEXPM_FLOAT32 ? 4 : 8;
+    ensure_io(e);
+    if (!e->d_h) FX_HIP(hipMalloc(&e->d_h, cnt * sizeof(double)));
+    // the adjacency is staged in the input buffer (>= n^2 * 8 bytes for complex
+    // engines), which fexpm_run_graph overwrites with A_exp anyway
+    FX_HIP(hipMemcpyAsync(e->d_in, adj, cnt * asz, hipMemcpyHostToDevice, e->stream));
+    double* rs = nullptr;
+    if (laplacian) FX_HIP(hipMalloc(&rs, (size_t)n * sizeof(double)));
+    dim3 grid((unsigned)((n + 255) / 256), (unsigned)n);
+    if (adj_dtype == FEXPM_FLOAT32) {
+      if (rs) graph_row_sums<float><<<n, 256, 0, e->stream>>>((const float*)e->d_in, n, rs);
+      graph_hamiltonian<float><<<grid, 256, 0, e->stream>>>((const float*)e->d_in, n, gamma, rs, e->d_h);
+    } else {
+      if (rs) graph_row_sums<double><<<n, 256, 0, e->stream>>>((const double*)e->d_in, n, rs);
+      graph_hamiltonian<double><<<grid, 256, 0, e->stream>>>((const double*)e->d_in, n, gamma, rs, e->d_h);
+    }
+    FX_LAUNCH();
+    FX_HIP(hipStreamSynchronize(e->stream));
+    if (rs) (void)hipFree(rs);
+    e->have_graph = true;
+  });
+}
+
+int fexpm_run_graph(fexpm_engine* e, double t) {
+  if (!e) return FEXPM_EINVAL;
+  std::lock_guard<std::mutex> lk(e->mtx);
+  if (e->freed) return FEXPM_EFREED;
+  if (e->n == 0) return FEXPM_OK;
+  if (!e->have_graph) {
+    e->err = "no graph: call fexpm_set_graph first";
+    return FEXPM_EINVAL;
+  }
+  int rc = guarded(e, [&] {
+    const int64_t cnt = e->n * e->n;
+    const unsigned blocks = (unsigned)((cnt + 255) / 256);
+    if (e->dtype == FEXPM_COMPLEX64)
+      walk_input<float2><<<blocks, 256, 0, e->stream>>>(e->d_h, cnt, t, (float2*)e->d_in);
+    else
+      walk_input<double2><<<blocks, 256, 0, e->stream>>>(e->d_h, cnt, t, (double2*)e->d_in);
+    FX_LAUNCH();
+  });
+  if (rc != FEXPM_OK) return rc;
   return run_device_locked(e, e->d_in, e->d_out, e->stream);
 }
 

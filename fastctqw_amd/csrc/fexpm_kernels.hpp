@@ -1307,4 +1307,52 @@ __global__ __launch_bounds__(256) void chol_diag(double* __restrict__ s, int64_t
 }
 
 
+// ---------------------------------------------------------------------------
+// Walk Hamiltonian on the device (the reference builds it on the host,
+// FastCTQW/Operator.py:49-56): H = -gamma * A, or -gamma * (D - A) with
+// D = diag(row sums of A); then A_exp = -i t H in the engine's input type.
+
+// One workgroup per row: FP64 row sum of a real adjacency (fixed tree order).
+template <typename Ta>
+__global__ __launch_bounds__(256) void graph_row_sums(const Ta* __restrict__ adj, int n, double* __restrict__ rs) {
+  __shared__ double part[256];
+  const int i = blockIdx.x;
+  double acc = 0.0;
+  for (int j = threadIdx.x; j < n; j += 256) acc += (double)adj[(int64_t)i * n + j];
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) rs[i] = part[0];
+}
+
+template <typename Ta>
+__global__ __launch_bounds__(256) void graph_hamiltonian(const Ta* __restrict__ adj, int n, double gamma,
+                                                         const double* __restrict__ rs, double* __restrict__ h) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  const int i = blockIdx.y;
+  if (j >= n) return;
+  const double a = (double)adj[(int64_t)i * n + j];
+  double v;
+  if (rs)
+    v = (i == j) ? rs[i] - a : -a;  // Laplacian D - A
+  else
+    v = a;
+  h[(int64_t)i * n + j] = -gamma * v;
+}
+
+// A_exp = -i t H (real H) into the engine input: complex64 / complex128.
+template <typename Tc>
+__global__ __launch_bounds__(256) void walk_input(const double* __restrict__ h, int64_t cnt, double t,
+                                                  Tc* __restrict__ out) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k >= cnt) return;
+  Tc v;
+  v.x = 0;
+  v.y = -t * h[k];
+  out[k] = v;
+}
+
 }  // namespace fexpm

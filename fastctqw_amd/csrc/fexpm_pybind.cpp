@@ -90,6 +90,36 @@ template <typename T, int DT> class ExpMat {
     raise_status(rc, e_);
   }
 
+  void set_graph(py::array adj, double gamma, bool laplacian) {
+    if (adj.ndim() != 2 || adj.shape(0) != n_ || adj.shape(1) != n_)
+      throw std::invalid_argument("fastexpm.set_graph: adjacency must have shape (n, n)");
+    int adt;
+    py::array a;
+    if (adj.dtype().is(py::dtype::of<float>())) {
+      a = py::array_t<float, py::array::c_style | py::array::forcecast>(adj);
+      adt = FEXPM_FLOAT32;
+    } else {
+      a = py::array_t<double, py::array::c_style | py::array::forcecast>(adj);
+      adt = FEXPM_FLOAT64;
+    }
+    const void* src = a.data();
+    int rc;
+    {
+      py::gil_scoped_release nogil;
+      rc = fexpm_set_graph(e_, src, adt, gamma, laplacian ? 1 : 0);
+    }
+    raise_status(rc, e_);
+  }
+
+  void run_graph(double t) {
+    int rc;
+    {
+      py::gil_scoped_release nogil;
+      rc = fexpm_run_graph(e_, t);
+    }
+    raise_status(rc, e_);
+  }
+
   py::array_t<T> download() {
     py::array_t<T> out({n_, n_});
     T* dst = out.mutable_data();
@@ -335,6 +365,8 @@ template <typename T, int DT> static void bind(py::module_& m, const char* name)
       .def("free", &C::free, "Free memory")
       .def("upload", &C::upload, py::arg("arr_a"))
       .def("run_resident", &C::run_resident)
+      .def("set_graph", &C::set_graph, py::arg("adj"), py::arg("gamma") = 1.0, py::arg("laplacian") = false)
+      .def("run_graph", &C::run_graph, py::arg("t"))
       .def("download", &C::download)
       .def("run_device", &C::run_device, py::arg("d_a"), py::arg("d_u"), py::arg("stream") = 0)
       .def("apply", &C::apply, py::arg("psi0"))

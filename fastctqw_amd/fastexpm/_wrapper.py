@@ -67,6 +67,20 @@ class _ExpMatBase:
     def download(self) -> np.ndarray:
         return self._backend.download()
 
+    def set_graph(self, adj, gamma: float = 1.0, laplacian: bool = False) -> None:
+        """Keep H = -gamma*A (or -gamma*(D - A)) on the device, formed in FP64
+        from a real adjacency (float32 inputs stay float32 on the wire)."""
+        adj = np.asarray(adj)
+        if np.iscomplexobj(adj):
+            raise TypeError("set_graph expects a real adjacency matrix")
+        if adj.dtype != np.float32:
+            adj = adj.astype(np.float64, copy=False)
+        self._backend.set_graph(adj, float(gamma), bool(laplacian))
+
+    def run_graph(self, t: float) -> None:
+        """U = expm(-i t H) for the graph of set_graph, resident on the device."""
+        self._backend.run_graph(float(t))
+
     def apply(self, psi0) -> np.ndarray:
         """psi = U psi0 (complex128) for the U of the last run, FP64 accumulation."""
         return self._backend.apply(np.asarray(psi0, dtype=np.complex128))

@@ -130,6 +130,19 @@ int fexpm_upload(fexpm_engine* e, const void* a);
 int fexpm_run_resident(fexpm_engine* e);
 int fexpm_download(fexpm_engine* e, void* u);
 
+/* Keep a walk Hamiltonian on the device (complex engines): H = -gamma * A,
+ * or -gamma * (D - A) with D = diag(row sums of A) when laplacian != 0.
+ * `adj` is a row-major n x n host array of dtype FEXPM_FLOAT32 or
+ * FEXPM_FLOAT64; H is formed in FP64 (row sums included).  Replaces the host
+ * construction in Operator._buildHamiltonian (FastCTQW/Operator.py:49-53). */
+int fexpm_set_graph(fexpm_engine* e, const void* adj, int adj_dtype, double gamma, int laplacian);
+
+/* U = expm(-i t H) for the Hamiltonian kept by fexpm_set_graph, resident
+ * (download / apply / propagate as after fexpm_run_resident).  Replaces
+ * Operator._buildUnitory + buildExpmOperator (FastCTQW/Operator.py:55-56,
+ * 70-78): A_exp is formed on the device, no matrix crosses PCIe. */
+int fexpm_run_graph(fexpm_engine* e, double t);
+
 /* psi[k] = U psi0[k] for k < nstates with the U of the last run, FP64
  * accumulation on the device; psi0 and psi are host complex128 arrays of
  * nstates x n (replaces the host product at FastCTQW/Ctqw.py:94). */

@@ -394,3 +394,64 @@ def test_ctqw_drop_in(fx):
     w_cpu = Ctqw.from_numpy_array(adj, psi0, device="cpu")
     w_cpu.runWalk(time=4.5)
     assert infidelity(w_cpu.getFinalState().getState(), psi) < 1e-9
+
+
+@pytest.mark.parametrize("adt", [np.float32, np.float64])
+@pytest.mark.parametrize("laplacian", [False, True])
+def test_operator_device_hamiltonian(fx, adt, laplacian):
+    """H = -gamma*A or -gamma*(D - A) formed on the device (fexpm_set_graph)
+    and A_exp = -i t H per walk (fexpm_run_graph): against the eigen oracle
+    of the FP64 H, against the host route (setHamiltonian with the same H),
+    and the lazily built host H against the reference formula."""
+    from fastctqw_amd.Operator import Operator
+
+    n, gamma, t = 150, 0.7, 3.0
+    adj = erdos_renyi(n, 0.3, seed=77, weighted=True).astype(adt)
+    _say(f"device hamiltonian {np.dtype(adt).name} laplacian={laplacian}")
+    op = Operator(adj, gamma=gamma, laplacian=laplacian, device="cuda")
+    assert op._graph_on_device and op._h is None
+    a64 = adj.astype(np.float64)
+    h = -gamma * ((np.diag(a64.sum(axis=1)) - a64) if laplacian else a64)
+    ref = eig_propagator(h, t)
+    op.buildExpmOperator(t)
+    u_dev = op.getOperator()
+    st = op.lastRunStats()
+    assert st["structured"]
+    tol = _tol(np.complex64 if adt == np.float32 else np.complex128, st["s"])
+    assert rel_fro(ref, u_dev) <= tol
+    # same t twice gives the same U; the host H is built only when read
+    op.buildExpmOperator(t)
+    assert np.array_equal(op.getOperator(), u_dev)
+    hh = op._hamiltonian
+    assert hh.dtype == op._dtype and np.allclose(hh, h, rtol=1e-6, atol=1e-6)
+    # host route with the same H
+    op.setHamiltonian(hh)
+    assert not op._graph_on_device
+    op.buildExpmOperator(t)
+    assert rel_fro(ref, op.getOperator()) <= tol
+    # reset to another graph keeps the device route
+    adj2 = erdos_renyi(n, 0.2, seed=78, weighted=True).astype(adt)
+    op.reset_numpy_array(adj2, gamma=gamma, laplacian=laplacian)
+    assert op._graph_on_device
+    op.buildExpmOperator(t)
+    a2 = adj2.astype(np.float64)
+    h2 = -gamma * ((np.diag(a2.sum(axis=1)) - a2) if laplacian else a2)
+    assert rel_fro(eig_propagator(h2, t), op.getOperator()) <= tol
+    op.free()
+
+
+def test_device_graph_errors(fx):
+    _say("device graph errors")
+    with fx.ExpMatComplex64(8) as e:
+        with pytest.raises(Exception):
+            e.run_graph(1.0)  # no graph yet
+        with pytest.raises(ValueError):
+            e.set_graph(np.zeros((7, 7), np.float32))
+        with pytest.raises(TypeError):
+            e.set_graph(np.zeros((8, 8), np.complex64))
+        e.set_graph(np.zeros((8, 8), np.float32))
+        e.run_graph(2.0)
+        assert np.allclose(e.download(), np.eye(8))
+    with fx.ExpMatFloat32(8) as e:
+        with pytest.raises(Exception):
+            e.set_graph(np.zeros((8, 8), np.float32))
