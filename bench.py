@@ -451,6 +451,33 @@ def run_c4(args):
                    "model_tflops": flops / (dev * 1e-3) / 1e12,
                    "wall_ms_per_step_incl_psi_copies": 1000.0 * wall / args.steps},
     }
+    # the same batch with fp16x2 squarings, and walk by walk on one lane (the unbatched form)
+    variants = {}
+    if not args.no_variants:
+        b.set_precision("fp16x2")
+        b.run_resident(uni)
+        vm = []
+        for _ in range(args.steps):
+            psi_h = b.run_resident(uni)
+            vm.append(b.last_ms())
+        variants["fp16x2"] = {"walks_per_s": count / (float(np.median(vm)) / 1000.0),
+                              "ms_per_step": float(np.median(vm))}
+        if ref is not None:
+            variants["fp16x2"]["worst_infidelity_uniform"] = float(
+                max(_infidelity(ref[f"uni{i}"], psi_h[i]) for i in range(count)))
+        b.set_precision("native")
+        b1 = ExpMatBatch(nmax=max(ns), dtype=np.complex64, lanes=1)
+        b1.upload(mats)
+        b1.run_resident(uni)
+        t1 = []
+        for _ in range(max(1, args.steps - 1)):
+            b1.run_resident(uni)
+            t1.append(b1.last_ms())
+        variants["one_lane"] = {"walks_per_s": count / (float(np.median(t1)) / 1000.0),
+                                "ms_per_step": float(np.median(t1)),
+                                "note": "same walks one after another on one stream (no batching)"}
+        del b1
+    res["variants"] = variants
     if ref is not None:
         psi_r = b.run_resident(rnd)
         inf_u = [_infidelity(ref[f"uni{i}"], psi[i]) for i in range(count)]
@@ -597,7 +624,8 @@ def main():
     ap.add_argument("--c3-ref", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--c4-ref", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--c4-count", type=int, default=C4_COUNT)
-    ap.add_argument("--lanes", type=int, default=4, help="C4: concurrent lanes (streams + host threads)")
+    ap.add_argument("--lanes", type=int, default=8, help="C4: concurrent lanes (streams + host threads)")
+    ap.add_argument("--no-variants", action="store_true", help="c4: skip the fp16x2 and one-lane runs")
     ap.add_argument("--config", choices=["c2", "c3", "c4", "c5"], default="c2",
                     help="c2: N=10^4 single expm (default); c3: precision study at N=10^4; "
                          "c4: 256 uneven walks; c5: 32 times x 128 states, N=8192")

@@ -106,6 +106,9 @@ template <int BM_, int BN_, int BK_, int PAD_, int WM_, int WN_> struct TileCfg 
 };
 template <typename T> struct GemmCfg;
 template <> struct GemmCfg<float> : TileCfg<128, 128, 16, 4, 64, 32> {};
+// FP32 squarings of small orders: 4x the workgroups of the default tile
+// (a 128-wide tile grid of a few hundred rows leaves most CUs idle).
+using FCfgSmall = TileCfg<64, 64, 16, 4, 32, 32>;
 // FP64: 8 waves of 32x16 keep every variant within 256 registers (two
 // waves per SIMD); measured 63 TFLOP/s (4M, K = 256) and 61 TFLOP/s (real
 // symmetric, K = 10^4) on MI355X versus 50 / 57 for 4 waves of 32x32.

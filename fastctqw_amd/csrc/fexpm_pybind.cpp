@@ -41,6 +41,33 @@ static void raise_status(int rc, const fexpm_engine* e) {
   }
 }
 
+static py::dict stats_dict(const fexpm_stats& s) {
+  py::dict d;
+  d["n"] = s.n;
+  d["dtype"] = s.dtype;
+  d["m"] = s.m;
+  d["s"] = s.s;
+  d["structured"] = (bool)s.structured;
+  d["shifted"] = (bool)s.shifted;
+  d["nonfinite"] = (bool)s.nonfinite;
+  d["mu"] = std::complex<double>(s.mu_re, s.mu_im);
+  d["norm"] = s.norm;
+  d["norm_plain"] = s.norm_plain;
+  d["ms_total"] = s.ms_total;
+  d["ms_analyze"] = s.ms_analyze;
+  d["ms_pade"] = s.ms_pade;
+  d["ms_solve"] = s.ms_solve;
+  d["ms_square"] = s.ms_square;
+  d["ms_output"] = s.ms_output;
+  d["flops_issued"] = s.flops_issued;
+  d["s_fp64"] = s.s_fp64;
+  d["kblock"] = s.kblock;
+  d["pade_fp64"] = (bool)s.pade_fp64;
+  d["precision"] = s.precision == FEXPM_PREC_FP16X2 ? "fp16x2" : (s.precision == FEXPM_PREC_BF16X3 ? "bf16x3" : "native");
+  d["ms_propagate"] = s.ms_propagate;
+  return d;
+}
+
 template <typename T, int DT> class ExpMat {
  public:
   explicit ExpMat(int64_t n) : n_(n) {
@@ -195,30 +222,7 @@ template <typename T, int DT> class ExpMat {
   py::dict stats() const {
     fexpm_stats s;
     raise_status(fexpm_get_stats(e_, &s), e_);
-    py::dict d;
-    d["n"] = s.n;
-    d["dtype"] = s.dtype;
-    d["m"] = s.m;
-    d["s"] = s.s;
-    d["structured"] = (bool)s.structured;
-    d["shifted"] = (bool)s.shifted;
-    d["nonfinite"] = (bool)s.nonfinite;
-    d["mu"] = std::complex<double>(s.mu_re, s.mu_im);
-    d["norm"] = s.norm;
-    d["norm_plain"] = s.norm_plain;
-    d["ms_total"] = s.ms_total;
-    d["ms_analyze"] = s.ms_analyze;
-    d["ms_pade"] = s.ms_pade;
-    d["ms_solve"] = s.ms_solve;
-    d["ms_square"] = s.ms_square;
-    d["ms_output"] = s.ms_output;
-    d["flops_issued"] = s.flops_issued;
-    d["s_fp64"] = s.s_fp64;
-    d["kblock"] = s.kblock;
-    d["pade_fp64"] = (bool)s.pade_fp64;
-    d["precision"] = s.precision == FEXPM_PREC_FP16X2 ? "fp16x2" : (s.precision == FEXPM_PREC_BF16X3 ? "bf16x3" : "native");
-    d["ms_propagate"] = s.ms_propagate;
-    return d;
+    return stats_dict(s);
   }
 
   int64_t n() const { return n_; }
@@ -267,6 +271,8 @@ class ExpMatBatch {
       throw std::runtime_error(std::string("fastexpm batch: ") + fexpm_status_string(rc) + ": " +
                                fexpm_batch_last_error(b_));
   }
+  void set_option(int key, int64_t value) { check(fexpm_batch_set_option(b_, key, value)); }
+
   // mats: list of square complex arrays (cast to the batch dtype)
   void upload(py::list mats) {
     n_.clear();
@@ -338,13 +344,7 @@ class ExpMatBatch {
   py::dict stats(int64_t i) {
     fexpm_stats s;
     check(fexpm_batch_get_stats(b_, i, &s));
-    py::dict d;
-    d["n"] = s.n;
-    d["m"] = s.m;
-    d["s"] = s.s;
-    d["structured"] = (bool)s.structured;
-    d["ms_total"] = s.ms_total;
-    return d;
+    return stats_dict(s);
   }
   double last_ms() const { return fexpm_batch_last_ms(b_); }
 
@@ -398,6 +398,7 @@ PYBIND11_MODULE(_fastexpm_core, m) {
       .def(py::init<const std::string&, int64_t, int>(), py::arg("dtype"), py::arg("nmax"), py::arg("lanes") = 4)
       .def("upload", &ExpMatBatch::upload, py::arg("mats"))
       .def("run_resident", &ExpMatBatch::run_resident, py::arg("psi0") = py::none())
+      .def("set_option", &ExpMatBatch::set_option, py::arg("key"), py::arg("value"))
       .def("download", &ExpMatBatch::download, py::arg("i"))
       .def("stats", &ExpMatBatch::stats, py::arg("i"))
       .def("last_ms", &ExpMatBatch::last_ms);

@@ -134,6 +134,13 @@ class ExpMatBatch:
         name = np.dtype(dtype).name
         self._backend = _fastexpm_core.ExpMatBatch(name, int(nmax), int(lanes))
 
+    def set_precision(self, mode: str) -> None:
+        """Squaring precision for every lane ("native", "fp16x2", "bf16x3")."""
+        if mode not in _ExpMatBase._PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(_ExpMatBase._PRECISIONS)}")
+        self._backend.set_option(_fastexpm_core.OPT_PRECISION,
+                                 getattr(_fastexpm_core, _ExpMatBase._PRECISIONS[mode]))
+
     def upload(self, mats):
         self._backend.upload([np.asarray(m) for m in mats])
         self._count = len(mats)

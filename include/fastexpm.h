@@ -171,6 +171,8 @@ typedef struct fexpm_batch fexpm_batch;
 int fexpm_batch_create(int dtype, int64_t nmax, int lanes, fexpm_batch** out);
 int fexpm_batch_destroy(fexpm_batch* b);
 const char* fexpm_batch_last_error(const fexpm_batch* b);
+/* FEXPM_OPT_PRECISION, FEXPM_OPT_KBLOCK or FEXPM_OPT_SPD_SOLVE for every lane. */
+int fexpm_batch_set_option(fexpm_batch* b, int key, int64_t value);
 /* `count` row-major matrices of orders n[i] (engine dtype), packed back to
  * back in host memory, copied to the device. */
 int fexpm_batch_upload(fexpm_batch* b, int64_t count, const int64_t* n, const void* a);

@@ -369,6 +369,16 @@ def test_batch_uneven_walks(fx):
         err = rel_fro(ref, u)
         assert err <= _tol(np.complex64, st["s"]), (i, w["kind"], err)
         assert infidelity(ref @ psi0[i], psi) < 1e-9
+    # the same walks with fp16x2 split-precision squarings
+    b.set_precision("fp16x2")
+    us_h, _ = b.run(mats, psi0)
+    for i, u in enumerate(us_h):
+        G = np.ascontiguousarray(mats[i].imag, dtype=np.float64)
+        lam, V = np.linalg.eigh(G)
+        ref = (V * np.exp(1j * lam)[None, :]) @ V.T
+        assert rel_fro(ref, u) <= _tol(np.complex64, b.stats(i)["s"]), i
+        assert b.stats(i)["precision"] in ("fp16x2", "native")  # s = 0 walks have no squaring
+    b.set_precision("native")
     # an order above the capacity is rejected
     with pytest.raises(RuntimeError):
         b.upload([np.zeros((301, 301), np.complex64)])
