@@ -24,6 +24,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -108,6 +109,9 @@ struct PipelineBase {
   virtual bool has_result() const = 0;
   virtual void debug_plane(int which, double* re, double* im) = 0;
   virtual void set_size(int n_new, hipStream_t st) = 0;
+  virtual void analyze(const void* d_a, Scalars* slot, hipStream_t st) = 0;
+  virtual void run_planned(const void* d_a, void* d_u, const Scalars& hs, hipStream_t st, fexpm_stats& S,
+                           int opt_structure, int* info_slot) = 0;
   int max_squarings = -1;
   double last_prop_ms = 0.0;  // device time of the last propagate()
   int kblock = 8;          // FP32 squarings: k-tiles (of 16) per accumulation block, 0 = off
@@ -342,10 +346,10 @@ template <typename W> struct Pipeline : PipelineBase {
   double2* dsq[2] = {nullptr, nullptr};  // split-form diagonals (squarings)
   int* flags = nullptr;
   Scalars* scal = nullptr;
+  int* info_buf = nullptr;  // exception bits of the last run (device)
   void* sbuf = nullptr;  // k-blocked 16-bit split parts (split-precision squarings)
   size_t sbuf_bytes = 0;
-  int* sq_order = nullptr;  // grouped lower-triangular tile order for the split squaring
-  int sq_order_T = -1;
+  std::map<int, int*> sq_orders;  // grouped lower-triangular tile orders, per tile count
   hipEvent_t ev[8];
   bool events = false;
   // result of the last run (for apply)
@@ -385,6 +389,7 @@ template <typename W> struct Pipeline : PipelineBase {
     dsq[1] = dalloc<double2>(std::max(n, 1));
     flags = dalloc<int>(4);
     scal = dalloc<Scalars>(1);
+    info_buf = dalloc<int>(1);
     for (int k = 0; k < 8; ++k) FX_HIP(hipEventCreate(&ev[k]));
     events = true;
   }
@@ -415,8 +420,9 @@ template <typename W> struct Pipeline : PipelineBase {
     dfree(norm0);
     dfree(flags);
     dfree(scal);
+    dfree(info_buf);
     if (sbuf) (void)hipFree(sbuf);
-    if (sq_order) (void)hipFree(sq_order);
+    for (auto& kv : sq_orders) (void)hipFree(kv.second);
     if (events)
       for (int k = 0; k < 8; ++k) (void)hipEventDestroy(ev[k]);
   }
@@ -430,11 +436,19 @@ template <typename W> struct Pipeline : PipelineBase {
     if (n_new == n) return;
     n = n_new;
     np = round_up(std::max(n, 1), 128);
-    Plane2<W>* all[] = {&a, &p2, &p4, &p6, &t1, &t2, &u, &v, &x0, &x1};
-    const size_t bytes = (size_t)npcap * (size_t)ld * sizeof(W);
-    for (Plane2<W>* p : all) {
-      FX_HIP(hipMemsetAsync(p->r, 0, bytes, st));
-      FX_HIP(hipMemsetAsync(p->i, 0, bytes, st));
+    // Kernels write only the n x n region and GEMM k-loops read columns up
+    // to np of rows < n: zeroing the strip [n, np) x [0, n) of every
+    // working plane restores the padding invariant after a change of order.
+    if (np > n && n > 0) {
+      Plane2<W>* all[] = {&a, &p2, &p4, &p6, &t1, &t2, &u, &v, &x0, &x1};
+      PlaneList<W> L;
+      L.count = 0;
+      for (Plane2<W>* p : all) {
+        L.p[L.count++] = p->r;
+        L.p[L.count++] = p->i;
+      }
+      zero_strip<W><<<(unsigned)n, 128, 0, st>>>(L, n, (int)np, ld);
+      FX_LAUNCH();
     }
     for (auto& P : ps)
       if (P.r) {
@@ -817,14 +831,14 @@ template <typename W> struct Pipeline : PipelineBase {
     }
     const int T = (n + SQ_BM - 1) / SQ_BM;
     const int nb = T * (T + 1) / 2;
-    if (sq_order_T != T) {
-      const int Tc = (int)((npcap + SQ_BM - 1) / SQ_BM);
-      if (!sq_order) FX_HIP(hipMalloc(&sq_order, (size_t)Tc * (Tc + 1) / 2 * sizeof(int)));
+    int*& sq_order = sq_orders[T];
+    if (!sq_order) {
+      // built once per tile count (the copy waits for the host buffer)
       std::vector<int> h((size_t)nb);
       sym_tile_order(T, SQ_GROUP, h.data());
+      FX_HIP(hipMalloc(&sq_order, h.size() * sizeof(int)));
       FX_HIP(hipMemcpyAsync(sq_order, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice, st));
       FX_HIP(hipStreamSynchronize(st));
-      sq_order_T = T;
     }
     for (int q = 0; q < s; ++q) {
       dim3 sg((unsigned)((kpad / 16 + 15) / 16), (unsigned)((rows + 15) / 16));
@@ -927,11 +941,9 @@ template <typename W> struct Pipeline : PipelineBase {
   }
   Scalars* companion_scal() { return hi ? hi->scal : scal; }
 
-  template <typename Tin, typename Tout>
-  void run_typed(const Tin* d_a, Tout* d_u, hipStream_t st, fexpm_stats& S, int opt_structure) {
-    flops = 0.0;
-    have_result = false;
-    FX_HIP(hipEventRecord(ev[0], st));
+  // ---- one run = analysis, host plan, enqueued stages ----
+  template <typename Tin>
+  void analyze_enqueue(const Tin* d_a, hipStream_t st, Scalars* slot) {
     const int nt = (n + 31) / 32;
     FX_HIP(hipMemsetAsync(flags, 0, sizeof(int), st));
     analyze_tiles<Tin><<<nt * (nt + 1) / 2, 256, 0, st>>>(d_a, n, nt, colpart, diag, flags);
@@ -940,119 +952,186 @@ template <typename W> struct Pipeline : PipelineBase {
     FX_LAUNCH();
     finalize_norms<<<1, 1024, 0, st>>>(off, diag, n, flags, scal);
     FX_LAUNCH();
-    Scalars hs;
-    FX_HIP(hipMemcpyAsync(&hs, scal, sizeof(Scalars), hipMemcpyDeviceToHost, st));
-    FX_HIP(hipStreamSynchronize(st));
-    FX_HIP(hipEventRecord(ev[1], st));
+    if (slot) FX_HIP(hipMemcpyAsync(slot, scal, sizeof(Scalars), hipMemcpyDeviceToDevice, st));
+  }
 
+  struct Plan {
+    bool nonfinite, structured, shifted, hybrid, split16;
+    double mu_re, mu_im, norm, norm_plain;
+    int m, s, s_do, q64;
+  };
+
+  Plan make_plan(const Scalars& hs, int opt_structure) const {
+    Plan P;
+    std::memset(&P, 0, sizeof(P));
     const bool is_complex = (dtype == FEXPM_COMPLEX64 || dtype == FEXPM_COMPLEX128);
-    S.nonfinite = (hs.flags & FLAG_NONFINITE) ? 1 : 0;
-    S.norm_plain = hs.norm_plain;
-    if (S.nonfinite || !std::isfinite(hs.norm_plain)) {
-      S.nonfinite = 1;
+    P.norm_plain = hs.norm_plain;
+    P.nonfinite = (hs.flags & FLAG_NONFINITE) || !std::isfinite(hs.norm_plain);
+    if (P.nonfinite) {
+      P.norm = hs.norm_plain;
+      return P;
+    }
+    P.structured = is_complex && opt_structure && !(hs.flags & (FLAG_NONZERO_RE | FLAG_ASYM));
+    P.shifted = hs.norm_shift <= hs.norm_plain;
+    P.mu_re = P.shifted ? hs.mu_re : 0.0;
+    P.mu_im = P.shifted ? hs.mu_im : 0.0;
+    P.norm = P.shifted ? hs.norm_shift : hs.norm_plain;
+    choose_degree(P.norm, P.m, P.s);
+    P.s_do = max_squarings >= 0 ? std::min(P.s, max_squarings) : P.s;
+    P.hybrid = std::is_same<W, float>::value && pade_fp64;
+    P.q64 = (std::is_same<W, float>::value && (P.hybrid || P.structured)) ? std::min(P.s_do, fp64_squarings) : 0;
+    P.split16 = std::is_same<W, float>::value && P.structured && precision != FEXPM_PREC_NATIVE;
+    return P;
+  }
+
+  // Stages after the analysis, enqueued on st without host waits (ev[1..5]
+  // recorded when `timing`).  With info_slot, the exception bits of this run
+  // (singular pivot, barrier time-out, 16-bit range) are copied there.
+  template <typename Tin, typename Tout>
+  void enqueue_rest(const Tin* d_a, Tout* d_u, const Plan& P, hipStream_t st, bool timing, int* info_slot) {
+    flops = 0.0;
+    have_result = false;
+    if (P.nonfinite) {
       const int64_t cnt = (int64_t)n * n;
       fill_nan<Tout><<<(unsigned)((cnt + 255) / 256), 256, 0, st>>>(d_u, cnt);
       FX_LAUNCH();
-      FX_HIP(hipStreamSynchronize(st));
-      S.m = 0;
-      S.s = 0;
-      S.structured = 0;
-      S.shifted = 0;
-      S.mu_re = S.mu_im = 0.0;
-      S.norm = hs.norm_plain;
-      S.ms_total = S.ms_analyze = S.ms_pade = S.ms_solve = S.ms_square = S.ms_output = 0.0;
-      S.flops_issued = 0.0;
+      if (info_slot) FX_HIP(hipMemsetAsync(info_slot, 0, sizeof(int), st));
       return;
     }
-    const bool structured =
-        is_complex && opt_structure && !(hs.flags & (FLAG_NONZERO_RE | FLAG_ASYM));
-    const bool shifted = hs.norm_shift <= hs.norm_plain;
-    const double mu_re = shifted ? hs.mu_re : 0.0, mu_im = shifted ? hs.mu_im : 0.0;
-    const double norm = shifted ? hs.norm_shift : hs.norm_plain;
-    int m, s;
-    choose_degree(norm, m, s);
-    const double scale = std::ldexp(1.0, -s);
-    const int s_do = max_squarings >= 0 ? std::min(s, max_squarings) : s;
-    const bool hybrid = std::is_same<W, float>::value && pade_fp64;
-    int q64 = 0, dc = 0;
-    if (hybrid) {
+    FX_HIP(hipMemsetAsync(&scal->info, 0, sizeof(int), st));
+    const double scale = std::ldexp(1.0, -P.s);
+    int dc = 0;
+    Pipeline<double>* h = nullptr;
+    if (P.hybrid) {
       // FP64 Pade + solve (+ leading squarings), then FP32 squarings
-      Pipeline<double>* h = companion();
+      h = companion();
       h->flops = 0.0;
       h->panel_coop = panel_coop;
       h->spd_solve = spd_solve;
-      h->front_pade<Tin>(d_a, structured, m, mu_re, mu_im, scale, st);
-      FX_HIP(hipEventRecord(ev[2], st));
-      q64 = std::min(s_do, fp64_squarings);
-      const int dh = h->solve(structured, 0, st);
-      auto r = h->square_split(q64, structured, dh, st);
+      h->front_pade<Tin>(d_a, P.structured, P.m, P.mu_re, P.mu_im, scale, st);
+      if (timing) FX_HIP(hipEventRecord(ev[2], st));
+      const int dh = h->solve(P.structured, 0, st);
+      auto r = h->square_split(P.q64, P.structured, dh, st);
       dim3 grid((n + 255) / 256, n);
       fp64_to_work<W><<<grid, 256, 0, st>>>(r.first->r, r.first->i, h->ld, x0.r, x0.i, ld, n);
       FX_LAUNCH();
       FX_HIP(hipMemcpyAsync(dsq[0], h->dsq[r.second], (size_t)n * sizeof(double2), hipMemcpyDeviceToDevice, st));
       flops += h->flops;
-      dc = 0;
     } else {
-      front_pade<Tin>(d_a, structured, m, mu_re, mu_im, scale, st);
-      FX_HIP(hipEventRecord(ev[2], st));
-      q64 = (std::is_same<W, float>::value && structured) ? std::min(s_do, fp64_squarings) : 0;
-      dc = solve(structured, q64, st);
+      front_pade<Tin>(d_a, P.structured, P.m, P.mu_re, P.mu_im, scale, st);
+      if (timing) FX_HIP(hipEventRecord(ev[2], st));
+      dc = solve(P.structured, P.q64, st);
     }
-    FX_HIP(hipEventRecord(ev[3], st));
-    Plane2<W>* X = square(s_do - q64, structured, dc, st);
-    FX_HIP(hipEventRecord(ev[4], st));
+    if (timing) FX_HIP(hipEventRecord(ev[3], st));
+    Plane2<W>* X = square(P.s_do - P.q64, P.structured, dc, st);
+    if (timing) FX_HIP(hipEventRecord(ev[4], st));
     // phase e^mu
-    const double er = std::exp(mu_re);
-    ph_re = er * std::cos(mu_im);
-    ph_im = er * std::sin(mu_im);
+    const double er = std::exp(P.mu_re);
+    ph_re = er * std::cos(P.mu_im);
+    ph_im = er * std::sin(P.mu_im);
     {
       dim3 grid((n + 255) / 256, n);
       write_output<W, Tout><<<grid, 256, 0, st>>>(X->r, X->i, ld, n, ph_re, ph_im, d_u);
       FX_LAUNCH();
     }
-    FX_HIP(hipEventRecord(ev[5], st));
-    FX_HIP(hipEventSynchronize(ev[5]));
-    Scalars hs2;
-    FX_HIP(hipMemcpy(&hs2, hybrid ? companion_scal() : scal, sizeof(Scalars), hipMemcpyDeviceToHost));
-    const bool split16 = std::is_same<W, float>::value && structured && precision != FEXPM_PREC_NATIVE;
-    int split_info = 0;
-    if (split16) {
-      Scalars hs3;
-      FX_HIP(hipMemcpy(&hs3, scal, sizeof(Scalars), hipMemcpyDeviceToHost));
-      split_info = hs3.info & INFO_SPLIT_RANGE;
-      hs2.info &= ~INFO_SPLIT_RANGE;
+    if (timing) FX_HIP(hipEventRecord(ev[5], st));
+    if (info_slot) {
+      collect_info<<<1, 1, 0, st>>>(scal, h ? h->scal : nullptr, info_slot);
+      FX_LAUNCH();
     }
     res = *X;
     have_result = true;
-    last_structured = structured;
+    last_structured = P.structured;
+  }
+
+  void fill_stats(const Plan& P, fexpm_stats& S) const {
+    S.nonfinite = P.nonfinite ? 1 : 0;
+    S.norm_plain = P.norm_plain;
+    S.norm = P.norm;
+    S.m = P.m;
+    S.s = P.s;
+    S.structured = P.structured ? 1 : 0;
+    S.shifted = P.shifted ? 1 : 0;
+    S.mu_re = P.mu_re;
+    S.mu_im = P.mu_im;
+    S.flops_issued = flops;
+    S.s_fp64 = P.q64;
+    S.pade_fp64 = (P.hybrid && !P.nonfinite) ? 1 : 0;
+    S.kblock = kblock;
+    S.precision = P.split16 ? precision : FEXPM_PREC_NATIVE;
+  }
+
+  static void check_info(int info) {
+    if (info & INFO_SPLIT_RANGE)
+      throw Error(FEXPM_EINVAL, "split-precision squaring: a scaled operand value left the 16-bit range");
+    if (info & 2) throw Error(FEXPM_EHIP, "LU panel grid barrier timed out");
+    if (info != 0) throw Error(FEXPM_ESINGULAR, "Pade denominator is exactly singular");
+  }
+
+  // Single run with per-phase timing (one host wait for the analysis, one
+  // at the end).
+  template <typename Tin, typename Tout>
+  void run_typed(const Tin* d_a, Tout* d_u, hipStream_t st, fexpm_stats& S, int opt_structure) {
+    FX_HIP(hipEventRecord(ev[0], st));
+    analyze_enqueue<Tin>(d_a, st, nullptr);
+    Scalars hs;
+    FX_HIP(hipMemcpyAsync(&hs, scal, sizeof(Scalars), hipMemcpyDeviceToHost, st));
+    FX_HIP(hipStreamSynchronize(st));
+    FX_HIP(hipEventRecord(ev[1], st));
+    const Plan P = make_plan(hs, opt_structure);
+    enqueue_rest<Tin, Tout>(d_a, d_u, P, st, true, info_buf);
+    int info = 0;
+    FX_HIP(hipMemcpyAsync(&info, info_buf, sizeof(int), hipMemcpyDeviceToHost, st));
+    FX_HIP(hipStreamSynchronize(st));
+    fill_stats(P, S);
+    if (P.nonfinite) {
+      S.ms_total = S.ms_analyze = S.ms_pade = S.ms_solve = S.ms_square = S.ms_output = 0.0;
+      return;
+    }
     float t01, t12, t23, t34, t45;
     FX_HIP(hipEventElapsedTime(&t01, ev[0], ev[1]));
     FX_HIP(hipEventElapsedTime(&t12, ev[1], ev[2]));
     FX_HIP(hipEventElapsedTime(&t23, ev[2], ev[3]));
     FX_HIP(hipEventElapsedTime(&t34, ev[3], ev[4]));
     FX_HIP(hipEventElapsedTime(&t45, ev[4], ev[5]));
-    S.m = m;
-    S.s = s;
-    S.structured = structured ? 1 : 0;
-    S.shifted = shifted ? 1 : 0;
-    S.mu_re = mu_re;
-    S.mu_im = mu_im;
-    S.norm = norm;
     S.ms_analyze = t01;
     S.ms_pade = t12;
     S.ms_solve = t23;
     S.ms_square = t34;
     S.ms_output = t45;
     S.ms_total = t01 + t12 + t23 + t34 + t45;
-    S.flops_issued = flops;
-    S.s_fp64 = q64;
-    S.pade_fp64 = hybrid ? 1 : 0;
-    S.kblock = kblock;
-    S.precision = split16 ? precision : FEXPM_PREC_NATIVE;
-    if (split_info)
-      throw Error(FEXPM_EINVAL, "split-precision squaring: a scaled operand value left the 16-bit range");
-    if (hs2.info & 2) throw Error(FEXPM_EHIP, "LU panel grid barrier timed out");
-    if (hs2.info != 0) throw Error(FEXPM_ESINGULAR, "Pade denominator is exactly singular");
+    check_info(info);
+  }
+
+  // Batches: analysis into a device slot; then the rest from the analysis
+  // read back by the caller, without host waits.
+  void analyze(const void* d_a, Scalars* slot, hipStream_t st) override {
+    switch (dtype) {
+      case FEXPM_FLOAT32: analyze_enqueue<float>((const float*)d_a, st, slot); break;
+      case FEXPM_FLOAT64: analyze_enqueue<double>((const double*)d_a, st, slot); break;
+      case FEXPM_COMPLEX64: analyze_enqueue<float2>((const float2*)d_a, st, slot); break;
+      default: analyze_enqueue<double2>((const double2*)d_a, st, slot); break;
+    }
+  }
+
+  void run_planned(const void* d_a, void* d_u, const Scalars& hs, hipStream_t st, fexpm_stats& S,
+                   int opt_structure, int* info_slot) override {
+    const Plan P = make_plan(hs, opt_structure);
+    switch (dtype) {
+      case FEXPM_FLOAT32:
+        enqueue_rest<float, float>((const float*)d_a, (float*)d_u, P, st, false, info_slot);
+        break;
+      case FEXPM_FLOAT64:
+        enqueue_rest<double, double>((const double*)d_a, (double*)d_u, P, st, false, info_slot);
+        break;
+      case FEXPM_COMPLEX64:
+        enqueue_rest<float2, float2>((const float2*)d_a, (float2*)d_u, P, st, false, info_slot);
+        break;
+      default:
+        enqueue_rest<double2, double2>((const double2*)d_a, (double2*)d_u, P, st, false, info_slot);
+        break;
+    }
+    fill_stats(P, S);
   }
 
   void run(const void* d_a, void* d_u, hipStream_t st, fexpm_stats& S, int opt_structure) override {
@@ -1588,6 +1667,10 @@ struct fexpm_batch {
   size_t cap_states = 0;
   std::vector<fexpm_stats> stats;
   double last_ms = 0.0;
+  Scalars* d_scal = nullptr;  // per-walk analysis results
+  int* d_info = nullptr;      // per-walk exception bits
+  size_t cap_walks = 0;
+  std::vector<hipEvent_t> ev;  // per-walk start / end
 };
 
 static void batch_release(fexpm_batch* b) {
@@ -1600,6 +1683,13 @@ static void batch_release(fexpm_batch* b) {
   if (b->d_psi) (void)hipFree(b->d_psi);
   b->d_in = b->d_out = nullptr;
   b->d_psi0 = b->d_psi = nullptr;
+  if (b->d_scal) (void)hipFree(b->d_scal);
+  if (b->d_info) (void)hipFree(b->d_info);
+  b->d_scal = nullptr;
+  b->d_info = nullptr;
+  b->cap_walks = 0;
+  for (hipEvent_t e : b->ev) (void)hipEventDestroy(e);
+  b->ev.clear();
 }
 
 template <typename F> static int guarded_b(fexpm_batch* b, F&& f) {
@@ -1728,6 +1818,20 @@ int fexpm_batch_upload(fexpm_batch* b, int64_t count, const int64_t* n, const vo
     }
     if (tot) FX_HIP(hipMemcpy(b->d_in, a, tot * b->esize, hipMemcpyHostToDevice));
     b->stats.assign(count, fexpm_stats());
+    if ((size_t)count > b->cap_walks) {
+      if (b->d_scal) (void)hipFree(b->d_scal);
+      if (b->d_info) (void)hipFree(b->d_info);
+      b->d_scal = nullptr;
+      b->d_info = nullptr;
+      FX_HIP(hipMalloc(&b->d_scal, (size_t)count * sizeof(Scalars)));
+      FX_HIP(hipMalloc(&b->d_info, (size_t)count * sizeof(int)));
+      b->cap_walks = (size_t)count;
+    }
+    while (b->ev.size() < 2 * (size_t)count) {
+      hipEvent_t e;
+      FX_HIP(hipEventCreate(&e));
+      b->ev.push_back(e);
+    }
   });
 }
 
@@ -1759,22 +1863,48 @@ int fexpm_batch_run_resident(fexpm_batch* b, const void* psi0, void* psi) {
     std::vector<std::string> errs(b->lanes);
     std::vector<int> codes(b->lanes, FEXPM_OK);
     auto t0 = std::chrono::steady_clock::now();
+    // Two phases per lane: the analyses of all its walks (one host wait),
+    // then every walk's stages enqueued back to back with no host wait in
+    // between; exception bits and times are read after the lane drains.
     auto worker = [&](int l) {
       try {
         FX_HIP(hipSetDevice(0));
         PipelineBase* p = b->pipes[l].get();
         hipStream_t st = b->streams[l];
-        for (int64_t i : assign[l]) {
+        const std::vector<int64_t>& mine = assign[l];
+        for (int64_t i : mine) {
+          p->set_size((int)b->n[i], st);
+          p->analyze((const char*)b->d_in + b->off[i] * b->esize, b->d_scal + i, st);
+        }
+        std::vector<Scalars> hs(mine.size());
+        for (size_t q = 0; q < mine.size(); ++q)
+          FX_HIP(hipMemcpyAsync(&hs[q], b->d_scal + mine[q], sizeof(Scalars), hipMemcpyDeviceToHost, st));
+        FX_HIP(hipStreamSynchronize(st));
+        for (size_t q = 0; q < mine.size(); ++q) {
+          const int64_t i = mine[q];
           p->set_size((int)b->n[i], st);
           fexpm_stats S;
           std::memset(&S, 0, sizeof(S));
           S.n = b->n[i];
           S.dtype = b->dtype;
-          p->run((const char*)b->d_in + b->off[i] * b->esize, (char*)b->d_out + b->off[i] * b->esize, st, S, 1);
+          FX_HIP(hipEventRecord(b->ev[2 * i], st));
+          p->run_planned((const char*)b->d_in + b->off[i] * b->esize, (char*)b->d_out + b->off[i] * b->esize, hs[q],
+                         st, S, 1, b->d_info + i);
           if (psi0) p->apply(b->d_psi0 + b->voff[i], 1, b->d_psi + b->voff[i], st);
+          FX_HIP(hipEventRecord(b->ev[2 * i + 1], st));
           b->stats[i] = S;
         }
         FX_HIP(hipStreamSynchronize(st));
+        for (int64_t i : mine) {
+          int info = 0;
+          FX_HIP(hipMemcpy(&info, b->d_info + i, sizeof(int), hipMemcpyDeviceToHost));
+          float ms = 0.f;
+          FX_HIP(hipEventElapsedTime(&ms, b->ev[2 * i], b->ev[2 * i + 1]));
+          b->stats[i].ms_total = ms;
+          if (info & INFO_SPLIT_RANGE)
+            throw Error(FEXPM_EINVAL, "walk " + std::to_string(i) + ": split-precision operand out of range");
+          if (info != 0) throw Error(FEXPM_ESINGULAR, "walk " + std::to_string(i) + ": Pade denominator singular");
+        }
       } catch (const Error& x) {
         errs[l] = x.what();
         codes[l] = x.code;

@@ -1355,4 +1355,26 @@ __global__ __launch_bounds__(256) void walk_input(const double* __restrict__ h, 
   out[k] = v;
 }
 
+// Zero columns [n, np) of rows [0, n) in up to 24 planes of leading
+// dimension ld (the padding strip that GEMM k-loops read); one workgroup
+// per row, np - n < 128.
+template <typename W> struct PlaneList {
+  W* p[24];
+  int count;
+};
+
+template <typename W>
+__global__ __launch_bounds__(128) void zero_strip(PlaneList<W> L, int n, int np, int64_t ld) {
+  const int j = n + (int)threadIdx.x;
+  if (j >= np) return;
+  const int64_t o = (int64_t)blockIdx.x * ld + j;
+  for (int q = 0; q < L.count; ++q) L.p[q][o] = (W)0;
+}
+
+// Exception bits of one run: its own scalars and those of its FP64
+// companion (LU / Cholesky pivots), gathered into one device int.
+__global__ void collect_info(const Scalars* a, const Scalars* b, int* out) {
+  out[0] = a->info | (b ? b->info : 0);
+}
+
 }  // namespace fexpm
