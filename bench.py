@@ -391,6 +391,9 @@ def run_c4(args):
     from fastctqw_amd.graphs import iter_c4_walks
 
     count = args.c4_count
+    # one hardware queue per lane (read when HIP initialises in this process):
+    # lanes that share a queue execute their kernels one after another
+    os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, max(4, args.lanes)))
     ref_path = os.path.join(tempfile.gettempdir(), f"fexpm_c4_{os.getpid()}.npz")
     child = None
     if not args.no_accuracy:
@@ -445,7 +448,8 @@ def run_c4(args):
         "vs_baseline": None, "dtype": "c64 (fp32 mfma products, fp64 pade/solve, fp64 psi)",
         "data": "synthetic: seeded d-regular / ER graphs, weights U(0,1], adjacency or Laplacian, t in [0.1,20]",
         "config": {"workload": f"C4: {count} walks, U_i=expm(-i t_i H_i), psi_i=U_i psi0 (uniform)",
-                   "lanes": args.lanes, "sum_n3": float(sum(float(n) ** 3 for n in ns)),
+                   "lanes": args.lanes, "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
+                   "sum_n3": float(sum(float(n) ** 3 for n in ns)),
                    "s_histogram": {str(k): s_list.count(k) for k in sorted(set(s_list))},
                    "structured_walks": int(sum(x["structured"] for x in st)),
                    "model_tflops": flops / (dev * 1e-3) / 1e12,
@@ -624,7 +628,8 @@ def main():
     ap.add_argument("--c3-ref", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--c4-ref", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--c4-count", type=int, default=C4_COUNT)
-    ap.add_argument("--lanes", type=int, default=8, help="C4: concurrent lanes (streams + host threads)")
+    ap.add_argument("--lanes", type=int, default=16,
+                    help="C4: concurrent lanes (streams + host threads, at most 16); sets GPU_MAX_HW_QUEUES")
     ap.add_argument("--no-variants", action="store_true", help="c4: skip the fp16x2 and one-lane runs")
     ap.add_argument("--config", choices=["c2", "c3", "c4", "c5"], default="c2",
                     help="c2: N=10^4 single expm (default); c3: precision study at N=10^4; "
