@@ -29,6 +29,24 @@ if ROOT not in sys.path:
 
 FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
+FP16_PEAK_TFLOPS = 2500.0  # dense fp16 / bf16 MFMA
+TRAFFIC_PROFILE = "profiles/squaring_er10000_c64_counters.md"
+
+
+def _profiled_traffic(key):
+    """HBM bytes per launch of a squaring kernel at N = 10^4, as measured by
+    the profiler counter runs summarised in TRAFFIC_PROFILE (None if absent)."""
+    if key is None:
+        return None
+    try:
+        with open(os.path.join(ROOT, TRAFFIC_PROFILE)) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == key:
+                    return float(parts[2])
+    except OSError:
+        pass
+    return None
 
 
 def work_flops(n, s, products=6):
@@ -181,6 +199,24 @@ def run_c5(args):
                            "oracle": "FP64 eigendecomposition of G = Im(A) (same rounded input)"}
     elif ref_error:
         res["accuracy"] = {"oracle_error": ref_error}
+    if not args.no_split:
+        # the same walks with the expm squarings in fp16x2
+        eng.set_precision("fp16x2")
+        eng.run_resident()
+        eng.propagate(psi0, nsteps)
+        hm = []
+        for _ in range(args.steps):
+            eng.run_resident()
+            out_h = eng.propagate(psi0, nsteps)
+            sth = eng.stats()
+            hm.append(sth["ms_total"] + sth["ms_propagate"])
+        sp = {"precision": "fp16x2", "walks_per_s": walks / (float(np.median(hm)) / 1000.0),
+              "ms_per_step": float(np.median(hm))}
+        if ref is not None:
+            sp["worst_infidelity"] = float(max(_infidelity(ref[k, j], out_h[k, j])
+                                               for k in range(nsteps) for j in range(nstates)))
+        res["split_precision"] = sp
+        eng.set_precision("native")
     eng.free()
     print(json.dumps(res), flush=True)
 
@@ -737,15 +773,28 @@ def main():
     ms_step = 1000.0 * wall / args.steps
     n, s = args.n, st["s"]
 
-    # dominant kernel: the squaring GEMM (s launches per step)
+    # dominant kernel: the squaring (s launches per step); the phase time
+    # comes from HIP events on the stream the squarings run on
     sq_ms = float(np.mean([p["ms_square"] for p in phase])) / max(s, 1)
     alg_flops = 8.0 * n ** 3  # per squaring, 4M complex convention
-    peak = FP32_PEAK_TFLOPS if cdt == np.complex64 else FP64_PEAK_TFLOPS
     achieved = alg_flops / (sq_ms * 1e-3) / 1e12
     T = (n + 127) // 128
     np_pad = T * 128
-    issued = 3 * 2.0 * (T * (T + 1) // 2) * 128 * 128 * np_pad if st["structured"] else 3 * 2.0 * np_pad ** 3
+    kpad = (n + 15) // 16 * 16
+    split = cdt == np.complex64 and args.precision != "native" and st["structured"]
+    if split:
+        npair = 3 if args.precision == "fp16x2" else 6
+        peak = FP16_PEAK_TFLOPS
+        issued = 4 * npair * 2.0 * (T * (T + 1) // 2) * 128 * 128 * kpad
+        kname = f"sq_split_kernel<{args.precision}> (4M complex, 16-bit MFMA, FP32 accumulation)"
+        traffic_key = f"{args.precision}_squaring"
+    else:
+        peak = FP32_PEAK_TFLOPS if cdt == np.complex64 else FP64_PEAK_TFLOPS
+        issued = 3 * 2.0 * (T * (T + 1) // 2) * 128 * 128 * np_pad if st["structured"] else 3 * 2.0 * np_pad ** 3
+        kname = "gemm_nt_kernel (3M symmetric squaring)"
+        traffic_key = "fp32_squaring" if cdt == np.complex64 else None
     issued_tf = issued / (sq_ms * 1e-3) / 1e12
+    traffic = _profiled_traffic(traffic_key) if n == 10000 else None
 
     # accuracy: invariants and, unless disabled, infidelity against an FP64
     # eigendecomposition of the same H computed in a child process
@@ -793,12 +842,15 @@ def main():
                    "m": st["m"], "s": s, "structured": st["structured"]},
         "phases_ms": {k: st[k] for k in ("ms_analyze", "ms_pade", "ms_solve", "ms_square", "ms_output",
                                           "ms_total")},
-        "roofline": {"kernel": "gemm_nt_kernel (3M symmetric squaring)", "bound": "mfma",
+        "roofline": {"kernel": kname, "bound": "mfma",
                      "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-                     "traffic": None, "avg_launch_ms": sq_ms,
+                     "traffic": traffic, "traffic_source": TRAFFIC_PROFILE if traffic else None,
+                     "avg_launch_ms": sq_ms,
                      "issued_tflops": issued_tf, "issued_frac": issued_tf / peak,
                      "note": "achieved = 8 N^3 algorithmic flops per squaring (4M complex convention) / "
-                             "avg launch; issued = real MFMA flops of the 3M half-tile kernel"},
+                             "avg launch; issued = MFMA flops the kernel executes (half tiles; equal to the "
+                             "profiler's SQ_INSTS_VALU_MFMA_MOPS x 512); traffic = HBM bytes per launch from "
+                             "FETCH_SIZE / WRITE_SIZE scaled by a copy of known size"},
         "accuracy": acc,
     }
     if cdt == np.complex64 and args.precision == "native" and not args.no_split:
