@@ -675,6 +675,8 @@ def main():
     ap.add_argument("--no-split", action="store_true",
                     help="c2: skip the fp16x2 split-precision comparison run")
     ap.add_argument("--no-c128", action="store_true", help="c2: skip the complex128 run")
+    ap.add_argument("--kblock", type=int, default=None,
+                    help="c2: accumulation block of the squarings in 16-k tiles (0 = one block)")
     ap.add_argument("--no-api-timing", action="store_true",
                     help="c2: skip the host-inclusive run() and Ctqw.runWalk timings")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -759,6 +761,9 @@ def main():
     eng = (ExpMatComplex64 if cdt == np.complex64 else ExpMatComplex128)(args.n)
     if args.precision != "native":
         eng.set_precision(args.precision)
+    if args.kblock is not None:
+        from fastctqw_amd.fastexpm import _fastexpm_core
+        eng._backend.set_option(_fastexpm_core.OPT_KBLOCK, args.kblock)
     eng.upload(a)
     for _ in range(args.warmup):
         eng.run_resident()

@@ -23,6 +23,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -1862,6 +1863,7 @@ int fexpm_batch_run_resident(fexpm_batch* b, const void* psi0, void* psi) {
     }
     std::vector<std::string> errs(b->lanes);
     std::vector<int> codes(b->lanes, FEXPM_OK);
+    std::vector<double> lane_enqueue_ms(b->lanes, 0.0), lane_done_ms(b->lanes, 0.0), lane_phase1_ms(b->lanes, 0.0);
     auto t0 = std::chrono::steady_clock::now();
     // Two phases per lane: the analyses of all its walks (one host wait),
     // then every walk's stages enqueued back to back with no host wait in
@@ -1880,6 +1882,8 @@ int fexpm_batch_run_resident(fexpm_batch* b, const void* psi0, void* psi) {
         for (size_t q = 0; q < mine.size(); ++q)
           FX_HIP(hipMemcpyAsync(&hs[q], b->d_scal + mine[q], sizeof(Scalars), hipMemcpyDeviceToHost, st));
         FX_HIP(hipStreamSynchronize(st));
+        const auto t_lane0 = std::chrono::steady_clock::now();
+        lane_phase1_ms[l] = std::chrono::duration<double, std::milli>(t_lane0 - t0).count();
         for (size_t q = 0; q < mine.size(); ++q) {
           const int64_t i = mine[q];
           p->set_size((int)b->n[i], st);
@@ -1894,7 +1898,11 @@ int fexpm_batch_run_resident(fexpm_batch* b, const void* psi0, void* psi) {
           FX_HIP(hipEventRecord(b->ev[2 * i + 1], st));
           b->stats[i] = S;
         }
+        const auto t_enq = std::chrono::steady_clock::now();
         FX_HIP(hipStreamSynchronize(st));
+        const auto t_done = std::chrono::steady_clock::now();
+        lane_enqueue_ms[l] = std::chrono::duration<double, std::milli>(t_enq - t_lane0).count();
+        lane_done_ms[l] = std::chrono::duration<double, std::milli>(t_done - t_lane0).count();
         for (int64_t i : mine) {
           int info = 0;
           FX_HIP(hipMemcpy(&info, b->d_info + i, sizeof(int), hipMemcpyDeviceToHost));
@@ -1918,6 +1926,12 @@ int fexpm_batch_run_resident(fexpm_batch* b, const void* psi0, void* psi) {
     worker(0);
     for (auto& x : th) x.join();
     b->last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    // FEXPM_BATCH_TRACE=1: per-lane host timeline (analysis done, stages
+    // enqueued, lane drained) on stderr, to tell host-bound from device-bound
+    if (std::getenv("FEXPM_BATCH_TRACE"))
+      for (int l = 0; l < b->lanes; ++l)
+        std::fprintf(stderr, "lane %d: phase 1 done at %.1f ms; phase-2 enqueue %.1f ms, drained %.1f ms\n", l,
+                     lane_phase1_ms[l], lane_enqueue_ms[l], lane_done_ms[l]);
     for (int l = 0; l < b->lanes; ++l)
       if (codes[l] != FEXPM_OK) throw Error(codes[l], "lane " + std::to_string(l) + ": " + errs[l]);
     if (psi0) FX_HIP(hipMemcpy(psi, b->d_psi, vt * sizeof(double2), hipMemcpyDeviceToHost));
