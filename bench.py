@@ -463,16 +463,19 @@ def run_c4(args):
 
     uni, rnd = _c4_states(ns)
     b = ExpMatBatch(nmax=max(ns), dtype=np.complex64, lanes=args.lanes)
+    b.set_grouped(not args.c4_lanes)
     b.upload(mats)
     for _ in range(args.warmup):
         b.run_resident(uni)
-        print(f"[bench] warmup {b.last_ms():.1f} ms", file=sys.stderr, flush=True)
-    ms = []
+        print(f"[bench] warmup {b.last_ms():.1f} ms {b.report()}", file=sys.stderr, flush=True)
+    ms, dev_ms = [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         psi = b.run_resident(uni)
         ms.append(b.last_ms())
+        dev_ms.append(b.report()["device_ms"])
     wall = time.perf_counter() - t0
+    report = b.report()
     st = [b.stats(i) for i in range(count)]
     dev = float(np.median(ms))
     s_list = [x["s"] for x in st]
@@ -484,16 +487,30 @@ def run_c4(args):
         "vs_baseline": None, "dtype": "c64 (fp32 mfma products, fp64 pade/solve, fp64 psi)",
         "data": "synthetic: seeded d-regular / ER graphs, weights U(0,1], adjacency or Laplacian, t in [0.1,20]",
         "config": {"workload": f"C4: {count} walks, U_i=expm(-i t_i H_i), psi_i=U_i psi0 (uniform)",
+                   "form": "lanes" if args.c4_lanes else "grouped launches, hipGraph, autotuned tiles",
                    "lanes": args.lanes, "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
                    "sum_n3": float(sum(float(n) ** 3 for n in ns)),
                    "s_histogram": {str(k): s_list.count(k) for k in sorted(set(s_list))},
                    "structured_walks": int(sum(x["structured"] for x in st)),
                    "model_tflops": flops / (dev * 1e-3) / 1e12,
                    "wall_ms_per_step_incl_psi_copies": 1000.0 * wall / args.steps},
+        "batch_report": report,
+        "grouped_device_ms": float(np.median(dev_ms)) if report.get("grouped") else None,
     }
     # the same batch with fp16x2 squarings, and walk by walk on one lane (the unbatched form)
     variants = {}
     if not args.no_variants:
+        if not args.c4_lanes:
+            # the previous form: concurrent lanes (streams + host threads)
+            b.set_grouped(False)
+            b.run_resident(uni)
+            lm = []
+            for _ in range(args.steps):
+                b.run_resident(uni)
+                lm.append(b.last_ms())
+            variants["lanes"] = {"walks_per_s": count / (float(np.median(lm)) / 1000.0),
+                                 "ms_per_step": float(np.median(lm)), "lanes": args.lanes}
+            b.set_grouped(True)
         b.set_precision("fp16x2")
         b.run_resident(uni)
         vm = []
@@ -507,6 +524,7 @@ def run_c4(args):
                 max(_infidelity(ref[f"uni{i}"], psi_h[i]) for i in range(count)))
         b.set_precision("native")
         b1 = ExpMatBatch(nmax=max(ns), dtype=np.complex64, lanes=1)
+        b1.set_grouped(False)
         b1.upload(mats)
         b1.run_resident(uni)
         t1 = []
@@ -666,7 +684,9 @@ def main():
     ap.add_argument("--c4-count", type=int, default=C4_COUNT)
     ap.add_argument("--lanes", type=int, default=16,
                     help="C4: concurrent lanes (streams + host threads, at most 16); sets GPU_MAX_HW_QUEUES")
-    ap.add_argument("--no-variants", action="store_true", help="c4: skip the fp16x2 and one-lane runs")
+    ap.add_argument("--no-variants", action="store_true", help="c4: skip the lanes, fp16x2 and one-lane runs")
+    ap.add_argument("--c4-lanes", action="store_true", help="c4: time the concurrent-lanes form instead of "
+                                                             "the grouped launches")
     ap.add_argument("--config", choices=["c2", "c3", "c4", "c5"], default="c2",
                     help="c2: N=10^4 single expm (default); c3: precision study at N=10^4; "
                          "c4: 256 uneven walks; c5: 32 times x 128 states, N=8192")

@@ -1228,6 +1228,8 @@ template <typename W> struct Pipeline : PipelineBase {
 
 }  // namespace fexpm
 
+#include "fexpm_group.hpp"
+
 // ===========================================================================
 // C ABI
 using namespace fexpm;
@@ -1672,9 +1674,16 @@ struct fexpm_batch {
   int* d_info = nullptr;      // per-walk exception bits
   size_t cap_walks = 0;
   std::vector<hipEvent_t> ev;  // per-walk start / end
+  // grouped launches (complex dtypes)
+  std::unique_ptr<GroupRunnerBase> grp;
+  int grouped = 1;
+  int precision = FEXPM_PREC_NATIVE;
+  int spd_solve = 1;
+  fexpm_batch_report report{};
 };
 
 static void batch_release(fexpm_batch* b) {
+  b->grp.reset();
   b->pipes.clear();
   for (hipStream_t s : b->streams) (void)hipStreamDestroy(s);
   b->streams.clear();
@@ -1732,6 +1741,10 @@ int fexpm_batch_create(int dtype, int64_t nmax, int lanes, fexpm_batch** out) {
       p->panel_coop = 0;  // lanes run concurrently: no cooperative launches
       b->pipes.emplace_back(p);
     }
+    if (dtype == FEXPM_COMPLEX64)
+      b->grp.reset(new GroupRunner<float2, float>());
+    else if (dtype == FEXPM_COMPLEX128)
+      b->grp.reset(new GroupRunner<double2, double>());
   });
   if (rc != FEXPM_OK) {
     g_last_error = b->err;
@@ -1766,6 +1779,7 @@ int fexpm_batch_set_option(fexpm_batch* b, int key, int64_t value) {
         return FEXPM_EINVAL;
       }
       for (auto& p : b->pipes) p->precision = (int)value;
+      b->precision = (int)value;
       return FEXPM_OK;
     case FEXPM_OPT_KBLOCK:
       if (value < 0 || value > 1024) {
@@ -1773,9 +1787,27 @@ int fexpm_batch_set_option(fexpm_batch* b, int key, int64_t value) {
         return FEXPM_EINVAL;
       }
       for (auto& p : b->pipes) p->kblock = (int)value;
+      if (b->grp) b->grp->kblock = (int)value;
       return FEXPM_OK;
     case FEXPM_OPT_SPD_SOLVE:
       for (auto& p : b->pipes) p->spd_solve = value ? 1 : 0;
+      b->spd_solve = value ? 1 : 0;
+      return FEXPM_OK;
+    case FEXPM_OPT_BATCH_GROUPED:
+      b->grouped = value ? 1 : 0;
+      return FEXPM_OK;
+    case FEXPM_OPT_AUTOTUNE:
+      if (b->grp) b->grp->autotune = value ? 1 : 0;
+      return FEXPM_OK;
+    case FEXPM_OPT_GRAPH:
+      if (b->grp) b->grp->use_graph = value ? 1 : 0;
+      return FEXPM_OK;
+    case FEXPM_OPT_GROUP_CANDIDATE:
+      if (value < -1 || value >= GROUP_CANDIDATES) {
+        b->err = "candidate must be in [-1, 5]";
+        return FEXPM_EINVAL;
+      }
+      if (b->grp) b->grp->force_candidate((int)value);
       return FEXPM_OK;
   }
   b->err = "unknown or unsupported batch option";
@@ -1833,7 +1865,128 @@ int fexpm_batch_upload(fexpm_batch* b, int64_t count, const int64_t* n, const vo
       FX_HIP(hipEventCreate(&e));
       b->ev.push_back(e);
     }
+    if (b->grp) b->grp->set_layout(b->n, b->off, b->streams[0]);
   });
+}
+
+// psi_i = U_i psi0_i on lane pipeline p after its run of walk i; a walk with
+// non-finite input has no U (its output is NaN), so psi_i is NaN too.
+static void batch_apply(fexpm_batch* b, PipelineBase* p, int64_t i, const fexpm_stats& S, hipStream_t st) {
+  if (S.nonfinite) {
+    const int64_t cnt = b->n[i];
+    fill_nan<double2><<<(unsigned)((cnt + 255) / 256), 256, 0, st>>>(b->d_psi + b->voff[i], cnt);
+    FX_LAUNCH();
+    return;
+  }
+  p->apply(b->d_psi0 + b->voff[i], 1, b->d_psi + b->voff[i], st);
+}
+
+// Grouped run of an uploaded batch of complex walks (see fexpm_batch_report):
+// one analysis over all walks and one read-back of the plans; CTQW-form
+// walks then run as the grouped program on stream 0, other walks on lane
+// pipeline 0 afterwards.
+static void batch_run_grouped(fexpm_batch* b, const void* psi0, void* psi) {
+  const int64_t count = (int64_t)b->n.size();
+  const size_t vt = b->voff.back() + (size_t)b->n.back();
+  hipStream_t st = b->streams[0];
+  const auto t0 = std::chrono::steady_clock::now();
+  if (psi0) FX_HIP(hipMemcpy(b->d_psi0, psi0, vt * sizeof(double2), hipMemcpyHostToDevice));
+  b->grp->analyze(b->d_in, b->d_scal, st);
+  std::vector<Scalars> hs((size_t)count);
+  FX_HIP(hipMemcpyAsync(hs.data(), b->d_scal, (size_t)count * sizeof(Scalars), hipMemcpyDeviceToHost, st));
+  FX_HIP(hipStreamSynchronize(st));
+  std::vector<GroupPlan> plans;
+  std::vector<int64_t> rest;
+  const bool is_complex = b->dtype == FEXPM_COMPLEX64 || b->dtype == FEXPM_COMPLEX128;
+  for (int64_t i = 0; i < count; ++i) {
+    const Scalars& h = hs[(size_t)i];
+    const bool nonfinite = (h.flags & FLAG_NONFINITE) || !std::isfinite(h.norm_plain);
+    const bool structured = is_complex && !(h.flags & (FLAG_NONZERO_RE | FLAG_ASYM));
+    if (nonfinite || !structured) {
+      rest.push_back(i);
+      continue;
+    }
+    GroupPlan P;
+    std::memset(&P, 0, sizeof(P));
+    P.idx = (int)i;
+    P.shifted = h.norm_shift <= h.norm_plain ? 1 : 0;
+    P.mu_re = P.shifted ? h.mu_re : 0.0;
+    P.mu_im = P.shifted ? h.mu_im : 0.0;
+    P.norm = P.shifted ? h.norm_shift : h.norm_plain;
+    P.norm_plain = h.norm_plain;
+    choose_degree(P.norm, P.m, P.s);
+    plans.push_back(P);
+  }
+  const double plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  GroupRunnerBase& G = *b->grp;
+  G.run(plans, b->d_in, b->d_out, b->d_scal, psi0 ? b->d_psi0 : nullptr, b->d_psi, b->voff, st);
+  // walks outside the CTQW form: lane pipeline 0, after the grouped program
+  PipelineBase* p = b->pipes[0].get();
+  for (int64_t i : rest) {
+    p->set_size((int)b->n[i], st);
+    fexpm_stats S;
+    std::memset(&S, 0, sizeof(S));
+    S.n = b->n[i];
+    S.dtype = b->dtype;
+    FX_HIP(hipEventRecord(b->ev[2 * i], st));
+    p->run_planned((const char*)b->d_in + b->off[i] * b->esize, (char*)b->d_out + b->off[i] * b->esize,
+                   hs[(size_t)i], st, S, 1, b->d_info + i);
+    if (psi0) batch_apply(b, p, i, S, st);
+    FX_HIP(hipEventRecord(b->ev[2 * i + 1], st));
+    b->stats[i] = S;
+  }
+  FX_HIP(hipStreamSynchronize(st));
+  // exception bits: Cholesky pivots of grouped walks, lane bits of the rest
+  FX_HIP(hipMemcpy(hs.data(), b->d_scal, (size_t)count * sizeof(Scalars), hipMemcpyDeviceToHost));
+  int64_t failed = -1;
+  for (const GroupPlan& P : plans) {
+    const int64_t i = P.idx;
+    if (hs[(size_t)i].info != 0 && failed < 0) failed = i;
+    fexpm_stats& S = b->stats[i];
+    std::memset(&S, 0, sizeof(S));
+    S.n = b->n[i];
+    S.dtype = b->dtype;
+    S.m = P.m;
+    S.s = P.s;
+    S.structured = 1;
+    S.shifted = P.shifted;
+    S.mu_re = P.mu_re;
+    S.mu_im = P.mu_im;
+    S.norm = P.norm;
+    S.norm_plain = P.norm_plain;
+    S.ms_total = G.device_ms;  // device time of the whole grouped program
+    S.flops_issued = (size_t)i < G.walk_flops.size() ? G.walk_flops[(size_t)i] : 0.0;
+    S.kblock = G.kblock;
+    S.pade_fp64 = b->dtype == FEXPM_COMPLEX64 ? 1 : 0;
+    S.precision = FEXPM_PREC_NATIVE;
+  }
+  if (failed >= 0)
+    throw Error(FEXPM_ESINGULAR, "walk " + std::to_string(failed) + ": Pade denominator not positive definite");
+  for (int64_t i : rest) {
+    int info = 0;
+    FX_HIP(hipMemcpy(&info, b->d_info + i, sizeof(int), hipMemcpyDeviceToHost));
+    float ms = 0.f;
+    FX_HIP(hipEventElapsedTime(&ms, b->ev[2 * i], b->ev[2 * i + 1]));
+    b->stats[i].ms_total = ms;
+    if (info != 0) throw Error(FEXPM_ESINGULAR, "walk " + std::to_string(i) + ": Pade denominator singular");
+  }
+  if (psi0) FX_HIP(hipMemcpy(psi, b->d_psi, vt * sizeof(double2), hipMemcpyDeviceToHost));
+  fexpm_batch_report& R = b->report;
+  std::memset(&R, 0, sizeof(R));
+  R.grouped = 1;
+  R.grouped_walks = (int32_t)plans.size();
+  R.fallback_walks = (int32_t)rest.size();
+  R.launches = G.launches;
+  R.graph = G.graph_used;
+  R.graph_built = G.graph_built;
+  R.tuned = G.tuned;
+  for (int c = 0; c < GC_COUNT; ++c) {
+    R.candidate[c] = G.cand[c];
+    R.class_ms[c] = G.class_ms[c];
+  }
+  R.tune_ms = G.tuned ? G.tune_ms : 0.0;
+  R.device_ms = G.device_ms;
+  R.plan_ms = plan_ms;
 }
 
 // U_i = expm(A_i) for the uploaded batch on the device; with psi0 (packed
@@ -1847,6 +2000,13 @@ int fexpm_batch_run_resident(fexpm_batch* b, const void* psi0, void* psi) {
   const int64_t count = (int64_t)b->n.size();
   if (count == 0) return FEXPM_OK;
   return guarded_b(b, [&] {
+    if (b->grp && b->grouped && b->precision == FEXPM_PREC_NATIVE && b->spd_solve) {
+      const auto t0 = std::chrono::steady_clock::now();
+      batch_run_grouped(b, psi0, psi);
+      b->last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+      return;
+    }
+    std::memset(&b->report, 0, sizeof(b->report));
     size_t vt = b->voff.back() + (size_t)b->n.back();
     if (psi0) FX_HIP(hipMemcpy(b->d_psi0, psi0, vt * sizeof(double2), hipMemcpyHostToDevice));
     std::vector<int64_t> order(count);
@@ -1894,7 +2054,7 @@ int fexpm_batch_run_resident(fexpm_batch* b, const void* psi0, void* psi) {
           FX_HIP(hipEventRecord(b->ev[2 * i], st));
           p->run_planned((const char*)b->d_in + b->off[i] * b->esize, (char*)b->d_out + b->off[i] * b->esize, hs[q],
                          st, S, 1, b->d_info + i);
-          if (psi0) p->apply(b->d_psi0 + b->voff[i], 1, b->d_psi + b->voff[i], st);
+          if (psi0) batch_apply(b, p, i, S, st);
           FX_HIP(hipEventRecord(b->ev[2 * i + 1], st));
           b->stats[i] = S;
         }
@@ -1955,6 +2115,12 @@ int fexpm_batch_get_stats(const fexpm_batch* b, int64_t i, fexpm_stats* out) {
 }
 
 double fexpm_batch_last_ms(const fexpm_batch* b) { return b ? b->last_ms : -1.0; }
+
+int fexpm_batch_get_report(const fexpm_batch* b, fexpm_batch_report* out) {
+  if (!b || !out) return FEXPM_EINVAL;
+  *out = b->report;
+  return FEXPM_OK;
+}
 
 }  // extern "C"
 

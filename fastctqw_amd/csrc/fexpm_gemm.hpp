@@ -188,9 +188,13 @@ __device__ __forceinline__ int xcd_remap(int b, int nblocks) {
 // accumulators cover kblock*16 values of k at a time and are then folded
 // into outer Re = T1 - T2 and Im = T3 - T1 - T2 sums, which bounds the
 // length of every sequential MFMA sum.
-template <typename T, int MODE, bool SYM, bool KBLK = false, class CFG = GemmCfg<T>>
-__global__ __launch_bounds__(CFG::NT)
-void gemm_nt_kernel(GemmArgs<T> g) {
+//
+// gemm_tile computes output tile (tm, tn) of one problem; `sym` selects the
+// lower-triangular form (diagonal tiles keep row >= col, values mirrored).
+// It ends with every LDS read complete, so a workgroup may call it again
+// for another tile.
+template <typename T, int MODE, bool KBLK, class CFG>
+__device__ __forceinline__ void gemm_tile(const GemmArgs<T>& g, int tm, int tn, bool sym) {
   using C = CFG;
   constexpr int BM = C::BM, BN = C::BN, BK = C::BK, PAD = C::PAD, WM = C::WM, WN = C::WN;
   constexpr int WAVES_N = BN / WN;
@@ -203,16 +207,6 @@ void gemm_nt_kernel(GemmArgs<T> g) {
 
   __shared__ __attribute__((aligned(16))) T smem[2][NPL * (TA + TB)];
 
-  int tm, tn;
-  if (SYM) {
-    const int b = xcd_remap((int)blockIdx.x, (int)gridDim.x);
-    tri_tile(b, tm, tn);
-  } else {
-    const int nbn = (int)gridDim.x;
-    const int lin = xcd_remap((int)(blockIdx.y * gridDim.x + blockIdx.x), (int)(gridDim.x * gridDim.y));
-    tm = lin / nbn;
-    tn = lin % nbn;
-  }
   const int row0 = tm * BM, col0 = tn * BN;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wr = (wave / WAVES_N) * WM, wc = (wave % WAVES_N) * WN;
@@ -281,9 +275,9 @@ void gemm_nt_kernel(GemmArgs<T> g) {
       for (int i = 0; i < FI; ++i) {
 #pragma unroll
         for (int j = 0; j < FJ; ++j) {
-          if (MODE == MODE_REAL) {
+          if constexpr (MODE == MODE_REAL) {
             acc[0][i][j] = mfma16(fa[0][i][s], fb[0][j][s], acc[0][i][j]);
-          } else if (MODE == MODE_C4M) {
+          } else if constexpr (MODE == MODE_C4M) {
             acc[0][i][j] = mfma16(fa[0][i][s], fb[0][j][s], acc[0][i][j]);
             acc[0][i][j] = mfma16(-fa[1][i][s], fb[1][j][s], acc[0][i][j]);
             acc[1][i][j] = mfma16(fa[0][i][s], fb[1][j][s], acc[1][i][j]);
@@ -296,7 +290,8 @@ void gemm_nt_kernel(GemmArgs<T> g) {
         }
       }
     }
-    if (KBLK && ((kt + 1) % g.kblock == 0 || kt + 1 == KT)) {
+    if constexpr (KBLK) {
+      if ((kt + 1) % g.kblock == 0 || kt + 1 == KT) {
 #pragma unroll
       for (int i = 0; i < FI; ++i) {
 #pragma unroll
@@ -307,6 +302,7 @@ void gemm_nt_kernel(GemmArgs<T> g) {
           acc[1][i][j] = Acc{};
           acc[2][i][j] = Acc{};
         }
+      }
       }
     }
     if (kt + 1 < KT) {
@@ -322,7 +318,7 @@ void gemm_nt_kernel(GemmArgs<T> g) {
 
   // ---- epilogue ----
   const Epilogue<T>& e = g.epi;
-  const bool diag_tile = SYM && (tm == tn);
+  const bool diag_tile = sym && (tm == tn);
 #pragma unroll
   for (int i = 0; i < FI; ++i) {
 #pragma unroll
@@ -334,12 +330,12 @@ void gemm_nt_kernel(GemmArgs<T> g) {
         if (row >= g.m || col >= g.n) continue;
         if (diag_tile && row < col) continue;
         double vr, vi = 0.0;
-        if (MODE == MODE_REAL) {
+        if constexpr (MODE == MODE_REAL) {
           vr = (double)acc[0][i][j][r];
-        } else if (MODE == MODE_C4M) {
+        } else if constexpr (MODE == MODE_C4M) {
           vr = (double)acc[0][i][j][r];
           vi = (double)acc[1][i][j][r];
-        } else if (KBLK) {
+        } else if constexpr (KBLK) {
           vr = (double)outer[0][i][j][r];
           vi = (double)outer[1][i][j][r];
         } else {
@@ -379,13 +375,84 @@ void gemm_nt_kernel(GemmArgs<T> g) {
         if (row == col) outr += (T)e.diag;
         e.cr[o] = outr;
         if (NPL == 2) e.ci[o] = outi;
-        if (SYM && row != col) {
+        if (sym && row != col) {
           const int64_t om = (int64_t)col * e.ldc + row;
           e.cr[om] = outr;
           if (NPL == 2) e.ci[om] = outi;
         }
       }
     }
+  }
+}
+
+// One problem per launch: a rectangular tile grid (gridDim.y x gridDim.x) or,
+// with SYM, T(T+1)/2 lower-triangular tiles; consecutive tiles are spread
+// over the XCDs by xcd_remap.
+template <typename T, int MODE, bool SYM, bool KBLK = false, class CFG = GemmCfg<T>>
+__global__ __launch_bounds__(CFG::NT)
+void gemm_nt_kernel(GemmArgs<T> g) {
+  int tm, tn;
+  if (SYM) {
+    const int b = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    tri_tile(b, tm, tn);
+  } else {
+    const int nbn = (int)gridDim.x;
+    const int lin = xcd_remap((int)(blockIdx.y * gridDim.x + blockIdx.x), (int)(gridDim.x * gridDim.y));
+    tm = lin / nbn;
+    tn = lin % nbn;
+  }
+  gemm_tile<T, MODE, KBLK, CFG>(g, tm, tn, SYM);
+}
+
+// ---------------------------------------------------------------------------
+// Grouped launches: many independent problems (one per walk of a batch, or
+// several per walk) in one persistent launch.  Problem p owns the tiles
+// [tile0, tile0 + tiles) of the launch; problems are listed by the host in
+// decreasing k, so the longest tiles are dispatched first.  Workgroup b
+// takes tiles b, b + gridDim.x, ... of the chunk-remapped order.
+template <typename T> struct GroupGemm {
+  GemmArgs<T> g;
+  int tile0;    // first tile of this problem within the launch
+  int tiles;    // number of tiles
+  int tiles_n;  // column tiles of a rectangular problem
+  int sym;      // 1: lower-triangular tiles with mirrored stores
+};
+
+// Tile order of a grouped launch: workgroups are dispatched round-robin
+// over the 8 XCDs (b % 8), so logical tile t is placed such that every XCD
+// receives runs of `chunk` consecutive tiles (sharing operand rows in its
+// L2) while chunks alternate between XCDs (balancing uneven tiles).
+// A bijection on [0, total): the tail beyond whole 8*chunk groups maps to itself.
+__device__ __forceinline__ int xcd_chunk_remap(int b, int total, int chunk) {
+  constexpr int NX = 8;
+  const int grp = NX * chunk;
+  const int full = (total / grp) * grp;
+  if (chunk <= 1 || b >= full) return b;
+  const int x = b % NX, slot = b / NX;
+  return ((slot / chunk) * NX + x) * chunk + slot % chunk;
+}
+
+template <typename T, int MODE, bool KBLK, class CFG>
+__global__ __launch_bounds__(CFG::NT)
+void gemm_group_kernel(const GroupGemm<T>* __restrict__ probs, int nprob, int total, int chunk) {
+  for (int b = (int)blockIdx.x; b < total; b += (int)gridDim.x) {
+    const int t = xcd_chunk_remap(b, total, chunk);
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (probs[mid].tile0 <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    const GroupGemm<T>& P = probs[lo];
+    const int local = t - P.tile0;
+    int tm, tn;
+    if (P.sym) {
+      tri_tile(local, tm, tn);
+    } else {
+      tm = local / P.tiles_n;
+      tn = local % P.tiles_n;
+    }
+    gemm_tile<T, MODE, KBLK, CFG>(P.g, tm, tn, P.sym != 0);
   }
 }
 

@@ -1,0 +1,1177 @@
+// Grouped execution of a batch of CTQW walks of uneven order and time.
+//
+// Every stage of the structured pipeline (A = i G, G real symmetric) is one
+// launch over all walks that need it: the analysis, the scaled input, each
+// Pade product, the Cholesky panels of the FP64 solve, and each squaring
+// round r (walks with s_i > r).  A launch is described by a device array of
+// problems; workgroup b finds its problem by a binary search over the
+// problems' first blocks (element-wise stages) or first tiles (GEMM
+// stages).  GEMM stages list their problems by decreasing k, so the longest
+// tiles start first.
+//
+// The host turns the per-walk plans (Pade degree m_i, squarings s_i, trace
+// shift) into a program of stages, "materialises" it for one choice of tile
+// shapes (descriptor arrays in one device buffer), and enqueues it.  The
+// sequence after the analysis is captured once into a hipGraph and
+// replayed while the plans stay the same.  Tile shapes are chosen by an
+// autotuner: the program is run once per candidate, every GEMM launch is
+// timed with events, and each class of launches (squarings, symmetric FP64
+// products, solve updates) keeps its fastest candidate.
+//
+// Memory per walk (np = order padded to 64, P = np^2): FP64 planes
+// a, u, v, p2, p4, p6, t1, t2, x0 (9P) and the inverses of the 64 x 64
+// diagonal blocks of the Cholesky factor (64 np); squaring planes in the
+// working precision W (4P); analysis and diagonal buffers.  p2..t1 are
+// contiguous and become the solve's work area [Re Z; Im Z], S -> L, V V^T
+// once the Pade products are done.  Padding columns of every plane stay
+// zero: kernels write only the n x n region (or zeros), and the arenas are
+// cleared when the batch's layout changes.
+#pragma once
+#include <deque>
+
+namespace fexpm {
+
+// ---------------------------------------------------------------------------
+// Device side
+
+// Problem p with start[p] <= b < start[p + 1] (start ascending, start[0] = 0).
+__device__ __forceinline__ int group_find(const int* __restrict__ start, int nprob, int b) {
+  int lo = 0, hi = nprob - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <class Op>
+__global__ __launch_bounds__(Op::NT) void group_op_kernel(const typename Op::Prob* __restrict__ probs,
+                                                          const int* __restrict__ start, int nprob) {
+  const int b = (int)blockIdx.x;
+  const int p = group_find(start, nprob, b);
+  Op::run(probs[p], b - start[p]);
+}
+
+__global__ void zero_ints(int* __restrict__ p, int count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = 0;
+}
+
+// Analysis of one walk's input, 32 x 32 tile pairs (see analyze_tiles).
+template <typename Tin> struct OpAnalyze {
+  static constexpr int NT = 256;
+  struct Prob {
+    const Tin* a;
+    int n, nt;
+    double* colpart;
+    double2* diag;
+    int* flags;
+  };
+  static int blocks(const Prob& p) { return p.nt * (p.nt + 1) / 2; }
+  static __device__ __forceinline__ void run(const Prob& p, int b) {
+    analyze_tile_pair<Tin>(p.a, p.n, p.nt, p.colpart, p.diag, p.flags, b);
+  }
+};
+
+// off[j] = sum over row blocks of colpart (fixed order).
+struct OpReduceCols {
+  static constexpr int NT = 256;
+  struct Prob {
+    const double* colpart;
+    int n, nt;
+    double* off;
+  };
+  static int blocks(const Prob& p) { return (p.n + 255) / 256; }
+  static __device__ __forceinline__ void run(const Prob& p, int b) {
+    const int j = b * 256 + (int)threadIdx.x;
+    if (j >= p.n) return;
+    double s = 0.0;
+    for (int q = 0; q < p.nt; ++q) s += p.colpart[(int64_t)q * p.n + j];
+    p.off[j] = s;
+  }
+};
+
+// Trace, mu and the two 1-norms of one walk (one workgroup).
+struct OpFinalize {
+  static constexpr int NT = 1024;
+  struct Prob {
+    const double* off;
+    const double2* diag;
+    int n;
+    const int* flags;
+    Scalars* out;
+  };
+  static int blocks(const Prob&) { return 1; }
+  static __device__ __forceinline__ void run(const Prob& p, int) {
+    finalize_norms_block(p.off, p.diag, p.n, p.flags, p.out);
+  }
+};
+
+// g = (Im(A) - Im(mu) I) * scale into an FP64 plane (structured walks).
+template <typename Tin> struct OpPrepare {
+  static constexpr int NT = 256;
+  struct Prob {
+    const Tin* a;
+    int n;
+    int64_t ld;
+    double* g;
+    double mu_im, scale;
+  };
+  static int blocks(const Prob& p) { return p.n; }
+  static __device__ __forceinline__ void run(const Prob& p, int i) {
+    for (int j = (int)threadIdx.x; j < p.n; j += NT) {
+      double im = InElem<Tin>::im(p.a[(int64_t)i * p.n + j]);
+      if (i == j) im -= p.mu_im;
+      p.g[(int64_t)i * p.ld + j] = im * p.scale;
+    }
+  }
+};
+
+// out = sum_t c_t in_t + diag I on the n x n region (FP64, one row per block).
+struct OpLincomb {
+  static constexpr int NT = 256;
+  struct Prob {
+    double* out;
+    const double* in[4];
+    double c[4];
+    int nin;
+    double diag;
+    int n;
+    int64_t ld;
+  };
+  static int blocks(const Prob& p) { return p.n; }
+  static __device__ __forceinline__ void run(const Prob& p, int i) {
+    for (int j = (int)threadIdx.x; j < p.n; j += NT) {
+      const int64_t o = (int64_t)i * p.ld + j;
+      double s = (i == j) ? p.diag : 0.0;
+      for (int t = 0; t < p.nin; ++t) s += p.c[t] * p.in[t][o];
+      p.out[o] = s;
+    }
+  }
+};
+
+// Stacked [V; Uq] (2np x np, zero padding) for the Cholesky solve.
+struct OpPack {
+  static constexpr int NT = 256;
+  struct Prob {
+    const double* v;
+    const double* u;
+    int n, np;
+    int64_t ld;
+    double* zb;
+  };
+  static int blocks(const Prob& p) { return 2 * p.np; }
+  static __device__ __forceinline__ void run(const Prob& p, int r) {
+    const int i = r < p.np ? r : r - p.np;
+    const double* src = r < p.np ? p.v : p.u;
+    for (int j = (int)threadIdx.x; j < p.np; j += NT)
+      p.zb[(int64_t)r * p.ld + j] = (i < p.n && j < p.n) ? src[(int64_t)i * p.ld + j] : 0.0;
+  }
+};
+
+// FP64 solution X -> split form in W: Y = X - diag(X), d = diag(X).
+template <typename W> struct OpSolution {
+  static constexpr int NT = 256;
+  struct Prob {
+    const double* xr;
+    const double* xi;
+    int64_t ldx;
+    W* wr;
+    W* wi;
+    int64_t ldw;
+    int n;
+    double2* d;
+  };
+  static int blocks(const Prob& p) { return p.n; }
+  static __device__ __forceinline__ void run(const Prob& p, int i) {
+    for (int j = (int)threadIdx.x; j < p.n; j += NT) {
+      double vr = p.xr[(int64_t)i * p.ldx + j], vi = p.xi[(int64_t)i * p.ldx + j];
+      if (i == j) {
+        p.d[i] = make_double2(vr, vi);
+        vr = 0.0;
+        vi = 0.0;
+      }
+      p.wr[(int64_t)i * p.ldw + j] = (W)vr;
+      p.wi[(int64_t)i * p.ldw + j] = (W)vi;
+    }
+  }
+};
+
+// U = e^mu (Y + diag(d)) in the caller's interleaved layout.
+template <typename W, typename Tout> struct OpOutput {
+  static constexpr int NT = 256;
+  struct Prob {
+    const W* yr;
+    const W* yi;
+    int64_t ld;
+    int n;
+    const double2* d;
+    double ph_re, ph_im;
+    Tout* out;
+  };
+  static int blocks(const Prob& p) { return p.n; }
+  static __device__ __forceinline__ void run(const Prob& p, int i) {
+    for (int j = (int)threadIdx.x; j < p.n; j += NT) {
+      double a, b;
+      if (i == j) {
+        a = p.d[i].x;
+        b = p.d[i].y;
+      } else {
+        a = (double)p.yr[(int64_t)i * p.ld + j];
+        b = (double)p.yi[(int64_t)i * p.ld + j];
+      }
+      p.out[(int64_t)i * p.n + j] = OutElem<Tout>::make(p.ph_re * a - p.ph_im * b, p.ph_re * b + p.ph_im * a);
+    }
+  }
+};
+
+// psi = e^mu (Y + diag(d)) psi0 with FP64 accumulation; one wavefront per row.
+template <typename W> struct OpApply {
+  static constexpr int NT = 256;
+  struct Prob {
+    const W* yr;
+    const W* yi;
+    int64_t ld;
+    int n;
+    const double2* d;
+    const double2* psi0;
+    double ph_re, ph_im;
+    double2* out;
+  };
+  static int blocks(const Prob& p) { return (p.n + 3) / 4; }
+  static __device__ __forceinline__ void run(const Prob& p, int b) {
+    const int lane = (int)threadIdx.x & 63;
+    const int row = b * 4 + ((int)threadIdx.x >> 6);
+    if (row >= p.n) return;
+    double ar = 0.0, ai = 0.0;
+    for (int j = lane; j < p.n; j += 64) {
+      const double a = (double)p.yr[(int64_t)row * p.ld + j], c = (double)p.yi[(int64_t)row * p.ld + j];
+      const double2 v = p.psi0[j];
+      ar += a * v.x - c * v.y;
+      ai += a * v.y + c * v.x;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      ar += __shfl_xor(ar, off);
+      ai += __shfl_xor(ai, off);
+    }
+    if (lane == 0) {
+      const double2 dd = p.d[row], v = p.psi0[row];
+      ar += dd.x * v.x - dd.y * v.y;
+      ai += dd.x * v.y + dd.y * v.x;
+      p.out[row] = make_double2(p.ph_re * ar - p.ph_im * ai, p.ph_re * ai + p.ph_im * ar);
+    }
+  }
+};
+
+// Diagonal 64 x 64 block of the Cholesky factor and its inverse.
+struct OpChol {
+  static constexpr int NT = 256;
+  struct Prob {
+    double* s;
+    int64_t ld;
+    int k0, n;
+    double* inv;
+    Scalars* sc;
+  };
+  static int blocks(const Prob&) { return 1; }
+  static __device__ __forceinline__ void run(const Prob& p, int) { chol_diag_block(p.s, p.ld, p.k0, p.n, p.inv, p.sc); }
+};
+
+// ---------------------------------------------------------------------------
+// Host side
+
+enum GroupKind {
+  GK_ANALYZE,
+  GK_REDUCE,
+  GK_FINALIZE,
+  GK_PREPARE,
+  GK_LINCOMB,
+  GK_PACK,
+  GK_SOLUTION,
+  GK_OUTPUT,
+  GK_APPLY,
+  GK_CHOL,
+  GK_GEMM
+};
+
+// Tuning classes of GEMM launches.
+enum GroupClass { GC_SQUARE = 0, GC_SYM64 = 1, GC_SOLVE = 2, GC_COUNT = 3 };
+
+// Autotuner candidates: tile configuration x dispatch.  Dispatch 0: one
+// workgroup per tile in launch order; 1: one workgroup per tile, runs of 4
+// tiles per XCD; 2: persistent grid (as many workgroups as fit on the
+// device at once), runs of 4 tiles per XCD.
+constexpr int GROUP_CANDIDATES = 6;
+static inline int cand_cfg(int c) { return c / 3; }
+static inline int cand_dispatch(int c) { return c % 3; }
+
+struct GroupStage {
+  int kind = 0;
+  int cls = -1;      // GEMM: tuning class
+  int mode = 0;      // GEMM: MODE_REAL or MODE_C3M
+  bool f32 = false;  // GEMM: FP32 operands (blocked-accumulation squarings)
+  size_t psize = 0;
+  std::vector<unsigned char> probs;
+  std::vector<int> blocks;  // element-wise: workgroups per problem
+  std::vector<char> sym;    // GEMM: lower-triangular problem
+  std::vector<int> walk;    // GEMM: owning walk (flop accounting)
+  std::vector<int> inplace; // GEMM: output overwrites operand A (needs one column tile)
+
+  template <class P> void add(const P& p, int nb) {
+    psize = sizeof(P);
+    const size_t o = probs.size();
+    probs.resize(o + sizeof(P));
+    std::memcpy(probs.data() + o, &p, sizeof(P));
+    blocks.push_back(nb);
+  }
+  template <typename T> void add_gemm(const GemmArgs<T>& g, bool is_sym, int w, bool in_place = false) {
+    add(g, 0);
+    sym.push_back(is_sym ? 1 : 0);
+    walk.push_back(w);
+    inplace.push_back(in_place ? 1 : 0);
+  }
+  int count() const { return psize ? (int)(probs.size() / psize) : 0; }
+};
+
+struct GroupLaunch {
+  int kind, cls, mode;
+  bool f32;
+  int cfg, dispatch;
+  int nprob, total, grid, chunk;
+  size_t off_probs, off_start;
+};
+
+// Tile edge of a GEMM configuration (all grouped configurations are square).
+static inline int group_tile(bool f32, int cfg) {
+  if (f32) return cfg == 0 ? FCfgSmall::BM : GemmCfg<float>::BM;
+  return cfg == 0 ? GemmCfg<double>::BM : DCfg4w::BM;
+}
+
+template <typename KernelT> static int group_capacity(KernelT kernel, int nt) {
+  static std::map<const void*, int> cache;
+  static std::mutex mtx;
+  std::lock_guard<std::mutex> lk(mtx);
+  auto it = cache.find((const void*)kernel);
+  if (it != cache.end()) return it->second;
+  int dev = 0, per_cu = 0, cap = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)kernel, nt, 0) == hipSuccess)
+    cap = per_cu * prop.multiProcessorCount;
+  cap = std::max(cap, 8);
+  cache[(const void*)kernel] = cap;
+  return cap;
+}
+
+template <typename T, int MODE, bool KBLK, class CFG> static int group_gemm_capacity() {
+  return group_capacity(gemm_group_kernel<T, MODE, KBLK, CFG>, CFG::NT);
+}
+
+// A materialised program: descriptor arrays in one device buffer.
+struct GroupProgram {
+  std::vector<GroupLaunch> launches;
+  unsigned char* d_arena = nullptr;
+  size_t cap = 0;
+  std::vector<double> walk_flops;
+  ~GroupProgram() {
+    if (d_arena) (void)hipFree(d_arena);
+  }
+
+  void materialize(const std::deque<GroupStage>& stages, const int* cand, int nwalks) {
+    std::vector<unsigned char> host;
+    auto align = [&]() { host.resize((host.size() + 255) / 256 * 256); };
+    launches.clear();
+    walk_flops.assign((size_t)nwalks, 0.0);
+    for (const GroupStage& S : stages) {
+      const int np_ = S.count();
+      if (np_ == 0) continue;
+      GroupLaunch L;
+      std::memset(&L, 0, sizeof(L));
+      L.kind = S.kind;
+      L.cls = S.cls;
+      L.mode = S.mode;
+      L.f32 = S.f32;
+      L.nprob = np_;
+      L.chunk = 1;
+      if (S.kind != GK_GEMM) {
+        std::vector<int> start((size_t)np_);
+        int tot = 0;
+        for (int p = 0; p < np_; ++p) {
+          start[p] = tot;
+          tot += S.blocks[p];
+        }
+        L.total = tot;
+        L.grid = tot;
+        align();
+        L.off_probs = host.size();
+        host.insert(host.end(), S.probs.begin(), S.probs.end());
+        align();
+        L.off_start = host.size();
+        host.resize(host.size() + start.size() * sizeof(int));
+        std::memcpy(host.data() + L.off_start, start.data(), start.size() * sizeof(int));
+      } else {
+        const int c = cand[S.cls];
+        L.cfg = cand_cfg(c);
+        L.dispatch = cand_dispatch(c);
+        const int tile = group_tile(S.f32, L.cfg);
+        // problems by decreasing k (stable)
+        std::vector<int> order((size_t)np_);
+        for (int p = 0; p < np_; ++p) order[p] = p;
+        auto kof = [&](int p) {
+          return S.f32 ? reinterpret_cast<const GemmArgs<float>*>(S.probs.data() + (size_t)p * S.psize)->k
+                       : reinterpret_cast<const GemmArgs<double>*>(S.probs.data() + (size_t)p * S.psize)->k;
+        };
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return kof(x) > kof(y); });
+        const double mult = S.mode == MODE_REAL ? 1.0 : (S.mode == MODE_C4M ? 4.0 : 3.0);
+        const size_t gsize = S.f32 ? sizeof(GroupGemm<float>) : sizeof(GroupGemm<double>);
+        std::vector<unsigned char> buf((size_t)np_ * gsize);
+        int tot = 0;
+        for (int q = 0; q < np_; ++q) {
+          const int p = order[q];
+          int m, n, k;
+          if (S.f32) {
+            const GemmArgs<float>& g = *reinterpret_cast<const GemmArgs<float>*>(S.probs.data() + (size_t)p * S.psize);
+            m = g.m; n = g.n; k = g.k;
+          } else {
+            const GemmArgs<double>& g = *reinterpret_cast<const GemmArgs<double>*>(S.probs.data() + (size_t)p * S.psize);
+            m = g.m; n = g.n; k = g.k;
+          }
+          if (m <= 0 || n <= 0 || k <= 0 || k % 16 != 0) throw Error(FEXPM_EINVAL, "grouped GEMM: bad problem shape");
+          const int tm = (m + tile - 1) / tile, tn = (n + tile - 1) / tile;
+          if (S.inplace[p] && tn != 1) throw Error(FEXPM_EINVAL, "grouped GEMM: in-place problem wider than a tile");
+          const bool sy = S.sym[p] != 0;
+          if (sy && m != n) throw Error(FEXPM_EINVAL, "grouped GEMM: symmetric problem must be square");
+          const int tiles = sy ? tm * (tm + 1) / 2 : tm * tn;
+          walk_flops[(size_t)S.walk[p]] += mult * 2.0 * tiles * (double)tile * tile * (double)k;
+          if (S.f32) {
+            GroupGemm<float> G;
+            std::memset(&G, 0, sizeof(G));
+            G.g = *reinterpret_cast<const GemmArgs<float>*>(S.probs.data() + (size_t)p * S.psize);
+            G.tile0 = tot;
+            G.tiles = tiles;
+            G.tiles_n = tn;
+            G.sym = sy ? 1 : 0;
+            std::memcpy(buf.data() + (size_t)q * gsize, &G, sizeof(G));
+          } else {
+            GroupGemm<double> G;
+            std::memset(&G, 0, sizeof(G));
+            G.g = *reinterpret_cast<const GemmArgs<double>*>(S.probs.data() + (size_t)p * S.psize);
+            G.tile0 = tot;
+            G.tiles = tiles;
+            G.tiles_n = tn;
+            G.sym = sy ? 1 : 0;
+            std::memcpy(buf.data() + (size_t)q * gsize, &G, sizeof(G));
+          }
+          tot += tiles;
+        }
+        L.total = tot;
+        L.chunk = L.dispatch == 0 ? 1 : 4;
+        int capw = tot;
+        if (L.dispatch == 2) {
+          if (S.f32)
+            capw = L.cfg == 0 ? group_gemm_capacity<float, MODE_C3M, true, FCfgSmall>()
+                              : group_gemm_capacity<float, MODE_C3M, true, GemmCfg<float>>();
+          else if (S.mode == MODE_REAL)
+            capw = L.cfg == 0 ? group_gemm_capacity<double, MODE_REAL, false, GemmCfg<double>>()
+                              : group_gemm_capacity<double, MODE_REAL, false, DCfg4w>();
+          else
+            capw = L.cfg == 0 ? group_gemm_capacity<double, MODE_C3M, false, GemmCfg<double>>()
+                              : group_gemm_capacity<double, MODE_C3M, false, DCfg4w>();
+          capw = std::max(8, capw / 8 * 8);  // a multiple of the XCD count keeps each workgroup on its runs
+        }
+        L.grid = std::min(tot, capw);
+        align();
+        L.off_probs = host.size();
+        host.insert(host.end(), buf.begin(), buf.end());
+      }
+      launches.push_back(L);
+    }
+    if (host.size() > cap) {
+      if (d_arena) (void)hipFree(d_arena);
+      d_arena = nullptr;
+      cap = 0;
+      FX_HIP(hipMalloc(&d_arena, host.size()));
+      cap = host.size();
+    }
+    if (!host.empty()) FX_HIP(hipMemcpy(d_arena, host.data(), host.size(), hipMemcpyHostToDevice));
+  }
+};
+
+// Launch of one materialised stage.
+template <typename Tin, typename W>
+static void group_enqueue_one(const GroupLaunch& L, unsigned char* base, hipStream_t st) {
+  using OutOp = OpOutput<W, Tin>;
+  const void* pp = base + L.off_probs;
+  const int* ps = reinterpret_cast<const int*>(base + L.off_start);
+  const unsigned grid = (unsigned)L.grid;
+#define FX_OP(OP)                                                                                          \
+  group_op_kernel<OP><<<grid, OP::NT, 0, st>>>(reinterpret_cast<const typename OP::Prob*>(pp), ps, L.nprob); \
+  break;
+  switch (L.kind) {
+    case GK_ANALYZE: FX_OP(OpAnalyze<Tin>)
+    case GK_REDUCE: FX_OP(OpReduceCols)
+    case GK_FINALIZE: FX_OP(OpFinalize)
+    case GK_PREPARE: FX_OP(OpPrepare<Tin>)
+    case GK_LINCOMB: FX_OP(OpLincomb)
+    case GK_PACK: FX_OP(OpPack)
+    case GK_SOLUTION: FX_OP(OpSolution<W>)
+    case GK_OUTPUT: FX_OP(OutOp)
+    case GK_APPLY: FX_OP(OpApply<W>)
+    case GK_CHOL: FX_OP(OpChol)
+    case GK_GEMM: {
+#define FX_GG(T, MODE, KB, CF)                                                                             \
+  gemm_group_kernel<T, MODE, KB, CF><<<grid, CF::NT, 0, st>>>(reinterpret_cast<const GroupGemm<T>*>(pp), \
+                                                               L.nprob, L.total, L.chunk)
+      if (L.f32) {
+        if (L.cfg == 0) FX_GG(float, MODE_C3M, true, FCfgSmall);
+        else FX_GG(float, MODE_C3M, true, GemmCfg<float>);
+      } else if (L.mode == MODE_REAL) {
+        if (L.cfg == 0) FX_GG(double, MODE_REAL, false, GemmCfg<double>);
+        else FX_GG(double, MODE_REAL, false, DCfg4w);
+      } else {
+        if (L.cfg == 0) FX_GG(double, MODE_C3M, false, GemmCfg<double>);
+        else FX_GG(double, MODE_C3M, false, DCfg4w);
+      }
+#undef FX_GG
+      break;
+    }
+    default: throw Error(FEXPM_EINVAL, "grouped launch: unknown stage");
+  }
+#undef FX_OP
+  FX_LAUNCH();
+}
+
+// Per-walk plan of a grouped (structured, finite) walk.
+struct GroupPlan {
+  int idx;  // index in the batch
+  int m, s;
+  int shifted;
+  double mu_re, mu_im, norm, norm_plain;
+};
+
+// Runner for one batch dtype: Tin = float2 (complex64, FP32 squarings) or
+// double2 (complex128, FP64 squarings); Pade products and solve in FP64.
+struct GroupRunnerBase {
+  virtual ~GroupRunnerBase() {}
+  virtual void set_layout(const std::vector<int64_t>& n, const std::vector<size_t>& off, hipStream_t st) = 0;
+  virtual void analyze(const void* d_in, Scalars* d_scal, hipStream_t st) = 0;
+  // Builds (or reuses) the program for these plans, tunes it when asked,
+  // and runs it on st; psi0/psi are packed device states (or null).
+  virtual void run(const std::vector<GroupPlan>& plans, const void* d_in, void* d_out, Scalars* d_scal,
+                   const double2* d_psi0, double2* d_psi, const std::vector<size_t>& voff, hipStream_t st) = 0;
+  // testing: every class runs candidate c (no tuning); -1 = autotuner
+  virtual void force_candidate(int c) = 0;
+  int kblock = 8;
+  int autotune = 1;
+  int use_graph = 1;
+  int forced = -1;
+  // report of the last run
+  int launches = 0, graph_used = 0, graph_built = 0, tuned = 0;
+  int cand[GC_COUNT] = {0, 0, 0};
+  double class_ms[GC_COUNT] = {0, 0, 0};
+  double tune_ms = 0.0, device_ms = 0.0;
+  std::vector<double> walk_flops;  // by batch index
+};
+
+template <typename Tin, typename W> struct GroupRunner : GroupRunnerBase {
+  struct WalkMem {
+    int n = 0, np = 0;
+    double* f64 = nullptr;  // 9 planes of np^2, then 64 np inverse blocks
+    W* w = nullptr;         // 4 planes of np^2
+    double* colpart = nullptr;
+    double* off = nullptr;
+    double2* diag = nullptr;
+    double2* dsq = nullptr;  // 2 x n
+    int* flags = nullptr;
+    size_t in_off = 0;       // element offset of the walk's input/output
+  };
+  std::vector<WalkMem> wm;
+  double* f64_arena = nullptr;
+  W* w_arena = nullptr;
+  unsigned char* small_arena = nullptr;
+  size_t f64_cap = 0, w_cap = 0, small_cap = 0;
+  int* d_flags = nullptr;  // all walks' flags, contiguous (one clear launch)
+  int nwalk = 0;
+
+  GroupProgram analysis;   // analysis program (per layout)
+  const void* analysis_in = nullptr;
+  Scalars* analysis_scal = nullptr;
+  GroupProgram prog;       // program of the current plans
+  std::deque<GroupStage> stages;
+  std::vector<double> plan_key;
+  const void* key_in = nullptr;
+  void* key_out = nullptr;
+  const void* key_psi0 = nullptr;
+  bool tuned_layout = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<hipEvent_t> evpool;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  GroupRunner() {
+    FX_HIP(hipEventCreate(&ev0));
+    FX_HIP(hipEventCreate(&ev1));
+  }
+  ~GroupRunner() override {
+    drop_graph();
+    if (f64_arena) (void)hipFree(f64_arena);
+    if (w_arena) (void)hipFree(w_arena);
+    if (small_arena) (void)hipFree(small_arena);
+    for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+  }
+
+  void force_candidate(int c) override {
+    forced = c;
+    drop_graph();
+    stages.clear();
+    plan_key.clear();
+    tuned_layout = false;
+    for (int k = 0; k < GC_COUNT; ++k) cand[k] = c >= 0 ? c : 0;
+  }
+
+  void drop_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    gexec = nullptr;
+    graph = nullptr;
+  }
+
+  static int64_t pad64(int64_t x) { return (std::max<int64_t>(x, 1) + 63) / 64 * 64; }
+
+  void set_layout(const std::vector<int64_t>& n, const std::vector<size_t>& off, hipStream_t) override {
+    const int cnt = (int)n.size();
+    bool same = cnt == nwalk;
+    for (int i = 0; same && i < cnt; ++i) same = wm[i].n == (int)n[i] && wm[i].in_off == off[i];
+    if (same) return;
+    drop_graph();
+    stages.clear();
+    plan_key.clear();
+    tuned_layout = false;
+    analysis_in = nullptr;
+    nwalk = cnt;
+    wm.assign(cnt, WalkMem());
+    size_t f64_tot = 0, w_tot = 0, small_tot = 0;
+    auto al = [](size_t x) { return (x + 255) / 256 * 256; };
+    std::vector<size_t> fo(cnt), wo(cnt), so(cnt);
+    for (int i = 0; i < cnt; ++i) {
+      const int64_t np_ = pad64(n[i]);
+      const size_t P = (size_t)np_ * (size_t)np_;
+      fo[i] = f64_tot;
+      f64_tot += al((9 * P + 64 * (size_t)np_) * sizeof(double));
+      wo[i] = w_tot;
+      w_tot += al(4 * P * sizeof(W));
+      so[i] = small_tot;
+      const int nt = (int)((n[i] + 31) / 32);
+      small_tot += al((size_t)nt * n[i] * sizeof(double)) + al(n[i] * sizeof(double)) +
+                   al(n[i] * sizeof(double2)) + al(2 * n[i] * sizeof(double2));
+    }
+    const size_t flags_off = small_tot;
+    small_tot += al((size_t)std::max(cnt, 1) * sizeof(int));
+    auto grow = [](auto*& p, size_t& capb, size_t need) {
+      if (need > capb) {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        capb = 0;
+        FX_HIP(hipMalloc((void**)&p, need));
+        capb = need;
+      }
+    };
+    grow(f64_arena, f64_cap, std::max<size_t>(f64_tot, 256));
+    grow(w_arena, w_cap, std::max<size_t>(w_tot, 256));
+    grow(small_arena, small_cap, std::max<size_t>(small_tot, 256));
+    // padding strips must read as zero: clear everything once per layout
+    FX_HIP(hipMemset(f64_arena, 0, f64_tot));
+    FX_HIP(hipMemset(w_arena, 0, w_tot));
+    FX_HIP(hipMemset(small_arena, 0, small_tot));
+    d_flags = reinterpret_cast<int*>(small_arena + flags_off);
+    for (int i = 0; i < cnt; ++i) {
+      WalkMem& M = wm[i];
+      M.n = (int)n[i];
+      M.np = (int)pad64(n[i]);
+      M.in_off = off[i];
+      M.f64 = reinterpret_cast<double*>(reinterpret_cast<unsigned char*>(f64_arena) + fo[i]);
+      M.w = reinterpret_cast<W*>(reinterpret_cast<unsigned char*>(w_arena) + wo[i]);
+      unsigned char* sp = small_arena + so[i];
+      const int nt = (M.n + 31) / 32;
+      M.colpart = reinterpret_cast<double*>(sp);
+      sp += al((size_t)nt * M.n * sizeof(double));
+      M.off = reinterpret_cast<double*>(sp);
+      sp += al(M.n * sizeof(double));
+      M.diag = reinterpret_cast<double2*>(sp);
+      sp += al(M.n * sizeof(double2));
+      M.dsq = reinterpret_cast<double2*>(sp);
+      M.flags = d_flags + i;
+    }
+  }
+
+  // FP64 plane q of walk i: 0 a, 1 u, 2 v, 3 p2, 4 p4, 5 p6, 6 t1, 7 t2, 8 x0.
+  double* fp(int i, int q) const { return wm[i].f64 + (size_t)q * wm[i].np * wm[i].np; }
+  double* dinv(int i) const { return wm[i].f64 + (size_t)9 * wm[i].np * wm[i].np; }
+  // squaring plane of walk i: buffer b in {0, 1}, part 0 re / 1 im
+  W* wp(int i, int b, int part) const { return wm[i].w + (size_t)(2 * b + part) * wm[i].np * wm[i].np; }
+
+  void analyze(const void* d_in, Scalars* d_scal, hipStream_t st) override {
+    if (nwalk == 0) return;
+    if (analysis_in != d_in || analysis_scal != d_scal) {
+      std::deque<GroupStage> S(3);
+      S[0].kind = GK_ANALYZE;
+      S[1].kind = GK_REDUCE;
+      S[2].kind = GK_FINALIZE;
+      for (int i = 0; i < nwalk; ++i) {
+        const WalkMem& M = wm[i];
+        const int nt = (M.n + 31) / 32;
+        typename OpAnalyze<Tin>::Prob a{reinterpret_cast<const Tin*>(d_in) + M.in_off, M.n, nt, M.colpart, M.diag,
+                                        M.flags};
+        S[0].add(a, OpAnalyze<Tin>::blocks(a));
+        OpReduceCols::Prob r{M.colpart, M.n, nt, M.off};
+        S[1].add(r, OpReduceCols::blocks(r));
+        OpFinalize::Prob f{M.off, M.diag, M.n, M.flags, d_scal + i};
+        S[2].add(f, 1);
+      }
+      const int none[GC_COUNT] = {0, 0, 0};
+      analysis.materialize(S, none, nwalk);
+      analysis_in = d_in;
+      analysis_scal = d_scal;
+    }
+    zero_ints<<<(nwalk + 255) / 256, 256, 0, st>>>(d_flags, nwalk);
+    FX_LAUNCH();
+    for (const GroupLaunch& L : analysis.launches) group_enqueue_one<Tin, W>(L, analysis.d_arena, st);
+  }
+
+  static GemmArgs<double> g64(const double* a, const double* b, int64_t ld, int m, int n, int k, double* c) {
+    GemmArgs<double> g;
+    std::memset(&g, 0, sizeof(g));
+    g.ar = a;
+    g.br = b;
+    g.lda = ld;
+    g.ldb = ld;
+    g.m = m;
+    g.n = n;
+    g.k = k;
+    g.epi.cr = c;
+    g.epi.ldc = ld;
+    g.epi.ld_add = ld;
+    g.epi.alpha = 1.0;
+    return g;
+  }
+
+  // The program of the structured pipeline for the given plans.
+  // Stages are held in a deque: references returned by stage() stay valid
+  // while later stages are appended.
+  void build(const std::vector<GroupPlan>& plans, const void* d_in, void* d_out, Scalars* d_scal,
+             const double2* d_psi0, double2* d_psi, const std::vector<size_t>& voff) {
+    stages.clear();
+    auto stage = [&](int kind, int cls = -1, int mode = 0, bool f32 = false) -> GroupStage& {
+      stages.emplace_back();
+      GroupStage& S = stages.back();
+      S.kind = kind;
+      S.cls = cls;
+      S.mode = mode;
+      S.f32 = f32;
+      return S;
+    };
+    const int G = (int)plans.size();
+    // 1. scaled, shifted G_i = (Im A_i - Im(mu_i) I) / 2^s_i
+    {
+      GroupStage& S = stage(GK_PREPARE);
+      for (const GroupPlan& P : plans) {
+        const int i = P.idx;
+        typename OpPrepare<Tin>::Prob p{reinterpret_cast<const Tin*>(d_in) + wm[i].in_off, wm[i].n, wm[i].np,
+                                        fp(i, 0), P.shifted ? P.mu_im : 0.0, std::ldexp(1.0, -P.s)};
+        S.add(p, OpPrepare<Tin>::blocks(p));
+      }
+    }
+    // 2. Pade products (real symmetric, half tiles); A^{2k} = (-1)^k G^{2k}
+    {
+      GroupStage& s1 = stage(GK_GEMM, GC_SYM64, MODE_REAL);
+      for (const GroupPlan& P : plans) {
+        const int i = P.idx, n = wm[i].n, np_ = wm[i].np;
+        s1.add_gemm(g64(fp(i, 0), fp(i, 0), np_, n, n, np_, fp(i, 3)), true, i);  // C2
+      }
+      GroupStage& s2 = stage(GK_GEMM, GC_SYM64, MODE_REAL);
+      for (const GroupPlan& P : plans)
+        if (P.m >= 5) {
+          const int i = P.idx, n = wm[i].n, np_ = wm[i].np;
+          s2.add_gemm(g64(fp(i, 3), fp(i, 3), np_, n, n, np_, fp(i, 4)), true, i);  // C4
+        }
+      GroupStage& s3 = stage(GK_GEMM, GC_SYM64, MODE_REAL);
+      for (const GroupPlan& P : plans)
+        if (P.m >= 7) {
+          const int i = P.idx, n = wm[i].n, np_ = wm[i].np;
+          s3.add_gemm(g64(fp(i, 4), fp(i, 3), np_, n, n, np_, fp(i, 5)), true, i);  // C6
+        }
+      GroupStage& s4 = stage(GK_GEMM, GC_SYM64, MODE_REAL);
+      for (const GroupPlan& P : plans)
+        if (P.m == 9) {
+          const int i = P.idx, n = wm[i].n, np_ = wm[i].np;
+          s4.add_gemm(g64(fp(i, 5), fp(i, 3), np_, n, n, np_, fp(i, 7)), true, i);  // C8 in t2
+        }
+      GroupStage& lc = stage(GK_LINCOMB);
+      for (const GroupPlan& P : plans) {
+        const int i = P.idx, n = wm[i].n, np_ = wm[i].np;
+        double b[14];
+        for (int j = 0; j <= P.m; ++j) b[j] = pade_b(P.m, j);
+        const double* p2 = fp(i, 3);
+        const double* p4 = fp(i, 4);
+        const double* p6 = fp(i, 5);
+        const double* c8 = fp(i, 7);
+        auto add = [&](double* out, std::initializer_list<std::pair<double, const double*>> terms, double dg) {
+          OpLincomb::Prob L;
+          std::memset(&L, 0, sizeof(L));
+          L.out = out;
+          int k = 0;
+          for (auto& t : terms) {
+            L.c[k] = t.first;
+            L.in[k] = t.second;
+            ++k;
+          }
+          L.nin = k;
+          L.diag = dg;
+          L.n = n;
+          L.ld = np_;
+          lc.add(L, OpLincomb::blocks(L));
+        };
+        if (P.m == 13) {
+          add(fp(i, 6), {{b[13], p6}, {-b[11], p4}, {b[9], p2}}, 0.0);  // Z1 in t1
+          add(fp(i, 7), {{b[12], p6}, {-b[10], p4}, {b[8], p2}}, 0.0);  // Z2 in t2
+        } else if (P.m == 3) {
+          add(fp(i, 6), {{-b[3], p2}}, b[1]);
+          add(fp(i, 2), {{-b[2], p2}}, b[0]);
+        } else if (P.m == 5) {
+          add(fp(i, 6), {{b[5], p4}, {-b[3], p2}}, b[1]);
+          add(fp(i, 2), {{b[4], p4}, {-b[2], p2}}, b[0]);
+        } else if (P.m == 7) {
+          add(fp(i, 6), {{-b[7], p6}, {b[5], p4}, {-b[3], p2}}, b[1]);
+          add(fp(i, 2), {{-b[6], p6}, {b[4], p4}, {-b[2], p2}}, b[0]);
+        } else {  // 9
+          add(fp(i, 6), {{b[9], c8}, {-b[7], p6}, {b[5], p4}, {-b[3], p2}}, b[1]);
+          add(fp(i, 2), {{b[8], c8}, {-b[6], p6}, {b[4], p4}, {-b[2], p2}}, b[0]);
+        }
+      }
+      // m = 13: Wb = C6 Z1 - b7 C6 + b5 C4 - b3 C2 + b1 I (x0); V = C6 Z2 - b6 C6 + b4 C4 - b2 C2 + b0 I
+      GroupStage& s5 = stage(GK_GEMM, GC_SYM64, MODE_REAL);
+      for (const GroupPlan& P : plans)
+        if (P.m == 13) {
+          const int i = P.idx, n = wm[i].n, np_ = wm[i].np;
+          double b[14];
+          for (int j = 0; j <= 13; ++j) b[j] = pade_b(13, j);
+          GemmArgs<double> g = g64(fp(i, 5), fp(i, 6), np_, n, n, np_, fp(i, 8));
+          g.epi.nadd = 3;
+          g.epi.addc[0] = -b[7];
+          g.epi.add_r[0] = fp(i, 5);
+          g.epi.addc[1] = b[5];
+          g.epi.add_r[1] = fp(i, 4);
+          g.epi.addc[2] = -b[3];
+          g.epi.add_r[2] = fp(i, 3);
+          g.epi.diag = b[1];
+          s5.add_gemm(g, true, i);
+          GemmArgs<double> h = g64(fp(i, 5), fp(i, 7), np_, n, n, np_, fp(i, 2));
+          h.epi.nadd = 3;
+          h.epi.addc[0] = -b[6];
+          h.epi.add_r[0] = fp(i, 5);
+          h.epi.addc[1] = b[4];
+          h.epi.add_r[1] = fp(i, 4);
+          h.epi.addc[2] = -b[2];
+          h.epi.add_r[2] = fp(i, 3);
+          h.epi.diag = b[0];
+          s5.add_gemm(h, true, i);
+        }
+      // Uq = G * (Wb or odd polynomial)
+      GroupStage& s6 = stage(GK_GEMM, GC_SYM64, MODE_REAL);
+      for (const GroupPlan& P : plans) {
+        const int i = P.idx, n = wm[i].n, np_ = wm[i].np;
+        s6.add_gemm(g64(fp(i, 0), fp(i, P.m == 13 ? 8 : 6), np_, n, n, np_, fp(i, 1)), true, i);
+      }
+    }
+    // 3. Cholesky solve: S = V V^T + Uq Uq^T = L L^T, Z = [V; Uq] L^{-T}, X = Z Z^T
+    {
+      GroupStage& pk = stage(GK_PACK);
+      for (const GroupPlan& P : plans) {
+        const int i = P.idx;
+        OpPack::Prob p{fp(i, 2), fp(i, 1), wm[i].n, wm[i].np, wm[i].np, fp(i, 3)};
+        pk.add(p, OpPack::blocks(p));
+      }
+      GroupStage& vv = stage(GK_GEMM, GC_SYM64, MODE_REAL);
+      for (const GroupPlan& P : plans) {
+        const int i = P.idx, np_ = wm[i].np, kpad = (int)round_up(wm[i].n, 16);
+        const size_t Q = (size_t)np_ * np_;
+        double* zb = fp(i, 3);
+        vv.add_gemm(g64(zb, zb, np_, np_, np_, kpad, zb + 3 * Q), true, i);
+      }
+      GroupStage& sm = stage(GK_GEMM, GC_SYM64, MODE_REAL);
+      for (const GroupPlan& P : plans) {
+        const int i = P.idx, np_ = wm[i].np, kpad = (int)round_up(wm[i].n, 16);
+        const size_t Q = (size_t)np_ * np_;
+        double* zb = fp(i, 3);
+        GemmArgs<double> g = g64(zb + Q, zb + Q, np_, np_, np_, kpad, zb + 2 * Q);
+        g.epi.nadd = 1;
+        g.epi.addc[0] = 1.0;
+        g.epi.add_r[0] = zb + 3 * Q;
+        sm.add_gemm(g, true, i);
+      }
+      int max_np = 0;
+      for (const GroupPlan& P : plans) max_np = std::max(max_np, wm[P.idx].np);
+      for (int k0 = 0; k0 < max_np; k0 += LU_NB) {
+        GroupStage& ch = stage(GK_CHOL);
+        GroupStage& pn = stage(GK_GEMM, GC_SOLVE, MODE_REAL);
+        GroupStage& up = stage(GK_GEMM, GC_SOLVE, MODE_REAL);
+        for (const GroupPlan& P : plans) {
+          const int i = P.idx, n = wm[i].n, ip = wm[i].np;
+          if (k0 >= ip) continue;
+          const size_t Q = (size_t)ip * ip;
+          const int64_t L = ip;
+          double* zb = fp(i, 3);
+          double* s = zb + 2 * Q;
+          double* dk = dinv(i) + (size_t)(k0 / LU_NB) * LU_NB * LU_NB;
+          const int K0 = k0 / CH_OUTER * CH_OUTER;
+          const int K1 = std::min(ip, K0 + CH_OUTER);
+          const int k1 = k0 + LU_NB;
+          const int rest = ip - k1;
+          OpChol::Prob c{s, L, k0, n, dk, d_scal + i};
+          ch.add(c, 1);
+          double* panel = s + (int64_t)k1 * L + k0;  // L[k1:, k0:k1]
+          if (rest > 0) {
+            GemmArgs<double> g = g64(panel, dk, L, rest, LU_NB, LU_NB, panel);
+            g.ldb = LU_NB;
+            g.epi.alpha = 1.0;
+            pn.add_gemm(g, false, i, true);
+          }
+          {
+            GemmArgs<double> g = g64(zb + k0, dk, L, 2 * ip, LU_NB, LU_NB, zb + k0);
+            g.ldb = LU_NB;
+            pn.add_gemm(g, false, i, true);
+          }
+          if (k1 < K1) {
+            GemmArgs<double> g = g64(panel, panel, L, rest, K1 - k1, LU_NB, s + (int64_t)k1 * L + k1);
+            g.epi.alpha = -1.0;
+            g.epi.beta = 1.0;
+            up.add_gemm(g, false, i);
+            GemmArgs<double> h = g64(zb + k0, panel, L, 2 * ip, K1 - k1, LU_NB, zb + k1);
+            h.epi.alpha = -1.0;
+            h.epi.beta = 1.0;
+            up.add_gemm(h, false, i);
+          } else if (K1 < ip) {
+            const int r2 = ip - K1, kk = K1 - K0;
+            double* p2 = s + (int64_t)K1 * L + K0;  // L[K1:, K0:K1]
+            GemmArgs<double> g = g64(p2, p2, L, r2, r2, kk, s + (int64_t)K1 * L + K1);
+            g.epi.alpha = -1.0;
+            g.epi.beta = 1.0;
+            up.add_gemm(g, true, i);
+            GemmArgs<double> h = g64(zb + K0, p2, L, 2 * ip, r2, kk, zb + K1);
+            h.epi.alpha = -1.0;
+            h.epi.beta = 1.0;
+            up.add_gemm(h, false, i);
+          }
+        }
+      }
+      GroupStage& xs = stage(GK_GEMM, GC_SYM64, MODE_C3M);
+      for (const GroupPlan& P : plans) {
+        const int i = P.idx, n = wm[i].n, np_ = wm[i].np, kpad = (int)round_up(n, 16);
+        const size_t Q = (size_t)np_ * np_;
+        double* zb = fp(i, 3);
+        GemmArgs<double> g = g64(zb, zb, np_, n, n, kpad, zb + 2 * Q);
+        g.ai = zb + Q;
+        g.bi = zb + Q;
+        g.epi.ci = zb + 3 * Q;
+        xs.add_gemm(g, true, i);
+      }
+      GroupStage& so = stage(GK_SOLUTION);
+      for (const GroupPlan& P : plans) {
+        const int i = P.idx, np_ = wm[i].np;
+        const size_t Q = (size_t)np_ * np_;
+        double* zb = fp(i, 3);
+        typename OpSolution<W>::Prob p{zb + 2 * Q, zb + 3 * Q, np_, wp(i, 0, 0), wp(i, 0, 1), np_, wm[i].n,
+                                       wm[i].dsq};
+        so.add(p, OpSolution<W>::blocks(p));
+      }
+    }
+    // 4. squaring rounds: round r squares every walk with s_i > r
+    int smax = 0;
+    for (const GroupPlan& P : plans) smax = std::max(smax, P.s);
+    const bool f32 = std::is_same<W, float>::value;
+    for (int r = 0; r < smax; ++r) {
+      GroupStage& sq = stage(GK_GEMM, GC_SQUARE, MODE_C3M, f32);
+      for (const GroupPlan& P : plans) {
+        if (P.s <= r) continue;
+        const int i = P.idx, n = wm[i].n, np_ = wm[i].np;
+        const int cb = r & 1, nb = cb ^ 1;
+        GemmArgs<W> g;
+        std::memset(&g, 0, sizeof(g));
+        g.ar = wp(i, cb, 0);
+        g.ai = wp(i, cb, 1);
+        g.lda = np_;
+        g.br = wp(i, cb, 0);
+        g.bi = wp(i, cb, 1);
+        g.ldb = np_;
+        g.m = n;
+        g.n = n;
+        g.k = np_;
+        g.kblock = kblock > 0 ? kblock : (1 << 30);
+        g.epi.cr = wp(i, nb, 0);
+        g.epi.ci = wp(i, nb, 1);
+        g.epi.ldc = np_;
+        g.epi.ld_add = np_;
+        g.epi.alpha = 1.0;
+        g.epi.d_in = wm[i].dsq + (size_t)cb * wm[i].n;
+        g.epi.d_out = wm[i].dsq + (size_t)nb * wm[i].n;
+        sq.add_gemm(g, true, i);
+      }
+    }
+    // 5. U_i = e^mu_i X_i and psi_i = U_i psi0_i
+    {
+      GroupStage& out = stage(GK_OUTPUT);
+      GroupStage& ap = stage(GK_APPLY);
+      for (const GroupPlan& P : plans) {
+        const int i = P.idx, b = P.s & 1;
+        const double er = std::exp(P.shifted ? P.mu_re : 0.0);
+        const double th = P.shifted ? P.mu_im : 0.0;
+        const double phr = er * std::cos(th), phi = er * std::sin(th);
+        const double2* d = wm[i].dsq + (size_t)b * wm[i].n;
+        typename OpOutput<W, Tin>::Prob o{wp(i, b, 0), wp(i, b, 1), wm[i].np, wm[i].n, d, phr, phi,
+                                          reinterpret_cast<Tin*>(d_out) + wm[i].in_off};
+        out.add(o, OpOutput<W, Tin>::blocks(o));
+        if (d_psi0) {
+          typename OpApply<W>::Prob a{wp(i, b, 0), wp(i, b, 1), wm[i].np, wm[i].n, d, d_psi0 + voff[i], phr, phi,
+                                      d_psi + voff[i]};
+          ap.add(a, OpApply<W>::blocks(a));
+        }
+      }
+    }
+    (void)G;
+  }
+
+  void enqueue(hipStream_t st) {
+    for (const GroupLaunch& L : prog.launches) group_enqueue_one<Tin, W>(L, prog.d_arena, st);
+  }
+
+  // One eager run with every GEMM launch timed; adds per-class device time.
+  void timed_run(hipStream_t st, double* cls_ms) {
+    size_t ng = 0;
+    for (const GroupLaunch& L : prog.launches) ng += L.kind == GK_GEMM;
+    while (evpool.size() < 2 * ng) {
+      hipEvent_t e;
+      FX_HIP(hipEventCreate(&e));
+      evpool.push_back(e);
+    }
+    size_t q = 0;
+    std::vector<int> cls;
+    for (const GroupLaunch& L : prog.launches) {
+      if (L.kind == GK_GEMM) FX_HIP(hipEventRecord(evpool[2 * q], st));
+      group_enqueue_one<Tin, W>(L, prog.d_arena, st);
+      if (L.kind == GK_GEMM) {
+        FX_HIP(hipEventRecord(evpool[2 * q + 1], st));
+        cls.push_back(L.cls);
+        ++q;
+      }
+    }
+    FX_HIP(hipStreamSynchronize(st));
+    for (size_t k = 0; k < q; ++k) {
+      float ms = 0.f;
+      FX_HIP(hipEventElapsedTime(&ms, evpool[2 * k], evpool[2 * k + 1]));
+      cls_ms[cls[k]] += ms;
+    }
+  }
+
+  void tune(hipStream_t st, int nw) {
+    const auto t0 = std::chrono::steady_clock::now();
+    double best[GC_COUNT];
+    int arg[GC_COUNT];
+    for (int c = 0; c < GC_COUNT; ++c) {
+      best[c] = 1e300;
+      arg[c] = 0;
+    }
+    for (int k = 0; k < GROUP_CANDIDATES; ++k) {
+      const int ck[GC_COUNT] = {k, k, k};
+      prog.materialize(stages, ck, nw);
+      double ms[GC_COUNT] = {0, 0, 0};
+      timed_run(st, ms);
+      for (int c = 0; c < GC_COUNT; ++c)
+        if (ms[c] < best[c]) {
+          best[c] = ms[c];
+          arg[c] = k;
+        }
+    }
+    for (int c = 0; c < GC_COUNT; ++c) {
+      cand[c] = arg[c];
+      class_ms[c] = best[c];
+    }
+    tuned_layout = true;
+    tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+
+  void run(const std::vector<GroupPlan>& plans, const void* d_in, void* d_out, Scalars* d_scal,
+           const double2* d_psi0, double2* d_psi, const std::vector<size_t>& voff, hipStream_t st) override {
+    graph_built = 0;
+    tuned = 0;
+    graph_used = 0;
+    if (plans.empty()) {
+      launches = 0;
+      device_ms = 0.0;
+      return;
+    }
+    // key: everything the program depends on
+    std::vector<double> key;
+    key.reserve(plans.size() * 6 + 1);
+    key.push_back((double)kblock);
+    for (const GroupPlan& P : plans) {
+      key.push_back(P.idx);
+      key.push_back(P.m);
+      key.push_back(P.s);
+      key.push_back(P.shifted);
+      key.push_back(P.mu_re);
+      key.push_back(P.mu_im);
+    }
+    const bool same = key == plan_key && key_in == d_in && key_out == d_out && key_psi0 == (const void*)d_psi0 &&
+                      !stages.empty();
+    if (!same) {
+      drop_graph();
+      build(plans, d_in, d_out, d_scal, d_psi0, d_psi, voff);
+      plan_key = key;
+      key_in = d_in;
+      key_out = d_out;
+      key_psi0 = d_psi0;
+      if (forced >= 0) {
+        for (int c = 0; c < GC_COUNT; ++c) cand[c] = forced;
+      } else if (autotune && !tuned_layout) {
+        tune(st, nwalk);
+        tuned = 1;
+      }
+      prog.materialize(stages, cand, nwalk);
+      walk_flops = prog.walk_flops;
+    }
+    launches = (int)prog.launches.size();
+    FX_HIP(hipEventRecord(ev0, st));
+    if (use_graph) {
+      if (!gexec) {
+        FX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        try {
+          enqueue(st);
+        } catch (...) {
+          hipGraph_t g = nullptr;
+          (void)hipStreamEndCapture(st, &g);
+          if (g) (void)hipGraphDestroy(g);
+          throw;
+        }
+        FX_HIP(hipStreamEndCapture(st, &graph));
+        FX_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+        graph_built = 1;
+        // the capture recorded nothing on the stream: ev0 is re-recorded
+        FX_HIP(hipEventRecord(ev0, st));
+      }
+      FX_HIP(hipGraphLaunch(gexec, st));
+      graph_used = 1;
+    } else {
+      enqueue(st);
+    }
+    FX_HIP(hipEventRecord(ev1, st));
+    FX_HIP(hipEventSynchronize(ev1));
+    float ms = 0.f;
+    FX_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+    device_ms = ms;
+  }
+};
+
+}  // namespace fexpm

@@ -43,13 +43,12 @@ enum AnalyzeFlags { FLAG_NONZERO_RE = 1, FLAG_ASYM = 2, FLAG_NONFINITE = 4 };
 //    later reduction over row blocks is deterministic;
 //  * diagonal entries (re, im) into diag[].
 template <typename Tin>
-__global__ __launch_bounds__(256) void analyze_tiles(const Tin* __restrict__ a, int n, int nt,
-                                                     double* __restrict__ colpart,
-                                                     double2* __restrict__ diag, int* flags) {
+__device__ __forceinline__ void analyze_tile_pair(const Tin* __restrict__ a, int n, int nt,
+                                                  double* __restrict__ colpart, double2* __restrict__ diag,
+                                                  int* flags, int b) {
   __shared__ double2 t0[32][33];
   __shared__ double2 t1[32][33];
   __shared__ int sflag;
-  int b = blockIdx.x;
   int I, J;
   {
     int i = (int)((sqrtf(8.0f * (float)b + 1.0f) - 1.0f) * 0.5f);
@@ -121,6 +120,13 @@ __global__ __launch_bounds__(256) void analyze_tiles(const Tin* __restrict__ a, 
   if (threadIdx.x == 0 && sflag) atomicOr(flags, sflag);
 }
 
+template <typename Tin>
+__global__ __launch_bounds__(256) void analyze_tiles(const Tin* __restrict__ a, int n, int nt,
+                                                     double* __restrict__ colpart,
+                                                     double2* __restrict__ diag, int* flags) {
+  analyze_tile_pair<Tin>(a, n, nt, colpart, diag, flags, (int)blockIdx.x);
+}
+
 // off[j] = sum over row blocks of colpart (fixed order -> deterministic).
 __global__ void reduce_colpart(const double* __restrict__ colpart, int n, int nt, double* __restrict__ off) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -142,9 +148,9 @@ struct Scalars {
 
 // Single workgroup of 1024 threads: trace, mu and both 1-norms.  The trace
 // is summed in a fixed tree order.
-__global__ __launch_bounds__(1024) void finalize_norms(const double* __restrict__ off,
-                                                       const double2* __restrict__ diag, int n,
-                                                       const int* flags, Scalars* out) {
+__device__ __forceinline__ void finalize_norms_block(const double* __restrict__ off,
+                                                     const double2* __restrict__ diag, int n,
+                                                     const int* flags, Scalars* out) {
   __shared__ double sr[1024], si[1024], s0[1024], s1[1024];
   const int t = threadIdx.x;
   double ar = 0.0, ai = 0.0;
@@ -197,6 +203,12 @@ __global__ __launch_bounds__(1024) void finalize_norms(const double* __restrict_
     out->flags = *flags;
     out->info = 0;
   }
+}
+
+__global__ __launch_bounds__(1024) void finalize_norms(const double* __restrict__ off,
+                                                       const double2* __restrict__ diag, int n,
+                                                       const int* flags, Scalars* out) {
+  finalize_norms_block(off, diag, n, flags, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -1215,8 +1227,8 @@ __global__ __launch_bounds__(256) void pack_stacked(const W* __restrict__ vr, co
 // zero above the diagonal).  Rows and columns at or beyond n are padding
 // and are factored as the identity.  A pivot that is not positive sets bit
 // 0 of sc->info and is replaced by 1.
-__global__ __launch_bounds__(256) void chol_diag(double* __restrict__ s, int64_t ld, int k0, int n,
-                                                 double* __restrict__ inv, Scalars* sc) {
+__device__ __forceinline__ void chol_diag_block(double* __restrict__ s, int64_t ld, int k0, int n,
+                                                double* __restrict__ inv, Scalars* sc) {
   __shared__ double a[64][65];   // scaled factor (input of the inverse)
   __shared__ double col[2][64];  // column j of step j (unscaled), double-buffered by parity
   __shared__ double ldg[64], rd[64];
@@ -1304,6 +1316,11 @@ __global__ __launch_bounds__(256) void chol_diag(double* __restrict__ s, int64_t
     if ((i & 3) == q) x[i >> 2] = xi;
     if (q == 0) inv[i * 64 + c] = xi;
   }
+}
+
+__global__ __launch_bounds__(256) void chol_diag(double* __restrict__ s, int64_t ld, int k0, int n,
+                                                 double* __restrict__ inv, Scalars* sc) {
+  chol_diag_block(s, ld, k0, n, inv, sc);
 }
 
 

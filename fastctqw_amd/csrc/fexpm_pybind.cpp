@@ -347,6 +347,29 @@ class ExpMatBatch {
     return stats_dict(s);
   }
   double last_ms() const { return fexpm_batch_last_ms(b_); }
+  py::dict report() {
+    fexpm_batch_report r;
+    check(fexpm_batch_get_report(b_, &r));
+    py::dict d;
+    d["grouped"] = r.grouped;
+    d["grouped_walks"] = r.grouped_walks;
+    d["fallback_walks"] = r.fallback_walks;
+    d["launches"] = r.launches;
+    d["graph"] = r.graph;
+    d["graph_built"] = r.graph_built;
+    d["tuned"] = r.tuned;
+    py::list cand, cms;
+    for (int c = 0; c < 3; ++c) {
+      cand.append(r.candidate[c]);
+      cms.append(r.class_ms[c]);
+    }
+    d["candidate"] = cand;
+    d["class_ms"] = cms;
+    d["tune_ms"] = r.tune_ms;
+    d["device_ms"] = r.device_ms;
+    d["plan_ms"] = r.plan_ms;
+    return d;
+  }
 
  private:
   fexpm_batch* b_ = nullptr;
@@ -391,6 +414,10 @@ PYBIND11_MODULE(_fastexpm_core, m) {
   m.attr("OPT_PANEL_COOP") = (int)FEXPM_OPT_PANEL_COOP;
   m.attr("OPT_SPD_SOLVE") = (int)FEXPM_OPT_SPD_SOLVE;
   m.attr("OPT_MAX_SQUARINGS") = (int)FEXPM_OPT_MAX_SQUARINGS;
+  m.attr("OPT_GROUP_CANDIDATE") = (int)FEXPM_OPT_GROUP_CANDIDATE;
+  m.attr("OPT_BATCH_GROUPED") = (int)FEXPM_OPT_BATCH_GROUPED;
+  m.attr("OPT_AUTOTUNE") = (int)FEXPM_OPT_AUTOTUNE;
+  m.attr("OPT_GRAPH") = (int)FEXPM_OPT_GRAPH;
   m.attr("PREC_NATIVE") = (int)FEXPM_PREC_NATIVE;
   m.attr("PREC_FP16X2") = (int)FEXPM_PREC_FP16X2;
   m.attr("PREC_BF16X3") = (int)FEXPM_PREC_BF16X3;
@@ -401,6 +428,7 @@ PYBIND11_MODULE(_fastexpm_core, m) {
       .def("set_option", &ExpMatBatch::set_option, py::arg("key"), py::arg("value"))
       .def("download", &ExpMatBatch::download, py::arg("i"))
       .def("stats", &ExpMatBatch::stats, py::arg("i"))
-      .def("last_ms", &ExpMatBatch::last_ms);
+      .def("last_ms", &ExpMatBatch::last_ms)
+      .def("report", &ExpMatBatch::report);
   m.def("gpu_memory_gb", &fexpm_gpu_memory_gb);
 }

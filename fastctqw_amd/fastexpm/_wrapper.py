@@ -162,3 +162,15 @@ class ExpMatBatch:
 
     def last_ms(self) -> float:
         return self._backend.last_ms()
+
+    def set_grouped(self, on: bool = True, autotune: bool = True, graph: bool = True) -> None:
+        """Grouped launches over all walks (default) or concurrent lanes; the
+        autotuner and hipGraph replay of grouped runs."""
+        self._backend.set_option(_fastexpm_core.OPT_BATCH_GROUPED, int(bool(on)))
+        self._backend.set_option(_fastexpm_core.OPT_AUTOTUNE, int(bool(autotune)))
+        self._backend.set_option(_fastexpm_core.OPT_GRAPH, int(bool(graph)))
+
+    def report(self) -> dict:
+        """How the last run executed: grouped or lanes, walks per path, launch
+        count, graph replay, autotuner choices and device time."""
+        return self._backend.report()

@@ -66,8 +66,14 @@ enum fexpm_option {
                                     0 = Pade products in FP32 */
   FEXPM_OPT_PANEL_COOP = 6,      /* LU panels: 1 = one cooperative launch per panel (default),
                                     0 = one launch per column (process-wide setting) */
-  FEXPM_OPT_SPD_SOLVE = 7        /* structured A = iG: 1 = real Cholesky solve of the Pade system
+  FEXPM_OPT_SPD_SOLVE = 7,       /* structured A = iG: 1 = real Cholesky solve of the Pade system
                                     (default), 0 = pivoted complex LU as for general A */
+  FEXPM_OPT_BATCH_GROUPED = 8,   /* batches: 1 = grouped launches over all walks (default),
+                                    0 = concurrent lanes */
+  FEXPM_OPT_AUTOTUNE = 9,        /* grouped batches: 1 = time the tile-shape candidates on the first
+                                    run of a new layout and keep the fastest (default), 0 = default tiles */
+  FEXPM_OPT_GRAPH = 10           /* grouped batches: 1 = capture the program in a hipGraph and replay it
+                                    while the plans stay the same (default), 0 = eager launches */
 };
 
 /* Per-run report (filled by fexpm_get_stats after a run). */
@@ -171,8 +177,35 @@ typedef struct fexpm_batch fexpm_batch;
 int fexpm_batch_create(int dtype, int64_t nmax, int lanes, fexpm_batch** out);
 int fexpm_batch_destroy(fexpm_batch* b);
 const char* fexpm_batch_last_error(const fexpm_batch* b);
-/* FEXPM_OPT_PRECISION, FEXPM_OPT_KBLOCK or FEXPM_OPT_SPD_SOLVE for every lane. */
+/* FEXPM_OPT_PRECISION, FEXPM_OPT_KBLOCK, FEXPM_OPT_SPD_SOLVE (every lane) and
+ * FEXPM_OPT_BATCH_GROUPED, FEXPM_OPT_AUTOTUNE, FEXPM_OPT_GRAPH (grouped runs). */
 int fexpm_batch_set_option(fexpm_batch* b, int key, int64_t value);
+/* Grouped batches (complex dtypes, option FEXPM_OPT_BATCH_GROUPED): every stage
+ * of the structured pipeline is one launch over all walks that need it (the
+ * analysis, each Pade product, each 64-wide Cholesky panel, each squaring
+ * round r over the walks with s_i > r); the stages after the analysis are
+ * captured once in a hipGraph and replayed while the walks' plans (m_i, s_i,
+ * trace shift) stay the same; tile shapes come from an autotuner that times
+ * each candidate on the batch itself.  Walks that are not of the CTQW form
+ * (A = iG, G real symmetric) or have non-finite entries run on a lane
+ * pipeline instead.  Split precision and the LU solve use lanes. */
+typedef struct fexpm_batch_report {
+  int32_t grouped;         /* 1: the last run used grouped launches */
+  int32_t grouped_walks;   /* walks run by the grouped program */
+  int32_t fallback_walks;  /* walks run on a lane pipeline */
+  int32_t launches;        /* kernel launches of the grouped program (after the analysis) */
+  int32_t graph;           /* 1: the program ran as a hipGraph */
+  int32_t graph_built;     /* 1: this run captured a new graph */
+  int32_t tuned;           /* 1: this run ran the autotuner */
+  int32_t candidate[3];    /* autotuner choice per class (squarings, symmetric FP64 products, solve
+                              updates): tile configuration * 3 + dispatch */
+  double class_ms[3];      /* per class: GEMM device time of the chosen candidate in the tuning runs */
+  double tune_ms;          /* host time of the tuning (0 if it did not run) */
+  double device_ms;        /* device time of the grouped program in the last run */
+  double plan_ms;          /* host time from the start of the run to the plans (analysis + read-back) */
+} fexpm_batch_report;
+int fexpm_batch_get_report(const fexpm_batch* b, fexpm_batch_report* out);
+
 /* `count` row-major matrices of orders n[i] (engine dtype), packed back to
  * back in host memory, copied to the device. */
 int fexpm_batch_upload(fexpm_batch* b, int64_t count, const int64_t* n, const void* a);

@@ -24,6 +24,11 @@ extern "C" {
 
 enum { FEXPM_OPT_MAX_SQUARINGS = 100 };
 
+/* Batch option: run grouped programs with autotuner candidate `value` for
+ * every class of GEMM launches (tile configuration * 3 + dispatch, 0..5) and
+ * no tuning; -1 restores the autotuner.  Lets tests check every candidate. */
+enum { FEXPM_OPT_GROUP_CANDIDATE = 101 };
+
 /* Internal planes readable by fexpm_debug_plane (working precision, n x n). */
 enum fexpm_plane {
   FEXPM_PLANE_A = 0,   /* (A - mu I) / 2^s; structured: its imaginary part G */

@@ -384,6 +384,106 @@ def test_batch_uneven_walks(fx):
         b.upload([np.zeros((301, 301), np.complex64)])
 
 
+def _eig_ref(a):
+    G = np.ascontiguousarray(np.asarray(a).imag, dtype=np.float64)
+    lam, V = np.linalg.eigh(G)
+    return (V * np.exp(1j * lam)[None, :]) @ V.T
+
+
+@pytest.mark.parametrize("dt", [np.complex64, np.complex128])
+def test_batch_grouped(fx, dt):
+    """Grouped launches over a batch of uneven walks: every Pade degree
+    (t from 1e-4 to 20), orders 1 .. 300 around the 64-wide padding, a
+    general complex matrix and a non-finite one (both leave the grouped
+    program for a lane).  Each U against the eigen oracle of the same
+    rounded input (general: the Pade oracle), psi against U psi0, the
+    grouped results against the lane results, bitwise replay of the
+    captured graph, re-planning after new times, eager launches equal to the
+    graph, and default tiles without the autotuner."""
+    from fastctqw_amd.graphs import c4_walks
+
+    rng = np.random.default_rng(77)
+    walks = c4_walks(count=8, seed=4200, nmin=16, nmax=300)
+    mats, kinds = [], []
+    for w, t in zip(walks, [1e-4, 0.01, 0.2, 1.0, 3.0, 7.5, 12.0, 20.0]):
+        mats.append((-1j * t * w["H"]).astype(dt))
+        kinds.append("ctqw")
+    for n, t in [(1, 2.0), (63, 4.5), (64, 4.5), (65, 4.5), (129, 0.7)]:
+        L = laplacian(erdos_renyi(n, 0.4, seed=900 + n, weighted=True))
+        mats.append((-1j * t * L).astype(dt))
+        kinds.append("ctqw")
+    g = (rng.standard_normal((40, 40)) + 1j * rng.standard_normal((40, 40))) * 0.3
+    mats.append(g.astype(dt))
+    kinds.append("general")
+    bad = (-1j * laplacian(erdos_renyi(20, 0.5, seed=5))).astype(dt)
+    bad[3, 4] = np.nan
+    mats.append(bad)
+    kinds.append("nonfinite")
+    ns = [m.shape[0] for m in mats]
+    _say(f"grouped batch {np.dtype(dt).name}: n = {ns}")
+    psi0 = [np.exp(1j * np.arange(n) * 0.37) / np.sqrt(n) for n in ns]
+    b = fx.ExpMatBatch(nmax=300, dtype=dt, lanes=2)
+    us, psis = b.run(mats, psi0)
+    rep = b.report()
+    print(f"  report {rep}", flush=True)
+    assert rep["grouped"] == 1 and rep["tuned"] == 1 and rep["graph"] == 1 and rep["graph_built"] == 1
+    assert rep["grouped_walks"] == len(mats) - 2 and rep["fallback_walks"] == 2
+    tol_fn = (lambda s: 1e-11) if dt == np.complex128 else (lambda s: _tol(np.complex64, s))
+    for i, (a, u, psi, kind) in enumerate(zip(mats, us, psis, kinds)):
+        st = b.stats(i)
+        if kind == "nonfinite":
+            assert st["nonfinite"] == 1 and np.isnan(u).all()
+            continue
+        ref_st = pade_expm(a.astype(np.complex128), stages=True)
+        assert st["m"] == ref_st["m"] and st["s"] == ref_st["s"], (i, st, ref_st["m"], ref_st["s"])
+        ref = ref_st["E"] if kind == "general" else _eig_ref(a)
+        err = rel_fro(ref, u)
+        assert err <= tol_fn(ref_st["s"]), (i, kind, ns[i], err)
+        assert st["structured"] == (1 if kind == "ctqw" else 0)
+        psi_ref = u.astype(np.complex128) @ psi0[i]
+        assert np.linalg.norm(psi - psi_ref) <= 1e-5 * max(1.0, np.linalg.norm(psi_ref)), i
+    # replay: same plans -> the captured graph again, bitwise the same U
+    us2, psis2 = b.run(mats, psi0)
+    rep2 = b.report()
+    assert rep2["graph"] == 1 and rep2["graph_built"] == 0 and rep2["tuned"] == 0, rep2
+    for i in range(len(mats)):
+        if kinds[i] != "nonfinite":
+            assert np.array_equal(us[i], us2[i]), i
+    # lanes on the same inputs agree within the precision's tolerance
+    b.set_grouped(False)
+    us_l, psis_l = b.run(mats, psi0)
+    assert b.report()["grouped"] == 0
+    for i in range(len(mats)):
+        if kinds[i] == "nonfinite":
+            continue
+        s_i = b.stats(i)["s"]
+        assert rel_fro(us_l[i], us[i]) <= 2 * tol_fn(s_i), i
+    # new times (same orders): new plans, a new graph, the layout keeps its tuning
+    b.set_grouped(True)
+    mats3 = [m * (0.5 if k == "ctqw" else 1.0) for m, k in zip(mats, kinds)]
+    us3, _ = b.run(mats3, psi0)
+    rep3 = b.report()
+    assert rep3["graph_built"] == 1 and rep3["tuned"] == 0, rep3
+    for i, (a, u, kind) in enumerate(zip(mats3, us3, kinds)):
+        if kind != "ctqw":
+            continue
+        assert rel_fro(_eig_ref(a), u) <= tol_fn(b.stats(i)["s"]), i
+    # eager launches of the same program give the graph's results bitwise
+    b.set_grouped(True, autotune=True, graph=False)
+    us4, _ = b.run(mats3, psi0)
+    assert b.report()["graph"] == 0
+    for i in range(len(mats)):
+        if kinds[i] == "ctqw":
+            assert np.array_equal(us3[i], us4[i]), i
+    # a fresh batch without the autotuner: default tiles, same accuracy
+    b2 = fx.ExpMatBatch(nmax=300, dtype=dt, lanes=1)
+    b2.set_grouped(True, autotune=False, graph=True)
+    us5 = b2.run(mats[:8])
+    assert b2.report()["tuned"] == 0 and b2.report()["candidate"] == [0, 0, 0]
+    for i in range(8):
+        assert rel_fro(_eig_ref(mats[i]), us5[i]) <= tol_fn(b2.stats(i)["s"]), i
+
+
 def test_ctqw_drop_in(fx):
     """README usage (README.md:61-72) with device='cuda'."""
     _say("Ctqw drop-in")
