@@ -511,17 +511,19 @@ def run_c4(args):
             variants["lanes"] = {"walks_per_s": count / (float(np.median(lm)) / 1000.0),
                                  "ms_per_step": float(np.median(lm)), "lanes": args.lanes}
             b.set_grouped(True)
-        b.set_precision("fp16x2")
-        b.run_resident(uni)
-        vm = []
-        for _ in range(args.steps):
-            psi_h = b.run_resident(uni)
-            vm.append(b.last_ms())
-        variants["fp16x2"] = {"walks_per_s": count / (float(np.median(vm)) / 1000.0),
-                              "ms_per_step": float(np.median(vm))}
-        if ref is not None:
-            variants["fp16x2"]["worst_infidelity_uniform"] = float(
-                max(_infidelity(ref[f"uni{i}"], psi_h[i]) for i in range(count)))
+        for prec in ("fp16x2", "bf16x3"):
+            # split-precision squaring rounds (grouped unless --c4-lanes); the first run tunes
+            b.set_precision(prec)
+            b.run_resident(uni)
+            vm = []
+            for _ in range(args.steps):
+                psi_h = b.run_resident(uni)
+                vm.append(b.last_ms())
+            variants[prec] = {"walks_per_s": count / (float(np.median(vm)) / 1000.0),
+                              "ms_per_step": float(np.median(vm)), "report": b.report()}
+            if ref is not None:
+                variants[prec]["worst_infidelity_uniform"] = float(
+                    max(_infidelity(ref[f"uni{i}"], psi_h[i]) for i in range(count)))
         b.set_precision("native")
         b1 = ExpMatBatch(nmax=max(ns), dtype=np.complex64, lanes=1)
         b1.set_grouped(False)

@@ -1780,6 +1780,7 @@ int fexpm_batch_set_option(fexpm_batch* b, int key, int64_t value) {
       }
       for (auto& p : b->pipes) p->precision = (int)value;
       b->precision = (int)value;
+      if (b->grp) b->grp->precision = (int)value;
       return FEXPM_OK;
     case FEXPM_OPT_KBLOCK:
       if (value < 0 || value > 1024) {
@@ -1938,10 +1939,12 @@ static void batch_run_grouped(fexpm_batch* b, const void* psi0, void* psi) {
   FX_HIP(hipStreamSynchronize(st));
   // exception bits: Cholesky pivots of grouped walks, lane bits of the rest
   FX_HIP(hipMemcpy(hs.data(), b->d_scal, (size_t)count * sizeof(Scalars), hipMemcpyDeviceToHost));
-  int64_t failed = -1;
+  int64_t failed = -1, out_of_range = -1;
   for (const GroupPlan& P : plans) {
     const int64_t i = P.idx;
-    if (hs[(size_t)i].info != 0 && failed < 0) failed = i;
+    const int info = hs[(size_t)i].info;
+    if ((info & INFO_SPLIT_RANGE) && out_of_range < 0) out_of_range = i;
+    if ((info & ~INFO_SPLIT_RANGE) && failed < 0) failed = i;
     fexpm_stats& S = b->stats[i];
     std::memset(&S, 0, sizeof(S));
     S.n = b->n[i];
@@ -1958,8 +1961,11 @@ static void batch_run_grouped(fexpm_batch* b, const void* psi0, void* psi) {
     S.flops_issued = (size_t)i < G.walk_flops.size() ? G.walk_flops[(size_t)i] : 0.0;
     S.kblock = G.kblock;
     S.pade_fp64 = b->dtype == FEXPM_COMPLEX64 ? 1 : 0;
-    S.precision = FEXPM_PREC_NATIVE;
+    S.precision = b->dtype == FEXPM_COMPLEX64 ? G.precision : FEXPM_PREC_NATIVE;
   }
+  if (out_of_range >= 0)
+    throw Error(FEXPM_EINVAL, "walk " + std::to_string(out_of_range) +
+                                  ": split-precision squaring: a scaled operand value left the 16-bit range");
   if (failed >= 0)
     throw Error(FEXPM_ESINGULAR, "walk " + std::to_string(failed) + ": Pade denominator not positive definite");
   for (int64_t i : rest) {
@@ -2000,7 +2006,7 @@ int fexpm_batch_run_resident(fexpm_batch* b, const void* psi0, void* psi) {
   const int64_t count = (int64_t)b->n.size();
   if (count == 0) return FEXPM_OK;
   return guarded_b(b, [&] {
-    if (b->grp && b->grouped && b->precision == FEXPM_PREC_NATIVE && b->spd_solve) {
+    if (b->grp && b->grouped && b->spd_solve) {
       const auto t0 = std::chrono::steady_clock::now();
       batch_run_grouped(b, psi0, psi);
       b->last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

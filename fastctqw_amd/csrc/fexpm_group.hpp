@@ -278,6 +278,25 @@ struct OpChol {
   static __device__ __forceinline__ void run(const Prob& p, int) { chol_diag_block(p.s, p.ld, p.k0, p.n, p.inv, p.sc); }
 };
 
+// Split of one walk's squaring operand into k-blocked 16-bit parts.
+template <class P> struct OpSplit {
+  static constexpr int NT = 256;
+  struct Prob {
+    const float* yr;
+    const float* yi;
+    int64_t ldy;
+    int n, rows, kpad;
+    typename P::H* out;
+    int* info;
+  };
+  static int gx(const Prob& p) { return (p.kpad / 16 + 15) / 16; }
+  static int blocks(const Prob& p) { return gx(p) * ((p.rows + 15) / 16); }
+  static __device__ __forceinline__ void run(const Prob& p, int b) {
+    const int g = (p.kpad / 16 + 15) / 16;
+    split_blocked_block<P>(p.yr, p.yi, p.ldy, p.n, p.n, p.rows, p.kpad, p.out, p.info, b % g, b / g);
+  }
+};
+
 // ---------------------------------------------------------------------------
 // Host side
 
@@ -292,8 +311,14 @@ enum GroupKind {
   GK_OUTPUT,
   GK_APPLY,
   GK_CHOL,
-  GK_GEMM
+  GK_GEMM,
+  GK_SPLIT_F16,   // split of the squaring operand (fp16x2 parts)
+  GK_SPLIT_BF16,  // (bf16x3 parts)
+  GK_SQ_F16,      // split squaring on 16-bit MFMA, tiles of SQ_BM (GEMM-like stage)
+  GK_SQ_BF16
 };
+
+static inline bool gk_tiled(int kind) { return kind == GK_GEMM || kind == GK_SQ_F16 || kind == GK_SQ_BF16; }
 
 // Tuning classes of GEMM launches.
 enum GroupClass { GC_SQUARE = 0, GC_SYM64 = 1, GC_SOLVE = 2, GC_COUNT = 3 };
@@ -378,6 +403,47 @@ struct GroupProgram {
     if (d_arena) (void)hipFree(d_arena);
   }
 
+  // Split squaring stage: problems are SqArgs<P>, lower tiles of SQ_BM.
+  template <class P>
+  void materialize_sq(const GroupStage& S, GroupLaunch& L, std::vector<unsigned char>& host, const int* cand) {
+    const int np_ = S.count();
+    const int c = cand[S.cls];
+    L.cfg = 0;
+    L.dispatch = cand_dispatch(c);
+    std::vector<int> order((size_t)np_);
+    for (int p = 0; p < np_; ++p) order[p] = p;
+    auto arg = [&](int p) -> const SqArgs<P>& {
+      return *reinterpret_cast<const SqArgs<P>*>(S.probs.data() + (size_t)p * S.psize);
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return arg(x).k > arg(y).k; });
+    std::vector<GroupSq<P>> buf((size_t)np_);
+    int tot = 0;
+    for (int q = 0; q < np_; ++q) {
+      const SqArgs<P>& g = arg(order[q]);
+      if (g.m <= 0 || g.k <= 0 || g.k % SQ_BK != 0 || g.rows % SQ_BM != 0 || g.rows < g.m)
+        throw Error(FEXPM_EINVAL, "grouped split squaring: bad problem shape");
+      const int T = (g.m + SQ_BM - 1) / SQ_BM;
+      const int tiles = T * (T + 1) / 2;
+      walk_flops[(size_t)S.walk[order[q]]] += 4.0 * P::NPAIR * 2.0 * tiles * (double)SQ_BM * SQ_BM * (double)g.k;
+      std::memset(&buf[q], 0, sizeof(GroupSq<P>));
+      buf[q].g = g;
+      buf[q].tile0 = tot;
+      buf[q].tiles = tiles;
+      tot += tiles;
+    }
+    L.total = tot;
+    L.chunk = L.dispatch == 0 ? 1 : 4;
+    int capw = tot;
+    if (L.dispatch == 2)
+      capw = std::max(8, group_capacity(sq_split_group_kernel<P>, sq_threads<P>()) / 8 * 8);
+    L.grid = std::min(tot, capw);
+    host.resize((host.size() + 255) / 256 * 256);
+    L.off_probs = host.size();
+    const size_t bytes = buf.size() * sizeof(GroupSq<P>);
+    host.resize(host.size() + bytes);
+    std::memcpy(host.data() + L.off_probs, buf.data(), bytes);
+  }
+
   void materialize(const std::deque<GroupStage>& stages, const int* cand, int nwalks) {
     std::vector<unsigned char> host;
     auto align = [&]() { host.resize((host.size() + 255) / 256 * 256); };
@@ -394,7 +460,11 @@ struct GroupProgram {
       L.f32 = S.f32;
       L.nprob = np_;
       L.chunk = 1;
-      if (S.kind != GK_GEMM) {
+      if (S.kind == GK_SQ_F16) {
+        materialize_sq<Fp16x2>(S, L, host, cand);
+      } else if (S.kind == GK_SQ_BF16) {
+        materialize_sq<Bf16x3>(S, L, host, cand);
+      } else if (S.kind != GK_GEMM) {
         std::vector<int> start((size_t)np_);
         int tot = 0;
         for (int p = 0; p < np_; ++p) {
@@ -519,6 +589,16 @@ static void group_enqueue_one(const GroupLaunch& L, unsigned char* base, hipStre
     case GK_OUTPUT: FX_OP(OutOp)
     case GK_APPLY: FX_OP(OpApply<W>)
     case GK_CHOL: FX_OP(OpChol)
+    case GK_SPLIT_F16: FX_OP(OpSplit<Fp16x2>)
+    case GK_SPLIT_BF16: FX_OP(OpSplit<Bf16x3>)
+    case GK_SQ_F16:
+      sq_split_group_kernel<Fp16x2><<<grid, sq_threads<Fp16x2>(), 0, st>>>(
+          reinterpret_cast<const GroupSq<Fp16x2>*>(pp), L.nprob, L.total, L.chunk);
+      break;
+    case GK_SQ_BF16:
+      sq_split_group_kernel<Bf16x3><<<grid, sq_threads<Bf16x3>(), 0, st>>>(
+          reinterpret_cast<const GroupSq<Bf16x3>*>(pp), L.nprob, L.total, L.chunk);
+      break;
     case GK_GEMM: {
 #define FX_GG(T, MODE, KB, CF)                                                                             \
   gemm_group_kernel<T, MODE, KB, CF><<<grid, CF::NT, 0, st>>>(reinterpret_cast<const GroupGemm<T>*>(pp), \
@@ -566,6 +646,7 @@ struct GroupRunnerBase {
   int autotune = 1;
   int use_graph = 1;
   int forced = -1;
+  int precision = FEXPM_PREC_NATIVE;  // squarings of complex64 batches: split 16-bit forms
   // report of the last run
   int launches = 0, graph_used = 0, graph_built = 0, tuned = 0;
   int cand[GC_COUNT] = {0, 0, 0};
@@ -593,6 +674,12 @@ template <typename Tin, typename W> struct GroupRunner : GroupRunnerBase {
   size_t f64_cap = 0, w_cap = 0, small_cap = 0;
   int* d_flags = nullptr;  // all walks' flags, contiguous (one clear launch)
   int nwalk = 0;
+  // k-blocked 16-bit split parts of the squaring operand (split precision),
+  // rows padded to SQ_BM and k to 16, room for three parts per plane
+  void* split_arena = nullptr;
+  size_t split_cap = 0;
+  std::vector<size_t> split_off;  // bytes, per walk
+  bool split_ready = false;
 
   GroupProgram analysis;   // analysis program (per layout)
   const void* analysis_in = nullptr;
@@ -603,7 +690,8 @@ template <typename Tin, typename W> struct GroupRunner : GroupRunnerBase {
   const void* key_in = nullptr;
   void* key_out = nullptr;
   const void* key_psi0 = nullptr;
-  bool tuned_layout = false;
+  bool tuned_prec[3] = {false, false, false};  // the autotuner ran for this layout, per precision
+  int cand_prec[3][GC_COUNT] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   std::vector<hipEvent_t> evpool;
@@ -618,6 +706,7 @@ template <typename Tin, typename W> struct GroupRunner : GroupRunnerBase {
     if (f64_arena) (void)hipFree(f64_arena);
     if (w_arena) (void)hipFree(w_arena);
     if (small_arena) (void)hipFree(small_arena);
+    if (split_arena) (void)hipFree(split_arena);
     for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -628,7 +717,7 @@ template <typename Tin, typename W> struct GroupRunner : GroupRunnerBase {
     drop_graph();
     stages.clear();
     plan_key.clear();
-    tuned_layout = false;
+    for (bool& t : tuned_prec) t = false;
     for (int k = 0; k < GC_COUNT; ++k) cand[k] = c >= 0 ? c : 0;
   }
 
@@ -649,8 +738,9 @@ template <typename Tin, typename W> struct GroupRunner : GroupRunnerBase {
     drop_graph();
     stages.clear();
     plan_key.clear();
-    tuned_layout = false;
     analysis_in = nullptr;
+    split_ready = false;
+    for (bool& t : tuned_prec) t = false;
     nwalk = cnt;
     wm.assign(cnt, WalkMem());
     size_t f64_tot = 0, w_tot = 0, small_tot = 0;
@@ -712,6 +802,58 @@ template <typename Tin, typename W> struct GroupRunner : GroupRunnerBase {
   double* dinv(int i) const { return wm[i].f64 + (size_t)9 * wm[i].np * wm[i].np; }
   // squaring plane of walk i: buffer b in {0, 1}, part 0 re / 1 im
   W* wp(int i, int b, int part) const { return wm[i].w + (size_t)(2 * b + part) * wm[i].np * wm[i].np; }
+
+  static int split_rows(int n) { return (int)round_up(std::max(n, 1), SQ_BM); }
+  static int split_kpad(int n) { return (int)round_up(std::max(n, 1), 16); }
+
+  // Split-part buffers of the current layout (allocated on first use).
+  void ensure_split() {
+    if (split_ready) return;
+    split_off.assign((size_t)nwalk, 0);
+    size_t tot = 0;
+    for (int i = 0; i < nwalk; ++i) {
+      split_off[i] = tot;
+      tot += ((size_t)split_rows(wm[i].n) * split_kpad(wm[i].n) * 2 * 3 * sizeof(uint16_t) + 255) / 256 * 256;
+    }
+    if (tot > split_cap) {
+      if (split_arena) (void)hipFree(split_arena);
+      split_arena = nullptr;
+      split_cap = 0;
+      FX_HIP(hipMalloc(&split_arena, tot));
+      split_cap = tot;
+    }
+    split_ready = true;
+  }
+  void* split_buf(int i) const { return static_cast<unsigned char*>(split_arena) + split_off[i]; }
+
+  template <class P>
+  void split_round(GroupStage& sp, GroupStage& sq, const GroupPlan& P_, int r, Scalars* d_scal) {
+    const int i = P_.idx, n = wm[i].n, np_ = wm[i].np;
+    const int cb = r & 1, nb = cb ^ 1;
+    typename OpSplit<P>::Prob p{reinterpret_cast<const float*>(wp(i, cb, 0)),
+                                reinterpret_cast<const float*>(wp(i, cb, 1)),
+                                np_, n, split_rows(n), split_kpad(n),
+                                static_cast<typename P::H*>(split_buf(i)), &d_scal[i].info};
+    sp.add(p, OpSplit<P>::blocks(p));
+    SqArgs<P> a;
+    std::memset(&a, 0, sizeof(a));
+    a.a = static_cast<const typename P::H*>(split_buf(i));
+    a.rows = split_rows(n);
+    a.yr = reinterpret_cast<const float*>(wp(i, cb, 0));
+    a.yi = reinterpret_cast<const float*>(wp(i, cb, 1));
+    a.ldy = np_;
+    a.cr = reinterpret_cast<float*>(wp(i, nb, 0));
+    a.ci = reinterpret_cast<float*>(wp(i, nb, 1));
+    a.ldc = np_;
+    a.d_in = wm[i].dsq + (size_t)cb * n;
+    a.d_out = wm[i].dsq + (size_t)nb * n;
+    a.m = n;
+    a.k = split_kpad(n);
+    a.kblock = kblock > 0 ? kblock : (1 << 30);
+    a.order = nullptr;
+    sq.add(a, 0);
+    sq.walk.push_back(i);
+  }
 
   void analyze(const void* d_in, Scalars* d_scal, hipStream_t st) override {
     if (nwalk == 0) return;
@@ -992,7 +1134,19 @@ template <typename Tin, typename W> struct GroupRunner : GroupRunnerBase {
     int smax = 0;
     for (const GroupPlan& P : plans) smax = std::max(smax, P.s);
     const bool f32 = std::is_same<W, float>::value;
-    for (int r = 0; r < smax; ++r) {
+    const int split = f32 ? precision : FEXPM_PREC_NATIVE;
+    if (split != FEXPM_PREC_NATIVE) ensure_split();
+    for (int r = 0; r < smax && split != FEXPM_PREC_NATIVE; ++r) {
+      const bool h = split == FEXPM_PREC_FP16X2;
+      GroupStage& sp = stage(h ? GK_SPLIT_F16 : GK_SPLIT_BF16);
+      GroupStage& sq = stage(h ? GK_SQ_F16 : GK_SQ_BF16, GC_SQUARE);
+      for (const GroupPlan& P : plans)
+        if (P.s > r) {
+          if (h) split_round<Fp16x2>(sp, sq, P, r, d_scal);
+          else split_round<Bf16x3>(sp, sq, P, r, d_scal);
+        }
+    }
+    for (int r = 0; r < smax && split == FEXPM_PREC_NATIVE; ++r) {
       GroupStage& sq = stage(GK_GEMM, GC_SQUARE, MODE_C3M, f32);
       for (const GroupPlan& P : plans) {
         if (P.s <= r) continue;
@@ -1050,7 +1204,7 @@ template <typename Tin, typename W> struct GroupRunner : GroupRunnerBase {
   // One eager run with every GEMM launch timed; adds per-class device time.
   void timed_run(hipStream_t st, double* cls_ms) {
     size_t ng = 0;
-    for (const GroupLaunch& L : prog.launches) ng += L.kind == GK_GEMM;
+    for (const GroupLaunch& L : prog.launches) ng += gk_tiled(L.kind);
     while (evpool.size() < 2 * ng) {
       hipEvent_t e;
       FX_HIP(hipEventCreate(&e));
@@ -1059,9 +1213,9 @@ template <typename Tin, typename W> struct GroupRunner : GroupRunnerBase {
     size_t q = 0;
     std::vector<int> cls;
     for (const GroupLaunch& L : prog.launches) {
-      if (L.kind == GK_GEMM) FX_HIP(hipEventRecord(evpool[2 * q], st));
+      if (gk_tiled(L.kind)) FX_HIP(hipEventRecord(evpool[2 * q], st));
       group_enqueue_one<Tin, W>(L, prog.d_arena, st);
-      if (L.kind == GK_GEMM) {
+      if (gk_tiled(L.kind)) {
         FX_HIP(hipEventRecord(evpool[2 * q + 1], st));
         cls.push_back(L.cls);
         ++q;
@@ -1098,7 +1252,6 @@ template <typename Tin, typename W> struct GroupRunner : GroupRunnerBase {
       cand[c] = arg[c];
       class_ms[c] = best[c];
     }
-    tuned_layout = true;
     tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
 
@@ -1116,6 +1269,7 @@ template <typename Tin, typename W> struct GroupRunner : GroupRunnerBase {
     std::vector<double> key;
     key.reserve(plans.size() * 6 + 1);
     key.push_back((double)kblock);
+    key.push_back((double)precision);
     for (const GroupPlan& P : plans) {
       key.push_back(P.idx);
       key.push_back(P.m);
@@ -1133,11 +1287,16 @@ template <typename Tin, typename W> struct GroupRunner : GroupRunnerBase {
       key_in = d_in;
       key_out = d_out;
       key_psi0 = d_psi0;
+      const int pi = std::min(std::max(precision, 0), 2);
       if (forced >= 0) {
         for (int c = 0; c < GC_COUNT; ++c) cand[c] = forced;
-      } else if (autotune && !tuned_layout) {
+      } else if (autotune && !tuned_prec[pi]) {
         tune(st, nwalk);
         tuned = 1;
+        tuned_prec[pi] = true;
+        for (int c = 0; c < GC_COUNT; ++c) cand_prec[pi][c] = cand[c];
+      } else if (tuned_prec[pi]) {
+        for (int c = 0; c < GC_COUNT; ++c) cand[c] = cand_prec[pi][c];
       }
       prog.materialize(stages, cand, nwalk);
       walk_flops = prog.walk_flops;

@@ -87,14 +87,15 @@ constexpr int INFO_SPLIT_RANGE = 4;
 // multiple of 16); values outside nr x nc are zero.  A workgroup covers 16
 // rows x 16 k-blocks: the 16 threads of a row read 16 consecutive 64-byte
 // runs, and every thread writes its own 2 * NQ * 32-byte output block.
+// Workgroup (bx, by) of the split: rows by*16.., k-blocks bx*16...
 template <class P>
-__global__ __launch_bounds__(256) void split_blocked(const float* __restrict__ yr, const float* __restrict__ yi,
-                                                     int64_t ldy, int nr, int nc, int rows, int kpad,
-                                                     typename P::H* __restrict__ out, int* info) {
+__device__ __forceinline__ void split_blocked_block(const float* __restrict__ yr, const float* __restrict__ yi,
+                                                    int64_t ldy, int nr, int nc, int rows, int kpad,
+                                                    typename P::H* __restrict__ out, int* info, int bx, int by) {
   using H = typename P::H;
   constexpr int NQ = P::NQ;
-  const int r = blockIdx.y * 16 + (threadIdx.x >> 4);
-  const int kb = blockIdx.x * 16 + (threadIdx.x & 15);
+  const int r = by * 16 + (threadIdx.x >> 4);
+  const int kb = bx * 16 + (threadIdx.x & 15);
   if (r >= rows || kb * 16 >= kpad) return;
   float v[2][16];
   const float* src[2] = {yr, yi};
@@ -150,6 +151,13 @@ __global__ __launch_bounds__(256) void split_blocked(const float* __restrict__ y
   if (bad) atomicOr(info, INFO_SPLIT_RANGE);
 }
 
+template <class P>
+__global__ __launch_bounds__(256) void split_blocked(const float* __restrict__ yr, const float* __restrict__ yi,
+                                                     int64_t ldy, int nr, int nc, int rows, int kpad,
+                                                     typename P::H* __restrict__ out, int* info) {
+  split_blocked_block<P>(yr, yi, ldy, nr, nc, rows, kpad, out, info, (int)blockIdx.x, (int)blockIdx.y);
+}
+
 template <class P> struct SqArgs {
   const typename P::H* a;  // k-blocked split parts
   int rows;                // rows of the k-blocked layout (multiple of 128)
@@ -190,8 +198,10 @@ __device__ __forceinline__ int sq_lds(int row, int ch) {
 // arrays) do not hit the same banks.
 constexpr int SQ_TILE_STRIDE = SQ_BM * SQ_BK + 16;
 
-template <class P, int WAVES_M = P::WAVES_M, int WAVES_N = P::WAVES_N>
-__global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void sq_split_kernel(SqArgs<P> g) {
+// Output tile (tm, tn), tm >= tn, of the split squaring; every LDS read is
+// complete on return, so a workgroup may call it again for another tile.
+template <class P, int WAVES_M, int WAVES_N>
+__device__ __forceinline__ void sq_split_tile(const SqArgs<P>& g, int tm, int tn) {
   using H = typename P::H;
   using V = typename P::V;
   constexpr int NT = 64 * WAVES_M * WAVES_N;
@@ -203,15 +213,6 @@ __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void sq_split_kernel(SqArgs
   static_assert(CH % NT == 0, "stage chunks");
   __shared__ __attribute__((aligned(16))) H smem[2][2 * W * TILE];
 
-  int tm, tn;
-  const int lb = xcd_remap((int)blockIdx.x, (int)gridDim.x);
-  if (g.order) {
-    const int code = g.order[lb];
-    tm = code >> 16;
-    tn = code & 0xFFFF;
-  } else {
-    tri_tile(lb, tm, tn);
-  }
   const int row0 = tm * SQ_BM, col0 = tn * SQ_BM;
   constexpr int FI = SQ_BM / WAVES_M / 32, FJ = SQ_BM / WAVES_N / 32;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -365,6 +366,45 @@ __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void sq_split_kernel(SqArgs
         }
       }
     }
+  }
+}
+
+template <class P, int WAVES_M = P::WAVES_M, int WAVES_N = P::WAVES_N>
+__global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void sq_split_kernel(SqArgs<P> g) {
+  int tm, tn;
+  const int lb = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+  if (g.order) {
+    const int code = g.order[lb];
+    tm = code >> 16;
+    tn = code & 0xFFFF;
+  } else {
+    tri_tile(lb, tm, tn);
+  }
+  sq_split_tile<P, WAVES_M, WAVES_N>(g, tm, tn);
+}
+
+// Grouped split squarings: one problem per walk, lower tiles of SQ_BM; the
+// same tile placement and persistent striding as gemm_group_kernel.
+template <class P> struct GroupSq {
+  SqArgs<P> g;
+  int tile0;
+  int tiles;
+};
+
+template <class P>
+__global__ __launch_bounds__(sq_threads<P>()) void sq_split_group_kernel(const GroupSq<P>* __restrict__ probs,
+                                                                         int nprob, int total, int chunk) {
+  for (int b = (int)blockIdx.x; b < total; b += (int)gridDim.x) {
+    const int t = xcd_chunk_remap(b, total, chunk);
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (probs[mid].tile0 <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    int tm, tn;
+    tri_tile(t - probs[lo].tile0, tm, tn);
+    sq_split_tile<P, P::WAVES_M, P::WAVES_N>(probs[lo].g, tm, tn);
   }
 }
 

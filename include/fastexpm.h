@@ -188,7 +188,8 @@ int fexpm_batch_set_option(fexpm_batch* b, int key, int64_t value);
  * trace shift) stay the same; tile shapes come from an autotuner that times
  * each candidate on the batch itself.  Walks that are not of the CTQW form
  * (A = iG, G real symmetric) or have non-finite entries run on a lane
- * pipeline instead.  Split precision and the LU solve use lanes. */
+ * pipeline instead.  Split precision (complex64) runs as grouped split squaring
+ * rounds; the LU solve option (FEXPM_OPT_SPD_SOLVE = 0) uses lanes. */
 typedef struct fexpm_batch_report {
   int32_t grouped;         /* 1: the last run used grouped launches */
   int32_t grouped_walks;   /* walks run by the grouped program */

@@ -482,6 +482,31 @@ def test_batch_grouped(fx, dt):
     assert b2.report()["tuned"] == 0 and b2.report()["candidate"] == [0, 0, 0]
     for i in range(8):
         assert rel_fro(_eig_ref(mats[i]), us5[i]) <= tol_fn(b2.stats(i)["s"]), i
+    # every autotuner candidate (tile shape x dispatch) on its own
+    from fastctqw_amd.fastexpm import _fastexpm_core as core
+
+    ctqw = [i for i, k in enumerate(kinds) if k == "ctqw"]
+    for c in range(6):
+        b2._backend.set_option(core.OPT_GROUP_CANDIDATE, c)
+        us_c = b2.run([mats[i] for i in ctqw])
+        assert b2.report()["candidate"] == [c, c, c]
+        for q, i in enumerate(ctqw):
+            assert rel_fro(_eig_ref(mats[i]), us_c[q]) <= tol_fn(b2.stats(q)["s"]), (c, i)
+    b2._backend.set_option(core.OPT_GROUP_CANDIDATE, -1)
+    if dt == np.complex64:
+        # grouped split-precision squaring rounds
+        for prec in ("fp16x2", "bf16x3"):
+            b2.set_precision(prec)
+            for c in (0, 2):
+                b2._backend.set_option(core.OPT_GROUP_CANDIDATE, c)
+                us_h = b2.run([mats[i] for i in ctqw])
+                assert b2.report()["grouped"] == 1
+                for q, i in enumerate(ctqw):
+                    st_q = b2.stats(q)
+                    assert st_q["precision"] == prec
+                    assert rel_fro(_eig_ref(mats[i]), us_h[q]) <= tol_fn(st_q["s"]), (prec, c, i)
+        b2._backend.set_option(core.OPT_GROUP_CANDIDATE, -1)
+        b2.set_precision("native")
 
 
 def test_ctqw_drop_in(fx):
