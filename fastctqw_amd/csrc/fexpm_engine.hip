@@ -2344,6 +2344,73 @@ int fexpm_debug_plane(fexpm_engine* e, int which, double* re, double* im) {
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
+// Grouped program description on a virtual layout (testing ABI): the same
+// builder as the batch runs, no device memory touched.
+namespace {
+template <typename Tin, typename W>
+std::string group_program_text(int precision, int64_t count, const int64_t* n, const int32_t* m, const int32_t* s,
+                               const int32_t* shifted, const double* mu_re, const double* mu_im, int with_psi) {
+  GroupRunner<Tin, W> R(true);
+  R.precision = precision;
+  std::vector<int64_t> nv(n, n + count);
+  std::vector<size_t> off((size_t)count), voff((size_t)count);
+  size_t tot = 0, vt = 0;
+  for (int64_t i = 0; i < count; ++i) {
+    if (n[i] < 1 || n[i] > (1 << 16)) throw Error(FEXPM_EINVAL, "order outside [1, 65536]");
+    if (s[i] < 0 || s[i] > 64 || (m[i] != 3 && m[i] != 5 && m[i] != 7 && m[i] != 9 && m[i] != 13))
+      throw Error(FEXPM_EINVAL, "bad plan (m or s)");
+    off[(size_t)i] = tot;
+    voff[(size_t)i] = vt;
+    tot += (size_t)n[i] * n[i];
+    vt += (size_t)n[i];
+  }
+  R.set_layout(nv, off, nullptr);
+  std::vector<GroupPlan> plans((size_t)count);
+  for (int64_t i = 0; i < count; ++i) {
+    GroupPlan& P = plans[(size_t)i];
+    std::memset(&P, 0, sizeof(P));
+    P.idx = (int)i;
+    P.m = m[i];
+    P.s = s[i];
+    P.shifted = shifted[i] ? 1 : 0;
+    P.mu_re = mu_re[i];
+    P.mu_im = mu_im[i];
+  }
+  R.build(plans, vaddr(VR_IN), vaddr(VR_OUT), reinterpret_cast<Scalars*>(vaddr(VR_SCAL)),
+          with_psi ? reinterpret_cast<const double2*>(vaddr(VR_PSI0)) : nullptr,
+          reinterpret_cast<double2*>(vaddr(VR_PSI)), voff);
+  return R.describe();
+}
+}  // namespace
+
+extern "C" int fexpm_test_group_program(int dtype, int precision, int64_t count, const int64_t* n, const int32_t* m,
+                                        const int32_t* s, const int32_t* shifted, const double* mu_re,
+                                        const double* mu_im, int with_psi, char* buf, int64_t cap, int64_t* len) {
+  if (count < 1 || !n || !m || !s || !shifted || !mu_re || !mu_im || !len) return FEXPM_EINVAL;
+  if (dtype != FEXPM_COMPLEX64 && dtype != FEXPM_COMPLEX128) return FEXPM_EINVAL;
+  if (precision < FEXPM_PREC_NATIVE || precision > FEXPM_PREC_BF16X3 ||
+      (precision != FEXPM_PREC_NATIVE && dtype != FEXPM_COMPLEX64))
+    return FEXPM_EINVAL;
+  try {
+    const std::string t = dtype == FEXPM_COMPLEX64
+                              ? group_program_text<float2, float>(precision, count, n, m, s, shifted, mu_re, mu_im,
+                                                                  with_psi)
+                              : group_program_text<double2, double>(precision, count, n, m, s, shifted, mu_re,
+                                                                    mu_im, with_psi);
+    *len = (int64_t)t.size();
+    if (!buf || cap < (int64_t)t.size() + 1) return FEXPM_EINVAL;
+    std::memcpy(buf, t.c_str(), t.size() + 1);
+    return FEXPM_OK;
+  } catch (const Error& x) {
+    g_last_error = x.what();
+    return x.code;
+  } catch (const std::exception& x) {
+    g_last_error = x.what();
+    return FEXPM_EINVAL;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Device-side GEMM timing (testing ABI): average launch time over reps
 // launches on device-resident operands filled with a deterministic pattern.
 namespace {

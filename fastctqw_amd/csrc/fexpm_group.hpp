@@ -630,6 +630,126 @@ struct GroupPlan {
   double mu_re, mu_im, norm, norm_plain;
 };
 
+// Virtual address regions of a host-only program description.
+enum VRegion { VR_F64 = 1, VR_W, VR_SMALL, VR_SPLIT, VR_IN, VR_OUT, VR_PSI0, VR_PSI, VR_SCAL };
+constexpr int VR_SHIFT = 40;
+inline void* vaddr(int region) { return reinterpret_cast<void*>((uintptr_t)region << VR_SHIFT); }
+
+// "region:byte offset" of a virtual address (0:0 for null).
+inline std::string vfmt(const void* p) {
+  const uintptr_t a = reinterpret_cast<uintptr_t>(p);
+  char b[64];
+  std::snprintf(b, sizeof(b), "%llu:%llu", (unsigned long long)(a >> VR_SHIFT),
+                (unsigned long long)(a & (((uintptr_t)1 << VR_SHIFT) - 1)));
+  return b;
+}
+inline std::string dfmt(double v) {
+  char b[40];
+  std::snprintf(b, sizeof(b), "%.17g", v);
+  return b;
+}
+
+// One text line per problem of a stage (testing ABI).
+template <typename T> static std::string describe_gemm(const GroupStage& S, int p) {
+  const GemmArgs<T>& g = *reinterpret_cast<const GemmArgs<T>*>(S.probs.data() + (size_t)p * S.psize);
+  std::string o = "GEMM cls=" + std::to_string(S.cls) + " mode=" + std::to_string(S.mode) +
+                  " sym=" + std::to_string((int)S.sym[p]) + " inplace=" + std::to_string(S.inplace[p]) +
+                  " ar=" + vfmt(g.ar) + " ai=" + vfmt(g.ai) + " lda=" + std::to_string(g.lda) +
+                  " br=" + vfmt(g.br) + " bi=" + vfmt(g.bi) + " ldb=" + std::to_string(g.ldb) +
+                  " m=" + std::to_string(g.m) + " n=" + std::to_string(g.n) + " k=" + std::to_string(g.k) +
+                  " cr=" + vfmt(g.epi.cr) + " ci=" + vfmt(g.epi.ci) + " ldc=" + std::to_string(g.epi.ldc) +
+                  " alpha=" + dfmt(g.epi.alpha) + " beta=" + dfmt(g.epi.beta) +
+                  " nadd=" + std::to_string(g.epi.nadd) + " ld_add=" + std::to_string(g.epi.ld_add) +
+                  " diag=" + dfmt(g.epi.diag) + " din=" + vfmt(g.epi.d_in) + " dout=" + vfmt(g.epi.d_out);
+  for (int t = 0; t < g.epi.nadd; ++t)
+    o += " add" + std::to_string(t) + "=" + vfmt(g.epi.add_r[t]) + " c" + std::to_string(t) + "=" +
+         dfmt(g.epi.addc[t]);
+  return o;
+}
+
+template <class P> static std::string describe_sq(const GroupStage& S, int p) {
+  const SqArgs<P>& a = *reinterpret_cast<const SqArgs<P>*>(S.probs.data() + (size_t)p * S.psize);
+  return "SQ a=" + vfmt(a.a) + " rows=" + std::to_string(a.rows) + " yr=" + vfmt(a.yr) + " yi=" + vfmt(a.yi) +
+         " ldy=" + std::to_string(a.ldy) + " cr=" + vfmt(a.cr) + " ci=" + vfmt(a.ci) +
+         " ldc=" + std::to_string(a.ldc) + " din=" + vfmt(a.d_in) + " dout=" + vfmt(a.d_out) +
+         " m=" + std::to_string(a.m) + " k=" + std::to_string(a.k);
+}
+
+template <class P> static std::string describe_split(const GroupStage& S, int p) {
+  const typename OpSplit<P>::Prob& q =
+      *reinterpret_cast<const typename OpSplit<P>::Prob*>(S.probs.data() + (size_t)p * S.psize);
+  return "SPLIT yr=" + vfmt(q.yr) + " yi=" + vfmt(q.yi) + " ldy=" + std::to_string(q.ldy) +
+         " n=" + std::to_string(q.n) + " rows=" + std::to_string(q.rows) + " kpad=" + std::to_string(q.kpad) +
+         " out=" + vfmt(q.out) + " info=" + vfmt(q.info);
+}
+
+template <typename Tin, typename W> static std::string describe_stage(const GroupStage& S) {
+  std::string out;
+  const int cnt = S.count();
+  out += "STAGE kind=" + std::to_string(S.kind) + " problems=" + std::to_string(cnt) + "\n";
+  for (int p = 0; p < cnt; ++p) {
+    const unsigned char* b = S.probs.data() + (size_t)p * S.psize;
+    std::string l;
+    switch (S.kind) {
+      case GK_PREPARE: {
+        const auto& q = *reinterpret_cast<const typename OpPrepare<Tin>::Prob*>(b);
+        l = "PREPARE a=" + vfmt(q.a) + " n=" + std::to_string(q.n) + " ld=" + std::to_string(q.ld) +
+            " g=" + vfmt(q.g) + " mu_im=" + dfmt(q.mu_im) + " scale=" + dfmt(q.scale);
+        break;
+      }
+      case GK_LINCOMB: {
+        const auto& q = *reinterpret_cast<const OpLincomb::Prob*>(b);
+        l = "LINCOMB out=" + vfmt(q.out) + " nin=" + std::to_string(q.nin) + " diag=" + dfmt(q.diag) +
+            " n=" + std::to_string(q.n) + " ld=" + std::to_string(q.ld);
+        for (int t = 0; t < q.nin; ++t)
+          l += " in" + std::to_string(t) + "=" + vfmt(q.in[t]) + " c" + std::to_string(t) + "=" + dfmt(q.c[t]);
+        break;
+      }
+      case GK_PACK: {
+        const auto& q = *reinterpret_cast<const OpPack::Prob*>(b);
+        l = "PACK v=" + vfmt(q.v) + " u=" + vfmt(q.u) + " n=" + std::to_string(q.n) + " np=" + std::to_string(q.np) +
+            " ld=" + std::to_string(q.ld) + " zb=" + vfmt(q.zb);
+        break;
+      }
+      case GK_CHOL: {
+        const auto& q = *reinterpret_cast<const OpChol::Prob*>(b);
+        l = "CHOL s=" + vfmt(q.s) + " ld=" + std::to_string(q.ld) + " k0=" + std::to_string(q.k0) +
+            " n=" + std::to_string(q.n) + " inv=" + vfmt(q.inv) + " sc=" + vfmt(q.sc);
+        break;
+      }
+      case GK_SOLUTION: {
+        const auto& q = *reinterpret_cast<const typename OpSolution<W>::Prob*>(b);
+        l = "SOLUTION xr=" + vfmt(q.xr) + " xi=" + vfmt(q.xi) + " ldx=" + std::to_string(q.ldx) +
+            " wr=" + vfmt(q.wr) + " wi=" + vfmt(q.wi) + " ldw=" + std::to_string(q.ldw) +
+            " n=" + std::to_string(q.n) + " d=" + vfmt(q.d);
+        break;
+      }
+      case GK_OUTPUT: {
+        const auto& q = *reinterpret_cast<const typename OpOutput<W, Tin>::Prob*>(b);
+        l = "OUTPUT yr=" + vfmt(q.yr) + " yi=" + vfmt(q.yi) + " ld=" + std::to_string(q.ld) +
+            " n=" + std::to_string(q.n) + " d=" + vfmt(q.d) + " phr=" + dfmt(q.ph_re) + " phi=" + dfmt(q.ph_im) +
+            " out=" + vfmt(q.out);
+        break;
+      }
+      case GK_APPLY: {
+        const auto& q = *reinterpret_cast<const typename OpApply<W>::Prob*>(b);
+        l = "APPLY yr=" + vfmt(q.yr) + " yi=" + vfmt(q.yi) + " ld=" + std::to_string(q.ld) +
+            " n=" + std::to_string(q.n) + " d=" + vfmt(q.d) + " psi0=" + vfmt(q.psi0) +
+            " phr=" + dfmt(q.ph_re) + " phi=" + dfmt(q.ph_im) + " out=" + vfmt(q.out);
+        break;
+      }
+      case GK_GEMM: l = S.f32 ? describe_gemm<float>(S, p) : describe_gemm<double>(S, p); break;
+      case GK_SPLIT_F16: l = describe_split<Fp16x2>(S, p); break;
+      case GK_SPLIT_BF16: l = describe_split<Bf16x3>(S, p); break;
+      case GK_SQ_F16: l = describe_sq<Fp16x2>(S, p); break;
+      case GK_SQ_BF16: l = describe_sq<Bf16x3>(S, p); break;
+      default: l = "UNKNOWN"; break;
+    }
+    out += l + "\n";
+  }
+  return out;
+}
+
 // Runner for one batch dtype: Tin = float2 (complex64, FP32 squarings) or
 // double2 (complex128, FP64 squarings); Pade products and solve in FP64.
 struct GroupRunnerBase {
@@ -697,19 +817,27 @@ template <typename Tin, typename W> struct GroupRunner : GroupRunnerBase {
   std::vector<hipEvent_t> evpool;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
-  GroupRunner() {
-    FX_HIP(hipEventCreate(&ev0));
-    FX_HIP(hipEventCreate(&ev1));
-  }
+  // virt: no device memory; arenas get the addresses region << 40 (regions
+  // of vaddr) so that the program can be built and described on the host.
+  bool virt = false;
+
+  explicit GroupRunner(bool virtual_layout = false) : virt(virtual_layout) {}
   ~GroupRunner() override {
     drop_graph();
-    if (f64_arena) (void)hipFree(f64_arena);
-    if (w_arena) (void)hipFree(w_arena);
-    if (small_arena) (void)hipFree(small_arena);
-    if (split_arena) (void)hipFree(split_arena);
+    if (!virt) {
+      if (f64_arena) (void)hipFree(f64_arena);
+      if (w_arena) (void)hipFree(w_arena);
+      if (small_arena) (void)hipFree(small_arena);
+      if (split_arena) (void)hipFree(split_arena);
+    }
     for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+  }
+
+  void ensure_events() {
+    if (!ev0) FX_HIP(hipEventCreate(&ev0));
+    if (!ev1) FX_HIP(hipEventCreate(&ev1));
   }
 
   void force_candidate(int c) override {
@@ -769,13 +897,22 @@ template <typename Tin, typename W> struct GroupRunner : GroupRunnerBase {
         capb = need;
       }
     };
-    grow(f64_arena, f64_cap, std::max<size_t>(f64_tot, 256));
-    grow(w_arena, w_cap, std::max<size_t>(w_tot, 256));
-    grow(small_arena, small_cap, std::max<size_t>(small_tot, 256));
-    // padding strips must read as zero: clear everything once per layout
-    FX_HIP(hipMemset(f64_arena, 0, f64_tot));
-    FX_HIP(hipMemset(w_arena, 0, w_tot));
-    FX_HIP(hipMemset(small_arena, 0, small_tot));
+    if (virt) {
+      f64_arena = reinterpret_cast<double*>(vaddr(VR_F64));
+      w_arena = reinterpret_cast<W*>(vaddr(VR_W));
+      small_arena = reinterpret_cast<unsigned char*>(vaddr(VR_SMALL));
+      f64_cap = f64_tot;
+      w_cap = w_tot;
+      small_cap = small_tot;
+    } else {
+      grow(f64_arena, f64_cap, std::max<size_t>(f64_tot, 256));
+      grow(w_arena, w_cap, std::max<size_t>(w_tot, 256));
+      grow(small_arena, small_cap, std::max<size_t>(small_tot, 256));
+      // padding strips must read as zero: clear everything once per layout
+      FX_HIP(hipMemset(f64_arena, 0, f64_tot));
+      FX_HIP(hipMemset(w_arena, 0, w_tot));
+      FX_HIP(hipMemset(small_arena, 0, small_tot));
+    }
     d_flags = reinterpret_cast<int*>(small_arena + flags_off);
     for (int i = 0; i < cnt; ++i) {
       WalkMem& M = wm[i];
@@ -815,7 +952,10 @@ template <typename Tin, typename W> struct GroupRunner : GroupRunnerBase {
       split_off[i] = tot;
       tot += ((size_t)split_rows(wm[i].n) * split_kpad(wm[i].n) * 2 * 3 * sizeof(uint16_t) + 255) / 256 * 256;
     }
-    if (tot > split_cap) {
+    if (virt) {
+      split_arena = vaddr(VR_SPLIT);
+      split_cap = tot;
+    } else if (tot > split_cap) {
       if (split_arena) (void)hipFree(split_arena);
       split_arena = nullptr;
       split_cap = 0;
@@ -1197,6 +1337,16 @@ template <typename Tin, typename W> struct GroupRunner : GroupRunnerBase {
     (void)G;
   }
 
+  // Text description of the program (testing ABI): region sizes, then every
+  // stage and problem.
+  std::string describe() const {
+    std::string o = "REGIONS f64=" + std::to_string(f64_cap) + " w=" + std::to_string(w_cap) +
+                    " small=" + std::to_string(small_cap) + " split=" + std::to_string(split_cap) +
+                    " wsize=" + std::to_string(sizeof(W)) + " insize=" + std::to_string(sizeof(Tin)) + "\n";
+    for (const GroupStage& S : stages) o += describe_stage<Tin, W>(S);
+    return o;
+  }
+
   void enqueue(hipStream_t st) {
     for (const GroupLaunch& L : prog.launches) group_enqueue_one<Tin, W>(L, prog.d_arena, st);
   }
@@ -1302,6 +1452,7 @@ template <typename Tin, typename W> struct GroupRunner : GroupRunnerBase {
       walk_flops = prog.walk_flops;
     }
     launches = (int)prog.launches.size();
+    ensure_events();
     FX_HIP(hipEventRecord(ev0, st));
     if (use_graph) {
       if (!gexec) {

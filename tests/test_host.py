@@ -70,7 +70,7 @@ def test_laplacian_and_state():
 # --------------------------------------------------------------------------- C ABI
 
 def _declared_symbols():
-    text = open(HEADER).read()
+    text = "".join(open(h).read() for h in (HEADER, os.path.join(ROOT, "include", "fastexpm_testing.h")))
     return sorted(set(re.findall(r"^\s*(?:int|int64_t|double|const char\*)\s+(fexpm_\w+)\(", text, re.M)))
 
 
@@ -79,6 +79,7 @@ def test_library_exports_every_declared_symbol():
         pytest.skip("HIP library not built")
     syms = _declared_symbols()
     assert len(syms) >= 14
+    assert "fexpm_batch_get_report" in syms and "fexpm_test_group_program" in syms
     lib = ctypes.CDLL(LIB)
     for s in syms:
         assert hasattr(lib, s), s
