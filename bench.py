@@ -49,6 +49,23 @@ def _profiled_traffic(key):
     return None
 
 
+MFMA_PROFILE = "profiles/mfma_sustained_rates.md"
+
+
+def _sustained_peak(key):
+    """Sustained MFMA TFLOP/s of the device measured by the in-register
+    probe (bench.py --mfma-rate), as recorded in MFMA_PROFILE (None if absent)."""
+    try:
+        with open(os.path.join(ROOT, MFMA_PROFILE)) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "sustained_tflops" and parts[1] == key:
+                    return float(parts[2])
+    except OSError:
+        pass
+    return None
+
+
 def work_flops(n, s, products=6):
     # 4M complex convention: 8 n^3 per complex product/squaring, LU 8/3 n^3, solve 8 n^3
     return (8.0 * (products + s) + 32.0 / 3.0) * float(n) ** 3
@@ -640,6 +657,27 @@ def _copy_calibration(mib):
                       "GB_s": 2.0 * nbytes / (ms.value * 1e-3) / 1e9}), flush=True)
 
 
+def _mfma_rates():
+    """Sustained MFMA rates of the device (in-register MFMA loop, testing
+    ABI), beside the datasheet peaks the roofline uses."""
+    import ctypes
+
+    lib = ctypes.CDLL(os.path.join(ROOT, "fastctqw_amd", "fastexpm", "libfastexpm_hip.so"))
+    lib.fexpm_last_error.restype = ctypes.c_char_p
+    out = {}
+    for kind, name, peak, iters in ((0, "fp32_16x16x4", FP32_PEAK_TFLOPS, 20000),
+                                    (1, "fp64_16x16x4", FP64_PEAK_TFLOPS, 20000),
+                                    (2, "fp16_32x32x16", FP16_PEAK_TFLOPS, 8000),
+                                    (3, "bf16_32x32x16", FP16_PEAK_TFLOPS, 8000)):
+        tf, ms = ctypes.c_double(0.0), ctypes.c_double(0.0)
+        rc = lib.fexpm_test_mfma_rate(kind, iters, ctypes.byref(tf), ctypes.byref(ms))
+        if rc != 0:
+            raise RuntimeError(lib.fexpm_last_error(None).decode())
+        out[name] = {"tflops": tf.value, "datasheet_tflops": peak, "fraction_of_datasheet": tf.value / peak,
+                     "kernel_ms": ms.value}
+    print(json.dumps({"mfma_sustained": out}), flush=True)
+
+
 def _host_info(cores):
     import platform
 
@@ -701,12 +739,17 @@ def main():
                     help="c2: accumulation block of the squarings in 16-k tiles (0 = one block)")
     ap.add_argument("--no-api-timing", action="store_true",
                     help="c2: skip the host-inclusive run() and Ctqw.runWalk timings")
+    ap.add_argument("--mfma-rate", action="store_true",
+                    help="only measure the sustained MFMA rates (in-register loop) and exit")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
 
     if args.copy_calibration:
         _copy_calibration(args.copy_calibration)
+        return
+    if args.mfma_rate:
+        _mfma_rates()
         return
     if args.c3_ref:
         _c3_reference(args.n, args.c3_ref)
@@ -822,6 +865,8 @@ def main():
         traffic_key = "fp32_squaring" if cdt == np.complex64 else None
     issued_tf = issued / (sq_ms * 1e-3) / 1e12
     traffic = _profiled_traffic(traffic_key) if n == 10000 else None
+    sustained = _sustained_peak("fp16" if split and args.precision == "fp16x2" else
+                                "bf16" if split else "fp32") if cdt == np.complex64 else None
 
     # accuracy: invariants and, unless disabled, infidelity against an FP64
     # eigendecomposition of the same H computed in a child process
@@ -874,6 +919,8 @@ def main():
                      "traffic": traffic, "traffic_source": TRAFFIC_PROFILE if traffic else None,
                      "avg_launch_ms": sq_ms,
                      "issued_tflops": issued_tf, "issued_frac": issued_tf / peak,
+                     "sustained_peak": sustained, "sustained_source": MFMA_PROFILE if sustained else None,
+                     "issued_frac_of_sustained": issued_tf / sustained if sustained else None,
                      "note": "achieved = 8 N^3 algorithmic flops per squaring (4M complex convention) / "
                              "avg launch; issued = MFMA flops the kernel executes (half tiles; equal to the "
                              "profiler's SQ_INSTS_VALU_MFMA_MOPS x 512); traffic = HBM bytes per launch from "

@@ -2556,6 +2556,100 @@ __global__ __launch_bounds__(256) void copy_kernel(const u32x4* __restrict__ src
   for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < cnt; k += (size_t)gridDim.x * 256) dst[k] = src[k];
 }
 
+// In-register MFMA throughput: every wave issues `iters` rounds of 8
+// independent MFMAs (no memory traffic in the loop), so the rate is the
+// sustained matrix-core rate of the device at its running clock.
+// kind 0: v_mfma_f32_16x16x4f32, 1: v_mfma_f64_16x16x4f64,
+// 2: v_mfma_f32_32x32x16_f16, 3: v_mfma_f32_32x32x16_bf16.
+template <int KIND>
+__global__ __launch_bounds__(256) void mfma_rate_kernel(float* out, int iters) {
+  const float seed = (float)(threadIdx.x & 7) * 1e-3f;
+  float keep = 0.0f;
+  if constexpr (KIND == 0) {
+    f32x4 acc[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) acc[q] = f32x4{};
+    const float a = 1.0f + seed, b = 1.0f - seed;
+    for (int it = 0; it < iters; ++it)
+#pragma unroll
+      for (int q = 0; q < 8; ++q) acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[q], 0, 0, 0);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) keep += acc[q][0];
+  } else if constexpr (KIND == 1) {
+    f64x4 acc[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) acc[q] = f64x4{};
+    const double a = 1.0 + seed, b = 1.0 - seed;
+    for (int it = 0; it < iters; ++it)
+#pragma unroll
+      for (int q = 0; q < 8; ++q) acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[q], 0, 0, 0);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) keep += (float)acc[q][0];
+  } else {
+    using V = typename std::conditional<KIND == 2, f16x8, bf16x8>::type;
+    using E = typename std::conditional<KIND == 2, _Float16, __bf16>::type;
+    V a, b;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      a[e] = (E)(1.0f + seed);
+      b[e] = (E)(1.0f - seed);
+    }
+    f32x16 acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] = f32x16{};
+    for (int it = 0; it < iters; ++it)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if constexpr (KIND == 2) acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, acc[q], 0, 0, 0);
+        else acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[q], 0, 0, 0);
+      }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) keep += acc[q][0];
+  }
+  if (keep == 1234.5f) out[blockIdx.x] = keep;  // keeps the loop alive; never true for these inputs
+}
+
+extern "C" int fexpm_test_mfma_rate(int kind, int iters, double* tflops, double* ms) {
+  if (!tflops || !ms || kind < 0 || kind > 3 || iters <= 0 || iters > (1 << 20)) return FEXPM_EINVAL;
+  return guarded(nullptr, [&] {
+    check_device();
+    int dev = 0;
+    hipDeviceProp_t prop;
+    FX_HIP(hipGetDevice(&dev));
+    FX_HIP(hipGetDeviceProperties(&prop, dev));
+    const unsigned grid = (unsigned)prop.multiProcessorCount * 8;  // 8 waves per SIMD
+    float* out = nullptr;
+    FX_HIP(hipMalloc(&out, grid * sizeof(float)));
+    hipEvent_t e0, e1;
+    FX_HIP(hipEventCreate(&e0));
+    FX_HIP(hipEventCreate(&e1));
+    auto launch = [&](int n) {
+      switch (kind) {
+        case 0: mfma_rate_kernel<0><<<grid, 256>>>(out, n); break;
+        case 1: mfma_rate_kernel<1><<<grid, 256>>>(out, n); break;
+        case 2: mfma_rate_kernel<2><<<grid, 256>>>(out, n); break;
+        default: mfma_rate_kernel<3><<<grid, 256>>>(out, n); break;
+      }
+      FX_LAUNCH();
+    };
+    launch(std::max(1, iters / 8));
+    FX_HIP(hipEventRecord(e0));
+    launch(iters);
+    FX_HIP(hipEventRecord(e1));
+    FX_HIP(hipEventSynchronize(e1));
+    float t = 0.f;
+    FX_HIP(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(out);
+    // flops per MFMA instruction (one wave): 16x16x4 -> 2048, 32x32x16 -> 32768
+    const double per = kind < 2 ? 2048.0 * 8 : 32768.0 * 4;
+    const double flops = (double)grid * 4.0 * (double)iters * per;
+    *ms = t;
+    *tflops = flops / ((double)t * 1e-3) / 1e12;
+  });
+}
+
 extern "C" int fexpm_test_copy(int64_t bytes, int reps, double* ms) {
   if (!ms || bytes <= 0 || bytes % 16 != 0 || reps <= 0) return FEXPM_EINVAL;
   return guarded(nullptr, [&] {

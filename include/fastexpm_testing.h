@@ -72,6 +72,13 @@ int fexpm_test_gemm_time(int wtype, int mode, int sym, int cfg, int m, int n, in
  * FETCH_SIZE / WRITE_SIZE counters (each launch reads and writes bytes). */
 int fexpm_test_copy(int64_t bytes, int reps, double* ms);
 
+/* Sustained MFMA rate of the device: a kernel whose waves issue only
+ * independent MFMAs from registers (kind 0 FP32 16x16x4, 1 FP64 16x16x4,
+ * 2 FP16 32x32x16, 3 BF16 32x32x16), 8 waves per SIMD; `iters` rounds per
+ * wave.  Returns TFLOP/s and the kernel time.  Measures the practical peak
+ * that the GEMM kernels' rates are compared with. */
+int fexpm_test_mfma_rate(int kind, int iters, double* tflops, double* ms);
+
 /* Grouped batch program for `count` CTQW walks with the given orders and
  * plans (Pade degree m, squarings s, trace shift), built by the same code as
  * a grouped batch run but on a virtual address layout (no device memory is
