@@ -675,7 +675,24 @@ def _mfma_rates():
             raise RuntimeError(lib.fexpm_last_error(None).decode())
         out[name] = {"tflops": tf.value, "datasheet_tflops": peak, "fraction_of_datasheet": tf.value / peak,
                      "kernel_ms": ms.value}
-    print(json.dumps({"mfma_sustained": out}), flush=True)
+    # the GEMM kernels at the N = 10^4 shapes, issued MFMA flops per launch time
+    gem = {}
+    n, kp = 10000, 10112
+    lib.fexpm_test_gemm_time.argtypes = [ctypes.c_int] * 8 + [ctypes.POINTER(ctypes.c_double)]
+    for name, w, mode, cfg, tile, mult in (("fp32_c3m_sym_128x128_8waves", 0, 2, 0, 128, 3),
+                                           ("fp32_c3m_sym_64x64_4waves", 0, 2, 4, 64, 3),
+                                           ("fp64_real_sym_64x64_8waves", 1, 0, 1, 64, 1),
+                                           ("fp64_real_sym_64x64_4waves", 1, 0, 0, 64, 1),
+                                           ("fp64_c3m_sym_64x64_8waves", 1, 2, 1, 64, 3),
+                                           ("fp64_c3m_sym_64x64_4waves", 1, 2, 0, 64, 3)):
+        ms = ctypes.c_double(0.0)
+        rc = lib.fexpm_test_gemm_time(w, mode, 1, cfg, n, n, kp, 3, ctypes.byref(ms))
+        if rc != 0:
+            raise RuntimeError(lib.fexpm_last_error(None).decode())
+        T = (n + tile - 1) // tile
+        issued = mult * 2.0 * T * (T + 1) / 2 * tile * tile * kp
+        gem[name] = {"ms": ms.value, "issued_tflops": issued / (ms.value * 1e-3) / 1e12}
+    print(json.dumps({"mfma_sustained": out, "gemm_n10000": gem}), flush=True)
 
 
 def _host_info(cores):
@@ -740,7 +757,8 @@ def main():
     ap.add_argument("--no-api-timing", action="store_true",
                     help="c2: skip the host-inclusive run() and Ctqw.runWalk timings")
     ap.add_argument("--mfma-rate", action="store_true",
-                    help="only measure the sustained MFMA rates (in-register loop) and exit")
+                    help="only measure the sustained MFMA rates (in-register loop) and the GEMM kernels "
+                         "at the N=10^4 shapes, then exit")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
