@@ -204,7 +204,8 @@ def run_c5(args):
         "metric": "batched walks/s (32 times x 128 states, N=8192)",
         "value": walks / (dev / 1000.0), "unit": "walks/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": dev, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "c64 (fp32 mfma squarings and propagation, fp64 pade/solve/refinement)",
+        "vs_baseline": None, "dtype": "c64 (fp32 mfma squarings and propagation, pade products from 6 exact int8 slices, "
+                                            "fp64 solve/refinement)",
         "data": "synthetic: seeded weighted ER(8192,0.3) Laplacian, 128 seeded random states",
         "config": {"workload": "C5: U(0.5) once, psi(t_k)=U^k psi0, k=1..32, 128 states", "n": n,
                    "m": st["m"], "s": st["s"], "expm_ms": st["ms_total"], "propagate_ms": st["ms_propagate"],
@@ -352,7 +353,7 @@ def run_c3(args):
         "metric": "s per N=10k complex expm, split-precision squarings (fastest mode at t=4.5, unweighted)",
         "value": best["ms"] / 1000.0, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": best["ms"], "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": f"c64 ({best['precision']} squarings, fp64 pade/solve)",
+        "dtype": f"c64 ({best['precision']} squarings, pade products from 6 exact int8 slices, fp64 solve)",
         "data": "synthetic: seeded ER(10000,0.5) Laplacian, unweighted and weighted U(0,1]",
         "config": {"workload": "C3: expm(-i t L), t in {1, 4.5, 20}, precision native/fp16x2/bf16x3",
                    "n": n, "native_ms_t4.5": native["ms"], "speedup_vs_native": native["ms"] / best["ms"]},
@@ -681,10 +682,16 @@ def _mfma_rates():
     lib.fexpm_test_gemm_time.argtypes = [ctypes.c_int] * 8 + [ctypes.POINTER(ctypes.c_double)]
     for name, w, mode, cfg, tile, mult in (("fp32_c3m_sym_128x128_8waves", 0, 2, 0, 128, 3),
                                            ("fp32_c3m_sym_64x64_4waves", 0, 2, 4, 64, 3),
+                                           ("fp32_c3m_sym_128x128_8waves_pad8", 0, 2, 5, 128, 3),
+                                           ("fp32_c3m_sym_128x128_8waves_32x64", 0, 2, 6, 128, 3),
+                                           ("fp32_c3m_sym_128x128_8waves_pad12", 0, 2, 7, 128, 3),
+                                           ("fp32_c3m_sym_128x128_8waves_32x64_pad8", 0, 2, 8, 128, 3),
                                            ("fp64_real_sym_64x64_8waves", 1, 0, 1, 64, 1),
                                            ("fp64_real_sym_64x64_4waves", 1, 0, 0, 64, 1),
                                            ("fp64_c3m_sym_64x64_8waves", 1, 2, 1, 64, 3),
-                                           ("fp64_c3m_sym_64x64_4waves", 1, 2, 0, 64, 3)):
+                                           ("fp64_c3m_sym_64x64_4waves", 1, 2, 0, 64, 3),
+                                           ("fp64_real_sym_64x64_8waves_pad4", 1, 0, 4, 64, 1),
+                                           ("fp64_c3m_sym_64x64_8waves_pad4", 1, 2, 4, 64, 3)):
         ms = ctypes.c_double(0.0)
         rc = lib.fexpm_test_gemm_time(w, mode, 1, cfg, n, n, kp, 3, ctypes.byref(ms))
         if rc != 0:
@@ -752,6 +759,11 @@ def main():
     ap.add_argument("--no-split", action="store_true",
                     help="c2: skip the fp16x2 split-precision comparison run")
     ap.add_argument("--no-c128", action="store_true", help="c2: skip the complex128 run")
+    ap.add_argument("--fp64-slices", type=int, default=None,
+                    help="c2: FP64 Pade products from this many exact int8 slices (4..6) on the int8 MFMA; "
+                         "0 = FP64 MFMA (default: the engine's, 6)")
+    ap.add_argument("--fp64-slices-scope", type=int, default=3, choices=[1, 2, 3],
+                    help=argparse.SUPPRESS)
     ap.add_argument("--kblock", type=int, default=None,
                     help="c2: accumulation block of the squarings in 16-k tiles (0 = one block)")
     ap.add_argument("--no-api-timing", action="store_true",
@@ -847,6 +859,10 @@ def main():
     if args.kblock is not None:
         from fastctqw_amd.fastexpm import _fastexpm_core
         eng._backend.set_option(_fastexpm_core.OPT_KBLOCK, args.kblock)
+    if args.fp64_slices is not None and cdt == np.complex64:
+        from fastctqw_amd.fastexpm import _fastexpm_core
+        eng._backend.set_option(_fastexpm_core.OPT_FP64_SLICES, args.fp64_slices)
+        eng._backend.set_option(_fastexpm_core.OPT_FP64_SLICES_SCOPE, args.fp64_slices_scope)
     eng.upload(a)
     for _ in range(args.warmup):
         eng.run_resident()
@@ -924,12 +940,14 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": (f"c64 ({'fp32 mfma' if args.precision == 'native' else args.precision} squarings, fp64 solve)"
-                  if cdt == np.complex64 else "c128 (fp64)"),
+        "dtype": (f"c64 ({'fp32 mfma' if args.precision == 'native' else args.precision} squarings, "
+                  f"{'pade products from %d exact int8 slices (int32 accumulation), ' % st['fp64_slices'] if st.get('fp64_slices') else 'fp64 pade, '}"
+                  f"fp64 solve)" if cdt == np.complex64 else "c128 (fp64)"),
         "data": "synthetic: seeded ER(N,p) Laplacian, A=-i*t*L",
         "config": {"workload": f"expm(-i*{args.t}*L), L Laplacian of ER({n},{args.p}) seed {args.seed}, "
                                f"{args.dtype}, full U resident", "n": n, "t": args.t,
-                   "m": st["m"], "s": s, "structured": st["structured"]},
+                   "m": st["m"], "s": s, "structured": st["structured"],
+                   "pade_int8_slices": st.get("fp64_slices", 0), "int8_slice_scope": st.get("fp64_slices_scope", 0)},
         "phases_ms": {k: st[k] for k in ("ms_analyze", "ms_pade", "ms_solve", "ms_square", "ms_output",
                                           "ms_total")},
         "roofline": {"kernel": kname, "bound": "mfma",

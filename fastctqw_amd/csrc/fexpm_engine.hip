@@ -38,6 +38,7 @@
 #include "fexpm_gemm.hpp"
 #include "fexpm_kernels.hpp"
 #include "fexpm_split.hpp"
+#include "fexpm_ozaki.hpp"
 
 namespace fexpm {
 
@@ -121,6 +122,8 @@ struct PipelineBase {
   int panel_coop = 1;      // LU panels by one cooperative launch (0: one launch per column)
   int spd_solve = 1;       // structured inputs: Cholesky solve (0: pivoted LU as for general A)
   int precision = 0;       // enum fexpm_precision for the structured squarings of FP32 engines
+  int oz_slices = 0;       // FP64 companion: symmetric FP64 products from int8 slices (0 = FP64 MFMA)
+  int oz_scope = 3;        // stages using them: bit 0 the Pade products, bit 1 the solve (S, X = Z Z^T)
 };
 
 // ---------------------------------------------------------------------------
@@ -360,6 +363,12 @@ template <typename W> struct Pipeline : PipelineBase {
   double flops = 0.0;
 
   Pipeline(int n_, int dtype_) : n(n_), dtype(dtype_), ncap(n_) {
+    // FP32 engines: the Pade products of their FP64 stages from 6 int8
+    // slices (same accuracy as FP64 MFMA, measured), the solve in FP64 MFMA
+    if (std::is_same<W, float>::value) {
+      oz_slices = 6;
+      oz_scope = 1;
+    }
     npcap = round_up(std::max(ncap, 1), 128);
     np = npcap;
     ld = npcap;
@@ -422,6 +431,10 @@ template <typename W> struct Pipeline : PipelineBase {
     dfree(flags);
     dfree(scal);
     dfree(info_buf);
+    for (int b = 0; b < 2; ++b) {
+      dfree(oz_s[b]);
+      dfree(oz_e[b]);
+    }
     if (sbuf) (void)hipFree(sbuf);
     for (auto& kv : sq_orders) (void)hipFree(kv.second);
     if (events)
@@ -458,6 +471,99 @@ template <typename W> struct Pipeline : PipelineBase {
       }
     have_result = false;
     if (hi) hi->set_size(n_new, st);
+  }
+
+  // ---- FP64 products from exact int8 slices (FP64 pipelines, structured path) ----
+  // oz_slices (PipelineBase) = 0: FP64 MFMA products; 4..6: the real
+  // symmetric products of the Pade stage and of the Cholesky solve (S,
+  // X = Z Z^T) run on the int8 MFMA from that many 7-bit slices per value
+  // (fexpm_ozaki.hpp).
+  int8_t* oz_s[2] = {nullptr, nullptr};
+  int* oz_e[2] = {nullptr, nullptr};
+  size_t oz_cap = 0;  // bytes per slice buffer
+  int oz_rows = 0;
+
+  // Operand row r = s1 * p1[r, 0:k1] followed by s2 * p2[r, 0:nk-k1] (leading dimension ldx).
+  struct OzOp {
+    const double* p1;
+    double s1;
+    const double* p2;
+    double s2;
+    int k1;
+    int64_t ldx;
+  };
+
+  void oz_reserve(int rows, int kp) {
+    const size_t need = (size_t)rows * kp * OZ_MAX_SLICES;
+    if (need <= oz_cap && rows <= oz_rows) return;
+    for (int b = 0; b < 2; ++b) {
+      dfree(oz_s[b]);
+      dfree(oz_e[b]);
+      oz_s[b] = nullptr;
+      oz_e[b] = nullptr;
+    }
+    oz_cap = 0;
+    oz_rows = 0;
+    for (int b = 0; b < 2; ++b) {
+      FX_HIP(hipMalloc(&oz_s[b], need));
+      FX_HIP(hipMalloc(&oz_e[b], (size_t)rows * sizeof(int)));
+    }
+    oz_cap = need;
+    oz_rows = rows;
+  }
+
+  template <int S>
+  void oz_slice_t(const OzOp& x, int nr, int nk, int rows, int kp, int buf, hipStream_t st) {
+    oz_slice<S><<<(unsigned)rows, 256, 0, st>>>(x.p1, x.s1, x.p2, x.s2, x.k1, x.ldx, nr, nk, rows, kp, oz_s[buf],
+                                                oz_e[buf]);
+    FX_LAUNCH();
+  }
+  void oz_slice_op(const OzOp& x, int nr, int nk, int rows, int kp, int buf, hipStream_t st) {
+    switch (oz_slices) {
+      case 4: oz_slice_t<4>(x, nr, nk, rows, kp, buf, st); break;
+      case 5: oz_slice_t<5>(x, nr, nk, rows, kp, buf, st); break;
+      default: oz_slice_t<6>(x, nr, nk, rows, kp, buf, st); break;
+    }
+  }
+  template <int S>
+  void oz_launch_t(int ba, int ra, int bb, int rb, int m, int nn, int kp, const Epilogue<double>& e, bool sym,
+                   hipStream_t st) {
+    OzArgs<S> g;
+    std::memset(&g, 0, sizeof(g));
+    g.a = oz_s[ba];
+    g.ea = oz_e[ba];
+    g.rows_a = ra;
+    g.b = oz_s[bb];
+    g.eb = oz_e[bb];
+    g.rows_b = rb;
+    g.m = m;
+    g.n = nn;
+    g.kpad = kp;
+    g.epi = e;
+    if (sym) {
+      const int T = ra / OZ_BM;
+      oz_gemm_kernel<S, true><<<dim3((unsigned)(T * (T + 1) / 2)), dim3(512), 0, st>>>(g);
+    } else {
+      oz_gemm_kernel<S, false><<<dim3((unsigned)(rb / OZ_BM), (unsigned)(ra / OZ_BM)), dim3(512), 0, st>>>(g);
+    }
+    FX_LAUNCH();
+  }
+  // C (epilogue e) = A B^T from slices (B null: B = A, sliced once).  sym:
+  // the product is symmetric (m == n), lower tiles computed and mirrored.
+  // Returns the FP64-equivalent flops of the product.
+  double oz_product(const OzOp& A, const OzOp* B, int m, int nn, int nk, const Epilogue<double>& e, bool sym,
+                    hipStream_t st) {
+    const int kp = (int)round_up(nk, OZ_BK), ra = (int)round_up(m, OZ_BM), rb = (int)round_up(nn, OZ_BM);
+    oz_reserve(std::max(ra, rb), kp);
+    oz_slice_op(A, m, nk, ra, kp, 0, st);
+    if (B) oz_slice_op(*B, nn, nk, rb, kp, 1, st);
+    const int bb = B ? 1 : 0, rbb = B ? rb : ra;
+    switch (oz_slices) {
+      case 4: oz_launch_t<4>(0, ra, bb, rbb, m, nn, kp, e, sym, st); break;
+      case 5: oz_launch_t<5>(0, ra, bb, rbb, m, nn, kp, e, sym, st); break;
+      default: oz_launch_t<6>(0, ra, bb, rbb, m, nn, kp, e, sym, st); break;
+    }
+    return (sym ? 1.0 : 2.0) * (double)m * nn * nk;
   }
 
   // ---- launch helpers ----
@@ -543,6 +649,14 @@ template <typename W> struct Pipeline : PipelineBase {
     }
     g.epi.nadd = k;
     g.epi.diag = dg;
+    if constexpr (std::is_same<W, double>::value) {
+      if (oz_slices > 0 && (oz_scope & 1)) {
+        const OzOp a{A.r, 1.0, nullptr, 0.0, (int)np, ld};
+        const OzOp b{B.r, 1.0, nullptr, 0.0, (int)np, ld};
+        flops += oz_product(a, A.r == B.r ? nullptr : &b, n, n, (int)np, g.epi, true, st);
+        return;
+      }
+    }
     gemm<MODE_REAL, true>(g, st);
   }
 
@@ -684,7 +798,18 @@ template <typename W> struct Pipeline : PipelineBase {
       FX_LAUNCH();
     }
     // S = Uq Uq^T + V V^T (both symmetric: products of commuting polynomials),
-    // over the padded order so that the padding rows and columns are zeros
+    // over the padded order so that the padding rows and columns are zeros.
+    // From int8 slices: one product of the concatenated rows [V Uq].
+    if (oz_slices > 0 && (oz_scope & 2)) {
+      Epilogue<double> e;
+      std::memset(&e, 0, sizeof(e));
+      e.cr = sm;
+      e.ldc = L;
+      e.ld_add = L;
+      e.alpha = 1.0;
+      const OzOp vu{zb, 1.0, zb + P, 1.0, kpad, L};
+      flops += oz_product(vu, nullptr, ip, ip, 2 * kpad, e, true, st);
+    } else {
     flops += dgemm_real(zb, L, zb, L, vv, L, ip, ip, kpad, 1.0, 0.0, true, st);
     {
       GemmArgs<double> g;
@@ -704,6 +829,7 @@ template <typename W> struct Pipeline : PipelineBase {
       g.epi.addc[0] = 1.0;
       g.epi.add_r[0] = vv;
       flops += gemm64<MODE_REAL, true>(g, st);
+    }
     }
     // Two levels: 64-wide inner panels update only the columns of their
     // CH_OUTER-wide outer block; the outer block then updates the trailing
@@ -735,10 +861,24 @@ template <typename W> struct Pipeline : PipelineBase {
         flops += dgemm_real(zb + K0, L, p2, L, zb + K1, L, 2 * ip, r2, kk, -1.0, 1.0, false, st);
       }
     }
-    // X = Z Z^T (3M, half tiles) over S and V V^T
+    // X = Z Z^T (3M, half tiles) over S and V V^T.  From int8 slices: the two
+    // real symmetric products Re X = [Zr Zi][Zr -Zi]^T, Im X = [Zr Zi][Zi Zr]^T.
     double* xr = lr + 2 * P;
     double* xi = lr + 3 * P;
-    {
+    if (oz_slices > 0 && (oz_scope & 2)) {
+      Epilogue<double> e;
+      std::memset(&e, 0, sizeof(e));
+      e.ldc = L;
+      e.ld_add = L;
+      e.alpha = 1.0;
+      const OzOp zz{zb, 1.0, zb + P, 1.0, kpad, L};
+      const OzOp zre{zb, 1.0, zb + P, -1.0, kpad, L};
+      const OzOp zim{zb + P, 1.0, zb, 1.0, kpad, L};
+      e.cr = xr;
+      flops += oz_product(zz, &zre, n, n, 2 * kpad, e, true, st);
+      e.cr = xi;
+      flops += oz_product(zz, &zim, n, n, 2 * kpad, e, true, st);
+    } else {
       GemmArgs<double> g;
       std::memset(&g, 0, sizeof(g));
       g.ar = zb;
@@ -1009,6 +1149,8 @@ template <typename W> struct Pipeline : PipelineBase {
       h->flops = 0.0;
       h->panel_coop = panel_coop;
       h->spd_solve = spd_solve;
+      h->oz_slices = oz_slices;
+      h->oz_scope = oz_scope;
       h->front_pade<Tin>(d_a, P.structured, P.m, P.mu_re, P.mu_im, scale, st);
       if (timing) FX_HIP(hipEventRecord(ev[2], st));
       const int dh = h->solve(P.structured, 0, st);
@@ -1060,6 +1202,8 @@ template <typename W> struct Pipeline : PipelineBase {
     S.pade_fp64 = (P.hybrid && !P.nonfinite) ? 1 : 0;
     S.kblock = kblock;
     S.precision = P.split16 ? precision : FEXPM_PREC_NATIVE;
+    S.fp64_slices = (P.hybrid && P.structured && !P.nonfinite) ? oz_slices : 0;
+    S.fp64_slices_scope = S.fp64_slices ? oz_scope : 0;
   }
 
   static void check_info(int info) {
@@ -1603,6 +1747,24 @@ int fexpm_set_option(fexpm_engine* e, int key, int64_t value) {
       return FEXPM_OK;
     case FEXPM_OPT_PADE_FP64:
       if (e->pipe) e->pipe->pade_fp64 = value ? 1 : 0;
+      return FEXPM_OK;
+    case FEXPM_OPT_FP64_SLICES_SCOPE:
+      if (value < 1 || value > 3) {
+        e->err = "scope must be 1 (Pade), 2 (solve) or 3 (both)";
+        return FEXPM_EINVAL;
+      }
+      if (e->pipe) e->pipe->oz_scope = (int)value;
+      return FEXPM_OK;
+    case FEXPM_OPT_FP64_SLICES:
+      if (value != 0 && (value < 4 || value > OZ_MAX_SLICES)) {
+        e->err = "int8 slices must be 0 or in [4, 6]";
+        return FEXPM_EINVAL;
+      }
+      if (value != 0 && e->dtype != FEXPM_COMPLEX64 && e->dtype != FEXPM_FLOAT32) {
+        e->err = "int8-slice FP64 products apply to the FP64 stages of complex64 / float32 engines";
+        return FEXPM_EINVAL;
+      }
+      if (e->pipe) e->pipe->oz_slices = (int)value;
       return FEXPM_OK;
     case FEXPM_OPT_FP64_SQUARINGS:
       if (value < 0 || value > 64) {
@@ -2344,6 +2506,106 @@ int fexpm_debug_plane(fexpm_engine* e, int which, double* re, double* im) {
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
+// FP64 product by int8 slices (testing ABI): C = A B^T on host data, and the
+// device time of the product kernel alone.
+namespace {
+template <typename W>
+__global__ void fill_pattern(W* p, size_t cnt, uint32_t seed);
+
+struct OzBufs {
+  double *da = nullptr, *db = nullptr, *dc = nullptr;
+  int8_t *sa = nullptr, *sb = nullptr;
+  int *ea = nullptr, *eb = nullptr;
+  ~OzBufs() {
+    for (void* p : {(void*)da, (void*)db, (void*)dc, (void*)sa, (void*)sb, (void*)ea, (void*)eb})
+      if (p) (void)hipFree(p);
+  }
+};
+
+template <int S>
+double oz_run_t(int sym, int m, int n, int k, const double* a, const double* b, double* c, int reps) {
+  const int kpad = (int)round_up(k, OZ_BK), ra = (int)round_up(m, OZ_BM), rb = (int)round_up(n, OZ_BM);
+  OzBufs B;
+  FX_HIP(hipMalloc(&B.da, (size_t)m * k * sizeof(double)));
+  FX_HIP(hipMalloc(&B.db, (size_t)n * k * sizeof(double)));
+  FX_HIP(hipMalloc(&B.dc, (size_t)m * n * sizeof(double)));
+  FX_HIP(hipMalloc(&B.sa, (size_t)ra * kpad * S));
+  FX_HIP(hipMalloc(&B.sb, (size_t)rb * kpad * S));
+  FX_HIP(hipMalloc(&B.ea, (size_t)ra * sizeof(int)));
+  FX_HIP(hipMalloc(&B.eb, (size_t)rb * sizeof(int)));
+  if (a) {
+    FX_HIP(hipMemcpy(B.da, a, (size_t)m * k * sizeof(double), hipMemcpyHostToDevice));
+    FX_HIP(hipMemcpy(B.db, b, (size_t)n * k * sizeof(double), hipMemcpyHostToDevice));
+  } else {
+    fill_pattern<double><<<(unsigned)(((size_t)m * k + 255) / 256), 256>>>(B.da, (size_t)m * k, 1u);
+    fill_pattern<double><<<(unsigned)(((size_t)n * k + 255) / 256), 256>>>(B.db, (size_t)n * k, 2u);
+    FX_LAUNCH();
+  }
+  oz_slice<S><<<ra, 256>>>(B.da, 1.0, nullptr, 0.0, k, k, m, k, ra, kpad, B.sa, B.ea);
+  oz_slice<S><<<rb, 256>>>(B.db, 1.0, nullptr, 0.0, k, k, n, k, rb, kpad, B.sb, B.eb);
+  FX_LAUNCH();
+  OzArgs<S> g;
+  std::memset(&g, 0, sizeof(g));
+  g.a = B.sa;
+  g.ea = B.ea;
+  g.rows_a = ra;
+  g.b = sym ? B.sa : B.sb;
+  g.eb = sym ? B.ea : B.eb;
+  g.rows_b = sym ? ra : rb;
+  g.m = m;
+  g.n = n;
+  g.kpad = kpad;
+  g.epi.cr = B.dc;
+  g.epi.ldc = n;
+  g.epi.ld_add = n;
+  g.epi.alpha = 1.0;
+  auto launch = [&]() {
+    if (sym) {
+      const int T = ra / OZ_BM;
+      oz_gemm_kernel<S, true><<<dim3(T * (T + 1) / 2), dim3(512)>>>(g);
+    } else {
+      oz_gemm_kernel<S, false><<<dim3(rb / OZ_BM, ra / OZ_BM), dim3(512)>>>(g);
+    }
+    FX_LAUNCH();
+  };
+  launch();
+  double ms = 0.0;
+  if (reps > 0) {
+    hipEvent_t e0, e1;
+    FX_HIP(hipEventCreate(&e0));
+    FX_HIP(hipEventCreate(&e1));
+    FX_HIP(hipEventRecord(e0));
+    for (int r = 0; r < reps; ++r) launch();
+    FX_HIP(hipEventRecord(e1));
+    FX_HIP(hipEventSynchronize(e1));
+    float t = 0.f;
+    FX_HIP(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ms = (double)t / reps;
+  }
+  if (c) FX_HIP(hipMemcpy(c, B.dc, (size_t)m * n * sizeof(double), hipMemcpyDeviceToHost));
+  FX_HIP(hipDeviceSynchronize());
+  return ms;
+}
+}  // namespace
+
+extern "C" int fexpm_test_ozaki_gemm(int slices, int sym, int m, int n, int k, const double* a, const double* b,
+                                     double* c) {
+  if (!a || !b || !c || m <= 0 || n <= 0 || k <= 0 || (sym && m != n) || slices < 3 || slices > OZ_MAX_SLICES)
+    return FEXPM_EINVAL;
+  return guarded(nullptr, [&] {
+    check_device();
+    switch (slices) {
+      case 3: oz_run_t<3>(sym, m, n, k, a, b, c, 0); break;
+      case 4: oz_run_t<4>(sym, m, n, k, a, b, c, 0); break;
+      case 5: oz_run_t<5>(sym, m, n, k, a, b, c, 0); break;
+      default: oz_run_t<6>(sym, m, n, k, a, b, c, 0); break;
+    }
+  });
+}
+
+// ---------------------------------------------------------------------------
 // Grouped program description on a virtual layout (testing ABI): the same
 // builder as the batch runs, no device memory touched.
 namespace {
@@ -2685,7 +2947,12 @@ extern "C" int fexpm_test_gemm_time(int wtype, int mode, int sym, int cfg, int m
   return guarded(nullptr, [&] {
     check_device();
     double t = -1.0;
-    if (wtype >= 2) {
+    if (wtype == 4) {
+      // FP64 product by int8 slices, cfg = number of slices (operands from a fill pattern)
+      if (cfg == 5) t = oz_run_t<5>(sym, m, n, k, nullptr, nullptr, nullptr, reps);
+      else if (cfg == 6) t = oz_run_t<6>(sym, m, n, k, nullptr, nullptr, nullptr, reps);
+      else if (cfg == 4) t = oz_run_t<4>(sym, m, n, k, nullptr, nullptr, nullptr, reps);
+    } else if (wtype >= 2) {
       // cfg % 10: 0 = 8 waves 2x4, 1 = 4 waves 2x2, 2 = 8 waves 4x2, 3 = 16 waves 4x4 (fp16x2);
       // cfg / 10: tile group of the order table (0 = row-major triangle)
       const int lay = cfg % 10, grp = cfg / 10;
@@ -2700,7 +2967,16 @@ extern "C" int fexpm_test_gemm_time(int wtype, int mode, int sym, int cfg, int m
         else if (lay == 2) t = time_split_t<Bf16x3, 4, 2>(m, k, reps, grp);
       }
     } else if (wtype == 0) {
+      // squaring tilings: 0 default, 4 FCfgSmall, 5..8 LDS padding / wave-tile variants of the default
       if (mode == 2 && sym && cfg == 4) t = time_gemm_t<float, MODE_C3M, true, true, FCfgSmall>(m, n, k, reps);
+      else if (mode == 2 && sym && cfg == 5)
+        t = time_gemm_t<float, MODE_C3M, true, true, TileCfg<128, 128, 16, 8, 64, 32>>(m, n, k, reps);
+      else if (mode == 2 && sym && cfg == 6)
+        t = time_gemm_t<float, MODE_C3M, true, true, TileCfg<128, 128, 16, 4, 32, 64>>(m, n, k, reps);
+      else if (mode == 2 && sym && cfg == 7)
+        t = time_gemm_t<float, MODE_C3M, true, true, TileCfg<128, 128, 16, 12, 64, 32>>(m, n, k, reps);
+      else if (mode == 2 && sym && cfg == 8)
+        t = time_gemm_t<float, MODE_C3M, true, true, TileCfg<128, 128, 16, 8, 32, 64>>(m, n, k, reps);
       else if (mode == 2 && sym) t = time_gemm_t<float, MODE_C3M, true, true, GemmCfg<float>>(m, n, k, reps);
       else if (mode == 2) t = time_gemm_t<float, MODE_C3M, false, true, GemmCfg<float>>(m, n, k, reps);
       else if (mode == 0 && sym) t = time_gemm_t<float, MODE_REAL, true, false, GemmCfg<float>>(m, n, k, reps);
@@ -2718,6 +2994,7 @@ extern "C" int fexpm_test_gemm_time(int wtype, int mode, int sym, int cfg, int m
       else if (cfg == 1) FX_TIME_D(DCfg8w);
       else if (cfg == 2) FX_TIME_D(DCfgTall);
       else if (cfg == 3) FX_TIME_D(DCfgWide);
+      else if (cfg == 4) FX_TIME_D(DCfg8wPad4);
 #undef FX_TIME_D
     }
     if (t < 0) throw Error(FEXPM_EINVAL, "unsupported gemm timing variant");

@@ -118,6 +118,7 @@ using DCfg4w = TileCfg<64, 64, 16, 2, 32, 32>;     // 4 waves, 32x32 wave tiles
 using DCfg8w = TileCfg<64, 64, 16, 2, 32, 16>;     // 8 waves, 32x16 wave tiles
 using DCfgTall = TileCfg<128, 64, 16, 2, 32, 32>;  // 8 waves, 128x64 workgroup tile
 using DCfgWide = TileCfg<64, 128, 16, 2, 32, 32>;  // 8 waves, 64x128 workgroup tile
+using DCfg8wPad4 = TileCfg<64, 64, 16, 4, 32, 16>;  // the default with 4-double LDS row padding
 
 // Staging of a ROWS x BK tile through registers, NT threads.
 template <typename T, int ROWS, int BK, int PAD, int NT> struct TileStage {

@@ -64,6 +64,8 @@ static py::dict stats_dict(const fexpm_stats& s) {
   d["kblock"] = s.kblock;
   d["pade_fp64"] = (bool)s.pade_fp64;
   d["precision"] = s.precision == FEXPM_PREC_FP16X2 ? "fp16x2" : (s.precision == FEXPM_PREC_BF16X3 ? "bf16x3" : "native");
+  d["fp64_slices"] = s.fp64_slices;
+  d["fp64_slices_scope"] = s.fp64_slices_scope;
   d["ms_propagate"] = s.ms_propagate;
   return d;
 }
@@ -415,9 +417,11 @@ PYBIND11_MODULE(_fastexpm_core, m) {
   m.attr("OPT_SPD_SOLVE") = (int)FEXPM_OPT_SPD_SOLVE;
   m.attr("OPT_MAX_SQUARINGS") = (int)FEXPM_OPT_MAX_SQUARINGS;
   m.attr("OPT_GROUP_CANDIDATE") = (int)FEXPM_OPT_GROUP_CANDIDATE;
+  m.attr("OPT_FP64_SLICES_SCOPE") = (int)FEXPM_OPT_FP64_SLICES_SCOPE;
   m.attr("OPT_BATCH_GROUPED") = (int)FEXPM_OPT_BATCH_GROUPED;
   m.attr("OPT_AUTOTUNE") = (int)FEXPM_OPT_AUTOTUNE;
   m.attr("OPT_GRAPH") = (int)FEXPM_OPT_GRAPH;
+  m.attr("OPT_FP64_SLICES") = (int)FEXPM_OPT_FP64_SLICES;
   m.attr("PREC_NATIVE") = (int)FEXPM_PREC_NATIVE;
   m.attr("PREC_FP16X2") = (int)FEXPM_PREC_FP16X2;
   m.attr("PREC_BF16X3") = (int)FEXPM_PREC_BF16X3;

@@ -72,8 +72,12 @@ enum fexpm_option {
                                     0 = concurrent lanes */
   FEXPM_OPT_AUTOTUNE = 9,        /* grouped batches: 1 = time the tile-shape candidates on the first
                                     run of a new layout and keep the fastest (default), 0 = default tiles */
-  FEXPM_OPT_GRAPH = 10           /* grouped batches: 1 = capture the program in a hipGraph and replay it
+  FEXPM_OPT_GRAPH = 10,          /* grouped batches: 1 = capture the program in a hipGraph and replay it
                                     while the plans stay the same (default), 0 = eager launches */
+  FEXPM_OPT_FP64_SLICES = 11     /* complex64 / float32 engines, structured input: the symmetric FP64
+                                    Pade products from S exact int8 slices per value (S = 4..6, about
+                                    7S - 1 bits of each row's largest value) on the int8 MFMA with int32
+                                    accumulation; default 6 (the solve stays on FP64 MFMA); 0 = FP64 MFMA */
 };
 
 /* Per-run report (filled by fexpm_get_stats after a run). */
@@ -100,6 +104,8 @@ typedef struct fexpm_stats {
   int32_t pade_fp64;    /* 1: Pade products and solve ran in FP64 */
   double ms_propagate;  /* device time of the last fexpm_propagate (ms) */
   int32_t precision;    /* enum fexpm_precision used by the squarings of this run */
+  int32_t fp64_slices;  /* int8 slices per value of the FP64 products (0 = FP64 MFMA) */
+  int32_t fp64_slices_scope;  /* stages using them: 1 Pade products, 2 solve, 3 both */
 } fexpm_stats;
 
 /* Create an engine for n x n matrices of the given dtype and allocate its

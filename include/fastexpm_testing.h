@@ -29,6 +29,11 @@ enum { FEXPM_OPT_MAX_SQUARINGS = 100 };
  * no tuning; -1 restores the autotuner.  Lets tests check every candidate. */
 enum { FEXPM_OPT_GROUP_CANDIDATE = 101 };
 
+/* Engine option: stages whose symmetric FP64 products use the int8 slices
+ * of FEXPM_OPT_FP64_SLICES: 1 the Pade products, 2 the Cholesky solve
+ * (S, X = Z Z^T), 3 both (default).  For precision studies. */
+enum { FEXPM_OPT_FP64_SLICES_SCOPE = 102 };
+
 /* Internal planes readable by fexpm_debug_plane (working precision, n x n). */
 enum fexpm_plane {
   FEXPM_PLANE_A = 0,   /* (A - mu I) / 2^s; structured: its imaginary part G */
@@ -66,6 +71,13 @@ int fexpm_debug_plane(fexpm_engine* e, int which, double* re, double* im);
  * 2 and 3 are not square and are invalid with sym).  wtype 2 / 3 time the
  * split-precision squaring kernel alone (mode 2, sym). */
 int fexpm_test_gemm_time(int wtype, int mode, int sym, int cfg, int m, int n, int k, int reps, double* ms);
+
+/* C = A B^T (FP64, host row-major A m x k, B n x k, C m x n) computed from
+ * exact int8 slices of the rows of A and B (`slices` in 3..6) with int32
+ * accumulation on the int8 MFMA and FP64 combination; with sym, B must be A
+ * and only lower tiles are computed (mirrored).  wtype 4 of
+ * fexpm_test_gemm_time times this product kernel (cfg = slices). */
+int fexpm_test_ozaki_gemm(int slices, int sym, int m, int n, int k, const double* a, const double* b, double* c);
 
 /* reps + 1 launches of a plain device copy of `bytes` (multiple of 16);
  * average ms of the last reps.  Reference traffic for the profiler's

@@ -119,6 +119,33 @@ def test_gemm_split_squaring(lib, wtype, shape):
     assert np.array_equal(c, c.T)
 
 
+@pytest.mark.parametrize("slices", [4, 5, 6])
+@pytest.mark.parametrize("shape,sym", [((300, 200, 250), False), ((129, 129, 1000), True), ((64, 64, 64), True),
+                                       ((1, 70, 33), False)])
+def test_gemm_int8_slices(lib, slices, shape, sym):
+    """FP64 product from exact int8 row slices: every element within the
+    slicing bound (S + 2) 2^-7S k 2^(e_i + f_j), with 2^e_i < 2 max_k |a_ik|;
+    rows of very different scale keep their own precision."""
+    m, n, k = shape
+    _say(f"int8-slice product S={slices} m={m} n={n} k={k} sym={sym}")
+    rng = np.random.default_rng(m + 7 * n + k + slices)
+    a = rng.standard_normal((m, k)) * np.exp2(rng.integers(-30, 30, size=(m, 1)))
+    a[0, :] = 0.0  # a zero row
+    b = a.copy() if sym else rng.standard_normal((n, k)) * np.exp2(rng.integers(-30, 30, size=(n, 1)))
+    c = np.zeros((m, n))
+    rc = lib.fexpm_test_ozaki_gemm(slices, int(sym), m, n, k, _p(np.ascontiguousarray(a)),
+                                   _p(np.ascontiguousarray(b)), _p(c))
+    assert rc == 0, lib.fexpm_last_error(None)
+    ref = a @ b.T
+    bound = (slices + 2) * 2.0 ** (-7 * slices) * k * 4.0 * np.outer(np.abs(a).max(1), np.abs(b).max(1))
+    err = np.abs(c - ref)
+    worst = float((err / np.maximum(bound, 1e-300)).max())
+    print(f"  worst error / bound = {worst:.3e}", flush=True)
+    assert np.all(err <= bound), worst
+    if sym:
+        assert np.array_equal(c, c.T)
+
+
 def test_gemm_split_range_error(lib):
     """fp16 parts hold |x| * 2^14 <= 65504: a larger operand is reported."""
     _say("split squaring range error")

@@ -135,6 +135,50 @@ def test_split_precision_ctqw(fx, prec, n):
     assert infidelity(ref @ psi0, u.astype(np.complex128) @ psi0) < 1e-9
 
 
+@pytest.mark.parametrize("slices,scope", [(6, 1), (5, 1), (6, 3), (5, 3)])
+@pytest.mark.parametrize("n", [64, 300, 700])
+def test_fp64_slice_products_ctqw(fx, slices, scope, n):
+    """Complex64 engine with symmetric FP64 products from int8 slices (the
+    Pade products by default, scope 3 also S and X = Z Z^T of the solve): U
+    against the eigen oracle within the complex64 tolerance, and close to
+    the U of the FP64-MFMA products."""
+    from fastctqw_amd.fastexpm import _fastexpm_core as core
+
+    _say(f"fp64 slices {slices} scope {scope} n={n}")
+    L = laplacian(erdos_renyi(n, 0.5, seed=40 + n, weighted=True))
+    a = (-1j * 4.5 * L).astype(np.complex64)
+    lam, V = np.linalg.eigh(np.ascontiguousarray(a.imag, dtype=np.float64))
+    ref = (V * np.exp(1j * lam)[None, :]) @ V.T
+    with fx.ExpMatComplex64(n) as e:
+        e.run(a)
+        assert e.stats()["fp64_slices"] == 6 and e.stats()["fp64_slices_scope"] == 1  # the default
+        e._backend.set_option(core.OPT_FP64_SLICES, 0)
+        u64 = e.run(a)
+        assert e.stats()["fp64_slices"] == 0
+        e._backend.set_option(core.OPT_FP64_SLICES, slices)
+        e._backend.set_option(core.OPT_FP64_SLICES_SCOPE, scope)
+        u8 = e.run(a)
+        st = e.stats()
+        assert st["fp64_slices"] == slices and st["fp64_slices_scope"] == scope
+    err = rel_fro(ref, u8)
+    print(f"  s={st['s']} relF={err:.3e} vs fp64 path {rel_fro(u64, u8):.3e}", flush=True)
+    assert err <= _tol(np.complex64, st["s"])
+    assert rel_fro(u64, u8) <= _tol(np.complex64, st["s"])
+    psi_ref = ref[:, 0]
+    assert infidelity(psi_ref, u8[:, 0].astype(np.complex128)) < 1e-9
+
+
+def test_fp64_slices_option_errors(fx):
+    from fastctqw_amd.fastexpm import _fastexpm_core as core
+
+    with fx.ExpMatComplex128(8) as e:
+        with pytest.raises(ValueError):
+            e._backend.set_option(core.OPT_FP64_SLICES, 5)
+    with fx.ExpMatComplex64(8) as e:
+        with pytest.raises(ValueError):
+            e._backend.set_option(core.OPT_FP64_SLICES, 9)
+
+
 def test_split_precision_options(fx):
     _say("split precision options")
     with fx.ExpMatComplex128(8) as e:
