@@ -48,12 +48,12 @@ __device__ __forceinline__ double oz_elem(const double* __restrict__ x1, double 
 }
 
 template <int S>
-__global__ __launch_bounds__(256) void oz_slice(const double* __restrict__ x1, double s1,
-                                                const double* __restrict__ x2, double s2, int k1, int64_t ld, int nr,
-                                                int nk, int rows, int kpad, int8_t* __restrict__ out,
-                                                int* __restrict__ ex) {
+__device__ __forceinline__ void oz_slice_row(const double* __restrict__ x1, double s1,
+                                             const double* __restrict__ x2, double s2, int k1, int64_t ld, int nr,
+                                             int nk, int rows, int kpad, int8_t* __restrict__ out,
+                                             int* __restrict__ ex, int r) {
   __shared__ double red[256];
-  const int r = blockIdx.x, t = threadIdx.x;
+  const int t = threadIdx.x;
   double mx = 0.0;
   if (r < nr)
     for (int k = t; k < nk; k += 256) mx = fmax(mx, fabs(oz_elem(x1, s1, x2, s2, k1, ld, r, k)));
@@ -67,35 +67,37 @@ __global__ __launch_bounds__(256) void oz_slice(const double* __restrict__ x1, d
   if (red[0] > 0.0) (void)frexp(red[0], &e);  // red[0] = f 2^e, f in [0.5, 1): |x| < 2^e
   if (t == 0) ex[r] = e;
   const double sc = ldexp(1.0, 6 - e);  // first slice: |x 2^-e| < 1, times 2^6
-  for (int kb = t; kb * OZ_BK < kpad; kb += 256) {
-    int8_t part[S][OZ_BK];
-#pragma unroll
-    for (int c = 0; c < OZ_BK; ++c) {
-      const int k = kb * OZ_BK + c;
-      double y = (r < nr && k < nk) ? oz_elem(x1, s1, x2, s2, k1, ld, r, k) * sc : 0.0;
-#pragma unroll
-      for (int p = 0; p < S; ++p) {
-        const double a = rint(y);  // |y| <= 64, so a fits; the rest is exact
-        part[p][c] = (int8_t)a;
-        y = (y - a) * 128.0;
-      }
-    }
-    int8_t* dst = out + ((int64_t)kb * rows + r) * S * OZ_BK;
+  // 256 consecutive values per pass: thread t reads value t (coalesced),
+  // its S slice bytes go to LDS, and 16-byte runs of every slice are
+  // written out (8 k-blocks x S slices x 2 halves = 16 S chunks)
+  __shared__ __attribute__((aligned(16))) int8_t buf[S][256];
+  for (int k0 = 0; k0 < kpad; k0 += 256) {
+    const int k = k0 + t;
+    double y = (r < nr && k < nk) ? oz_elem(x1, s1, x2, s2, k1, ld, r, k) * sc : 0.0;
 #pragma unroll
     for (int p = 0; p < S; ++p) {
-      i32x4 lo, hi;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        lo[q] = (int)(uint8_t)part[p][4 * q] | ((int)(uint8_t)part[p][4 * q + 1] << 8) |
-                ((int)(uint8_t)part[p][4 * q + 2] << 16) | ((int)(uint8_t)part[p][4 * q + 3] << 24);
-        hi[q] = (int)(uint8_t)part[p][16 + 4 * q] | ((int)(uint8_t)part[p][16 + 4 * q + 1] << 8) |
-                ((int)(uint8_t)part[p][16 + 4 * q + 2] << 16) | ((int)(uint8_t)part[p][16 + 4 * q + 3] << 24);
-      }
-      i32x4* d = reinterpret_cast<i32x4*>(dst + p * OZ_BK);
-      d[0] = lo;
-      d[1] = hi;
+      const double a = rint(y);  // |y| <= 64, so a fits; the rest is exact
+      buf[p][t] = (int8_t)a;
+      y = (y - a) * 128.0;
     }
+    __syncthreads();
+    if (t < 16 * S) {
+      const int kbl = t / (2 * S), w = t % (2 * S), p = w >> 1, h = w & 1;
+      const int kb = k0 / OZ_BK + kbl;
+      if (kb * OZ_BK < kpad)
+        *reinterpret_cast<i32x4*>(out + ((int64_t)kb * rows + r) * S * OZ_BK + p * OZ_BK + 16 * h) =
+            *reinterpret_cast<const i32x4*>(&buf[p][kbl * OZ_BK + 16 * h]);
+    }
+    __syncthreads();
   }
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void oz_slice(const double* __restrict__ x1, double s1,
+                                                const double* __restrict__ x2, double s2, int k1, int64_t ld, int nr,
+                                                int nk, int rows, int kpad, int8_t* __restrict__ out,
+                                                int* __restrict__ ex) {
+  oz_slice_row<S>(x1, s1, x2, s2, k1, ld, nr, nk, rows, kpad, out, ex, (int)blockIdx.x);
 }
 
 template <int S> struct OzArgs {
@@ -249,6 +251,40 @@ __global__ __launch_bounds__(512) void oz_gemm_kernel(OzArgs<S> g) {
     tn = lin % nbn;
   }
   oz_tile<S>(g, tm, tn, SYM);
+}
+
+// Grouped form: many products (one or more per walk of a batch) in one
+// persistent launch, with the tile placement of gemm_group_kernel.
+template <int S> struct GroupOz {
+  OzArgs<S> g;
+  int tile0;
+  int tiles;
+  int tiles_n;
+  int sym;
+};
+
+template <int S>
+__global__ __launch_bounds__(512) void oz_group_kernel(const GroupOz<S>* __restrict__ probs, int nprob, int total,
+                                                       int chunk) {
+  for (int b = (int)blockIdx.x; b < total; b += (int)gridDim.x) {
+    const int t = xcd_chunk_remap(b, total, chunk);
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (probs[mid].tile0 <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    const GroupOz<S>& P = probs[lo];
+    const int local = t - P.tile0;
+    int tm, tn;
+    if (P.sym) {
+      tri_tile(local, tm, tn);
+    } else {
+      tm = local / P.tiles_n;
+      tn = local % P.tiles_n;
+    }
+    oz_tile<S>(P.g, tm, tn, P.sym != 0);
+  }
 }
 
 }  // namespace fexpm
