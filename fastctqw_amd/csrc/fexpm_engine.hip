@@ -1965,6 +1965,18 @@ int fexpm_batch_set_option(fexpm_batch* b, int key, int64_t value) {
     case FEXPM_OPT_GRAPH:
       if (b->grp) b->grp->use_graph = value ? 1 : 0;
       return FEXPM_OK;
+    case FEXPM_OPT_FP64_SLICES:
+      if (value != 0 && value != GROUP_OZ_SLICES) {
+        b->err = "batches: int8 slices must be 0 or 6";
+        return FEXPM_EINVAL;
+      }
+      if (value != 0 && b->dtype != FEXPM_COMPLEX64) {
+        b->err = "int8-slice FP64 products apply to complex64 batches";
+        return FEXPM_EINVAL;
+      }
+      for (auto& p : b->pipes) p->oz_slices = (int)value;
+      if (b->grp) b->grp->oz_slices = (int)value;
+      return FEXPM_OK;
     case FEXPM_OPT_GROUP_CANDIDATE:
       if (value < -1 || value >= GROUP_CANDIDATES) {
         b->err = "candidate must be in [-1, 5]";
@@ -2124,6 +2136,8 @@ static void batch_run_grouped(fexpm_batch* b, const void* psi0, void* psi) {
     S.kblock = G.kblock;
     S.pade_fp64 = b->dtype == FEXPM_COMPLEX64 ? 1 : 0;
     S.precision = b->dtype == FEXPM_COMPLEX64 ? G.precision : FEXPM_PREC_NATIVE;
+    S.fp64_slices = b->dtype == FEXPM_COMPLEX64 ? G.oz_slices : 0;
+    S.fp64_slices_scope = S.fp64_slices ? 1 : 0;
   }
   if (out_of_range >= 0)
     throw Error(FEXPM_EINVAL, "walk " + std::to_string(out_of_range) +
@@ -2610,10 +2624,12 @@ extern "C" int fexpm_test_ozaki_gemm(int slices, int sym, int m, int n, int k, c
 // builder as the batch runs, no device memory touched.
 namespace {
 template <typename Tin, typename W>
-std::string group_program_text(int precision, int64_t count, const int64_t* n, const int32_t* m, const int32_t* s,
-                               const int32_t* shifted, const double* mu_re, const double* mu_im, int with_psi) {
+std::string group_program_text(int precision, int fp64_slices, int64_t count, const int64_t* n, const int32_t* m,
+                               const int32_t* s, const int32_t* shifted, const double* mu_re, const double* mu_im,
+                               int with_psi) {
   GroupRunner<Tin, W> R(true);
   R.precision = precision;
+  if (fp64_slices >= 0) R.oz_slices = fp64_slices;
   std::vector<int64_t> nv(n, n + count);
   std::vector<size_t> off((size_t)count), voff((size_t)count);
   size_t tot = 0, vt = 0;
@@ -2645,20 +2661,22 @@ std::string group_program_text(int precision, int64_t count, const int64_t* n, c
 }
 }  // namespace
 
-extern "C" int fexpm_test_group_program(int dtype, int precision, int64_t count, const int64_t* n, const int32_t* m,
-                                        const int32_t* s, const int32_t* shifted, const double* mu_re,
-                                        const double* mu_im, int with_psi, char* buf, int64_t cap, int64_t* len) {
+extern "C" int fexpm_test_group_program(int dtype, int precision, int fp64_slices, int64_t count, const int64_t* n,
+                                        const int32_t* m, const int32_t* s, const int32_t* shifted,
+                                        const double* mu_re, const double* mu_im, int with_psi, char* buf,
+                                        int64_t cap, int64_t* len) {
   if (count < 1 || !n || !m || !s || !shifted || !mu_re || !mu_im || !len) return FEXPM_EINVAL;
   if (dtype != FEXPM_COMPLEX64 && dtype != FEXPM_COMPLEX128) return FEXPM_EINVAL;
   if (precision < FEXPM_PREC_NATIVE || precision > FEXPM_PREC_BF16X3 ||
-      (precision != FEXPM_PREC_NATIVE && dtype != FEXPM_COMPLEX64))
+      (precision != FEXPM_PREC_NATIVE && dtype != FEXPM_COMPLEX64) ||
+      (fp64_slices != -1 && fp64_slices != 0 && (fp64_slices != GROUP_OZ_SLICES || dtype != FEXPM_COMPLEX64)))
     return FEXPM_EINVAL;
   try {
     const std::string t = dtype == FEXPM_COMPLEX64
-                              ? group_program_text<float2, float>(precision, count, n, m, s, shifted, mu_re, mu_im,
-                                                                  with_psi)
-                              : group_program_text<double2, double>(precision, count, n, m, s, shifted, mu_re,
-                                                                    mu_im, with_psi);
+                              ? group_program_text<float2, float>(precision, fp64_slices, count, n, m, s, shifted,
+                                                                  mu_re, mu_im, with_psi)
+                              : group_program_text<double2, double>(precision, fp64_slices, count, n, m, s, shifted,
+                                                                    mu_re, mu_im, with_psi);
     *len = (int64_t)t.size();
     if (!buf || cap < (int64_t)t.size() + 1) return FEXPM_EINVAL;
     std::memcpy(buf, t.c_str(), t.size() + 1);

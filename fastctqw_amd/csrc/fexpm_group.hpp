@@ -297,6 +297,28 @@ template <class P> struct OpSplit {
   }
 };
 
+// Exact int8 slices of one FP64 operand (rows of a walk's plane, optionally
+// the concatenation of two planes), one workgroup per row.
+constexpr int GROUP_OZ_SLICES = 6;
+struct OpOzSlice {
+  static constexpr int NT = 256;
+  struct Prob {
+    const double* x1;
+    double s1;
+    const double* x2;
+    double s2;
+    int k1;
+    int64_t ld;
+    int nr, nk, rows, kpad;
+    int8_t* out;
+    int* ex;
+  };
+  static int blocks(const Prob& p) { return p.rows; }
+  static __device__ __forceinline__ void run(const Prob& p, int r) {
+    oz_slice_row<GROUP_OZ_SLICES>(p.x1, p.s1, p.x2, p.s2, p.k1, p.ld, p.nr, p.nk, p.rows, p.kpad, p.out, p.ex, r);
+  }
+};
+
 // ---------------------------------------------------------------------------
 // Host side
 
@@ -315,10 +337,14 @@ enum GroupKind {
   GK_SPLIT_F16,   // split of the squaring operand (fp16x2 parts)
   GK_SPLIT_BF16,  // (bf16x3 parts)
   GK_SQ_F16,      // split squaring on 16-bit MFMA, tiles of SQ_BM (GEMM-like stage)
-  GK_SQ_BF16
+  GK_SQ_BF16,
+  GK_OZSLICE,     // int8 slices of FP64 operands
+  GK_OZGEMM       // FP64 products from int8 slices, tiles of OZ_BM (GEMM-like stage)
 };
 
-static inline bool gk_tiled(int kind) { return kind == GK_GEMM || kind == GK_SQ_F16 || kind == GK_SQ_BF16; }
+static inline bool gk_tiled(int kind) {
+  return kind == GK_GEMM || kind == GK_SQ_F16 || kind == GK_SQ_BF16 || kind == GK_OZGEMM;
+}
 
 // Tuning classes of GEMM launches.
 enum GroupClass { GC_SQUARE = 0, GC_SYM64 = 1, GC_SOLVE = 2, GC_COUNT = 3 };
@@ -444,6 +470,48 @@ struct GroupProgram {
     std::memcpy(host.data() + L.off_probs, buf.data(), bytes);
   }
 
+  // FP64 products from int8 slices: problems are OzArgs, tiles of OZ_BM.
+  void materialize_oz(const GroupStage& S, GroupLaunch& L, std::vector<unsigned char>& host, const int* cand) {
+    using A = OzArgs<GROUP_OZ_SLICES>;
+    const int np_ = S.count();
+    L.cfg = 0;
+    L.dispatch = cand_dispatch(cand[S.cls]);
+    std::vector<int> order((size_t)np_);
+    for (int p = 0; p < np_; ++p) order[p] = p;
+    auto arg = [&](int p) -> const A& { return *reinterpret_cast<const A*>(S.probs.data() + (size_t)p * S.psize); };
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return arg(x).kpad > arg(y).kpad; });
+    std::vector<GroupOz<GROUP_OZ_SLICES>> buf((size_t)np_);
+    int tot = 0;
+    for (int q = 0; q < np_; ++q) {
+      const int p = order[q];
+      const A& g = arg(p);
+      const bool sy = S.sym[p] != 0;
+      if (g.m <= 0 || g.n <= 0 || g.kpad <= 0 || g.kpad % OZ_BK != 0 || g.rows_a < g.m || g.rows_b < g.n ||
+          g.rows_a % OZ_BM != 0 || g.rows_b % OZ_BM != 0 || (sy && g.m != g.n))
+        throw Error(FEXPM_EINVAL, "grouped int8-slice product: bad problem shape");
+      const int tm = (g.m + OZ_BM - 1) / OZ_BM, tn = (g.n + OZ_BM - 1) / OZ_BM;
+      const int tiles = sy ? tm * (tm + 1) / 2 : tm * tn;
+      walk_flops[(size_t)S.walk[p]] += 2.0 * tiles * (double)OZ_BM * OZ_BM * (double)g.kpad;  // FP64-equivalent
+      std::memset(&buf[q], 0, sizeof(buf[q]));
+      buf[q].g = g;
+      buf[q].tile0 = tot;
+      buf[q].tiles = tiles;
+      buf[q].tiles_n = tn;
+      buf[q].sym = sy ? 1 : 0;
+      tot += tiles;
+    }
+    L.total = tot;
+    L.chunk = L.dispatch == 0 ? 1 : 4;
+    int capw = tot;
+    if (L.dispatch == 2) capw = std::max(8, group_capacity(oz_group_kernel<GROUP_OZ_SLICES>, 512) / 8 * 8);
+    L.grid = std::min(tot, capw);
+    host.resize((host.size() + 255) / 256 * 256);
+    L.off_probs = host.size();
+    const size_t bytes = buf.size() * sizeof(buf[0]);
+    host.resize(host.size() + bytes);
+    std::memcpy(host.data() + L.off_probs, buf.data(), bytes);
+  }
+
   void materialize(const std::deque<GroupStage>& stages, const int* cand, int nwalks) {
     std::vector<unsigned char> host;
     auto align = [&]() { host.resize((host.size() + 255) / 256 * 256); };
@@ -460,7 +528,9 @@ struct GroupProgram {
       L.f32 = S.f32;
       L.nprob = np_;
       L.chunk = 1;
-      if (S.kind == GK_SQ_F16) {
+      if (S.kind == GK_OZGEMM) {
+        materialize_oz(S, L, host, cand);
+      } else if (S.kind == GK_SQ_F16) {
         materialize_sq<Fp16x2>(S, L, host, cand);
       } else if (S.kind == GK_SQ_BF16) {
         materialize_sq<Bf16x3>(S, L, host, cand);
@@ -589,6 +659,11 @@ static void group_enqueue_one(const GroupLaunch& L, unsigned char* base, hipStre
     case GK_OUTPUT: FX_OP(OutOp)
     case GK_APPLY: FX_OP(OpApply<W>)
     case GK_CHOL: FX_OP(OpChol)
+    case GK_OZSLICE: FX_OP(OpOzSlice)
+    case GK_OZGEMM:
+      oz_group_kernel<GROUP_OZ_SLICES><<<grid, 512, 0, st>>>(
+          reinterpret_cast<const GroupOz<GROUP_OZ_SLICES>*>(pp), L.nprob, L.total, L.chunk);
+      break;
     case GK_SPLIT_F16: FX_OP(OpSplit<Fp16x2>)
     case GK_SPLIT_BF16: FX_OP(OpSplit<Bf16x3>)
     case GK_SQ_F16:
@@ -631,7 +706,7 @@ struct GroupPlan {
 };
 
 // Virtual address regions of a host-only program description.
-enum VRegion { VR_F64 = 1, VR_W, VR_SMALL, VR_SPLIT, VR_IN, VR_OUT, VR_PSI0, VR_PSI, VR_SCAL };
+enum VRegion { VR_F64 = 1, VR_W, VR_SMALL, VR_SPLIT, VR_IN, VR_OUT, VR_PSI0, VR_PSI, VR_SCAL, VR_OZ };
 constexpr int VR_SHIFT = 40;
 inline void* vaddr(int region) { return reinterpret_cast<void*>((uintptr_t)region << VR_SHIFT); }
 
@@ -739,6 +814,27 @@ template <typename Tin, typename W> static std::string describe_stage(const Grou
         break;
       }
       case GK_GEMM: l = S.f32 ? describe_gemm<float>(S, p) : describe_gemm<double>(S, p); break;
+      case GK_OZSLICE: {
+        const auto& q = *reinterpret_cast<const OpOzSlice::Prob*>(b);
+        l = "OZSLICE x1=" + vfmt(q.x1) + " s1=" + dfmt(q.s1) + " x2=" + vfmt(q.x2) + " s2=" + dfmt(q.s2) +
+            " k1=" + std::to_string(q.k1) + " ld=" + std::to_string(q.ld) + " nr=" + std::to_string(q.nr) +
+            " nk=" + std::to_string(q.nk) + " rows=" + std::to_string(q.rows) + " kpad=" + std::to_string(q.kpad) +
+            " out=" + vfmt(q.out) + " ex=" + vfmt(q.ex);
+        break;
+      }
+      case GK_OZGEMM: {
+        const auto& g = *reinterpret_cast<const OzArgs<GROUP_OZ_SLICES>*>(b);
+        l = "OZGEMM sym=" + std::to_string((int)S.sym[p]) + " a=" + vfmt(g.a) + " rows_a=" + std::to_string(g.rows_a) +
+            " b=" + vfmt(g.b) + " rows_b=" + std::to_string(g.rows_b) + " m=" + std::to_string(g.m) +
+            " n=" + std::to_string(g.n) + " kpad=" + std::to_string(g.kpad) + " cr=" + vfmt(g.epi.cr) +
+            " ldc=" + std::to_string(g.epi.ldc) + " alpha=" + dfmt(g.epi.alpha) + " beta=" + dfmt(g.epi.beta) +
+            " nadd=" + std::to_string(g.epi.nadd) + " ld_add=" + std::to_string(g.epi.ld_add) +
+            " diag=" + dfmt(g.epi.diag);
+        for (int t = 0; t < g.epi.nadd; ++t)
+          l += " add" + std::to_string(t) + "=" + vfmt(g.epi.add_r[t]) + " c" + std::to_string(t) + "=" +
+               dfmt(g.epi.addc[t]);
+        break;
+      }
       case GK_SPLIT_F16: l = describe_split<Fp16x2>(S, p); break;
       case GK_SPLIT_BF16: l = describe_split<Bf16x3>(S, p); break;
       case GK_SQ_F16: l = describe_sq<Fp16x2>(S, p); break;
@@ -767,6 +863,7 @@ struct GroupRunnerBase {
   int use_graph = 1;
   int forced = -1;
   int precision = FEXPM_PREC_NATIVE;  // squarings of complex64 batches: split 16-bit forms
+  int oz_slices = 0;                  // Pade products from int8 slices (complex64 batches: 6), 0 = FP64 MFMA
   // report of the last run
   int launches = 0, graph_used = 0, graph_built = 0, tuned = 0;
   int cand[GC_COUNT] = {0, 0, 0};
@@ -800,6 +897,12 @@ template <typename Tin, typename W> struct GroupRunner : GroupRunnerBase {
   size_t split_cap = 0;
   std::vector<size_t> split_off;  // bytes, per walk
   bool split_ready = false;
+  // int8 slices of the Pade products: three slice buffers and their row
+  // exponents per walk (rows padded to OZ_BM, k = np)
+  void* oz_arena = nullptr;
+  size_t oz_cap = 0;
+  std::vector<size_t> oz_off;  // bytes, per walk
+  bool oz_ready = false;
 
   GroupProgram analysis;   // analysis program (per layout)
   const void* analysis_in = nullptr;
@@ -821,7 +924,10 @@ template <typename Tin, typename W> struct GroupRunner : GroupRunnerBase {
   // of vaddr) so that the program can be built and described on the host.
   bool virt = false;
 
-  explicit GroupRunner(bool virtual_layout = false) : virt(virtual_layout) {}
+  explicit GroupRunner(bool virtual_layout = false) : virt(virtual_layout) {
+    // complex64 batches: Pade products from 6 int8 slices (FP64 accuracy)
+    if (std::is_same<W, float>::value) oz_slices = GROUP_OZ_SLICES;
+  }
   ~GroupRunner() override {
     drop_graph();
     if (!virt) {
@@ -829,6 +935,7 @@ template <typename Tin, typename W> struct GroupRunner : GroupRunnerBase {
       if (w_arena) (void)hipFree(w_arena);
       if (small_arena) (void)hipFree(small_arena);
       if (split_arena) (void)hipFree(split_arena);
+      if (oz_arena) (void)hipFree(oz_arena);
     }
     for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
     if (ev0) (void)hipEventDestroy(ev0);
@@ -868,6 +975,7 @@ template <typename Tin, typename W> struct GroupRunner : GroupRunnerBase {
     plan_key.clear();
     analysis_in = nullptr;
     split_ready = false;
+    oz_ready = false;
     for (bool& t : tuned_prec) t = false;
     nwalk = cnt;
     wm.assign(cnt, WalkMem());
@@ -966,6 +1074,79 @@ template <typename Tin, typename W> struct GroupRunner : GroupRunnerBase {
   }
   void* split_buf(int i) const { return static_cast<unsigned char*>(split_arena) + split_off[i]; }
 
+  static int oz_rows(int n) { return (int)round_up(std::max(n, 1), OZ_BM); }
+  static int oz_kp(int n) { return (int)round_up(std::max(n, 1), OZ_BK); }
+  static size_t oz_buf_bytes(int n) {
+    return ((size_t)oz_rows(n) * oz_kp(n) * GROUP_OZ_SLICES + 255) / 256 * 256;
+  }
+  static size_t oz_ex_bytes(int n) { return ((size_t)oz_rows(n) * sizeof(int) + 255) / 256 * 256; }
+  // Slice buffers of the current layout (allocated on first use): per walk
+  // three slice buffers, then their three exponent arrays.
+  void ensure_oz() {
+    if (oz_ready) return;
+    oz_off.assign((size_t)nwalk, 0);
+    size_t tot = 0;
+    for (int i = 0; i < nwalk; ++i) {
+      oz_off[i] = tot;
+      tot += 3 * (oz_buf_bytes(wm[i].n) + oz_ex_bytes(wm[i].n));
+    }
+    if (virt) {
+      oz_arena = vaddr(VR_OZ);
+      oz_cap = tot;
+    } else if (tot > oz_cap) {
+      if (oz_arena) (void)hipFree(oz_arena);
+      oz_arena = nullptr;
+      oz_cap = 0;
+      FX_HIP(hipMalloc(&oz_arena, tot));
+      oz_cap = tot;
+    }
+    oz_ready = true;
+  }
+  int8_t* ozb(int i, int b) const {
+    return static_cast<int8_t*>(oz_arena) + oz_off[i] + (size_t)b * oz_buf_bytes(wm[i].n);
+  }
+  int* oze(int i, int b) const {
+    return reinterpret_cast<int*>(static_cast<unsigned char*>(oz_arena) + oz_off[i] +
+                                  3 * oz_buf_bytes(wm[i].n) + (size_t)b * oz_ex_bytes(wm[i].n));
+  }
+  void oz_slice_prob(GroupStage& S, int i, const double* x, int b) {
+    const int n = wm[i].n;
+    OpOzSlice::Prob p{x, 1.0, nullptr, 0.0, n, wm[i].np, n, n, oz_rows(n), oz_kp(n), ozb(i, b), oze(i, b)};
+    S.add(p, OpOzSlice::blocks(p));
+  }
+  // C = (slices ba)(slices bb)^T with the Pade epilogue (symmetric product).
+  void oz_prod(GroupStage& S, int i, int ba, int bb, double* c,
+               std::initializer_list<std::pair<double, const double*>> add = {}, double dg = 0.0) {
+    const int n = wm[i].n;
+    OzArgs<GROUP_OZ_SLICES> g;
+    std::memset(&g, 0, sizeof(g));
+    g.a = ozb(i, ba);
+    g.ea = oze(i, ba);
+    g.rows_a = oz_rows(n);
+    g.b = ozb(i, bb);
+    g.eb = oze(i, bb);
+    g.rows_b = oz_rows(n);
+    g.m = n;
+    g.n = n;
+    g.kpad = oz_kp(n);
+    g.epi.cr = c;
+    g.epi.ldc = wm[i].np;
+    g.epi.ld_add = wm[i].np;
+    g.epi.alpha = 1.0;
+    int k = 0;
+    for (auto& t : add) {
+      g.epi.addc[k] = t.first;
+      g.epi.add_r[k] = t.second;
+      ++k;
+    }
+    g.epi.nadd = k;
+    g.epi.diag = dg;
+    S.add(g, 0);
+    S.sym.push_back(1);
+    S.walk.push_back(i);
+    S.inplace.push_back(0);
+  }
+
   template <class P>
   void split_round(GroupStage& sp, GroupStage& sq, const GroupPlan& P_, int r, Scalars* d_scal) {
     const int i = P_.idx, n = wm[i].n, np_ = wm[i].np;
@@ -1041,6 +1222,116 @@ template <typename Tin, typename W> struct GroupRunner : GroupRunnerBase {
   }
 
   // The program of the structured pipeline for the given plans.
+  // The Pade lincombs (odd and even polynomial sums) of every walk.
+  void pade_lincombs(const std::vector<GroupPlan>& plans, GroupStage& lc) {
+    for (const GroupPlan& P : plans) {
+      const int i = P.idx, n = wm[i].n, np_ = wm[i].np;
+      double b[14];
+      for (int j = 0; j <= P.m; ++j) b[j] = pade_b(P.m, j);
+      const double* p2 = fp(i, 3);
+      const double* p4 = fp(i, 4);
+      const double* p6 = fp(i, 5);
+      const double* c8 = fp(i, 7);
+      auto add = [&](double* out, std::initializer_list<std::pair<double, const double*>> terms, double dg) {
+        OpLincomb::Prob L;
+        std::memset(&L, 0, sizeof(L));
+        L.out = out;
+        int k = 0;
+        for (auto& t : terms) {
+          L.c[k] = t.first;
+          L.in[k] = t.second;
+          ++k;
+        }
+        L.nin = k;
+        L.diag = dg;
+        L.n = n;
+        L.ld = np_;
+        lc.add(L, OpLincomb::blocks(L));
+      };
+      if (P.m == 13) {
+        add(fp(i, 6), {{b[13], p6}, {-b[11], p4}, {b[9], p2}}, 0.0);  // Z1 in t1
+        add(fp(i, 7), {{b[12], p6}, {-b[10], p4}, {b[8], p2}}, 0.0);  // Z2 in t2
+      } else if (P.m == 3) {
+        add(fp(i, 6), {{-b[3], p2}}, b[1]);
+        add(fp(i, 2), {{-b[2], p2}}, b[0]);
+      } else if (P.m == 5) {
+        add(fp(i, 6), {{b[5], p4}, {-b[3], p2}}, b[1]);
+        add(fp(i, 2), {{b[4], p4}, {-b[2], p2}}, b[0]);
+      } else if (P.m == 7) {
+        add(fp(i, 6), {{-b[7], p6}, {b[5], p4}, {-b[3], p2}}, b[1]);
+        add(fp(i, 2), {{-b[6], p6}, {b[4], p4}, {-b[2], p2}}, b[0]);
+      } else {  // 9
+        add(fp(i, 6), {{b[9], c8}, {-b[7], p6}, {b[5], p4}, {-b[3], p2}}, b[1]);
+        add(fp(i, 2), {{b[8], c8}, {-b[6], p6}, {b[4], p4}, {-b[2], p2}}, b[0]);
+      }
+    }
+  }
+
+  // Pade products from int8 slices (three slice buffers per walk; each
+  // operand is sliced in the stage before the product that reads it):
+  //   C2 = a a (0,0); C4 = C2 C2 (1,1); C6 = C4 C2 (2,1); C8 = C6 C2 (2,1);
+  //   Wb = C6 Z1 + ..., V = C6 Z2 + ... (2,0), (2,1); Uq = a W (0,1).
+  template <class F> void build_pade_oz(const std::vector<GroupPlan>& plans, F& stage) {
+    ensure_oz();
+    {
+      GroupStage& z = stage(GK_OZSLICE);
+      for (const GroupPlan& P : plans) oz_slice_prob(z, P.idx, fp(P.idx, 0), 0);
+      GroupStage& g = stage(GK_OZGEMM, GC_SYM64);
+      for (const GroupPlan& P : plans) oz_prod(g, P.idx, 0, 0, fp(P.idx, 3));  // C2
+    }
+    {
+      GroupStage& z = stage(GK_OZSLICE);
+      for (const GroupPlan& P : plans)
+        if (P.m >= 5) oz_slice_prob(z, P.idx, fp(P.idx, 3), 1);
+      GroupStage& g = stage(GK_OZGEMM, GC_SYM64);
+      for (const GroupPlan& P : plans)
+        if (P.m >= 5) oz_prod(g, P.idx, 1, 1, fp(P.idx, 4));  // C4
+    }
+    {
+      GroupStage& z = stage(GK_OZSLICE);
+      for (const GroupPlan& P : plans)
+        if (P.m >= 7) oz_slice_prob(z, P.idx, fp(P.idx, 4), 2);
+      GroupStage& g = stage(GK_OZGEMM, GC_SYM64);
+      for (const GroupPlan& P : plans)
+        if (P.m >= 7) oz_prod(g, P.idx, 2, 1, fp(P.idx, 5));  // C6
+    }
+    {
+      GroupStage& z = stage(GK_OZSLICE);
+      for (const GroupPlan& P : plans)
+        if (P.m >= 9) oz_slice_prob(z, P.idx, fp(P.idx, 5), 2);  // C6 (C4's slices are no longer read)
+      GroupStage& g = stage(GK_OZGEMM, GC_SYM64);
+      for (const GroupPlan& P : plans)
+        if (P.m == 9) oz_prod(g, P.idx, 2, 1, fp(P.idx, 7));  // C8 in t2
+    }
+    pade_lincombs(plans, stage(GK_LINCOMB));
+    {
+      GroupStage& z = stage(GK_OZSLICE);
+      for (const GroupPlan& P : plans)
+        if (P.m == 13) {
+          oz_slice_prob(z, P.idx, fp(P.idx, 6), 0);  // Z1
+          oz_slice_prob(z, P.idx, fp(P.idx, 7), 1);  // Z2
+        }
+      GroupStage& g = stage(GK_OZGEMM, GC_SYM64);
+      for (const GroupPlan& P : plans)
+        if (P.m == 13) {
+          const int i = P.idx;
+          double b[14];
+          for (int j = 0; j <= 13; ++j) b[j] = pade_b(13, j);
+          oz_prod(g, i, 2, 0, fp(i, 8), {{-b[7], fp(i, 5)}, {b[5], fp(i, 4)}, {-b[3], fp(i, 3)}}, b[1]);  // Wb
+          oz_prod(g, i, 2, 1, fp(i, 2), {{-b[6], fp(i, 5)}, {b[4], fp(i, 4)}, {-b[2], fp(i, 3)}}, b[0]);  // V
+        }
+    }
+    {
+      GroupStage& z = stage(GK_OZSLICE);
+      for (const GroupPlan& P : plans) {
+        oz_slice_prob(z, P.idx, fp(P.idx, 0), 0);
+        oz_slice_prob(z, P.idx, fp(P.idx, P.m == 13 ? 8 : 6), 1);
+      }
+      GroupStage& g = stage(GK_OZGEMM, GC_SYM64);
+      for (const GroupPlan& P : plans) oz_prod(g, P.idx, 0, 1, fp(P.idx, 1));  // Uq
+    }
+  }
+
   // Stages are held in a deque: references returned by stage() stay valid
   // while later stages are appended.
   void build(const std::vector<GroupPlan>& plans, const void* d_in, void* d_out, Scalars* d_scal,
@@ -1067,7 +1358,9 @@ template <typename Tin, typename W> struct GroupRunner : GroupRunnerBase {
       }
     }
     // 2. Pade products (real symmetric, half tiles); A^{2k} = (-1)^k G^{2k}
-    {
+    const bool oz = oz_slices > 0 && std::is_same<W, float>::value;
+    if (oz) build_pade_oz(plans, stage);
+    else {
       GroupStage& s1 = stage(GK_GEMM, GC_SYM64, MODE_REAL);
       for (const GroupPlan& P : plans) {
         const int i = P.idx, n = wm[i].n, np_ = wm[i].np;
@@ -1342,6 +1635,7 @@ template <typename Tin, typename W> struct GroupRunner : GroupRunnerBase {
   std::string describe() const {
     std::string o = "REGIONS f64=" + std::to_string(f64_cap) + " w=" + std::to_string(w_cap) +
                     " small=" + std::to_string(small_cap) + " split=" + std::to_string(split_cap) +
+                    " oz=" + std::to_string(oz_cap) +
                     " wsize=" + std::to_string(sizeof(W)) + " insize=" + std::to_string(sizeof(Tin)) + "\n";
     for (const GroupStage& S : stages) o += describe_stage<Tin, W>(S);
     return o;
@@ -1420,6 +1714,7 @@ template <typename Tin, typename W> struct GroupRunner : GroupRunnerBase {
     key.reserve(plans.size() * 6 + 1);
     key.push_back((double)kblock);
     key.push_back((double)precision);
+    key.push_back((double)oz_slices);
     for (const GroupPlan& P : plans) {
       key.push_back(P.idx);
       key.push_back(P.m);

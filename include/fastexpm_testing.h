@@ -96,13 +96,15 @@ int fexpm_test_mfma_rate(int kind, int iters, double* tflops, double* ms);
  * a grouped batch run but on a virtual address layout (no device memory is
  * touched): address = region << 40 | byte offset, regions 1 FP64 planes,
  * 2 squaring planes, 3 per-walk scalars and diagonals, 4 split parts,
- * 5 input, 6 output, 7 psi0, 8 psi, 9 walk scalars.  Writes a text listing
- * of every stage and problem (NUL-terminated) into buf and its length into
+ * 5 input, 6 output, 7 psi0, 8 psi, 9 walk scalars, 10 int8 slices.
+ * fp64_slices: -1 the batch default (complex64: Pade products from 6 int8
+ * slices), 0 FP64 MFMA, 6 int8 slices (complex64).  Writes a text listing of
+ * every stage and problem (NUL-terminated) into buf and its length into
  * *len; with cap too small returns FEXPM_EINVAL and only sets *len.  Lets a
  * CPU interpreter check the program against the oracle. */
-int fexpm_test_group_program(int dtype, int precision, int64_t count, const int64_t* n, const int32_t* m,
-                             const int32_t* s, const int32_t* shifted, const double* mu_re, const double* mu_im,
-                             int with_psi, char* buf, int64_t cap, int64_t* len);
+int fexpm_test_group_program(int dtype, int precision, int fp64_slices, int64_t count, const int64_t* n,
+                             const int32_t* m, const int32_t* s, const int32_t* shifted, const double* mu_re,
+                             const double* mu_im, int with_psi, char* buf, int64_t cap, int64_t* len);
 
 #ifdef __cplusplus
 }

@@ -28,7 +28,7 @@ COMPLEX64, COMPLEX128 = 2, 3
 PREC = {"native": 0, "fp16x2": 1, "bf16x3": 2}
 
 
-def _program(dtype, precision, n, m, s, mu, with_psi):
+def _program(dtype, precision, n, m, s, mu, with_psi, slices=-1):
     lib = ctypes.CDLL(LIB)
     cnt = len(n)
     n_ = np.asarray(n, dtype=np.int64)
@@ -39,7 +39,8 @@ def _program(dtype, precision, n, m, s, mu, with_psi):
     mi = np.asarray([complex(x).imag for x in mu], dtype=np.float64)
     ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
     ln = ctypes.c_int64(0)
-    args = (ctypes.c_int(dtype), ctypes.c_int(precision), ctypes.c_int64(cnt), ptr(n_), ptr(m_), ptr(s_), ptr(sh),
+    args = (ctypes.c_int(dtype), ctypes.c_int(precision), ctypes.c_int(slices), ctypes.c_int64(cnt), ptr(n_), ptr(m_),
+            ptr(s_), ptr(sh),
             ptr(mr), ptr(mi), ctypes.c_int(1 if with_psi else 0))
     lib.fexpm_test_group_program(*args, None, ctypes.c_int64(0), ctypes.byref(ln))
     buf = ctypes.create_string_buffer(ln.value + 1)
@@ -66,6 +67,7 @@ class Interp:
         }
         self.lines = lines[1:]
         self.kinds = []
+        self.oz = {}  # int8 slice buffer -> FP64 values of the operand it holds
 
     def _elem(self, tok):
         r, off = (int(x) for x in tok.split(":"))
@@ -202,6 +204,31 @@ class Interp:
         self.put(kv["wr"], ldw, X.real)
         self.put(kv["wi"], ldw, X.imag)
 
+    def op_ozslice(self, kv):
+        """Record the operand a slice buffer holds (its FP64 values now);
+        the slicing itself is checked on the GPU against its error bound."""
+        nr, nk, rows, kpad = int(kv["nr"]), int(kv["nk"]), int(kv["rows"]), int(kv["kpad"])
+        k1, ld = int(kv["k1"]), int(kv["ld"])
+        x = np.zeros((rows, kpad))
+        x[:nr, :min(k1, nk)] = float(kv["s1"]) * self.get(kv["x1"], ld, nr, min(k1, nk))
+        if nk > k1:
+            x[:nr, k1:nk] = float(kv["s2"]) * self.get(kv["x2"], ld, nr, nk - k1)
+        self.oz[kv["out"]] = x
+
+    def op_ozgemm(self, kv):
+        m, n, kp, ldc = int(kv["m"]), int(kv["n"]), int(kv["kpad"]), int(kv["ldc"])
+        A, B = self.oz[kv["a"]][:m, :kp], self.oz[kv["b"]][:n, :kp]
+        assert self.oz[kv["a"]].shape[0] == int(kv["rows_a"]) and self.oz[kv["b"]].shape[0] == int(kv["rows_b"])
+        C = float(kv["alpha"]) * (A @ B.T)
+        if float(kv["beta"]) != 0.0:
+            C = C + float(kv["beta"]) * self.get(kv["cr"], ldc, m, n)
+        for t in range(int(kv["nadd"])):
+            C = C + float(kv[f"c{t}"]) * self.get(kv[f"add{t}"], int(kv["ld_add"]), m, n)
+        C = C + float(kv["diag"]) * np.eye(m, n)
+        if int(kv["sym"]) == 1:
+            C = self._sym(C)
+        self.put(kv["cr"], ldc, C)
+
     def op_split(self, kv):
         pass  # the 16-bit parts are a device representation of Y; SQ reads Y itself
 
@@ -245,9 +272,10 @@ def _walks(seed):
     return mats
 
 
-@pytest.mark.parametrize("dt,precision", [(COMPLEX128, "native"), (COMPLEX64, "native"),
-                                          (COMPLEX64, "fp16x2"), (COMPLEX64, "bf16x3")])
-def test_group_program_matches_oracle(dt, precision):
+@pytest.mark.parametrize("dt,precision,slices", [(COMPLEX128, "native", -1), (COMPLEX64, "native", -1),
+                                                 (COMPLEX64, "native", 0), (COMPLEX64, "fp16x2", -1),
+                                                 (COMPLEX64, "bf16x3", 0)])
+def test_group_program_matches_oracle(dt, precision, slices):
     mats = _walks(11)
     plans = []
     for a in mats:
@@ -258,7 +286,8 @@ def test_group_program_matches_oracle(dt, precision):
     assert ms == [3, 5, 7, 9, 13], ms  # every Pade degree is exercised
     psi0 = [np.exp(0.3j * np.arange(a.shape[0])) / np.sqrt(a.shape[0]) for a in mats]
     text = _program(dt, PREC[precision], [a.shape[0] for a in mats], [p[0] for p in plans],
-                    [p[1] for p in plans], [p[2] for p in plans], True)
+                    [p[1] for p in plans], [p[2] for p in plans], True, slices)
+    assert ("OZGEMM" in text) == (dt == COMPLEX64 and slices != 0)
     it = Interp(text, mats, psi0)
     it.run()
     off = 0
@@ -309,12 +338,17 @@ def test_group_program_rejects_bad_plans():
     d = np.zeros(1)
     ln = ctypes.c_int64(0)
     p = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
-    assert lib.fexpm_test_group_program(2, 0, ctypes.c_int64(1), p(n), p(bad_m), p(z), p(z), p(d), p(d), 0, None,
-                                        ctypes.c_int64(0), ctypes.byref(ln)) == -1
+    assert lib.fexpm_test_group_program(2, 0, -1, ctypes.c_int64(1), p(n), p(bad_m), p(z), p(z), p(d), p(d), 0,
+                                        None, ctypes.c_int64(0), ctypes.byref(ln)) == -1
     good_m = np.array([13], np.int32)
     # split precision is for complex64 only
-    assert lib.fexpm_test_group_program(3, 1, ctypes.c_int64(1), p(n), p(good_m), p(z), p(z), p(d), p(d), 0, None,
-                                        ctypes.c_int64(0), ctypes.byref(ln)) == -1
+    assert lib.fexpm_test_group_program(3, 1, -1, ctypes.c_int64(1), p(n), p(good_m), p(z), p(z), p(d), p(d), 0,
+                                        None, ctypes.c_int64(0), ctypes.byref(ln)) == -1
+    # int8-slice products: 0 or 6, complex64 only
+    assert lib.fexpm_test_group_program(3, 0, 6, ctypes.c_int64(1), p(n), p(good_m), p(z), p(z), p(d), p(d), 0,
+                                        None, ctypes.c_int64(0), ctypes.byref(ln)) == -1
+    assert lib.fexpm_test_group_program(2, 0, 5, ctypes.c_int64(1), p(n), p(good_m), p(z), p(z), p(d), p(d), 0,
+                                        None, ctypes.c_int64(0), ctypes.byref(ln)) == -1
 
 
 def test_group_program_detects_planted_mistakes():
