@@ -298,7 +298,7 @@ template <class P> struct OpSplit {
 };
 
 // Exact int8 slices of one FP64 operand (rows of a walk's plane, optionally
-// the concatenation of two planes), one workgroup per row.
+// the concatenation of two planes), four rows per workgroup.
 constexpr int GROUP_OZ_SLICES = 6;
 struct OpOzSlice {
   static constexpr int NT = 256;
@@ -313,9 +313,10 @@ struct OpOzSlice {
     int8_t* out;
     int* ex;
   };
-  static int blocks(const Prob& p) { return p.rows; }
-  static __device__ __forceinline__ void run(const Prob& p, int r) {
-    oz_slice_row<GROUP_OZ_SLICES>(p.x1, p.s1, p.x2, p.s2, p.k1, p.ld, p.nr, p.nk, p.rows, p.kpad, p.out, p.ex, r);
+  static int blocks(const Prob& p) { return p.rows / 4; }  // rows: a multiple of OZ_BM
+  static __device__ __forceinline__ void run(const Prob& p, int b) {
+    oz_slice_rows4<GROUP_OZ_SLICES>(p.x1, p.s1, p.x2, p.s2, p.k1, p.ld, p.nr, p.nk, p.rows, p.kpad, p.out, p.ex,
+                                    4 * b);
   }
 };
 
