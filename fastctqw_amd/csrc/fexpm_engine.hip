@@ -124,6 +124,7 @@ struct PipelineBase {
   int precision = 0;       // enum fexpm_precision for the structured squarings of FP32 engines
   int oz_slices = 0;       // FP64 companion: symmetric FP64 products from int8 slices (0 = FP64 MFMA)
   int oz_scope = 3;        // stages using them: bit 0 the Pade products, bit 1 the solve (S, X = Z Z^T)
+  int tail_split = 1;      // FP32 squarings: last partial wave of 128-tiles as 64-tiles
 };
 
 // ---------------------------------------------------------------------------
@@ -1010,6 +1011,49 @@ template <typename W> struct Pipeline : PipelineBase {
     return {cur, dc};
   }
 
+  // FP32 structured squaring in 128 x 128 tiles.  When the tile count
+  // leaves the last wave of workgroups less than half full, the tiles of
+  // the full waves run in 128 x 128 and the rest as 64 x 64 sub-tiles
+  // (gemm_sym_tail_kernel), which spread over the whole device.
+  void square_big_tail(const GemmArgs<W>& g, hipStream_t st) {
+    if constexpr (std::is_same<W, float>::value) {
+      using C = GemmCfg<float>;
+      const int T = (g.m + C::BM - 1) / C::BM;
+      const int nb = T * (T + 1) / 2;
+      static int cap = -1;
+      if (cap < 0) {
+        int dev = 0, per_cu = 0;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, (const void*)gemm_nt_kernel<float, MODE_C3M, true, true, C>, C::NT, 0) == hipSuccess)
+          cap = std::max(1, per_cu) * prop.multiProcessorCount;
+        else
+          cap = 0;
+      }
+      static_assert(C::BM == 2 * FCfgSmall::BM && C::BN == 2 * FCfgSmall::BN, "tail sub-tiles halve the tile");
+      const int rest = cap > 0 ? nb % cap : 0;
+      const int full = nb - rest;
+      if (tail_split && cap > 0 && full > 0 && rest > 0 && 2 * rest <= cap) {
+        gemm_nt_kernel<float, MODE_C3M, true, true, C><<<dim3(full), dim3(C::NT), 0, st>>>(g);
+        FX_LAUNCH();
+        gemm_sym_tail_kernel<float, MODE_C3M, true, FCfgSmall><<<dim3(4 * rest), dim3(FCfgSmall::NT), 0, st>>>(g, full);
+        FX_LAUNCH();
+        int sub = 4 * rest;  // sub-tiles computed: all but the upper one of each diagonal tile
+        for (int t = full; t < nb; ++t) {
+          int tm = (int)((std::sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+          while ((tm + 1) * (tm + 2) / 2 <= t) ++tm;
+          while (tm * (tm + 1) / 2 > t) --tm;
+          if (t - tm * (tm + 1) / 2 == tm) --sub;
+        }
+        flops += 3.0 * 2.0 * ((double)full * C::BM * C::BN + (double)sub * FCfgSmall::BM * FCfgSmall::BN) *
+                 (double)g.k;
+        return;
+      }
+    }
+    gemm<MODE_C3M, true, true>(g, st);
+  }
+
   std::pair<Plane2<W>*, int> square_split(int s, bool structured, int dc, hipStream_t st) {
     if (std::is_same<W, float>::value && structured && precision == FEXPM_PREC_FP16X2)
       return square_split16<Fp16x2>(s, dc, st);
@@ -1035,7 +1079,7 @@ template <typename W> struct Pipeline : PipelineBase {
       const bool small = std::is_same<W, float>::value && n <= 4096;
       if (structured) {
         if (blk && small) gemm<MODE_C3M, true, true, FCfgSmall>(g, st);
-        else if (blk) gemm<MODE_C3M, true, true>(g, st);
+        else if (blk) square_big_tail(g, st);
         else gemm<MODE_C3M, true>(g, st);
       } else {
         if (blk) gemm<MODE_C3M, false, true>(g, st);
@@ -1747,6 +1791,9 @@ int fexpm_set_option(fexpm_engine* e, int key, int64_t value) {
       return FEXPM_OK;
     case FEXPM_OPT_PADE_FP64:
       if (e->pipe) e->pipe->pade_fp64 = value ? 1 : 0;
+      return FEXPM_OK;
+    case FEXPM_OPT_TAIL_SPLIT:
+      if (e->pipe) e->pipe->tail_split = value ? 1 : 0;
       return FEXPM_OK;
     case FEXPM_OPT_FP64_SLICES_SCOPE:
       if (value < 1 || value > 3) {

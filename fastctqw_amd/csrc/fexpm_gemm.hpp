@@ -405,6 +405,23 @@ void gemm_nt_kernel(GemmArgs<T> g) {
   gemm_tile<T, MODE, KBLK, CFG>(g, tm, tn, SYM);
 }
 
+// Tail of a symmetric product: lower-triangular tiles [t0, t0 + count) of
+// edge 2 * CFG::BM, each computed as its sub-tiles of edge CFG::BM (three of
+// the four for a diagonal tile).  Pairs with a launch of tiles [0, t0) in the
+// larger configuration when those leave a nearly empty last wave of
+// workgroups on the device; every output value sees the same k order and
+// accumulation blocks as in the larger tile.
+template <typename T, int MODE, bool KBLK, class CFG>
+__global__ __launch_bounds__(CFG::NT)
+void gemm_sym_tail_kernel(GemmArgs<T> g, int t0) {
+  const int q = (int)blockIdx.x;
+  int tm, tn;
+  tri_tile(t0 + q / 4, tm, tn);
+  const int sm = 2 * tm + ((q & 3) >> 1), sn = 2 * tn + (q & 1);
+  if (sn > sm) return;  // upper sub-tile of a diagonal tile
+  gemm_tile<T, MODE, KBLK, CFG>(g, sm, sn, true);
+}
+
 // ---------------------------------------------------------------------------
 // Grouped launches: many independent problems (one per walk of a batch, or
 // several per walk) in one persistent launch.  Problem p owns the tiles

@@ -34,6 +34,11 @@ enum { FEXPM_OPT_GROUP_CANDIDATE = 101 };
  * (S, X = Z Z^T), 3 both (default).  For precision studies. */
 enum { FEXPM_OPT_FP64_SLICES_SCOPE = 102 };
 
+/* Engine option: 1 (default) runs the last, partly filled wave of 128 x 128
+ * tiles of an FP32 symmetric squaring as 64 x 64 sub-tiles; 0 keeps one
+ * launch of 128 x 128 tiles.  Both give the same values bit for bit. */
+enum { FEXPM_OPT_TAIL_SPLIT = 103 };
+
 /* Internal planes readable by fexpm_debug_plane (working precision, n x n). */
 enum fexpm_plane {
   FEXPM_PLANE_A = 0,   /* (A - mu I) / 2^s; structured: its imaginary part G */
