@@ -125,6 +125,7 @@ struct PipelineBase {
   int oz_slices = 0;       // FP64 companion: symmetric FP64 products from int8 slices (0 = FP64 MFMA)
   int oz_scope = 3;        // stages using them: bit 0 the Pade products, bit 1 the solve (S, X = Z Z^T)
   int tail_split = 1;      // FP32 squarings: last partial wave of 128-tiles as 64-tiles
+  int z_overlap = 1;       // Cholesky solve: Z updates on a second stream
 };
 
 // ---------------------------------------------------------------------------
@@ -438,6 +439,8 @@ template <typename W> struct Pipeline : PipelineBase {
     }
     if (sbuf) (void)hipFree(sbuf);
     for (auto& kv : sq_orders) (void)hipFree(kv.second);
+    for (hipEvent_t e : zev) (void)hipEventDestroy(e);
+    if (zst) (void)hipStreamDestroy(zst);
     if (events)
       for (int k = 0; k < 8; ++k) (void)hipEventDestroy(ev[k]);
   }
@@ -773,6 +776,22 @@ template <typename W> struct Pipeline : PipelineBase {
     return solution_tail(lr + inp, li + inp, ldl, lr, li, 1, q64, st);
   }
 
+  // Second stream and events of the overlapped Z updates (created on first use).
+  hipStream_t zst = nullptr;
+  std::vector<hipEvent_t> zev;
+  hipStream_t z_stream(hipStream_t) {
+    if (!zst) FX_HIP(hipStreamCreateWithFlags(&zst, hipStreamNonBlocking));
+    return zst;
+  }
+  hipEvent_t z_event(int q) {
+    while ((int)zev.size() <= q) {
+      hipEvent_t e;
+      FX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      zev.push_back(e);
+    }
+    return zev[(size_t)q];
+  }
+
   // Structured solve by Cholesky (A = iC; V in v.r, U = i Uq with Uq in u.r).
   // M = V - i Uq is normal, S = M^H M = V^2 + Uq^2 is symmetric positive
   // definite, and with S = L L^T, Z = (V + i Uq) L^{-T}:
@@ -835,6 +854,13 @@ template <typename W> struct Pipeline : PipelineBase {
     // Two levels: 64-wide inner panels update only the columns of their
     // CH_OUTER-wide outer block; the outer block then updates the trailing
     // S (half tiles) and the remaining columns of Z with K = CH_OUTER.
+    // With z_overlap the Z updates run on a second stream: they read only
+    // finished columns of L (the panel's own, after its panel product), so
+    // each waits for the event recorded after that panel product, and the
+    // latency-bound factorisation of S proceeds meanwhile.
+    const bool ov = z_overlap != 0;
+    hipStream_t zs = ov ? z_stream(st) : st;
+    int nev = 0;
     for (int K0 = 0; K0 < ip; K0 += CH_OUTER) {
       const int K1 = std::min(ip, K0 + CH_OUTER);
       for (int k0 = K0; k0 < K1; k0 += LU_NB) {
@@ -846,21 +872,31 @@ template <typename W> struct Pipeline : PipelineBase {
         double* panel = sm + (int64_t)k1 * L + k0;  // L[k1:, k0:k1]
         if (rest > 0)
           flops += dgemm_real(panel, L, dk, LU_NB, panel, L, rest, LU_NB, LU_NB, 1.0, 0.0, false, st);
+        if (ov) {
+          hipEvent_t e = z_event(nev++);
+          FX_HIP(hipEventRecord(e, st));
+          FX_HIP(hipStreamWaitEvent(zs, e, 0));
+        }
         // Z[:, k0:k1] <- Z[:, k0:k1] L_kk^{-T}
-        flops += dgemm_real(zb + k0, L, dk, LU_NB, zb + k0, L, 2 * ip, LU_NB, LU_NB, 1.0, 0.0, false, st);
+        flops += dgemm_real(zb + k0, L, dk, LU_NB, zb + k0, L, 2 * ip, LU_NB, LU_NB, 1.0, 0.0, false, zs);
         if (k1 < K1) {
           // S[k1:, k1:K1] -= L[k1:, k0:k1] L[k1:K1, k0:k1]^T;  Z[:, k1:K1] -= Z[:, k0:k1] L[k1:K1, k0:k1]^T
           flops += dgemm_real(panel, L, panel, L, sm + (int64_t)k1 * L + k1, L, rest, K1 - k1, LU_NB, -1.0, 1.0,
                               false, st);
-          flops += dgemm_real(zb + k0, L, panel, L, zb + k1, L, 2 * ip, K1 - k1, LU_NB, -1.0, 1.0, false, st);
+          flops += dgemm_real(zb + k0, L, panel, L, zb + k1, L, 2 * ip, K1 - k1, LU_NB, -1.0, 1.0, false, zs);
         }
       }
       if (K1 < ip) {
         const int r2 = ip - K1, kk = K1 - K0;
         double* p2 = sm + (int64_t)K1 * L + K0;  // L[K1:, K0:K1]
         flops += dgemm_real(p2, L, p2, L, sm + (int64_t)K1 * L + K1, L, r2, r2, kk, -1.0, 1.0, true, st);
-        flops += dgemm_real(zb + K0, L, p2, L, zb + K1, L, 2 * ip, r2, kk, -1.0, 1.0, false, st);
+        flops += dgemm_real(zb + K0, L, p2, L, zb + K1, L, 2 * ip, r2, kk, -1.0, 1.0, false, zs);
       }
+    }
+    if (ov) {  // X = Z Z^T needs every Z update
+      hipEvent_t e = z_event(nev++);
+      FX_HIP(hipEventRecord(e, zs));
+      FX_HIP(hipStreamWaitEvent(st, e, 0));
     }
     // X = Z Z^T (3M, half tiles) over S and V V^T.  From int8 slices: the two
     // real symmetric products Re X = [Zr Zi][Zr -Zi]^T, Im X = [Zr Zi][Zi Zr]^T.
@@ -1195,6 +1231,7 @@ template <typename W> struct Pipeline : PipelineBase {
       h->spd_solve = spd_solve;
       h->oz_slices = oz_slices;
       h->oz_scope = oz_scope;
+      h->z_overlap = z_overlap;
       h->front_pade<Tin>(d_a, P.structured, P.m, P.mu_re, P.mu_im, scale, st);
       if (timing) FX_HIP(hipEventRecord(ev[2], st));
       const int dh = h->solve(P.structured, 0, st);
@@ -1794,6 +1831,9 @@ int fexpm_set_option(fexpm_engine* e, int key, int64_t value) {
       return FEXPM_OK;
     case FEXPM_OPT_TAIL_SPLIT:
       if (e->pipe) e->pipe->tail_split = value ? 1 : 0;
+      return FEXPM_OK;
+    case FEXPM_OPT_Z_OVERLAP:
+      if (e->pipe) e->pipe->z_overlap = value ? 1 : 0;
       return FEXPM_OK;
     case FEXPM_OPT_FP64_SLICES_SCOPE:
       if (value < 1 || value > 3) {

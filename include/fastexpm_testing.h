@@ -39,6 +39,11 @@ enum { FEXPM_OPT_FP64_SLICES_SCOPE = 102 };
  * launch of 128 x 128 tiles.  Both give the same values bit for bit. */
 enum { FEXPM_OPT_TAIL_SPLIT = 103 };
 
+/* Engine option: 1 (default) runs the Z updates of the structured Cholesky
+ * solve on a second stream, overlapped with the factorisation of S; 0 keeps
+ * one stream.  Both give the same values bit for bit. */
+enum { FEXPM_OPT_Z_OVERLAP = 104 };
+
 /* Internal planes readable by fexpm_debug_plane (working precision, n x n). */
 enum fexpm_plane {
   FEXPM_PLANE_A = 0,   /* (A - mu I) / 2^s; structured: its imaginary part G */

@@ -193,6 +193,24 @@ def test_squaring_tail_split(fx, n):
     assert err <= _tol(np.complex64, st["s"])
 
 
+@pytest.mark.parametrize("dt,n", [(np.complex64, 700), (np.complex128, 330)])
+def test_cholesky_z_overlap(fx, dt, n):
+    """Z updates of the Cholesky solve on a second stream give the same U bit
+    for bit as one stream."""
+    from fastctqw_amd.fastexpm import _fastexpm_core as core
+
+    _say(f"z overlap {np.dtype(dt).name} n={n}")
+    L = laplacian(erdos_renyi(n, 0.3, seed=70 + n, weighted=True))
+    a = (-1j * 6.0 * L).astype(dt)
+    with _engine(fx, dt, n) as e:
+        u_two = e.run(a)
+        e._backend.set_option(core.OPT_Z_OVERLAP, 0)
+        u_one = e.run(a)
+        e._backend.set_option(core.OPT_Z_OVERLAP, 1)
+        u_two2 = e.run(a)
+    assert np.array_equal(u_two, u_one) and np.array_equal(u_two, u_two2)
+
+
 def test_fp64_slices_option_errors(fx):
     from fastctqw_amd.fastexpm import _fastexpm_core as core
 
