@@ -546,6 +546,8 @@ template <typename W> struct Pipeline : PipelineBase {
     g.epi = e;
     if (sym) {
       const int T = ra / OZ_BM;
+      static_assert(OZ_BM == SQ_BM, "the grouped tile orders are built for SQ_BM tiles");
+      g.order = tile_order(T, st);
       oz_gemm_kernel<S, true><<<dim3((unsigned)(T * (T + 1) / 2)), dim3(512), 0, st>>>(g);
     } else {
       oz_gemm_kernel<S, false><<<dim3((unsigned)(rb / OZ_BM), (unsigned)(ra / OZ_BM)), dim3(512), 0, st>>>(g);
@@ -994,6 +996,21 @@ template <typename W> struct Pipeline : PipelineBase {
   // holding the last Y and the index of its diagonal buffer.
   // Split-precision squarings (FP32 engines, structured path): each step
   // splits the current Y into 16-bit parts and squares on 16-bit MFMA.
+  // Grouped lower-triangular order of T x T tiles of SQ_BM (sym_tile_order,
+  // groups of SQ_GROUP), built once per tile count; the copy waits for the
+  // host buffer.
+  const int* tile_order(int T, hipStream_t st) {
+    int*& o = sq_orders[T];
+    if (!o) {
+      std::vector<int> h((size_t)T * (T + 1) / 2);
+      sym_tile_order(T, SQ_GROUP, h.data());
+      FX_HIP(hipMalloc(&o, h.size() * sizeof(int)));
+      FX_HIP(hipMemcpyAsync(o, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice, st));
+      FX_HIP(hipStreamSynchronize(st));
+    }
+    return o;
+  }
+
   template <class P>
   std::pair<Plane2<W>*, int> square_split16(int s, int dc, hipStream_t st) {
     Plane2<W>* cur = &x0;
@@ -1009,15 +1026,7 @@ template <typename W> struct Pipeline : PipelineBase {
     }
     const int T = (n + SQ_BM - 1) / SQ_BM;
     const int nb = T * (T + 1) / 2;
-    int*& sq_order = sq_orders[T];
-    if (!sq_order) {
-      // built once per tile count (the copy waits for the host buffer)
-      std::vector<int> h((size_t)nb);
-      sym_tile_order(T, SQ_GROUP, h.data());
-      FX_HIP(hipMalloc(&sq_order, h.size() * sizeof(int)));
-      FX_HIP(hipMemcpyAsync(sq_order, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice, st));
-      FX_HIP(hipStreamSynchronize(st));
-    }
+    const int* sq_order = tile_order(T, st);
     for (int q = 0; q < s; ++q) {
       dim3 sg((unsigned)((kpad / 16 + 15) / 16), (unsigned)((rows + 15) / 16));
       split_blocked<P><<<sg, 256, 0, st>>>((const float*)cur->r, (const float*)cur->i, ld, n, n, rows, kpad,

@@ -149,6 +149,7 @@ template <int S> struct OzArgs {
   int rows_b;
   int m, n, kpad;
   Epilogue<double> epi;
+  const int* order;  // optional symmetric tile order (tm << 16 | tn) per logical block, else row-major
 };
 
 // Output tile (tm, tn) of C = A B^T from the slices: 8 waves (4 x 2), each
@@ -283,7 +284,14 @@ template <int S, bool SYM>
 __global__ __launch_bounds__(512) void oz_gemm_kernel(OzArgs<S> g) {
   int tm, tn;
   if (SYM) {
-    tri_tile(xcd_remap((int)blockIdx.x, (int)gridDim.x), tm, tn);
+    const int lb = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    if (g.order) {
+      const int code = g.order[lb];
+      tm = code >> 16;
+      tn = code & 0xFFFF;
+    } else {
+      tri_tile(lb, tm, tn);
+    }
   } else {
     const int nbn = (int)gridDim.x;
     const int lin = xcd_remap((int)(blockIdx.y * gridDim.x + blockIdx.x), (int)(gridDim.x * gridDim.y));
