@@ -900,8 +900,13 @@ template <typename W> struct Pipeline : PipelineBase {
       FX_HIP(hipEventRecord(e, zs));
       FX_HIP(hipStreamWaitEvent(st, e, 0));
     }
-    // X = Z Z^T (3M, half tiles) over S and V V^T.  From int8 slices: the two
-    // real symmetric products Re X = [Zr Zi][Zr -Zi]^T, Im X = [Zr Zi][Zi Zr]^T.
+    // X = Z Z^T over S and V V^T, from two real symmetric products: with
+    // Zr = V L^{-T}, Zi = Uq L^{-T} and V, Uq, S commuting,
+    //     Zr Zr^T + Zi Zi^T = (V^2 + Uq^2) S^{-1} = I,
+    // so Re X = Zr Zr^T - Zi Zi^T = 2 Zr Zr^T - I and
+    //    Im X = Zr Zi^T + Zi Zr^T = 2 Zi Zr^T (= 2 Uq S^{-1} V, symmetric),
+    // both to FP64 rounding (the identity holds as far as L L^T = S does).
+    // Two thirds of the flops of the 3M complex product.
     double* xr = lr + 2 * P;
     double* xi = lr + 3 * P;
     if (oz_slices > 0 && (oz_scope & 2)) {
@@ -909,31 +914,34 @@ template <typename W> struct Pipeline : PipelineBase {
       std::memset(&e, 0, sizeof(e));
       e.ldc = L;
       e.ld_add = L;
-      e.alpha = 1.0;
-      const OzOp zz{zb, 1.0, zb + P, 1.0, kpad, L};
-      const OzOp zre{zb, 1.0, zb + P, -1.0, kpad, L};
-      const OzOp zim{zb + P, 1.0, zb, 1.0, kpad, L};
+      e.alpha = 2.0;
+      const OzOp zr{zb, 1.0, nullptr, 0.0, kpad, L};
+      const OzOp zi{zb + P, 1.0, nullptr, 0.0, kpad, L};
       e.cr = xr;
-      flops += oz_product(zz, &zre, n, n, 2 * kpad, e, true, st);
+      e.diag = -1.0;
+      flops += oz_product(zr, nullptr, n, n, kpad, e, true, st);
       e.cr = xi;
-      flops += oz_product(zz, &zim, n, n, 2 * kpad, e, true, st);
+      e.diag = 0.0;
+      flops += oz_product(zi, &zr, n, n, kpad, e, true, st);
     } else {
       GemmArgs<double> g;
       std::memset(&g, 0, sizeof(g));
       g.ar = zb;
-      g.ai = zb + P;
       g.lda = L;
       g.br = zb;
-      g.bi = zb + P;
       g.ldb = L;
       g.m = n;
       g.n = n;
       g.k = kpad;
       g.epi.cr = xr;
-      g.epi.ci = xi;
       g.epi.ldc = L;
-      g.epi.alpha = 1.0;
-      flops += gemm64<MODE_C3M, true>(g, st);
+      g.epi.alpha = 2.0;
+      g.epi.diag = -1.0;
+      flops += gemm64<MODE_REAL, true>(g, st);
+      g.ar = zb + P;
+      g.epi.cr = xi;
+      g.epi.diag = 0.0;
+      flops += gemm64<MODE_REAL, true>(g, st);
     }
     return solution_tail(xr, xi, L, zb, zb + P, 0, q64, st);
   }

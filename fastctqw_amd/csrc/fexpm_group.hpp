@@ -1543,16 +1543,20 @@ template <typename Tin, typename W> struct GroupRunner : GroupRunnerBase {
           }
         }
       }
-      GroupStage& xs = stage(GK_GEMM, GC_SYM64, MODE_C3M);
+      // X = Z Z^T as Re X = 2 Zr Zr^T - I, Im X = 2 Zi Zr^T (Zr Zr^T + Zi Zi^T
+      // = I because L L^T = V^2 + Uq^2): two real symmetric products.
+      GroupStage& xs = stage(GK_GEMM, GC_SYM64, MODE_REAL);
       for (const GroupPlan& P : plans) {
         const int i = P.idx, n = wm[i].n, np_ = wm[i].np, kpad = (int)round_up(n, 16);
         const size_t Q = (size_t)np_ * np_;
         double* zb = fp(i, 3);
         GemmArgs<double> g = g64(zb, zb, np_, n, n, kpad, zb + 2 * Q);
-        g.ai = zb + Q;
-        g.bi = zb + Q;
-        g.epi.ci = zb + 3 * Q;
+        g.epi.alpha = 2.0;
+        g.epi.diag = -1.0;
         xs.add_gemm(g, true, i);
+        GemmArgs<double> h = g64(zb + Q, zb, np_, n, n, kpad, zb + 3 * Q);
+        h.epi.alpha = 2.0;
+        xs.add_gemm(h, true, i);
       }
       GroupStage& so = stage(GK_SOLUTION);
       for (const GroupPlan& P : plans) {
