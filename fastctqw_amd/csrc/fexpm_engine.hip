@@ -126,6 +126,7 @@ struct PipelineBase {
   int oz_scope = 3;        // stages using them: bit 0 the Pade products, bit 1 the solve (S, X = Z Z^T)
   int tail_split = 1;      // FP32 squarings: last partial wave of 128-tiles as 64-tiles
   int z_overlap = 1;       // Cholesky solve: Z updates on a second stream
+  int ch_outer = 256;      // Cholesky solve: outer block width (a multiple of 64)
 };
 
 // ---------------------------------------------------------------------------
@@ -146,7 +147,7 @@ struct PipelineBase {
 // temp (tr/ti) holds 2np x LU_OUTER values per plane.  Returns the real
 // flops issued by the MFMA GEMMs.
 constexpr int LU_OUTER = 256;
-constexpr int CH_OUTER = 256;  // outer block of the structured (Cholesky) solve
+constexpr int CH_OUTER = 256;  // outer block of the grouped structured (Cholesky) solve
 constexpr int SQ_GROUP = 8;    // tile group of the split squaring order
 
 // Panel factorisation: one cooperative launch per 64-wide panel (default)
@@ -854,8 +855,8 @@ template <typename W> struct Pipeline : PipelineBase {
     }
     }
     // Two levels: 64-wide inner panels update only the columns of their
-    // CH_OUTER-wide outer block; the outer block then updates the trailing
-    // S (half tiles) and the remaining columns of Z with K = CH_OUTER.
+    // ch_outer-wide outer block; the outer block then updates the trailing
+    // S (half tiles) and the remaining columns of Z with K = ch_outer.
     // With z_overlap the Z updates run on a second stream: they read only
     // finished columns of L (the panel's own, after its panel product), so
     // each waits for the event recorded after that panel product, and the
@@ -863,8 +864,8 @@ template <typename W> struct Pipeline : PipelineBase {
     const bool ov = z_overlap != 0;
     hipStream_t zs = ov ? z_stream(st) : st;
     int nev = 0;
-    for (int K0 = 0; K0 < ip; K0 += CH_OUTER) {
-      const int K1 = std::min(ip, K0 + CH_OUTER);
+    for (int K0 = 0; K0 < ip; K0 += ch_outer) {
+      const int K1 = std::min(ip, K0 + ch_outer);
       for (int k0 = K0; k0 < K1; k0 += LU_NB) {
         const int k1 = k0 + LU_NB;
         const int rest = ip - k1;
@@ -1249,6 +1250,7 @@ template <typename W> struct Pipeline : PipelineBase {
       h->oz_slices = oz_slices;
       h->oz_scope = oz_scope;
       h->z_overlap = z_overlap;
+      h->ch_outer = ch_outer;
       h->front_pade<Tin>(d_a, P.structured, P.m, P.mu_re, P.mu_im, scale, st);
       if (timing) FX_HIP(hipEventRecord(ev[2], st));
       const int dh = h->solve(P.structured, 0, st);
@@ -1851,6 +1853,13 @@ int fexpm_set_option(fexpm_engine* e, int key, int64_t value) {
       return FEXPM_OK;
     case FEXPM_OPT_Z_OVERLAP:
       if (e->pipe) e->pipe->z_overlap = value ? 1 : 0;
+      return FEXPM_OK;
+    case FEXPM_OPT_CHOL_OUTER:
+      if (value < 64 || value > 4096 || value % 64 != 0) {
+        e->err = "Cholesky outer block must be a multiple of 64 in [64, 4096]";
+        return FEXPM_EINVAL;
+      }
+      if (e->pipe) e->pipe->ch_outer = (int)value;
       return FEXPM_OK;
     case FEXPM_OPT_FP64_SLICES_SCOPE:
       if (value < 1 || value > 3) {

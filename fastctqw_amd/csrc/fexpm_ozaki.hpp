@@ -196,7 +196,7 @@ __device__ __forceinline__ void oz_tile(const OzArgs<S>& g, int tm, int tn, bool
 #pragma unroll
     for (int u = 0; u < CPT; ++u) {
       const int c = threadIdx.x + NT * u;
-      if (c < CH) {
+      if (CH % NT == 0 || c < CH) {  // no guard where the chunks divide evenly
         ra[u] = pa[c];
         rb[u] = pb[c];
       }
@@ -206,7 +206,7 @@ __device__ __forceinline__ void oz_tile(const OzArgs<S>& g, int tm, int tn, bool
 #pragma unroll
     for (int u = 0; u < CPT; ++u) {
       const int c = threadIdx.x + NT * u;
-      if (c < CH) {
+      if (CH % NT == 0 || c < CH) {
         const int off = (c / (2 * S)) * RSTR + (c % (2 * S)) * 16;
         *reinterpret_cast<i32x4*>(s + off) = ra[u];
         *reinterpret_cast<i32x4*>(s + LSTAGE + off) = rb[u];

@@ -44,6 +44,12 @@ enum { FEXPM_OPT_TAIL_SPLIT = 103 };
  * one stream.  Both give the same values bit for bit. */
 enum { FEXPM_OPT_Z_OVERLAP = 104 };
 
+/* Engine option: width of the outer blocks of the structured Cholesky solve
+ * (a multiple of 64 in [64, 4096], default 256): 64-wide panels update only
+ * their outer block, which then updates the trailing matrix and the
+ * remaining columns of Z with K equal to this width. */
+enum { FEXPM_OPT_CHOL_OUTER = 105 };
+
 /* Internal planes readable by fexpm_debug_plane (working precision, n x n). */
 enum fexpm_plane {
   FEXPM_PLANE_A = 0,   /* (A - mu I) / 2^s; structured: its imaginary part G */
