@@ -30,6 +30,7 @@ if ROOT not in sys.path:
 FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
 FP16_PEAK_TFLOPS = 2500.0  # dense fp16 / bf16 MFMA
+INT8_PEAK_TOPS = 5000.0  # dense int8 MFMA
 TRAFFIC_PROFILE = "profiles/squaring_er10000_c64_counters.md"
 
 
@@ -669,13 +670,23 @@ def _mfma_rates():
     for kind, name, peak, iters in ((0, "fp32_16x16x4", FP32_PEAK_TFLOPS, 20000),
                                     (1, "fp64_16x16x4", FP64_PEAK_TFLOPS, 20000),
                                     (2, "fp16_32x32x16", FP16_PEAK_TFLOPS, 8000),
-                                    (3, "bf16_32x32x16", FP16_PEAK_TFLOPS, 8000)):
+                                    (3, "bf16_32x32x16", FP16_PEAK_TFLOPS, 8000),
+                                    (4, "int8_32x32x32", INT8_PEAK_TOPS, 4000),
+                                    (5, "fp32_32x32x2", FP32_PEAK_TFLOPS, 20000)):
         tf, ms = ctypes.c_double(0.0), ctypes.c_double(0.0)
         rc = lib.fexpm_test_mfma_rate(kind, iters, ctypes.byref(tf), ctypes.byref(ms))
         if rc != 0:
             raise RuntimeError(lib.fexpm_last_error(None).decode())
         out[name] = {"tflops": tf.value, "datasheet_tflops": peak, "fraction_of_datasheet": tf.value / peak,
                      "kernel_ms": ms.value}
+        # the same MFMA stream at 1, 2 and 4 waves per SIMD
+        by_waves = {}
+        for w in (1, 2, 4):
+            rc = lib.fexpm_test_mfma_rate_waves(kind, w, iters, ctypes.byref(tf), ctypes.byref(ms))
+            if rc != 0:
+                raise RuntimeError(lib.fexpm_last_error(None).decode())
+            by_waves[w] = tf.value
+        out[name]["tflops_by_waves_per_simd"] = by_waves
     # the GEMM kernels at the N = 10^4 shapes, issued MFMA flops per launch time
     gem = {}
     n, kp = 10000, 10112
@@ -691,7 +702,8 @@ def _mfma_rates():
                                            ("fp64_c3m_sym_64x64_8waves", 1, 2, 1, 64, 3),
                                            ("fp64_c3m_sym_64x64_4waves", 1, 2, 0, 64, 3),
                                            ("fp64_real_sym_64x64_8waves_pad4", 1, 0, 4, 64, 1),
-                                           ("fp64_c3m_sym_64x64_8waves_pad4", 1, 2, 4, 64, 3)):
+                                           ("fp64_c3m_sym_64x64_8waves_pad4", 1, 2, 4, 64, 3),
+                                           ("int8_slices6_sym_128x128_8waves", 4, 0, 6, 128, 21)):
         ms = ctypes.c_double(0.0)
         rc = lib.fexpm_test_gemm_time(w, mode, 1, cfg, n, n, kp, 3, ctypes.byref(ms))
         if rc != 0:

@@ -126,7 +126,7 @@ struct PipelineBase {
   int oz_scope = 3;        // stages using them: bit 0 the Pade products, bit 1 the solve (S, X = Z Z^T)
   int tail_split = 1;      // FP32 squarings: last partial wave of 128-tiles as 64-tiles
   int z_overlap = 1;       // Cholesky solve: Z updates on a second stream
-  int ch_outer = 256;      // Cholesky solve: outer block width (a multiple of 64)
+  int ch_outer = 512;      // Cholesky solve: outer block width (a multiple of 64)
 };
 
 // ---------------------------------------------------------------------------
@@ -2014,6 +2014,7 @@ int fexpm_batch_create(int dtype, int64_t nmax, int lanes, fexpm_batch** out) {
       else
         p = new Pipeline<double>((int)nmax, dtype);
       p->panel_coop = 0;  // lanes run concurrently: no cooperative launches
+      p->z_overlap = 0;   // nor second streams (lanes already overlap one another)
       b->pipes.emplace_back(p);
     }
     if (dtype == FEXPM_COMPLEX64)
@@ -2953,7 +2954,8 @@ __global__ __launch_bounds__(256) void copy_kernel(const u32x4* __restrict__ src
 // independent MFMAs (no memory traffic in the loop), so the rate is the
 // sustained matrix-core rate of the device at its running clock.
 // kind 0: v_mfma_f32_16x16x4f32, 1: v_mfma_f64_16x16x4f64,
-// 2: v_mfma_f32_32x32x16_f16, 3: v_mfma_f32_32x32x16_bf16.
+// 2: v_mfma_f32_32x32x16_f16, 3: v_mfma_f32_32x32x16_bf16,
+// 4: v_mfma_i32_32x32x32_i8, 5: v_mfma_f32_32x32x2f32.
 template <int KIND>
 __global__ __launch_bounds__(256) void mfma_rate_kernel(float* out, int iters) {
   const float seed = (float)(threadIdx.x & 7) * 1e-3f;
@@ -2978,6 +2980,27 @@ __global__ __launch_bounds__(256) void mfma_rate_kernel(float* out, int iters) {
       for (int q = 0; q < 8; ++q) acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[q], 0, 0, 0);
 #pragma unroll
     for (int q = 0; q < 8; ++q) keep += (float)acc[q][0];
+  } else if constexpr (KIND == 4) {
+    const int v = 0x01010101 * (1 + (int)(threadIdx.x & 3));
+    const i32x4 a = {v, v, v, v}, b = {v, -v, v, -v};
+    i32x16 acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] = i32x16{};
+    for (int it = 0; it < iters; ++it)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[q] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, acc[q], 0, 0, 0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) keep += (float)acc[q][0];
+  } else if constexpr (KIND == 5) {
+    f32x16 acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] = f32x16{};
+    const float a = 1.0f + seed, b = 1.0f - seed;
+    for (int it = 0; it < iters; ++it)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc[q], 0, 0, 0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) keep += acc[q][0];
   } else {
     using V = typename std::conditional<KIND == 2, f16x8, bf16x8>::type;
     using E = typename std::conditional<KIND == 2, _Float16, __bf16>::type;
@@ -3002,15 +3025,17 @@ __global__ __launch_bounds__(256) void mfma_rate_kernel(float* out, int iters) {
   if (keep == 1234.5f) out[blockIdx.x] = keep;  // keeps the loop alive; never true for these inputs
 }
 
-extern "C" int fexpm_test_mfma_rate(int kind, int iters, double* tflops, double* ms) {
-  if (!tflops || !ms || kind < 0 || kind > 3 || iters <= 0 || iters > (1 << 20)) return FEXPM_EINVAL;
+extern "C" int fexpm_test_mfma_rate_waves(int kind, int waves, int iters, double* tflops, double* ms) {
+  if (!tflops || !ms || kind < 0 || kind > 5 || iters <= 0 || iters > (1 << 20) || waves < 1 || waves > 8)
+    return FEXPM_EINVAL;
   return guarded(nullptr, [&] {
     check_device();
     int dev = 0;
     hipDeviceProp_t prop;
     FX_HIP(hipGetDevice(&dev));
     FX_HIP(hipGetDeviceProperties(&prop, dev));
-    const unsigned grid = (unsigned)prop.multiProcessorCount * 8;  // 8 waves per SIMD
+    // workgroups of 4 waves, `waves` of them per CU: `waves` waves per SIMD
+    const unsigned grid = (unsigned)prop.multiProcessorCount * (unsigned)waves;
     float* out = nullptr;
     FX_HIP(hipMalloc(&out, grid * sizeof(float)));
     hipEvent_t e0, e1;
@@ -3021,7 +3046,9 @@ extern "C" int fexpm_test_mfma_rate(int kind, int iters, double* tflops, double*
         case 0: mfma_rate_kernel<0><<<grid, 256>>>(out, n); break;
         case 1: mfma_rate_kernel<1><<<grid, 256>>>(out, n); break;
         case 2: mfma_rate_kernel<2><<<grid, 256>>>(out, n); break;
-        default: mfma_rate_kernel<3><<<grid, 256>>>(out, n); break;
+        case 3: mfma_rate_kernel<3><<<grid, 256>>>(out, n); break;
+        case 4: mfma_rate_kernel<4><<<grid, 256>>>(out, n); break;
+        default: mfma_rate_kernel<5><<<grid, 256>>>(out, n); break;
       }
       FX_LAUNCH();
     };
@@ -3035,12 +3062,17 @@ extern "C" int fexpm_test_mfma_rate(int kind, int iters, double* tflops, double*
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipFree(out);
-    // flops per MFMA instruction (one wave): 16x16x4 -> 2048, 32x32x16 -> 32768
-    const double per = kind < 2 ? 2048.0 * 8 : 32768.0 * 4;
+    // flops per MFMA instruction (one wave): 16x16x4 -> 2048, 32x32x16 -> 32768,
+    // 32x32x32 -> 65536, 32x32x2 -> 4096
+    const double per = kind < 2 ? 2048.0 * 8 : (kind == 4 ? 65536.0 * 4 : (kind == 5 ? 4096.0 * 4 : 32768.0 * 4));
     const double flops = (double)grid * 4.0 * (double)iters * per;
     *ms = t;
     *tflops = flops / ((double)t * 1e-3) / 1e12;
   });
+}
+
+extern "C" int fexpm_test_mfma_rate(int kind, int iters, double* tflops, double* ms) {
+  return fexpm_test_mfma_rate_waves(kind, 8, iters, tflops, ms);
 }
 
 extern "C" int fexpm_test_copy(int64_t bytes, int reps, double* ms) {

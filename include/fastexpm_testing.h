@@ -45,7 +45,7 @@ enum { FEXPM_OPT_TAIL_SPLIT = 103 };
 enum { FEXPM_OPT_Z_OVERLAP = 104 };
 
 /* Engine option: width of the outer blocks of the structured Cholesky solve
- * (a multiple of 64 in [64, 4096], default 256): 64-wide panels update only
+ * (a multiple of 64 in [64, 4096], default 512): 64-wide panels update only
  * their outer block, which then updates the trailing matrix and the
  * remaining columns of Z with K equal to this width. */
 enum { FEXPM_OPT_CHOL_OUTER = 105 };
@@ -102,10 +102,16 @@ int fexpm_test_copy(int64_t bytes, int reps, double* ms);
 
 /* Sustained MFMA rate of the device: a kernel whose waves issue only
  * independent MFMAs from registers (kind 0 FP32 16x16x4, 1 FP64 16x16x4,
- * 2 FP16 32x32x16, 3 BF16 32x32x16), 8 waves per SIMD; `iters` rounds per
- * wave.  Returns TFLOP/s and the kernel time.  Measures the practical peak
+ * 2 FP16 32x32x16, 3 BF16 32x32x16, 4 INT8 32x32x32 (TOP/s), 5 FP32
+ * 32x32x2), 8 waves per SIMD; `iters` rounds per wave.  Returns TFLOP/s
+ * and the kernel time.  Measures the practical peak
  * that the GEMM kernels' rates are compared with. */
 int fexpm_test_mfma_rate(int kind, int iters, double* tflops, double* ms);
+
+/* The same probe with `waves` (1..8) waves per SIMD: the rate an MFMA stream
+ * can reach at the occupancy of a register-heavy GEMM kernel (two waves per
+ * SIMD for the squaring and int8-slice kernels). */
+int fexpm_test_mfma_rate_waves(int kind, int waves, int iters, double* tflops, double* ms);
 
 /* Grouped batch program for `count` CTQW walks with the given orders and
  * plans (Pade degree m, squarings s, trace shift), built by the same code as

@@ -211,6 +211,33 @@ def test_cholesky_z_overlap(fx, dt, n):
     assert np.array_equal(u_two, u_one) and np.array_equal(u_two, u_two2)
 
 
+@pytest.mark.parametrize("dt,n", [(np.complex64, 700), (np.complex128, 330)])
+def test_cholesky_outer_width(fx, dt, n):
+    """Outer block widths of the Cholesky solve (64 .. 1024, wider than the
+    order included) change only the FP64 summation order: every width agrees
+    with the default to FP64 rounding carried through the squarings, and
+    with the eigendecomposition oracle; invalid widths are rejected."""
+    from fastctqw_amd.fastexpm import _fastexpm_core as core
+
+    _say(f"cholesky outer width {np.dtype(dt).name} n={n}")
+    L = laplacian(erdos_renyi(n, 0.3, seed=90 + n, weighted=True))
+    a = (-1j * 6.0 * L).astype(dt)
+    lam, V = np.linalg.eigh(np.ascontiguousarray(a.imag, dtype=np.float64))
+    psi = (V * np.exp(1j * lam)[None, :]) @ (V.T[:, :1])
+    with _engine(fx, dt, n) as e:
+        u_def = e.run(a)
+        s = e.stats()["s"]
+        for w in (64, 256, 1024, 512):
+            e._backend.set_option(core.OPT_CHOL_OUTER, w)
+            u = e.run(a)
+            assert rel_fro(u, u_def) <= (2e-6 if dt == np.complex64 else 1e-12), w
+            err = float(np.linalg.norm(psi[:, 0] - u[:, 0]) / np.linalg.norm(psi))
+            assert err <= _tol(dt, s), (w, err)
+        for bad in (0, 96, 8192):
+            with pytest.raises(ValueError):
+                e._backend.set_option(core.OPT_CHOL_OUTER, bad)
+
+
 def test_fp64_slices_option_errors(fx):
     from fastctqw_amd.fastexpm import _fastexpm_core as core
 
