@@ -697,6 +697,9 @@ def _mfma_rates():
                                            ("fp32_c3m_sym_128x128_8waves_32x64", 0, 2, 6, 128, 3),
                                            ("fp32_c3m_sym_128x128_8waves_pad12", 0, 2, 7, 128, 3),
                                            ("fp32_c3m_sym_128x128_8waves_32x64_pad8", 0, 2, 8, 128, 3),
+                                           ("fp32_c3m_sym_128x128_8waves_mfma16x16x4", 0, 2, 9, 128, 3),
+                                           ("fp32_c3m_sym_128x128_8waves_pad8_mfma32", 0, 2, 10, 128, 3),
+                                           ("fp32_c3m_sym_128x128_8waves_32x64_mfma32", 0, 2, 11, 128, 3),
                                            ("fp64_real_sym_64x64_8waves", 1, 0, 1, 64, 1),
                                            ("fp64_real_sym_64x64_4waves", 1, 0, 0, 64, 1),
                                            ("fp64_c3m_sym_64x64_8waves", 1, 2, 1, 64, 3),

@@ -1131,8 +1131,10 @@ template <typename W> struct Pipeline : PipelineBase {
       // FP32 orders up to 4096: 64 x 64 tiles (measured 2.3x faster up to
       // n = 1536, equal at 2048, 10-20% faster at 3072-4096, slower beyond)
       const bool small = std::is_same<W, float>::value && n <= 4096;
+      // (FCfgSmall runs 32x32x2 FP32 MFMAs; FP64 engines never take this branch)
+      using SmallC = typename std::conditional<std::is_same<W, float>::value, FCfgSmall, GemmCfg<W>>::type;
       if (structured) {
-        if (blk && small) gemm<MODE_C3M, true, true, FCfgSmall>(g, st);
+        if (blk && small) gemm<MODE_C3M, true, true, SmallC>(g, st);
         else if (blk) square_big_tail(g, st);
         else gemm<MODE_C3M, true>(g, st);
       } else {
@@ -3130,8 +3132,14 @@ extern "C" int fexpm_test_gemm_time(int wtype, int mode, int sym, int cfg, int m
         else if (lay == 2) t = time_split_t<Bf16x3, 4, 2>(m, k, reps, grp);
       }
     } else if (wtype == 0) {
-      // squaring tilings: 0 default, 4 FCfgSmall, 5..8 LDS padding / wave-tile variants of the default
+      // squaring tilings: 0 default (32x32x2 MFMAs), 4 FCfgSmall, 5..8 LDS padding / wave-tile variants on
+      // 16x16x4 MFMAs, 9 the default tiling on 16x16x4, 10 / 11 padding / wave-tile variants on 32x32x2
       if (mode == 2 && sym && cfg == 4) t = time_gemm_t<float, MODE_C3M, true, true, FCfgSmall>(m, n, k, reps);
+      else if (mode == 2 && sym && cfg == 9) t = time_gemm_t<float, MODE_C3M, true, true, FCfg16>(m, n, k, reps);
+      else if (mode == 2 && sym && cfg == 10)
+        t = time_gemm_t<float, MODE_C3M, true, true, TileCfg<128, 128, 16, 8, 64, 32, 32>>(m, n, k, reps);
+      else if (mode == 2 && sym && cfg == 11)
+        t = time_gemm_t<float, MODE_C3M, true, true, TileCfg<128, 128, 16, 4, 32, 64, 32>>(m, n, k, reps);
       else if (mode == 2 && sym && cfg == 5)
         t = time_gemm_t<float, MODE_C3M, true, true, TileCfg<128, 128, 16, 8, 64, 32>>(m, n, k, reps);
       else if (mode == 2 && sym && cfg == 6)

@@ -31,10 +31,13 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace fexpm {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 template <typename T> struct AccT;
 template <> struct AccT<float> { using type = f32x4; };
@@ -50,6 +53,15 @@ __device__ __forceinline__ f32x4 mfma16(float a, float b, f32x4 c) {
 __device__ __forceinline__ f64x4 mfma16(double a, double b, f64x4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
+// 32x32 FP32 blocks: lane l supplies A[l%32][k] and B^T[l%32][k] with
+// k = l/32 of the MFMA's two k values; accumulator element r of lane l holds
+// column l%32 and row 8*(r/4) + 4*(l/32) + r%4 of the 32x32 block.  Same
+// issue rate per flop as 16x16x4 on paper, measured faster at two waves per
+// SIMD (151.9 vs 134.1 TFLOP/s in the register probe).
+__device__ __forceinline__ f32x16 mfma32(float a, float b, f32x16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ int acc_row32(int lane, int r) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
 
 enum GemmMode { MODE_REAL = 0, MODE_C4M = 1, MODE_C3M = 2 };
 
@@ -99,16 +111,19 @@ template <typename T> struct GemmArgs {
 };
 
 // Tile configuration: workgroup tile BM x BN, k-tile BK, LDS row padding
-// PAD, wave tile WM x WN; NT threads.
-template <int BM_, int BN_, int BK_, int PAD_, int WM_, int WN_> struct TileCfg {
-  static constexpr int BM = BM_, BN = BN_, BK = BK_, PAD = PAD_, WM = WM_, WN = WN_;
+// PAD, wave tile WM x WN, MFMA block edge MF (16: 16x16x4; 32: FP32
+// 32x32x2, BK = 16); NT threads.
+template <int BM_, int BN_, int BK_, int PAD_, int WM_, int WN_, int MF_ = 16> struct TileCfg {
+  static constexpr int BM = BM_, BN = BN_, BK = BK_, PAD = PAD_, WM = WM_, WN = WN_, MF = MF_;
   static constexpr int NT = (BM / WM) * (BN / WN) * 64;
 };
 template <typename T> struct GemmCfg;
-template <> struct GemmCfg<float> : TileCfg<128, 128, 16, 4, 64, 32> {};
+template <> struct GemmCfg<float> : TileCfg<128, 128, 16, 4, 64, 32, 32> {};
+// The same tiling on 16x16x4 MFMAs (the earlier default; timing comparisons).
+using FCfg16 = TileCfg<128, 128, 16, 4, 64, 32, 16>;
 // FP32 squarings of small orders: 4x the workgroups of the default tile
 // (a 128-wide tile grid of a few hundred rows leaves most CUs idle).
-using FCfgSmall = TileCfg<64, 64, 16, 4, 32, 32>;
+using FCfgSmall = TileCfg<64, 64, 16, 4, 32, 32, 32>;
 // FP64: 8 waves of 32x16 keep every variant within 256 registers (two
 // waves per SIMD); measured 63 TFLOP/s (4M, K = 256) and 61 TFLOP/s (real
 // symmetric, K = 10^4) on MI355X versus 50 / 57 for 4 waves of 32x32.
@@ -197,22 +212,29 @@ __device__ __forceinline__ int xcd_remap(int b, int nblocks) {
 template <typename T, int MODE, bool KBLK, class CFG>
 __device__ __forceinline__ void gemm_tile(const GemmArgs<T>& g, int tm, int tn, bool sym) {
   using C = CFG;
-  constexpr int BM = C::BM, BN = C::BN, BK = C::BK, PAD = C::PAD, WM = C::WM, WN = C::WN;
+  constexpr int BM = C::BM, BN = C::BN, BK = C::BK, PAD = C::PAD, WM = C::WM, WN = C::WN, MF = C::MF;
+  static_assert(MF == 16 || (MF == 32 && sizeof(T) == 4 && BK == 16), "32x32 blocks: FP32, BK = 16");
   constexpr int WAVES_N = BN / WN;
   constexpr int NT = (BM / WM) * WAVES_N * 64;
-  constexpr int FI = WM / 16, FJ = WN / 16;
+  constexpr int FI = WM / MF, FJ = WN / MF;
+  constexpr int NRD = MF == 32 ? 2 : 1;  // 16-byte fragment reads per row and k-tile
+  constexpr int NS = 4 * NRD;            // MFMA steps per k-tile
+  constexpr int NEL = MF == 32 ? 16 : 4;  // accumulator elements per lane
   constexpr int NPL = (MODE == MODE_REAL) ? 1 : 2;
   constexpr int NACC = (MODE == MODE_REAL) ? 1 : (MODE == MODE_C4M ? 2 : 3);
   constexpr int TA = BM * (BK + PAD), TB = BN * (BK + PAD);
-  using Acc = typename AccT<T>::type;
+  using Acc = typename std::conditional<MF == 32, f32x16, typename AccT<T>::type>::type;
 
   __shared__ __attribute__((aligned(16))) T smem[2][NPL * (TA + TB)];
 
   const int row0 = tm * BM, col0 = tn * BN;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wr = (wave / WAVES_N) * WM, wc = (wave % WAVES_N) * WN;
-  const int kq = 4 * (lane >> 4);
-  const int l16 = lane & 15;
+  // 16x16x4: lane l reads k 4*(l/16) .. +3 of row l%16 (four steps);
+  // 32x32x2: lane l reads k 4*(l/32) .. +3 and 8 + the same of row l%32
+  // (eight steps; lanes below 32 give the first k of each step)
+  const int kq = MF == 32 ? 4 * (lane >> 5) : 4 * (lane >> 4);
+  const int lrow = lane & (MF - 1);
 
   Acc acc[NACC][FI][FJ];
 #pragma unroll
@@ -260,33 +282,40 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs<T>& g, int tm, int tn, 
         sb[p].load(pb[p], g.ldb, nvalid, (kt + 1) * BK);
       }
     }
-    T fa[NPL][FI][4], fb[NPL][FJ][4];
+    T fa[NPL][FI][NS], fb[NPL][FJ][NS];
 #pragma unroll
     for (int p = 0; p < NPL; ++p) {
       const T* la = &smem[cur][p * (TA + TB)];
       const T* lb = la + TA;
 #pragma unroll
-      for (int i = 0; i < FI; ++i) frag4<BK, PAD>(la, wr + 16 * i + l16, kq, fa[p][i]);
+      for (int q = 0; q < NRD; ++q) {
 #pragma unroll
-      for (int j = 0; j < FJ; ++j) frag4<BK, PAD>(lb, wc + 16 * j + l16, kq, fb[p][j]);
+        for (int i = 0; i < FI; ++i) frag4<BK, PAD>(la, wr + MF * i + lrow, kq + 8 * q, &fa[p][i][4 * q]);
+#pragma unroll
+        for (int j = 0; j < FJ; ++j) frag4<BK, PAD>(lb, wc + MF * j + lrow, kq + 8 * q, &fb[p][j][4 * q]);
+      }
     }
+    auto mfma = [](T a, T b, Acc c) -> Acc {
+      if constexpr (MF == 32) return mfma32(a, b, c);
+      else return mfma16(a, b, c);
+    };
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
+    for (int s = 0; s < NS; ++s) {
 #pragma unroll
       for (int i = 0; i < FI; ++i) {
 #pragma unroll
         for (int j = 0; j < FJ; ++j) {
           if constexpr (MODE == MODE_REAL) {
-            acc[0][i][j] = mfma16(fa[0][i][s], fb[0][j][s], acc[0][i][j]);
+            acc[0][i][j] = mfma(fa[0][i][s], fb[0][j][s], acc[0][i][j]);
           } else if constexpr (MODE == MODE_C4M) {
-            acc[0][i][j] = mfma16(fa[0][i][s], fb[0][j][s], acc[0][i][j]);
-            acc[0][i][j] = mfma16(-fa[1][i][s], fb[1][j][s], acc[0][i][j]);
-            acc[1][i][j] = mfma16(fa[0][i][s], fb[1][j][s], acc[1][i][j]);
-            acc[1][i][j] = mfma16(fa[1][i][s], fb[0][j][s], acc[1][i][j]);
+            acc[0][i][j] = mfma(fa[0][i][s], fb[0][j][s], acc[0][i][j]);
+            acc[0][i][j] = mfma(-fa[1][i][s], fb[1][j][s], acc[0][i][j]);
+            acc[1][i][j] = mfma(fa[0][i][s], fb[1][j][s], acc[1][i][j]);
+            acc[1][i][j] = mfma(fa[1][i][s], fb[0][j][s], acc[1][i][j]);
           } else {
-            acc[0][i][j] = mfma16(fa[0][i][s], fb[0][j][s], acc[0][i][j]);
-            acc[1][i][j] = mfma16(fa[1][i][s], fb[1][j][s], acc[1][i][j]);
-            acc[2][i][j] = mfma16(fa[0][i][s] + fa[1][i][s], fb[0][j][s] + fb[1][j][s], acc[2][i][j]);
+            acc[0][i][j] = mfma(fa[0][i][s], fb[0][j][s], acc[0][i][j]);
+            acc[1][i][j] = mfma(fa[1][i][s], fb[1][j][s], acc[1][i][j]);
+            acc[2][i][j] = mfma(fa[0][i][s] + fa[1][i][s], fb[0][j][s] + fb[1][j][s], acc[2][i][j]);
           }
         }
       }
@@ -325,9 +354,9 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs<T>& g, int tm, int tn, 
 #pragma unroll
     for (int j = 0; j < FJ; ++j) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = row0 + wr + 16 * i + acc_row<T>(lane, r);
-        const int col = col0 + wc + 16 * j + l16;
+      for (int r = 0; r < NEL; ++r) {
+        const int row = row0 + wr + MF * i + (MF == 32 ? acc_row32(lane, r) : acc_row<T>(lane, r));
+        const int col = col0 + wc + MF * j + lrow;
         if (row >= g.m || col >= g.n) continue;
         if (diag_tile && row < col) continue;
         double vr, vi = 0.0;
