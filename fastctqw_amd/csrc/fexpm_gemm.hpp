@@ -20,13 +20,16 @@
 //              X = Xr + i Xi with Xr, Xi symmetric costs 3 half products).
 //
 // Tiles: a workgroup computes BM x BN outputs with (BM/WM)*(BN/WN) waves of
-// 64 lanes; every wave owns a WM x WN block made of 16x16 MFMA blocks
-// (v_mfma_{f32,f64}_16x16x4).  Operand tiles of BK=16 columns are staged
+// 64 lanes; every wave owns a WM x WN block made of MF x MF MFMA blocks:
+// 16x16 (v_mfma_{f32,f64}_16x16x4, MF = 16) or, for FP32 by default, 32x32
+// (v_mfma_f32_32x32x2, MF = 32).  Operand tiles of BK=16 columns are staged
 // global -> registers -> LDS with double buffering; rows keep k contiguous
 // so one 16-byte LDS read feeds four MFMA k-steps: lane l of a 16x16x4 MFMA
-// supplies A[l%16][k] with k = 4*(l/16) + step, and the same permutation of
-// k is used for B, so the products pair correctly.  Accumulator element r
-// of lane l holds column l%16 and row acc_row<T>(l, r) of the 16x16 block.
+// supplies A[l%16][k] with k = 4*(l/16) + step (of a 32x32x2 one A[l%32][k]
+// with k = 4*(l/32) + step, two reads per k-tile), and the same permutation
+// of k is used for B, so the products pair correctly.  Accumulator element r
+// of lane l holds column l%16 and row acc_row<T>(l, r) of a 16x16 block,
+// column l%32 and row acc_row32(l, r) of a 32x32 block.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
