@@ -190,6 +190,41 @@ __device__ __forceinline__ void tri_tile(int b, int& ti, int& tj) {
   tj = b - i * (i + 1) / 2;
 }
 
+// Lower-triangular tile index b -> (ti, tj) in groups of TILE_GROUP x
+// TILE_GROUP tiles: group rows gi in order, within one the groups gj <= gi,
+// within a group its tiles row-major (tj <= ti in the diagonal group).  The
+// same enumeration as sym_tile_order on the host, computed in place.  The
+// tiles that run together on one XCD (consecutive b after xcd_remap) then
+// share both the row blocks of A and those of B in that XCD's L2.
+constexpr int TILE_GROUP = 8;
+__host__ __device__ inline void grouped_tri_tile(int b, int T, int& ti, int& tj) {
+  constexpr int G = TILE_GROUP;
+  int base = 0;
+  for (int gi = 0; gi * G < T; ++gi) {
+    const int h = T - gi * G < G ? T - gi * G : G;  // tile rows in this group row
+    const int full = gi * h * G;                    // tiles of its off-diagonal groups
+    const int cnt = full + h * (h + 1) / 2;
+    if (b < base + cnt) {
+      const int r = b - base;
+      if (r < full) {
+        const int gj = r / (h * G), w = r % (h * G);
+        ti = gi * G + w / G;
+        tj = gj * G + w % G;
+      } else {
+        const int w = r - full;
+        int t = 0;
+        while ((t + 1) * (t + 2) / 2 <= w) ++t;
+        ti = gi * G + t;
+        tj = gi * G + (w - t * (t + 1) / 2);
+      }
+      return;
+    }
+    base += cnt;
+  }
+  ti = 0;  // b beyond T(T+1)/2 (never launched)
+  tj = 0;
+}
+
 // Spread consecutive tiles over the 8 XCDs so that workgroups that run on
 // one XCD (dispatched round-robin: block b on XCD b % 8) work on
 // neighbouring tiles and share their operand rows in that XCD's L2.
@@ -427,7 +462,7 @@ void gemm_nt_kernel(GemmArgs<T> g) {
   int tm, tn;
   if (SYM) {
     const int b = xcd_remap((int)blockIdx.x, (int)gridDim.x);
-    tri_tile(b, tm, tn);
+    grouped_tri_tile(b, (g.m + CFG::BM - 1) / CFG::BM, tm, tn);
   } else {
     const int nbn = (int)gridDim.x;
     const int lin = xcd_remap((int)(blockIdx.y * gridDim.x + blockIdx.x), (int)(gridDim.x * gridDim.y));
@@ -448,7 +483,7 @@ __global__ __launch_bounds__(CFG::NT)
 void gemm_sym_tail_kernel(GemmArgs<T> g, int t0) {
   const int q = (int)blockIdx.x;
   int tm, tn;
-  tri_tile(t0 + q / 4, tm, tn);
+  grouped_tri_tile(t0 + q / 4, (g.m + 2 * CFG::BM - 1) / (2 * CFG::BM), tm, tn);
   const int sm = 2 * tm + ((q & 3) >> 1), sn = 2 * tn + (q & 1);
   if (sn > sm) return;  // upper sub-tile of a diagonal tile
   gemm_tile<T, MODE, KBLK, CFG>(g, sm, sn, true);
