@@ -3073,6 +3073,19 @@ extern "C" int fexpm_test_mfma_rate_waves(int kind, int waves, int iters, double
   });
 }
 
+// The tile order of the symmetric GEMM launches, evaluated on the host:
+// out[b] = (ti << 16) | tj for b < T(T+1)/2 (no device needed).
+extern "C" int fexpm_test_grouped_tile_order(int T, int* out) {
+  if (!out || T < 1 || T > 4096) return FEXPM_EINVAL;
+  const int nb = T * (T + 1) / 2;
+  for (int b = 0; b < nb; ++b) {
+    int ti = 0, tj = 0;
+    grouped_tri_tile(b, T, ti, tj);
+    out[b] = (ti << 16) | tj;
+  }
+  return FEXPM_OK;
+}
+
 extern "C" int fexpm_test_mfma_rate(int kind, int iters, double* tflops, double* ms) {
   return fexpm_test_mfma_rate_waves(kind, 8, iters, tflops, ms);
 }

@@ -113,6 +113,12 @@ int fexpm_test_mfma_rate(int kind, int iters, double* tflops, double* ms);
  * SIMD for the squaring and int8-slice kernels). */
 int fexpm_test_mfma_rate_waves(int kind, int waves, int iters, double* tflops, double* ms);
 
+/* Tile order of the symmetric (lower-triangular) GEMM launches, evaluated on
+ * the host: out[b] = (ti << 16) | tj for every b < T(T+1)/2, 1 <= T <= 4096.
+ * Groups of 8 x 8 tiles, group rows in order, within one the groups left to
+ * right, within a group row-major with tj <= ti. */
+int fexpm_test_grouped_tile_order(int T, int* out);
+
 /* Grouped batch program for `count` CTQW walks with the given orders and
  * plans (Pade degree m, squarings s, trace shift), built by the same code as
  * a grouped batch run but on a virtual address layout (no device memory is

@@ -102,6 +102,33 @@ def test_library_no_device_and_argument_errors():
     assert lib.fexpm_get_n(None) == -1
 
 
+def _grouped_order(T, g=8):
+    """The 8 x 8 grouped lower-triangular enumeration, written out directly."""
+    out = []
+    for gi in range((T + g - 1) // g):
+        for gj in range(gi + 1):
+            for ti in range(gi * g, min(T, gi * g + g)):
+                for tj in range(gj * g, min(gj * g + g, ti + 1)):
+                    out.append((ti, tj))
+    return out
+
+
+@pytest.mark.parametrize("T", [1, 2, 7, 8, 9, 16, 17, 79, 158, 159])
+def test_grouped_tile_order(T):
+    """The symmetric GEMM launches visit every lower tile exactly once, in
+    the grouped order (evaluated on the host through the testing ABI)."""
+    if not os.path.exists(LIB):
+        pytest.skip("HIP library not built")
+    lib = ctypes.CDLL(LIB)
+    nb = T * (T + 1) // 2
+    buf = (ctypes.c_int * nb)()
+    assert lib.fexpm_test_grouped_tile_order(T, buf) == 0
+    got = [(v >> 16, v & 0xFFFF) for v in buf]
+    assert got == _grouped_order(T)
+    assert sorted(got) == sorted((i, j) for i in range(T) for j in range(i + 1))
+    assert lib.fexpm_test_grouped_tile_order(0, buf) == -1
+
+
 # --------------------------------------------------------------------------- wrappers
 
 class _FakeBackend:
