@@ -3086,6 +3086,18 @@ extern "C" int fexpm_test_grouped_tile_order(int T, int* out) {
   return FEXPM_OK;
 }
 
+// The tile order of the rectangular GEMM launches (nbm x nbn tiles),
+// evaluated on the host: out[lin] = (ti << 16) | tj.
+extern "C" int fexpm_test_grouped_rect_order(int nbm, int nbn, int* out) {
+  if (!out || nbm < 1 || nbn < 1 || nbm > 4096 || nbn > 4096) return FEXPM_EINVAL;
+  for (int lin = 0; lin < nbm * nbn; ++lin) {
+    int ti = 0, tj = 0;
+    grouped_rect_tile(lin, nbm, nbn, ti, tj);
+    out[lin] = (ti << 16) | tj;
+  }
+  return FEXPM_OK;
+}
+
 extern "C" int fexpm_test_mfma_rate(int kind, int iters, double* tflops, double* ms) {
   return fexpm_test_mfma_rate_waves(kind, 8, iters, tflops, ms);
 }

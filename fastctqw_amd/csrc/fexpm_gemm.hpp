@@ -225,6 +225,19 @@ __host__ __device__ inline void grouped_tri_tile(int b, int T, int& ti, int& tj)
   tj = 0;
 }
 
+// Rectangular tile grid (nbm x nbn) in groups of TILE_GROUP tile rows: a
+// group sweeps its columns, visiting its rows for each column, so
+// consecutive tiles share the row blocks of B as well as those of A.
+__host__ __device__ inline void grouped_rect_tile(int lin, int nbm, int nbn, int& ti, int& tj) {
+  constexpr int G = TILE_GROUP;
+  const int per = G * nbn;
+  const int first = (lin / per) * G;
+  const int h = nbm - first < G ? nbm - first : G;
+  const int r = lin % per;
+  ti = first + r % h;
+  tj = r / h;
+}
+
 // Spread consecutive tiles over the 8 XCDs so that workgroups that run on
 // one XCD (dispatched round-robin: block b on XCD b % 8) work on
 // neighbouring tiles and share their operand rows in that XCD's L2.
@@ -464,10 +477,8 @@ void gemm_nt_kernel(GemmArgs<T> g) {
     const int b = xcd_remap((int)blockIdx.x, (int)gridDim.x);
     grouped_tri_tile(b, (g.m + CFG::BM - 1) / CFG::BM, tm, tn);
   } else {
-    const int nbn = (int)gridDim.x;
     const int lin = xcd_remap((int)(blockIdx.y * gridDim.x + blockIdx.x), (int)(gridDim.x * gridDim.y));
-    tm = lin / nbn;
-    tn = lin % nbn;
+    grouped_rect_tile(lin, (int)gridDim.y, (int)gridDim.x, tm, tn);
   }
   gemm_tile<T, MODE, KBLK, CFG>(g, tm, tn, SYM);
 }

@@ -119,6 +119,11 @@ int fexpm_test_mfma_rate_waves(int kind, int waves, int iters, double* tflops, d
  * right, within a group row-major with tj <= ti. */
 int fexpm_test_grouped_tile_order(int T, int* out);
 
+/* Tile order of the rectangular GEMM launches (nbm x nbn tiles, each at
+ * most 4096), evaluated on the host: out[lin] = (ti << 16) | tj.  Groups of
+ * 8 tile rows; a group sweeps its columns, visiting its rows per column. */
+int fexpm_test_grouped_rect_order(int nbm, int nbn, int* out);
+
 /* Grouped batch program for `count` CTQW walks with the given orders and
  * plans (Pade degree m, squarings s, trace shift), built by the same code as
  * a grouped batch run but on a virtual address layout (no device memory is

@@ -129,6 +129,21 @@ def test_grouped_tile_order(T):
     assert lib.fexpm_test_grouped_tile_order(0, buf) == -1
 
 
+@pytest.mark.parametrize("nbm,nbn", [(1, 1), (1, 5), (5, 1), (8, 3), (9, 4), (17, 154), (316, 8)])
+def test_grouped_rect_order(nbm, nbn):
+    """Rectangular GEMM launches visit every tile once, 8 tile rows at a
+    time, each group column by column."""
+    if not os.path.exists(LIB):
+        pytest.skip("HIP library not built")
+    lib = ctypes.CDLL(LIB)
+    buf = (ctypes.c_int * (nbm * nbn))()
+    assert lib.fexpm_test_grouped_rect_order(nbm, nbn, buf) == 0
+    got = [(v >> 16, v & 0xFFFF) for v in buf]
+    want = [(i, j) for g0 in range(0, nbm, 8) for j in range(nbn) for i in range(g0, min(nbm, g0 + 8))]
+    assert got == want
+    assert sorted(got) == [(i, j) for i in range(nbm) for j in range(nbn)]
+
+
 # --------------------------------------------------------------------------- wrappers
 
 class _FakeBackend:
