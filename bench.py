@@ -909,7 +909,7 @@ def main():
         traffic_key = f"{args.precision}_squaring"
     else:
         peak = FP32_PEAK_TFLOPS if cdt == np.complex64 else FP64_PEAK_TFLOPS
-        issued = 3 * 2.0 * (T * (T + 1) // 2) * 128 * 128 * np_pad if st["structured"] else 3 * 2.0 * np_pad ** 3
+        issued = 3 * 2.0 * (T * (T + 1) // 2) * 128 * 128 * kpad if st["structured"] else 3 * 2.0 * np_pad ** 3
         kname = ("gemm_nt_kernel (3M symmetric squaring; above N=4096 each squaring is a launch of 128x128 tiles "
                  "plus, when the last wave of tiles would be under half full, a launch of the rest as 64x64 "
                  "sub-tiles (gemm_sym_tail_kernel); avg_launch_ms covers both)")

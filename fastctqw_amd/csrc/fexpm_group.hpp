@@ -1600,7 +1600,7 @@ template <typename Tin, typename W> struct GroupRunner : GroupRunnerBase {
         g.ldb = np_;
         g.m = n;
         g.n = n;
-        g.k = np_;
+        g.k = (int)round_up(n, 16);  // the padding columns are zero
         g.kblock = kblock > 0 ? kblock : (1 << 30);
         g.epi.cr = wp(i, nb, 0);
         g.epi.ci = wp(i, nb, 1);
