@@ -1450,7 +1450,14 @@ template <typename W> struct Pipeline : PipelineBase {
       g.epi.ldc = ld;
       g.epi.ld_add = ld;
       g.kblock = kblock > 0 ? kblock : 1 << 20;
-      if (std::is_same<W, float>::value)
+      // A few hundred states against n columns make few 128 x 128 tiles
+      // (128 x 8192: 64 for 256 CUs): 64 x 64 tiles then (same k order and
+      // accumulation blocks per element, so the same values).
+      using SmallC = typename std::conditional<std::is_same<W, float>::value, FCfgSmall, GemmCfg<W>>::type;
+      const int tiles128 = ((nstates + 127) / 128) * ((n + 127) / 128);
+      if (std::is_same<W, float>::value && tiles128 < 512)
+        gemm<MODE_C3M, false, true, SmallC>(g, st);
+      else if (std::is_same<W, float>::value)
         gemm<MODE_C3M, false, true>(g, st);
       else
         gemm<MODE_C3M, false>(g, st);
