@@ -3200,6 +3200,7 @@ extern "C" int fexpm_test_gemm_time(int wtype, int mode, int sym, int cfg, int m
       else if (cfg == 2) FX_TIME_D(DCfgTall);
       else if (cfg == 3) FX_TIME_D(DCfgWide);
       else if (cfg == 4) FX_TIME_D(DCfg8wPad4);
+      else if (cfg == 5) FX_TIME_D(DCfg8wW4);
 #undef FX_TIME_D
     }
     if (t < 0) throw Error(FEXPM_EINVAL, "unsupported gemm timing variant");

@@ -116,9 +116,10 @@ template <typename T> struct GemmArgs {
 // Tile configuration: workgroup tile BM x BN, k-tile BK, LDS row padding
 // PAD, wave tile WM x WN, MFMA block edge MF (16: 16x16x4; 32: FP32
 // 32x32x2, BK = 16); NT threads.
-template <int BM_, int BN_, int BK_, int PAD_, int WM_, int WN_, int MF_ = 16> struct TileCfg {
+template <int BM_, int BN_, int BK_, int PAD_, int WM_, int WN_, int MF_ = 16, int MINW_ = 1> struct TileCfg {
   static constexpr int BM = BM_, BN = BN_, BK = BK_, PAD = PAD_, WM = WM_, WN = WN_, MF = MF_;
   static constexpr int NT = (BM / WM) * (BN / WN) * 64;
+  static constexpr int MINW = MINW_;  // waves per SIMD the register budget must allow
 };
 template <typename T> struct GemmCfg;
 template <> struct GemmCfg<float> : TileCfg<128, 128, 16, 4, 64, 32, 32> {};
@@ -127,16 +128,20 @@ using FCfg16 = TileCfg<128, 128, 16, 4, 64, 32, 16>;
 // FP32 squarings of small orders: 4x the workgroups of the default tile
 // (a 128-wide tile grid of a few hundred rows leaves most CUs idle).
 using FCfgSmall = TileCfg<64, 64, 16, 4, 32, 32, 32>;
-// FP64: 8 waves of 32x16 keep every variant within 256 registers (two
-// waves per SIMD); measured 63 TFLOP/s (4M, K = 256) and 61 TFLOP/s (real
-// symmetric, K = 10^4) on MI355X versus 50 / 57 for 4 waves of 32x32.
-template <> struct GemmCfg<double> : TileCfg<64, 64, 16, 2, 32, 16> {};
+// FP64: 8 waves of 32x16, compiled for four waves per SIMD (at most 128
+// VGPRs, two workgroups per CU); measured 63 TFLOP/s (4M, K = 256) and
+// 61 TFLOP/s (real symmetric, K = 10^4) on MI355X versus 50 / 57 for 4 waves
+// of 32x32.  The 3M kernel needs 134 VGPRs unconstrained (three waves per
+// SIMD, so one workgroup per CU): within 128 it runs 57.4 -> 46.9 ms per
+// n = 10^4 symmetric product.
+template <> struct GemmCfg<double> : TileCfg<64, 64, 16, 2, 32, 16, 16, 4> {};
 // Alternative FP64 tilings (selected per call site, see fexpm_test_gemm_time).
 using DCfg4w = TileCfg<64, 64, 16, 2, 32, 32>;     // 4 waves, 32x32 wave tiles
 using DCfg8w = TileCfg<64, 64, 16, 2, 32, 16>;     // 8 waves, 32x16 wave tiles
 using DCfgTall = TileCfg<128, 64, 16, 2, 32, 32>;  // 8 waves, 128x64 workgroup tile
 using DCfgWide = TileCfg<64, 128, 16, 2, 32, 32>;  // 8 waves, 64x128 workgroup tile
 using DCfg8wPad4 = TileCfg<64, 64, 16, 4, 32, 16>;  // the default with 4-double LDS row padding
+using DCfg8wW4 = TileCfg<64, 64, 16, 2, 32, 16, 16, 4>;  // the default within 128 VGPRs (4 waves per SIMD)
 
 // Staging of a ROWS x BK tile through registers, NT threads.
 template <typename T, int ROWS, int BK, int PAD, int NT> struct TileStage {
@@ -470,7 +475,7 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs<T>& g, int tm, int tn, 
 // with SYM, T(T+1)/2 lower-triangular tiles; consecutive tiles are spread
 // over the XCDs by xcd_remap.
 template <typename T, int MODE, bool SYM, bool KBLK = false, class CFG = GemmCfg<T>>
-__global__ __launch_bounds__(CFG::NT)
+__global__ __launch_bounds__(CFG::NT, CFG::MINW)
 void gemm_nt_kernel(GemmArgs<T> g) {
   int tm, tn;
   if (SYM) {
@@ -490,7 +495,7 @@ void gemm_nt_kernel(GemmArgs<T> g) {
 // workgroups on the device; every output value sees the same k order and
 // accumulation blocks as in the larger tile.
 template <typename T, int MODE, bool KBLK, class CFG>
-__global__ __launch_bounds__(CFG::NT)
+__global__ __launch_bounds__(CFG::NT, CFG::MINW)
 void gemm_sym_tail_kernel(GemmArgs<T> g, int t0) {
   const int q = (int)blockIdx.x;
   int tm, tn;
@@ -529,7 +534,7 @@ __device__ __forceinline__ int xcd_chunk_remap(int b, int total, int chunk) {
 }
 
 template <typename T, int MODE, bool KBLK, class CFG>
-__global__ __launch_bounds__(CFG::NT)
+__global__ __launch_bounds__(CFG::NT, CFG::MINW)
 void gemm_group_kernel(const GroupGemm<T>* __restrict__ probs, int nprob, int total, int chunk) {
   for (int b = (int)blockIdx.x; b < total; b += (int)gridDim.x) {
     const int t = xcd_chunk_remap(b, total, chunk);
