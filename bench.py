@@ -909,10 +909,13 @@ def main():
         traffic_key = f"{args.precision}_squaring"
     else:
         peak = FP32_PEAK_TFLOPS if cdt == np.complex64 else FP64_PEAK_TFLOPS
-        issued = 3 * 2.0 * (T * (T + 1) // 2) * 128 * 128 * kpad if st["structured"] else 3 * 2.0 * np_pad ** 3
-        kname = ("gemm_nt_kernel (3M symmetric squaring; above N=4096 each squaring is a launch of 128x128 tiles "
-                 "plus, when the last wave of tiles would be under half full, a launch of the rest as 64x64 "
-                 "sub-tiles (gemm_sym_tail_kernel); avg_launch_ms covers both)")
+        # FP32 squarings up to N = 12288 and all FP64 ones: 64 x 64 tiles;
+        # FP32 above: 128 x 128 tiles (+ a 64 x 64 tail launch)
+        tile = 64 if (cdt != np.complex64 or n <= 12288) else 128
+        Tt = (n + tile - 1) // tile
+        issued = 3 * 2.0 * (Tt * (Tt + 1) // 2) * tile * tile * kpad if st["structured"] else 3 * 2.0 * np_pad ** 3
+        kname = ("gemm_nt_kernel (3M symmetric squaring, lower 64x64 tiles in 8x8 groups, 4 waves of 32x32 on "
+                 "v_mfma_f32_32x32x2, four waves per SIMD; above N=12288 128x128 tiles plus a 64x64 tail launch)")
         traffic_key = "fp32_squaring" if cdt == np.complex64 else None
     issued_tf = issued / (sq_ms * 1e-3) / 1e12
     traffic = _profiled_traffic(traffic_key) if n == 10000 else None

@@ -127,6 +127,7 @@ struct PipelineBase {
   int tail_split = 1;      // FP32 squarings: last partial wave of 128-tiles as 64-tiles
   int z_overlap = 1;       // Cholesky solve: Z updates on a second stream
   int ch_outer = 512;      // Cholesky solve: outer block width (a multiple of 64)
+  int sq_tile = 0;         // FP32 structured squarings: 0 auto, 64 or 128 (tile edge)
 };
 
 // ---------------------------------------------------------------------------
@@ -1130,9 +1131,11 @@ template <typename W> struct Pipeline : PipelineBase {
       g.epi.d_out = dsq[dc ^ 1];
       g.kblock = kblock;
       const bool blk = kblock > 0 && std::is_same<W, float>::value;
-      // FP32 orders up to 4096: 64 x 64 tiles (measured 2.3x faster up to
-      // n = 1536, equal at 2048, 10-20% faster at 3072-4096, slower beyond)
-      const bool small = std::is_same<W, float>::value && n <= 4096;
+      // FP32: 64 x 64 tiles at four waves per SIMD up to n = 12288 (measured
+      // equal or faster than 128 x 128 tiles with the tail split at every
+      // order from 1024 to 10^4: 0.97 vs 1.42 ms at 3072, 26.2 vs 26.5 ms at
+      // 10^4; same values), 128 x 128 tiles beyond; sq_tile forces either.
+      const bool small = std::is_same<W, float>::value && (sq_tile == 64 || (sq_tile == 0 && n <= 12288));
       // (FCfgSmall runs 32x32x2 FP32 MFMAs; FP64 engines never take this branch)
       using SmallC = typename std::conditional<std::is_same<W, float>::value, FCfgSmall, GemmCfg<W>>::type;
       if (structured) {
@@ -1864,6 +1867,13 @@ int fexpm_set_option(fexpm_engine* e, int key, int64_t value) {
       return FEXPM_OK;
     case FEXPM_OPT_Z_OVERLAP:
       if (e->pipe) e->pipe->z_overlap = value ? 1 : 0;
+      return FEXPM_OK;
+    case FEXPM_OPT_SQUARE_TILE:
+      if (value != 0 && value != 64 && value != 128) {
+        e->err = "squaring tile must be 0 (auto), 64 or 128";
+        return FEXPM_EINVAL;
+      }
+      if (e->pipe) e->pipe->sq_tile = (int)value;
       return FEXPM_OK;
     case FEXPM_OPT_CHOL_OUTER:
       if (value < 64 || value > 4096 || value % 64 != 0) {

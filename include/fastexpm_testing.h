@@ -50,6 +50,11 @@ enum { FEXPM_OPT_Z_OVERLAP = 104 };
  * remaining columns of Z with K equal to this width. */
 enum { FEXPM_OPT_CHOL_OUTER = 105 };
 
+/* Engine option: tile edge of the FP32 structured squarings: 0 (default,
+ * automatic: 64 up to n = 12288, else 128 with the tail split), 64 or 128.
+ * Every choice gives the same values bit for bit. */
+enum { FEXPM_OPT_SQUARE_TILE = 106 };
+
 /* Internal planes readable by fexpm_debug_plane (working precision, n x n). */
 enum fexpm_plane {
   FEXPM_PLANE_A = 0,   /* (A - mu I) / 2^s; structured: its imaginary part G */

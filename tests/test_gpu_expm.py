@@ -170,22 +170,31 @@ def test_fp64_slice_products_ctqw(fx, slices, scope, n):
 
 @pytest.mark.parametrize("n", [4200])
 def test_squaring_tail_split(fx, n):
-    """FP32 squarings above n = 4096 (128 x 128 tiles): the last partial wave
-    of tiles as 64 x 64 sub-tiles gives the same U bit for bit as one launch
-    of 128 x 128 tiles, and U matches the eigen oracle."""
+    """FP32 squarings on 128 x 128 tiles: the last partial wave of tiles as
+    64 x 64 sub-tiles gives the same U bit for bit as one launch of 128 x 128
+    tiles, and as the 64 x 64 tiling throughout (the automatic choice at this
+    order); U matches the eigen oracle; invalid tile edges are rejected."""
     from fastctqw_amd.fastexpm import _fastexpm_core as core
 
     _say(f"squaring tail split n={n}")
     L = laplacian(erdos_renyi(n, 0.02, seed=60 + n, weighted=True))
     a = (-1j * 4.5 * L).astype(np.complex64)
     with fx.ExpMatComplex64(n) as e:
+        e._backend.set_option(core.OPT_SQUARE_TILE, 128)
         u_split = e.run(a)
         st = e.stats()
         e._backend.set_option(core.OPT_TAIL_SPLIT, 0)
         u_one = e.run(a)
         e._backend.set_option(core.OPT_TAIL_SPLIT, 1)
+        e._backend.set_option(core.OPT_SQUARE_TILE, 64)
+        u_64 = e.run(a)
+        e._backend.set_option(core.OPT_SQUARE_TILE, 0)
+        u_auto = e.run(a)
+        with pytest.raises(ValueError):
+            e._backend.set_option(core.OPT_SQUARE_TILE, 32)
     assert st["s"] > 0
     assert np.array_equal(u_split, u_one)
+    assert np.array_equal(u_split, u_64) and np.array_equal(u_split, u_auto)
     lam, V = np.linalg.eigh(np.ascontiguousarray(a.imag, dtype=np.float64))
     psi = (V * np.exp(1j * lam)[None, :]) @ (V.T[:, :1])
     err = float(np.linalg.norm(psi[:, 0] - u_split[:, 0]) / np.linalg.norm(psi))
