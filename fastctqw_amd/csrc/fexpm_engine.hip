@@ -1143,7 +1143,8 @@ template <typename W> struct Pipeline : PipelineBase {
         else if (blk) square_big_tail(g, st);
         else gemm<MODE_C3M, true>(g, st);
       } else {
-        if (blk) gemm<MODE_C3M, false, true>(g, st);
+        if (blk && small) gemm<MODE_C3M, false, true, SmallC>(g, st);
+        else if (blk) gemm<MODE_C3M, false, true>(g, st);
         else gemm<MODE_C3M, false>(g, st);
       }
       std::swap(cur, nxt);
