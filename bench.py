@@ -706,6 +706,8 @@ def _mfma_rates():
                                            ("fp64_c3m_sym_64x64_4waves", 1, 2, 0, 64, 3),
                                            ("fp64_real_sym_64x64_8waves_pad4", 1, 0, 4, 64, 1),
                                            ("fp64_c3m_sym_64x64_8waves_pad4", 1, 2, 4, 64, 3),
+                                           ("fp64_real_sym_64x64_8waves_w4", 1, 0, 5, 64, 1),
+                                           ("fp64_c3m_sym_64x64_8waves_w4", 1, 2, 5, 64, 3),
                                            ("int8_slices6_sym_128x128_8waves", 4, 0, 6, 128, 21)):
         ms = ctypes.c_double(0.0)
         rc = lib.fexpm_test_gemm_time(w, mode, 1, cfg, n, n, kp, 3, ctypes.byref(ms))
