@@ -2673,7 +2673,8 @@ struct OzBufs {
 };
 
 template <int S>
-double oz_run_t(int sym, int m, int n, int k, const double* a, const double* b, double* c, int reps) {
+double oz_run_t(int sym, int m, int n, int k, const double* a, const double* b, double* c, int reps,
+                bool tile64 = false) {
   const int kpad = (int)round_up(k, OZ_BK), ra = (int)round_up(m, OZ_BM), rb = (int)round_up(n, OZ_BM);
   OzBufs B;
   FX_HIP(hipMalloc(&B.da, (size_t)m * k * sizeof(double)));
@@ -2710,7 +2711,14 @@ double oz_run_t(int sym, int m, int n, int k, const double* a, const double* b, 
   g.epi.ld_add = n;
   g.epi.alpha = 1.0;
   auto launch = [&]() {
-    if (sym) {
+    if (tile64) {
+      if (sym) {
+        const int T = ra / 64;
+        oz_gemm_kernel64<S, true><<<dim3(T * (T + 1) / 2), dim3(256)>>>(g);
+      } else {
+        oz_gemm_kernel64<S, false><<<dim3(rb / 64, ra / 64), dim3(256)>>>(g);
+      }
+    } else if (sym) {
       const int T = ra / OZ_BM;
       oz_gemm_kernel<S, true><<<dim3(T * (T + 1) / 2), dim3(512)>>>(g);
     } else {
@@ -3161,6 +3169,7 @@ extern "C" int fexpm_test_gemm_time(int wtype, int mode, int sym, int cfg, int m
       // FP64 product by int8 slices, cfg = number of slices (operands from a fill pattern)
       if (cfg == 5) t = oz_run_t<5>(sym, m, n, k, nullptr, nullptr, nullptr, reps);
       else if (cfg == 6) t = oz_run_t<6>(sym, m, n, k, nullptr, nullptr, nullptr, reps);
+      else if (cfg == 16) t = oz_run_t<6>(sym, m, n, k, nullptr, nullptr, nullptr, reps, true);
       else if (cfg == 4) t = oz_run_t<4>(sym, m, n, k, nullptr, nullptr, nullptr, reps);
     } else if (wtype >= 2) {
       // cfg % 10: 0 = 8 waves 2x4, 1 = 4 waves 2x2, 2 = 8 waves 4x2, 3 = 16 waves 4x4 (fp16x2);

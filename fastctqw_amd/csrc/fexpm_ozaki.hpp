@@ -155,12 +155,12 @@ template <int S> struct OzArgs {
 // Output tile (tm, tn) of C = A B^T from the slices: 8 waves (4 x 2), each
 // 32 rows x 64 columns (two 32 x 32 MFMA blocks); every LDS read is
 // complete on return.
-template <int S>
+template <int S, int BMT = OZ_BM, int WAVES_M = 4, int WAVES_N = 2>
 __device__ __forceinline__ void oz_tile(const OzArgs<S>& g, int tm, int tn, bool sym) {
-  constexpr int WAVES_M = 4, WAVES_N = 2;
   constexpr int NT = 64 * WAVES_M * WAVES_N;
-  constexpr int FI = OZ_BM / WAVES_M / 32, FJ = OZ_BM / WAVES_N / 32;  // 1 x 2 blocks per wave
-  constexpr int STAGE = OZ_BM * S * OZ_BK;  // bytes of one operand stage in memory
+  constexpr int FI = BMT / WAVES_M / 32, FJ = BMT / WAVES_N / 32;  // 32 x 32 blocks per wave
+  static_assert(FI >= 1 && FJ >= 1 && OZ_BM % BMT == 0, "tile edge must divide the row padding");
+  constexpr int STAGE = BMT * S * OZ_BK;  // bytes of one operand stage in memory
   constexpr int CH = STAGE / 16;            // 16-byte chunks per operand stage
   constexpr int CPT = (CH + NT - 1) / NT;
   // LDS rows of S * 32 bytes, padded by 16 bytes for S < 6: a row stride of
@@ -169,10 +169,10 @@ __device__ __forceinline__ void oz_tile(const OzArgs<S>& g, int tm, int tn, bool
   // (192 bytes) the unpadded layout measured faster (9.3 vs 10.4 ms at
   // n = 10^4).
   constexpr int RSTR = S * OZ_BK + (S < 6 ? 16 : 0);
-  constexpr int LSTAGE = OZ_BM * RSTR;
+  constexpr int LSTAGE = BMT * RSTR;
   __shared__ __attribute__((aligned(16))) int8_t smem[2][2 * LSTAGE];
 
-  const int row0 = tm * OZ_BM, col0 = tn * OZ_BM;
+  const int row0 = tm * BMT, col0 = tn * BMT;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wr = (wave / WAVES_N) * (32 * FI), wc = (wave % WAVES_N) * (32 * FJ);
   const int l32 = lane & 31, kh = lane >> 5;
@@ -278,6 +278,20 @@ __device__ __forceinline__ void oz_tile(const OzArgs<S>& g, int tm, int tn, bool
       }
     }
   }
+}
+
+// The same product on 64 x 64 tiles of 4 waves (32 x 32 each), 48 KB of
+// LDS for S = 6: three workgroups per CU; tiles in 8 x 8 groups.
+template <int S, bool SYM>
+__global__ __launch_bounds__(256, 3) void oz_gemm_kernel64(OzArgs<S> g) {
+  int tm, tn;
+  if (SYM) {
+    grouped_tri_tile(xcd_remap((int)blockIdx.x, (int)gridDim.x), g.rows_a / 64, tm, tn);
+  } else {
+    const int lin = xcd_remap((int)(blockIdx.y * gridDim.x + blockIdx.x), (int)(gridDim.x * gridDim.y));
+    grouped_rect_tile(lin, (int)gridDim.y, (int)gridDim.x, tm, tn);
+  }
+  oz_tile<S, 64, 2, 2>(g, tm, tn, SYM);
 }
 
 template <int S, bool SYM>
