@@ -1482,6 +1482,10 @@ template <typename W> struct Pipeline : PipelineBase {
   void apply(const double2* d_psi0, int nstates, double2* d_out, hipStream_t st) override {
     apply_states<W><<<(n + 3) / 4, 256, 0, st>>>(res.r, res.i, ld, n, d_psi0, nstates, ph_re, ph_im, d_out);
     FX_LAUNCH();
+    if (last_structured && nstates > 0) {  // unitary U: FP64 norm refinement of every psi
+      renorm_states<<<nstates, 256, 0, st>>>(d_psi0, d_out, n);
+      FX_LAUNCH();
+    }
   }
 };
 

@@ -264,6 +264,19 @@ template <typename W> struct OpApply {
   }
 };
 
+// FP64 norm refinement of psi = U psi0 (unitary U): psi rescaled to the
+// norm of psi0 (renorm_state_block), one workgroup per walk.
+struct OpRenorm {
+  static constexpr int NT = 256;
+  struct Prob {
+    const double2* psi0;
+    double2* out;
+    int n;
+  };
+  static int blocks(const Prob&) { return 1; }
+  static __device__ __forceinline__ void run(const Prob& p, int) { renorm_state_block(p.psi0, p.out, p.n); }
+};
+
 // Diagonal 64 x 64 block of the Cholesky factor and its inverse.
 struct OpChol {
   static constexpr int NT = 256;
@@ -340,7 +353,8 @@ enum GroupKind {
   GK_SQ_F16,      // split squaring on 16-bit MFMA, tiles of SQ_BM (GEMM-like stage)
   GK_SQ_BF16,
   GK_OZSLICE,     // int8 slices of FP64 operands
-  GK_OZGEMM       // FP64 products from int8 slices, tiles of OZ_BM (GEMM-like stage)
+  GK_OZGEMM,      // FP64 products from int8 slices, tiles of OZ_BM (GEMM-like stage)
+  GK_RENORM       // FP64 norm refinement of the states psi_i
 };
 
 static inline bool gk_tiled(int kind) {
@@ -659,6 +673,7 @@ static void group_enqueue_one(const GroupLaunch& L, unsigned char* base, hipStre
     case GK_SOLUTION: FX_OP(OpSolution<W>)
     case GK_OUTPUT: FX_OP(OutOp)
     case GK_APPLY: FX_OP(OpApply<W>)
+    case GK_RENORM: FX_OP(OpRenorm)
     case GK_CHOL: FX_OP(OpChol)
     case GK_OZSLICE: FX_OP(OpOzSlice)
     case GK_OZGEMM:
@@ -812,6 +827,11 @@ template <typename Tin, typename W> static std::string describe_stage(const Grou
         l = "APPLY yr=" + vfmt(q.yr) + " yi=" + vfmt(q.yi) + " ld=" + std::to_string(q.ld) +
             " n=" + std::to_string(q.n) + " d=" + vfmt(q.d) + " psi0=" + vfmt(q.psi0) +
             " phr=" + dfmt(q.ph_re) + " phi=" + dfmt(q.ph_im) + " out=" + vfmt(q.out);
+        break;
+      }
+      case GK_RENORM: {
+        const auto& q = *reinterpret_cast<const OpRenorm::Prob*>(b);
+        l = "RENORM psi0=" + vfmt(q.psi0) + " out=" + vfmt(q.out) + " n=" + std::to_string(q.n);
         break;
       }
       case GK_GEMM: l = S.f32 ? describe_gemm<float>(S, p) : describe_gemm<double>(S, p); break;
@@ -1616,6 +1636,7 @@ template <typename Tin, typename W> struct GroupRunner : GroupRunnerBase {
     {
       GroupStage& out = stage(GK_OUTPUT);
       GroupStage& ap = stage(GK_APPLY);
+      GroupStage& rn = stage(GK_RENORM);  // every grouped walk is a unitary CTQW propagator
       for (const GroupPlan& P : plans) {
         const int i = P.idx, b = P.s & 1;
         const double er = std::exp(P.shifted ? P.mu_re : 0.0);
@@ -1629,6 +1650,8 @@ template <typename Tin, typename W> struct GroupRunner : GroupRunnerBase {
           typename OpApply<W>::Prob a{wp(i, b, 0), wp(i, b, 1), wm[i].np, wm[i].n, d, d_psi0 + voff[i], phr, phi,
                                       d_psi + voff[i]};
           ap.add(a, OpApply<W>::blocks(a));
+          OpRenorm::Prob r{d_psi0 + voff[i], d_psi + voff[i], wm[i].n};
+          rn.add(r, OpRenorm::blocks(r));
         }
       }
     }

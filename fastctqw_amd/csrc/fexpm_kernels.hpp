@@ -1203,6 +1203,37 @@ __global__ __launch_bounds__(256) void apply_states(const W* __restrict__ xr, co
   }
 }
 
+// One-step FP64 refinement of psi = U psi0 for a unitary U (structured
+// input, |e^mu| = 1): each state is rescaled to the norm of its psi0, which
+// removes the radial part of the error of the FP32 U.  One workgroup per
+// state; zero states are left as they are.
+__device__ __forceinline__ void renorm_state_block(const double2* __restrict__ p, double2* __restrict__ q, int n) {
+  __shared__ double r0[256], r1[256];
+  const int t = threadIdx.x;
+  double a = 0.0, b = 0.0;
+  for (int j = t; j < n; j += 256) {
+    a += p[j].x * p[j].x + p[j].y * p[j].y;
+    b += q[j].x * q[j].x + q[j].y * q[j].y;
+  }
+  r0[t] = a;
+  r1[t] = b;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      r0[t] += r0[t + w];
+      r1[t] += r1[t + w];
+    }
+    __syncthreads();
+  }
+  if (!(r0[0] > 0.0) || !(r1[0] > 0.0)) return;
+  const double f = sqrt(r0[0] / r1[0]);
+  for (int j = t; j < n; j += 256) q[j] = make_double2(f * q[j].x, f * q[j].y);
+}
+__global__ __launch_bounds__(256) void renorm_states(const double2* __restrict__ psi0, double2* __restrict__ out,
+                                                     int n) {
+  renorm_state_block(psi0 + (int64_t)blockIdx.x * n, out + (int64_t)blockIdx.x * n, n);
+}
+
 // ---------------------------------------------------------------------------
 // Structured solve (A = iC, C real symmetric).  With V and U = i Uq from the
 // real Pade products, M = V - i Uq is normal (V, Uq commute) and

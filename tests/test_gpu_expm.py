@@ -440,6 +440,12 @@ def test_device_resident_and_apply(fx):
     assert rel_fro(ref, u) < _tol(np.complex64, 10)
     assert infidelity(ref[:, 0], psi[0]) < 1e-9
     assert np.allclose(psi[0], u[:, 0].astype(np.complex128), atol=1e-6)
+    # one-step FP64 refinement (unitary U): every psi keeps the norm of its psi0
+    for k in range(2):
+        assert abs(np.linalg.norm(psi[k]) - np.linalg.norm(psi0[k])) < 1e-12
+    # unnormalised infidelity 1 - |<ref|psi>|^2 equals the normalised one to rounding
+    raw = 1.0 - abs(np.vdot(ref[:, 0], psi[0])) ** 2
+    assert abs(raw - infidelity(ref[:, 0], psi[0])) < 1e-12
 
 
 @pytest.mark.parametrize("dt", [np.complex64, np.complex128])
@@ -565,6 +571,8 @@ def test_batch_grouped(fx, dt):
         assert st["structured"] == (1 if kind == "ctqw" else 0)
         psi_ref = u.astype(np.complex128) @ psi0[i]
         assert np.linalg.norm(psi - psi_ref) <= 1e-5 * max(1.0, np.linalg.norm(psi_ref)), i
+        if kind == "ctqw":  # one-step FP64 refinement: psi keeps the norm of psi0
+            assert abs(np.linalg.norm(psi) - np.linalg.norm(psi0[i])) < 1e-12, i
     # replay: same plans -> the captured graph again, bitwise the same U
     us2, psis2 = b.run(mats, psi0)
     rep2 = b.report()
@@ -581,6 +589,8 @@ def test_batch_grouped(fx, dt):
             continue
         s_i = b.stats(i)["s"]
         assert rel_fro(us_l[i], us[i]) <= 2 * tol_fn(s_i), i
+        if kinds[i] == "ctqw":
+            assert abs(np.linalg.norm(psis_l[i]) - np.linalg.norm(psi0[i])) < 1e-12, i
     # new times (same orders): new plans, a new graph, the layout keeps its tuning
     b.set_grouped(True)
     mats3 = [m * (0.5 if k == "ctqw" else 1.0) for m, k in zip(mats, kinds)]

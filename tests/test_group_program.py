@@ -256,6 +256,14 @@ class Interp:
         arr, e = self._elem(kv["out"])
         arr[e:e + n] = self._u(kv) @ p0[e0:e0 + n]
 
+    def op_renorm(self, kv):
+        n = int(kv["n"])
+        p0, e0 = self._elem(kv["psi0"])
+        arr, e = self._elem(kv["out"])
+        a, b = np.linalg.norm(p0[e0:e0 + n]), np.linalg.norm(arr[e:e + n])
+        if a > 0 and b > 0:
+            arr[e:e + n] *= a / b
+
     def result(self, i_off, n):
         arr = self.reg[6][0]
         return arr[i_off:i_off + n * n].reshape(n, n)
