@@ -947,6 +947,10 @@ def main():
             "infidelity_random_state": _infidelity(ref["psi_rand"], psi_rand),
             "rel_err_e0": float(np.linalg.norm(psi_e0 - ref["psi_e0"]) / np.linalg.norm(ref["psi_e0"])),
             "raw_infidelity_e0": float(1.0 - abs(np.vdot(ref["psi_e0"], psi_e0)) ** 2),
+            # |‖psi‖ - 1|: psi_e0 is column 0 of the complex64 U as stored; the random
+            # state comes from the device apply, refined to the norm of psi0 in FP64
+            "norm_deviation_e0": float(abs(np.linalg.norm(psi_e0) - 1.0)),
+            "norm_deviation_random_state": float(abs(np.linalg.norm(psi_rand) - 1.0)),
         })
     elif ref_error:
         acc["oracle_error"] = ref_error
