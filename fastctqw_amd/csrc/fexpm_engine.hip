@@ -3198,6 +3198,8 @@ extern "C" int fexpm_test_gemm_time(int wtype, int mode, int sym, int cfg, int m
         t = time_gemm_t<float, MODE_C3M, true, true, TileCfg<128, 128, 16, 8, 64, 32, 32>>(m, n, k, reps);
       else if (mode == 2 && sym && cfg == 11)
         t = time_gemm_t<float, MODE_C3M, true, true, TileCfg<128, 128, 16, 4, 32, 64, 32>>(m, n, k, reps);
+      else if (mode == 2 && sym && cfg == 13)
+        t = time_gemm_t<float, MODE_C3M, true, true, TileCfg<64, 64, 16, 4, 32, 16, 16, 4>>(m, n, k, reps);
       else if (mode == 2 && sym && cfg == 5)
         t = time_gemm_t<float, MODE_C3M, true, true, TileCfg<128, 128, 16, 8, 64, 32>>(m, n, k, reps);
       else if (mode == 2 && sym && cfg == 6)
