@@ -549,12 +549,8 @@ void gemm_group_kernel(const GroupGemm<T>* __restrict__ probs, int nprob, int to
     const GroupGemm<T>& P = probs[lo];
     const int local = t - P.tile0;
     int tm, tn;
-    if (P.sym) {
-      tri_tile(local, tm, tn);
-    } else {
-      tm = local / P.tiles_n;
-      tn = local % P.tiles_n;
-    }
+    if (P.sym) grouped_tri_tile(local, P.tiles_n, tm, tn);  // tiles_n = tiles per side
+    else grouped_rect_tile(local, P.tiles / P.tiles_n, P.tiles_n, tm, tn);
     gemm_tile<T, MODE, KBLK, CFG>(P.g, tm, tn, P.sym != 0);
   }
 }

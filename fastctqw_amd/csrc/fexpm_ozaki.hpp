@@ -339,12 +339,8 @@ __global__ __launch_bounds__(512) void oz_group_kernel(const GroupOz<S>* __restr
     const GroupOz<S>& P = probs[lo];
     const int local = t - P.tile0;
     int tm, tn;
-    if (P.sym) {
-      tri_tile(local, tm, tn);
-    } else {
-      tm = local / P.tiles_n;
-      tn = local % P.tiles_n;
-    }
+    if (P.sym) grouped_tri_tile(local, P.tiles_n, tm, tn);  // tiles_n = tiles per side
+    else grouped_rect_tile(local, P.tiles / P.tiles_n, P.tiles_n, tm, tn);
     oz_tile<S>(P.g, tm, tn, P.sym != 0);
   }
 }

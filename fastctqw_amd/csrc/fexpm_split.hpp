@@ -403,7 +403,7 @@ __global__ __launch_bounds__(sq_threads<P>()) void sq_split_group_kernel(const G
       else hi = mid - 1;
     }
     int tm, tn;
-    tri_tile(t - probs[lo].tile0, tm, tn);
+    grouped_tri_tile(t - probs[lo].tile0, (probs[lo].g.m + SQ_BM - 1) / SQ_BM, tm, tn);
     sq_split_tile<P, P::WAVES_M, P::WAVES_N>(probs[lo].g, tm, tn);
   }
 }
