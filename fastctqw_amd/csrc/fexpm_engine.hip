@@ -128,6 +128,7 @@ struct PipelineBase {
   int z_overlap = 1;       // Cholesky solve: Z updates on a second stream
   int ch_outer = 512;      // Cholesky solve: outer block width (a multiple of 64)
   int sq_tile = 0;         // FP32 structured squarings: 0 auto, 64 or 128 (tile edge)
+  int z_left = 0;          // Cholesky solve: left-looking Z updates (one long-K product per outer block)
 };
 
 // ---------------------------------------------------------------------------
@@ -867,6 +868,12 @@ template <typename W> struct Pipeline : PipelineBase {
     int nev = 0;
     for (int K0 = 0; K0 < ip; K0 += ch_outer) {
       const int K1 = std::min(ip, K0 + ch_outer);
+      if (z_left && K0 > 0) {
+        // left-looking Z: bring the block's columns up to date with every
+        // finished column at once (K = K0), instead of the right-looking outer
+        // update after each block; zs is already ordered after every earlier panel
+        flops += dgemm_real(zb, L, sm + (int64_t)K0 * L, L, zb + K0, L, 2 * ip, K1 - K0, K0, -1.0, 1.0, false, zs);
+      }
       for (int k0 = K0; k0 < K1; k0 += LU_NB) {
         const int k1 = k0 + LU_NB;
         const int rest = ip - k1;
@@ -894,7 +901,7 @@ template <typename W> struct Pipeline : PipelineBase {
         const int r2 = ip - K1, kk = K1 - K0;
         double* p2 = sm + (int64_t)K1 * L + K0;  // L[K1:, K0:K1]
         flops += dgemm_real(p2, L, p2, L, sm + (int64_t)K1 * L + K1, L, r2, r2, kk, -1.0, 1.0, true, st);
-        flops += dgemm_real(zb + K0, L, p2, L, zb + K1, L, 2 * ip, r2, kk, -1.0, 1.0, false, zs);
+        if (!z_left) flops += dgemm_real(zb + K0, L, p2, L, zb + K1, L, 2 * ip, r2, kk, -1.0, 1.0, false, zs);
       }
     }
     if (ov) {  // X = Z Z^T needs every Z update
@@ -1259,6 +1266,7 @@ template <typename W> struct Pipeline : PipelineBase {
       h->oz_scope = oz_scope;
       h->z_overlap = z_overlap;
       h->ch_outer = ch_outer;
+      h->z_left = z_left;
       h->front_pade<Tin>(d_a, P.structured, P.m, P.mu_re, P.mu_im, scale, st);
       if (timing) FX_HIP(hipEventRecord(ev[2], st));
       const int dh = h->solve(P.structured, 0, st);
@@ -1872,6 +1880,9 @@ int fexpm_set_option(fexpm_engine* e, int key, int64_t value) {
       return FEXPM_OK;
     case FEXPM_OPT_Z_OVERLAP:
       if (e->pipe) e->pipe->z_overlap = value ? 1 : 0;
+      return FEXPM_OK;
+    case FEXPM_OPT_Z_LEFT:
+      if (e->pipe) e->pipe->z_left = value ? 1 : 0;
       return FEXPM_OK;
     case FEXPM_OPT_SQUARE_TILE:
       if (value != 0 && value != 64 && value != 128) {

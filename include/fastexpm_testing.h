@@ -55,6 +55,12 @@ enum { FEXPM_OPT_CHOL_OUTER = 105 };
  * Every choice gives the same values bit for bit. */
 enum { FEXPM_OPT_SQUARE_TILE = 106 };
 
+/* Engine option: 1 applies the Z updates of the structured Cholesky solve
+ * left-looking (one product with K = all finished columns at the start of
+ * each outer block), 0 (default) right-looking (one product with K = the
+ * outer width after each block).  Same flops, different summation order. */
+enum { FEXPM_OPT_Z_LEFT = 107 };
+
 /* Internal planes readable by fexpm_debug_plane (working precision, n x n). */
 enum fexpm_plane {
   FEXPM_PLANE_A = 0,   /* (A - mu I) / 2^s; structured: its imaginary part G */

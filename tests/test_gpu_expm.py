@@ -247,6 +247,36 @@ def test_cholesky_outer_width(fx, dt, n):
                 e._backend.set_option(core.OPT_CHOL_OUTER, bad)
 
 
+@pytest.mark.parametrize("dt,n", [(np.complex64, 700), (np.complex128, 650)])
+def test_cholesky_left_looking_z(fx, dt, n):
+    """Left-looking Z updates (one long-K product per outer block): the same
+    values with and without the second stream, agreement with the
+    right-looking form to FP64 rounding carried through the squarings, and
+    with the eigendecomposition oracle; small outer blocks give several."""
+    from fastctqw_amd.fastexpm import _fastexpm_core as core
+
+    _say(f"left-looking Z {np.dtype(dt).name} n={n}")
+    L = laplacian(erdos_renyi(n, 0.3, seed=110 + n, weighted=True))
+    a = (-1j * 6.0 * L).astype(dt)
+    lam, V = np.linalg.eigh(np.ascontiguousarray(a.imag, dtype=np.float64))
+    psi = (V * np.exp(1j * lam)[None, :]) @ (V.T[:, :1])
+    with _engine(fx, dt, n) as e:
+        e._backend.set_option(core.OPT_CHOL_OUTER, 128)
+        u_right = e.run(a)
+        s = e.stats()["s"]
+        e._backend.set_option(core.OPT_Z_LEFT, 1)
+        u_left = e.run(a)
+        e._backend.set_option(core.OPT_Z_OVERLAP, 0)
+        u_left1 = e.run(a)
+        e._backend.set_option(core.OPT_Z_OVERLAP, 1)
+        e._backend.set_option(core.OPT_Z_LEFT, 0)
+        e._backend.set_option(core.OPT_CHOL_OUTER, 512)
+    assert np.array_equal(u_left, u_left1)
+    assert rel_fro(u_left, u_right) <= (2e-6 if dt == np.complex64 else 1e-12)
+    err = float(np.linalg.norm(psi[:, 0] - u_left[:, 0]) / np.linalg.norm(psi))
+    assert err <= _tol(dt, s), err
+
+
 def test_fp64_slices_option_errors(fx):
     from fastctqw_amd.fastexpm import _fastexpm_core as core
 
