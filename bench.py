@@ -693,6 +693,7 @@ def _mfma_rates():
     lib.fexpm_test_gemm_time.argtypes = [ctypes.c_int] * 8 + [ctypes.POINTER(ctypes.c_double)]
     for name, w, mode, cfg, tile, mult in (("fp32_c3m_sym_128x128_8waves", 0, 2, 0, 128, 3),
                                            ("fp32_c3m_sym_64x64_4waves", 0, 2, 4, 64, 3),
+                                           ("fp32_c3m_sym_128x128_16waves", 0, 2, 12, 128, 3),
                                            ("fp32_c3m_sym_128x128_8waves_pad8", 0, 2, 5, 128, 3),
                                            ("fp32_c3m_sym_128x128_8waves_32x64", 0, 2, 6, 128, 3),
                                            ("fp32_c3m_sym_128x128_8waves_pad12", 0, 2, 7, 128, 3),
@@ -708,7 +709,12 @@ def _mfma_rates():
                                            ("fp64_c3m_sym_64x64_8waves_pad4", 1, 2, 4, 64, 3),
                                            ("fp64_real_sym_64x64_8waves_w4", 1, 0, 5, 64, 1),
                                            ("fp64_c3m_sym_64x64_8waves_w4", 1, 2, 5, 64, 3),
-                                           ("int8_slices6_sym_128x128_8waves", 4, 0, 6, 128, 21)):
+                                           ("int8_slices6_sym_128x128_8waves", 4, 0, 6, 128, 21),
+                                           # split squarings: 4M x the part products (3 for fp16x2, 6 bf16x3)
+                                           ("fp16x2_c4m_sym_128x128_16waves_2kblocks", 2, 2, 84, 128, 12),
+                                           ("fp16x2_c4m_sym_128x128_16waves_1kblock", 2, 2, 83, 128, 12),
+                                           ("fp16x2_c4m_sym_128x128_8waves_1kblock", 2, 2, 80, 128, 12),
+                                           ("bf16x3_c4m_sym_128x128_8waves", 3, 2, 80, 128, 24)):
         ms = ctypes.c_double(0.0)
         rc = lib.fexpm_test_gemm_time(w, mode, 1, cfg, n, n, kp, 3, ctypes.byref(ms))
         if rc != 0:
@@ -717,6 +723,25 @@ def _mfma_rates():
         issued = mult * 2.0 * T * (T + 1) / 2 * tile * tile * kp
         gem[name] = {"ms": ms.value, "issued_tflops": issued / (ms.value * 1e-3) / 1e12}
     print(json.dumps({"mfma_sustained": out, "gemm_n10000": gem}), flush=True)
+
+
+def _issued_structured_squaring(n, kpad, fp32):
+    """MFMA flops one structured 3M squaring issues: lower tiles of 64 x 64
+    for FP64; for FP32 the engine's own launch plan (64 x 64 tiles up to
+    N = 8192, above it 128 x 128 tiles and, when the last wave of workgroups
+    is less than half full, 64 x 64 sub-tiles for it)."""
+    if not fp32:
+        T = (n + 63) // 64
+        return 3 * 2.0 * (T * (T + 1) // 2) * 64 * 64 * kpad
+    import ctypes
+
+    lib = ctypes.CDLL(os.path.join(ROOT, "fastctqw_amd", "fastexpm", "libfastexpm_hip.so"))
+    lib.fexpm_test_square_issued.argtypes = [ctypes.c_int64, ctypes.POINTER(ctypes.c_double)]
+    fl = ctypes.c_double(0.0)
+    rc = lib.fexpm_test_square_issued(n, ctypes.byref(fl))
+    if rc != 0:
+        raise RuntimeError(f"fexpm_test_square_issued: {rc}")
+    return fl.value
 
 
 def _host_info(cores):
@@ -911,13 +936,11 @@ def main():
         traffic_key = f"{args.precision}_squaring"
     else:
         peak = FP32_PEAK_TFLOPS if cdt == np.complex64 else FP64_PEAK_TFLOPS
-        # FP32 squarings up to N = 12288 and all FP64 ones: 64 x 64 tiles;
-        # FP32 above: 128 x 128 tiles (+ a 64 x 64 tail launch)
-        tile = 64 if (cdt != np.complex64 or n <= 12288) else 128
-        Tt = (n + tile - 1) // tile
-        issued = 3 * 2.0 * (Tt * (Tt + 1) // 2) * tile * tile * kpad if st["structured"] else 3 * 2.0 * np_pad ** 3
-        kname = ("gemm_nt_kernel (3M symmetric squaring, lower 64x64 tiles in 8x8 groups, 4 waves of 32x32 on "
-                 "v_mfma_f32_32x32x2, four waves per SIMD; above N=12288 128x128 tiles plus a 64x64 tail launch)")
+        issued = (_issued_structured_squaring(n, kpad, cdt == np.complex64) if st["structured"]
+                  else 3 * 2.0 * np_pad ** 3)
+        kname = ("gemm_nt_kernel (3M symmetric squaring on v_mfma_f32_32x32x2, four waves per SIMD: up to "
+                 "N=8192 lower 64x64 tiles of 4 waves, above it 128x128 tiles of 16 waves plus a 64x64 launch "
+                 "for a last partial wave of tiles; 8x8 tile groups; each wave 32x32)")
         traffic_key = "fp32_squaring" if cdt == np.complex64 else None
     issued_tf = issued / (sq_ms * 1e-3) / 1e12
     traffic = _profiled_traffic(traffic_key) if n == 10000 else None

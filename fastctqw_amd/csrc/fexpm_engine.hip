@@ -86,6 +86,57 @@ static void choose_degree(double norm, int& m, int& s) {
 
 static inline int64_t round_up(int64_t x, int64_t r) { return (x + r - 1) / r * r; }
 
+// Tile edge of the FP32 structured squarings of order n: 64 x 64 tiles of 4
+// waves up to n = 8192, 128 x 128 tiles of 16 waves with the tail split
+// beyond (ms per squaring, 64 vs 128: 0.89 vs 0.93 at 3072, 6.30 vs 6.31 at
+// 6144, 26.15 vs 25.24 at 10^4, 57.05 vs 54.39 at 13000; the same values);
+// forced = 64 or 128 overrides.
+static int square_tile(int64_t n, int forced) {
+  if (forced == 64 || forced == 128) return forced;
+  return n <= 8192 ? 64 : 128;
+}
+
+// Launch plan of one FP32 structured squaring of order m in tile x tile
+// lower tiles: nb tiles; with 128 tiles and the tail split, when the last
+// wave of resident workgroups would be at most half full, `full` tiles run
+// in one launch and the last `rest` as 64 x 64 sub-tiles, `sub` of them
+// (the upper sub-tile of each diagonal tile is skipped); rest = 0 otherwise.
+struct SquarePlan {
+  int nb = 0, full = 0, rest = 0, sub = 0;
+};
+static SquarePlan square_plan(int m, int tile, bool tail_split) {
+  SquarePlan p;
+  const int T = (m + tile - 1) / tile;
+  p.nb = T * (T + 1) / 2;
+  p.full = p.nb;
+  if (tile != 128 || !tail_split) return p;
+  static int cap = -1;
+  if (cap < 0) {
+    int dev = 0, per_cu = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, (const void*)gemm_nt_kernel<float, MODE_C3M, true, true, FCfg16w>, FCfg16w::NT, 0) ==
+            hipSuccess)
+      cap = std::max(1, per_cu) * prop.multiProcessorCount;
+    else
+      cap = 0;
+  }
+  static_assert(FCfg16w::BM == 2 * FCfgSmall::BM && FCfg16w::BN == 2 * FCfgSmall::BN, "tail sub-tiles halve the tile");
+  const int rest = cap > 0 ? p.nb % cap : 0;
+  if (!(cap > 0 && p.nb - rest > 0 && rest > 0 && 2 * rest <= cap)) return p;
+  p.rest = rest;
+  p.full = p.nb - rest;
+  p.sub = 4 * rest;
+  for (int t = p.full; t < p.nb; ++t) {
+    int tm = (int)((std::sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+    while ((tm + 1) * (tm + 2) / 2 <= t) ++tm;
+    while (tm * (tm + 1) / 2 > t) --tm;
+    if (t - tm * (tm + 1) / 2 == tm) --p.sub;
+  }
+  return p;
+}
+
 // ---------------------------------------------------------------------------
 template <typename T> static T* dalloc(size_t count) {
   void* p = nullptr;
@@ -1079,41 +1130,22 @@ template <typename W> struct Pipeline : PipelineBase {
   // (gemm_sym_tail_kernel), which spread over the whole device.
   void square_big_tail(const GemmArgs<W>& g, hipStream_t st) {
     if constexpr (std::is_same<W, float>::value) {
-      using C = GemmCfg<float>;
-      const int T = (g.m + C::BM - 1) / C::BM;
-      const int nb = T * (T + 1) / 2;
-      static int cap = -1;
-      if (cap < 0) {
-        int dev = 0, per_cu = 0;
-        hipDeviceProp_t prop;
-        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &per_cu, (const void*)gemm_nt_kernel<float, MODE_C3M, true, true, C>, C::NT, 0) == hipSuccess)
-          cap = std::max(1, per_cu) * prop.multiProcessorCount;
-        else
-          cap = 0;
-      }
-      static_assert(C::BM == 2 * FCfgSmall::BM && C::BN == 2 * FCfgSmall::BN, "tail sub-tiles halve the tile");
-      const int rest = cap > 0 ? nb % cap : 0;
-      const int full = nb - rest;
-      if (tail_split && cap > 0 && full > 0 && rest > 0 && 2 * rest <= cap) {
-        gemm_nt_kernel<float, MODE_C3M, true, true, C><<<dim3(full), dim3(C::NT), 0, st>>>(g);
+      using C = FCfg16w;
+      const SquarePlan pl = square_plan(g.m, 128, tail_split != 0);
+      if (pl.rest > 0) {
+        gemm_nt_kernel<float, MODE_C3M, true, true, C><<<dim3(pl.full), dim3(C::NT), 0, st>>>(g);
         FX_LAUNCH();
-        gemm_sym_tail_kernel<float, MODE_C3M, true, FCfgSmall><<<dim3(4 * rest), dim3(FCfgSmall::NT), 0, st>>>(g, full);
+        gemm_sym_tail_kernel<float, MODE_C3M, true, FCfgSmall><<<dim3(4 * pl.rest), dim3(FCfgSmall::NT), 0, st>>>(
+            g, pl.full);
         FX_LAUNCH();
-        int sub = 4 * rest;  // sub-tiles computed: all but the upper one of each diagonal tile
-        for (int t = full; t < nb; ++t) {
-          int tm = (int)((std::sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-          while ((tm + 1) * (tm + 2) / 2 <= t) ++tm;
-          while (tm * (tm + 1) / 2 > t) --tm;
-          if (t - tm * (tm + 1) / 2 == tm) --sub;
-        }
-        flops += 3.0 * 2.0 * ((double)full * C::BM * C::BN + (double)sub * FCfgSmall::BM * FCfgSmall::BN) *
+        flops += 3.0 * 2.0 * ((double)pl.full * C::BM * C::BN + (double)pl.sub * FCfgSmall::BM * FCfgSmall::BN) *
                  (double)g.k;
         return;
       }
+      gemm<MODE_C3M, true, true, C>(g, st);
+    } else {
+      gemm<MODE_C3M, true, true>(g, st);
     }
-    gemm<MODE_C3M, true, true>(g, st);
   }
 
   std::pair<Plane2<W>*, int> square_split(int s, bool structured, int dc, hipStream_t st) {
@@ -1138,11 +1170,7 @@ template <typename W> struct Pipeline : PipelineBase {
       g.epi.d_out = dsq[dc ^ 1];
       g.kblock = kblock;
       const bool blk = kblock > 0 && std::is_same<W, float>::value;
-      // FP32: 64 x 64 tiles at four waves per SIMD up to n = 12288 (measured
-      // equal or faster than 128 x 128 tiles with the tail split at every
-      // order from 1024 to 10^4: 0.97 vs 1.42 ms at 3072, 26.2 vs 26.5 ms at
-      // 10^4; same values), 128 x 128 tiles beyond; sq_tile forces either.
-      const bool small = std::is_same<W, float>::value && (sq_tile == 64 || (sq_tile == 0 && n <= 12288));
+      const bool small = std::is_same<W, float>::value && square_tile(n, sq_tile) == 64;
       // (FCfgSmall runs 32x32x2 FP32 MFMAs; FP64 engines never take this branch)
       using SmallC = typename std::conditional<std::is_same<W, float>::value, FCfgSmall, GemmCfg<W>>::type;
       if (structured) {
@@ -2923,7 +2951,7 @@ __global__ void fill_pattern16(typename P::H* p, size_t cnt, uint32_t seed) {
 }
 
 // Split-precision squaring kernel alone (parts already in the k-blocked layout).
-template <class P, int WM, int WN>
+template <class P, int WM, int WN, int KPB = 1>
 double time_split_t(int m, int k, int reps, int group) {
   const int kpad = (int)round_up(std::max(k, 1), 16), rows = (int)round_up(m, SQ_BM);
   const size_t cp = (size_t)rows * kpad * 2 * P::NQ, cc = (size_t)rows * rows;
@@ -2969,10 +2997,10 @@ double time_split_t(int m, int k, int reps, int group) {
   hipEvent_t e0, e1;
   FX_HIP(hipEventCreate(&e0));
   FX_HIP(hipEventCreate(&e1));
-  sq_split_kernel<P, WM, WN><<<grid, block>>>(a);
+  sq_split_kernel<P, WM, WN, KPB><<<grid, block>>>(a);
   FX_LAUNCH();
   FX_HIP(hipEventRecord(e0));
-  for (int r = 0; r < reps; ++r) sq_split_kernel<P, WM, WN><<<grid, block>>>(a);
+  for (int r = 0; r < reps; ++r) sq_split_kernel<P, WM, WN, KPB><<<grid, block>>>(a);
   FX_HIP(hipEventRecord(e1));
   FX_HIP(hipEventSynchronize(e1));
   float ms = 0.f;
@@ -3145,6 +3173,17 @@ extern "C" int fexpm_test_mfma_rate(int kind, int iters, double* tflops, double*
   return fexpm_test_mfma_rate_waves(kind, 8, iters, tflops, ms);
 }
 
+extern "C" int fexpm_test_square_issued(int64_t n, double* flops) {
+  if (!flops || n < 1 || n > (int64_t)1 << 20) return FEXPM_EINVAL;
+  return guarded(nullptr, [&] {
+    check_device();
+    const int64_t k = round_up(n, 16);
+    const int tile = square_tile(n, 0);
+    const SquarePlan p = square_plan((int)n, tile, true);
+    *flops = 3.0 * 2.0 * ((double)p.full * tile * tile + (double)p.sub * 64 * 64) * (double)k;
+  });
+}
+
 extern "C" int fexpm_test_copy(int64_t bytes, int reps, double* ms) {
   if (!ms || bytes <= 0 || bytes % 16 != 0 || reps <= 0) return FEXPM_EINVAL;
   return guarded(nullptr, [&] {
@@ -3187,14 +3226,17 @@ extern "C" int fexpm_test_gemm_time(int wtype, int mode, int sym, int cfg, int m
       else if (cfg == 16) t = oz_run_t<6>(sym, m, n, k, nullptr, nullptr, nullptr, reps, true);
       else if (cfg == 4) t = oz_run_t<4>(sym, m, n, k, nullptr, nullptr, nullptr, reps);
     } else if (wtype >= 2) {
-      // cfg % 10: 0 = 8 waves 2x4, 1 = 4 waves 2x2, 2 = 8 waves 4x2, 3 = 16 waves 4x4 (fp16x2);
+      // cfg % 10: 0 = 8 waves 2x4, 1 = 4 waves 2x2, 2 = 8 waves 4x2, 3 = 16 waves 4x4, one k-block per LDS
+      // stage; 4 / 5 = 16 waves 4x4 (the fp16x2 default) / 8 waves 2x4 with two k-blocks per stage;
       // cfg / 10: tile group of the order table (0 = row-major triangle)
       const int lay = cfg % 10, grp = cfg / 10;
       if (mode == 2 && sym && wtype == 2) {
-        if (lay == 0) t = time_split_t<Fp16x2, 2, 4>(m, k, reps, grp);
-        else if (lay == 1) t = time_split_t<Fp16x2, 2, 2>(m, k, reps, grp);
-        else if (lay == 2) t = time_split_t<Fp16x2, 4, 2>(m, k, reps, grp);
-        else if (lay == 3) t = time_split_t<Fp16x2, 4, 4>(m, k, reps, grp);
+        if (lay == 0) t = time_split_t<Fp16x2, 2, 4, 1>(m, k, reps, grp);
+        else if (lay == 1) t = time_split_t<Fp16x2, 2, 2, 1>(m, k, reps, grp);
+        else if (lay == 2) t = time_split_t<Fp16x2, 4, 2, 1>(m, k, reps, grp);
+        else if (lay == 3) t = time_split_t<Fp16x2, 4, 4, 1>(m, k, reps, grp);
+        else if (lay == 4) t = time_split_t<Fp16x2, 4, 4, 2>(m, k, reps, grp);
+        else if (lay == 5) t = time_split_t<Fp16x2, 2, 4, 2>(m, k, reps, grp);
       } else if (mode == 2 && sym && wtype == 3) {
         if (lay == 0) t = time_split_t<Bf16x3, 2, 4>(m, k, reps, grp);
         else if (lay == 1) t = time_split_t<Bf16x3, 2, 2>(m, k, reps, grp);
@@ -3202,13 +3244,20 @@ extern "C" int fexpm_test_gemm_time(int wtype, int mode, int sym, int cfg, int m
       }
     } else if (wtype == 0) {
       // squaring tilings: 0 default (32x32x2 MFMAs), 4 FCfgSmall, 5..8 LDS padding / wave-tile variants on
-      // 16x16x4 MFMAs, 9 the default tiling on 16x16x4, 10 / 11 padding / wave-tile variants on 32x32x2
+      // 16x16x4 MFMAs, 9 the default tiling on 16x16x4, 10 / 11 padding / wave-tile variants on 32x32x2,
+      // 12 128x128 in 16 waves of 32x32 (four waves per SIMD), 14 / 15 the 16-wave 128x128 / default 64x64
+      // tiling with 32-deep k-steps
       if (mode == 2 && sym && cfg == 4) t = time_gemm_t<float, MODE_C3M, true, true, FCfgSmall>(m, n, k, reps);
       else if (mode == 2 && sym && cfg == 9) t = time_gemm_t<float, MODE_C3M, true, true, FCfg16>(m, n, k, reps);
       else if (mode == 2 && sym && cfg == 10)
         t = time_gemm_t<float, MODE_C3M, true, true, TileCfg<128, 128, 16, 8, 64, 32, 32>>(m, n, k, reps);
       else if (mode == 2 && sym && cfg == 11)
         t = time_gemm_t<float, MODE_C3M, true, true, TileCfg<128, 128, 16, 4, 32, 64, 32>>(m, n, k, reps);
+      else if (mode == 2 && sym && cfg == 12) t = time_gemm_t<float, MODE_C3M, true, true, FCfg16w>(m, n, k, reps);
+      else if (mode == 2 && sym && cfg == 14)
+        t = time_gemm_t<float, MODE_C3M, true, true, TileCfg<128, 128, 32, 4, 32, 32, 32, 4>>(m, n, k, reps);
+      else if (mode == 2 && sym && cfg == 15)
+        t = time_gemm_t<float, MODE_C3M, true, true, TileCfg<64, 64, 32, 4, 32, 32, 32, 4>>(m, n, k, reps);
       else if (mode == 2 && sym && cfg == 13)
         t = time_gemm_t<float, MODE_C3M, true, true, TileCfg<64, 64, 16, 4, 32, 16, 16, 4>>(m, n, k, reps);
       else if (mode == 2 && sym && cfg == 5)

@@ -130,6 +130,11 @@ using FCfg16 = TileCfg<128, 128, 16, 4, 64, 32, 16>;
 // waves per SIMD (128 VGPRs: four workgroups per CU, 18-25% faster at
 // n = 1024-4096 than the unconstrained 146 VGPRs).
 using FCfgSmall = TileCfg<64, 64, 16, 4, 32, 32, 32, 4>;
+// FP32 128 x 128 tiles as 16 waves of 32x32 (the per-wave work of FCfgSmall,
+// four waves per SIMD, one workgroup per CU): half the operand loads per
+// MFMA of the 64 x 64 tiles; 26.0 ms per n = 10^4 symmetric 3M squaring vs
+// 27.2 ms for GemmCfg<float>.
+using FCfg16w = TileCfg<128, 128, 16, 4, 32, 32, 32, 4>;
 // FP64: 8 waves of 32x16, compiled for four waves per SIMD (at most 128
 // VGPRs, two workgroups per CU); measured 63 TFLOP/s (4M, K = 256) and
 // 61 TFLOP/s (real symmetric, K = 10^4) on MI355X versus 50 / 57 for 4 waves
@@ -271,11 +276,12 @@ template <typename T, int MODE, bool KBLK, class CFG>
 __device__ __forceinline__ void gemm_tile(const GemmArgs<T>& g, int tm, int tn, bool sym) {
   using C = CFG;
   constexpr int BM = C::BM, BN = C::BN, BK = C::BK, PAD = C::PAD, WM = C::WM, WN = C::WN, MF = C::MF;
-  static_assert(MF == 16 || (MF == 32 && sizeof(T) == 4 && BK == 16), "32x32 blocks: FP32, BK = 16");
+  static_assert(BK % 16 == 0 && (MF == 16 || (MF == 32 && sizeof(T) == 4)), "32x32 blocks: FP32; BK: 16s");
   constexpr int WAVES_N = BN / WN;
   constexpr int NT = (BM / WM) * WAVES_N * 64;
   constexpr int FI = WM / MF, FJ = WN / MF;
-  constexpr int NRD = MF == 32 ? 2 : 1;  // 16-byte fragment reads per row and k-tile
+  constexpr int NRD = MF == 32 ? BK / 8 : BK / 16;  // 16-byte fragment reads per row and k-tile
+  constexpr int KSTEP = MF == 32 ? 8 : 16;            // k covered by one read across the wave
   constexpr int NS = 4 * NRD;            // MFMA steps per k-tile
   constexpr int NEL = MF == 32 ? 16 : 4;  // accumulator elements per lane
   constexpr int NPL = (MODE == MODE_REAL) ? 1 : 2;
@@ -348,9 +354,9 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs<T>& g, int tm, int tn, 
 #pragma unroll
       for (int q = 0; q < NRD; ++q) {
 #pragma unroll
-        for (int i = 0; i < FI; ++i) frag4<BK, PAD>(la, wr + MF * i + lrow, kq + 8 * q, &fa[p][i][4 * q]);
+        for (int i = 0; i < FI; ++i) frag4<BK, PAD>(la, wr + MF * i + lrow, kq + KSTEP * q, &fa[p][i][4 * q]);
 #pragma unroll
-        for (int j = 0; j < FJ; ++j) frag4<BK, PAD>(lb, wc + MF * j + lrow, kq + 8 * q, &fb[p][j][4 * q]);
+        for (int j = 0; j < FJ; ++j) frag4<BK, PAD>(lb, wc + MF * j + lrow, kq + KSTEP * q, &fb[p][j][4 * q]);
       }
     }
     auto mfma = [](T a, T b, Acc c) -> Acc {

@@ -51,6 +51,7 @@ struct Fp16x2 {
   static constexpr float SCALE = 16384.0f;
   static constexpr float LIMIT = 65504.0f;
   static constexpr int WAVES_M = 4, WAVES_N = 4;  // 16 waves of 32 x 32 (measured fastest)
+  static constexpr int KPB = 2;  // k-blocks per LDS stage: 132 KB, 4% faster than one at n = 10^4
   __device__ static constexpr int qa(int p) { return p == 2 ? 1 : 0; }
   __device__ static constexpr int qb(int p) { return p == 1 ? 1 : 0; }
   __device__ static __forceinline__ f32x16 mfma(V a, V b, f32x16 c) {
@@ -66,6 +67,7 @@ struct Bf16x3 {
   static constexpr float SCALE = 1.0f;
   static constexpr float LIMIT = 3.0e38f;
   static constexpr int WAVES_M = 2, WAVES_N = 4;  // 8 waves of 64 x 32 (measured fastest)
+  static constexpr int KPB = 1;  // two k-blocks per stage would need 198 KB of LDS
   // pairs (0,0) (0,1) (1,0) (0,2) (1,1) (2,0)
   __device__ static constexpr int qa(int p) { return p == 2 ? 1 : (p == 4 ? 1 : (p == 5 ? 2 : 0)); }
   __device__ static constexpr int qb(int p) { return p == 1 ? 1 : (p == 3 ? 2 : (p == 4 ? 1 : 0)); }
@@ -200,7 +202,7 @@ constexpr int SQ_TILE_STRIDE = SQ_BM * SQ_BK + 16;
 
 // Output tile (tm, tn), tm >= tn, of the split squaring; every LDS read is
 // complete on return, so a workgroup may call it again for another tile.
-template <class P, int WAVES_M, int WAVES_N>
+template <class P, int WAVES_M, int WAVES_N, int KPB = 1>
 __device__ __forceinline__ void sq_split_tile(const SqArgs<P>& g, int tm, int tn) {
   using H = typename P::H;
   using V = typename P::V;
@@ -211,7 +213,9 @@ __device__ __forceinline__ void sq_split_tile(const SqArgs<P>& g, int tm, int tn
   constexpr int CH = SQ_BM * W * 2;           // 16-byte chunks per operand stage
   constexpr int CPT = CH / NT;                // chunks per thread per operand
   static_assert(CH % NT == 0, "stage chunks");
-  __shared__ __attribute__((aligned(16))) H smem[2][2 * W * TILE];
+  static_assert(KPB == 1 || KPB == 2, "one or two k-blocks per barrier");
+  // KPB k-blocks per LDS stage (and per barrier), two stages
+  __shared__ __attribute__((aligned(16))) H smem[2][KPB][2 * W * TILE];
 
   const int row0 = tm * SQ_BM, col0 = tn * SQ_BM;
   constexpr int FI = SQ_BM / WAVES_M / 32, FJ = SQ_BM / WAVES_N / 32;
@@ -308,25 +312,49 @@ __device__ __forceinline__ void sq_split_tile(const SqArgs<P>& g, int tm, int tn
 
   load(sa0, sb0, 0);
   load(sa1, sb1, min(1, KT - 1));
-  store(sa0, sb0, smem[0]);
-  __syncthreads();
-  // The loads are unconditional (k-block clamped to the last one; the tail
-  // re-reads are never stored), so the memory-counter waits before each
-  // store only cover the older register set.
-  for (int kt = 0; kt < KT; kt += 2) {
-    // even step: k-block kt in smem[0]; set 0 is free for kt + 2
-    load(sa0, sb0, min(kt + 2, KT - 1));
-    compute(smem[0]);
-    flush(kt);
-    if (kt + 1 < KT) store(sa1, sb1, smem[1]);
+  store(sa0, sb0, smem[0][0]);
+  if constexpr (KPB == 1) {
     __syncthreads();
-    if (kt + 1 >= KT) break;
-    // odd step: k-block kt + 1 in smem[1]; set 1 is free for kt + 3
-    load(sa1, sb1, min(kt + 3, KT - 1));
-    compute(smem[1]);
-    flush(kt + 1);
-    if (kt + 2 < KT) store(sa0, sb0, smem[0]);
+    // The loads are unconditional (k-block clamped to the last one; the tail
+    // re-reads are never stored), so the memory-counter waits before each
+    // store only cover the older register set.
+    for (int kt = 0; kt < KT; kt += 2) {
+      // even step: k-block kt in smem[0]; set 0 is free for kt + 2
+      load(sa0, sb0, min(kt + 2, KT - 1));
+      compute(smem[0][0]);
+      flush(kt);
+      if (kt + 1 < KT) store(sa1, sb1, smem[1][0]);
+      __syncthreads();
+      if (kt + 1 >= KT) break;
+      // odd step: k-block kt + 1 in smem[1]; set 1 is free for kt + 3
+      load(sa1, sb1, min(kt + 3, KT - 1));
+      compute(smem[1][0]);
+      flush(kt + 1);
+      if (kt + 2 < KT) store(sa0, sb0, smem[0][0]);
+      __syncthreads();
+    }
+  } else {
+    // Two k-blocks per stage and per barrier: stage cur holds k-blocks kt
+    // and kt + 1; both register sets load kt + 2, kt + 3 (clamped) while
+    // they are multiplied and are stored into the other stage, which every
+    // wave finished reading before the previous barrier.
+    if (KT > 1) store(sa1, sb1, smem[0][1]);
     __syncthreads();
+    int cur = 0;
+    for (int kt = 0; kt < KT; kt += 2) {
+      load(sa0, sb0, min(kt + 2, KT - 1));
+      load(sa1, sb1, min(kt + 3, KT - 1));
+      compute(smem[cur][0]);
+      flush(kt);
+      if (kt + 1 < KT) {
+        compute(smem[cur][1]);
+        flush(kt + 1);
+      }
+      if (kt + 2 < KT) store(sa0, sb0, smem[cur ^ 1][0]);
+      if (kt + 3 < KT) store(sa1, sb1, smem[cur ^ 1][1]);
+      __syncthreads();
+      cur ^= 1;
+    }
   }
 
   // ---- epilogue (diagonal split, lower tiles mirrored) ----
@@ -369,7 +397,7 @@ __device__ __forceinline__ void sq_split_tile(const SqArgs<P>& g, int tm, int tn
   }
 }
 
-template <class P, int WAVES_M = P::WAVES_M, int WAVES_N = P::WAVES_N>
+template <class P, int WAVES_M = P::WAVES_M, int WAVES_N = P::WAVES_N, int KPB = P::KPB>
 __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void sq_split_kernel(SqArgs<P> g) {
   int tm, tn;
   const int lb = xcd_remap((int)blockIdx.x, (int)gridDim.x);
@@ -380,7 +408,7 @@ __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void sq_split_kernel(SqArgs
   } else {
     tri_tile(lb, tm, tn);
   }
-  sq_split_tile<P, WAVES_M, WAVES_N>(g, tm, tn);
+  sq_split_tile<P, WAVES_M, WAVES_N, KPB>(g, tm, tn);
 }
 
 // Grouped split squarings: one problem per walk, lower tiles of SQ_BM; the
@@ -404,7 +432,7 @@ __global__ __launch_bounds__(sq_threads<P>()) void sq_split_group_kernel(const G
     }
     int tm, tn;
     grouped_tri_tile(t - probs[lo].tile0, (probs[lo].g.m + SQ_BM - 1) / SQ_BM, tm, tn);
-    sq_split_tile<P, P::WAVES_M, P::WAVES_N>(probs[lo].g, tm, tn);
+    sq_split_tile<P, P::WAVES_M, P::WAVES_N, P::KPB>(probs[lo].g, tm, tn);
   }
 }
 

@@ -124,6 +124,11 @@ int fexpm_test_mfma_rate(int kind, int iters, double* tflops, double* ms);
  * SIMD for the squaring and int8-slice kernels). */
 int fexpm_test_mfma_rate_waves(int kind, int waves, int iters, double* tflops, double* ms);
 
+/* MFMA flops one FP32 structured (complex symmetric, 3M) squaring of order n
+ * issues under the automatic tile choice, including the tail sub-tiles: the
+ * figure an issued-rate report divides by the launch time. */
+int fexpm_test_square_issued(int64_t n, double* flops);
+
 /* Tile order of the symmetric (lower-triangular) GEMM launches, evaluated on
  * the host: out[b] = (ti << 16) | tj for every b < T(T+1)/2, 1 <= T <= 4096.
  * Groups of 8 x 8 tiles, group rows in order, within one the groups left to
