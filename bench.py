@@ -728,7 +728,7 @@ def _mfma_rates():
 def _issued_structured_squaring(n, kpad, fp32):
     """MFMA flops one structured 3M squaring issues: lower tiles of 64 x 64
     for FP64; for FP32 the engine's own launch plan (64 x 64 tiles up to
-    N = 8192, above it 128 x 128 tiles and, when the last wave of workgroups
+    N = 6144, above it 128 x 128 tiles and, when the last wave of workgroups
     is less than half full, 64 x 64 sub-tiles for it)."""
     if not fp32:
         T = (n + 63) // 64
@@ -939,7 +939,7 @@ def main():
         issued = (_issued_structured_squaring(n, kpad, cdt == np.complex64) if st["structured"]
                   else 3 * 2.0 * np_pad ** 3)
         kname = ("gemm_nt_kernel (3M symmetric squaring on v_mfma_f32_32x32x2, four waves per SIMD: up to "
-                 "N=8192 lower 64x64 tiles of 4 waves, above it 128x128 tiles of 16 waves plus a 64x64 launch "
+                 "N=6144 lower 64x64 tiles of 4 waves, above it 128x128 tiles of 16 waves plus a 64x64 launch "
                  "for a last partial wave of tiles; 8x8 tile groups; each wave 32x32)")
         traffic_key = "fp32_squaring" if cdt == np.complex64 else None
     issued_tf = issued / (sq_ms * 1e-3) / 1e12

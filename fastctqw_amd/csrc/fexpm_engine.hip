@@ -87,13 +87,13 @@ static void choose_degree(double norm, int& m, int& s) {
 static inline int64_t round_up(int64_t x, int64_t r) { return (x + r - 1) / r * r; }
 
 // Tile edge of the FP32 structured squarings of order n: 64 x 64 tiles of 4
-// waves up to n = 8192, 128 x 128 tiles of 16 waves with the tail split
+// waves up to n = 6144, 128 x 128 tiles of 16 waves with the tail split
 // beyond (ms per squaring, 64 vs 128: 0.89 vs 0.93 at 3072, 6.30 vs 6.31 at
-// 6144, 26.15 vs 25.24 at 10^4, 57.05 vs 54.39 at 13000; the same values);
-// forced = 64 or 128 overrides.
+// 6144, 9.55 vs 9.31 at 7168, 14.9 vs 13.95 at 8192, 26.15 vs 25.24 at 10^4,
+// 57.05 vs 54.39 at 13000; the same values); forced = 64 or 128 overrides.
 static int square_tile(int64_t n, int forced) {
   if (forced == 64 || forced == 128) return forced;
-  return n <= 8192 ? 64 : 128;
+  return n <= 6144 ? 64 : 128;
 }
 
 // Launch plan of one FP32 structured squaring of order m in tile x tile

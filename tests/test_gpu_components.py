@@ -148,7 +148,7 @@ def test_gemm_int8_slices(lib, slices, shape, sym):
 
 def test_square_issued_plan(lib):
     """Issued MFMA flops of one FP32 structured squaring: 64 x 64 lower tiles
-    up to n = 8192; above it 128 x 128 tiles whose last partial wave may run
+    up to n = 6144; above it 128 x 128 tiles whose last partial wave may run
     as 64 x 64 sub-tiles, so the count lies between the 64-tile and the full
     128-tile figures and is a whole number of 64 x 64 sub-tile products."""
     lib.fexpm_test_square_issued.argtypes = [ctypes.c_int64, ctypes.POINTER(ctypes.c_double)]
@@ -162,10 +162,10 @@ def test_square_issued_plan(lib):
         T = (n + t - 1) // t
         return T * (T + 1) // 2
 
-    for n in (100, 4096, 8192):
+    for n in (100, 4096, 6144):
         k = (n + 15) // 16 * 16
         assert issued(n) == 6.0 * tiles(n, 64) * 64 * 64 * k
-    for n in (10000, 13000):
+    for n in (6145, 8192, 10000, 13000):
         k = (n + 15) // 16 * 16
         v = issued(n)
         assert 6.0 * tiles(n, 64) * 64 * 64 * k <= v <= 6.0 * tiles(n, 128) * 128 * 128 * k
