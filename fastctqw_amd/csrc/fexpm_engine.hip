@@ -30,6 +30,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -1587,6 +1588,43 @@ static void check_device() {
   if (hipGetDeviceCount(&count) != hipSuccess || count < 1) throw Error(FEXPM_ENODEVICE, "no HIP device available");
 }
 
+// Host output buffers are usually fresh memory (a new numpy array): a copy
+// into them takes a page fault per 4 KB page on the thread doing the copy
+// (about 50 ms per 0.8 GB).  Prefault writes one byte per page on `threads`
+// host threads; the buffer is overwritten afterwards, so the values written
+// do not matter.  The joins happen in the destructor, so every return path
+// waits for the threads before the buffer is copied into or handed back.
+class Prefault {
+ public:
+  Prefault(void* p, size_t bytes, int threads) {
+    if (!p || bytes < ((size_t)16 << 20)) return;
+    constexpr size_t kPage = 4096;
+    const size_t pages = (bytes + kPage - 1) / kPage;
+    const size_t per = (pages + threads - 1) / threads;
+    char* base = static_cast<char*>(p);
+    try {
+      for (int t = 0; t < threads; ++t) {
+        const size_t lo = (size_t)t * per, hi = std::min(pages, lo + per);
+        if (lo >= hi) break;
+        th_.emplace_back([base, lo, hi, bytes] {
+          volatile char* q = base;
+          for (size_t pg = lo; pg < hi; ++pg) q[std::min(pg * kPage, bytes - 1)] = 0;
+        });
+      }
+    } catch (...) {
+      // no thread: the copy faults the pages in itself
+    }
+  }
+  ~Prefault() { join(); }
+  void join() {
+    for (auto& t : th_) t.join();
+    th_.clear();
+  }
+
+ private:
+  std::vector<std::thread> th_;
+};
+
 static void ensure_io(fexpm_engine* e) {
   const size_t bytes = (size_t)e->n * (size_t)e->n * e->esize;
   if (!e->d_in) {
@@ -1771,7 +1809,11 @@ int fexpm_download(fexpm_engine* e, void* u) {
   if (e->n == 0) return FEXPM_OK;
   if (!e->d_out) return FEXPM_EINVAL;
   return guarded(e, [&] {
-    FX_HIP(hipMemcpy(u, e->d_out, (size_t)e->n * e->n * e->esize, hipMemcpyDeviceToHost));
+    const size_t bytes = (size_t)e->n * e->n * e->esize;
+    {
+      Prefault pf(u, bytes, 8);
+    }
+    FX_HIP(hipMemcpy(u, e->d_out, bytes, hipMemcpyDeviceToHost));
   });
 }
 
@@ -1783,15 +1825,22 @@ int fexpm_run(fexpm_engine* e, const void* a, void* u) {
     return FEXPM_EFREED;
   }
   if (e->n == 0) return FEXPM_OK;
+  const size_t bytes = (size_t)e->n * e->n * e->esize;
+  // the output's pages are faulted in on one host thread while the device
+  // works (not when input and output overlap: the input is read meanwhile)
+  const char *ca = static_cast<const char*>(a), *cu = static_cast<const char*>(u);
+  const bool overlap = ca < cu + bytes && cu < ca + bytes;
+  Prefault pf(overlap ? nullptr : u, bytes, 1);
   int rc = guarded(e, [&] {
     ensure_io(e);
-    FX_HIP(hipMemcpyAsync(e->d_in, a, (size_t)e->n * e->n * e->esize, hipMemcpyHostToDevice, e->stream));
+    FX_HIP(hipMemcpyAsync(e->d_in, a, bytes, hipMemcpyHostToDevice, e->stream));
   });
   if (rc != FEXPM_OK) return rc;
   rc = run_device_locked(e, e->d_in, e->d_out, e->stream);
   if (rc != FEXPM_OK) return rc;
+  pf.join();
   return guarded(e, [&] {
-    FX_HIP(hipMemcpyAsync(u, e->d_out, (size_t)e->n * e->n * e->esize, hipMemcpyDeviceToHost, e->stream));
+    FX_HIP(hipMemcpyAsync(u, e->d_out, bytes, hipMemcpyDeviceToHost, e->stream));
     FX_HIP(hipStreamSynchronize(e->stream));
   });
 }
@@ -1988,7 +2037,6 @@ double fexpm_gpu_memory_gb(void) {
 
 // ===========================================================================
 // Batches of independent expms of uneven order (many walks).
-#include <thread>
 
 struct fexpm_batch {
   int dtype = 0;

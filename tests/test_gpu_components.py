@@ -173,6 +173,34 @@ def test_square_issued_plan(lib):
     assert lib.fexpm_test_square_issued(0, ctypes.byref(ctypes.c_double())) != 0
 
 
+def test_run_host_buffers_in_place(lib):
+    """fexpm_run on host buffers above the size at which the engine faults
+    the output's pages in on a host thread while the device works (16 MiB):
+    a fresh output buffer and the input buffer reused as the output (no
+    prefault then, the input is still being read) give the same U bit for
+    bit, and the fresh one matches the eigen oracle."""
+    n = 1500
+    _say(f"fexpm_run host buffers n={n}")
+    L = laplacian(erdos_renyi(n, 0.05, seed=71, weighted=True))
+    a = np.ascontiguousarray((-1j * 2.0 * L).astype(np.complex64))
+    e = ctypes.c_void_p()
+    assert lib.fexpm_create(ctypes.c_int64(n), 2, ctypes.byref(e)) == 0
+    try:
+        u = np.empty((n, n), np.complex64)
+        rc = lib.fexpm_run(e, a.ctypes.data_as(ctypes.c_void_p), u.ctypes.data_as(ctypes.c_void_p))
+        assert rc == 0, lib.fexpm_last_error(e)
+        buf = a.copy()
+        p = buf.ctypes.data_as(ctypes.c_void_p)
+        rc = lib.fexpm_run(e, p, p)
+        assert rc == 0, lib.fexpm_last_error(e)
+    finally:
+        lib.fexpm_destroy(e)
+    assert np.array_equal(u, buf)
+    lam, V = np.linalg.eigh(np.ascontiguousarray(a.imag, dtype=np.float64))
+    ref = (V * np.exp(1j * lam)[None, :]) @ V.T
+    assert rel_fro(ref, u.astype(np.complex128)) < 1e-4
+
+
 def test_gemm_split_range_error(lib):
     """fp16 parts hold |x| * 2^14 <= 65504: a larger operand is reported."""
     _say("split squaring range error")
