@@ -1534,6 +1534,19 @@ template <typename W> struct Pipeline : PipelineBase {
 // C ABI
 using namespace fexpm;
 
+// Copies of n x n matrices between pageable host memory and the device
+// through two pinned staging buffers: the host side of each chunk is copied
+// by several threads while the DMA engine moves the neighbouring chunk (the
+// runtime's own pageable path copies through its staging on one thread).
+struct HostStaging {
+  static constexpr size_t kChunk = (size_t)32 << 20;
+  static constexpr size_t kMin = (size_t)64 << 20;  // smaller copies: plain hipMemcpy
+  static constexpr int kThreads = 8;
+  void* buf[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  bool ready = false;
+};
+
 struct fexpm_engine {
   int64_t n = 0;
   int dtype = 0;
@@ -1550,6 +1563,7 @@ struct fexpm_engine {
   int opt_precision = 0;
   double* d_h = nullptr;  // walk Hamiltonian kept by fexpm_set_graph (FP64, n x n)
   bool have_graph = false;
+  HostStaging stg;
 };
 
 static size_t elem_size(int dtype) {
@@ -1625,6 +1639,80 @@ class Prefault {
   std::vector<std::thread> th_;
 };
 
+// dst[0, len) = src[0, len) on `threads` host threads (the calling one included)
+static void par_copy(void* dst, const void* src, size_t len, int threads) {
+  const size_t part = ((len + threads - 1) / threads + 4095) / 4096 * 4096;
+  std::vector<std::thread> th;
+  size_t off = part;
+  try {
+    for (; off < len; off += part) {
+      const size_t l = std::min(part, len - off);
+      th.emplace_back([=] { std::memcpy((char*)dst + off, (const char*)src + off, l); });
+    }
+  } catch (...) {
+    for (; off < len; off += part) std::memcpy((char*)dst + off, (const char*)src + off, std::min(part, len - off));
+  }
+  std::memcpy(dst, src, std::min(part, len));
+  for (auto& t : th) t.join();
+}
+
+static void staging_init(HostStaging& h) {
+  if (h.ready) return;
+  for (int b = 0; b < 2; ++b) {
+    FX_HIP(hipHostMalloc(&h.buf[b], HostStaging::kChunk, hipHostMallocDefault));
+    FX_HIP(hipEventCreateWithFlags(&h.ev[b], hipEventDisableTiming));
+  }
+  h.ready = true;
+}
+
+static void staging_free(HostStaging& h) {
+  for (int b = 0; b < 2; ++b) {
+    if (h.ev[b]) (void)hipEventSynchronize(h.ev[b]);
+    if (h.buf[b]) (void)hipHostFree(h.buf[b]);
+    if (h.ev[b]) (void)hipEventDestroy(h.ev[b]);
+    h.buf[b] = nullptr;
+    h.ev[b] = nullptr;
+  }
+  h.ready = false;
+}
+
+// device d[0, bytes) = host h[0, bytes), enqueued on st (the last chunks may
+// still be in flight on return; later work on st is ordered after them)
+static void staged_h2d(HostStaging& hs, void* d, const void* h, size_t bytes, hipStream_t st) {
+  staging_init(hs);
+  const size_t C = HostStaging::kChunk, nch = (bytes + C - 1) / C;
+  for (size_t c = 0; c < nch; ++c) {
+    const int b = (int)(c & 1);
+    const size_t off = c * C, len = std::min(C, bytes - off);
+    FX_HIP(hipEventSynchronize(hs.ev[b]));  // the DMA that last read buf[b] is done
+    par_copy(hs.buf[b], (const char*)h + off, len, HostStaging::kThreads);
+    FX_HIP(hipMemcpyAsync((char*)d + off, hs.buf[b], len, hipMemcpyHostToDevice, st));
+    FX_HIP(hipEventRecord(hs.ev[b], st));
+  }
+}
+
+// host h[0, bytes) = device d[0, bytes) after the work enqueued on st; complete on return
+static void staged_d2h(HostStaging& hs, void* h, const void* d, size_t bytes, hipStream_t st) {
+  staging_init(hs);
+  const size_t C = HostStaging::kChunk, nch = (bytes + C - 1) / C;
+  auto issue = [&](size_t c) {
+    const int b = (int)(c & 1);
+    const size_t off = c * C;
+    FX_HIP(hipEventSynchronize(hs.ev[b]));
+    FX_HIP(hipMemcpyAsync(hs.buf[b], (const char*)d + off, std::min(C, bytes - off), hipMemcpyDeviceToHost, st));
+    FX_HIP(hipEventRecord(hs.ev[b], st));
+  };
+  issue(0);
+  if (nch > 1) issue(1);
+  for (size_t c = 0; c < nch; ++c) {
+    const int b = (int)(c & 1);
+    const size_t off = c * C;
+    FX_HIP(hipEventSynchronize(hs.ev[b]));
+    par_copy((char*)h + off, hs.buf[b], std::min(C, bytes - off), HostStaging::kThreads);
+    if (c + 2 < nch) issue(c + 2);
+  }
+}
+
 static void ensure_io(fexpm_engine* e) {
   const size_t bytes = (size_t)e->n * (size_t)e->n * e->esize;
   if (!e->d_in) {
@@ -1636,6 +1724,7 @@ static void ensure_io(fexpm_engine* e) {
 }
 
 static void release(fexpm_engine* e) {
+  staging_free(e->stg);
   e->pipe.reset();
   if (e->d_in) (void)hipFree(e->d_in);
   if (e->d_out) (void)hipFree(e->d_out);
@@ -1726,7 +1815,13 @@ int fexpm_upload(fexpm_engine* e, const void* a) {
   if (e->n == 0) return FEXPM_OK;
   return guarded(e, [&] {
     ensure_io(e);
-    FX_HIP(hipMemcpy(e->d_in, a, (size_t)e->n * e->n * e->esize, hipMemcpyHostToDevice));
+    const size_t bytes = (size_t)e->n * e->n * e->esize;
+    if (bytes >= HostStaging::kMin) {
+      staged_h2d(e->stg, e->d_in, a, bytes, e->stream);
+      FX_HIP(hipStreamSynchronize(e->stream));
+    } else {
+      FX_HIP(hipMemcpy(e->d_in, a, bytes, hipMemcpyHostToDevice));
+    }
   });
 }
 
@@ -1810,6 +1905,11 @@ int fexpm_download(fexpm_engine* e, void* u) {
   if (!e->d_out) return FEXPM_EINVAL;
   return guarded(e, [&] {
     const size_t bytes = (size_t)e->n * e->n * e->esize;
+    if (bytes >= HostStaging::kMin) {
+      FX_HIP(hipStreamSynchronize(e->stream));
+      staged_d2h(e->stg, u, e->d_out, bytes, e->stream);
+      return;
+    }
     {
       Prefault pf(u, bytes, 8);
     }
@@ -1831,15 +1931,21 @@ int fexpm_run(fexpm_engine* e, const void* a, void* u) {
   const char *ca = static_cast<const char*>(a), *cu = static_cast<const char*>(u);
   const bool overlap = ca < cu + bytes && cu < ca + bytes;
   Prefault pf(overlap ? nullptr : u, bytes, 1);
+  const bool staged = bytes >= HostStaging::kMin && !overlap;
   int rc = guarded(e, [&] {
     ensure_io(e);
-    FX_HIP(hipMemcpyAsync(e->d_in, a, bytes, hipMemcpyHostToDevice, e->stream));
+    if (staged) staged_h2d(e->stg, e->d_in, a, bytes, e->stream);
+    else FX_HIP(hipMemcpyAsync(e->d_in, a, bytes, hipMemcpyHostToDevice, e->stream));
   });
   if (rc != FEXPM_OK) return rc;
   rc = run_device_locked(e, e->d_in, e->d_out, e->stream);
   if (rc != FEXPM_OK) return rc;
   pf.join();
   return guarded(e, [&] {
+    if (staged) {
+      staged_d2h(e->stg, u, e->d_out, bytes, e->stream);
+      return;
+    }
     FX_HIP(hipMemcpyAsync(u, e->d_out, bytes, hipMemcpyDeviceToHost, e->stream));
     FX_HIP(hipStreamSynchronize(e->stream));
   });

@@ -173,32 +173,41 @@ def test_square_issued_plan(lib):
     assert lib.fexpm_test_square_issued(0, ctypes.byref(ctypes.c_double())) != 0
 
 
-def test_run_host_buffers_in_place(lib):
+@pytest.mark.parametrize("n", [1500, 3000])
+def test_run_host_buffers_in_place(lib, n):
     """fexpm_run on host buffers above the size at which the engine faults
-    the output's pages in on a host thread while the device works (16 MiB):
-    a fresh output buffer and the input buffer reused as the output (no
-    prefault then, the input is still being read) give the same U bit for
-    bit, and the fresh one matches the eigen oracle."""
-    n = 1500
+    the output's pages in on a host thread while the device works (16 MiB)
+    and, at n = 3000 (72 MB), above the size at which host copies go through
+    pinned staging buffers (64 MiB): a fresh output buffer, the input buffer
+    reused as the output (plain copies then: the input is still being read),
+    and fexpm_upload / fexpm_run_resident / fexpm_download give the same U
+    bit for bit; at n = 1500 U matches the eigen oracle."""
     _say(f"fexpm_run host buffers n={n}")
     L = laplacian(erdos_renyi(n, 0.05, seed=71, weighted=True))
     a = np.ascontiguousarray((-1j * 2.0 * L).astype(np.complex64))
     e = ctypes.c_void_p()
+    vp = ctypes.c_void_p
     assert lib.fexpm_create(ctypes.c_int64(n), 2, ctypes.byref(e)) == 0
     try:
         u = np.empty((n, n), np.complex64)
-        rc = lib.fexpm_run(e, a.ctypes.data_as(ctypes.c_void_p), u.ctypes.data_as(ctypes.c_void_p))
+        rc = lib.fexpm_run(e, a.ctypes.data_as(vp), u.ctypes.data_as(vp))
         assert rc == 0, lib.fexpm_last_error(e)
         buf = a.copy()
-        p = buf.ctypes.data_as(ctypes.c_void_p)
+        p = buf.ctypes.data_as(vp)
         rc = lib.fexpm_run(e, p, p)
         assert rc == 0, lib.fexpm_last_error(e)
+        u2 = np.empty((n, n), np.complex64)
+        assert lib.fexpm_upload(e, a.ctypes.data_as(vp)) == 0, lib.fexpm_last_error(e)
+        assert lib.fexpm_run_resident(e) == 0, lib.fexpm_last_error(e)
+        assert lib.fexpm_download(e, u2.ctypes.data_as(vp)) == 0, lib.fexpm_last_error(e)
     finally:
         lib.fexpm_destroy(e)
     assert np.array_equal(u, buf)
-    lam, V = np.linalg.eigh(np.ascontiguousarray(a.imag, dtype=np.float64))
-    ref = (V * np.exp(1j * lam)[None, :]) @ V.T
-    assert rel_fro(ref, u.astype(np.complex128)) < 1e-4
+    assert np.array_equal(u, u2)
+    if n <= 1500:
+        lam, V = np.linalg.eigh(np.ascontiguousarray(a.imag, dtype=np.float64))
+        ref = (V * np.exp(1j * lam)[None, :]) @ V.T
+        assert rel_fro(ref, u.astype(np.complex128)) < 1e-4
 
 
 def test_gemm_split_range_error(lib):
