@@ -410,7 +410,7 @@ struct GroupLaunch {
 
 // Tile edge of a GEMM configuration (all grouped configurations are square).
 static inline int group_tile(bool f32, int cfg) {
-  if (f32) return cfg == 0 ? FCfgSmall::BM : GemmCfg<float>::BM;
+  if (f32) return cfg == 0 ? FCfgSmall::BM : FCfg16w::BM;
   return cfg == 0 ? GemmCfg<double>::BM : DCfg4w::BM;
 }
 
@@ -626,7 +626,7 @@ struct GroupProgram {
         if (L.dispatch == 2) {
           if (S.f32)
             capw = L.cfg == 0 ? group_gemm_capacity<float, MODE_C3M, true, FCfgSmall>()
-                              : group_gemm_capacity<float, MODE_C3M, true, GemmCfg<float>>();
+                              : group_gemm_capacity<float, MODE_C3M, true, FCfg16w>();
           else if (S.mode == MODE_REAL)
             capw = L.cfg == 0 ? group_gemm_capacity<double, MODE_REAL, false, GemmCfg<double>>()
                               : group_gemm_capacity<double, MODE_REAL, false, DCfg4w>();
@@ -696,7 +696,7 @@ static void group_enqueue_one(const GroupLaunch& L, unsigned char* base, hipStre
                                                                L.nprob, L.total, L.chunk)
       if (L.f32) {
         if (L.cfg == 0) FX_GG(float, MODE_C3M, true, FCfgSmall);
-        else FX_GG(float, MODE_C3M, true, GemmCfg<float>);
+        else FX_GG(float, MODE_C3M, true, FCfg16w);
       } else if (L.mode == MODE_REAL) {
         if (L.cfg == 0) FX_GG(double, MODE_REAL, false, GemmCfg<double>);
         else FX_GG(double, MODE_REAL, false, DCfg4w);
