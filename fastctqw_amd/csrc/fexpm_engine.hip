@@ -2174,9 +2174,11 @@ struct fexpm_batch {
   int precision = FEXPM_PREC_NATIVE;
   int spd_solve = 1;
   fexpm_batch_report report{};
+  HostStaging stg;  // host copies of 64 MiB and more
 };
 
 static void batch_release(fexpm_batch* b) {
+  staging_free(b->stg);
   b->grp.reset();
   b->pipes.clear();
   for (hipStream_t s : b->streams) (void)hipStreamDestroy(s);
@@ -2357,7 +2359,12 @@ int fexpm_batch_upload(fexpm_batch* b, int64_t count, const int64_t* n, const vo
       FX_HIP(hipMalloc(&b->d_psi, vt * sizeof(double2)));
       b->cap_states = vt;
     }
-    if (tot) FX_HIP(hipMemcpy(b->d_in, a, tot * b->esize, hipMemcpyHostToDevice));
+    if (tot * b->esize >= HostStaging::kMin) {
+      staged_h2d(b->stg, b->d_in, a, tot * b->esize, nullptr);
+      FX_HIP(hipStreamSynchronize(nullptr));
+    } else if (tot) {
+      FX_HIP(hipMemcpy(b->d_in, a, tot * b->esize, hipMemcpyHostToDevice));
+    }
     b->stats.assign(count, fexpm_stats());
     if ((size_t)count > b->cap_walks) {
       if (b->d_scal) (void)hipFree(b->d_scal);

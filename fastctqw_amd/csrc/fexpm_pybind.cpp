@@ -13,11 +13,14 @@
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 
+#include <algorithm>
 #include <complex>
 #include <cstring>
-#include <vector>
+#include <memory>
 #include <stdexcept>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "../../include/fastexpm.h"
 #include "../../include/fastexpm_testing.h"
@@ -275,25 +278,53 @@ class ExpMatBatch {
   }
   void set_option(int key, int64_t value) { check(fexpm_batch_set_option(b_, key, value)); }
 
-  // mats: list of square complex arrays (cast to the batch dtype)
+  // mats: list of square complex arrays (cast to the batch dtype), packed
+  // into one host buffer (allocated once, filled by several threads)
   void upload(py::list mats) {
-    n_.clear();
-    std::vector<char> host;
+    std::vector<int64_t> ns;
+    std::vector<py::array> cs;
+    std::vector<size_t> offs;
+    size_t total = 0;
     for (auto h : mats) {
       py::array a = py::array::ensure(h, py::array::c_style | py::array::forcecast);
       if (a.ndim() != 2 || a.shape(0) != a.shape(1)) throw std::invalid_argument("batch: square matrices expected");
       py::array c = dt_ == FEXPM_COMPLEX64
                         ? (py::array)py::array_t<std::complex<float>, py::array::c_style | py::array::forcecast>(a)
                         : (py::array)py::array_t<std::complex<double>, py::array::c_style | py::array::forcecast>(a);
-      const size_t off = host.size();
-      host.resize(off + (size_t)c.nbytes());
-      std::memcpy(host.data() + off, c.data(), (size_t)c.nbytes());
-      n_.push_back(a.shape(0));
+      offs.push_back(total);
+      total += (size_t)c.nbytes();
+      ns.push_back(a.shape(0));
+      cs.push_back(c);
+    }
+    n_ = ns;
+    std::unique_ptr<char[]> host(new char[std::max<size_t>(total, 1)]);
+    std::vector<const void*> src;
+    std::vector<size_t> len;
+    for (const py::array& c : cs) {
+      src.push_back(c.data());
+      len.push_back((size_t)c.nbytes());
     }
     int rc;
     {
       py::gil_scoped_release nogil;
-      rc = fexpm_batch_upload(b_, (int64_t)n_.size(), n_.data(), host.data());
+      // matrix i goes to thread i % nth (the arrays stay alive in cs)
+      const int nth = total >= ((size_t)64 << 20) ? 8 : 1;
+      auto work = [&](int t) {
+        for (size_t i = (size_t)t; i < src.size(); i += (size_t)nth) std::memcpy(host.get() + offs[i], src[i], len[i]);
+      };
+      std::vector<std::thread> th;
+      int t0 = 0;
+      try {
+        for (int t = 1; t < nth; ++t) {
+          th.emplace_back(work, t);
+          t0 = t;
+        }
+      } catch (...) {
+      }
+      for (int t = t0 + 1; t < nth; ++t) work(t);  // threads that could not be started
+      work(0);
+      for (auto& t : th) t.join();
+      rc = fexpm_batch_upload(b_, (int64_t)n_.size(), n_.data(), host.get());
     }
     check(rc);
   }
