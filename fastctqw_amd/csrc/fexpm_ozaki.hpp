@@ -92,43 +92,64 @@ __device__ __forceinline__ void oz_slice_row(const double* __restrict__ x1, doub
   }
 }
 
+// Orders the LDS writes of a wave before its later LDS reads (lanes of one
+// wave exchanging data through LDS): wavefront-scope release fence, the
+// wave barrier, acquire fence.  No other wave takes part.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // The same slicing, four rows per workgroup of 256 threads (one wave per
-// row, rows r0 .. r0 + 3 of one operand, so every wave makes the same
-// number of passes): for the small orders of batched walks, where one
-// workgroup per row is mostly idle.  rows must be a multiple of 4.
+// row, rows r0 .. r0 + 3 of one operand; each wave syncs only with itself):
+// for the small orders of batched walks, where one workgroup per row is
+// mostly idle.  rows must be a multiple of 4.
 template <int S>
 __device__ __forceinline__ void oz_slice_rows4(const double* __restrict__ x1, double s1,
                                                const double* __restrict__ x2, double s2, int k1, int64_t ld, int nr,
                                                int nk, int rows, int kpad, int8_t* __restrict__ out,
                                                int* __restrict__ ex, int r0) {
-  __shared__ __attribute__((aligned(16))) int8_t buf[4][S][64];
+  constexpr int KV = 256;  // values of the row per pass: four loads per lane in flight
+  __shared__ __attribute__((aligned(16))) int8_t buf[4][S][KV];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, r = r0 + w;
   double mx = 0.0;
-  if (r < nr)
+  if (r < nr) {
+#pragma unroll 4
     for (int k = lane; k < nk; k += 64) mx = fmax(mx, fabs(oz_elem(x1, s1, x2, s2, k1, ld, r, k)));
+  }
   for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
   int e = 0;
   if (mx > 0.0) (void)frexp(mx, &e);
   if (lane == 0 && r < rows) ex[r] = e;
   const double sc = ldexp(1.0, 6 - e);
-  for (int k0 = 0; k0 < kpad; k0 += 64) {
-    const int k = k0 + lane;
-    double y = (r < nr && k < nk) ? oz_elem(x1, s1, x2, s2, k1, ld, r, k) * sc : 0.0;
+  for (int k0 = 0; k0 < kpad; k0 += KV) {
+    double y[KV / 64];
 #pragma unroll
-    for (int p = 0; p < S; ++p) {
-      const double a = rint(y);
-      buf[w][p][lane] = (int8_t)a;
-      y = (y - a) * 128.0;
+    for (int j = 0; j < KV / 64; ++j) {
+      const int k = k0 + 64 * j + lane;
+      y[j] = (r < nr && k < nk) ? oz_elem(x1, s1, x2, s2, k1, ld, r, k) * sc : 0.0;
     }
-    __syncthreads();
-    if (lane < 4 * S && r < rows) {  // 2 k-blocks x S slices x 2 halves of 16 bytes
-      const int kbl = lane / (2 * S), q = lane % (2 * S), p = q >> 1, h = q & 1;
-      const int kb = k0 / OZ_BK + kbl;
-      if (kb * OZ_BK < kpad)
-        *reinterpret_cast<i32x4*>(out + ((int64_t)kb * rows + r) * S * OZ_BK + p * OZ_BK + 16 * h) =
-            *reinterpret_cast<const i32x4*>(&buf[w][p][kbl * OZ_BK + 16 * h]);
+#pragma unroll
+    for (int j = 0; j < KV / 64; ++j)
+#pragma unroll
+      for (int p = 0; p < S; ++p) {
+        const double a = rint(y[j]);  // |y| <= 64, so a fits; the rest is exact
+        buf[w][p][64 * j + lane] = (int8_t)a;
+        y[j] = (y[j] - a) * 128.0;
+      }
+    wave_sync();  // buf[w] is this wave's own: no other wave reads it
+    if (r < rows) {
+      // KV / OZ_BK k-blocks x S slices x 2 halves of 16 bytes
+      for (int c = lane; c < (KV / OZ_BK) * S * 2; c += 64) {
+        const int kbl = c / (2 * S), q = c % (2 * S), p = q >> 1, h = q & 1;
+        const int kb = k0 / OZ_BK + kbl;
+        if (kb * OZ_BK < kpad)
+          *reinterpret_cast<i32x4*>(out + ((int64_t)kb * rows + r) * S * OZ_BK + p * OZ_BK + 16 * h) =
+              *reinterpret_cast<const i32x4*>(&buf[w][p][kbl * OZ_BK + 16 * h]);
+      }
     }
-    __syncthreads();
+    wave_sync();  // the reads above complete before the next pass writes buf[w]
   }
 }
 
