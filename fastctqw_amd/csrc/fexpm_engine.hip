@@ -1545,6 +1545,7 @@ struct HostStaging {
   void* buf[2] = {nullptr, nullptr};
   hipEvent_t ev[2] = {nullptr, nullptr};
   bool ready = false;
+  bool failed = false;  // pinned memory unavailable: plain copies from then on
 };
 
 struct fexpm_engine {
@@ -1656,15 +1657,6 @@ static void par_copy(void* dst, const void* src, size_t len, int threads) {
   for (auto& t : th) t.join();
 }
 
-static void staging_init(HostStaging& h) {
-  if (h.ready) return;
-  for (int b = 0; b < 2; ++b) {
-    FX_HIP(hipHostMalloc(&h.buf[b], HostStaging::kChunk, hipHostMallocDefault));
-    FX_HIP(hipEventCreateWithFlags(&h.ev[b], hipEventDisableTiming));
-  }
-  h.ready = true;
-}
-
 static void staging_free(HostStaging& h) {
   for (int b = 0; b < 2; ++b) {
     if (h.ev[b]) (void)hipEventSynchronize(h.ev[b]);
@@ -1674,12 +1666,37 @@ static void staging_free(HostStaging& h) {
     h.ev[b] = nullptr;
   }
   h.ready = false;
+  h.failed = false;
+}
+
+// Allocates the staging buffers on first use; false when pinned host memory
+// is not available (e.g. a low locked-memory limit): the caller then copies
+// through the runtime's pageable path.
+static bool staging_init(HostStaging& h) {
+  if (h.ready) return true;
+  if (h.failed) return false;
+  bool ok = true;
+  for (int b = 0; b < 2 && ok; ++b) {
+    ok = hipHostMalloc(&h.buf[b], HostStaging::kChunk, hipHostMallocDefault) == hipSuccess &&
+         hipEventCreateWithFlags(&h.ev[b], hipEventDisableTiming) == hipSuccess;
+  }
+  if (!ok) {
+    (void)hipGetLastError();
+    staging_free(h);
+    h.failed = true;
+    return false;
+  }
+  h.ready = true;
+  return true;
 }
 
 // device d[0, bytes) = host h[0, bytes), enqueued on st (the last chunks may
 // still be in flight on return; later work on st is ordered after them)
 static void staged_h2d(HostStaging& hs, void* d, const void* h, size_t bytes, hipStream_t st) {
-  staging_init(hs);
+  if (!staging_init(hs)) {
+    FX_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, st));
+    return;
+  }
   const size_t C = HostStaging::kChunk, nch = (bytes + C - 1) / C;
   for (size_t c = 0; c < nch; ++c) {
     const int b = (int)(c & 1);
@@ -1693,7 +1710,11 @@ static void staged_h2d(HostStaging& hs, void* d, const void* h, size_t bytes, hi
 
 // host h[0, bytes) = device d[0, bytes) after the work enqueued on st; complete on return
 static void staged_d2h(HostStaging& hs, void* h, const void* d, size_t bytes, hipStream_t st) {
-  staging_init(hs);
+  if (!staging_init(hs)) {
+    FX_HIP(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, st));
+    FX_HIP(hipStreamSynchronize(st));
+    return;
+  }
   const size_t C = HostStaging::kChunk, nch = (bytes + C - 1) / C;
   auto issue = [&](size_t c) {
     const int b = (int)(c & 1);
@@ -2066,6 +2087,10 @@ int fexpm_set_option(fexpm_engine* e, int key, int64_t value) {
       return FEXPM_OK;
     case FEXPM_OPT_Z_LEFT:
       if (e->pipe) e->pipe->z_left = value ? 1 : 0;
+      return FEXPM_OK;
+    case FEXPM_OPT_HOST_STAGING:
+      staging_free(e->stg);
+      e->stg.failed = value == 0;
       return FEXPM_OK;
     case FEXPM_OPT_SQUARE_TILE:
       if (value != 0 && value != 64 && value != 128) {

@@ -454,6 +454,7 @@ PYBIND11_MODULE(_fastexpm_core, m) {
   m.attr("OPT_CHOL_OUTER") = (int)FEXPM_OPT_CHOL_OUTER;
   m.attr("OPT_SQUARE_TILE") = (int)FEXPM_OPT_SQUARE_TILE;
   m.attr("OPT_Z_LEFT") = (int)FEXPM_OPT_Z_LEFT;
+  m.attr("OPT_HOST_STAGING") = (int)FEXPM_OPT_HOST_STAGING;
   m.attr("OPT_BATCH_GROUPED") = (int)FEXPM_OPT_BATCH_GROUPED;
   m.attr("OPT_AUTOTUNE") = (int)FEXPM_OPT_AUTOTUNE;
   m.attr("OPT_GRAPH") = (int)FEXPM_OPT_GRAPH;

@@ -61,6 +61,11 @@ enum { FEXPM_OPT_SQUARE_TILE = 106 };
  * outer width after each block).  Same flops, different summation order. */
 enum { FEXPM_OPT_Z_LEFT = 107 };
 
+/* Engine option (fexpm_set_option): host copies of 64 MiB and more through
+ * the pinned staging buffers (1, default) or the runtime's pageable path (0,
+ * also taken when pinned host memory cannot be allocated). */
+enum { FEXPM_OPT_HOST_STAGING = 108 };
+
 /* Internal planes readable by fexpm_debug_plane (working precision, n x n). */
 enum fexpm_plane {
   FEXPM_PLANE_A = 0,   /* (A - mu I) / 2^s; structured: its imaginary part G */

@@ -200,10 +200,16 @@ def test_run_host_buffers_in_place(lib, n):
         assert lib.fexpm_upload(e, a.ctypes.data_as(vp)) == 0, lib.fexpm_last_error(e)
         assert lib.fexpm_run_resident(e) == 0, lib.fexpm_last_error(e)
         assert lib.fexpm_download(e, u2.ctypes.data_as(vp)) == 0, lib.fexpm_last_error(e)
+        # the pageable path (taken when pinned memory cannot be allocated)
+        assert lib.fexpm_set_option(e, 108, ctypes.c_int64(0)) == 0  # FEXPM_OPT_HOST_STAGING
+        u3 = np.empty((n, n), np.complex64)
+        rc = lib.fexpm_run(e, a.ctypes.data_as(vp), u3.ctypes.data_as(vp))
+        assert rc == 0, lib.fexpm_last_error(e)
     finally:
         lib.fexpm_destroy(e)
     assert np.array_equal(u, buf)
     assert np.array_equal(u, u2)
+    assert np.array_equal(u, u3)
     if n <= 1500:
         lam, V = np.linalg.eigh(np.ascontiguousarray(a.imag, dtype=np.float64))
         ref = (V * np.exp(1j * lam)[None, :]) @ V.T
