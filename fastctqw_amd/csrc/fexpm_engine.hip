@@ -111,18 +111,17 @@ static SquarePlan square_plan(int m, int tile, bool tail_split) {
   p.nb = T * (T + 1) / 2;
   p.full = p.nb;
   if (tile != 128 || !tail_split) return p;
-  static int cap = -1;
-  if (cap < 0) {
+  // resident workgroups of the 128-tile kernel on the device (thread-safe one-time initialisation)
+  static const int cap = [] {
     int dev = 0, per_cu = 0;
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
         hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &per_cu, (const void*)gemm_nt_kernel<float, MODE_C3M, true, true, FCfg16w>, FCfg16w::NT, 0) ==
             hipSuccess)
-      cap = std::max(1, per_cu) * prop.multiProcessorCount;
-    else
-      cap = 0;
-  }
+      return std::max(1, per_cu) * prop.multiProcessorCount;
+    return 0;
+  }();
   static_assert(FCfg16w::BM == 2 * FCfgSmall::BM && FCfg16w::BN == 2 * FCfgSmall::BN, "tail sub-tiles halve the tile");
   const int rest = cap > 0 ? p.nb % cap : 0;
   if (!(cap > 0 && p.nb - rest > 0 && rest > 0 && 2 * rest <= cap)) return p;
