@@ -209,16 +209,14 @@ static int g_panel_coop = 1;
 
 // Largest cooperative grid of lu_panel_coop that the device can hold.
 static int panel_coop_capacity() {
-  static int cap = -1;
-  if (cap < 0) {
+  static const int cap = [] {  // thread-safe one-time initialisation
     int dev = 0, per_cu = 0;
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)lu_panel_coop, 256, 0) != hipSuccess)
-      cap = 0;
-    else
-      cap = per_cu * prop.multiProcessorCount;
-  }
+      return 0;
+    return per_cu * prop.multiProcessorCount;
+  }();
   return cap;
 }
 
