@@ -35,17 +35,35 @@ template <> struct InElem<double> {
 
 enum AnalyzeFlags { FLAG_NONZERO_RE = 1, FLAG_ASYM = 2, FLAG_NONFINITE = 4 };
 
+// Double-double accumulation (Knuth's two-sum, additions only, so FMA
+// contraction cannot change it): (hi, lo) holds a sum to about 106 bits.
+__device__ __forceinline__ void dd_add(double& hi, double& lo, double x) {
+  const double s = hi + x;
+  const double bb = s - hi;
+  const double err = (hi - (s - bb)) + (x - bb);
+  hi = s;
+  lo += err;
+}
+// Normalised pair: hi = fl(hi + lo), lo = the exact remainder.
+__device__ __forceinline__ double2 dd_norm(double hi, double lo) {
+  const double s = hi + lo;
+  return make_double2(s, lo - (s - hi));
+}
+
 // One 32x32 tile pair (I, J), I >= J, per workgroup of 256 threads:
 //  * exact symmetry test a_ij == a_ji (both parts), nonzero real parts,
 //    non-finite entries -> atomic OR into flags;
 //  * per-row-block partial column sums of |a_ij| over off-diagonal entries,
 //    colpart[rowblock * n + col], each entry written exactly once, so the
 //    later reduction over row blocks is deterministic;
-//  * diagonal entries (re, im) into diag[].
+//  * diagonal entries (re, im) into diag[];
+//  * with colsig: per-row-block signed column sums of Im(a_ij), diagonal
+//    included, in double-double (colsig[rowblock * n + col] = (hi, lo)).
+//    For a symmetric matrix they give its exact row sums.
 template <typename Tin>
 __device__ __forceinline__ void analyze_tile_pair(const Tin* __restrict__ a, int n, int nt,
                                                   double* __restrict__ colpart, double2* __restrict__ diag,
-                                                  int* flags, int b) {
+                                                  int* flags, int b, double2* __restrict__ colsig = nullptr) {
   __shared__ double2 t0[32][33];
   __shared__ double2 t1[32][33];
   __shared__ int sflag;
@@ -116,6 +134,19 @@ __device__ __forceinline__ void analyze_tile_pair(const Tin* __restrict__ a, int
     }
     if (gj < n) colpart[(int64_t)J * n + gj] = s;
   }
+  if (colsig) {
+    if (threadIdx.x >= 64 && threadIdx.x < 96) {
+      const int c = threadIdx.x - 64, gj = J * 32 + c;
+      double hi = 0.0, lo = 0.0;
+      for (int r = 0; r < 32; ++r) dd_add(hi, lo, t0[r][c].y);
+      if (gj < n) colsig[(int64_t)I * n + gj] = dd_norm(hi, lo);
+    } else if (threadIdx.x >= 96 && threadIdx.x < 128 && I != J) {
+      const int c = threadIdx.x - 96, gj = I * 32 + c;
+      double hi = 0.0, lo = 0.0;
+      for (int r = 0; r < 32; ++r) dd_add(hi, lo, t1[r][c].y);
+      if (gj < n) colsig[(int64_t)J * n + gj] = dd_norm(hi, lo);
+    }
+  }
   __syncthreads();
   if (threadIdx.x == 0 && sflag) atomicOr(flags, sflag);
 }
@@ -123,17 +154,29 @@ __device__ __forceinline__ void analyze_tile_pair(const Tin* __restrict__ a, int
 template <typename Tin>
 __global__ __launch_bounds__(256) void analyze_tiles(const Tin* __restrict__ a, int n, int nt,
                                                      double* __restrict__ colpart,
-                                                     double2* __restrict__ diag, int* flags) {
-  analyze_tile_pair<Tin>(a, n, nt, colpart, diag, flags, (int)blockIdx.x);
+                                                     double2* __restrict__ diag, int* flags,
+                                                     double2* __restrict__ colsig) {
+  analyze_tile_pair<Tin>(a, n, nt, colpart, diag, flags, (int)blockIdx.x, colsig);
 }
 
-// off[j] = sum over row blocks of colpart (fixed order -> deterministic).
-__global__ void reduce_colpart(const double* __restrict__ colpart, int n, int nt, double* __restrict__ off) {
+// off[j] = sum over row blocks of colpart (fixed order -> deterministic);
+// with colsig also rsum[j] = the double-double sum of colsig over row blocks.
+__global__ void reduce_colpart(const double* __restrict__ colpart, int n, int nt, double* __restrict__ off,
+                               const double2* __restrict__ colsig = nullptr, double2* __restrict__ rsum = nullptr) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   double s = 0.0;
   for (int b = 0; b < nt; ++b) s += colpart[(int64_t)b * n + j];
   off[j] = s;
+  if (colsig) {
+    double hi = 0.0, lo = 0.0;
+    for (int b = 0; b < nt; ++b) {
+      const double2 v = colsig[(int64_t)b * n + j];
+      dd_add(hi, lo, v.x);
+      dd_add(hi, lo, v.y);
+    }
+    rsum[j] = dd_norm(hi, lo);
+  }
 }
 
 // Scalars produced on the device and read back by the host once per run.
@@ -144,15 +187,32 @@ struct Scalars {
   double norm_shift;     // ||A - mu I||_1
   int flags;
   int info;              // LU: nonzero if an exactly zero pivot was met
+  int rowsum_equal;      // 1: every row sum of Im(A) equals rowsum_c (double-double sums)
+  int pad_;
+  double rowsum_c;       // that common row sum
 };
 
 // Single workgroup of 1024 threads: trace, mu and both 1-norms.  The trace
-// is summed in a fixed tree order.
+// is summed in a fixed tree order.  With rsum (the double-double row sums
+// of Im(A)): whether all of them are equal, and their common value.
 __device__ __forceinline__ void finalize_norms_block(const double* __restrict__ off,
                                                      const double2* __restrict__ diag, int n,
-                                                     const int* flags, Scalars* out) {
+                                                     const int* flags, Scalars* out,
+                                                     const double2* __restrict__ rsum = nullptr) {
   __shared__ double sr[1024], si[1024], s0[1024], s1[1024];
+  __shared__ int s_uneq;
   const int t = threadIdx.x;
+  if (t == 0) s_uneq = 0;
+  __syncthreads();
+  if (rsum && n > 0) {
+    const double2 r0 = rsum[0];
+    int uneq = 0;
+    for (int j = t; j < n; j += 1024) {
+      const double2 r = rsum[j];
+      if (r.x != r0.x || r.y != r0.y) uneq = 1;
+    }
+    if (uneq) atomicOr(&s_uneq, 1);
+  }
   double ar = 0.0, ai = 0.0;
   for (int j = t; j < n; j += 1024) {
     ar += diag[j].x;
@@ -202,21 +262,28 @@ __device__ __forceinline__ void finalize_norms_block(const double* __restrict__ 
     out->norm_shift = s1[0];
     out->flags = *flags;
     out->info = 0;
+    out->rowsum_equal = (rsum && n > 0 && !s_uneq) ? 1 : 0;
+    out->pad_ = 0;
+    out->rowsum_c = (rsum && n > 0) ? rsum[0].x + rsum[0].y : 0.0;
   }
 }
 
 __global__ __launch_bounds__(1024) void finalize_norms(const double* __restrict__ off,
                                                        const double2* __restrict__ diag, int n,
-                                                       const int* flags, Scalars* out) {
-  finalize_norms_block(off, diag, n, flags, out);
+                                                       const int* flags, Scalars* out,
+                                                       const double2* __restrict__ rsum) {
+  finalize_norms_block(off, diag, n, flags, out, rsum);
 }
 
 // ---------------------------------------------------------------------------
 // Input -> planar working matrix: w = (A - mu I) * scale.
-// structured: A = i*G with G real symmetric; only wr = (G - Im(mu) I)*scale.
+// structured: A = i*G with G real symmetric; only
+// wr = (G - Im(mu) I + alpha J) * scale, J the all-ones matrix (alpha != 0
+// deflates an exact eigenvector 1 of G, see Pipeline::spectral_front).
 template <typename Tin, typename W>
 __global__ void prepare_planar(const Tin* __restrict__ a, int n, W* __restrict__ wr, W* __restrict__ wi,
-                               int64_t ld, double mu_re, double mu_im, double scale, int structured) {
+                               int64_t ld, double mu_re, double mu_im, double scale, int structured,
+                               double alpha = 0.0) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y;
   if (j >= n) return;
@@ -227,7 +294,7 @@ __global__ void prepare_planar(const Tin* __restrict__ a, int n, W* __restrict__
     im -= mu_im;
   }
   if (structured) {
-    wr[(int64_t)i * ld + j] = (W)(im * scale);
+    wr[(int64_t)i * ld + j] = (W)((im + alpha) * scale);
   } else {
     wr[(int64_t)i * ld + j] = (W)(re * scale);
     wi[(int64_t)i * ld + j] = (W)(im * scale);
@@ -421,6 +488,53 @@ __global__ void merge_diagonal(W* __restrict__ wr, W* __restrict__ wi, int64_t l
   if (i >= n) return;
   wr[(int64_t)i * ldw + i] = (W)d[i].x;
   wi[(int64_t)i * ldw + i] = (W)d[i].y;
+}
+
+// x_ij += c for every i, j < n (both planes): the rank-one term c * J of a
+// deflated exponential.
+template <typename W>
+__global__ void add_constant(W* __restrict__ wr, W* __restrict__ wi, int64_t ldw, int n, double cr, double ci) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (j >= n) return;
+  const int64_t o = (int64_t)i * ldw + j;
+  wr[o] = (W)((double)wr[o] + cr);
+  wi[o] = (W)((double)wi[o] + ci);
+}
+
+// 1-norm of a real n x n FP64 plane in two deterministic steps: partial
+// column sums of |x_ij| over row chunks of `rows` rows (part[chunk * n + j]),
+// then one workgroup sums the chunks per column and takes the maximum.
+__global__ __launch_bounds__(256) void norm1_part(const double* __restrict__ x, int64_t ld, int n, int rows,
+                                                  double* __restrict__ part) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  const int c = blockIdx.y;
+  if (j >= n) return;
+  const int r1 = min(n, (c + 1) * rows);
+  double s = 0.0;
+  for (int i = c * rows; i < r1; ++i) s += fabs(x[(int64_t)i * ld + j]);
+  part[(int64_t)c * n + j] = s;
+}
+
+__global__ __launch_bounds__(1024) void norm1_final(const double* __restrict__ part, int nchunks, int n,
+                                                    double* __restrict__ out) {
+  __shared__ double sm[1024];
+  double m = 0.0;
+  for (int j = threadIdx.x; j < n; j += 1024) {
+    double s = 0.0;
+    for (int c = 0; c < nchunks; ++c) s += part[(int64_t)c * n + j];
+    m = (s != s) ? s : fmax(m, s);
+  }
+  sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      const double y = sm[threadIdx.x + w];
+      sm[threadIdx.x] = (y != y) ? y : fmax(sm[threadIdx.x], y);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sm[0];
 }
 
 // Output: out[i][j] = e^mu * (xr + i xi), interleaved complex or real.
@@ -729,17 +843,22 @@ __device__ __forceinline__ int ld_agent(const int* p) {
 
 // Grid barrier over G co-resident workgroups: one arrival counter that is
 // never reset within a launch (barrier b waits for G * (b + 1) arrivals).
-// The wait is bounded: after about 2^24 polls it records an error in
-// sc->info (bit 1) and returns, so a launch can never hang.
+// The wait is bounded in time: after 2^28 ticks of the constant-rate device
+// clock (about 2.7 s at its 100 MHz) it records an error in sc->info (bit 1)
+// and returns.  The error is sticky: once bit 1 is set every later barrier
+// returns at once and later panel launches exit at their start, so a lost
+// workgroup ends the factorisation with an error instead of a long spin.
+constexpr uint64_t kBarrierTicks = (uint64_t)1 << 28;
 __device__ __forceinline__ void grid_barrier(unsigned* ctr, unsigned target, Scalars* sc) {
   __syncthreads();
   if (threadIdx.x == 0) {
     __threadfence();
     __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    unsigned polls = 0;
+    const uint64_t t0 = wall_clock64();
     while (__hip_atomic_load(ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (ld_agent(&sc->info) & 2) break;  // an earlier barrier timed out
       __builtin_amdgcn_s_sleep(1);
-      if (++polls > (1u << 24)) {
+      if (wall_clock64() - t0 > kBarrierTicks) {
         atomicOr(&sc->info, 2);
         break;
       }
@@ -758,7 +877,12 @@ __global__ __launch_bounds__(256) void lu_panel_coop(double* __restrict__ sr, do
   __shared__ double sv[256];
   __shared__ int sk[256];
   __shared__ int sw[256];
+  __shared__ int s_abort;
   const int t = threadIdx.x;
+  // an earlier panel's barrier timed out: the factorisation has failed
+  if (t == 0) s_abort = ld_agent(&sc->info) & 2;
+  __syncthreads();
+  if (s_abort) return;
   const int w = blockIdx.x;
   const int r0 = k0 + w * PANEL_R;
   const int hi = min(r0 + PANEL_R, np);

@@ -69,6 +69,10 @@ static py::dict stats_dict(const fexpm_stats& s) {
   d["precision"] = s.precision == FEXPM_PREC_FP16X2 ? "fp16x2" : (s.precision == FEXPM_PREC_BF16X3 ? "bf16x3" : "native");
   d["fp64_slices"] = s.fp64_slices;
   d["fp64_slices_scope"] = s.fp64_slices_scope;
+  d["spectral"] = s.spectral;
+  d["deflated"] = s.deflated;
+  d["s_onenorm"] = s.s_onenorm;
+  d["rho_bound"] = s.rho_bound;
   d["ms_propagate"] = s.ms_propagate;
   return d;
 }
@@ -277,6 +281,7 @@ class ExpMatBatch {
                                fexpm_batch_last_error(b_));
   }
   void set_option(int key, int64_t value) { check(fexpm_batch_set_option(b_, key, value)); }
+  int64_t count() const { return fexpm_batch_count(b_); }
 
   // mats: list of square complex arrays (cast to the batch dtype), packed
   // into one host buffer (allocated once, filled by several threads)
@@ -296,7 +301,6 @@ class ExpMatBatch {
       ns.push_back(a.shape(0));
       cs.push_back(c);
     }
-    n_ = ns;
     std::unique_ptr<char[]> host(new char[std::max<size_t>(total, 1)]);
     std::vector<const void*> src;
     std::vector<size_t> len;
@@ -324,8 +328,12 @@ class ExpMatBatch {
       for (int t = t0 + 1; t < nth; ++t) work(t);  // threads that could not be started
       work(0);
       for (auto& t : th) t.join();
-      rc = fexpm_batch_upload(b_, (int64_t)n_.size(), n_.data(), host.get());
+      rc = fexpm_batch_upload(b_, (int64_t)ns.size(), ns.data(), host.get());
     }
+    // the sizes change only with the engine's: a rejected upload leaves the
+    // previous batch (or, after a failed allocation, an empty one)
+    if (rc == FEXPM_OK) n_ = ns;
+    else if (fexpm_batch_count(b_) == 0) n_.clear();
     check(rc);
   }
   // Runs the uploaded batch; with psi0 (list of vectors) returns the list of psi.
@@ -459,6 +467,9 @@ PYBIND11_MODULE(_fastexpm_core, m) {
   m.attr("OPT_AUTOTUNE") = (int)FEXPM_OPT_AUTOTUNE;
   m.attr("OPT_GRAPH") = (int)FEXPM_OPT_GRAPH;
   m.attr("OPT_FP64_SLICES") = (int)FEXPM_OPT_FP64_SLICES;
+  m.attr("OPT_SPECTRAL") = (int)FEXPM_OPT_SPECTRAL;
+  m.attr("OPT_FAIL_ALLOC") = (int)FEXPM_OPT_FAIL_ALLOC;
+  m.attr("OPT_DEFLATE") = (int)FEXPM_OPT_DEFLATE;
   m.attr("PREC_NATIVE") = (int)FEXPM_PREC_NATIVE;
   m.attr("PREC_FP16X2") = (int)FEXPM_PREC_FP16X2;
   m.attr("PREC_BF16X3") = (int)FEXPM_PREC_BF16X3;
@@ -470,6 +481,7 @@ PYBIND11_MODULE(_fastexpm_core, m) {
       .def("download", &ExpMatBatch::download, py::arg("i"))
       .def("stats", &ExpMatBatch::stats, py::arg("i"))
       .def("last_ms", &ExpMatBatch::last_ms)
+      .def("count", &ExpMatBatch::count)
       .def("report", &ExpMatBatch::report);
   m.def("gpu_memory_gb", &fexpm_gpu_memory_gb);
 }

@@ -151,6 +151,10 @@ class ExpMatBatch:
     def download(self, i: int) -> np.ndarray:
         return self._backend.download(int(i))
 
+    def count(self) -> int:
+        """Number of matrices of the uploaded batch."""
+        return int(self._backend.count())
+
     def run(self, mats, psi0=None):
         self.upload(mats)
         psi = self.run_resident(psi0)

@@ -74,10 +74,18 @@ enum fexpm_option {
                                     run of a new layout and keep the fastest (default), 0 = default tiles */
   FEXPM_OPT_GRAPH = 10,          /* grouped batches: 1 = capture the program in a hipGraph and replay it
                                     while the plans stay the same (default), 0 = eager launches */
-  FEXPM_OPT_FP64_SLICES = 11     /* complex64 / float32 engines, structured input: the symmetric FP64
+  FEXPM_OPT_FP64_SLICES = 11,    /* complex64 / float32 engines, structured input: the symmetric FP64
                                     Pade products from S exact int8 slices per value (S = 4..6, about
                                     7S - 1 bits of each row's largest value) on the int8 MFMA with int32
                                     accumulation; default 6 (the solve stays on FP64 MFMA); 0 = FP64 MFMA */
+  FEXPM_OPT_SPECTRAL = 12,       /* structured input (A = iG, G real symmetric, hence normal), degree 13:
+                                    1 = s from the spectral-radius bound min_k ||B^k||_1^(1/k), k = 1, 2,
+                                    4, 6, of the shifted B (default; single runs), 0 = s from ||B||_1
+                                    (Higham 2005).  Batches always use the 1-norm rule. */
+  FEXPM_OPT_DEFLATE = 13         /* with FEXPM_OPT_SPECTRAL: 1 = when every row sum of G is the same c
+                                    (double-double sums), deflate the exact eigenvector 1 first:
+                                    B = G - mu I + (mu - c)/n J and exp(A) = e^{i mu} (exp(iB) +
+                                    (e^{i(c - mu)} - 1) J / n) (default); 0 = no deflation */
 };
 
 /* Per-run report (filled by fexpm_get_stats after a run). */
@@ -106,6 +114,11 @@ typedef struct fexpm_stats {
   int32_t precision;    /* enum fexpm_precision used by the squarings of this run */
   int32_t fp64_slices;  /* int8 slices per value of the FP64 products (0 = FP64 MFMA) */
   int32_t fp64_slices_scope;  /* stages using them: 1 Pade products, 2 solve, 3 both */
+  int32_t spectral;     /* 1: s came from the spectral-radius bound (FEXPM_OPT_SPECTRAL) */
+  int32_t deflated;     /* 1: the eigenvector 1 was deflated (FEXPM_OPT_DEFLATE) */
+  int32_t s_onenorm;    /* s of the 1-norm rule (equal to s when spectral = 0) */
+  int32_t reserved0;
+  double rho_bound;     /* spectral-radius bound of the shifted (deflated) B, when spectral = 1 */
 } fexpm_stats;
 
 /* Create an engine for n x n matrices of the given dtype and allocate its
@@ -214,8 +227,12 @@ typedef struct fexpm_batch_report {
 int fexpm_batch_get_report(const fexpm_batch* b, fexpm_batch_report* out);
 
 /* `count` row-major matrices of orders n[i] (engine dtype), packed back to
- * back in host memory, copied to the device. */
+ * back in host memory, copied to the device.  An order outside [1, nmax]
+ * is rejected (FEXPM_EINVAL) before anything changes; a failure after that
+ * (device allocation, copy) leaves the batch empty. */
 int fexpm_batch_upload(fexpm_batch* b, int64_t count, const int64_t* n, const void* a);
+/* Number of matrices of the current batch (-1 for a null handle). */
+int64_t fexpm_batch_count(const fexpm_batch* b);
 /* U_i = expm(A_i) for every uploaded matrix (U stays on the device); with
  * psi0 != NULL also psi_i = U_i psi0_i (packed complex128, length n_i each). */
 int fexpm_batch_run_resident(fexpm_batch* b, const void* psi0, void* psi);

@@ -66,6 +66,11 @@ enum { FEXPM_OPT_Z_LEFT = 107 };
  * also taken when pinned host memory cannot be allocated). */
 enum { FEXPM_OPT_HOST_STAGING = 108 };
 
+/* Batch option: 1 makes the next fexpm_batch_upload that must grow the
+ * matrix buffers fail with FEXPM_ENOMEM after the old buffers are released
+ * (host-side injection; no device call fails), to test the recovery path. */
+enum { FEXPM_OPT_FAIL_ALLOC = 109 };
+
 /* Internal planes readable by fexpm_debug_plane (working precision, n x n). */
 enum fexpm_plane {
   FEXPM_PLANE_A = 0,   /* (A - mu I) / 2^s; structured: its imaginary part G */

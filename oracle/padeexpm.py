@@ -15,6 +15,17 @@ d_u1, d_u2, d_v1, d_v2, d_mu, d_nrmA`` (include/MatrixExpCalculator.h:52-61)
   exp(A) = e^mu exp(A - mu I) with mu = tr(A)/n, applied when it does not
   increase the 1-norm.
 
+* Spectral scaling of normal inputs (``spectral=True``): for A = iG with G
+  real symmetric the backward error of the Pade approximant at A/2^s is
+  that of its eigenvalues, so Higham's condition may be read in the 2-norm,
+  rho(A - mu I)/2^s <= theta_13, with any upper bound of the spectral radius.
+  The bounds are rho(B) <= ||B^k||_1^(1/k) (Gelfand; k = 1, 2, 4, 6, the
+  powers the Pade-13 evaluation needs anyway; cf. the d_k = ||A^k||^(1/k)
+  of Al-Mohy & Higham, SIMAX 31(3), 2009).  When every row sum of G equals
+  the same c, the eigenvector 1 is deflated first: B = G - mu I + alpha J
+  with alpha = (mu - c)/n has B 1 = 0 and commutes with J = 1 1^T, and
+  exp(A) = e^{i mu} (exp(iB) + (e^{i(c - mu)} - 1) J / n).
+
 Stages are returned individually so that GPU tests can compare the engine's
 mu, norm, m and s exactly and its U, V, X within rounding.
 
@@ -84,6 +95,50 @@ def choose_degree(norm: float):
     return 13, s
 
 
+def scaling_for(rho: float) -> int:
+    """Squarings of degree 13 for a bound ``rho`` of the 2-norm (the rule of
+    choose_degree with rho in place of the 1-norm)."""
+    if not rho > THETA[13]:
+        return 0
+    s = max(0, int(math.ceil(math.log2(rho / THETA[13]))))
+    while rho / (2.0 ** s) > THETA[13]:
+        s += 1
+    return s
+
+
+def spectral_bound(b: np.ndarray) -> float:
+    """min over k in (1, 2, 4, 6) of ||b^k||_1^(1/k), each an upper bound of
+    the spectral radius of b, widened by 1e-9 for the rounding of the norms."""
+    b2 = b @ b
+    b4 = b2 @ b2
+    b6 = b4 @ b2
+    vals = [onenorm(b), onenorm(b2) ** 0.5, onenorm(b4) ** 0.25, onenorm(b6) ** (1.0 / 6.0)]
+    return min(vals) * (1.0 + 1e-9)
+
+
+def structured_part(a: np.ndarray):
+    """G if a = iG with G real and exactly symmetric (the CTQW form), else None."""
+    a = np.asarray(a)
+    if not np.iscomplexobj(a) or np.any(a.real != 0):
+        return None
+    g = np.ascontiguousarray(a.imag, dtype=np.float64)
+    return g if np.array_equal(g, g.T) else None
+
+
+def equal_row_sums(g: np.ndarray):
+    """(True, c) if every row of g has the same exact sum (c = that sum
+    rounded).  Each sum is kept as the pair (fl(sum), fl(sum - fl(sum))) of
+    correctly rounded math.fsum results, about 106 bits, like the device's
+    double-double row sums."""
+    pairs = []
+    for row in np.asarray(g, dtype=np.float64):
+        hi = math.fsum(row)
+        pairs.append((hi, math.fsum(list(row) + [-hi])))
+    if not pairs:
+        return True, 0.0
+    return all(p == pairs[0] for p in pairs), pairs[0][0] + pairs[0][1]
+
+
 def pade_uv(a: np.ndarray, m: int):
     """U (odd part) and V (even part) of the [m/m] Pade approximant at a."""
     n = a.shape[0]
@@ -106,11 +161,17 @@ def pade_uv(a: np.ndarray, m: int):
     return a @ odd, even
 
 
-def pade_expm(a: np.ndarray, stages: bool = False):
+def pade_expm(a: np.ndarray, stages: bool = False, spectral: bool = False, deflate: bool = True):
     """exp(a) in FP64 by trace shift + Higham 2005 scaling and squaring.
 
+    With ``spectral=True`` (the engine's default for single runs), inputs of
+    the form a = iG (G real symmetric) that need degree 13 with s > 0 take
+    s from the spectral-radius bound, after deflating the eigenvector 1
+    when all row sums of G are equal (``deflate``).
+
     With ``stages=True`` returns a dict with mu, norm, m, s, U, V, X (the
-    Pade value before squaring) and E (the result).
+    Pade value before squaring) and E (the result); spectral runs add
+    s_onenorm, rho, deflated, c, alpha and B (the unscaled shifted matrix).
     """
     a = np.asarray(a)
     if a.ndim != 2 or a.shape[0] != a.shape[1]:
@@ -122,16 +183,31 @@ def pade_expm(a: np.ndarray, stages: bool = False):
     mu, ash = trace_shift(a)
     norm = onenorm(ash)
     m, s = choose_degree(norm)
-    scaled = ash / (2.0 ** s)
+    info = {"spectral": False, "deflated": False, "s_onenorm": s}
+    g = structured_part(a) if (spectral and m == 13 and 0 < s < 120) else None
+    if g is not None:
+        equal, c = equal_row_sums(g) if deflate else (False, 0.0)
+        alpha = (mu.imag - c) / n if equal else 0.0
+        b = (g - mu.imag * np.eye(n)) + alpha
+        rho = spectral_bound(b)
+        s = scaling_for(rho)
+        scaled = 1j * (b / (2.0 ** s))
+        info.update({"spectral": True, "deflated": bool(equal), "rho": rho, "c": c if equal else 0.0,
+                     "alpha": alpha, "B": b})
+    else:
+        scaled = ash / (2.0 ** s)
     u, v = pade_uv(scaled, m)
     x = np.linalg.solve(v - u, v + u)
     e = x
     for _ in range(s):
         e = e @ e
+    if info["deflated"]:
+        e = e + (np.exp(1j * (info["c"] - mu.imag)) - 1.0) / n
     e = np.exp(mu) * e
     if stages:
-        return {"mu": mu, "norm": norm, "m": m, "s": s, "U": u, "V": v,
-                "X": x, "E": e}
+        out = {"mu": mu, "norm": norm, "m": m, "s": s, "U": u, "V": v, "X": x, "E": e}
+        out.update(info)
+        return out
     return e
 
 

@@ -9,8 +9,11 @@ against the oracle) are stated per precision:
   5e-6 * 2^(s/2), the random-walk growth of FP32 rounding over s squarings
   of a unitary-scaled matrix, floored at 2e-6.
 
-The degree m and scaling s chosen on the device must equal the oracle's.
+The degree m and scaling s chosen on the device must equal the oracle's
+(single runs: the spectral scaling rule of ``pade_expm(spectral=True)``;
+batches: the 1-norm rule).
 """
+import math
 import sys
 
 import numpy as np
@@ -20,7 +23,7 @@ from fastctqw_amd.graphs import erdos_renyi, laplacian, random_regular
 from oracle.closed import complete_graph, cycle_graph, expm_complete, expm_cycle
 from oracle.eig import eig_propagator, taylor_expm
 from oracle.metrics import infidelity, rel_fro, unitarity_defect
-from oracle.padeexpm import pade_expm
+from oracle.padeexpm import THETA, pade_expm
 
 pytestmark = pytest.mark.gpu
 
@@ -52,17 +55,29 @@ def _tol(dtype, s):
     return max(2e-6, 5e-6 * 2.0 ** (s / 2.0))
 
 
+def _same_s(stats, st):
+    """The device's s equals the oracle's; with the spectral rule a bound
+    within rounding (1e-9) of theta_13 times a power of two may round either way."""
+    if stats["s"] == st["s"]:
+        return True
+    if st.get("spectral"):
+        x = math.log2(st["rho"] / THETA[13])
+        return abs(x - round(x)) < 1e-9 and abs(stats["s"] - st["s"]) == 1
+    return False
+
+
 def _check(fx, a, dtype, name, ref=None):
     n = a.shape[0]
     _say(f"{name} n={n} dtype={np.dtype(dtype).name}")
     a = np.ascontiguousarray(a.astype(dtype))
-    st = pade_expm(a.astype(np.complex128), stages=True)
+    st = pade_expm(a.astype(np.complex128), stages=True, spectral=True)
     with _engine(fx, dtype, n) as e:
         u = e.run(a)
         stats = e.stats()
     assert u.shape == (n, n) and u.dtype == np.dtype(dtype)
     assert stats["m"] == st["m"], (stats, st["m"], st["s"])
-    assert stats["s"] == st["s"], (stats, st["m"], st["s"])
+    assert _same_s(stats, st), (stats, st["m"], st["s"])
+    assert stats["spectral"] == int(st["spectral"]) and stats["deflated"] == int(st["deflated"]), stats
     target = st["E"] if ref is None else ref
     err = rel_fro(target, u)
     tol = _tol(dtype, st["s"])
@@ -83,6 +98,64 @@ def test_ctqw_laplacian_er(fx, dt, n):
     ref = eig_propagator(L, t)
     u, stats = _check(fx, -1j * t * L, dt, "ER-laplacian", ref=ref)
     assert stats["structured"]
+
+
+def _spectral_case(case):
+    n = 400
+    if case == "er_unweighted":  # Laplacian, equal row sums (0): deflated
+        return 1j * 4.5 * -laplacian(erdos_renyi(n, 0.5, seed=2)), True
+    if case == "er_weighted":  # weighted Laplacian: row sums not exactly equal
+        return -1j * 20.0 * laplacian(erdos_renyi(n, 0.5, seed=3, weighted=True)), False
+    # adjacency of a 6-regular graph, reference sign A_exp = +i t A: row sums 6 t
+    return 1j * 4.5 * random_regular(n, 6, seed=5).astype(np.float64), True
+
+
+@pytest.mark.parametrize("dt", [np.complex64, np.complex128])
+@pytest.mark.parametrize("case", ["er_unweighted", "er_weighted", "regular"])
+def test_spectral_scaling(fx, dt, case):
+    """s from the spectral-radius bound (option OPT_SPECTRAL, default) with
+    the eigenvector 1 deflated when every row sum is equal: the device's m,
+    s, deflation and bound against the oracle's same rule; the Pade value
+    (no squarings: output e^mu (X + delta J)) against the FP64 stage
+    restatement; U against the eigen oracle; the 1-norm rule (option off)
+    against the same oracle with its own, larger s."""
+    core = fx._fastexpm_core
+    a, defl = _spectral_case(case)
+    n = a.shape[0]
+    a = np.ascontiguousarray(a.astype(dt))
+    a128 = a.astype(np.complex128)
+    st = pade_expm(a128, stages=True, spectral=True)
+    assert st["spectral"] and st["deflated"] == defl
+    ref = eig_propagator(-a128.imag, 1.0)  # expm(i G) for A = i G
+    _say(f"spectral scaling {case} n={n} {np.dtype(dt).name}")
+    tol = _tol(dt, st["s"])
+    with _engine(fx, dt, n) as e:
+        u = e.run(a)
+        stats = e.stats()
+        assert stats["spectral"] == 1 and stats["deflated"] == int(defl), stats
+        assert stats["m"] == 13 and _same_s(stats, st), (stats, st["s"], st["rho"])
+        assert stats["s_onenorm"] == st["s_onenorm"] and stats["s"] <= stats["s_onenorm"]
+        assert abs(stats["rho_bound"] - st["rho"]) <= 1e-9 * st["rho"], (stats["rho_bound"], st["rho"])
+        err = rel_fro(ref, u)
+        print(f"  s={stats['s']} (1-norm rule {stats['s_onenorm']}) rho={stats['rho_bound']:.2f} "
+              f"relF={err:.3e} tol={tol:.1e}", flush=True)
+        assert err <= tol
+        # the Pade value before the squarings
+        e._backend.set_option(core.OPT_MAX_SQUARINGS, 0)
+        x0 = e.run(a)
+        xr = st["X"] + ((np.exp(1j * (st["c"] - st["mu"].imag)) - 1.0) / n if defl else 0.0)
+        errx = rel_fro(np.exp(st["mu"]) * xr, x0)
+        print(f"  Pade value relF={errx:.3e}", flush=True)
+        assert errx <= (1e-6 if dt == np.complex64 else 1e-12)
+        e._backend.set_option(core.OPT_MAX_SQUARINGS, -1)
+        # the 1-norm rule: more squarings, same exponential
+        e._backend.set_option(core.OPT_SPECTRAL, 0)
+        u1 = e.run(a)
+        s1 = e.stats()
+        assert s1["spectral"] == 0 and s1["s"] == st["s_onenorm"]
+        err1 = rel_fro(ref, u1)
+        print(f"  1-norm rule s={s1['s']} relF={err1:.3e}", flush=True)
+        assert err1 <= _tol(dt, s1["s"])
 
 
 @pytest.mark.parametrize("dt", [np.complex64, np.complex128])
@@ -560,6 +633,41 @@ def test_batch_large_upload(fx):
         ref = _eig_ref(mats[i])
         assert rel_fro(ref, u) <= _tol(np.complex64, b.stats(i)["s"]), i
         assert infidelity(ref @ psi0[i], psi) < 1e-9
+
+
+def test_batch_upload_recovery(fx):
+    """A rejected upload (an order above nmax) leaves the previous batch as
+    it was; a failed allocation (injected on the host after the old matrix
+    buffers are released) leaves an empty batch, and the next upload
+    allocates again and runs."""
+    core = fx._fastexpm_core
+    mats = [(-1j * 2.0 * laplacian(erdos_renyi(n, 0.3, seed=90 + n, weighted=True))).astype(np.complex64)
+            for n in (40, 70)]
+    _say("batch upload recovery")
+    b = fx.ExpMatBatch(nmax=128, dtype=np.complex64, lanes=2)
+    us = b.run(mats)
+    with pytest.raises(RuntimeError):
+        b.upload([np.zeros((129, 129), np.complex64)])
+    assert b.count() == 2
+    u0 = b.download(0)
+    assert u0.shape == (40, 40) and np.array_equal(u0, us[0])
+    b.run_resident()
+    assert rel_fro(us[1], b.download(1)) <= 1e-6
+    big = [(-1j * laplacian(erdos_renyi(128, 0.3, seed=99 + q, weighted=True))).astype(np.complex64)
+           for q in range(3)]
+    b._backend.set_option(core.OPT_FAIL_ALLOC, 1)
+    with pytest.raises((RuntimeError, MemoryError)):
+        b.upload(big)
+    assert b.count() == 0
+    with pytest.raises(ValueError):
+        b.download(0)
+    us2 = b.run(mats)
+    for u, v, a in zip(us, us2, mats):
+        assert u.shape == v.shape and rel_fro(u, v) <= 1e-6
+        assert rel_fro(_eig_ref(a), v) <= _tol(np.complex64, 8)
+    us3 = b.run(big)
+    for u, a in zip(us3, big):
+        assert rel_fro(_eig_ref(a), u) <= _tol(np.complex64, 8)
 
 
 def _eig_ref(a):

@@ -195,3 +195,89 @@ def test_golden_vectors():
 def test_work_model():
     assert P.work_flops(10000, 13) == pytest.approx((8 * 19 + 32 / 3) * 1e12)
     assert math.isclose(P.work_flops(1000, 2, 13), (8 * 8 + 32 / 3) * 1e9)
+
+
+# ---------------------------------------------------------------------------
+# Spectral scaling and deflation (pade_expm(spectral=True))
+
+def _spectral_inputs():
+    yield "er_laplacian", -1j * 4.5 * laplacian(erdos_renyi(120, 0.5, seed=2)), True
+    yield "er_weighted", -1j * 20.0 * laplacian(erdos_renyi(120, 0.5, seed=3, weighted=True)), False
+    yield "regular_adjacency", 1j * 4.5 * random_regular(120, 6, seed=5).astype(np.float64), True
+    rng = np.random.default_rng(9)
+    g = rng.standard_normal((90, 90))
+    yield "gaussian_symmetric", 1j * 3.0 * (g + g.T), False
+
+
+@pytest.mark.parametrize("name,a,defl", list(_spectral_inputs()))
+def test_spectral_bound_is_an_upper_bound(name, a, defl):
+    """rho(B) <= ||B^k||_1^(1/k) for every k (Gelfand): the bound is at
+    least the largest |eigenvalue| (LAPACK eigvalsh) and at most ||B||_1."""
+    st = P.pade_expm(a, stages=True, spectral=True)
+    assert st["spectral"] and st["deflated"] == defl, name
+    b = st["B"]
+    rho = np.abs(np.linalg.eigvalsh(b)).max()
+    assert rho <= st["rho"] <= P.onenorm(b) * (1 + 1e-9) + 1e-300, (name, rho, st["rho"])
+    assert st["s"] == P.scaling_for(st["rho"]) and st["s"] <= st["s_onenorm"]
+    if defl:
+        # the deflated matrix annihilates 1 exactly in exact arithmetic
+        assert np.abs(b.sum(axis=1)).max() <= 1e-9 * P.onenorm(b)
+
+
+@pytest.mark.parametrize("name,a,defl", list(_spectral_inputs()))
+def test_spectral_oracle_vs_eig_and_scipy(name, a, defl):
+    """The spectrally scaled (and deflated) Pade exponential against the
+    eigendecomposition oracle and scipy.linalg.expm (a different scaling
+    rule); the dense graph cases need fewer squarings than the 1-norm rule."""
+    st = P.pade_expm(a, stages=True, spectral=True)
+    ref = eig_propagator(-np.asarray(a).imag, 1.0)
+    assert rel_fro(ref, st["E"]) < 1e-12, name
+    assert rel_fro(sl.expm(a), st["E"]) < 1e-11, name
+    if name.startswith("er_"):
+        assert st["s"] < st["s_onenorm"], (name, st["s"], st["s_onenorm"])
+
+
+def test_deflation_identity_against_scipy():
+    """exp(iG) = e^{i mu} (exp(iB) + (e^{i(c - mu)} - 1) J / n) with
+    B = G - mu I + (mu - c)/n J, evaluated with scipy.linalg.expm only."""
+    g = -4.5 * laplacian(erdos_renyi(60, 0.4, seed=8))
+    g = g + 2.0 * np.eye(60)  # equal row sums c = 2, nonzero trace
+    n = g.shape[0]
+    ok, c = P.equal_row_sums(g)
+    assert ok and c == 2.0
+    mu = np.trace(g) / n
+    b = g - mu * np.eye(n) + (mu - c) / n
+    lhs = sl.expm(1j * g)
+    rhs = np.exp(1j * mu) * (sl.expm(1j * b) + (np.exp(1j * (c - mu)) - 1.0) / n)
+    assert rel_fro(lhs, rhs) < 1e-12
+
+
+def test_planted_deflation_mistakes_are_caught():
+    """A wrong sign of the rank-one term or of alpha, or s one below the
+    spectral rule, each fails the eigen oracle by far more than rounding."""
+    a = -1j * 4.5 * laplacian(erdos_renyi(120, 0.5, seed=2))
+    st = P.pade_expm(a, stages=True, spectral=True)
+    ref = eig_propagator(-a.imag, 1.0)
+    n, mu, c, s = a.shape[0], st["mu"], st["c"], st["s"]
+
+    def run(b, s_, sign):
+        u, v = P.pade_uv(1j * b / 2.0 ** s_, 13)
+        x = np.linalg.solve(v - u, v + u)
+        for _ in range(s_):
+            x = x @ x
+        return np.exp(mu) * (x + sign * (np.exp(1j * (c - mu.imag)) - 1.0) / n)
+
+    assert rel_fro(ref, run(st["B"], s, 1.0)) < 1e-12
+    assert rel_fro(ref, run(st["B"], s, -1.0)) > 1e-3
+    b_wrong = st["B"] - 2 * st["alpha"]
+    assert rel_fro(ref, run(b_wrong, s, 1.0)) > 1e-3
+    assert rel_fro(ref, run(st["B"], s - 2, 1.0)) > 1e-10
+
+
+def test_equal_row_sums_is_exact():
+    """A row sum that differs in its last bit is not equal."""
+    g = np.ones((4, 4))
+    assert P.equal_row_sums(g) == (True, 4.0)
+    g[2, 3] = np.nextafter(1.0, 2.0)
+    g[3, 2] = g[2, 3]
+    assert not P.equal_row_sums(g)[0]
