@@ -67,46 +67,91 @@ def _sustained_peak(key):
     return None
 
 
+def squaring_roofline(n, sq_ms, issued, peak, split_parts=0):
+    """Roofline fields of one structured squaring X -> X^2 of order n
+    (complex symmetric X = Xr + i Xi).  The algorithm's own flops per
+    launch are those of three real symmetric products (3M rule, half of
+    each n x n x n product by symmetry): 3 n^3, times the part products of
+    a split-precision form (split_parts).  ``issued`` is what the kernel's
+    MFMAs execute (whole lower tiles, padded k), never less than that, so
+    frac <= issued_frac.  The 4M convention (8 n^3 per complex product) is
+    kept under its own key."""
+    alg = 3.0 * float(n) ** 3 * (split_parts if split_parts else 1)
+    t = sq_ms * 1e-3
+    return {
+        "achieved": alg / t / 1e12, "peak": peak, "unit": "TFLOP/s", "frac": alg / t / 1e12 / peak,
+        "algorithmic_flops_per_launch": alg, "issued_flops_per_launch": issued,
+        "issued_tflops": issued / t / 1e12, "issued_frac": issued / t / 1e12 / peak,
+        "achieved_4m_convention": 8.0 * float(n) ** 3 / t / 1e12,
+    }
+
+
 def work_flops(n, s, products=6):
     # 4M complex convention: 8 n^3 per complex product/squaring, LU 8/3 n^3, solve 8 n^3
     return (8.0 * (products + s) + 32.0 / 3.0) * float(n) ** 3
 
 
-def cpu_baseline(n_target, s_target, n_sample, t, seed, budget_s=20.0):
-    """Time scipy.linalg.expm on an ER(n_sample, 0.5) Laplacian (complex64)."""
+def _cpu_share():
+    """Host threads for the CPU baseline: the cores this process may run on,
+    at most 16 (the host CPU share of one GPU on the measurement boxes)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count() or 1
+    return max(1, min(16, avail))
+
+
+def cpu_baseline(n, t, p, seed, repeats=1, n_est=2000):
+    """The reference's CPU path (FastCTQW/Operator.py:76 scipy.linalg.expm,
+    then FastCTQW/Ctqw.py:94 U @ psi0) timed on the same N = 10^4 input as
+    the GPU run, complex64, psi0 = e_0, OpenBLAS on the host share of cores;
+    median of ``repeats`` runs.  The earlier work-model estimate from an
+    N = 2000 sample is kept beside it as ``estimate``."""
     import scipy.linalg as sl
     from threadpoolctl import threadpool_limits
 
     from fastctqw_amd.graphs import c2_laplacian
     from oracle.padeexpm import choose_degree, onenorm, trace_shift
 
-    cores = min(16, os.cpu_count() or 1)
-    L = c2_laplacian(n_sample, 0.5, seed, dtype=np.float64)
+    cores = _cpu_share()
+    L = c2_laplacian(n, p, seed, dtype=np.float64)
     a = (-1j * t * L).astype(np.complex64)
-    _, ash = trace_shift(a.astype(np.complex128))
-    m_s, s_s = choose_degree(onenorm(ash))
+    del L
+    psi0 = np.zeros(n, dtype=np.complex64)
+    psi0[0] = 1.0
     times = []
     with threadpool_limits(limits=cores):
-        sl.expm(a)  # warm-up
-        t_start = time.perf_counter()
-        while True:
+        for _ in range(max(1, repeats)):
             t0 = time.perf_counter()
             u = sl.expm(a)
-            _ = u @ np.ones(n_sample, dtype=np.complex64)  # U psi0 (FastCTQW/Ctqw.py:94)
+            _ = u @ psi0
             times.append(time.perf_counter() - t0)
-            if time.perf_counter() - t_start > budget_s or len(times) >= 5:
-                break
+            del u
+        del a
+        # work-model estimate from a small sample (the figure reported before)
+        Ls = c2_laplacian(n_est, p, seed, dtype=np.float64)
+        a_s = (-1j * t * Ls).astype(np.complex64)
+        _, ash = trace_shift(a_s.astype(np.complex128))
+        _, s_s = choose_degree(onenorm(ash))
+        sl.expm(a_s)
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            sl.expm(a_s)
+            ts.append(time.perf_counter() - t0)
     med = float(np.median(times))
-    scale = work_flops(n_target, s_target) / work_flops(n_sample, s_s)
     return {
-        "value": med * scale,
+        "value": med,
         "unit": "s per N=10k complex expm",
         "cores": cores,
         "kind": "port",
         "host": _host_info(cores),
-        "sample": (f"scipy.linalg.expm + U@psi0, complex64 ER({n_sample},0.5) Laplacian t={t} "
-                   f"(s={s_s}), median of {len(times)} = {med:.3f} s, scaled by "
-                   f"W({n_target},s={s_target})/W({n_sample},s={s_s}) = {scale:.1f}"),
+        "sample": (f"scipy.linalg.expm + U@psi0 on the same complex64 ER({n},{p}) seed {seed} Laplacian, "
+                   f"t={t}, psi0=e_0: {len(times)} run(s) at N={n}, median {med:.2f} s"),
+        "runs_s": times,
+        "estimate": {"n_sample": n_est, "s_sample": s_s, "median_s": float(np.median(ts)),
+                     "note": "the small sample behind the earlier work-model extrapolation; the value "
+                             "above is measured at the full N"},
     }
 
 
@@ -516,6 +561,15 @@ def run_c4(args):
         "batch_report": report,
         "grouped_device_ms": float(np.median(dev_ms)) if report.get("grouped") else None,
     }
+    # the dominant class of grouped launches: the FP32 squaring rounds, three
+    # real symmetric half products (3 n^3 flops) per squaring and walk, over
+    # the class's GEMM device time of the chosen autotuner candidate
+    cms = report.get("class_ms") or []
+    if report.get("grouped") and cms and cms[0] > 0:
+        sq_alg = sum(3.0 * float(n) ** 3 * x["s"] for n, x in zip(ns, st) if x["structured"])
+        res["squaring_class"] = {"algorithmic_flops": sq_alg, "class_ms": cms[0],
+                                 "tflops": sq_alg / (cms[0] * 1e-3) / 1e12,
+                                 "frac_of_fp32_peak": sq_alg / (cms[0] * 1e-3) / 1e12 / FP32_PEAK_TFLOPS}
     # the same batch with fp16x2 squarings, and walk by walk on one lane (the unbatched form)
     variants = {}
     if not args.no_variants:
@@ -780,7 +834,12 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--dtype", choices=["complex64", "complex128"], default="complex64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-n", type=int, default=2000)
+    ap.add_argument("--cpu-sample-n", type=int, default=2000,
+                    help="order of the small scipy sample behind the cpu_baseline estimate")
+    ap.add_argument("--cpu-repeats", type=int, default=1,
+                    help="scipy.linalg.expm runs at the full N for cpu_baseline.value (median)")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="c2: skip the C4 (256 uneven walks) and C5 (times x states) child runs")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the FP64 eigendecomposition reference (torch.linalg.eigh, child process)")
     ap.add_argument("--eigh-ref", default=None, help=argparse.SUPPRESS)
@@ -922,8 +981,6 @@ def main():
     # dominant kernel: the squaring (s launches per step); the phase time
     # comes from HIP events on the stream the squarings run on
     sq_ms = float(np.mean([p["ms_square"] for p in phase])) / max(s, 1)
-    alg_flops = 8.0 * n ** 3  # per squaring, 4M complex convention
-    achieved = alg_flops / (sq_ms * 1e-3) / 1e12
     T = (n + 127) // 128
     np_pad = T * 128
     kpad = (n + 15) // 16 * 16
@@ -942,7 +999,9 @@ def main():
                  "N=6144 lower 64x64 tiles of 4 waves, above it 128x128 tiles of 16 waves plus a 64x64 launch "
                  "for a last partial wave of tiles; 8x8 tile groups; each wave 32x32)")
         traffic_key = "fp32_squaring" if cdt == np.complex64 else None
-    issued_tf = issued / (sq_ms * 1e-3) / 1e12
+    roof = squaring_roofline(n, sq_ms, issued, peak,
+                             split_parts=(3 if args.precision == "fp16x2" else 6) if split else 0)
+    issued_tf = roof["issued_tflops"]
     traffic = _profiled_traffic(traffic_key) if n == 10000 else None
     sustained = _sustained_peak("fp16" if split and args.precision == "fp16x2" else
                                 "bf16" if split else "fp32") if cdt == np.complex64 else None
@@ -999,17 +1058,17 @@ def main():
                    "pade_int8_slices": st.get("fp64_slices", 0), "int8_slice_scope": st.get("fp64_slices_scope", 0)},
         "phases_ms": {k: st[k] for k in ("ms_analyze", "ms_pade", "ms_solve", "ms_square", "ms_output",
                                           "ms_total")},
-        "roofline": {"kernel": kname, "bound": "mfma",
-                     "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+        "roofline": {"kernel": kname, "bound": "mfma", **roof,
                      "traffic": traffic, "traffic_source": TRAFFIC_PROFILE if traffic else None,
-                     "avg_launch_ms": sq_ms,
-                     "issued_tflops": issued_tf, "issued_frac": issued_tf / peak,
+                     "avg_launch_ms": sq_ms, "launches_per_step": s,
                      "sustained_peak": sustained, "sustained_source": MFMA_PROFILE if sustained else None,
                      "issued_frac_of_sustained": issued_tf / sustained if sustained else None,
-                     "note": "achieved = 8 N^3 algorithmic flops per squaring (4M complex convention) / "
-                             "avg launch; issued = MFMA flops the kernel executes (half tiles; equal to the "
-                             "profiler's SQ_INSTS_VALU_MFMA_MOPS x 512); traffic = HBM bytes per launch from "
-                             "FETCH_SIZE / WRITE_SIZE scaled by a copy of known size"},
+                     "note": "achieved = 3 N^3 flops per squaring (the structured algorithm's three real "
+                             "symmetric half products) / avg launch time from HIP events on the squaring "
+                             "stream; issued = MFMA flops the kernel executes (whole lower tiles; equal to "
+                             "the profiler's SQ_INSTS_VALU_MFMA_MOPS x 512); achieved_4m_convention = "
+                             "8 N^3 / launch, the general complex convention; traffic = HBM bytes per launch "
+                             "from FETCH_SIZE / WRITE_SIZE scaled by a copy of known size"},
         "accuracy": acc,
     }
     if cdt == np.complex64 and args.precision == "native" and not args.no_split:
@@ -1033,11 +1092,43 @@ def main():
         out["api_timing"] = _api_timing(eng, a, args)
     if cdt == np.complex64 and not args.no_c128:
         out["complex128"] = _c128_run(a, args, ref)
+    eng.free()
+    del a
+    if not args.no_batched:
+        # the batched workloads in child processes (each its own GPU context)
+        for cfg, extra in (("c4", ["--no-variants", "--no-cpu-baseline"]), ("c5", ["--no-split"])):
+            print(f"[bench] {cfg} (child process) ...", file=sys.stderr, flush=True)
+            out[cfg] = _child_summary(cfg, extra, args)
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(n, s, args.cpu_sample_n, args.t, args.seed)
-    eng.free()
+        out["cpu_baseline"] = cpu_baseline(n, args.t, args.p, args.seed, repeats=args.cpu_repeats,
+                                           n_est=args.cpu_sample_n)
     print(json.dumps(out), flush=True)
+
+
+def _child_summary(cfg, extra, args):
+    """Runs ``bench.py --config cfg`` in a child process and keeps the
+    figures of its JSON line that the default line reports."""
+    import subprocess
+
+    cmd = [sys.executable, os.path.abspath(__file__), "--config", cfg, "--steps", str(max(3, args.steps)),
+           "--warmup", "1", *extra]
+    try:
+        cp = subprocess.run(cmd, capture_output=True, text=True, timeout=420)
+    except subprocess.TimeoutExpired:
+        return {"error": "timed out"}
+    sys.stderr.write(cp.stderr[-2000:])
+    try:
+        r = json.loads(cp.stdout.strip().splitlines()[-1])
+    except (ValueError, IndexError):
+        return {"error": f"exit code {cp.returncode}"}
+    keep = {"metric": r.get("metric"), "value": r.get("value"), "unit": r.get("unit"),
+            "ms_per_step": r.get("ms_per_step"), "steps": r.get("steps"), "config": r.get("config"),
+            "accuracy": r.get("accuracy")}
+    if cfg == "c4":
+        keep["batch_report"] = r.get("batch_report")
+        keep["squaring_class"] = r.get("squaring_class")
+    return keep
 
 
 if __name__ == "__main__":

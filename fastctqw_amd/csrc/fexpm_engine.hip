@@ -85,13 +85,18 @@ static void choose_degree(double norm, int& m, int& s) {
   while (norm / std::ldexp(1.0, s) > kTheta[4]) ++s;
 }
 
-// Smallest of the spectral-radius bounds ||B^k||_1^(1/k), k = 1, 2, 4, 6
-// (h[] = the four 1-norms), widened by 1e-9 to cover the rounding of the
-// norms and of the roots.
-static double spectral_bound(const double h[4]) {
-  static const double e[4] = {1.0, 2.0, 4.0, 6.0};
+// Smallest of the spectral-radius bounds of a real symmetric B:
+//   ||B^k||_1^(1/k), k = 1, 2, 4, 6 (h[0..3], the 1-norms), and
+//   trace(B^2k)^(1/2k), 2k = 4, 6, ..., 12 (h[4..8]; the sum of the 2k-th
+//   powers of the eigenvalues is at least rho^2k),
+// widened by 1e-9 to cover the rounding of the norms, sums and roots.  The
+// trace bounds are the tight ones when the spectrum has no isolated
+// extreme eigenvalue (about 1.2 rho against 1.6 rho at n = 4000).
+constexpr int kSpecNorms = 9;
+static double spectral_bound(const double h[kSpecNorms]) {
+  static const double e[kSpecNorms] = {1.0, 2.0, 4.0, 6.0, 4.0, 6.0, 8.0, 10.0, 12.0};
   double rho = INFINITY;
-  for (int k = 0; k < 4; ++k)
+  for (int k = 0; k < kSpecNorms; ++k)
     if (std::isfinite(h[k]) && h[k] >= 0.0) rho = std::min(rho, std::pow(h[k], 1.0 / e[k]));
   return rho * (1.0 + 1e-9);
 }
@@ -158,15 +163,37 @@ static SquarePlan square_plan(int m, int tile, bool tail_split) {
 }
 
 // ---------------------------------------------------------------------------
-template <typename T> static T* dalloc(size_t count) {
+// Device memory of the pipelines comes from hipMalloc, or, while an arena
+// is active on the calling thread, from that fixed region (the
+// no-allocation entry point, fexpm_run_device_noalloc): a bump allocator
+// that never frees and fails with FEXPM_ENOMEM when the region is full.
+struct Arena {
+  char* base = nullptr;
+  size_t cap = 0, used = 0;
+};
+static thread_local Arena* g_arena = nullptr;
+
+static void* dmalloc_bytes(size_t bytes) {
+  if (g_arena) {
+    const size_t off = (g_arena->used + 255) / 256 * 256;
+    if (off > g_arena->cap || bytes > g_arena->cap - off)
+      throw Error(FEXPM_ENOMEM, "no-allocation scratch array exhausted");
+    g_arena->used = off + bytes;
+    return g_arena->base + off;
+  }
   void* p = nullptr;
+  FX_HIP(hipMalloc(&p, bytes));
+  return p;
+}
+// Zeroed device array (arena memory is cleared once when the arena is set up).
+template <typename T> static T* dalloc(size_t count) {
   if (count == 0) count = 1;
-  FX_HIP(hipMalloc(&p, count * sizeof(T)));
-  FX_HIP(hipMemset(p, 0, count * sizeof(T)));
+  void* p = dmalloc_bytes(count * sizeof(T));
+  if (!g_arena) FX_HIP(hipMemset(p, 0, count * sizeof(T)));
   return static_cast<T*>(p);
 }
 static void dfree(void* p) {
-  if (p) (void)hipFree(p);
+  if (p && !g_arena) (void)hipFree(p);
 }
 
 template <typename W> struct Plane2 {
@@ -426,7 +453,7 @@ template <typename W> struct Pipeline : PipelineBase {
   double2* colsig = nullptr;  // double-double row-block partial row sums of Im(A)
   double2* rsum = nullptr;    // double-double row sums of Im(A)
   double* npart = nullptr;    // 1-norm partial column sums (spectral bounds)
-  double* d_norms = nullptr;  // ||B^k||_1, k = 1, 2, 4, 6
+  double* d_norms = nullptr;  // ||B^k||_1, k = 1, 2, 4, 6; trace B^4, B^6, ..., B^12
   double2* dsq[2] = {nullptr, nullptr};  // split-form diagonals (squarings)
   int* flags = nullptr;
   Scalars* scal = nullptr;
@@ -442,7 +469,10 @@ template <typename W> struct Pipeline : PipelineBase {
   bool have_result = false;
   double flops = 0.0;
 
-  Pipeline(int n_, int dtype_) : n(n_), dtype(dtype_), ncap(n_) {
+  bool in_arena = false;  // buffers carved from an arena: never freed
+
+  Pipeline(int n_, int dtype_, bool with_events = true) : n(n_), dtype(dtype_), ncap(n_) {
+    in_arena = g_arena != nullptr;
     // FP32 engines: the Pade products of their FP64 stages from 6 int8
     // slices (same accuracy as FP64 MFMA, measured), the solve in FP64 MFMA
     if (std::is_same<W, float>::value) {
@@ -482,11 +512,14 @@ template <typename W> struct Pipeline : PipelineBase {
     flags = dalloc<int>(4);
     scal = dalloc<Scalars>(1);
     info_buf = dalloc<int>(1);
-    for (int k = 0; k < 8; ++k) FX_HIP(hipEventCreate(&ev[k]));
-    events = true;
+    if (with_events) {
+      for (int k = 0; k < 8; ++k) FX_HIP(hipEventCreate(&ev[k]));
+      events = true;
+    }
   }
 
   ~Pipeline() override {
+    if (in_arena) return;  // arena memory belongs to the arena
     Plane2<W>* all[] = {&a, &p2, &p4, &p6, &t1, &t2, &u, &v, &x0, &x1};
     for (Plane2<W>* p : all) {
       dfree(p->r);
@@ -593,8 +626,8 @@ template <typename W> struct Pipeline : PipelineBase {
     oz_cap = 0;
     oz_rows = 0;
     for (int b = 0; b < 2; ++b) {
-      FX_HIP(hipMalloc(&oz_s[b], need));
-      FX_HIP(hipMalloc(&oz_e[b], (size_t)rows * sizeof(int)));
+      oz_s[b] = static_cast<int8_t*>(dmalloc_bytes(need));
+      oz_e[b] = static_cast<int*>(dmalloc_bytes((size_t)rows * sizeof(int)));
     }
     oz_cap = need;
     oz_rows = rows;
@@ -831,8 +864,9 @@ template <typename W> struct Pipeline : PipelineBase {
   // where alpha != 0 deflates an exact eigenvector 1 of G (all row sums of
   // G equal to c): alpha = (mu - c) / n moves its eigenvalue c - mu to 0, B
   // commutes with J, and exp(A) = e^{i mu} (exp(i B) + (e^{i(c - mu)} - 1) J / n).
-  // The bounds rho(B) <= ||B^k||_1^(1/k), k = 1, 2, 4, 6, use the Pade
-  // powers, computed here on the unscaled B and reused by pade_assemble with
+  // The bounds rho(B) <= ||B^k||_1^(1/k), k = 1, 2, 4, 6, and
+  // rho(B) <= trace(B^2k)^(1/2k), 2k = 4, ..., 12, use the Pade powers,
+  // computed here on the unscaled B and reused by pade_assemble with
   // scaled coefficients.  For the Laplacian of a dense graph, whose
   // eigenvalue 0 lies far from the rest of the spectrum, this removes 3-4 of
   // 13 squarings at N = 10^4.  Returns the smallest bound (FP64 planes only).
@@ -841,8 +875,10 @@ template <typename W> struct Pipeline : PipelineBase {
     if constexpr (std::is_same<W, double>::value) {
       constexpr int kRows = 64;
       const int nch = (n + kRows - 1) / kRows;
-      if (!npart) npart = dalloc<double>((size_t)std::max(1, (ncap + kRows - 1) / kRows) * std::max(ncap, 1));
-      if (!d_norms) d_norms = dalloc<double>(4);
+      if (!npart)
+        npart = dalloc<double>(std::max((size_t)std::max(1, (ncap + kRows - 1) / kRows) * std::max(ncap, 1),
+                                        (size_t)5 * std::max(ncap, 1)));
+      if (!d_norms) d_norms = dalloc<double>(kSpecNorms);
       dim3 grid((n + 255) / 256, n);
       prepare_planar<Tin, W><<<grid, 256, 0, st>>>(d_a, n, a.r, a.i, ld, 0.0, mu_im, 1.0, 1, alpha);
       FX_LAUNCH();
@@ -854,7 +890,11 @@ template <typename W> struct Pipeline : PipelineBase {
         norm1_final<<<1, 1024, 0, st>>>(npart, nch, n, d_norms + k);
         FX_LAUNCH();
       }
-      double h[4];
+      trace_dots_part<<<(unsigned)n, 256, 0, st>>>(p2.r, p4.r, p6.r, ld, n, npart);
+      FX_LAUNCH();
+      trace_dots_final<<<1, 256, 0, st>>>(npart, n, d_norms + 4);
+      FX_LAUNCH();
+      double h[kSpecNorms];
       FX_HIP(hipMemcpyAsync(h, d_norms, sizeof(h), hipMemcpyDeviceToHost, st));
       FX_HIP(hipStreamSynchronize(st));
       return spectral_bound(h);
@@ -1163,7 +1203,7 @@ template <typename W> struct Pipeline : PipelineBase {
     if (!o) {
       std::vector<int> h((size_t)T * (T + 1) / 2);
       sym_tile_order(T, SQ_GROUP, h.data());
-      FX_HIP(hipMalloc(&o, h.size() * sizeof(int)));
+      o = static_cast<int*>(dmalloc_bytes(h.size() * sizeof(int)));
       FX_HIP(hipMemcpyAsync(o, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice, st));
       FX_HIP(hipStreamSynchronize(st));
     }
@@ -1178,9 +1218,9 @@ template <typename W> struct Pipeline : PipelineBase {
     const int kpad = (int)round_up(n, 16);
     const size_t need = (size_t)npcap * (size_t)round_up(ncap, 16) * 6 * 2;  // room for either split
     if (!sbuf || sbuf_bytes < need) {
-      if (sbuf) FX_HIP(hipFree(sbuf));
+      dfree(sbuf);
       sbuf = nullptr;
-      FX_HIP(hipMalloc(&sbuf, need));
+      sbuf = dmalloc_bytes(need);
       sbuf_bytes = need;
     }
     const int T = (n + SQ_BM - 1) / SQ_BM;
@@ -1305,7 +1345,7 @@ template <typename W> struct Pipeline : PipelineBase {
   std::unique_ptr<Pipeline<double>> hi;
   Pipeline<double>* companion() {
     if (!hi) {
-      hi.reset(new Pipeline<double>(ncap, FEXPM_COMPLEX128));
+      hi.reset(new Pipeline<double>(ncap, FEXPM_COMPLEX128, events));
       hi->panel_coop = panel_coop;
       hi->set_size(n, nullptr);
       FX_HIP(hipDeviceSynchronize());
@@ -1474,6 +1514,28 @@ template <typename W> struct Pipeline : PipelineBase {
     last_structured = P.structured;
   }
 
+  // Every buffer that a run would otherwise allocate on first use, at the
+  // largest size an order up to the capacity needs (no-allocation pipelines:
+  // later runs then take nothing from the arena): the FP64 companion, the
+  // int8 slices of the FP64 products, the tile orders of the symmetric
+  // slice products and the spectral-bound work buffers.  The Cholesky
+  // solve's Z updates stay on the caller's stream.
+  void reserve_all() {
+    z_overlap = 0;
+    if constexpr (std::is_same<W, float>::value) {
+      Pipeline<double>* h = companion();
+      h->z_overlap = 0;
+      h->reserve_all();
+    } else {
+      oz_reserve((int)npcap, (int)round_up(2 * npcap, OZ_BK));
+      for (int T = 1; T * SQ_BM <= npcap; ++T) tile_order(T, nullptr);
+      constexpr int kRows = 64;
+      npart = dalloc<double>(std::max((size_t)std::max(1, (ncap + kRows - 1) / kRows) * std::max(ncap, 1),
+                                      (size_t)5 * std::max(ncap, 1)));
+      d_norms = dalloc<double>(kSpecNorms);
+    }
+  }
+
   void fill_stats(const Plan& P, fexpm_stats& S) const {
     S.nonfinite = P.nonfinite ? 1 : 0;
     S.norm_plain = P.norm_plain;
@@ -1508,21 +1570,23 @@ template <typename W> struct Pipeline : PipelineBase {
   // at the end).
   template <typename Tin, typename Tout>
   void run_typed(const Tin* d_a, Tout* d_u, hipStream_t st, fexpm_stats& S, int opt_structure) {
-    FX_HIP(hipEventRecord(ev[0], st));
+    const bool timing = events;  // pipelines without events (no-allocation path) report no phase times
+    if (timing) FX_HIP(hipEventRecord(ev[0], st));
     analyze_enqueue<Tin>(d_a, st, nullptr);
     Scalars hs;
     FX_HIP(hipMemcpyAsync(&hs, scal, sizeof(Scalars), hipMemcpyDeviceToHost, st));
     FX_HIP(hipStreamSynchronize(st));
-    FX_HIP(hipEventRecord(ev[1], st));
+    if (timing) FX_HIP(hipEventRecord(ev[1], st));
     Plan P = make_plan(hs, opt_structure, true);
     if (P.spectral) spectral_plan<Tin>(d_a, P, st);
-    enqueue_rest<Tin, Tout>(d_a, d_u, P, st, true, info_buf);
+    enqueue_rest<Tin, Tout>(d_a, d_u, P, st, timing, info_buf);
     int info = 0;
     FX_HIP(hipMemcpyAsync(&info, info_buf, sizeof(int), hipMemcpyDeviceToHost, st));
     FX_HIP(hipStreamSynchronize(st));
     fill_stats(P, S);
-    if (P.nonfinite) {
+    if (P.nonfinite || !timing) {
       S.ms_total = S.ms_analyze = S.ms_pade = S.ms_solve = S.ms_square = S.ms_output = 0.0;
+      if (!P.nonfinite) check_info(info);
       return;
     }
     float t01, t12, t23, t34, t45;

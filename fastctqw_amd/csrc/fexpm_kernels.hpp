@@ -537,6 +537,57 @@ __global__ __launch_bounds__(1024) void norm1_final(const double* __restrict__ p
   if (threadIdx.x == 0) *out = sm[0];
 }
 
+// Traces of even powers of a real symmetric B from its powers B^2, B^4,
+// B^6 (FP64 planes, n x n region): trace(B^(2j+2k)) = sum_il (B^2j)_il
+// (B^2k)_il.  One workgroup per row writes part[row * 5 + q] for
+// q = 0..4: trace B^4, B^6, B^8, B^10, B^12; trace_dots_final sums the rows
+// in order (deterministic).
+__global__ __launch_bounds__(256) void trace_dots_part(const double* __restrict__ b2, const double* __restrict__ b4,
+                                                       const double* __restrict__ b6, int64_t ld, int n,
+                                                       double* __restrict__ part) {
+  __shared__ double sm[5][256];
+  const int i = blockIdx.x;
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int j = threadIdx.x; j < n; j += 256) {
+    const int64_t o = (int64_t)i * ld + j;
+    const double x2 = b2[o], x4 = b4[o], x6 = b6[o];
+    acc[0] += x2 * x2;
+    acc[1] += x2 * x4;
+    acc[2] += x4 * x4;
+    acc[3] += x4 * x6;
+    acc[4] += x6 * x6;
+  }
+#pragma unroll
+  for (int q = 0; q < 5; ++q) sm[q][threadIdx.x] = acc[q];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+#pragma unroll
+      for (int q = 0; q < 5; ++q) sm[q][threadIdx.x] += sm[q][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 5) part[(int64_t)i * 5 + threadIdx.x] = sm[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(256) void trace_dots_final(const double* __restrict__ part, int n,
+                                                        double* __restrict__ out) {
+  __shared__ double sm[5][256];
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int i = threadIdx.x; i < n; i += 256)
+#pragma unroll
+    for (int q = 0; q < 5; ++q) acc[q] += part[(int64_t)i * 5 + q];
+#pragma unroll
+  for (int q = 0; q < 5; ++q) sm[q][threadIdx.x] = acc[q];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+#pragma unroll
+      for (int q = 0; q < 5; ++q) sm[q][threadIdx.x] += sm[q][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 5) out[threadIdx.x] = sm[threadIdx.x][0];
+}
+
 // Output: out[i][j] = e^mu * (xr + i xi), interleaved complex or real.
 template <typename Tout> struct OutElem;
 template <> struct OutElem<float2> {

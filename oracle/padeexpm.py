@@ -21,8 +21,10 @@ d_u1, d_u2, d_v1, d_v2, d_mu, d_nrmA`` (include/MatrixExpCalculator.h:52-61)
   rho(A - mu I)/2^s <= theta_13, with any upper bound of the spectral radius.
   The bounds are rho(B) <= ||B^k||_1^(1/k) (Gelfand; k = 1, 2, 4, 6, the
   powers the Pade-13 evaluation needs anyway; cf. the d_k = ||A^k||^(1/k)
-  of Al-Mohy & Higham, SIMAX 31(3), 2009).  When every row sum of G equals
-  the same c, the eigenvector 1 is deflated first: B = G - mu I + alpha J
+  of Al-Mohy & Higham, SIMAX 31(3), 2009) and, B being symmetric,
+  rho(B)^2k <= trace(B^2k) = sum of lambda^2k for 2k = 4, ..., 12.
+  When every row sum of G equals the same c, the eigenvector 1 is
+  deflated first: B = G - mu I + alpha J
   with alpha = (mu - c)/n has B 1 = 0 and commutes with J = 1 1^T, and
   exp(A) = e^{i mu} (exp(iB) + (e^{i(c - mu)} - 1) J / n).
 
@@ -107,12 +109,19 @@ def scaling_for(rho: float) -> int:
 
 
 def spectral_bound(b: np.ndarray) -> float:
-    """min over k in (1, 2, 4, 6) of ||b^k||_1^(1/k), each an upper bound of
-    the spectral radius of b, widened by 1e-9 for the rounding of the norms."""
+    """Upper bound of the spectral radius of a real symmetric b: the least of
+    ||b^k||_1^(1/k), k in (1, 2, 4, 6) (Gelfand), and trace(b^2k)^(1/2k),
+    2k in (4, 6, 8, 10, 12) (a sum of the 2k-th powers of the eigenvalues,
+    formed as elementwise products of b^2, b^4, b^6), widened by 1e-9 for
+    rounding."""
     b2 = b @ b
     b4 = b2 @ b2
     b6 = b4 @ b2
     vals = [onenorm(b), onenorm(b2) ** 0.5, onenorm(b4) ** 0.25, onenorm(b6) ** (1.0 / 6.0)]
+    for (x, y), k in (((b2, b2), 4), ((b2, b4), 6), ((b4, b4), 8), ((b4, b6), 10), ((b6, b6), 12)):
+        tr = float(np.sum(x * y))
+        if tr >= 0.0:
+            vals.append(tr ** (1.0 / k))
     return min(vals) * (1.0 + 1e-9)
 
 
@@ -161,6 +170,28 @@ def pade_uv(a: np.ndarray, m: int):
     return a @ odd, even
 
 
+def pade_plan(a: np.ndarray, spectral: bool = False, deflate: bool = True) -> dict:
+    """The decisions of ``pade_expm`` without the exponential: mu, the
+    1-norm, m, s (and, for spectral plans, s_onenorm, rho, deflated, c,
+    alpha and the unscaled B)."""
+    a = np.asarray(a)
+    n = a.shape[0]
+    mu, ash = trace_shift(a)
+    norm = onenorm(ash)
+    m, s = choose_degree(norm)
+    plan = {"mu": mu, "norm": norm, "m": m, "s": s, "ash": ash, "spectral": False, "deflated": False,
+            "s_onenorm": s}
+    g = structured_part(a) if (spectral and m == 13 and 0 < s < 120) else None
+    if g is not None:
+        equal, c = equal_row_sums(g) if deflate else (False, 0.0)
+        alpha = (mu.imag - c) / n if equal else 0.0
+        b = (g - mu.imag * np.eye(n)) + alpha
+        rho = spectral_bound(b)
+        plan.update({"s": scaling_for(rho), "spectral": True, "deflated": bool(equal), "rho": rho,
+                     "c": c if equal else 0.0, "alpha": alpha, "B": b})
+    return plan
+
+
 def pade_expm(a: np.ndarray, stages: bool = False, spectral: bool = False, deflate: bool = True):
     """exp(a) in FP64 by trace shift + Higham 2005 scaling and squaring.
 
@@ -180,22 +211,13 @@ def pade_expm(a: np.ndarray, stages: bool = False, spectral: bool = False, defla
     if n == 0:
         out = np.zeros((0, 0), dtype=np.complex128)
         return {"E": out, "m": 0, "s": 0, "mu": 0j, "norm": 0.0} if stages else out
-    mu, ash = trace_shift(a)
-    norm = onenorm(ash)
-    m, s = choose_degree(norm)
-    info = {"spectral": False, "deflated": False, "s_onenorm": s}
-    g = structured_part(a) if (spectral and m == 13 and 0 < s < 120) else None
-    if g is not None:
-        equal, c = equal_row_sums(g) if deflate else (False, 0.0)
-        alpha = (mu.imag - c) / n if equal else 0.0
-        b = (g - mu.imag * np.eye(n)) + alpha
-        rho = spectral_bound(b)
-        s = scaling_for(rho)
-        scaled = 1j * (b / (2.0 ** s))
-        info.update({"spectral": True, "deflated": bool(equal), "rho": rho, "c": c if equal else 0.0,
-                     "alpha": alpha, "B": b})
+    plan = pade_plan(a, spectral, deflate)
+    mu, norm, m, s = plan["mu"], plan["norm"], plan["m"], plan["s"]
+    info = {k: plan[k] for k in ("spectral", "deflated", "s_onenorm", "rho", "c", "alpha", "B") if k in plan}
+    if plan["spectral"]:
+        scaled = 1j * (plan["B"] / (2.0 ** s))
     else:
-        scaled = ash / (2.0 ** s)
+        scaled = plan["ash"] / (2.0 ** s)
     u, v = pade_uv(scaled, m)
     x = np.linalg.solve(v - u, v + u)
     e = x

@@ -18,3 +18,22 @@ def test_more_than_one_gpu_is_refused(extra):
     assert cp.returncode == 2, cp.stderr
     line = cp.stdout.strip().splitlines()[-1]
     assert "one GPU" in json.loads(line)["error"]
+
+
+def test_roofline_fraction_is_at_most_the_issued_fraction():
+    """The squaring roofline counts the structured algorithm's 3 n^3 flops
+    per launch; the kernel issues whole lower 128 x 128 (or 64 x 64) tiles
+    over the padded k, never fewer, so frac <= issued_frac, and a launch
+    time at the issued peak rate gives frac <= 1."""
+    sys.path.insert(0, ROOT)
+    import bench
+
+    for n in (1000, 4097, 6144, 8192, 10000, 13001):
+        for tile in (64, 128):
+            T = (n + tile - 1) // tile
+            kpad = (n + 15) // 16 * 16
+            issued = 3 * 2.0 * T * (T + 1) / 2 * tile * tile * kpad
+            ms = issued / (bench.FP32_PEAK_TFLOPS * 1e12) * 1e3
+            r = bench.squaring_roofline(n, ms, issued, bench.FP32_PEAK_TFLOPS)
+            assert r["frac"] <= r["issued_frac"] <= 1.0 + 1e-12, (n, tile, r)
+            assert r["achieved_4m_convention"] > r["achieved"]
