@@ -2040,6 +2040,116 @@ int fexpm_run_device(fexpm_engine* e, const void* d_a, void* d_u, void* stream) 
   return run_device_locked(e, d_a, d_u, (hipStream_t)stream);
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// No-allocation entry point (no counterpart in the reference).  Every work
+// buffer of a run of order n <= FEXPM_NOALLOC_NMAX lives in `scratch`, a
+// static device array that each GPU holds until the process ends.  Its
+// device address is looked up once per device (hipGetSymbolAddress, with
+// that device current) and only that address is used on the host; the
+// array is cleared once, and the pipelines carved from it keep their own
+// zero padding from run to run.  Calls are serialised in the process (one
+// mutex) and each returns after its device work is complete; work that a
+// caller leaves in flight on the array from another thread would race.
+static __device__ __align__(256) unsigned char scratch[FEXPM_NOALLOC_BYTES];
+
+namespace {
+struct NoallocCtx {
+  bool ready = false;
+  Arena arena;                      // over this device's copy of `scratch`
+  Pipeline<float>* pf = nullptr;    // float32 / complex64 (arena memory: never destroyed)
+  Pipeline<double>* pd = nullptr;   // float64 / complex128
+};
+std::mutex g_noalloc_mtx;
+std::map<int, NoallocCtx> g_noalloc;  // per device
+
+struct ArenaScope {
+  Arena* prev;
+  explicit ArenaScope(Arena* a) : prev(g_arena) { g_arena = a; }
+  ~ArenaScope() { g_arena = prev; }
+};
+}  // namespace
+
+__global__ __launch_bounds__(256) void zero_words(uint4* __restrict__ p, size_t count) {
+  for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < count; k += (size_t)gridDim.x * 256)
+    p[k] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+// Device address of `scratch` on the current device, the array cleared, and
+// the two pipelines of capacity FEXPM_NOALLOC_NMAX built inside it with every
+// buffer a run would allocate on first use.
+static void noalloc_setup(NoallocCtx& c) {
+  void* ptr = nullptr;
+  FX_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  c.arena.base = static_cast<char*>(ptr);
+  c.arena.cap = FEXPM_NOALLOC_BYTES;
+  c.arena.used = 0;
+  zero_words<<<1024, 256>>>(reinterpret_cast<uint4*>(c.arena.base), FEXPM_NOALLOC_BYTES / sizeof(uint4));
+  FX_LAUNCH();
+  FX_HIP(hipDeviceSynchronize());
+  ArenaScope scope(&c.arena);
+  c.pf = new Pipeline<float>(FEXPM_NOALLOC_NMAX, FEXPM_COMPLEX64, false);
+  c.pf->reserve_all();
+  c.pd = new Pipeline<double>(FEXPM_NOALLOC_NMAX, FEXPM_COMPLEX128, false);
+  c.pd->reserve_all();
+  FX_HIP(hipDeviceSynchronize());
+  c.ready = true;
+}
+
+extern "C" int fexpm_run_device_noalloc(int64_t n, int dtype, const void* d_a, void* d_u, void* stream) {
+  if (elem_size(dtype) == 0) {
+    g_last_error = "unknown dtype";
+    return FEXPM_EINVAL;
+  }
+  if (n < 0 || n > FEXPM_NOALLOC_NMAX) {
+    g_last_error = "fexpm_run_device_noalloc: order n must be in [0, " + std::to_string(FEXPM_NOALLOC_NMAX) +
+                   "], the capacity of the static scratch array";
+    return FEXPM_EINVAL;
+  }
+  if (n > 0 && (!d_a || !d_u)) {
+    g_last_error = "null device pointer";
+    return FEXPM_EINVAL;
+  }
+  if (n == 0) return FEXPM_OK;
+  std::lock_guard<std::mutex> lk(g_noalloc_mtx);
+  return guarded(nullptr, [&] {
+    int dev = 0;
+    FX_HIP(hipGetDevice(&dev));
+    NoallocCtx& c = g_noalloc[dev];
+    if (!c.ready) {
+      try {
+        noalloc_setup(c);
+      } catch (...) {
+        c.ready = false;
+        c.arena.used = 0;
+        delete c.pf;  // arena pipelines free nothing
+        delete c.pd;
+        c.pf = nullptr;
+        c.pd = nullptr;
+        throw;
+      }
+    }
+    ArenaScope scope(&c.arena);  // any allocation now comes from the array, or fails
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    fexpm_stats S;
+    std::memset(&S, 0, sizeof(S));
+    S.n = n;
+    S.dtype = dtype;
+    if (dtype == FEXPM_FLOAT32 || dtype == FEXPM_COMPLEX64) {
+      c.pf->dtype = dtype;
+      c.pf->set_size((int)n, st);
+      c.pf->run(d_a, d_u, st, S, 1);
+    } else {
+      c.pd->dtype = dtype;
+      c.pd->set_size((int)n, st);
+      c.pd->run(d_a, d_u, st, S, 1);
+    }
+  });
+}
+
+extern "C" {
+
 int fexpm_upload(fexpm_engine* e, const void* a) {
   if (!e || (e->n > 0 && !a)) return FEXPM_EINVAL;
   std::lock_guard<std::mutex> lk(e->mtx);

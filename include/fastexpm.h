@@ -19,6 +19,7 @@
 #ifndef FASTEXPM_H
 #define FASTEXPM_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -148,6 +149,20 @@ int fexpm_run(fexpm_engine* e, const void* a, void* u);
  * row-major n x n arrays of the engine's dtype; `stream` is a hipStream_t
  * (NULL = the engine's own stream).  No host transfer of matrix data. */
 int fexpm_run_device(fexpm_engine* e, const void* d_a, void* d_u, void* stream);
+
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The fexpm_run_device computation without an engine:
+ * U = expm(A) for device arrays d_a, d_u of order n <= FEXPM_NOALLOC_NMAX
+ * (row-major, dtype as in fexpm_create), on `stream` (NULL = the default
+ * stream).  Every work buffer lives in a static device array of
+ * FEXPM_NOALLOC_BYTES that each GPU holds until the process ends; the call
+ * allocates no device memory (it creates no engine, stream or event).  A
+ * larger n fails with FEXPM_EINVAL.  Calls in one process are serialised
+ * and each returns after its work is complete.  Errors: fexpm_last_error(NULL). */
+#define FEXPM_NOALLOC_NMAX 256
+#define FEXPM_NOALLOC_BYTES ((size_t)48 << 20)
+int fexpm_run_device_noalloc(int64_t n, int dtype, const void* d_a, void* d_u, void* stream);
 
 /* Engine-resident input/output for callers without device memory of their
  * own: upload A once, run any number of times, download U. */
