@@ -19,7 +19,13 @@ Deliberate differences (drop-in supersets):
   only if something reads ``_hamiltonian``; ``setHamiltonian`` switches back
   to uploading the host A_exp.  U stays in device memory and ``getOperator``
   copies it to the host only when asked (:165-166); ``applyTo`` computes
-  U psi on the device with FP64 accumulation.
+  U psi on the device with FP64 accumulation.  A complex adjacency whose
+  imaginary part is zero (``nx.to_numpy_array(G, dtype=np.complex64)``, the
+  README call) takes the same device route with its real part.
+* ``from_adjacency_list`` / ``reset_adjList`` (empty stubs in the reference,
+  Operator.py:114-116, 147-148) take an edge list; on the GPU only the edges
+  cross PCIe and H is formed on the device.  ``from_erdos_renyi`` generates
+  the seeded G(N, p) on the device (no counterpart in the reference).
 """
 from __future__ import annotations
 
@@ -27,6 +33,16 @@ import numpy as np
 from numpy.typing import NDArray
 
 _GPU_DEVICES = ("cuda", "gpu", "hip")
+
+
+def _operator_dtype(adjacency_dtype):
+    """Operator dtype of an adjacency dtype (the map of Operator.py:27-32)."""
+    dt = np.dtype(adjacency_dtype)
+    if dt in (np.float32, np.complex64):
+        return np.complex64
+    if dt in (np.float64, np.complex128):
+        return np.complex128
+    raise TypeError(f"A.dtype is error : {dt}")
 
 
 def _expm_cpu(a):
@@ -43,6 +59,8 @@ class Operator:
         self._on_device = False
         self._h = None
         self._adj = None
+        self._edges = None  # (n, src, dst, w, adjacency dtype) of an edge-list graph
+        self._er = None     # (n, p, seed, weighted, adjacency dtype) of a device-generated graph
         self._graph_on_device = False
         if isinstance(data, np.ndarray):
             if not self._is_adjacency_matrix(data):
@@ -74,10 +92,13 @@ class Operator:
         """H for adjacency A: on the device for a real A on a GPU operator
         (a real symmetric A gives a real symmetric, hence Hermitian, H),
         otherwise on the host with the reference's Hermitian check."""
-        if self._device in _GPU_DEVICES and not np.iscomplexobj(A):
+        self._edges = self._er = None
+        real = not np.iscomplexobj(A) or not np.any(A.imag)
+        if self._device in _GPU_DEVICES and real:
             if self._expMator is None:
                 self._expMator = self._initExpMator(self._n, self._dtype)
-            self._expMator.set_graph(A, self._gamma, self._laplacian)
+            ra = A.real if np.iscomplexobj(A) else A
+            self._expMator.set_graph(np.ascontiguousarray(ra), self._gamma, self._laplacian)
             self._adj, self._h, self._graph_on_device = A, None, True
             return
         self._adj, self._graph_on_device = None, False
@@ -89,6 +110,16 @@ class Operator:
 
     @property
     def _hamiltonian(self):
+        if self._h is None and self._adj is None and self._edges is not None:
+            from .graphs import dense_from_edges
+
+            n, src, dst, w, adt = self._edges
+            self._adj = dense_from_edges(n, src, dst, w, dtype=adt)
+        if self._h is None and self._adj is None and self._er is not None:
+            from .graphs import erdos_renyi
+
+            n, p, seed, weighted, adt = self._er
+            self._adj = erdos_renyi(n, p, seed, weighted=weighted).astype(adt)
         if self._h is None and self._adj is not None:
             self._h = self._buildHamiltonian(self._adj, self._gamma)
         return self._h
@@ -169,6 +200,70 @@ class Operator:
     def from_numpy_array(cls, A: np.ndarray, gamma: float = 1, laplacian: bool = False,
                          device: str = "cpu") -> "Operator":
         return cls(A, gamma, laplacian, device)
+
+    @classmethod
+    def from_adjacency_list(cls, adj_list, gamma: float = 1, laplacian: bool = False, device: str = "cpu",
+                            num_nodes=None, dtype=np.float32) -> "Operator":
+        """Operator of an undirected graph given as an adjacency list (see
+        fastctqw_amd.graphs.edge_list for the accepted forms); ``dtype`` is
+        the adjacency dtype (float32 -> complex64 operator, float64 ->
+        complex128).  The reference declares this constructor with an empty
+        body (Operator.py:114-116)."""
+        from .graphs import edge_list
+
+        n, src, dst, w = edge_list(adj_list, num_nodes)
+        op = cls(n, device=device, dtype=_operator_dtype(dtype))
+        op._set_edges(n, src, dst, w, gamma, laplacian, dtype)
+        return op
+
+    @classmethod
+    def from_erdos_renyi(cls, n: int, p: float, seed: int, weighted: bool = False, gamma: float = 1,
+                         laplacian: bool = False, device: str = "cpu", dtype=np.float32) -> "Operator":
+        """Operator of the seeded G(n, p) of fastctqw_amd.graphs.erdos_renyi;
+        on the GPU the graph is generated in device memory (same bits)."""
+        op = cls(n, device=device, dtype=_operator_dtype(dtype))
+        op._gamma, op._laplacian = gamma, laplacian
+        if device in _GPU_DEVICES:
+            op._expMator.set_graph_er(p, seed, weighted, gamma, laplacian)
+            op._adj, op._h, op._graph_on_device = None, None, True
+            op._er = (n, p, seed, weighted, dtype)
+        else:
+            from .graphs import erdos_renyi
+
+            op._set_graph(erdos_renyi(n, p, seed, weighted=weighted).astype(dtype))
+        return op
+
+    def _set_edges(self, n, src, dst, w, gamma, laplacian, adt) -> None:
+        self._gamma, self._laplacian = gamma, laplacian
+        if self._device in _GPU_DEVICES:
+            self._expMator.set_graph_edges(src, dst, w, gamma, laplacian)
+            self._adj, self._h, self._graph_on_device = None, None, True
+            self._edges, self._er = (n, src, dst, w, adt), None
+        else:
+            from .graphs import dense_from_edges
+
+            self._set_graph(dense_from_edges(n, src, dst, w, dtype=adt))
+        self._expmMat = None
+        self._on_device = False
+
+    def reset_adjList(self, adj_list, gamma: float = 1, time=None, laplacian: bool = False,
+                      device=None, num_nodes=None, dtype=np.float32) -> None:
+        """Replace the graph by an adjacency list (empty in the reference,
+        Operator.py:147-148; ``time`` is accepted for its signature and
+        unused: pass the time to buildExpmOperator)."""
+        from .graphs import edge_list
+
+        n, src, dst, w = edge_list(adj_list, num_nodes)
+        if device is not None and device != self._device:
+            self.free()
+            self._expMator, self._device = None, device
+        odt = _operator_dtype(dtype)
+        if n != self._n or odt != self._dtype or self._expMator is None:
+            if self._expMator is not None:
+                self._expMator.free()
+            self._n, self._dtype = n, odt
+            self._expMator = self._initExpMator(n, odt)
+        self._set_edges(n, src, dst, w, gamma, laplacian, dtype)
 
     @classmethod
     def from_num_nodes(cls, n: int, dtype=np.complex64, device: str = "cpu"):

@@ -2217,6 +2217,100 @@ int fexpm_set_graph(fexpm_engine* e, const void* adj, int adj_dtype, double gamm
   });
 }
 
+// H = -gamma A or -gamma (D - A) in place, A already in d_h.
+static void finish_graph(fexpm_engine* e, double gamma, int laplacian) {
+  const int n = (int)e->n;
+  double* rs = nullptr;
+  if (laplacian) FX_HIP(hipMalloc(&rs, (size_t)n * sizeof(double)));
+  try {
+    if (rs) {
+      graph_row_sums<double><<<n, 256, 0, e->stream>>>(e->d_h, n, rs);
+      FX_LAUNCH();
+    }
+    dim3 grid((unsigned)((n + 255) / 256), (unsigned)n);
+    graph_hamiltonian_inplace<<<grid, 256, 0, e->stream>>>(e->d_h, n, gamma, rs);
+    FX_LAUNCH();
+    FX_HIP(hipStreamSynchronize(e->stream));
+  } catch (...) {
+    if (rs) (void)hipFree(rs);
+    throw;
+  }
+  if (rs) (void)hipFree(rs);
+  e->have_graph = true;
+}
+
+static int graph_engine_check(fexpm_engine* e) {
+  if (e->freed) return FEXPM_EFREED;
+  if (e->dtype != FEXPM_COMPLEX64 && e->dtype != FEXPM_COMPLEX128) {
+    e->err = "graph input needs a complex engine (A_exp = -i t H is complex)";
+    return FEXPM_EINVAL;
+  }
+  return FEXPM_OK;
+}
+
+int fexpm_set_graph_edges(fexpm_engine* e, int64_t m, const int64_t* src, const int64_t* dst, const double* w,
+                          double gamma, int laplacian) {
+  if (!e || m < 0 || (m > 0 && (!src || !dst))) return FEXPM_EINVAL;
+  std::lock_guard<std::mutex> lk(e->mtx);
+  int rc = graph_engine_check(e);
+  if (rc != FEXPM_OK) return rc;
+  for (int64_t k = 0; k < m; ++k)
+    if (src[k] < 0 || src[k] >= e->n || dst[k] < 0 || dst[k] >= e->n) {
+      e->err = "edge " + std::to_string(k) + ": vertex index outside [0, n)";
+      return FEXPM_EINVAL;
+    }
+  if (e->n == 0) return FEXPM_OK;
+  return guarded(e, [&] {
+    const int n = (int)e->n;
+    const int64_t cnt = (int64_t)n * n;
+    e->have_graph = false;
+    ensure_io(e);  // fexpm_run_graph writes A_exp into the input buffer
+    if (!e->d_h) FX_HIP(hipMalloc(&e->d_h, (size_t)cnt * sizeof(double)));
+    fill_zero<<<(unsigned)std::min<int64_t>((cnt + 255) / 256, 65536), 256, 0, e->stream>>>(e->d_h, cnt);
+    FX_LAUNCH();
+    if (m > 0) {
+      const size_t ib = (size_t)m * sizeof(int64_t), wb = w ? (size_t)m * sizeof(double) : 0;
+      char* buf = nullptr;
+      FX_HIP(hipMalloc(&buf, 2 * ib + wb));
+      try {
+        FX_HIP(hipMemcpyAsync(buf, src, ib, hipMemcpyHostToDevice, e->stream));
+        FX_HIP(hipMemcpyAsync(buf + ib, dst, ib, hipMemcpyHostToDevice, e->stream));
+        if (w) FX_HIP(hipMemcpyAsync(buf + 2 * ib, w, wb, hipMemcpyHostToDevice, e->stream));
+        graph_edges<<<(unsigned)((m + 255) / 256), 256, 0, e->stream>>>(
+            e->d_h, n, m, (const int64_t*)buf, (const int64_t*)(buf + ib), w ? (const double*)(buf + 2 * ib) : nullptr);
+        FX_LAUNCH();
+        FX_HIP(hipStreamSynchronize(e->stream));
+      } catch (...) {
+        (void)hipStreamSynchronize(e->stream);
+        (void)hipFree(buf);
+        throw;
+      }
+      (void)hipFree(buf);
+    }
+    finish_graph(e, gamma, laplacian);
+  });
+}
+
+int fexpm_set_graph_er(fexpm_engine* e, double p, uint64_t seed, int weighted, double gamma, int laplacian) {
+  if (!e || !(p >= 0.0 && p <= 1.0)) return FEXPM_EINVAL;
+  std::lock_guard<std::mutex> lk(e->mtx);
+  int rc = graph_engine_check(e);
+  if (rc != FEXPM_OK) return rc;
+  if (e->n == 0) return FEXPM_OK;
+  return guarded(e, [&] {
+    const int n = (int)e->n;
+    e->have_graph = false;
+    ensure_io(e);  // fexpm_run_graph writes A_exp into the input buffer
+    if (!e->d_h) FX_HIP(hipMalloc(&e->d_h, (size_t)n * n * sizeof(double)));
+    // hash streams 1 (edges) and 2 (weights), as in fastctqw_amd/graphs.py
+    const uint64_t ke = hash_stream_key(seed, 1), kw = hash_stream_key(seed, 2);
+    dim3 grid((unsigned)((n + 255) / 256), (unsigned)n);
+    graph_er<<<grid, 256, 0, e->stream>>>(e->d_h, n, p, ke, kw, weighted ? 1 : 0);
+    FX_LAUNCH();
+    finish_graph(e, gamma, laplacian);
+  });
+}
+
 int fexpm_run_graph(fexpm_engine* e, double t) {
   if (!e) return FEXPM_EINVAL;
   std::lock_guard<std::mutex> lk(e->mtx);
@@ -2227,6 +2321,8 @@ int fexpm_run_graph(fexpm_engine* e, double t) {
     return FEXPM_EINVAL;
   }
   int rc = guarded(e, [&] {
+    ensure_io(e);  // A_exp goes to the input buffer, U to the output buffer
+    if (!e->d_in || !e->d_h) throw Error(FEXPM_EINVAL, "graph buffers missing");
     const int64_t cnt = e->n * e->n;
     const unsigned blocks = (unsigned)((cnt + 255) / 256);
     if (e->dtype == FEXPM_COMPLEX64)
@@ -3220,6 +3316,21 @@ int fexpm_test_lu_solve(int n, const double* mr, const double* mi, const double*
         xi[(size_t)i * n + j] = hi[(size_t)i * ldl + np + j];
       }
     if (info) *info = hs.info;
+  });
+}
+
+int fexpm_debug_hamiltonian(fexpm_engine* e, double* h) {
+  if (!e || (e->n > 0 && !h)) return FEXPM_EINVAL;
+  std::lock_guard<std::mutex> lk(e->mtx);
+  if (e->freed) return FEXPM_EFREED;
+  if (e->n == 0) return FEXPM_OK;
+  if (!e->have_graph) {
+    e->err = "no graph: call fexpm_set_graph first";
+    return FEXPM_EINVAL;
+  }
+  return guarded(e, [&] {
+    FX_HIP(hipStreamSynchronize(e->stream));
+    FX_HIP(hipMemcpy(h, e->d_h, (size_t)e->n * e->n * sizeof(double), hipMemcpyDeviceToHost));
   });
 }
 

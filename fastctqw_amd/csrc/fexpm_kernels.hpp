@@ -1566,6 +1566,67 @@ __global__ __launch_bounds__(256) void graph_hamiltonian(const Ta* __restrict__ 
   h[(int64_t)i * n + j] = -gamma * v;
 }
 
+// In place: h = -gamma * A or -gamma * (D - A), A held in h (rs: its row sums).
+__global__ __launch_bounds__(256) void graph_hamiltonian_inplace(double* __restrict__ h, int n, double gamma,
+                                                                 const double* __restrict__ rs) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  const int i = blockIdx.y;
+  if (j >= n) return;
+  const int64_t o = (int64_t)i * n + j;
+  const double a = h[o];
+  const double v = rs ? ((i == j) ? rs[i] - a : -a) : a;
+  h[o] = -gamma * v;
+}
+
+// Counter-based hash of the seeded graph generators (splitmix64; the same
+// bits as fastctqw_amd/graphs.py): u = 53 hash bits of counter ^ key in [0, 1),
+// key = splitmix64(splitmix64(seed) ^ stream).
+__host__ __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+  uint64_t z = x + 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__host__ __device__ __forceinline__ uint64_t hash_stream_key(uint64_t seed, uint64_t stream) {
+  return splitmix64(splitmix64(seed) ^ stream);
+}
+__device__ __forceinline__ double hash_uniform(uint64_t key, uint64_t ctr) {
+  return (double)(splitmix64(ctr ^ key) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// Dense adjacency of the seeded G(n, p): edge {i, j}, i < j, iff
+// u(edge key, i n + j) < p, weight 1 or 1 - u(weight key, i n + j).
+__global__ __launch_bounds__(256) void graph_er(double* __restrict__ a, int n, double p, uint64_t key_e,
+                                                uint64_t key_w, int weighted) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  const int i = blockIdx.y;
+  if (j >= n) return;
+  double v = 0.0;
+  if (i != j) {
+    const uint64_t lo = (uint64_t)min(i, j), hi = (uint64_t)max(i, j);
+    const uint64_t ctr = lo * (uint64_t)n + hi;
+    if (hash_uniform(key_e, ctr) < p) v = weighted ? 1.0 - hash_uniform(key_w, ctr) : 1.0;
+  }
+  a[(int64_t)i * n + j] = v;
+}
+
+__global__ __launch_bounds__(256) void fill_zero(double* __restrict__ a, int64_t cnt) {
+  for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < cnt; k += (int64_t)gridDim.x * 256) a[k] = 0.0;
+}
+
+// Undirected edge list -> dense adjacency (zeroed before): a_ij = a_ji = w,
+// a self-loop on the diagonal once; repeated pairs add their weights.
+__global__ __launch_bounds__(256) void graph_edges(double* __restrict__ a, int n, int64_t m,
+                                                   const int64_t* __restrict__ src, const int64_t* __restrict__ dst,
+                                                   const double* __restrict__ w) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k >= m) return;
+  const int64_t i = src[k], j = dst[k];
+  const double v = w ? w[k] : 1.0;
+  atomicAdd(&a[i * n + j], v);
+  if (i != j) atomicAdd(&a[j * n + i], v);
+}
+
 // A_exp = -i t H (real H) into the engine input: complex64 / complex128.
 template <typename Tc>
 __global__ __launch_bounds__(256) void walk_input(const double* __restrict__ h, int64_t cnt, double t,

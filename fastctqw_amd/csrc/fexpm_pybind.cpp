@@ -147,6 +147,51 @@ template <typename T, int DT> class ExpMat {
     raise_status(rc, e_);
   }
 
+  void set_graph_edges(py::array_t<int64_t, py::array::c_style | py::array::forcecast> src,
+                       py::array_t<int64_t, py::array::c_style | py::array::forcecast> dst, py::object w,
+                       double gamma, bool laplacian) {
+    if (src.ndim() != 1 || dst.ndim() != 1 || src.shape(0) != dst.shape(0))
+      throw std::invalid_argument("fastexpm.set_graph_edges: src and dst must be 1-D of equal length");
+    const int64_t m = src.shape(0);
+    py::array_t<double, py::array::c_style | py::array::forcecast> wa;
+    const double* wp = nullptr;
+    if (!w.is_none()) {
+      wa = py::array_t<double, py::array::c_style | py::array::forcecast>(w);
+      if (wa.ndim() != 1 || wa.shape(0) != m)
+        throw std::invalid_argument("fastexpm.set_graph_edges: one weight per edge expected");
+      wp = wa.data();
+    }
+    const int64_t* ps = src.data();
+    const int64_t* pd = dst.data();
+    int rc;
+    {
+      py::gil_scoped_release nogil;
+      rc = fexpm_set_graph_edges(e_, m, ps, pd, wp, gamma, laplacian ? 1 : 0);
+    }
+    raise_status(rc, e_);
+  }
+
+  void set_graph_er(double p, uint64_t seed, bool weighted, double gamma, bool laplacian) {
+    int rc;
+    {
+      py::gil_scoped_release nogil;
+      rc = fexpm_set_graph_er(e_, p, seed, weighted ? 1 : 0, gamma, laplacian ? 1 : 0);
+    }
+    raise_status(rc, e_);
+  }
+
+  py::array_t<double> debug_hamiltonian() {
+    py::array_t<double> out({n_, n_});
+    double* p = out.mutable_data();
+    int rc;
+    {
+      py::gil_scoped_release nogil;
+      rc = fexpm_debug_hamiltonian(e_, p);
+    }
+    raise_status(rc, e_);
+    return out;
+  }
+
   void run_graph(double t) {
     int rc;
     {
@@ -431,6 +476,11 @@ template <typename T, int DT> static void bind(py::module_& m, const char* name)
       .def("run_resident", &C::run_resident)
       .def("set_graph", &C::set_graph, py::arg("adj"), py::arg("gamma") = 1.0, py::arg("laplacian") = false)
       .def("run_graph", &C::run_graph, py::arg("t"))
+      .def("set_graph_edges", &C::set_graph_edges, py::arg("src"), py::arg("dst"), py::arg("w") = py::none(),
+           py::arg("gamma") = 1.0, py::arg("laplacian") = false)
+      .def("set_graph_er", &C::set_graph_er, py::arg("p"), py::arg("seed"), py::arg("weighted") = false,
+           py::arg("gamma") = 1.0, py::arg("laplacian") = false)
+      .def("debug_hamiltonian", &C::debug_hamiltonian)
       .def("download", &C::download)
       .def("run_device", &C::run_device, py::arg("d_a"), py::arg("d_u"), py::arg("stream") = 0)
       .def("apply", &C::apply, py::arg("psi0"))

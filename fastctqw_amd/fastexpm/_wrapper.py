@@ -77,6 +77,20 @@ class _ExpMatBase:
             adj = adj.astype(np.float64, copy=False)
         self._backend.set_graph(adj, float(gamma), bool(laplacian))
 
+    def set_graph_edges(self, src, dst, w=None, gamma: float = 1.0, laplacian: bool = False) -> None:
+        """Walk Hamiltonian H = -gamma*A or -gamma*(D - A) from an undirected
+        edge list (vertex indices src[k], dst[k], optional weights w[k]),
+        built in FP64 on the device."""
+        self._backend.set_graph_edges(np.asarray(src, dtype=np.int64), np.asarray(dst, dtype=np.int64),
+                                      None if w is None else np.asarray(w, dtype=np.float64),
+                                      float(gamma), bool(laplacian))
+
+    def set_graph_er(self, p: float, seed: int, weighted: bool = False, gamma: float = 1.0,
+                     laplacian: bool = False) -> None:
+        """Walk Hamiltonian of the seeded G(N, p) generated on the device: the
+        graph of fastctqw_amd.graphs.erdos_renyi(N, p, seed, weighted), bit for bit."""
+        self._backend.set_graph_er(float(p), int(seed), bool(weighted), float(gamma), bool(laplacian))
+
     def run_graph(self, t: float) -> None:
         """U = expm(-i t H) for the graph of set_graph, resident on the device."""
         self._backend.run_graph(float(t))

@@ -157,6 +157,63 @@ def _sw_attempt(n, d, rng):
     return edges
 
 
+def edge_list(adj_list, num_nodes=None):
+    """(n, src, dst, w) of an undirected graph given as an adjacency list.
+
+    Accepted forms: a networkx graph (vertices indexed in ``G.nodes`` order,
+    as ``nx.to_numpy_array`` does; weights from the 'weight' attribute,
+    default 1); a dict of lists ``{i: [j, ...]}`` or of dicts ``{i: {j: w}}``
+    (networkx's dict-of-lists / dict-of-dicts forms, each edge taken once);
+    or a sequence of pairs ``(i, j)`` / triples ``(i, j, w)`` of integer
+    vertices (a pair listed twice adds its weights).  n is ``num_nodes``, or
+    one more than the largest vertex index.
+    """
+    try:
+        import networkx as nx
+    except ImportError:  # pragma: no cover - networkx is a dependency of the package
+        nx = None
+    if isinstance(adj_list, dict) and nx is not None:
+        vals = list(adj_list.values())
+        if vals and isinstance(vals[0], dict):
+            adj_list = nx.from_dict_of_dicts(adj_list)
+        else:
+            adj_list = nx.from_dict_of_lists(adj_list)
+    if nx is not None and isinstance(adj_list, nx.Graph):
+        if adj_list.is_directed():
+            raise TypeError("graph is a directed graph, but Ctqw recive a undirected graph")
+        index = {v: k for k, v in enumerate(adj_list.nodes)}
+        ed = list(adj_list.edges(data="weight", default=1.0))
+        src = np.fromiter((index[u] for u, _, _ in ed), dtype=np.int64, count=len(ed))
+        dst = np.fromiter((index[v] for _, v, _ in ed), dtype=np.int64, count=len(ed))
+        w = np.fromiter((float(x) for _, _, x in ed), dtype=np.float64, count=len(ed))
+        n = adj_list.number_of_nodes() if num_nodes is None else int(num_nodes)
+        return n, src, dst, w
+    rows = [tuple(e) for e in adj_list]
+    if any(len(r) not in (2, 3) for r in rows):
+        raise ValueError("adjacency list entries must be (i, j) or (i, j, w)")
+    src = np.asarray([r[0] for r in rows], dtype=np.int64)
+    dst = np.asarray([r[1] for r in rows], dtype=np.int64)
+    w = np.asarray([float(r[2]) if len(r) == 3 else 1.0 for r in rows], dtype=np.float64)
+    n = int(max(src.max(initial=-1), dst.max(initial=-1)) + 1) if num_nodes is None else int(num_nodes)
+    if src.size and (min(src.min(), dst.min()) < 0 or max(src.max(), dst.max()) >= n):
+        raise ValueError("vertex index outside [0, n)")
+    return n, src, dst, w
+
+
+def dense_from_edges(n: int, src, dst, w=None, dtype=np.float64) -> np.ndarray:
+    """Dense symmetric adjacency of an undirected edge list: a_ij = a_ji = w,
+    a self-loop on the diagonal once, repeated pairs adding up (the matrix
+    nx.to_numpy_array gives for the graph)."""
+    src = np.asarray(src, dtype=np.int64)
+    dst = np.asarray(dst, dtype=np.int64)
+    w = np.ones(src.shape, dtype=np.float64) if w is None else np.asarray(w, dtype=np.float64)
+    a = np.zeros((n, n), dtype=np.float64)
+    np.add.at(a, (src, dst), w)
+    off = src != dst
+    np.add.at(a, (dst[off], src[off]), w[off])
+    return a.astype(dtype, copy=False)
+
+
 def laplacian(a: np.ndarray) -> np.ndarray:
     """L = D - A with D = diag(row sums) (FastCTQW/Operator.py:51)."""
     return np.diag(a.sum(axis=1)) - a

@@ -177,6 +177,26 @@ int fexpm_download(fexpm_engine* e, void* u);
  * construction in Operator._buildHamiltonian (FastCTQW/Operator.py:49-53). */
 int fexpm_set_graph(fexpm_engine* e, const void* adj, int adj_dtype, double gamma, int laplacian);
 
+/* The walk Hamiltonian from an undirected edge list: vertices src[k], dst[k]
+ * in [0, n), weight w[k] (w NULL: unit weights); A_ij = A_ji = w, a self-loop
+ * on the diagonal once, repeated pairs add up (nx.to_numpy_array of the
+ * graph); then H as in fexpm_set_graph, all in FP64 on the device.  Only
+ * the 24 bytes per edge cross PCIe.  Replaces the stubs
+ * Operator.from_adjacency_list / reset_adjList (FastCTQW/Operator.py:114-116,
+ * 147-148) and Ctqw.from_adjacency_list (FastCTQW/Ctqw.py:60-71), whose
+ * bodies are empty in the reference. */
+int fexpm_set_graph_edges(fexpm_engine* e, int64_t m, const int64_t* src, const int64_t* dst, const double* w,
+                          double gamma, int laplacian);
+
+/* The walk Hamiltonian of the seeded Erdos-Renyi graph G(n, p) generated on
+ * the device: edge {i, j} (i < j) iff u_1(i n + j) < p, weight 1 or (with
+ * weighted) 1 - u_2(i n + j), where u_s(c) = 53 bits of splitmix64(c ^ key_s),
+ * key_s = splitmix64(splitmix64(seed) ^ s), scaled to [0, 1): the same graph
+ * as fastctqw_amd.graphs.erdos_renyi(n, p, seed, weighted) on the host, bit
+ * for bit; no matrix crosses PCIe.  No counterpart in the reference (whose
+ * graphs come from networkx on the host). */
+int fexpm_set_graph_er(fexpm_engine* e, double p, uint64_t seed, int weighted, double gamma, int laplacian);
+
 /* U = expm(-i t H) for the Hamiltonian kept by fexpm_set_graph, resident
  * (download / apply / propagate as after fexpm_run_resident).  Replaces
  * Operator._buildUnitory + buildExpmOperator (FastCTQW/Operator.py:55-56,

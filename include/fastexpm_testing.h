@@ -102,6 +102,10 @@ int fexpm_test_lu_solve(int n, const double* mr, const double* mi, const double*
 
 int fexpm_debug_plane(fexpm_engine* e, int which, double* re, double* im);
 
+/* Copy of the walk Hamiltonian H (n x n, FP64) kept by fexpm_set_graph,
+ * fexpm_set_graph_edges or fexpm_set_graph_er. */
+int fexpm_debug_hamiltonian(fexpm_engine* e, double* h);
+
 /* Average device time (ms) of one GEMM launch over reps launches on
  * device-resident operands (m x k, n x k -> m x n); cfg selects an FP64
  * tiling (0 4-wave 32x32, 1 8-wave 32x16 = default, 2 128x64, 3 64x128;

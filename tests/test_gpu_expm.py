@@ -986,6 +986,117 @@ def test_operator_device_hamiltonian(fx, adt, laplacian):
     op.free()
 
 
+@pytest.mark.parametrize("weighted", [False, True])
+def test_device_er_generator(fx, weighted):
+    """The seeded G(n, p) generated on the device (fexpm_set_graph_er) is
+    graphs.erdos_renyi bit for bit (H = A with gamma = -1); with the
+    Laplacian, H and U equal those of the dense host adjacency sent through
+    fexpm_set_graph, bit for bit."""
+    n, p, seed = 1000, 0.3, 7
+    _say(f"device ER generator n={n} weighted={weighted}")
+    a = erdos_renyi(n, p, seed, weighted=weighted)
+    with fx.ExpMatComplex64(n) as e:
+        e.set_graph_er(p, seed, weighted=weighted, gamma=-1.0, laplacian=False)
+        assert np.array_equal(e._backend.debug_hamiltonian(), a)
+        e.set_graph_er(p, seed, weighted=weighted, gamma=1.0, laplacian=True)
+        h_dev = e._backend.debug_hamiltonian()
+        e.run_graph(2.0)
+        u_dev = e.download()
+        e.set_graph(a, gamma=1.0, laplacian=True)
+        assert np.array_equal(e._backend.debug_hamiltonian(), h_dev)
+        e.run_graph(2.0)
+        assert np.array_equal(e.download(), u_dev)
+    assert np.allclose(h_dev, -laplacian(a), rtol=0, atol=1e-12)
+
+
+def test_edge_list_graph(fx):
+    """An edge list (weighted, one self-loop, one repeated pair) sent
+    through fexpm_set_graph_edges gives the H and, bitwise, the U of its
+    dense adjacency sent through fexpm_set_graph; U against the eigen
+    oracle; vertex indices outside [0, n) are rejected."""
+    from fastctqw_amd.graphs import dense_from_edges
+
+    n = 300
+    rng = np.random.default_rng(21)
+    m = 2500
+    src = rng.integers(0, n, m)
+    dst = rng.integers(0, n, m)
+    src[0], dst[0] = 5, 5
+    src[1], dst[1] = src[2], dst[2] = 7, 9
+    w = rng.random(m) + 0.1
+    a = dense_from_edges(n, src, dst, w)
+    _say(f"edge list graph n={n} m={m}")
+    for dt in (np.complex64, np.complex128):
+        with _engine(fx, dt, n) as e:
+            e.set_graph_edges(src, dst, w, gamma=0.8, laplacian=True)
+            h_e = e._backend.debug_hamiltonian()
+            e.run_graph(1.5)
+            u_e = e.download()
+            e.set_graph(a, gamma=0.8, laplacian=True)
+            assert np.array_equal(e._backend.debug_hamiltonian(), h_e)
+            e.run_graph(1.5)
+            assert np.array_equal(e.download(), u_e)
+            st = e.stats()
+            with pytest.raises(ValueError):
+                e.set_graph_edges([0], [n], None)
+        assert rel_fro(eig_propagator(h_e, 1.5), u_e) <= _tol(dt, st["s"])
+
+
+def test_readme_from_networkx_graph(fx):
+    """The README call (README.md:61-72): Ctqw.from_networkx_graph(G, psi0,
+    device="cuda", dtype=np.complex64) on a random 4-regular graph, runWalk
+    at t = 4.5, the final state against the eigen oracle of the reference's
+    H = -A (uniform and random initial states); the complex64 adjacency with
+    zero imaginary part takes the device graph route."""
+    nx = pytest.importorskip("networkx")
+    from FastCTQW.Ctqw import Ctqw
+    from FastCTQW.State import State
+    from fastctqw_amd.graphs import random_state
+
+    for n in (10, 400):
+        _say(f"README from_networkx_graph n={n}")
+        G = nx.random_regular_graph(d=4, n=n, seed=n)
+        adj = nx.to_numpy_array(G)
+        ref_u = eig_propagator(-adj, 4.5)
+        for init in (State.getUniformSuperposition(n=n), State(random_state(n, seed=3).astype(np.complex64))):
+            walker = Ctqw.from_networkx_graph(G, init, device="cuda", dtype=np.complex64)
+            assert walker.getOperator()._graph_on_device
+            walker.runWalk(time=4.5)
+            psi = walker.getFinalState().getState()
+            assert psi.dtype == np.complex64
+            inf = infidelity(ref_u @ init.getState().astype(np.complex128), psi)
+            print(f"  infidelity {inf:.2e}", flush=True)
+            assert inf <= 1e-9
+            walker.getOperator().free()
+
+
+def test_from_adjacency_list_and_er_on_device(fx):
+    """Ctqw.from_adjacency_list and Ctqw.from_erdos_renyi with
+    device="cuda" against the same walks on the CPU route (scipy)."""
+    nx = pytest.importorskip("networkx")
+    from fastctqw_amd.Ctqw import Ctqw
+    from fastctqw_amd.State import State
+
+    G = nx.random_regular_graph(5, 120, seed=8)
+    psi0 = State.getUniformSuperposition(120)
+    _say("from_adjacency_list / from_erdos_renyi on the device")
+    for lap in (False, True):
+        wd = Ctqw.from_adjacency_list(list(G.edges), psi0, laplacian=lap, device="cuda")
+        wc = Ctqw.from_adjacency_list(list(G.edges), psi0, laplacian=lap, device="cpu")
+        wd.runWalk(time=3.0)
+        wc.runWalk(time=3.0)
+        assert infidelity(wc.getFinalState().getState(), wd.getFinalState().getState()) <= 1e-9
+        assert np.allclose(wd.getOperator()._hamiltonian, wc.getOperator()._hamiltonian)
+        wd.getOperator().free()
+    psi1 = State.getSingleNodeState(200, 3)
+    wd = Ctqw.from_erdos_renyi(200, 0.4, 12, psi1, weighted=True, laplacian=True, device="cuda")
+    wc = Ctqw.from_erdos_renyi(200, 0.4, 12, psi1, weighted=True, laplacian=True, device="cpu")
+    wd.runWalk(time=2.0)
+    wc.runWalk(time=2.0)
+    assert infidelity(wc.getFinalState().getState(), wd.getFinalState().getState()) <= 1e-9
+    wd.getOperator().free()
+
+
 def test_device_graph_errors(fx):
     _say("device graph errors")
     with fx.ExpMatComplex64(8) as e:

@@ -248,3 +248,62 @@ def test_product_does_not_import_oracle():
             "import FastCTQW; print(any(m == 'oracle' or m.startswith('oracle.') for m in sys.modules))")
     out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, cwd=ROOT)
     assert out.stdout.strip() == "False", out.stderr
+
+
+def test_edge_list_forms_match_networkx():
+    """Adjacency-list inputs (networkx graph with weights and a self-loop,
+    dict of lists, dict of dicts, pairs and triples) give the dense matrix of
+    nx.to_numpy_array; repeated pairs in a sequence add up."""
+    nx = pytest.importorskip("networkx")
+    from fastctqw_amd.graphs import dense_from_edges, edge_list
+
+    G = nx.random_regular_graph(4, 20, seed=2)
+    for k, (u, v) in enumerate(G.edges):
+        G[u][v]["weight"] = 0.25 + 0.01 * k
+    G.add_edge(5, 5, weight=3.0)
+    n, s, d, w = edge_list(G)
+    assert np.array_equal(dense_from_edges(n, s, d, w), nx.to_numpy_array(G))
+    dl = nx.to_dict_of_lists(G)
+    n, s, d, w = edge_list(dl)
+    assert np.array_equal(dense_from_edges(n, s, d, w), nx.to_numpy_array(nx.from_dict_of_lists(dl)))
+    dd = {0: {1: {"weight": 2.0}}, 1: {2: {"weight": 0.5}}}
+    n, s, d, w = edge_list(dd)
+    assert np.array_equal(dense_from_edges(n, s, d, w), nx.to_numpy_array(nx.from_dict_of_dicts(dd)))
+    n, s, d, w = edge_list([(0, 1), (1, 2, 0.5), (0, 1)], num_nodes=4)
+    a = dense_from_edges(n, s, d, w)
+    assert n == 4 and a[0, 1] == a[1, 0] == 2.0 and a[1, 2] == a[2, 1] == 0.5 and a.sum() == 5.0
+    with pytest.raises(ValueError):
+        edge_list([(0, 3)], num_nodes=3)
+    with pytest.raises(TypeError):
+        edge_list(nx.DiGraph([(0, 1)]))
+
+
+def test_adjacency_list_and_er_cpu_routes():
+    """Operator/Ctqw.from_adjacency_list and from_erdos_renyi on the CPU give
+    the walks of the dense adjacency (scipy path)."""
+    nx = pytest.importorskip("networkx")
+    from fastctqw_amd.Ctqw import Ctqw
+    from fastctqw_amd.Operator import Operator
+    from fastctqw_amd.State import State
+
+    G = nx.random_regular_graph(3, 16, seed=4)
+    A = nx.to_numpy_array(G, dtype=np.float32)
+    psi0 = State.getSingleNodeState(16, 0)
+    w1 = Ctqw.from_adjacency_list(list(G.edges), psi0, laplacian=True)
+    w2 = Ctqw.from_numpy_array(A, psi0, laplacian=True)
+    w1.runWalk(time=2.0)
+    w2.runWalk(time=2.0)
+    assert np.array_equal(w1.getFinalState().getState(), w2.getFinalState().getState())
+    op = Operator.from_adjacency_list(G, gamma=0.5, dtype=np.float64)
+    assert op._dtype == np.complex128
+    op.buildExpmOperator(1.5)
+    ref = Operator.from_numpy_array(nx.to_numpy_array(G), gamma=0.5)
+    ref.buildExpmOperator(1.5)
+    assert np.array_equal(op.getOperator(), ref.getOperator())
+    op.reset_adjList([(0, 1), (1, 2)], gamma=1.0, time=None, laplacian=False)
+    assert op._n == 3
+    op.buildExpmOperator(1.0)
+    assert op.getOperator().shape == (3, 3)
+    er = Operator.from_erdos_renyi(30, 0.4, 9, weighted=True, laplacian=True)
+    dense = Operator.from_numpy_array(erdos_renyi(30, 0.4, 9, weighted=True).astype(np.float32), laplacian=True)
+    assert np.array_equal(er._hamiltonian, dense._hamiltonian)
