@@ -818,6 +818,16 @@ def _host_info(cores):
             "scipy": scipy.__version__, "blas": blas}
 
 
+def _slice_phrase(st):
+    """How the FP64 stages of a complex64 run computed their symmetric products."""
+    k, scope = st.get("fp64_slices", 0), st.get("fp64_slices_scope", 0)
+    if not k:
+        return "fp64 pade and solve)"
+    where = {1: "pade products", 2: "solve products", 3: "pade and solve products"}.get(scope, "products")
+    rest = {1: ", fp64 cholesky solve", 2: ", fp64 pade", 3: ", fp64 cholesky factorisation"}.get(scope, "")
+    return f"fp64-accurate {where} from {k} exact int8 slices (int32 accumulation){rest})"
+
+
 def _infidelity(a, b):
     num = abs(np.vdot(a, b)) ** 2
     return float(max(0.0, 1.0 - num / (np.vdot(a, a).real * np.vdot(b, b).real)))
@@ -863,7 +873,7 @@ def main():
     ap.add_argument("--fp64-slices", type=int, default=None,
                     help="c2: FP64 Pade products from this many exact int8 slices (4..6) on the int8 MFMA; "
                          "0 = FP64 MFMA (default: the engine's, 6)")
-    ap.add_argument("--fp64-slices-scope", type=int, default=3, choices=[1, 2, 3],
+    ap.add_argument("--fp64-slices-scope", type=int, default=0, choices=[0, 1, 2, 3],
                     help=argparse.SUPPRESS)
     ap.add_argument("--kblock", type=int, default=None,
                     help="c2: accumulation block of the squarings in 16-k tiles (0 = one block)")
@@ -1049,8 +1059,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": (f"c64 ({'fp32 mfma' if args.precision == 'native' else args.precision} squarings, "
-                  f"{'pade products from %d exact int8 slices (int32 accumulation), ' % st['fp64_slices'] if st.get('fp64_slices') else 'fp64 pade, '}"
-                  f"fp64 solve)" if cdt == np.complex64 else "c128 (fp64)"),
+                  f"{_slice_phrase(st)}" if cdt == np.complex64 else "c128 (fp64)"),
         "data": "synthetic: seeded ER(N,p) Laplacian, A=-i*t*L",
         "config": {"workload": f"expm(-i*{args.t}*L), L Laplacian of ER({n},{args.p}) seed {args.seed}, "
                                f"{args.dtype}, full U resident", "n": n, "t": args.t,

@@ -221,7 +221,8 @@ struct PipelineBase {
   int spd_solve = 1;       // structured inputs: Cholesky solve (0: pivoted LU as for general A)
   int precision = 0;       // enum fexpm_precision for the structured squarings of FP32 engines
   int oz_slices = 0;       // FP64 companion: symmetric FP64 products from int8 slices (0 = FP64 MFMA)
-  int oz_scope = 3;        // stages using them: bit 0 the Pade products, bit 1 the solve (S, X = Z Z^T)
+  int oz_scope = 3;        // stages using them: bit 0 the Pade products, bit 1 the solve (S, X = Z Z^T);
+                           // 0 = automatic (slice_scope)
   int tail_split = 1;      // FP32 squarings: last partial wave of 128-tiles as 64-tiles
   int z_overlap = 1;       // Cholesky solve: Z updates on a second stream
   int ch_outer = 512;      // Cholesky solve: outer block width (a multiple of 64)
@@ -454,6 +455,7 @@ template <typename W> struct Pipeline : PipelineBase {
   double2* rsum = nullptr;    // double-double row sums of Im(A)
   double* npart = nullptr;    // 1-norm partial column sums (spectral bounds)
   double* d_norms = nullptr;  // ||B^k||_1, k = 1, 2, 4, 6; trace B^4, B^6, ..., B^12
+  double* ncmax = nullptr;    // per-workgroup column-sum maxima of the 1-norms
   double2* dsq[2] = {nullptr, nullptr};  // split-form diagonals (squarings)
   int* flags = nullptr;
   Scalars* scal = nullptr;
@@ -474,10 +476,11 @@ template <typename W> struct Pipeline : PipelineBase {
   Pipeline(int n_, int dtype_, bool with_events = true) : n(n_), dtype(dtype_), ncap(n_) {
     in_arena = g_arena != nullptr;
     // FP32 engines: the Pade products of their FP64 stages from 6 int8
-    // slices (same accuracy as FP64 MFMA, measured), the solve in FP64 MFMA
+    // slices (same accuracy as FP64 MFMA, measured); the solve's products
+    // too when the plan deflates (automatic scope, see slice_scope)
     if (std::is_same<W, float>::value) {
       oz_slices = 6;
-      oz_scope = 1;
+      oz_scope = 0;
     }
     npcap = round_up(std::max(ncap, 1), 128);
     np = npcap;
@@ -538,6 +541,7 @@ template <typename W> struct Pipeline : PipelineBase {
     dfree(rsum);
     dfree(npart);
     dfree(d_norms);
+    dfree(ncmax);
     dfree(off);
     dfree(diag);
     dfree(dsq[0]);
@@ -879,6 +883,7 @@ template <typename W> struct Pipeline : PipelineBase {
         npart = dalloc<double>(std::max((size_t)std::max(1, (ncap + kRows - 1) / kRows) * std::max(ncap, 1),
                                         (size_t)5 * std::max(ncap, 1)));
       if (!d_norms) d_norms = dalloc<double>(kSpecNorms);
+      if (!ncmax) ncmax = dalloc<double>((size_t)(ncap + 255) / 256 + 1);
       dim3 grid((n + 255) / 256, n);
       prepare_planar<Tin, W><<<grid, 256, 0, st>>>(d_a, n, a.r, a.i, ld, 0.0, mu_im, 1.0, 1, alpha);
       FX_LAUNCH();
@@ -887,7 +892,10 @@ template <typename W> struct Pipeline : PipelineBase {
       for (int k = 0; k < 4; ++k) {
         norm1_part<<<dim3((unsigned)((n + 255) / 256), (unsigned)nch), 256, 0, st>>>(pw[k]->r, ld, n, kRows, npart);
         FX_LAUNCH();
-        norm1_final<<<1, 1024, 0, st>>>(npart, nch, n, d_norms + k);
+        const int ncb = (n + 255) / 256;
+        norm1_cols<<<(unsigned)ncb, 256, 0, st>>>(npart, nch, n, ncmax);
+        FX_LAUNCH();
+        norm1_final<<<1, 256, 0, st>>>(ncmax, ncb, d_norms + k);
         FX_LAUNCH();
       }
       trace_dots_part<<<(unsigned)n, 256, 0, st>>>(p2.r, p4.r, p6.r, ld, n, npart);
@@ -1412,6 +1420,22 @@ template <typename W> struct Pipeline : PipelineBase {
     return P;
   }
 
+  // Stages whose symmetric FP64 products come from int8 slices (bit 0 the
+  // Pade products, bit 1 the Cholesky solve's S and X = Z Z^T).  Automatic
+  // (oz_scope = 0): the Pade products always; the solve's only for deflated
+  // plans of order n >= 4096.  The slices keep about 41 bits of each row's
+  // largest value; a Pade term with a dominant rank-one part (the outlier
+  // eigenvalue of a Laplacian that is not deflated) loses its small entries
+  // to that, which the factorisation amplifies (N = 10^4 weighted ER(0.5)
+  // Laplacian, t = 4.5: infidelity 1.5e-11 -> 6.1e-9), while deflated plans
+  // keep the FP64 accuracy (t = 4.5: 1.78e-11 -> 1.77e-11, t = 20:
+  // 7.7e-11 -> 8.0e-11) and the solve takes 121 -> 87 ms.  Below n = 4096
+  // the FP64 MFMA products are the faster ones.
+  int slice_scope(const Plan& P) const {
+    if (oz_scope != 0) return oz_scope;
+    return 1 | ((P.deflate && n >= 4096) ? 2 : 0);
+  }
+
   // Fields that follow from s.
   void finish_plan(Plan& P) const {
     P.s_do = max_squarings >= 0 ? std::min(P.s, max_squarings) : P.s;
@@ -1430,7 +1454,7 @@ template <typename W> struct Pipeline : PipelineBase {
   void spectral_plan(const Tin* d_a, Plan& P, hipStream_t st) {
     Pipeline<double>* h = fp64_pipe();
     h->oz_slices = oz_slices;
-    h->oz_scope = oz_scope;
+    h->oz_scope = slice_scope(P);
     P.rho = h->template spectral_front<Tin>(d_a, P.mu_im, P.alpha, st);
     P.s = scaling_for(P.rho);
     finish_plan(P);
@@ -1461,7 +1485,7 @@ template <typename W> struct Pipeline : PipelineBase {
       h->panel_coop = panel_coop;
       h->spd_solve = spd_solve;
       h->oz_slices = oz_slices;
-      h->oz_scope = oz_scope;
+      h->oz_scope = slice_scope(P);
       h->z_overlap = z_overlap;
       h->ch_outer = ch_outer;
       h->z_left = z_left;
@@ -1533,6 +1557,7 @@ template <typename W> struct Pipeline : PipelineBase {
       npart = dalloc<double>(std::max((size_t)std::max(1, (ncap + kRows - 1) / kRows) * std::max(ncap, 1),
                                       (size_t)5 * std::max(ncap, 1)));
       d_norms = dalloc<double>(kSpecNorms);
+      ncmax = dalloc<double>((size_t)(ncap + 255) / 256 + 1);
     }
   }
 
@@ -1552,7 +1577,7 @@ template <typename W> struct Pipeline : PipelineBase {
     S.kblock = kblock;
     S.precision = P.split16 ? precision : FEXPM_PREC_NATIVE;
     S.fp64_slices = (P.hybrid && P.structured && !P.nonfinite) ? oz_slices : 0;
-    S.fp64_slices_scope = S.fp64_slices ? oz_scope : 0;
+    S.fp64_slices_scope = S.fp64_slices ? slice_scope(P) : 0;
     S.spectral = P.spectral ? 1 : 0;
     S.deflated = P.deflate ? 1 : 0;
     S.s_onenorm = P.nonfinite ? 0 : P.s_onenorm;
@@ -2524,8 +2549,8 @@ int fexpm_set_option(fexpm_engine* e, int key, int64_t value) {
       if (e->pipe) e->pipe->ch_outer = (int)value;
       return FEXPM_OK;
     case FEXPM_OPT_FP64_SLICES_SCOPE:
-      if (value < 1 || value > 3) {
-        e->err = "scope must be 1 (Pade), 2 (solve) or 3 (both)";
+      if (value < 0 || value > 3) {
+        e->err = "scope must be 0 (automatic), 1 (Pade), 2 (solve) or 3 (both)";
         return FEXPM_EINVAL;
       }
       if (e->pipe) e->pipe->oz_scope = (int)value;

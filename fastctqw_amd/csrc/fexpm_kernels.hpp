@@ -502,9 +502,10 @@ __global__ void add_constant(W* __restrict__ wr, W* __restrict__ wi, int64_t ldw
   wi[o] = (W)((double)wi[o] + ci);
 }
 
-// 1-norm of a real n x n FP64 plane in two deterministic steps: partial
+// 1-norm of a real n x n FP64 plane in three deterministic steps: partial
 // column sums of |x_ij| over row chunks of `rows` rows (part[chunk * n + j]),
-// then one workgroup sums the chunks per column and takes the maximum.
+// the chunks summed per column with the maximum of each 256 columns, and one
+// workgroup's maximum of those.
 __global__ __launch_bounds__(256) void norm1_part(const double* __restrict__ x, int64_t ld, int n, int rows,
                                                   double* __restrict__ part) {
   const int j = blockIdx.x * 256 + threadIdx.x;
@@ -516,19 +517,40 @@ __global__ __launch_bounds__(256) void norm1_part(const double* __restrict__ x, 
   part[(int64_t)c * n + j] = s;
 }
 
-__global__ __launch_bounds__(1024) void norm1_final(const double* __restrict__ part, int nchunks, int n,
-                                                    double* __restrict__ out) {
-  __shared__ double sm[1024];
+// Column sums over the chunks (one thread per column), the maximum of each
+// workgroup's 256 columns into cmax[blockIdx.x] (NaN propagates).
+__global__ __launch_bounds__(256) void norm1_cols(const double* __restrict__ part, int nchunks, int n,
+                                                  double* __restrict__ cmax) {
+  __shared__ double sm[256];
+  const int j = blockIdx.x * 256 + threadIdx.x;
   double m = 0.0;
-  for (int j = threadIdx.x; j < n; j += 1024) {
-    double s = 0.0;
-    for (int c = 0; c < nchunks; ++c) s += part[(int64_t)c * n + j];
-    m = (s != s) ? s : fmax(m, s);
+  if (j < n) {
+    for (int c = 0; c < nchunks; ++c) m += part[(int64_t)c * n + j];
   }
   sm[threadIdx.x] = m;
   __syncthreads();
-  for (int w = 512; w > 0; w >>= 1) {
-    if (threadIdx.x < w) {
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      const double y = sm[threadIdx.x + w];
+      sm[threadIdx.x] = (y != y) ? y : fmax(sm[threadIdx.x], y);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) cmax[blockIdx.x] = sm[0];
+}
+
+__global__ __launch_bounds__(256) void norm1_final(const double* __restrict__ cmax, int nblocks,
+                                                   double* __restrict__ out) {
+  __shared__ double sm[256];
+  double m = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += 256) {
+    const double y = cmax[b];
+    m = (y != y) ? y : fmax(m, y);
+  }
+  sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
       const double y = sm[threadIdx.x + w];
       sm[threadIdx.x] = (y != y) ? y : fmax(sm[threadIdx.x], y);
     }

@@ -31,7 +31,9 @@ enum { FEXPM_OPT_GROUP_CANDIDATE = 101 };
 
 /* Engine option: stages whose symmetric FP64 products use the int8 slices
  * of FEXPM_OPT_FP64_SLICES: 1 the Pade products, 2 the Cholesky solve
- * (S, X = Z Z^T), 3 both (default).  For precision studies. */
+ * (S, X = Z Z^T), 3 both, 0 automatic (default for complex64 engines: the
+ * Pade products, and the solve's too for plans that deflate the eigenvector
+ * 1 at n >= 4096).  For precision studies. */
 enum { FEXPM_OPT_FP64_SLICES_SCOPE = 102 };
 
 /* Engine option: 1 (default) runs the last, partly filled wave of 128 x 128
