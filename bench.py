@@ -67,22 +67,29 @@ def _sustained_peak(key):
     return None
 
 
-def squaring_roofline(n, sq_ms, issued, peak, split_parts=0):
-    """Roofline fields of one structured squaring X -> X^2 of order n
-    (complex symmetric X = Xr + i Xi).  The algorithm's own flops per
-    launch are those of three real symmetric products (3M rule, half of
-    each n x n x n product by symmetry): 3 n^3, times the part products of
-    a split-precision form (split_parts).  ``issued`` is what the kernel's
-    MFMAs execute (whole lower tiles, padded k), never less than that, so
-    frac <= issued_frac.  The 4M convention (8 n^3 per complex product) is
-    kept under its own key."""
-    alg = 3.0 * float(n) ** 3 * (split_parts if split_parts else 1)
-    t = sq_ms * 1e-3
+def squaring_roofline(n, ms_square, s, two, issued_3m, peak, split_parts=0):
+    """Roofline fields of the s structured squarings X -> X^2 of order n
+    (complex symmetric X = Xr + i Xi) that took ms_square in all.  The
+    algorithm's own flops are those of three real symmetric products per
+    squaring (3M rule, half of each n x n x n product by symmetry), 3 n^3,
+    and 2 n^3 for the ``two`` leading squarings in the two-product form,
+    times the part products of a split-precision form (split_parts).
+    ``issued_3m`` is what one 3M squaring's MFMAs execute (whole lower tiles,
+    padded k; the two-product form issues 2/3 of it), never less than the
+    algorithmic count, so frac <= issued_frac.  The 4M convention (8 n^3 per
+    complex product) is kept under its own key."""
+    s = max(int(s), 1)
+    two = min(max(int(two), 0), s)
+    parts = split_parts if split_parts else 1
+    alg = (3.0 * (s - two) + 2.0 * two) * float(n) ** 3 * parts
+    issued = issued_3m * ((s - two) + two * 2.0 / 3.0)
+    t = ms_square * 1e-3
     return {
         "achieved": alg / t / 1e12, "peak": peak, "unit": "TFLOP/s", "frac": alg / t / 1e12 / peak,
-        "algorithmic_flops_per_launch": alg, "issued_flops_per_launch": issued,
+        "algorithmic_flops_per_launch": alg / s, "issued_flops_per_launch": issued / s,
         "issued_tflops": issued / t / 1e12, "issued_frac": issued / t / 1e12 / peak,
-        "achieved_4m_convention": 8.0 * float(n) ** 3 / t / 1e12,
+        "achieved_4m_convention": 8.0 * float(n) ** 3 * s / t / 1e12,
+        "two_product_launches": two,
     }
 
 
@@ -1009,8 +1016,8 @@ def main():
                  "N=6144 lower 64x64 tiles of 4 waves, above it 128x128 tiles of 16 waves plus a 64x64 launch "
                  "for a last partial wave of tiles; 8x8 tile groups; each wave 32x32)")
         traffic_key = "fp32_squaring" if cdt == np.complex64 else None
-    roof = squaring_roofline(n, sq_ms, issued, peak,
-                             split_parts=(3 if args.precision == "fp16x2" else 6) if split else 0)
+    roof = squaring_roofline(n, float(np.mean([p["ms_square"] for p in phase])), s, st.get("s_two_product", 0),
+                             issued, peak, split_parts=(3 if args.precision == "fp16x2" else 6) if split else 0)
     issued_tf = roof["issued_tflops"]
     traffic = _profiled_traffic(traffic_key) if n == 10000 else None
     sustained = _sustained_peak("fp16" if split and args.precision == "fp16x2" else
@@ -1072,12 +1079,13 @@ def main():
                      "avg_launch_ms": sq_ms, "launches_per_step": s,
                      "sustained_peak": sustained, "sustained_source": MFMA_PROFILE if sustained else None,
                      "issued_frac_of_sustained": issued_tf / sustained if sustained else None,
-                     "note": "achieved = 3 N^3 flops per squaring (the structured algorithm's three real "
-                             "symmetric half products) / avg launch time from HIP events on the squaring "
-                             "stream; issued = MFMA flops the kernel executes (whole lower tiles; equal to "
-                             "the profiler's SQ_INSTS_VALU_MFMA_MOPS x 512); achieved_4m_convention = "
-                             "8 N^3 / launch, the general complex convention; traffic = HBM bytes per launch "
-                             "from FETCH_SIZE / WRITE_SIZE scaled by a copy of known size"},
+                     "note": "achieved = the structured algorithm's flops of the s squarings (3 N^3 per 3M "
+                             "squaring, 2 N^3 per two-product squaring: real symmetric half products) / "
+                             "their device time from HIP events on the squaring stream; issued = MFMA flops "
+                             "the kernels execute (whole lower tiles; equal to the profiler's "
+                             "SQ_INSTS_VALU_MFMA_MOPS x 512); achieved_4m_convention = 8 N^3 per squaring, "
+                             "the general complex convention; traffic = HBM bytes per 3M launch from "
+                             "FETCH_SIZE / WRITE_SIZE scaled by a copy of known size"},
         "accuracy": acc,
     }
     if cdt == np.complex64 and args.precision == "native" and not args.no_split:

@@ -228,6 +228,7 @@ struct PipelineBase {
   int ch_outer = 512;      // Cholesky solve: outer block width (a multiple of 64)
   int sq_tile = 0;         // FP32 structured squarings: 0 auto, 64 or 128 (tile edge)
   int z_left = 0;          // Cholesky solve: left-looking Z updates (one long-K product per outer block)
+  int sq2_lead = 0;        // structured squarings: leading squarings in the two-product form
   int spectral = 1;        // structured inputs: s from a spectral-radius bound (0: the 1-norm rule)
   int deflate = 1;         // spectral scaling: deflate an exact eigenvector 1 (equal row sums)
 };
@@ -709,7 +710,7 @@ template <typename W> struct Pipeline : PipelineBase {
       tiles = (double)grid.x * grid.y;
     }
     FX_LAUNCH();
-    const double mult = MODE == MODE_REAL ? 1.0 : (MODE == MODE_C4M ? 4.0 : 3.0);
+    const double mult = MODE == MODE_REAL ? 1.0 : (MODE == MODE_C4M ? 4.0 : (MODE == MODE_SQ2 ? 2.0 : 3.0));
     flops += mult * 2.0 * tiles * C::BM * C::BN * (double)g.k;
   }
 
@@ -1267,27 +1268,31 @@ template <typename W> struct Pipeline : PipelineBase {
   // leaves the last wave of workgroups less than half full, the tiles of
   // the full waves run in 128 x 128 and the rest as 64 x 64 sub-tiles
   // (gemm_sym_tail_kernel), which spread over the whole device.
+  template <int MODE>
   void square_big_tail(const GemmArgs<W>& g, hipStream_t st) {
     if constexpr (std::is_same<W, float>::value) {
       using C = FCfg16w;
       const SquarePlan pl = square_plan(g.m, 128, tail_split != 0);
       if (pl.rest > 0) {
-        gemm_nt_kernel<float, MODE_C3M, true, true, C><<<dim3(pl.full), dim3(C::NT), 0, st>>>(g);
+        gemm_nt_kernel<float, MODE, true, true, C><<<dim3(pl.full), dim3(C::NT), 0, st>>>(g);
         FX_LAUNCH();
-        gemm_sym_tail_kernel<float, MODE_C3M, true, FCfgSmall><<<dim3(4 * pl.rest), dim3(FCfgSmall::NT), 0, st>>>(
+        gemm_sym_tail_kernel<float, MODE, true, FCfgSmall><<<dim3(4 * pl.rest), dim3(FCfgSmall::NT), 0, st>>>(
             g, pl.full);
         FX_LAUNCH();
-        flops += 3.0 * 2.0 * ((double)pl.full * C::BM * C::BN + (double)pl.sub * FCfgSmall::BM * FCfgSmall::BN) *
-                 (double)g.k;
+        const double mult = MODE == MODE_SQ2 ? 2.0 : 3.0;
+        flops += mult * 2.0 *
+                 ((double)pl.full * C::BM * C::BN + (double)pl.sub * FCfgSmall::BM * FCfgSmall::BN) * (double)g.k;
         return;
       }
-      gemm<MODE_C3M, true, true, C>(g, st);
+      gemm<MODE, true, true, C>(g, st);
     } else {
-      gemm<MODE_C3M, true, true>(g, st);
+      gemm<MODE, true, true>(g, st);
     }
   }
 
-  std::pair<Plane2<W>*, int> square_split(int s, bool structured, int dc, hipStream_t st) {
+  // s squarings of the split form; with structured input the first `two`
+  // of them in the two-product form (MODE_SQ2).
+  std::pair<Plane2<W>*, int> square_split(int s, bool structured, int dc, hipStream_t st, int two = 0) {
     if (std::is_same<W, float>::value && structured && precision == FEXPM_PREC_FP16X2)
       return square_split16<Fp16x2>(s, dc, st);
     if (std::is_same<W, float>::value && structured && precision == FEXPM_PREC_BF16X3)
@@ -1295,6 +1300,7 @@ template <typename W> struct Pipeline : PipelineBase {
     Plane2<W>* cur = &x0;
     Plane2<W>* nxt = &x1;
     for (int q = 0; q < s; ++q) {
+      const bool sq2 = structured && q < two;
       GemmArgs<W> g;
       if (structured) {
         // k over round_up(n, 16): the padding columns of X are zero, so the
@@ -1312,9 +1318,13 @@ template <typename W> struct Pipeline : PipelineBase {
       const bool small = std::is_same<W, float>::value && square_tile(n, sq_tile) == 64;
       // (FCfgSmall runs 32x32x2 FP32 MFMAs; FP64 engines never take this branch)
       using SmallC = typename std::conditional<std::is_same<W, float>::value, FCfgSmall, GemmCfg<W>>::type;
-      if (structured) {
+      if (structured && sq2) {
+        if (blk && small) gemm<MODE_SQ2, true, true, SmallC>(g, st);
+        else if (blk) square_big_tail<MODE_SQ2>(g, st);
+        else gemm<MODE_SQ2, true>(g, st);
+      } else if (structured) {
         if (blk && small) gemm<MODE_C3M, true, true, SmallC>(g, st);
-        else if (blk) square_big_tail(g, st);
+        else if (blk) square_big_tail<MODE_C3M>(g, st);
         else gemm<MODE_C3M, true>(g, st);
       } else {
         if (blk && small) gemm<MODE_C3M, false, true, SmallC>(g, st);
@@ -1327,8 +1337,8 @@ template <typename W> struct Pipeline : PipelineBase {
     return {cur, dc};
   }
 
-  Plane2<W>* square(int s, bool structured, int dc, hipStream_t st) {
-    auto r = square_split(s, structured, dc, st);
+  Plane2<W>* square(int s, bool structured, int dc, hipStream_t st, int two = 0) {
+    auto r = square_split(s, structured, dc, st, two);
     merge_diagonal<W><<<(n + 255) / 256, 256, 0, st>>>(r.first->r, r.first->i, ld, n, dsq[r.second]);
     FX_LAUNCH();
     return r.first;
@@ -1379,6 +1389,7 @@ template <typename W> struct Pipeline : PipelineBase {
   struct Plan {
     bool nonfinite, structured, shifted, hybrid, split16;
     bool spectral;  // s from the spectral-radius bound (powers formed before the scaling)
+    int two;        // leading squarings in the two-product form
     bool deflate;   // exact eigenvector 1 deflated (all row sums of Im(A) equal)
     double mu_re, mu_im, norm, norm_plain;
     double alpha;   // deflation: B = G - mu I + alpha J
@@ -1440,6 +1451,11 @@ template <typename W> struct Pipeline : PipelineBase {
   void finish_plan(Plan& P) const {
     P.s_do = max_squarings >= 0 ? std::min(P.s, max_squarings) : P.s;
     P.q64 = (std::is_same<W, float>::value && (P.hybrid || P.structured)) ? std::min(P.s_do, fp64_squarings) : 0;
+    // two-product squarings: structured, not split precision, never the last
+    // squaring of the plan (s counts the full plan, so max_squarings cuts
+    // never change which squarings use it), and none of the FP64 leading ones
+    P.two = (P.structured && !P.split16 && P.q64 == 0) ? std::max(0, std::min(sq2_lead, P.s - 1)) : 0;
+    P.two = std::min(P.two, P.s_do);
   }
 
   // The FP64 pipeline of the Pade stage: this one, or the companion of an FP32 engine.
@@ -1510,7 +1526,7 @@ template <typename W> struct Pipeline : PipelineBase {
       dc = solve(P.structured, P.q64, st);
     }
     if (timing) FX_HIP(hipEventRecord(ev[3], st));
-    Plane2<W>* X = square(P.s_do - P.q64, P.structured, dc, st);
+    Plane2<W>* X = square(P.s_do - P.q64, P.structured, dc, st, P.two);
     if (P.deflate) {
       // exp(A) = e^{i mu} (X + delta J), delta = (e^{i(c - mu)} - 1) / n
       const double th = P.c - P.mu_im, sh = std::sin(0.5 * th);
@@ -1582,6 +1598,7 @@ template <typename W> struct Pipeline : PipelineBase {
     S.deflated = P.deflate ? 1 : 0;
     S.s_onenorm = P.nonfinite ? 0 : P.s_onenorm;
     S.rho_bound = P.spectral ? P.rho : 0.0;
+    S.s_two_product = P.nonfinite ? 0 : P.two;
   }
 
   static void check_info(int info) {
@@ -2566,6 +2583,13 @@ int fexpm_set_option(fexpm_engine* e, int key, int64_t value) {
       }
       if (e->pipe) e->pipe->oz_slices = (int)value;
       return FEXPM_OK;
+    case FEXPM_OPT_TWO_PRODUCT:
+      if (value < 0 || value > 64) {
+        e->err = "two-product squarings must be in [0, 64]";
+        return FEXPM_EINVAL;
+      }
+      if (e->pipe) e->pipe->sq2_lead = (int)value;
+      return FEXPM_OK;
     case FEXPM_OPT_SPECTRAL:
       if (e->pipe) e->pipe->spectral = value ? 1 : 0;
       return FEXPM_OK;
@@ -3205,6 +3229,7 @@ void test_gemm_w(int mode, int sym, int m, int n, int k, const double* ar, const
   else if (mode == MODE_C4M && !sym) test_gemm_t<W, MODE_C4M, false>(m, n, k, ar, ai, br, bi, cr, ci);
   else if (mode == MODE_C3M && !sym) test_gemm_t<W, MODE_C3M, false>(m, n, k, ar, ai, br, bi, cr, ci);
   else if (mode == MODE_C3M) test_gemm_t<W, MODE_C3M, true>(m, n, k, ar, ai, br, bi, cr, ci);
+  else if (mode == MODE_SQ2 && sym) test_gemm_t<W, MODE_SQ2, true>(m, n, k, ar, ai, br, bi, cr, ci);
   else throw Error(FEXPM_EINVAL, "unsupported gemm variant");
 }
 

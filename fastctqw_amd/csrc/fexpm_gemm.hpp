@@ -66,7 +66,15 @@ __device__ __forceinline__ f32x16 mfma32(float a, float b, f32x16 c) {
 }
 __device__ __forceinline__ int acc_row32(int lane, int r) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
 
-enum GemmMode { MODE_REAL = 0, MODE_C4M = 1, MODE_C3M = 2 };
+// MODE_SQ2 (structured squarings only, with SYM and the diagonal split):
+// X^2 of X = Xr + i Xi whose symmetric parts commute, from two products,
+//   Re = (Xr + Xi)(Xr - Xi)^T,  Im = 2 Xr Xi^T
+// (equal to Xr^2 - Xi^2 and Xr Xi + Xi Xr when Xr Xi = Xi Xr), two thirds
+// of the 3M flops.  Rounding errors that do not commute are not damped by
+// it, so at most the first squarings may use it (an engine option, off by
+// default: with two MFMAs per k-step the loop is bound by its operand loads
+// and saves only about 7% of a squaring's time).
+enum GemmMode { MODE_REAL = 0, MODE_C4M = 1, MODE_C3M = 2, MODE_SQ2 = 3 };
 
 // Row (within a 16x16 block) of accumulator element r of a lane; the column
 // is lane % 16.  The FP32 16x16x4 MFMA keeps four consecutive rows per lane,
@@ -285,7 +293,7 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs<T>& g, int tm, int tn, 
   constexpr int NS = 4 * NRD;            // MFMA steps per k-tile
   constexpr int NEL = MF == 32 ? 16 : 4;  // accumulator elements per lane
   constexpr int NPL = (MODE == MODE_REAL) ? 1 : 2;
-  constexpr int NACC = (MODE == MODE_REAL) ? 1 : (MODE == MODE_C4M ? 2 : 3);
+  constexpr int NACC = (MODE == MODE_REAL) ? 1 : (MODE == MODE_C3M ? 3 : 2);
   constexpr int TA = BM * (BK + PAD), TB = BN * (BK + PAD);
   using Acc = typename std::conditional<MF == 32, f32x16, typename AccT<T>::type>::type;
 
@@ -307,7 +315,7 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs<T>& g, int tm, int tn, 
     for (int i = 0; i < FI; ++i)
 #pragma unroll
       for (int j = 0; j < FJ; ++j) acc[a][i][j] = Acc{};
-  static_assert(!KBLK || MODE == MODE_C3M, "blocked accumulation is for the 3M kernel");
+  static_assert(!KBLK || MODE == MODE_C3M || MODE == MODE_SQ2, "blocked accumulation: squaring kernels");
   constexpr int NOUT = KBLK ? 2 : 1;
   Acc outer[NOUT][FI][FJ];
 #pragma unroll
@@ -376,6 +384,9 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs<T>& g, int tm, int tn, 
             acc[0][i][j] = mfma(-fa[1][i][s], fb[1][j][s], acc[0][i][j]);
             acc[1][i][j] = mfma(fa[0][i][s], fb[1][j][s], acc[1][i][j]);
             acc[1][i][j] = mfma(fa[1][i][s], fb[0][j][s], acc[1][i][j]);
+          } else if constexpr (MODE == MODE_SQ2) {
+            acc[0][i][j] = mfma(fa[0][i][s] + fa[1][i][s], fb[0][j][s] - fb[1][j][s], acc[0][i][j]);
+            acc[1][i][j] = mfma(fa[0][i][s], fb[1][j][s], acc[1][i][j]);
           } else {
             acc[0][i][j] = mfma(fa[0][i][s], fb[0][j][s], acc[0][i][j]);
             acc[1][i][j] = mfma(fa[1][i][s], fb[1][j][s], acc[1][i][j]);
@@ -390,11 +401,15 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs<T>& g, int tm, int tn, 
       for (int i = 0; i < FI; ++i) {
 #pragma unroll
         for (int j = 0; j < FJ; ++j) {
-          outer[0][i][j] += acc[0][i][j] - acc[1][i][j];
-          outer[1][i][j] += acc[2][i][j] - acc[0][i][j] - acc[1][i][j];
-          acc[0][i][j] = Acc{};
-          acc[1][i][j] = Acc{};
-          acc[2][i][j] = Acc{};
+          if constexpr (MODE == MODE_SQ2) {
+            outer[0][i][j] += acc[0][i][j];
+            outer[1][i][j] += acc[1][i][j];
+          } else {
+            outer[0][i][j] += acc[0][i][j] - acc[1][i][j];
+            outer[1][i][j] += acc[NACC - 1][i][j] - acc[0][i][j] - acc[1][i][j];
+          }
+#pragma unroll
+          for (int a = 0; a < NACC; ++a) acc[a][i][j] = Acc{};
         }
       }
       }
@@ -432,13 +447,17 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs<T>& g, int tm, int tn, 
         } else if constexpr (KBLK) {
           vr = (double)outer[0][i][j][r];
           vi = (double)outer[1][i][j][r];
+        } else if constexpr (MODE == MODE_SQ2) {
+          vr = (double)acc[0][i][j][r];
+          vi = (double)acc[1][i][j][r];
         } else {
-          const T t1 = acc[0][i][j][r], t2 = acc[1][i][j][r], t3 = acc[2][i][j][r];
+          const T t1 = acc[0][i][j][r], t2 = acc[1][i][j][r], t3 = acc[NACC - 1][i][j][r];
           vr = (double)(t1 - t2);
           vi = (double)(t3 - t1 - t2);
         }
+        if constexpr (MODE == MODE_SQ2) vi *= 2.0;
         const int64_t o = (int64_t)row * e.ldc + col;
-        if (MODE == MODE_C3M && e.d_in) {
+        if ((MODE == MODE_C3M || MODE == MODE_SQ2) && e.d_in) {
           if (row == col) {
             const double2 dd = e.d_in[row];
             e.d_out[row] = make_double2(dd.x * dd.x - dd.y * dd.y + vr, 2.0 * dd.x * dd.y + vi);
@@ -446,11 +465,17 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs<T>& g, int tm, int tn, 
             vi = 0.0;
           } else {
             const double2 di = e.d_in[row], dj = e.d_in[col];
-            const double sr = di.x + dj.x, si = di.y + dj.y;
             const int64_t oy = (int64_t)row * g.lda + col;
             const double yr = (double)g.ar[oy], yi = (double)g.ai[oy];
-            vr += sr * yr - si * yi;
-            vi += sr * yi + si * yr;
+            if constexpr (MODE == MODE_SQ2) {
+              // (D_P Y_Q + Y_P D_Q) with P = Xr + Xi, Q = Xr - Xi; 2 (Dr Yi + Yr Di)
+              vr += (di.x + di.y) * (yr - yi) + (yr + yi) * (dj.x - dj.y);
+              vi += 2.0 * (di.x * yi + yr * dj.y);
+            } else {
+              const double sr = di.x + dj.x, si = di.y + dj.y;
+              vr += sr * yr - si * yi;
+              vi += sr * yi + si * yr;
+            }
           }
         }
         T outr = (T)(e.alpha * vr);

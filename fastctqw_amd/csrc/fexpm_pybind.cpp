@@ -73,6 +73,7 @@ static py::dict stats_dict(const fexpm_stats& s) {
   d["deflated"] = s.deflated;
   d["s_onenorm"] = s.s_onenorm;
   d["rho_bound"] = s.rho_bound;
+  d["s_two_product"] = s.s_two_product;
   d["ms_propagate"] = s.ms_propagate;
   return d;
 }
@@ -520,6 +521,7 @@ PYBIND11_MODULE(_fastexpm_core, m) {
   m.attr("OPT_SPECTRAL") = (int)FEXPM_OPT_SPECTRAL;
   m.attr("OPT_FAIL_ALLOC") = (int)FEXPM_OPT_FAIL_ALLOC;
   m.attr("OPT_DEFLATE") = (int)FEXPM_OPT_DEFLATE;
+  m.attr("OPT_TWO_PRODUCT") = (int)FEXPM_OPT_TWO_PRODUCT;
   m.attr("PREC_NATIVE") = (int)FEXPM_PREC_NATIVE;
   m.attr("PREC_FP16X2") = (int)FEXPM_PREC_FP16X2;
   m.attr("PREC_BF16X3") = (int)FEXPM_PREC_BF16X3;

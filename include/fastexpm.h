@@ -83,6 +83,12 @@ enum fexpm_option {
                                     1 = s from the spectral-radius bound min_k ||B^k||_1^(1/k), k = 1, 2,
                                     4, 6, of the shifted B (default; single runs), 0 = s from ||B||_1
                                     (Higham 2005).  Batches always use the 1-norm rule. */
+  FEXPM_OPT_TWO_PRODUCT = 14,    /* structured input: the first k squarings (at most s - 1) as two real
+                                    products, Re = (Xr + Xi)(Xr - Xi)^T, Im = 2 Xr Xi^T, which needs
+                                    Xr Xi = Xi Xr and issues 2/3 of the 3M flops; its non-commuting
+                                    rounding errors grow in later squarings.  Default 0 (3M throughout):
+                                    at N = 10^4 a two-product squaring takes 23.6 vs 25.4 ms (its
+                                    two-MFMA loop is bound by the operand loads) */
   FEXPM_OPT_DEFLATE = 13         /* with FEXPM_OPT_SPECTRAL: 1 = when every row sum of G is the same c
                                     (double-double sums), deflate the exact eigenvector 1 first:
                                     B = G - mu I + (mu - c)/n J and exp(A) = e^{i mu} (exp(iB) +
@@ -120,6 +126,8 @@ typedef struct fexpm_stats {
   int32_t s_onenorm;    /* s of the 1-norm rule (equal to s when spectral = 0) */
   int32_t reserved0;
   double rho_bound;     /* spectral-radius bound of the shifted (deflated) B, when spectral = 1 */
+  int32_t s_two_product;  /* leading squarings done in the two-product form (FEXPM_OPT_TWO_PRODUCT) */
+  int32_t reserved1;
 } fexpm_stats;
 
 /* Create an engine for n x n matrices of the given dtype and allocate its

@@ -88,8 +88,10 @@ enum fexpm_plane {
   FEXPM_PLANE_RESULT = 10 /* X after the squarings, before the phase e^mu */
 };
 
-/* C = A B^T (mode 0 real, 1 complex 4M, 2 complex 3M), sym != 0: symmetric
- * half-tile output (requires m == n).  wtype 0 = FP32 MFMA, 1 = FP64 MFMA,
+/* C = A B^T (mode 0 real, 1 complex 4M, 2 complex 3M; mode 3 with sym: the
+ * two-product squaring form, Re = lower((Ar + Ai)(Br - Bi)^T), Im =
+ * lower(2 Ar Bi^T), mirrored), sym != 0: symmetric half-tile output
+ * (requires m == n).  wtype 0 = FP32 MFMA, 1 = FP64 MFMA,
  * 2 = fp16x2 and 3 = bf16x3 split-precision squaring kernel (mode 2, sym,
  * B = A, complex 4M on 16-bit MFMA; |a| must stay below 65504 / 2^14 for
  * fp16x2).  A is m x k, B is n x k, C is m x n; imaginary pointers unused

@@ -22,9 +22,10 @@ def test_more_than_one_gpu_is_refused(extra):
 
 def test_roofline_fraction_is_at_most_the_issued_fraction():
     """The squaring roofline counts the structured algorithm's 3 n^3 flops
-    per launch; the kernel issues whole lower 128 x 128 (or 64 x 64) tiles
-    over the padded k, never fewer, so frac <= issued_frac, and a launch
-    time at the issued peak rate gives frac <= 1."""
+    per 3M squaring and 2 n^3 per two-product squaring; the kernels issue
+    whole lower 128 x 128 (or 64 x 64) tiles over the padded k, never fewer,
+    so frac <= issued_frac, and a phase time at the issued peak rate gives
+    frac <= 1."""
     sys.path.insert(0, ROOT)
     import bench
 
@@ -33,7 +34,9 @@ def test_roofline_fraction_is_at_most_the_issued_fraction():
             T = (n + tile - 1) // tile
             kpad = (n + 15) // 16 * 16
             issued = 3 * 2.0 * T * (T + 1) / 2 * tile * tile * kpad
-            ms = issued / (bench.FP32_PEAK_TFLOPS * 1e12) * 1e3
-            r = bench.squaring_roofline(n, ms, issued, bench.FP32_PEAK_TFLOPS)
-            assert r["frac"] <= r["issued_frac"] <= 1.0 + 1e-12, (n, tile, r)
-            assert r["achieved_4m_convention"] > r["achieved"]
+            for s, two in ((8, 0), (8, 2), (3, 2), (1, 0)):
+                total = issued * ((s - two) + two * 2.0 / 3.0)
+                ms = total / (bench.FP32_PEAK_TFLOPS * 1e12) * 1e3
+                r = bench.squaring_roofline(n, ms, s, two, issued, bench.FP32_PEAK_TFLOPS)
+                assert r["frac"] <= r["issued_frac"] <= 1.0 + 1e-12, (n, tile, s, two, r)
+                assert r["achieved_4m_convention"] > r["achieved"]

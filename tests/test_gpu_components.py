@@ -99,6 +99,33 @@ def test_gemm_symmetric(lib, wtype, n):
     assert np.array_equal(c, c.T)
 
 
+def _lower_mirror(p):
+    return np.tril(p) + np.tril(p, -1).T
+
+
+@pytest.mark.parametrize("wtype", [0, 1])
+@pytest.mark.parametrize("n", [16, 100, 129, 300])
+def test_gemm_two_product_squaring(lib, wtype, n):
+    """Mode 3 (the two-product squaring form): for any symmetric Xr, Xi the
+    lower triangle of (Xr + Xi)(Xr - Xi)^T and of 2 Xr Xi^T, mirrored; for
+    commuting parts (Xr = cos G, Xi = sin G) that is X^2 = exp(2iG)."""
+    _say(f"gemm two-product w={wtype} n={n}")
+    rng = np.random.default_rng(n + 11)
+    xr = rng.standard_normal((n, n))
+    xi = rng.standard_normal((n, n))
+    xr, xi = xr + xr.T, xi + xi.T
+    c = _gemm(lib, wtype, 3, 1, xr + 1j * xi, xr + 1j * xi)
+    ref = _lower_mirror((xr + xi) @ (xr - xi).T) + 1j * _lower_mirror(2 * xr @ xi.T)
+    tol = 4e-6 if wtype == 0 else 1e-13
+    assert rel_fro(ref, c) < tol
+    assert np.array_equal(c, c.T)
+    g = rng.standard_normal((n, n))
+    lam, v = np.linalg.eigh((g + g.T) / np.sqrt(n))
+    x = (v * np.exp(1j * lam)) @ v.T
+    c = _gemm(lib, wtype, 3, 1, x, x)
+    assert rel_fro((v * np.exp(2j * lam)) @ v.T, c) < tol
+
+
 @pytest.mark.parametrize("wtype", [2, 3])
 @pytest.mark.parametrize("shape", [(16, 16), (100, 100), (128, 128), (129, 129), (300, 300), (200, 77)])
 def test_gemm_split_squaring(lib, wtype, shape):

@@ -270,6 +270,36 @@ def test_c2_full_size(fx):
 
 
 @pytest.mark.parametrize("dt", [np.complex64, np.complex128])
+@pytest.mark.parametrize("case", ["er_unweighted", "er_weighted"])
+def test_two_product_squarings(fx, dt, case):
+    """Leading squarings in the two-product form (option OPT_TWO_PRODUCT,
+    default 0, never the last): the count in the stats, and U against the
+    eigen oracle for 0 (3M throughout), 2 and 4 of them; two stay within
+    1.5x of the 3M error (plus rounding)."""
+    core = fx._fastexpm_core
+    n = 600
+    L = laplacian(erdos_renyi(n, 0.5, seed=31, weighted=(case == "er_weighted")))
+    a = np.ascontiguousarray((-1j * 20.0 * L).astype(dt))
+    ref = eig_propagator(-a.astype(np.complex128).imag, 1.0)
+    _say(f"two-product squarings {case} n={n} {np.dtype(dt).name}")
+    errs = {}
+    with _engine(fx, dt, n) as e:
+        for k in (0, 2, 4):
+            e._backend.set_option(core.OPT_TWO_PRODUCT, k)
+            u = e.run(a)
+            st = e.stats()
+            assert st["s_two_product"] == min(k, st["s"] - 1), st
+            errs[k] = rel_fro(ref, u)
+            print(f"  two-product {st['s_two_product']} of s={st['s']}: relF={errs[k]:.3e}", flush=True)
+            assert errs[k] <= _tol(dt, st["s"])
+        e._backend.set_option(core.OPT_TWO_PRODUCT, 0)
+        e.run(a)
+        assert e.stats()["s_two_product"] == 0  # the default
+    floor = 1e-7 if dt == np.complex64 else 1e-14
+    assert errs[2] <= 1.5 * errs[0] + floor
+
+
+@pytest.mark.parametrize("dt", [np.complex64, np.complex128])
 @pytest.mark.parametrize("n", [5, 63, 200, 520])
 def test_structured_cholesky_vs_lu(fx, dt, n):
     """The structured solve (real Cholesky of S = V^2 + Uq^2, X = Z Z^T)
