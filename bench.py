@@ -704,29 +704,40 @@ def _api_timing(eng, a, args):
                     "H and A_exp formed on the device, expm, U psi0 on the device"}
 
 
-def _copy_calibration(mib):
-    """A plain device copy of known size through the testing ABI; profiler
-    counters of this launch calibrate FETCH_SIZE / WRITE_SIZE."""
+def _probes():
+    """The measurement probes library (built beside the engine by
+    fastctqw_amd.build; not used by the expm path)."""
     import ctypes
 
-    lib = ctypes.CDLL(os.path.join(ROOT, "fastctqw_amd", "fastexpm", "libfastexpm_hip.so"))
-    lib.fexpm_last_error.restype = ctypes.c_char_p
+    path = os.path.join(ROOT, "fastctqw_amd", "fastexpm", "libfastexpm_probes.so")
+    if not os.path.exists(path):
+        raise RuntimeError("libfastexpm_probes.so is not built: run python -m fastctqw_amd.build")
+    lib = ctypes.CDLL(path)
+    lib.fexpm_probe_last_error.restype = ctypes.c_char_p
+    return lib
+
+
+def _copy_calibration(mib):
+    """A plain device copy of known size (probes library); profiler counters
+    of this launch calibrate FETCH_SIZE / WRITE_SIZE."""
+    import ctypes
+
+    lib = _probes()
     ms = ctypes.c_double(0.0)
     nbytes = int(mib) << 20
-    rc = lib.fexpm_test_copy(ctypes.c_int64(nbytes), 3, ctypes.byref(ms))
+    rc = lib.fexpm_probe_copy(ctypes.c_int64(nbytes), 3, ctypes.byref(ms))
     if rc != 0:
-        raise RuntimeError(lib.fexpm_last_error(None).decode())
+        raise RuntimeError(lib.fexpm_probe_last_error().decode())
     print(json.dumps({"copy_bytes": nbytes, "launches": 4, "ms_per_launch": ms.value,
                       "GB_s": 2.0 * nbytes / (ms.value * 1e-3) / 1e9}), flush=True)
 
 
 def _mfma_rates():
-    """Sustained MFMA rates of the device (in-register MFMA loop, testing
-    ABI), beside the datasheet peaks the roofline uses."""
+    """Sustained MFMA rates of the device (in-register MFMA loop, probes
+    library), beside the datasheet peaks the roofline uses."""
     import ctypes
 
-    lib = ctypes.CDLL(os.path.join(ROOT, "fastctqw_amd", "fastexpm", "libfastexpm_hip.so"))
-    lib.fexpm_last_error.restype = ctypes.c_char_p
+    lib = _probes()
     out = {}
     for kind, name, peak, iters in ((0, "fp32_16x16x4", FP32_PEAK_TFLOPS, 20000),
                                     (1, "fp64_16x16x4", FP64_PEAK_TFLOPS, 20000),
@@ -735,23 +746,23 @@ def _mfma_rates():
                                     (4, "int8_32x32x32", INT8_PEAK_TOPS, 4000),
                                     (5, "fp32_32x32x2", FP32_PEAK_TFLOPS, 20000)):
         tf, ms = ctypes.c_double(0.0), ctypes.c_double(0.0)
-        rc = lib.fexpm_test_mfma_rate(kind, iters, ctypes.byref(tf), ctypes.byref(ms))
+        rc = lib.fexpm_probe_mfma_rate(kind, iters, ctypes.byref(tf), ctypes.byref(ms))
         if rc != 0:
-            raise RuntimeError(lib.fexpm_last_error(None).decode())
+            raise RuntimeError(lib.fexpm_probe_last_error().decode())
         out[name] = {"tflops": tf.value, "datasheet_tflops": peak, "fraction_of_datasheet": tf.value / peak,
                      "kernel_ms": ms.value}
         # the same MFMA stream at 1, 2 and 4 waves per SIMD
         by_waves = {}
         for w in (1, 2, 4):
-            rc = lib.fexpm_test_mfma_rate_waves(kind, w, iters, ctypes.byref(tf), ctypes.byref(ms))
+            rc = lib.fexpm_probe_mfma_rate_waves(kind, w, iters, ctypes.byref(tf), ctypes.byref(ms))
             if rc != 0:
-                raise RuntimeError(lib.fexpm_last_error(None).decode())
+                raise RuntimeError(lib.fexpm_probe_last_error().decode())
             by_waves[w] = tf.value
         out[name]["tflops_by_waves_per_simd"] = by_waves
     # the GEMM kernels at the N = 10^4 shapes, issued MFMA flops per launch time
     gem = {}
     n, kp = 10000, 10112
-    lib.fexpm_test_gemm_time.argtypes = [ctypes.c_int] * 8 + [ctypes.POINTER(ctypes.c_double)]
+    lib.fexpm_probe_gemm_time.argtypes = [ctypes.c_int] * 8 + [ctypes.POINTER(ctypes.c_double)]
     for name, w, mode, cfg, tile, mult in (("fp32_c3m_sym_128x128_8waves", 0, 2, 0, 128, 3),
                                            ("fp32_c3m_sym_64x64_4waves", 0, 2, 4, 64, 3),
                                            ("fp32_c3m_sym_128x128_16waves", 0, 2, 12, 128, 3),
@@ -777,9 +788,9 @@ def _mfma_rates():
                                            ("fp16x2_c4m_sym_128x128_8waves_1kblock", 2, 2, 80, 128, 12),
                                            ("bf16x3_c4m_sym_128x128_8waves", 3, 2, 80, 128, 24)):
         ms = ctypes.c_double(0.0)
-        rc = lib.fexpm_test_gemm_time(w, mode, 1, cfg, n, n, kp, 3, ctypes.byref(ms))
+        rc = lib.fexpm_probe_gemm_time(w, mode, 1, cfg, n, n, kp, 3, ctypes.byref(ms))
         if rc != 0:
-            raise RuntimeError(lib.fexpm_last_error(None).decode())
+            raise RuntimeError(lib.fexpm_probe_last_error().decode())
         T = (n + tile - 1) // tile
         issued = mult * 2.0 * T * (T + 1) / 2 * tile * tile * kp
         gem[name] = {"ms": ms.value, "issued_tflops": issued / (ms.value * 1e-3) / 1e12}

@@ -5,6 +5,9 @@ its $ORIGIN run path):
 
     fastctqw_amd/fastexpm/libfastexpm_hip.so      HIP kernels + C ABI
     fastctqw_amd/fastexpm/_fastexpm_core*.so      pybind11 layer (host only)
+    fastctqw_amd/fastexpm/libfastexpm_probes.so   measurement probes (kernel
+                                                  timing, MFMA rates, copy);
+                                                  not loaded by the engine
 
 Run ``python -m fastctqw_amd.build`` (or ``__graft_entry__.build()``).
 """
@@ -20,6 +23,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "fastexpm")
 LIB = os.path.join(OUT, "libfastexpm_hip.so")
+PROBES = os.path.join(OUT, "libfastexpm_probes.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -37,13 +41,20 @@ def _run(cmd):
     subprocess.run(cmd, check=True)
 
 
-def build_hip(verbose: bool = True) -> str:
+def _build_one(src_name: str, out: str) -> str:
     os.makedirs(OUT, exist_ok=True)
-    src = os.path.join(CSRC, "fexpm_engine.hip")
-    tmp = LIB + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-o", tmp, src])
-    os.replace(tmp, LIB)
-    return LIB
+    tmp = out + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-o", tmp, os.path.join(CSRC, src_name)])
+    os.replace(tmp, out)
+    return out
+
+
+def build_hip(verbose: bool = True) -> str:
+    return _build_one("fexpm_engine.hip", LIB)
+
+
+def build_probes() -> str:
+    return _build_one("fexpm_probes.hip", PROBES)
 
 
 def build_pybind() -> str:
@@ -68,6 +79,7 @@ def build_pybind() -> str:
 def build() -> None:
     build_hip()
     build_pybind()
+    build_probes()
 
 
 if __name__ == "__main__":

@@ -131,8 +131,6 @@ template <int BM_, int BN_, int BK_, int PAD_, int WM_, int WN_, int MF_ = 16, i
 };
 template <typename T> struct GemmCfg;
 template <> struct GemmCfg<float> : TileCfg<128, 128, 16, 4, 64, 32, 32> {};
-// The same tiling on 16x16x4 MFMAs (the earlier default; timing comparisons).
-using FCfg16 = TileCfg<128, 128, 16, 4, 64, 32, 16>;
 // FP32 squarings of small orders: 4x the workgroups of the default tile
 // (a 128-wide tile grid of a few hundred rows leaves most CUs idle); four
 // waves per SIMD (128 VGPRs: four workgroups per CU, 18-25% faster at
@@ -150,13 +148,9 @@ using FCfg16w = TileCfg<128, 128, 16, 4, 32, 32, 32, 4>;
 // SIMD, so one workgroup per CU): within 128 it runs 57.4 -> 46.9 ms per
 // n = 10^4 symmetric product.
 template <> struct GemmCfg<double> : TileCfg<64, 64, 16, 2, 32, 16, 16, 4> {};
-// Alternative FP64 tilings (selected per call site, see fexpm_test_gemm_time).
-using DCfg4w = TileCfg<64, 64, 16, 2, 32, 32>;     // 4 waves, 32x32 wave tiles
-using DCfg8w = TileCfg<64, 64, 16, 2, 32, 16>;     // 8 waves, 32x16 wave tiles
-using DCfgTall = TileCfg<128, 64, 16, 2, 32, 32>;  // 8 waves, 128x64 workgroup tile
-using DCfgWide = TileCfg<64, 128, 16, 2, 32, 32>;  // 8 waves, 64x128 workgroup tile
-using DCfg8wPad4 = TileCfg<64, 64, 16, 4, 32, 16>;  // the default with 4-double LDS row padding
-using DCfg8wW4 = TileCfg<64, 64, 16, 2, 32, 16, 16, 4>;  // the default within 128 VGPRs (4 waves per SIMD)
+// FP64 4 waves of 32x32 wave tiles: the grouped (batched) launches' wider
+// candidate.
+using DCfg4w = TileCfg<64, 64, 16, 2, 32, 32>;
 
 // Staging of a ROWS x BK tile through registers, NT threads.
 template <typename T, int ROWS, int BK, int PAD, int NT> struct TileStage {

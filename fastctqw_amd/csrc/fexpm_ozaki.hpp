@@ -301,20 +301,6 @@ __device__ __forceinline__ void oz_tile(const OzArgs<S>& g, int tm, int tn, bool
   }
 }
 
-// The same product on 64 x 64 tiles of 4 waves (32 x 32 each), 48 KB of
-// LDS for S = 6: three workgroups per CU; tiles in 8 x 8 groups.
-template <int S, bool SYM>
-__global__ __launch_bounds__(256, 3) void oz_gemm_kernel64(OzArgs<S> g) {
-  int tm, tn;
-  if (SYM) {
-    grouped_tri_tile(xcd_remap((int)blockIdx.x, (int)gridDim.x), g.rows_a / 64, tm, tn);
-  } else {
-    const int lin = xcd_remap((int)(blockIdx.y * gridDim.x + blockIdx.x), (int)(gridDim.x * gridDim.y));
-    grouped_rect_tile(lin, (int)gridDim.y, (int)gridDim.x, tm, tn);
-  }
-  oz_tile<S, 64, 2, 2>(g, tm, tn, SYM);
-}
-
 template <int S, bool SYM>
 __global__ __launch_bounds__(512) void oz_gemm_kernel(OzArgs<S> g) {
   int tm, tn;

@@ -110,37 +110,11 @@ int fexpm_debug_plane(fexpm_engine* e, int which, double* re, double* im);
  * fexpm_set_graph_edges or fexpm_set_graph_er. */
 int fexpm_debug_hamiltonian(fexpm_engine* e, double* h);
 
-/* Average device time (ms) of one GEMM launch over reps launches on
- * device-resident operands (m x k, n x k -> m x n); cfg selects an FP64
- * tiling (0 4-wave 32x32, 1 8-wave 32x16 = default, 2 128x64, 3 64x128;
- * 2 and 3 are not square and are invalid with sym).  wtype 2 / 3 time the
- * split-precision squaring kernel alone (mode 2, sym). */
-int fexpm_test_gemm_time(int wtype, int mode, int sym, int cfg, int m, int n, int k, int reps, double* ms);
-
 /* C = A B^T (FP64, host row-major A m x k, B n x k, C m x n) computed from
  * exact int8 slices of the rows of A and B (`slices` in 3..6) with int32
  * accumulation on the int8 MFMA and FP64 combination; with sym, B must be A
- * and only lower tiles are computed (mirrored).  wtype 4 of
- * fexpm_test_gemm_time times this product kernel (cfg = slices). */
+ * and only lower tiles are computed (mirrored). */
 int fexpm_test_ozaki_gemm(int slices, int sym, int m, int n, int k, const double* a, const double* b, double* c);
-
-/* reps + 1 launches of a plain device copy of `bytes` (multiple of 16);
- * average ms of the last reps.  Reference traffic for the profiler's
- * FETCH_SIZE / WRITE_SIZE counters (each launch reads and writes bytes). */
-int fexpm_test_copy(int64_t bytes, int reps, double* ms);
-
-/* Sustained MFMA rate of the device: a kernel whose waves issue only
- * independent MFMAs from registers (kind 0 FP32 16x16x4, 1 FP64 16x16x4,
- * 2 FP16 32x32x16, 3 BF16 32x32x16, 4 INT8 32x32x32 (TOP/s), 5 FP32
- * 32x32x2), 8 waves per SIMD; `iters` rounds per wave.  Returns TFLOP/s
- * and the kernel time.  Measures the practical peak
- * that the GEMM kernels' rates are compared with. */
-int fexpm_test_mfma_rate(int kind, int iters, double* tflops, double* ms);
-
-/* The same probe with `waves` (1..8) waves per SIMD: the rate an MFMA stream
- * can reach at the occupancy of a register-heavy GEMM kernel (two waves per
- * SIMD for the squaring and int8-slice kernels). */
-int fexpm_test_mfma_rate_waves(int kind, int waves, int iters, double* tflops, double* ms);
 
 /* MFMA flops one FP32 structured (complex symmetric, 3M) squaring of order n
  * issues under the automatic tile choice, including the tail sub-tiles: the

@@ -69,20 +69,40 @@ def test_laplacian_and_state():
 
 # --------------------------------------------------------------------------- C ABI
 
-def _declared_symbols():
-    text = "".join(open(h).read() for h in (HEADER, os.path.join(ROOT, "include", "fastexpm_testing.h")))
+PROBES = os.path.join(os.path.dirname(LIB), "libfastexpm_probes.so")
+
+
+def _declared_symbols(*headers):
+    text = "".join(open(os.path.join(ROOT, "include", h)).read() for h in headers)
     return sorted(set(re.findall(r"^\s*(?:int|int64_t|double|const char\*)\s+(fexpm_\w+)\(", text, re.M)))
 
 
 def test_library_exports_every_declared_symbol():
+    """Each header's declarations are exported by its library: fastexpm.h
+    and fastexpm_testing.h by the engine, fastexpm_probes.h by the probes
+    library (and the engine exports none of the probes)."""
     if not os.path.exists(LIB):
         pytest.skip("HIP library not built")
-    syms = _declared_symbols()
+    syms = _declared_symbols("fastexpm.h", "fastexpm_testing.h")
     assert len(syms) >= 14
     assert "fexpm_batch_get_report" in syms and "fexpm_test_group_program" in syms
     lib = ctypes.CDLL(LIB)
     for s in syms:
         assert hasattr(lib, s), s
+    probes = _declared_symbols("fastexpm_probes.h")
+    assert "fexpm_probe_gemm_time" in probes and "fexpm_probe_copy" in probes
+    for s in probes:
+        assert not hasattr(lib, s), s
+    if not os.path.exists(PROBES):
+        pytest.skip("probes library not built")
+    plib = ctypes.CDLL(PROBES)
+    for s in probes:
+        assert hasattr(plib, s), s
+    # argument errors are reported without a device call
+    plib.fexpm_probe_last_error.restype = ctypes.c_char_p
+    ms = ctypes.c_double(0.0)
+    assert plib.fexpm_probe_copy(ctypes.c_int64(17), 1, ctypes.byref(ms)) == -1
+    assert plib.fexpm_probe_gemm_time(0, 2, 1, 0, 64, 32, 64, 1, ctypes.byref(ms)) == -1
 
 
 def test_library_no_device_and_argument_errors():
