@@ -10,8 +10,12 @@ Prints ONE JSON line (see the keys below).  ``roofline`` describes the
 dominant kernel, the squaring GEMM: its average launch duration comes from
 HIP events recorded by the engine on the stream the squarings run on
 (phase time / s).  ``cpu_baseline`` times scipy.linalg.expm (the reference's
-CPU path, FastCTQW/Operator.py:76) on a bounded sample of the same graph
-family and scales it to N = 10^4 by the work model W(N, s).
+CPU path, FastCTQW/Operator.py:76) on the same N = 10^4 input on the host's
+share of cores.
+
+Other workloads (--config): c3 precision study, c4 256 uneven walks, c5
+times x states, general (dense general complex input on the LU route and the
+reference's performance inputs beside scipy).
 """
 from __future__ import annotations
 
@@ -289,6 +293,124 @@ def run_c5(args):
         res["split_precision"] = sp
         eng.set_precision("native")
     eng.free()
+    print(json.dumps(res), flush=True)
+
+
+GENERAL_SIZES = (16, 32, 64, 128, 256, 512, 1024)
+
+
+def _gaussian_general(n, seed=7):
+    """Dense general complex64 matrix (G_r + i G_i) * 3 / sqrt(n) with standard
+    normal G: spectrum in a disk of radius about 3, 1-norm about 3.8 sqrt(n)."""
+    rng = np.random.default_rng(seed)
+    g = rng.standard_normal((n, n), dtype=np.float32) + 1j * rng.standard_normal((n, n), dtype=np.float32)
+    return (g * np.float32(3.0 / np.sqrt(n))).astype(np.complex64)
+
+
+def run_general(args):
+    """General complex inputs on the LU route.
+
+    Timed metric: one expm of a dense general complex64 matrix of order N
+    (default 10^4, input resident).  Accuracy by the group law
+    e^{-A} (e^{A} x) = x on 4 random vectors (FP64 applies on the device).
+    Beside it, the reference's performance inputs A = Adj + 1j * rand
+    (complete graph, G(n, 0.05), 4-regular; tests/test_fastexpm_perf.py) at
+    its default sizes 16..1024, timed as its benchmark does (host array in,
+    host U out) next to scipy.linalg.expm on the same array, with the
+    relative error against scipy's complex128 expm of that array where the
+    result fits complex64."""
+    import scipy.linalg
+
+    from fastctqw_amd.fastexpm import ExpMatComplex64
+    from fastctqw_amd.graphs import reference_perf_input
+
+    n = args.n
+    a = _gaussian_general(n)
+    eng = ExpMatComplex64(n)
+    eng.upload(a)
+    for _ in range(args.warmup):
+        eng.run_resident()
+    ms, phases = [], []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        eng.run_resident()
+        ms.append(1000.0 * (time.perf_counter() - t0))
+        phases.append(eng.stats())
+    st = phases[-1]
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal((4, n)) + 1j * rng.standard_normal((4, n))
+    y = eng.apply(x)
+    eng.upload((-a).astype(np.complex64))
+    eng.run_resident()
+    back = eng.apply(y)
+    eng.free()
+    del a
+    group_law = float(np.linalg.norm(back - x) / np.linalg.norm(x))
+    ms_step = float(np.median(ms))
+    s = st["s"]
+    npad = (n + 127) // 128 * 128
+    sq_ms = float(np.mean([p["ms_square"] for p in phases])) / max(s, 1)
+    issued = 3 * 2.0 * npad ** 3  # 3M: three real products of npad^3 (all tiles)
+    res = {
+        "metric": f"s per N={n} general complex expm (LU route)",
+        "value": ms_step / 1000.0, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_step, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c64 (fp64 pade products and pivoted LU solve, fp32 mfma squarings)",
+        "data": "synthetic: seeded dense (G_r + i G_i) * 3/sqrt(N), G standard normal",
+        "config": {"workload": f"expm of a dense general complex64 matrix, N={n}", "n": n,
+                   "m": st["m"], "s": s, "structured": st["structured"]},
+        "phases_ms": {k: float(np.median([p[k] for p in phases]))
+                      for k in ("ms_analyze", "ms_pade", "ms_solve", "ms_square", "ms_output", "ms_total")},
+        "squaring": {"kernel": "gemm_nt_kernel (3M complex, all tiles, FP32 MFMA)", "avg_launch_ms": sq_ms,
+                     "issued_tflops": issued / (sq_ms * 1e-3) / 1e12,
+                     "issued_frac_of_fp32_peak": issued / (sq_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS},
+        "accuracy": {"group_law_residual": group_law,
+                     "check": "||e^{-A} (e^{A} x) - x|| / ||x|| over 4 random x (FP64 applies)"},
+    }
+    if args.no_perf_inputs:
+        print(json.dumps(res), flush=True)
+        return
+    from threadpoolctl import threadpool_limits
+
+    rows = []
+    pool = threadpool_limits(limits=_cpu_share())
+    for fam in ("complete", "er", "r4"):
+        for m in GENERAL_SIZES:
+            am = reference_perf_input(fam, m, seed=m)
+            with np.errstate(all="ignore"):
+                t0 = time.perf_counter()
+                ref = scipy.linalg.expm(am.astype(np.complex128))
+                ref_first = time.perf_counter() - t0
+            reps = max(3, min(50, int(0.5 / max(ref_first, 1e-4))))
+            cpu = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                scipy.linalg.expm(am)
+                cpu.append(1000.0 * (time.perf_counter() - t0))
+            with ExpMatComplex64(m) as e:
+                u = e.run(am)
+                gpu = []
+                for _ in range(max(5, reps)):
+                    t0 = time.perf_counter()
+                    u = e.run(am)
+                    gpu.append(1000.0 * (time.perf_counter() - t0))
+                se = e.stats()
+            fits = bool(np.isfinite(ref).all() and np.abs(ref).max() < np.finfo(np.float32).max)
+            row = {"family": fam, "n": m, "m": se["m"], "s": se["s"], "fastexpm_ms": float(np.median(gpu)),
+                   "scipy_complex64_ms": float(np.median(cpu)),
+                   "speedup": float(np.median(cpu)) / float(np.median(gpu))}
+            if fits:
+                row["rel_err_vs_scipy_c128"] = float(np.linalg.norm(u - ref) / np.linalg.norm(ref))
+            else:
+                row["result_overflows_complex64"] = True
+                row["output_nonfinite_fraction"] = float((~np.isfinite(u)).mean())
+            rows.append(row)
+    pool.unregister()
+    res["reference_perf_inputs"] = {
+        "note": "A = Adj + 1j*rand(n,n), complex64 (the reference's performance inputs); times are host "
+                "array in, host U out (median), as the reference's benchmark calls run(A); scipy on this "
+                f"host's cores ({_cpu_share()} threads); the complete graph overflows complex64 from n=128",
+        "rows": rows}
     print(json.dumps(res), flush=True)
 
 
@@ -867,7 +989,7 @@ def main():
     ap.add_argument("--cpu-repeats", type=int, default=1,
                     help="scipy.linalg.expm runs at the full N for cpu_baseline.value (median)")
     ap.add_argument("--no-batched", action="store_true",
-                    help="c2: skip the C4 (256 uneven walks) and C5 (times x states) child runs")
+                    help="c2: skip the C4 (256 uneven walks), C5 (times x states) and general-input child runs")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the FP64 eigendecomposition reference (torch.linalg.eigh, child process)")
     ap.add_argument("--eigh-ref", default=None, help=argparse.SUPPRESS)
@@ -880,14 +1002,17 @@ def main():
     ap.add_argument("--no-variants", action="store_true", help="c4: skip the lanes, fp16x2 and one-lane runs")
     ap.add_argument("--c4-lanes", action="store_true", help="c4: time the concurrent-lanes form instead of "
                                                              "the grouped launches")
-    ap.add_argument("--config", choices=["c2", "c3", "c4", "c5"], default="c2",
+    ap.add_argument("--config", choices=["c2", "c3", "c4", "c5", "general"], default="c2",
                     help="c2: N=10^4 single expm (default); c3: precision study at N=10^4; "
-                         "c4: 256 uneven walks; c5: 32 times x 128 states, N=8192")
+                         "c4: 256 uneven walks; c5: 32 times x 128 states, N=8192; general: dense general "
+                         "complex64 expm at N (LU route) and the reference's performance inputs at 16..1024")
     ap.add_argument("--precision", choices=list(C3_PRECISIONS), default="native",
                     help="c2: squaring precision of the complex64 engine")
     ap.add_argument("--no-split", action="store_true",
                     help="c2: skip the fp16x2 split-precision comparison run")
     ap.add_argument("--no-c128", action="store_true", help="c2: skip the complex128 run")
+    ap.add_argument("--no-perf-inputs", action="store_true",
+                    help="general: skip the reference's performance inputs (16..1024) beside scipy")
     ap.add_argument("--fp64-slices", type=int, default=None,
                     help="c2: FP64 Pade products from this many exact int8 slices (4..6) on the int8 MFMA; "
                          "0 = FP64 MFMA (default: the engine's, 6)")
@@ -940,6 +1065,13 @@ def main():
             print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}), flush=True)
             sys.exit(2)
         run_c5(args)
+        return
+
+    if args.config == "general":
+        if args.gpus != 1:
+            print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}), flush=True)
+            sys.exit(2)
+        run_general(args)
         return
 
     if args.eigh_ref:
@@ -1124,7 +1256,8 @@ def main():
     del a
     if not args.no_batched:
         # the batched workloads in child processes (each its own GPU context)
-        for cfg, extra in (("c4", ["--no-variants", "--no-cpu-baseline"]), ("c5", ["--no-split"])):
+        for cfg, extra in (("c4", ["--no-variants", "--no-cpu-baseline"]), ("c5", ["--no-split"]),
+                           ("general", [])):
             print(f"[bench] {cfg} (child process) ...", file=sys.stderr, flush=True)
             out[cfg] = _child_summary(cfg, extra, args)
     if not args.no_cpu_baseline:
@@ -1156,6 +1289,9 @@ def _child_summary(cfg, extra, args):
     if cfg == "c4":
         keep["batch_report"] = r.get("batch_report")
         keep["squaring_class"] = r.get("squaring_class")
+    if cfg == "general":
+        keep["phases_ms"] = r.get("phases_ms")
+        keep["reference_perf_inputs"] = r.get("reference_perf_inputs")
     return keep
 
 

@@ -237,6 +237,34 @@ def c2_laplacian(n: int = 10000, p: float = 0.5, seed: int = 2,
     return laplacian(erdos_renyi(n, p, seed, weighted=weighted, dtype=dtype))
 
 
+def reference_perf_input(family: str, n: int, seed: int = 0, dtype=np.complex64) -> np.ndarray:
+    """General complex input of the reference's performance tests:
+    A = Adj + 1j * rand(n, n) with Adj the adjacency of the complete graph,
+    of G(n, 0.05) or of a random 4-regular graph and rand uniform in [0, 1)
+    (tests/test_fastexpm_perf.py:10-59).  Not skew-Hermitian: for the
+    complete graph ||e^A|| grows like e^(n-1), so the complex64 result
+    overflows from n of about 90 on.  Seeded here (numpy PCG64 for rand);
+    the reference draws from unseeded networkx generators.  The reference
+    skips ER below n = 2 and 4-regular graphs for odd n or n < 5; this
+    raises ValueError there."""
+    if family == "complete":
+        adj = np.ones((n, n)) - np.eye(n)
+    elif family == "er":
+        if n < 2:
+            raise ValueError("ER input needs n >= 2")
+        adj = erdos_renyi(n, 0.05, seed)
+    elif family == "r4":
+        if n % 2 or n < 5:
+            raise ValueError("4-regular input needs even n >= 6")
+        adj = random_regular(n, 4, seed)
+    else:
+        raise ValueError(f"unknown family {family!r}")
+    rng = np.random.default_rng(seed)
+    rdt = np.float32 if np.dtype(dtype) == np.complex64 else np.float64
+    a = adj.astype(rdt) + 1j * rng.random((n, n)).astype(rdt)
+    return a.astype(dtype)
+
+
 def c4_walks(count: int = 256, seed: int = 4000, nmin: int = 64, nmax: int = 2048):
     """Seeded batch of uneven CTQW instances (the many-walks workload).
 
