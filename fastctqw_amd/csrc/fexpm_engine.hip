@@ -26,6 +26,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <unordered_map>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -173,6 +174,14 @@ struct Arena {
 };
 static thread_local Arena* g_arena = nullptr;
 
+// Accounting of the device memory an engine holds: the sizes of the live
+// hipMalloc blocks of dmalloc_bytes, and, while an engine call runs on a
+// thread, the counter of that engine (g_acct), which every allocation and
+// release through dmalloc_bytes / dfree adjusts.
+static std::mutex g_dev_mtx;
+static std::unordered_map<void*, size_t> g_dev_sizes;
+static thread_local int64_t* g_acct = nullptr;
+
 static void* dmalloc_bytes(size_t bytes) {
   if (g_arena) {
     const size_t off = (g_arena->used + 255) / 256 * 256;
@@ -183,6 +192,9 @@ static void* dmalloc_bytes(size_t bytes) {
   }
   void* p = nullptr;
   FX_HIP(hipMalloc(&p, bytes));
+  std::lock_guard<std::mutex> lk(g_dev_mtx);
+  g_dev_sizes[p] = bytes;
+  if (g_acct) *g_acct += (int64_t)bytes;
   return p;
 }
 // Zeroed device array (arena memory is cleared once when the arena is set up).
@@ -193,7 +205,16 @@ template <typename T> static T* dalloc(size_t count) {
   return static_cast<T*>(p);
 }
 static void dfree(void* p) {
-  if (p && !g_arena) (void)hipFree(p);
+  if (!p || g_arena) return;
+  {
+    std::lock_guard<std::mutex> lk(g_dev_mtx);
+    auto it = g_dev_sizes.find(p);
+    if (it != g_dev_sizes.end()) {
+      if (g_acct) *g_acct -= (int64_t)it->second;
+      g_dev_sizes.erase(it);
+    }
+  }
+  (void)hipFree(p);
 }
 
 template <typename W> struct Plane2 {
@@ -491,18 +512,16 @@ template <typename W> struct Pipeline : PipelineBase {
     np = npcap;
     ld = npcap;
     ldl = 2 * npcap;
-    Plane2<W>* all[] = {&a, &p2, &p4, &p6, &t1, &t2, &u, &v, &x0, &x1};
     const size_t cnt = (size_t)npcap * (size_t)ld;
-    for (Plane2<W>* p : all) {
+    for (Plane2<W>* p : {&t1, &x0, &x1}) {
       p->r = dalloc<W>(cnt);
       p->i = dalloc<W>(cnt);
     }
-    // lr and li are one allocation (li = lr + npcap*ldl): the structured
-    // solve uses the 4 npcap^2 doubles as one contiguous work area.
-    lr = dalloc<double>((size_t)2 * npcap * ldl);
-    li = lr + (size_t)npcap * ldl;
-    tr = dalloc<double>((size_t)2 * npcap * LU_OUTER);
-    ti = dalloc<double>((size_t)2 * npcap * LU_OUTER);
+    // The Pade and solve buffers come at the first run that needs them
+    // (ensure_stage_buffers): an FP32 pipeline runs those stages on its
+    // FP64 companion by default, and structured inputs use only the real
+    // planes.  A pipeline in an arena holds all of them from the start.
+    if (in_arena) ensure_stage_buffers(true);
     ipiv = dalloc<int>(npcap);
     gmax = (int)((npcap + PANEL_R - 1) / PANEL_R);
     cand = dalloc<PanelCand>((size_t)2 * gmax);
@@ -523,6 +542,30 @@ template <typename W> struct Pipeline : PipelineBase {
     if (with_events) {
       for (int k = 0; k < 8; ++k) FX_HIP(hipEventCreate(&ev[k]));
       events = true;
+    }
+  }
+
+  // The Pade and solve stages of this pipeline use the planes a, p2, p4,
+  // p6, t2, u, v and the FP64 system lr / li with the transposed block rows
+  // tr / ti; structured inputs (A = iG) only the real planes.  They are
+  // allocated (zeroed) here on first use; the squarings need only x0, x1
+  // and t1, which the constructor allocates.
+  bool stage_real = false, stage_cplx = false;
+  void ensure_stage_buffers(bool cplx) {
+    const size_t cnt = (size_t)npcap * (size_t)ld;
+    if (!stage_real) {
+      for (Plane2<W>* p : {&a, &p2, &p4, &p6, &t2, &u, &v}) p->r = dalloc<W>(cnt);
+      // lr and li are one allocation (li = lr + npcap*ldl): the structured
+      // solve uses the 4 npcap^2 doubles as one contiguous work area.
+      lr = dalloc<double>((size_t)2 * npcap * ldl);
+      li = lr + (size_t)npcap * ldl;
+      tr = dalloc<double>((size_t)2 * npcap * LU_OUTER);
+      ti = dalloc<double>((size_t)2 * npcap * LU_OUTER);
+      stage_real = true;
+    }
+    if (cplx && !stage_cplx) {
+      for (Plane2<W>* p : {&a, &p2, &p4, &p6, &t2, &u, &v}) p->i = dalloc<W>(cnt);
+      stage_cplx = true;
     }
   }
 
@@ -563,8 +606,8 @@ template <typename W> struct Pipeline : PipelineBase {
       dfree(oz_s[b]);
       dfree(oz_e[b]);
     }
-    if (sbuf) (void)hipFree(sbuf);
-    for (auto& kv : sq_orders) (void)hipFree(kv.second);
+    dfree(sbuf);
+    for (auto& kv : sq_orders) dfree(kv.second);
     for (hipEvent_t e : zev) (void)hipEventDestroy(e);
     if (zst) (void)hipStreamDestroy(zst);
     if (events)
@@ -587,9 +630,9 @@ template <typename W> struct Pipeline : PipelineBase {
       Plane2<W>* all[] = {&a, &p2, &p4, &p6, &t1, &t2, &u, &v, &x0, &x1};
       PlaneList<W> L;
       L.count = 0;
-      for (Plane2<W>* p : all) {
-        L.p[L.count++] = p->r;
-        L.p[L.count++] = p->i;
+      for (Plane2<W>* p : all) {  // planes not allocated yet are skipped
+        if (p->r) L.p[L.count++] = p->r;
+        if (p->i) L.p[L.count++] = p->i;
       }
       zero_strip<W><<<(unsigned)n, 128, 0, st>>>(L, n, (int)np, ld);
       FX_LAUNCH();
@@ -819,6 +862,7 @@ template <typename W> struct Pipeline : PipelineBase {
   // plane, pade_assemble forms U and V at X = f C (f a power of two, so
   // every scaled coefficient b_j f^j and product scale f^k is exact).
   void pade_powers(int m, hipStream_t st) {
+    ensure_stage_buffers(false);
     sym_real(a, a, p2, st);  // C2
     if (m >= 5) sym_real(p2, p2, p4, st);
     if (m >= 7) sym_real(p4, p2, p6, st);
@@ -826,6 +870,7 @@ template <typename W> struct Pipeline : PipelineBase {
   }
 
   void pade_assemble(int m, double f, hipStream_t st) {
+    ensure_stage_buffers(false);
     double b[14];
     for (int j = 0; j <= m; ++j) b[j] = pade_b(m, j);
     const double f2 = f * f, f4 = f2 * f2, f6 = f4 * f2, f8 = f4 * f4;
@@ -859,6 +904,7 @@ template <typename W> struct Pipeline : PipelineBase {
   }
 
   void pade_structured(int m, hipStream_t st) {
+    ensure_stage_buffers(false);
     pade_powers(m, st);
     pade_assemble(m, 1.0, st);
   }
@@ -881,6 +927,7 @@ template <typename W> struct Pipeline : PipelineBase {
   // 13 squarings at N = 10^4.  Returns the smallest bound (FP64 planes only).
   template <typename Tin>
   double spectral_front(const Tin* d_a, double mu_im, double alpha, hipStream_t st) {
+    ensure_stage_buffers(false);
     if constexpr (std::is_same<W, double>::value) {
       constexpr int kRows = 64;
       const int nch = (n + kRows - 1) / kRows;
@@ -922,6 +969,7 @@ template <typename W> struct Pipeline : PipelineBase {
 
   // General complex A in a (planar).  Produces U in u, V in v.
   void pade_general(int m, hipStream_t st) {
+    ensure_stage_buffers(true);
     double b[14];
     for (int j = 0; j <= m; ++j) b[j] = pade_b(m, j);
     transpose(a, t1, true, st);      // A^T
@@ -963,6 +1011,7 @@ template <typename W> struct Pipeline : PipelineBase {
   // leaves Y = X - diag(X) in x0.  With q64 > 0 (structured path of an FP32
   // engine) the first q64 squarings run in FP64 on the free LU buffer first.
   int solve(bool structured, int q64, hipStream_t st) {
+    ensure_stage_buffers(!structured);
     FX_HIP(hipMemsetAsync(&scal->info, 0, sizeof(int), st));
     if (structured && spd_solve) return solve_spd(q64, st);
     const int inp = (int)np;
@@ -1353,6 +1402,7 @@ template <typename W> struct Pipeline : PipelineBase {
   template <typename Tin>
   void front_pade(const Tin* d_a, bool structured, int m, double mu_re, double mu_im, double scale,
                   hipStream_t st) {
+    ensure_stage_buffers(!structured);
     dim3 grid((n + 255) / 256, n);
     prepare_planar<Tin, W><<<grid, 256, 0, st>>>(d_a, n, a.r, a.i, ld, mu_re, mu_im, scale, structured ? 1 : 0);
     FX_LAUNCH();
@@ -1707,6 +1757,10 @@ template <typename W> struct Pipeline : PipelineBase {
     FX_HIP(hipDeviceSynchronize());
     FX_HIP(hipMemcpy2D(buf.data(), n * sizeof(W), P.r, ld * sizeof(W), n * sizeof(W), n, hipMemcpyDeviceToHost));
     for (size_t k = 0; k < buf.size(); ++k) re[k] = (double)buf[k];
+    if (!P.i) {  // real plane of a structured run
+      std::fill(im, im + buf.size(), 0.0);
+      return;
+    }
     FX_HIP(hipMemcpy2D(buf.data(), n * sizeof(W), P.i, ld * sizeof(W), n * sizeof(W), n, hipMemcpyDeviceToHost));
     for (size_t k = 0; k < buf.size(); ++k) im[k] = (double)buf[k];
   }
@@ -1822,6 +1876,7 @@ struct fexpm_engine {
   double* d_h = nullptr;  // walk Hamiltonian kept by fexpm_set_graph (FP64, n x n)
   bool have_graph = false;
   HostStaging stg;
+  int64_t dev_bytes = 0;  // device memory held (pipelines, I/O buffers, H)
 };
 
 static size_t elem_size(int dtype) {
@@ -1840,6 +1895,11 @@ static thread_local std::string g_last_error;
 
 template <typename F> static int guarded(fexpm_engine* e, F&& f) {
   std::string& err = e ? e->err : g_last_error;
+  struct AcctScope {  // device allocations of this call count for e
+    int64_t* prev;
+    explicit AcctScope(int64_t* c) : prev(g_acct) { g_acct = c; }
+    ~AcctScope() { g_acct = prev; }
+  } acct(e ? &e->dev_bytes : g_acct);
   try {
     f();
     return FEXPM_OK;
@@ -1993,22 +2053,26 @@ static void staged_d2h(HostStaging& hs, void* h, const void* d, size_t bytes, hi
 
 static void ensure_io(fexpm_engine* e) {
   const size_t bytes = (size_t)e->n * (size_t)e->n * e->esize;
-  if (!e->d_in) {
-    FX_HIP(hipMalloc(&e->d_in, std::max<size_t>(bytes, 16)));
-  }
-  if (!e->d_out) {
-    FX_HIP(hipMalloc(&e->d_out, std::max<size_t>(bytes, 16)));
-  }
+  if (!e->d_in) e->d_in = dmalloc_bytes(std::max<size_t>(bytes, 16));
+  if (!e->d_out) e->d_out = dmalloc_bytes(std::max<size_t>(bytes, 16));
+}
+
+static double* ensure_h(fexpm_engine* e) {
+  if (!e->d_h) e->d_h = static_cast<double*>(dmalloc_bytes((size_t)e->n * (size_t)e->n * sizeof(double)));
+  return e->d_h;
 }
 
 static void release(fexpm_engine* e) {
+  int64_t* prev = g_acct;
+  g_acct = &e->dev_bytes;
   staging_free(e->stg);
   e->pipe.reset();
-  if (e->d_in) (void)hipFree(e->d_in);
-  if (e->d_out) (void)hipFree(e->d_out);
+  dfree(e->d_in);
+  dfree(e->d_out);
   e->d_in = e->d_out = nullptr;
-  if (e->d_h) (void)hipFree(e->d_h);
+  dfree(e->d_h);
   e->d_h = nullptr;
+  g_acct = prev;
   e->have_graph = false;
   if (e->stream) (void)hipStreamDestroy(e->stream);
   e->stream = nullptr;
@@ -2242,7 +2306,7 @@ int fexpm_set_graph(fexpm_engine* e, const void* adj, int adj_dtype, double gamm
     const int n = (int)e->n;
     const size_t cnt = (size_t)n * n, asz = adj_dtype == FEXPM_FLOAT32 ? 4 : 8;
     ensure_io(e);
-    if (!e->d_h) FX_HIP(hipMalloc(&e->d_h, cnt * sizeof(double)));
+    ensure_h(e);
     // the adjacency is staged in the input buffer (>= n^2 * 8 bytes for complex
     // engines), which fexpm_run_graph overwrites with A_exp anyway
     FX_HIP(hipMemcpyAsync(e->d_in, adj, cnt * asz, hipMemcpyHostToDevice, e->stream));
@@ -2311,7 +2375,7 @@ int fexpm_set_graph_edges(fexpm_engine* e, int64_t m, const int64_t* src, const 
     const int64_t cnt = (int64_t)n * n;
     e->have_graph = false;
     ensure_io(e);  // fexpm_run_graph writes A_exp into the input buffer
-    if (!e->d_h) FX_HIP(hipMalloc(&e->d_h, (size_t)cnt * sizeof(double)));
+    ensure_h(e);
     fill_zero<<<(unsigned)std::min<int64_t>((cnt + 255) / 256, 65536), 256, 0, e->stream>>>(e->d_h, cnt);
     FX_LAUNCH();
     if (m > 0) {
@@ -2347,7 +2411,7 @@ int fexpm_set_graph_er(fexpm_engine* e, double p, uint64_t seed, int weighted, d
     const int n = (int)e->n;
     e->have_graph = false;
     ensure_io(e);  // fexpm_run_graph writes A_exp into the input buffer
-    if (!e->d_h) FX_HIP(hipMalloc(&e->d_h, (size_t)n * n * sizeof(double)));
+    ensure_h(e);
     // hash streams 1 (edges) and 2 (weights), as in fastctqw_amd/graphs.py
     const uint64_t ke = hash_stream_key(seed, 1), kw = hash_stream_key(seed, 2);
     dim3 grid((unsigned)((n + 255) / 256), (unsigned)n);
@@ -2616,6 +2680,7 @@ int fexpm_get_stats(const fexpm_engine* e, fexpm_stats* out) {
   if (!e || !out) return FEXPM_EINVAL;
   *out = e->stats;
   out->ms_propagate = e->pipe ? e->pipe->last_prop_ms : 0.0;
+  out->device_bytes = e->dev_bytes;
   return FEXPM_OK;
 }
 

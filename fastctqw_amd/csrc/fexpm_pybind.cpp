@@ -74,6 +74,7 @@ static py::dict stats_dict(const fexpm_stats& s) {
   d["s_onenorm"] = s.s_onenorm;
   d["rho_bound"] = s.rho_bound;
   d["s_two_product"] = s.s_two_product;
+  d["device_bytes"] = s.device_bytes;
   d["ms_propagate"] = s.ms_propagate;
   return d;
 }

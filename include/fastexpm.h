@@ -128,6 +128,9 @@ typedef struct fexpm_stats {
   double rho_bound;     /* spectral-radius bound of the shifted (deflated) B, when spectral = 1 */
   int32_t s_two_product;  /* leading squarings done in the two-product form (FEXPM_OPT_TWO_PRODUCT) */
   int32_t reserved1;
+  int64_t device_bytes;  /* device memory the engine holds now (work planes, slices, I/O buffers, the
+                            walk Hamiltonian); the FP64 planes of a complex64 engine are allocated
+                            at its first run */
 } fexpm_stats;
 
 /* Create an engine for n x n matrices of the given dtype and allocate its

@@ -671,6 +671,40 @@ def test_zero_matrix_and_nonfinite(fx):
         assert e.stats()["nonfinite"]
 
 
+def test_fp32_pade_path_and_device_bytes(fx):
+    """Complex64 engines run the Pade products and the solve on their FP64
+    companion by default; FEXPM_OPT_PADE_FP64 = 0 runs them in the engine's
+    own FP32 planes, which are allocated at the first such run.  Both routes
+    match the oracle, and device_bytes follows the allocations."""
+    from fastctqw_amd.fastexpm import _fastexpm_core as core
+
+    n = 300
+    _say(f"fp32 pade route n={n}")
+    g = (-1j * 1.3 * laplacian(erdos_renyi(n, 0.3, seed=31))).astype(np.complex64)
+    rng = np.random.default_rng(32)
+    h = ((rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) * (2.0 / np.sqrt(n))).astype(np.complex64)
+    with fx.ExpMatComplex64(n) as e:
+        b0 = e.stats()["device_bytes"]
+        assert b0 > 0
+        e.run(g)
+        b1 = e.stats()["device_bytes"]
+        assert b1 > b0  # the FP64 companion
+        e._backend.set_option(core.OPT_PADE_FP64, 0)
+        for name, a in (("structured", g), ("general", h)):
+            st = pade_expm(a.astype(np.complex128), stages=True, spectral=True)
+            u = e.run(a)
+            stats = e.stats()
+            assert stats["pade_fp64"] == 0
+            err = rel_fro(st["E"], u)
+            tol = 4.0 * _tol(np.complex64, stats["s"])
+            print(f"  {name}: m={stats['m']} s={stats['s']} relF={err:.3e} tol={tol:.1e}", flush=True)
+            assert err <= tol, (name, err)
+        b2 = e.stats()["device_bytes"]
+        assert b2 > b1  # the FP32 Pade and solve buffers
+        e.free()
+        assert e.stats()["device_bytes"] == 0
+
+
 def test_free_idempotent_and_errors(fx):
     _say("free / errors")
     e = fx.ExpMatComplex64(8)
