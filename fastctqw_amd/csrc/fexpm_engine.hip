@@ -1020,7 +1020,14 @@ template <typename W> struct Pipeline : PipelineBase {
       form_system<W><<<grid, 256, 0, st>>>(u.r, u.i, v.r, v.i, ld, n, inp, lr, li, ldl, structured ? 1 : 0);
       FX_LAUNCH();
     }
-    flops += lu_factor_solve(lr, li, ldl, inp, tr, ti, ipiv, cand, gmax, top, scal, inv, ctrs, st, panel_coop != 0);
+    if (!structured && n <= GJ_NMAX) {
+      // small orders: one workgroup eliminates the whole system in LDS
+      gj_solve_small<<<1, 1024, 0, st>>>(lr, li, ldl, n, inp, scal);
+      FX_LAUNCH();
+    } else {
+      flops += lu_factor_solve(lr, li, ldl, inp, tr, ti, ipiv, cand, gmax, top, scal, inv, ctrs, st,
+                               panel_coop != 0);
+    }
     if (q64 <= 0) return solution_tail(lr + inp, li + inp, ldl, lr, li, structured ? 1 : 0, 0, st);
     return solution_tail(lr + inp, li + inp, ldl, lr, li, 1, q64, st);
   }

@@ -301,6 +301,96 @@ __global__ void prepare_planar(const Tin* __restrict__ a, int n, W* __restrict__
   }
 }
 
+// Small general systems: Gauss-Jordan elimination with partial pivoting of
+// the complex n x 2n system [M | B] (n <= GJ_NMAX) held in LDS by one
+// workgroup, replacing the blocked LU (a cooperative launch and about 30
+// launches per 64 columns, 1.3 ms at n <= 128) for the orders where that is
+// all launch overhead.  Input and output are the planar FP64 system of the
+// LU route: M in columns 0..n-1, B in columns inp..inp+n-1 of rows 0..n-1
+// (leading dimension ldl); on return the B columns hold X = M^-1 B.  Pivot:
+// the largest |re| + |im| in the column (lowest row on ties); an exactly
+// zero pivot sets info bit 1 and leaves X undefined, as the LU does.
+constexpr int GJ_NMAX = 64;
+__global__ __launch_bounds__(1024) void gj_solve_small(double* __restrict__ lr, double* __restrict__ li,
+                                                       int64_t ldl, int n, int inp, Scalars* __restrict__ sc) {
+  constexpr int RS = 2 * GJ_NMAX + 1;  // LDS row stride (complex elements)
+  __shared__ double2 s[GJ_NMAX * RS];
+  __shared__ int piv;
+  __shared__ int singular;
+  const int t = threadIdx.x, w = 2 * n;
+  if (t == 0) singular = 0;
+  for (int q = t; q < n * w; q += 1024) {
+    const int r = q / w, c = q % w;
+    const int64_t o = (int64_t)r * ldl + (c < n ? c : inp + (c - n));
+    s[r * RS + c] = make_double2(lr[o], li[o]);
+  }
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    if (t < 64) {  // wave 0: pivot row, argmax of |re| + |im| over rows k..n-1 (n <= 64: one row per lane)
+      const int r = k + t;
+      double v = -1.0;
+      if (r < n) {
+        const double2 x = s[r * RS + k];
+        v = fabs(x.x) + fabs(x.y);
+      }
+      int best = r;
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ov = __shfl_xor(v, off);
+        const int ob = __shfl_xor(best, off);
+        if (ov > v || (ov == v && ob < best)) {
+          v = ov;
+          best = ob;
+        }
+      }
+      if (t == 0) {
+        piv = best;
+        if (!(v > 0.0)) singular = 1;
+      }
+    }
+    __syncthreads();
+    const int p = piv;
+    if (p != k) {  // columns < k of rows k and p are zero
+      for (int c = k + t; c < w; c += 1024) {
+        const double2 a = s[k * RS + c];
+        s[k * RS + c] = s[p * RS + c];
+        s[p * RS + c] = a;
+      }
+    }
+    __syncthreads();
+    const double2 d = s[k * RS + k];
+    const double dd = d.x * d.x + d.y * d.y;
+    const double2 inv = dd > 0.0 ? make_double2(d.x / dd, -d.y / dd) : make_double2(0.0, 0.0);
+    __syncthreads();  // every thread has read the pivot before row k changes
+    for (int c = k + 1 + t; c < w; c += 1024) {
+      const double2 a = s[k * RS + c];
+      s[k * RS + c] = make_double2(a.x * inv.x - a.y * inv.y, a.x * inv.y + a.y * inv.x);
+    }
+    __syncthreads();
+    // eliminate column k from every other row (columns > k; column k itself is
+    // never read again)
+    const int span = w - k - 1;
+    for (int q = t; q < n * span; q += 1024) {
+      const int r = q / span, c = k + 1 + q % span;
+      if (r == k) continue;
+      const double2 f = s[r * RS + k];
+      const double2 b = s[k * RS + c];
+      double2 x = s[r * RS + c];
+      x.x -= f.x * b.x - f.y * b.y;
+      x.y -= f.x * b.y + f.y * b.x;
+      s[r * RS + c] = x;
+    }
+    __syncthreads();
+  }
+  for (int q = t; q < n * n; q += 1024) {
+    const int r = q / n, c = q % n;
+    const double2 x = s[r * RS + n + c];
+    const int64_t o = (int64_t)r * ldl + inp + c;
+    lr[o] = x.x;
+    li[o] = x.y;
+  }
+  if (t == 0 && singular) sc->info |= 1;
+}
+
 // out = sum_t c_t P_t + diag*I on an n x n planar region (im plane optional).
 template <typename W> struct Lincomb {
   W* out_r;
