@@ -404,7 +404,10 @@ static double lu_factor_solve(double* lr, double* li, int64_t ldl, int inp, doub
     // --- outer panel S[K0:, K0:K1] with 64-wide inner panels ---
     for (int k0 = K0; k0 < K1; k0 += nb) {
       const int G = (inp - k0 + PANEL_R - 1) / PANEL_R;
-      if (coop && g_panel_coop && G <= panel_coop_capacity()) {
+      if (inp - k0 <= PANEL_R1) {  // the panel's rows fit one workgroup
+        lu_panel_single<<<1, 256, 0, st>>>(lr, li, ldl, inp, k0, ipiv, scal);
+        FX_LAUNCH();
+      } else if (coop && g_panel_coop && G <= panel_coop_capacity()) {
         unsigned* ctr = ctrs + k0 / LU_NB;
         void* kargs[] = {&lr, &li, (void*)&ldl, &inp, (void*)&k0, (void*)&G, &cand, &top, &ipiv, &scal, &ctr};
         FX_HIP(hipLaunchCooperativeKernel((const void*)lu_panel_coop, dim3(G), dim3(256), kargs, 0, st));

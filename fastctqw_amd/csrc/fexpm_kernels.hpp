@@ -1183,6 +1183,101 @@ __global__ __launch_bounds__(256) void lu_panel_coop(double* __restrict__ sr, do
   }
 }
 
+// The panel of lu_panel_coop when its rows fit one workgroup (np - k0 <=
+// PANEL_R1 = 128 rows, 133 KB of LDS): the same pivot rule (largest
+// |re| + |im| among rows >= c, lowest row on ties), interchanges, scaling
+// and rank-1 updates, element by element with the same operations, but
+// with workgroup barriers instead of grid barriers (64 of those cost about
+// 0.6 ms per panel at any order).  One ordinary launch, usable by
+// concurrent lanes as well.
+constexpr int PANEL_R1 = 128;
+__global__ __launch_bounds__(256) void lu_panel_single(double* __restrict__ sr, double* __restrict__ si, int64_t ld,
+                                                       int np, int k0, int* __restrict__ ipiv, Scalars* sc) {
+  __shared__ double rr[PANEL_R1][LU_NB + 1];
+  __shared__ double ri[PANEL_R1][LU_NB + 1];
+  __shared__ double pr[LU_NB], pi[LU_NB];
+  __shared__ int s_piv;
+  const int t = threadIdx.x;
+  const int nrow = min(np - k0, PANEL_R1);
+  for (int e = t; e < nrow * LU_NB; e += 256) {
+    const int lr = e / LU_NB, col = e % LU_NB;
+    rr[lr][col] = sr[(int64_t)(k0 + lr) * ld + k0 + col];
+    ri[lr][col] = si[(int64_t)(k0 + lr) * ld + k0 + col];
+  }
+  __syncthreads();
+  for (int j = 0; j < LU_NB; ++j) {
+    const int c = k0 + j;
+    // 1. pivot among rows >= c (wave 0: rows t and t + 64, then shuffles)
+    if (t < 64) {
+      double v = -1.0;
+      int k = -1;
+      for (int lr = t; lr < nrow; lr += 64) {
+        if (k0 + lr >= c) {
+          double x = cabs1(rr[lr][j], ri[lr][j]);
+          int kx = k0 + lr;
+          if (!(x >= 0.0)) x = -1.0, kx = -1;
+          argmax_merge(v, k, x, kx);
+        }
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        const double v2 = __shfl_xor(v, off);
+        const int k2 = __shfl_xor(k, off);
+        argmax_merge(v, k, v2, k2);
+      }
+      if (t == 0) {
+        const int p = k >= 0 ? k : c;
+        s_piv = p;
+        ipiv[c] = p;
+        if (!(v > 0.0)) atomicOr(&sc->info, 1);
+      }
+    }
+    __syncthreads();
+    const int p = s_piv;
+    // 2. interchange rows c and p of the panel; pr / pi = the pivot row
+    if (t < LU_NB) {
+      const double ar = rr[p - k0][t], ai = ri[p - k0][t];
+      if (p != c) {
+        rr[p - k0][t] = rr[c - k0][t];
+        ri[p - k0][t] = ri[c - k0][t];
+        rr[c - k0][t] = ar;
+        ri[c - k0][t] = ai;
+      }
+      pr[t] = ar;
+      pi[t] = ai;
+    }
+    __syncthreads();
+    // 3. scale and rank-1 update of the rows below c
+    const double dr = pr[j], di = pi[j];
+    const double den = dr * dr + di * di;
+    const bool ok = den > 0.0;
+    const double qr = ok ? dr / den : 0.0, qi = ok ? -di / den : 0.0;
+    const int lr = t >> 1, half = t & 1;
+    const bool upd = lr < nrow && k0 + lr > c;
+    if (upd) {
+      const double ar = rr[lr][j], ai = ri[lr][j];
+      const double lre = ar * qr - ai * qi, lim = ar * qi + ai * qr;
+      for (int col = half * 32; col < half * 32 + 32; ++col) {
+        if (col > j) {
+          rr[lr][col] -= lre * pr[col] - lim * pi[col];
+          ri[lr][col] -= lre * pi[col] + lim * pr[col];
+        }
+      }
+    }
+    __syncthreads();
+    if (upd && half == 0) {
+      const double ar = rr[lr][j], ai = ri[lr][j];
+      rr[lr][j] = ar * qr - ai * qi;
+      ri[lr][j] = ar * qi + ai * qr;
+    }
+    __syncthreads();
+  }
+  for (int e = t; e < nrow * LU_NB; e += 256) {
+    const int lr = e / LU_NB, col = e % LU_NB;
+    sr[(int64_t)(k0 + lr) * ld + k0 + col] = rr[lr][col];
+    si[(int64_t)(k0 + lr) * ld + k0 + col] = ri[lr][col];
+  }
+}
+
 // Apply the panel's row interchanges to columns [c0, c1).
 __global__ void lu_laswp(double* __restrict__ sr, double* __restrict__ si, int64_t ld, int k0, int nb,
                          const int* __restrict__ ipiv, int c0, int c1) {
