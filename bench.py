@@ -1034,7 +1034,7 @@ def main():
                     help="c2: N=10^4 single expm (default); c3: precision study at N=10^4; "
                          "c4: 256 uneven walks; c5: 32 times x 128 states, N=8192; general: dense general "
                          "complex64 expm at N (LU route) and the reference's performance inputs at 16..1024")
-    ap.add_argument("--precision", choices=list(C3_PRECISIONS), default="native",
+    ap.add_argument("--precision", choices=list(C3_PRECISIONS) + ["int8crt"], default="native",
                     help="c2: squaring precision of the complex64 engine")
     ap.add_argument("--no-split", action="store_true",
                     help="c2: skip the fp16x2 split-precision comparison run")

@@ -1,0 +1,398 @@
+// FP32-level products on the int8 MFMA of gfx950 by exact residue
+// arithmetic: the integer-modular form of the Ozaki scheme (Ozaki, Uchino &
+// Imamura, "Ozaki Scheme II: a GEMM-oriented emulation of floating-point
+// matrix multiplication using an integer modular technique", 2024).
+//
+// Used for the structured squarings X^2 = X X^T of complex64 engines, with
+// X = diag(d) + Y (Y = Yr + i Yi symmetric, zero diagonal, FP32 planes):
+//
+//   1. Every row r of Y is scaled by 2^(BETA - e_r), e_r the exponent with
+//      max_k max(|Yr_rk|, |Yi_rk|) < 2^e_r, and rounded to integers
+//      yr', yi' (|.| <= 2^BETA); s' = yr' + yi'.
+//   2. For each modulus m of a set of pairwise coprime odd moduli (product
+//      M), the symmetric residues of yr', yi', s' (int8 in [-127, 127]) are
+//      the operands of three int8 products T1 = yr' yr'^T, T2 = yi' yi'^T,
+//      T3 = s' s'^T mod m.  Their int32 sums are exact (k 127^2 < 2^31).
+//   3. Re = T1 - T2 and Im = T3 - T1 - T2 are formed mod m (exact: the 3M
+//      combination costs no accuracy here), and the Chinese remainder theorem
+//      gives the integers Re, Im themselves: |Re|, |Im| <= 2^(2 BETA + 1) k
+//      < M / 2.  X/M = frac(sum_m s_m / m), s_m = (residue * u_m) mod m with
+//      u_m = (M / m)^-1 mod m, evaluated in FP64 (error about 2^-48 of M,
+//      i.e. 2^23 integer units against values of about 2^56).
+//   4. (Y Y^T)_ij = 2^(e_i + e_j - 2 BETA) (Re + i Im), then the diagonal-split
+//      epilogue of the squaring (FP64 diagonal d) and the FP32 planes.
+//
+// The only rounding is that of step 1 (BETA = 27 bits of each row's
+// largest element) and of the FP32 output; the products and their sums are
+// exact, unlike FP32 MFMA accumulation.  Nine moduli of about 2^7.9 (M of
+// 71.2 bits) leave room for BETA = 27 with k <= 16384.
+//
+// Layouts.  Operand residues ("fragment layout"): for a k-block kb (32 values
+// of k) and a group g of 32 rows, one 1 KB fragment at ((kb * G + g) * 1024)
+// bytes of its plane (G = rows / 32), in which lane l of a wave owns bytes
+// 16 l .. 16 l + 15: row 32 g + l % 32, k values 32 kb + 16 (l / 32) .. + 15,
+// which is the operand layout of v_mfma_i32_32x32x32_i8; one
+// global_load_lds_dwordx4 of a wave moves one fragment into LDS unchanged.
+// Product residues: one byte (C mod m in [0, m)) per element of the lower
+// tiles, stored transposed, at byte (col * rows + row) of its plane.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fexpm_gemm.hpp"
+
+namespace fexpm {
+
+constexpr int CRT_K = 9;         // moduli of the squaring products
+constexpr int CRT_BETA = 27;     // bits kept of each row's largest element
+constexpr int CRT_KMAX = 16384;  // longest k (keeps every bound above)
+constexpr int CRT_TILE = 256;    // workgroup tile edge of the residue products
+constexpr int CRT_KPAD = 64;     // k granularity (one LDS stage: two k-blocks)
+constexpr int CRT_NPROB = 3 * CRT_K;
+
+// Moduli and reconstruction constants (filled on the host by crt_table()).
+struct CrtTable {
+  uint32_t m[CRT_K];
+  uint32_t magic[CRT_K];  // ceil(2^39 / m): floor(u / m) = mulhi(u, magic) >> 7 for 0 <= u < 2^31
+  uint32_t inv[CRT_K];    // ((M / m)^-1) mod m
+  double rcp[CRT_K];      // 1 / m
+  double M;               // product of the moduli (rounded)
+};
+
+// u mod m for 0 <= u < 2^31 (exact: the error term of the rounded-up
+// reciprocal stays below 1 / m of the quotient's fraction).
+__device__ __forceinline__ uint32_t crt_umod(uint32_t u, uint32_t m, uint32_t magic) {
+  const uint32_t q = __umulhi(u, magic) >> 7;
+  return u - q * m;
+}
+
+// Symmetric residue in [-(m-1)/2, (m-1)/2] of an integer |x| <= 2^29 (odd m).
+__device__ __forceinline__ int crt_sres(int x, uint32_t m, uint32_t magic) {
+  const uint32_t u = (uint32_t)(x + (int)(m << 21));  // m 2^21 > 2^29 >= |x| for m >= 227... and u < 2^31
+  const int r = (int)crt_umod(u, m, magic);
+  return r > (int)(m >> 1) ? r - (int)m : r;
+}
+
+// Row exponents of a complex FP32 matrix (planes yr, yi, leading dimension
+// ld, padding columns zero): e[r] = e with max_k max(|yr_rk|, |yi_rk|) < 2^e
+// (0 for a zero row), rows r < n.  One wave per row, four rows per
+// workgroup of 256 threads.
+__global__ __launch_bounds__(256) void crt_row_exp(const float* __restrict__ yr, const float* __restrict__ yi,
+                                                    int64_t ld, int n, int* __restrict__ e) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r = (int)blockIdx.x * 4 + w;
+  if (r >= n) return;  // whole waves leave; no workgroup barrier follows
+  const int nv = (n + 3) & ~3;  // padding columns up to the next multiple of 4 are zero
+  const float* pr = yr + (int64_t)r * ld;
+  const float* pi = yi + (int64_t)r * ld;
+  float mx = 0.0f;
+  for (int k = 4 * lane; k < nv; k += 256) {
+    const float4 a = *reinterpret_cast<const float4*>(pr + k);
+    const float4 b = *reinterpret_cast<const float4*>(pi + k);
+    mx = fmaxf(mx, fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fmaxf(fabsf(a.z), fabsf(a.w))));
+    mx = fmaxf(mx, fmaxf(fmaxf(fabsf(b.x), fabsf(b.y)), fmaxf(fabsf(b.z), fabsf(b.w))));
+  }
+  for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+  int ex = 0;
+  if (mx > 0.0f) (void)frexpf(mx, &ex);  // mx = f 2^ex, f in [0.5, 1)
+  if (lane == 0) e[r] = ex;
+}
+
+// Operand residues of the structured squaring (3 * CRT_K planes of
+// rows x kpad bytes in the fragment layout: plane o * CRT_K + i holds
+// operand o (0: yr', 1: yi', 2: s') mod moduli i).  One wave per fragment
+// (k-block kb, row group g), four per workgroup; rows >= n and k >= n are
+// zero.  rows is a multiple of CRT_TILE, kpad of CRT_KPAD, kpad <= ld.
+__global__ __launch_bounds__(256) void crt_residues_sq(const float* __restrict__ yr, const float* __restrict__ yi,
+                                                        int64_t ld, int n, const int* __restrict__ e, int rows,
+                                                        int kpad, CrtTable tab, int8_t* __restrict__ out) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int G = rows / 32;
+  const int64_t jobs = (int64_t)G * (kpad / 32);
+  const int64_t job = (int64_t)blockIdx.x * 4 + w;
+  if (job >= jobs) return;
+  const int kb = (int)(job / G), g = (int)(job % G);
+  const int row = 32 * g + (lane & 31), k0 = 32 * kb + 16 * (lane >> 5);
+  int xr[16], xi[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    xr[q] = 0;
+    xi[q] = 0;
+  }
+  if (row < n) {
+    const float sc = ldexpf(1.0f, CRT_BETA - e[row]);  // exact power of two; |y sc| < 2^BETA
+    const float* pr = yr + (int64_t)row * ld;
+    const float* pi = yi + (int64_t)row * ld;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int k = k0 + q;
+      if (k < n) {
+        xr[q] = (int)rintf(pr[k] * sc);
+        xi[q] = (int)rintf(pi[k] * sc);
+      }
+    }
+  }
+  const int64_t plane = (int64_t)rows * kpad;
+  const int64_t off = ((int64_t)kb * G + g) * 1024 + lane * 16;
+#pragma unroll
+  for (int i = 0; i < CRT_K; ++i) {
+    const uint32_t m = tab.m[i], mg = tab.magic[i];
+    uint32_t pk[3][4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      uint32_t a = 0, b = 0, s = 0;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int q = 4 * c + t;
+        a |= ((uint32_t)crt_sres(xr[q], m, mg) & 0xFFu) << (8 * t);
+        b |= ((uint32_t)crt_sres(xi[q], m, mg) & 0xFFu) << (8 * t);
+        s |= ((uint32_t)crt_sres(xr[q] + xi[q], m, mg) & 0xFFu) << (8 * t);
+      }
+      pk[0][c] = a;
+      pk[1][c] = b;
+      pk[2][c] = s;
+    }
+#pragma unroll
+    for (int o = 0; o < 3; ++o) {
+      uint4 v;
+      v.x = pk[o][0];
+      v.y = pk[o][1];
+      v.z = pk[o][2];
+      v.w = pk[o][3];
+      *reinterpret_cast<uint4*>(out + (int64_t)(o * CRT_K + i) * plane + off) = v;
+    }
+  }
+}
+
+// One product of a batched residue launch: C = A B^T over the lower tiles,
+// written as residues mod m.
+struct CrtProb {
+  const int8_t* a;  // operand planes (fragment layout, `rows` rows)
+  const int8_t* b;
+  uint8_t* out;     // rows x rows bytes, element (row, col) at col * rows + row
+  uint32_t m;
+  uint32_t magic;
+};
+
+__device__ __forceinline__ void crt_dma16(const int8_t* g, int8_t* l) {
+  __builtin_amdgcn_global_load_lds((const void*)g, (__attribute__((address_space(3))) void*)l, 16, 0, 0);
+}
+// s_waitcnt vmcnt(N) alone (expcnt and lgkmcnt at their maxima), gfx9 encoding.
+template <int N> __device__ __forceinline__ void crt_wait_vm() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+// Batched residue products.  Workgroup: a 256 x 256 output tile of one
+// problem, 4 waves (one per SIMD) of 128 x 128 (4 x 4 blocks of 32 x 32 on
+// v_mfma_i32_32x32x32_i8, 256 accumulator registers).  Operand stages of two
+// k-blocks (16 KB each: 8 A fragments then 8 B fragments) move into a ring of
+// CRT_RING LDS buffers by direct-to-LDS copies issued CRT_RING - 1 stages
+// ahead, interleaved with the MFMAs (each wave copies its share: fragments
+// wave and wave + 4 of A and of B).  At the end of step s every wave waits for
+// its copies of stage s + 2 and meets the others at a barrier, so stages
+// <= s + 2 are complete in LDS and stage s - 1's buffer is free for the copy
+// issued in step s + 1; the first fragments of stage s + 1 are read before
+// that barrier.  Past the last stage the copies repeat the last k-block into a
+// buffer no wave reads, so the loop has no branches around them.
+constexpr int CRT_RING = 4;
+__global__ __launch_bounds__(256, 1) void crt_gemm_kernel(const CrtProb* __restrict__ probs, int tiles_per, int T,
+                                                          int G, int KB) {
+  constexpr int KBS = 2;
+  constexpr int KSTAGE = 16384;
+  constexpr int SB = KSTAGE * KBS;
+  constexpr int PER = 4 * KBS;  // copies per wave per stage
+  __shared__ __attribute__((aligned(1024))) int8_t sm[CRT_RING * SB];
+  const int total = (int)gridDim.x;
+  const int lin = xcd_remap((int)blockIdx.x, total);
+  const int p = lin / tiles_per;
+  int tm, tn;
+  grouped_tri_tile(lin - p * tiles_per, T, tm, tn);
+  const CrtProb P = probs[p];
+  const int t = threadIdx.x, lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int wm = wave >> 1, wn = wave & 1;
+  typedef int i32x4v __attribute__((ext_vector_type(4)));
+  typedef int i32x16v __attribute__((ext_vector_type(16)));
+  i32x16v acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = i32x16v{};
+  const int8_t* a0 = P.a + (int64_t)tm * 8192 + lane * 16;
+  const int8_t* b0 = P.b + (int64_t)tn * 8192 + lane * 16;
+  const int64_t kstride = (int64_t)G * 1024;  // bytes per k-block of a plane
+  auto issue_kb = [&](int s, int kbl) {
+    int8_t* buf = sm + (s % CRT_RING) * SB + kbl * KSTAGE;
+    int64_t kb = (int64_t)s * KBS + kbl;
+    kb = kb < KB - 1 ? kb : KB - 1;
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int gq = wave + 4 * u;  // A group gq and B group gq (wave < 4)
+      crt_dma16(a0 + kb * kstride + gq * 1024, buf + gq * 1024);
+      crt_dma16(b0 + kb * kstride + gq * 1024, buf + 8192 + gq * 1024);
+    }
+  };
+  i32x4v fa[2][4], fb[2][4];
+  auto frags = [&](int s, int kbl, int slot) {
+    const int8_t* q = sm + (s % CRT_RING) * SB + kbl * KSTAGE;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) fa[slot][i] = *reinterpret_cast<const i32x4v*>(q + (wm * 4 + i) * 1024 + lane * 16);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      fb[slot][j] = *reinterpret_cast<const i32x4v*>(q + 8192 + (wn * 4 + j) * 1024 + lane * 16);
+  };
+  const int NS = KB / KBS;
+#pragma unroll
+  for (int q = 0; q <= CRT_RING - 2; ++q)
+#pragma unroll
+    for (int kbl = 0; kbl < KBS; ++kbl) issue_kb(q, kbl);
+  crt_wait_vm<(CRT_RING - 3) * PER>();  // stages 0 and 1 landed
+  __builtin_amdgcn_s_barrier();
+  frags(0, 0, 0);
+  int slot = 0;
+  for (int s = 0; s < NS; ++s) {
+#pragma unroll
+    for (int kbl = 0; kbl < KBS; ++kbl) {
+      if (kbl + 1 < KBS) frags(s, kbl + 1, slot ^ 1);
+      else frags(s + 1, 0, slot ^ 1);  // past the last stage: a stale buffer, values unused
+      issue_kb(s + CRT_RING - 1, kbl);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[slot][i], fb[slot][j], acc[i][j], 0, 0, 0);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // one LDS read
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
+        __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // one copy
+      }
+      __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
+      slot ^= 1;
+    }
+    crt_wait_vm<(CRT_RING - 3) * PER>();  // this wave's copies of stage s + 2 landed
+    __builtin_amdgcn_s_barrier();
+  }
+  crt_wait_vm<0>();  // no copy in flight when the workgroup's LDS is released
+  // epilogue: residues mod m, four consecutive rows per 32-bit store
+  const int l32 = lane & 31, kh = lane >> 5;
+  const int rows = G * 32;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int col = tn * CRT_TILE + wn * 128 + j * 32 + l32;
+#pragma unroll
+      for (int rq = 0; rq < 4; ++rq) {
+        uint32_t w4 = 0;
+#pragma unroll
+        for (int r3 = 0; r3 < 4; ++r3) {
+          // |acc| <= k 127^2 < 2^28 and m 2^21 > 2^28: u in [0, 2^31)
+          const uint32_t u = (uint32_t)(acc[i][j][4 * rq + r3] + (int)(P.m << 21));
+          w4 |= crt_umod(u, P.m, P.magic) << (8 * r3);
+        }
+        const int row = tm * CRT_TILE + wm * 128 + i * 32 + 8 * rq + 4 * kh;
+        *reinterpret_cast<uint32_t*>(P.out + (int64_t)col * rows + row) = w4;
+      }
+    }
+}
+
+// (residue * u) mod m for residue in [0, 3m): below 2^31 for m <= 255.
+__device__ __forceinline__ uint32_t crt_digit(uint32_t r, uint32_t inv, uint32_t m, uint32_t magic) {
+  return crt_umod(r * inv, m, magic);
+}
+
+// X^2 of the split form from the residues of T1, T2, T3 (planes p, CRT_K + p,
+// 2 CRT_K + p for modulus p; each rows x rows bytes at col * rows + row),
+// for the 64 x 64 lower tiles (TI >= TJ, lower-triangular index blockIdx.x):
+//   (Y Y^T)_ij = 2^(e_i + e_j - 2 BETA) (Re + i Im) by CRT, then
+//   i != j: X^2_ij = (d_i + d_j) Y_ij + (Y Y^T)_ij (complex),
+//   i == j: d_out_i = d_i^2 + (Y Y^T)_ii and X^2_ii = 0 in the planes,
+// written at (i, j) and (j, i) of cr / ci (rows and columns < n only).
+// Thread t works on rows 4 (t % 16) .. + 3 of column (t / 16) + 16 c.
+__global__ __launch_bounds__(256) void crt_combine_sq(const uint8_t* __restrict__ res, int rows,
+                                                       const int* __restrict__ e, const float* __restrict__ yr,
+                                                       const float* __restrict__ yi, int64_t ld, int n,
+                                                       const double2* __restrict__ d_in, double2* __restrict__ d_out,
+                                                       float* __restrict__ cr, float* __restrict__ ci, CrtTable tab) {
+  __shared__ float tre[64][65];
+  __shared__ float tim[64][65];
+  int TI, TJ;
+  tri_tile((int)blockIdx.x, TI, TJ);
+  const int i0 = TI * 64, j0 = TJ * 64;
+  const int t = threadIdx.x;
+  const int il = 4 * (t & 15);
+  const int64_t plane = (int64_t)rows * rows;
+#pragma unroll 1
+  for (int c = 0; c < 4; ++c) {
+    const int jl = (t >> 4) + 16 * c;
+    const int j = j0 + jl;
+    const int ib = i0 + il;
+    if (j >= n || ib >= n) continue;
+    // residues of the 4 rows ib .. ib + 3 in column j, per product and modulus
+    uint32_t w[3][CRT_K];
+#pragma unroll
+    for (int o = 0; o < 3; ++o)
+#pragma unroll
+      for (int q = 0; q < CRT_K; ++q)
+        w[o][q] = *reinterpret_cast<const uint32_t*>(res + (o * CRT_K + q) * plane + (int64_t)j * rows + ib);
+    const double2 dj = d_in[j];
+    const int ej = e[j];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = ib + r;
+      if (i >= n || i < j) continue;  // lower part only (diagonal tiles hold both)
+      double yre = 0.0, yim = 0.0;
+#pragma unroll
+      for (int q = 0; q < CRT_K; ++q) {
+        const uint32_t m = tab.m[q], mg = tab.magic[q];
+        const uint32_t t1 = (w[0][q] >> (8 * r)) & 0xFFu;
+        const uint32_t t2 = (w[1][q] >> (8 * r)) & 0xFFu;
+        const uint32_t t3 = (w[2][q] >> (8 * r)) & 0xFFu;
+        const uint32_t re = t1 + m - t2;          // [0, 2m)
+        const uint32_t im = t3 + 2 * m - t1 - t2;  // [0, 3m)
+        yre += (double)crt_digit(re, tab.inv[q], m, mg) * tab.rcp[q];
+        yim += (double)crt_digit(im, tab.inv[q], m, mg) * tab.rcp[q];
+      }
+      const double sc = ldexp(tab.M, e[i] + ej - 2 * CRT_BETA);
+      double vr = (yre - rint(yre)) * sc;
+      double vi = (yim - rint(yim)) * sc;
+      if (i == j) {
+        const double2 dd = d_in[i];
+        d_out[i] = make_double2(dd.x * dd.x - dd.y * dd.y + vr, 2.0 * dd.x * dd.y + vi);
+        vr = 0.0;
+        vi = 0.0;
+      } else {
+        const double2 di = d_in[i];
+        const int64_t oy = (int64_t)j * ld + i;  // Y is symmetric: Y_ij = Y_ji
+        const double y_r = (double)yr[oy], y_i = (double)yi[oy];
+        const double sr = di.x + dj.x, si = di.y + dj.y;
+        vr += sr * y_r - si * y_i;
+        vi += sr * y_i + si * y_r;
+      }
+      const float fr = (float)vr, fi = (float)vi;
+      cr[(int64_t)j * ld + i] = fr;  // (j, i): consecutive i across the threads of a column
+      ci[(int64_t)j * ld + i] = fi;
+      tre[il + r][jl] = fr;
+      tim[il + r][jl] = fi;
+    }
+  }
+  __syncthreads();
+  // (i, j) from the tile in LDS: thread t writes column j0 + (t % 64) of rows i0 + t / 64 + 4 c
+  const int jl = t & 63, j = j0 + jl;
+#pragma unroll 1
+  for (int c = 0; c < 16; ++c) {
+    const int il2 = (t >> 6) + 4 * c;
+    const int i = i0 + il2;
+    if (i >= n || j >= n || i < j) continue;
+    cr[(int64_t)i * ld + j] = tre[il2][jl];
+    ci[(int64_t)i * ld + j] = tim[il2][jl];
+  }
+}
+
+}  // namespace fexpm

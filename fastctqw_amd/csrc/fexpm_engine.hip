@@ -41,6 +41,7 @@
 #include "fexpm_kernels.hpp"
 #include "fexpm_split.hpp"
 #include "fexpm_ozaki.hpp"
+#include "fexpm_crt.hpp"
 
 namespace fexpm {
 
@@ -215,6 +216,36 @@ static void dfree(void* p) {
     }
   }
   (void)hipFree(p);
+}
+
+// Moduli and reconstruction constants of the residue squarings: the nine
+// largest pairwise coprime odd moduli below 256, u_m = (M / m)^-1 mod m.
+static const CrtTable& crt_table() {
+  static const CrtTable tab = [] {
+    static const uint32_t mods[CRT_K] = {255, 253, 251, 247, 241, 239, 233, 229, 227};
+    CrtTable t;
+    double M = 1.0;
+    for (int i = 0; i < CRT_K; ++i) M *= (double)mods[i];
+    t.M = M;
+    for (int i = 0; i < CRT_K; ++i) {
+      const uint32_t m = mods[i];
+      t.m[i] = m;
+      t.magic[i] = (uint32_t)(((1ull << 39) + m - 1) / m);
+      t.rcp[i] = 1.0 / (double)m;
+      uint64_t mi = 1;  // (M / m) mod m
+      for (int j = 0; j < CRT_K; ++j)
+        if (j != i) mi = (mi * mods[j]) % m;
+      uint32_t u = 0;  // its inverse mod m (m < 256: by search)
+      for (uint32_t c = 1; c < m; ++c)
+        if ((mi * c) % m == 1) {
+          u = c;
+          break;
+        }
+      t.inv[i] = u;
+    }
+    return t;
+  }();
+  return tab;
 }
 
 template <typename W> struct Plane2 {
@@ -491,6 +522,15 @@ template <typename W> struct Pipeline : PipelineBase {
   int* info_buf = nullptr;  // exception bits of the last run (device)
   void* sbuf = nullptr;  // k-blocked 16-bit split parts (split-precision squarings)
   size_t sbuf_bytes = 0;
+  // residue squarings (FEXPM_PREC_INT8_CRT): row exponents, operand residues
+  // (3 CRT_K planes of rows x kpad), product residues (3 CRT_K planes of
+  // rows x rows), the launch's problem list for the current order
+  int* crt_e = nullptr;
+  int8_t* crt_ops = nullptr;
+  uint8_t* crt_res = nullptr;
+  CrtProb* crt_probs = nullptr;
+  int crt_probs_n = -1;
+  std::vector<CrtProb> crt_hprobs;
   std::map<int, int*> sq_orders;  // grouped lower-triangular tile orders, per tile count
   hipEvent_t ev[8];
   bool events = false;
@@ -610,6 +650,10 @@ template <typename W> struct Pipeline : PipelineBase {
       dfree(oz_e[b]);
     }
     dfree(sbuf);
+    dfree(crt_e);
+    dfree(crt_ops);
+    dfree(crt_res);
+    dfree(crt_probs);
     for (auto& kv : sq_orders) dfree(kv.second);
     for (hipEvent_t e : zev) (void)hipEventDestroy(e);
     if (zst) (void)hipStreamDestroy(zst);
@@ -1327,6 +1371,64 @@ template <typename W> struct Pipeline : PipelineBase {
     return {cur, dc};
   }
 
+  // s structured squarings by exact int8 residue products (fexpm_crt.hpp):
+  // per squaring the row exponents of Y, the operand residues, one batched
+  // launch of the 3 CRT_K residue products over the lower 256 x 256 tiles,
+  // and the reconstruction with the diagonal-split epilogue into the other
+  // plane.  FP32 engines, n <= CRT_KMAX.
+  std::pair<Plane2<W>*, int> square_crt(int s, int dc, hipStream_t st) {
+    if constexpr (!std::is_same<W, float>::value) {
+      throw Error(FEXPM_EINVAL, "residue squarings apply to FP32 planes");
+    } else {
+      const int rows = (int)round_up(n, CRT_TILE), kpad = (int)round_up(n, CRT_KPAD);
+      const int64_t rows_cap = round_up(ncap, CRT_TILE), kpad_cap = round_up(ncap, CRT_KPAD);
+      if (!crt_e) {
+        crt_e = dalloc<int>((size_t)npcap);
+        crt_ops = static_cast<int8_t*>(dmalloc_bytes((size_t)CRT_NPROB * rows_cap * kpad_cap));
+        crt_res = static_cast<uint8_t*>(dmalloc_bytes((size_t)CRT_NPROB * rows_cap * rows_cap));
+        crt_probs = dalloc<CrtProb>(CRT_NPROB);
+      }
+      const CrtTable& tab = crt_table();
+      const int64_t op_plane = (int64_t)rows * kpad, res_plane = (int64_t)rows * rows;
+      if (crt_probs_n != n) {
+        crt_hprobs.assign(CRT_NPROB, CrtProb{});
+        for (int p = 0; p < CRT_NPROB; ++p) {
+          // problem p: operand p / CRT_K (yr', yi', s') modulo moduli p % CRT_K; C = X X^T
+          CrtProb& q = crt_hprobs[p];
+          q.a = crt_ops + p * op_plane;
+          q.b = q.a;
+          q.out = crt_res + p * res_plane;
+          q.m = tab.m[p % CRT_K];
+          q.magic = tab.magic[p % CRT_K];
+        }
+        FX_HIP(hipMemcpyAsync(crt_probs, crt_hprobs.data(), CRT_NPROB * sizeof(CrtProb), hipMemcpyHostToDevice, st));
+        FX_HIP(hipStreamSynchronize(st));
+        crt_probs_n = n;
+      }
+      const int G = rows / 32, KB = kpad / 32, T = rows / CRT_TILE, tiles_per = T * (T + 1) / 2;
+      const int T64 = (n + 63) / 64;
+      const int64_t jobs = (int64_t)G * KB;
+      Plane2<W>* cur = &x0;
+      Plane2<W>* nxt = &x1;
+      for (int q = 0; q < s; ++q) {
+        crt_row_exp<<<(n + 3) / 4, 256, 0, st>>>(cur->r, cur->i, ld, n, crt_e);
+        FX_LAUNCH();
+        crt_residues_sq<<<(unsigned)((jobs + 3) / 4), 256, 0, st>>>(cur->r, cur->i, ld, n, crt_e, rows, kpad, tab,
+                                                                     crt_ops);
+        FX_LAUNCH();
+        crt_gemm_kernel<<<CRT_NPROB * tiles_per, 256, 0, st>>>(crt_probs, tiles_per, T, G, KB);
+        FX_LAUNCH();
+        crt_combine_sq<<<T64 * (T64 + 1) / 2, 256, 0, st>>>(crt_res, rows, crt_e, cur->r, cur->i, ld, n, dsq[dc],
+                                                             dsq[dc ^ 1], nxt->r, nxt->i, tab);
+        FX_LAUNCH();
+        flops += 2.0 * CRT_NPROB * (double)tiles_per * CRT_TILE * CRT_TILE * (double)kpad;  // int8 operations
+        std::swap(cur, nxt);
+        dc ^= 1;
+      }
+      return {cur, dc};
+    }
+  }
+
   // FP32 structured squaring in 128 x 128 tiles.  When the tile count
   // leaves the last wave of workgroups less than half full, the tiles of
   // the full waves run in 128 x 128 and the rest as 64 x 64 sub-tiles
@@ -1360,6 +1462,8 @@ template <typename W> struct Pipeline : PipelineBase {
       return square_split16<Fp16x2>(s, dc, st);
     if (std::is_same<W, float>::value && structured && precision == FEXPM_PREC_BF16X3)
       return square_split16<Bf16x3>(s, dc, st);
+    if (std::is_same<W, float>::value && structured && precision == FEXPM_PREC_INT8_CRT && n <= CRT_KMAX)
+      return square_crt(s, dc, st);
     Plane2<W>* cur = &x0;
     Plane2<W>* nxt = &x1;
     for (int q = 0; q < s; ++q) {
@@ -1483,7 +1587,8 @@ template <typename W> struct Pipeline : PipelineBase {
     choose_degree(P.norm, P.m, P.s);
     P.s_onenorm = P.s;
     P.hybrid = std::is_same<W, float>::value && pade_fp64;
-    P.split16 = std::is_same<W, float>::value && P.structured && precision != FEXPM_PREC_NATIVE;
+    P.split16 = std::is_same<W, float>::value && P.structured && precision != FEXPM_PREC_NATIVE &&
+                (precision != FEXPM_PREC_INT8_CRT || n <= CRT_KMAX);
     // spectral scaling: structured, degree 13 with squarings, FP64 Pade stage
     P.spectral = spectral_ok && spectral && P.structured && P.m == 13 && P.s > 0 && P.s < 120 &&
                  (P.hybrid || std::is_same<W, double>::value);
@@ -2583,7 +2688,8 @@ int fexpm_set_option(fexpm_engine* e, int key, int64_t value) {
   std::lock_guard<std::mutex> lk(e->mtx);
   switch (key) {
     case FEXPM_OPT_PRECISION:
-      if (value != FEXPM_PREC_NATIVE && value != FEXPM_PREC_FP16X2 && value != FEXPM_PREC_BF16X3) {
+      if (value != FEXPM_PREC_NATIVE && value != FEXPM_PREC_FP16X2 && value != FEXPM_PREC_BF16X3 &&
+          value != FEXPM_PREC_INT8_CRT) {
         e->err = "unknown precision mode";
         return FEXPM_EINVAL;
       }

@@ -66,7 +66,10 @@ static py::dict stats_dict(const fexpm_stats& s) {
   d["s_fp64"] = s.s_fp64;
   d["kblock"] = s.kblock;
   d["pade_fp64"] = (bool)s.pade_fp64;
-  d["precision"] = s.precision == FEXPM_PREC_FP16X2 ? "fp16x2" : (s.precision == FEXPM_PREC_BF16X3 ? "bf16x3" : "native");
+  d["precision"] = s.precision == FEXPM_PREC_FP16X2    ? "fp16x2"
+                   : s.precision == FEXPM_PREC_BF16X3  ? "bf16x3"
+                   : s.precision == FEXPM_PREC_INT8_CRT ? "int8crt"
+                                                        : "native";
   d["fp64_slices"] = s.fp64_slices;
   d["fp64_slices_scope"] = s.fp64_slices_scope;
   d["spectral"] = s.spectral;
@@ -526,6 +529,7 @@ PYBIND11_MODULE(_fastexpm_core, m) {
   m.attr("PREC_NATIVE") = (int)FEXPM_PREC_NATIVE;
   m.attr("PREC_FP16X2") = (int)FEXPM_PREC_FP16X2;
   m.attr("PREC_BF16X3") = (int)FEXPM_PREC_BF16X3;
+  m.attr("PREC_INT8_CRT") = (int)FEXPM_PREC_INT8_CRT;
   py::class_<ExpMatBatch>(m, "ExpMatBatch")
       .def(py::init<const std::string&, int64_t, int>(), py::arg("dtype"), py::arg("nmax"), py::arg("lanes") = 4)
       .def("upload", &ExpMatBatch::upload, py::arg("mats"))

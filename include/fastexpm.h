@@ -53,8 +53,12 @@ enum fexpm_precision {
   FEXPM_PREC_NATIVE = 0, /* FP32 MFMA for complex64/float32, FP64 for 128/64 */
   FEXPM_PREC_FP16X2 = 1, /* complex64 engines, structured squarings: fp16 hi/lo parts
                             (scale 2^14), 3 products on 16-bit MFMA, FP32 accumulation */
-  FEXPM_PREC_BF16X3 = 2  /* complex64 engines, structured squarings: three bf16 parts,
+  FEXPM_PREC_BF16X3 = 2, /* complex64 engines, structured squarings: three bf16 parts,
                             6 products on 16-bit MFMA, FP32 accumulation */
+  FEXPM_PREC_INT8_CRT = 3 /* complex64 engines, structured squarings of order n <= 16384
+                             (larger orders use FP32 MFMA): each row rounded to 27 bits of its
+                             largest element, exact int8 products modulo 9 coprime moduli,
+                             Chinese-remainder reconstruction (no accumulation rounding) */
 };
 
 enum fexpm_option {
