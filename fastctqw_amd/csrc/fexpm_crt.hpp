@@ -46,6 +46,7 @@ namespace fexpm {
 constexpr int CRT_K = 9;         // moduli of the squaring products
 constexpr int CRT_BETA = 27;     // bits kept of each row's largest element
 constexpr int CRT_KMAX = 16384;  // longest k (keeps every bound above)
+constexpr int CRT_NMIN = 768;    // automatic mode: smallest order for which the residue form is faster
 constexpr int CRT_TILE = 256;    // workgroup tile edge of the residue products
 constexpr int CRT_KPAD = 64;     // k granularity (one LDS stage: two k-blocks)
 constexpr int CRT_NPROB = 3 * CRT_K;
@@ -66,11 +67,20 @@ __device__ __forceinline__ uint32_t crt_umod(uint32_t u, uint32_t m, uint32_t ma
   return u - q * m;
 }
 
-// Symmetric residue in [-(m-1)/2, (m-1)/2] of an integer |x| <= 2^29 (odd m).
+// Symmetric residue in [-(m-1)/2, (m-1)/2] of an integer |x| <= 2^28 (odd
+// m in [227, 255]): u = x + m 2^21 lies in [0, 2^30), since m 2^21 > 2^28.
 __device__ __forceinline__ int crt_sres(int x, uint32_t m, uint32_t magic) {
-  const uint32_t u = (uint32_t)(x + (int)(m << 21));  // m 2^21 > 2^29 >= |x| for m >= 227... and u < 2^31
+  const uint32_t u = (uint32_t)(x + (int)(m << 21));
   const int r = (int)crt_umod(u, m, magic);
   return r > (int)(m >> 1) ? r - (int)m : r;
+}
+
+// Symmetric residue of a + b from symmetric residues a, b (|a + b| <= m - 1).
+__device__ __forceinline__ int crt_sadd(int a, int b, uint32_t m) {
+  const int h = (int)(m >> 1);
+  int r = a + b;
+  r = r > h ? r - (int)m : r;
+  return r < -h ? r + (int)m : r;
 }
 
 // Row exponents of a complex FP32 matrix (planes yr, yi, leading dimension
@@ -102,7 +112,9 @@ __global__ __launch_bounds__(256) void crt_row_exp(const float* __restrict__ yr,
 // rows x kpad bytes in the fragment layout: plane o * CRT_K + i holds
 // operand o (0: yr', 1: yi', 2: s') mod moduli i).  One wave per fragment
 // (k-block kb, row group g), four per workgroup; rows >= n and k >= n are
-// zero.  rows is a multiple of CRT_TILE, kpad of CRT_KPAD, kpad <= ld.
+// zero.  rows is a multiple of CRT_TILE, kpad of CRT_KPAD, kpad <= ld, and
+// the padding columns k in [n, ld) of the planes are zero, so a lane reads
+// its 16 values (k0 + 16 <= kpad <= ld) with four 16-byte loads per plane.
 __global__ __launch_bounds__(256) void crt_residues_sq(const float* __restrict__ yr, const float* __restrict__ yi,
                                                         int64_t ld, int n, const int* __restrict__ e, int rows,
                                                         int kpad, CrtTable tab, int8_t* __restrict__ out) {
@@ -119,17 +131,21 @@ __global__ __launch_bounds__(256) void crt_residues_sq(const float* __restrict__
     xr[q] = 0;
     xi[q] = 0;
   }
-  if (row < n) {
+  if (row < n && k0 < n) {
     const float sc = ldexpf(1.0f, CRT_BETA - e[row]);  // exact power of two; |y sc| < 2^BETA
-    const float* pr = yr + (int64_t)row * ld;
-    const float* pi = yi + (int64_t)row * ld;
+    const float4* pr = reinterpret_cast<const float4*>(yr + (int64_t)row * ld + k0);
+    const float4* pi = reinterpret_cast<const float4*>(yi + (int64_t)row * ld + k0);
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int k = k0 + q;
-      if (k < n) {
-        xr[q] = (int)rintf(pr[k] * sc);
-        xi[q] = (int)rintf(pi[k] * sc);
-      }
+    for (int c = 0; c < 4; ++c) {
+      const float4 a = pr[c], b = pi[c];
+      xr[4 * c + 0] = (int)rintf(a.x * sc);
+      xr[4 * c + 1] = (int)rintf(a.y * sc);
+      xr[4 * c + 2] = (int)rintf(a.z * sc);
+      xr[4 * c + 3] = (int)rintf(a.w * sc);
+      xi[4 * c + 0] = (int)rintf(b.x * sc);
+      xi[4 * c + 1] = (int)rintf(b.y * sc);
+      xi[4 * c + 2] = (int)rintf(b.z * sc);
+      xi[4 * c + 3] = (int)rintf(b.w * sc);
     }
   }
   const int64_t plane = (int64_t)rows * kpad;
@@ -144,9 +160,10 @@ __global__ __launch_bounds__(256) void crt_residues_sq(const float* __restrict__
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         const int q = 4 * c + t;
-        a |= ((uint32_t)crt_sres(xr[q], m, mg) & 0xFFu) << (8 * t);
-        b |= ((uint32_t)crt_sres(xi[q], m, mg) & 0xFFu) << (8 * t);
-        s |= ((uint32_t)crt_sres(xr[q] + xi[q], m, mg) & 0xFFu) << (8 * t);
+        const int ra = crt_sres(xr[q], m, mg), rb = crt_sres(xi[q], m, mg);
+        a |= ((uint32_t)ra & 0xFFu) << (8 * t);
+        b |= ((uint32_t)rb & 0xFFu) << (8 * t);
+        s |= ((uint32_t)crt_sadd(ra, rb, m) & 0xFFu) << (8 * t);
       }
       pk[0][c] = a;
       pk[1][c] = b;

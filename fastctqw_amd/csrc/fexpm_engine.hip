@@ -1455,15 +1455,24 @@ template <typename W> struct Pipeline : PipelineBase {
     }
   }
 
+  // Form of the structured squarings for the current order: the precision
+  // option resolved (FEXPM_PREC_AUTO: residue products for CRT_NMIN <= n <=
+  // CRT_KMAX, FP32 MFMA otherwise; residue products need n <= CRT_KMAX).
+  int square_mode() const {
+    if (!std::is_same<W, float>::value) return FEXPM_PREC_NATIVE;
+    if (precision == FEXPM_PREC_AUTO)
+      return (n >= CRT_NMIN && n <= CRT_KMAX) ? FEXPM_PREC_INT8_CRT : FEXPM_PREC_NATIVE;
+    if (precision == FEXPM_PREC_INT8_CRT && n > CRT_KMAX) return FEXPM_PREC_NATIVE;
+    return precision;
+  }
+
   // s squarings of the split form; with structured input the first `two`
   // of them in the two-product form (MODE_SQ2).
   std::pair<Plane2<W>*, int> square_split(int s, bool structured, int dc, hipStream_t st, int two = 0) {
-    if (std::is_same<W, float>::value && structured && precision == FEXPM_PREC_FP16X2)
-      return square_split16<Fp16x2>(s, dc, st);
-    if (std::is_same<W, float>::value && structured && precision == FEXPM_PREC_BF16X3)
-      return square_split16<Bf16x3>(s, dc, st);
-    if (std::is_same<W, float>::value && structured && precision == FEXPM_PREC_INT8_CRT && n <= CRT_KMAX)
-      return square_crt(s, dc, st);
+    const int mode = square_mode();
+    if (structured && mode == FEXPM_PREC_FP16X2) return square_split16<Fp16x2>(s, dc, st);
+    if (structured && mode == FEXPM_PREC_BF16X3) return square_split16<Bf16x3>(s, dc, st);
+    if (structured && mode == FEXPM_PREC_INT8_CRT) return square_crt(s, dc, st);
     Plane2<W>* cur = &x0;
     Plane2<W>* nxt = &x1;
     for (int q = 0; q < s; ++q) {
@@ -1587,8 +1596,7 @@ template <typename W> struct Pipeline : PipelineBase {
     choose_degree(P.norm, P.m, P.s);
     P.s_onenorm = P.s;
     P.hybrid = std::is_same<W, float>::value && pade_fp64;
-    P.split16 = std::is_same<W, float>::value && P.structured && precision != FEXPM_PREC_NATIVE &&
-                (precision != FEXPM_PREC_INT8_CRT || n <= CRT_KMAX);
+    P.split16 = P.structured && square_mode() != FEXPM_PREC_NATIVE;
     // spectral scaling: structured, degree 13 with squarings, FP64 Pade stage
     P.spectral = spectral_ok && spectral && P.structured && P.m == 13 && P.s > 0 && P.s < 120 &&
                  (P.hybrid || std::is_same<W, double>::value);
@@ -1760,7 +1768,7 @@ template <typename W> struct Pipeline : PipelineBase {
     S.s_fp64 = P.q64;
     S.pade_fp64 = (P.hybrid && !P.nonfinite) ? 1 : 0;
     S.kblock = kblock;
-    S.precision = P.split16 ? precision : FEXPM_PREC_NATIVE;
+    S.precision = P.split16 ? square_mode() : FEXPM_PREC_NATIVE;
     S.fp64_slices = (P.hybrid && P.structured && !P.nonfinite) ? oz_slices : 0;
     S.fp64_slices_scope = S.fp64_slices ? slice_scope(P) : 0;
     S.spectral = P.spectral ? 1 : 0;
@@ -1987,7 +1995,7 @@ struct fexpm_engine {
   void* d_out = nullptr;
   size_t esize = 0;
   int opt_structure = 1;
-  int opt_precision = 0;
+  int opt_precision = 0;  // FEXPM_PREC_AUTO for complex64 engines (fexpm_create)
   double* d_h = nullptr;  // walk Hamiltonian kept by fexpm_set_graph (FP64, n x n)
   bool have_graph = false;
   HostStaging stg;
@@ -2210,11 +2218,13 @@ int fexpm_create(int64_t n, int dtype, fexpm_engine** out) {
   const int rc = guarded(e, [&] {
     check_device();
     FX_HIP(hipStreamCreate(&e->stream));
+    if (dtype == FEXPM_COMPLEX64) e->opt_precision = FEXPM_PREC_AUTO;
     if (n > 0) {
       if (dtype == FEXPM_FLOAT32 || dtype == FEXPM_COMPLEX64)
         e->pipe.reset(new Pipeline<float>((int)n, dtype));
       else
         e->pipe.reset(new Pipeline<double>((int)n, dtype));
+      e->pipe->precision = e->opt_precision;
     }
   });
   if (rc != FEXPM_OK) {
@@ -2689,7 +2699,7 @@ int fexpm_set_option(fexpm_engine* e, int key, int64_t value) {
   switch (key) {
     case FEXPM_OPT_PRECISION:
       if (value != FEXPM_PREC_NATIVE && value != FEXPM_PREC_FP16X2 && value != FEXPM_PREC_BF16X3 &&
-          value != FEXPM_PREC_INT8_CRT) {
+          value != FEXPM_PREC_INT8_CRT && value != FEXPM_PREC_AUTO) {
         e->err = "unknown precision mode";
         return FEXPM_EINVAL;
       }

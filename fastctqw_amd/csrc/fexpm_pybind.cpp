@@ -530,6 +530,7 @@ PYBIND11_MODULE(_fastexpm_core, m) {
   m.attr("PREC_FP16X2") = (int)FEXPM_PREC_FP16X2;
   m.attr("PREC_BF16X3") = (int)FEXPM_PREC_BF16X3;
   m.attr("PREC_INT8_CRT") = (int)FEXPM_PREC_INT8_CRT;
+  m.attr("PREC_AUTO") = (int)FEXPM_PREC_AUTO;
   py::class_<ExpMatBatch>(m, "ExpMatBatch")
       .def(py::init<const std::string&, int64_t, int>(), py::arg("dtype"), py::arg("nmax"), py::arg("lanes") = 4)
       .def("upload", &ExpMatBatch::upload, py::arg("mats"))

@@ -112,14 +112,16 @@ class _ExpMatBase:
         self._backend.set_option(_fastexpm_core.OPT_STRUCTURE, 1 if enabled else 0)
 
     _PRECISIONS = {"native": "PREC_NATIVE", "fp16x2": "PREC_FP16X2", "bf16x3": "PREC_BF16X3",
-                   "int8crt": "PREC_INT8_CRT"}
+                   "int8crt": "PREC_INT8_CRT", "auto": "PREC_AUTO"}
 
     def set_precision(self, mode: str) -> None:
         """Squaring precision of a complex64 engine on structured (CTQW)
         inputs: "native" (FP32 MFMA), "fp16x2" or "bf16x3" (split 16-bit
         products with FP32 accumulation), "int8crt" (rows rounded to 27 bits,
         exact int8 products modulo 9 moduli, Chinese-remainder reconstruction;
-        orders up to 16384).  No counterpart in the reference."""
+        orders up to 16384) or "auto" (the complex64 default: "int8crt" for
+        768 <= n <= 16384, "native" otherwise; stats()["precision"] names the
+        form a run used).  No counterpart in the reference."""
         if mode not in self._PRECISIONS:
             raise ValueError(f"precision must be one of {sorted(self._PRECISIONS)}")
         self._backend.set_option(_fastexpm_core.OPT_PRECISION, getattr(_fastexpm_core, self._PRECISIONS[mode]))
@@ -153,7 +155,7 @@ class ExpMatBatch:
 
     def set_precision(self, mode: str) -> None:
         """Squaring precision for every lane ("native", "fp16x2", "bf16x3")."""
-        if mode not in _ExpMatBase._PRECISIONS or mode == "int8crt":
+        if mode not in _ExpMatBase._PRECISIONS or mode in ("int8crt", "auto"):
             raise ValueError("precision must be one of ['bf16x3', 'fp16x2', 'native']")
         self._backend.set_option(_fastexpm_core.OPT_PRECISION,
                                  getattr(_fastexpm_core, _ExpMatBase._PRECISIONS[mode]))

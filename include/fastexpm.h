@@ -55,10 +55,14 @@ enum fexpm_precision {
                             (scale 2^14), 3 products on 16-bit MFMA, FP32 accumulation */
   FEXPM_PREC_BF16X3 = 2, /* complex64 engines, structured squarings: three bf16 parts,
                             6 products on 16-bit MFMA, FP32 accumulation */
-  FEXPM_PREC_INT8_CRT = 3 /* complex64 engines, structured squarings of order n <= 16384
-                             (larger orders use FP32 MFMA): each row rounded to 27 bits of its
-                             largest element, exact int8 products modulo 9 coprime moduli,
-                             Chinese-remainder reconstruction (no accumulation rounding) */
+  FEXPM_PREC_INT8_CRT = 3, /* complex64 engines, structured squarings of order n <= 16384
+                              (larger orders use FP32 MFMA): each row rounded to 27 bits of its
+                              largest element, exact int8 products modulo 9 coprime moduli,
+                              Chinese-remainder reconstruction (no accumulation rounding) */
+  FEXPM_PREC_AUTO = 4      /* default of complex64 engines: FEXPM_PREC_INT8_CRT for orders
+                              768 <= n <= 16384 (where it is the faster form, and more accurate
+                              than FP32 MFMA), FEXPM_PREC_NATIVE otherwise; fexpm_stats.precision
+                              reports the form a run used */
 };
 
 enum fexpm_option {

@@ -365,6 +365,7 @@ def test_int8crt_squarings(fx, n):
     ref = (V * np.exp(1j * lam)[None, :]) @ V.T
     _say(f"int8crt squarings n={n}")
     with fx.ExpMatComplex64(n) as e:
+        e.set_precision("native")
         u_native = e.run(a)
         st_native = e.stats()
         e.set_precision("int8crt")
@@ -380,6 +381,29 @@ def test_int8crt_squarings(fx, n):
     psi0 = rng.standard_normal(n) + 1j * rng.standard_normal(n)
     psi0 /= np.linalg.norm(psi0)
     assert infidelity(ref @ psi0, u.astype(np.complex128) @ psi0) < 1e-9
+
+
+def test_precision_auto_default(fx):
+    """Complex64 engines default to the automatic squaring form: residue
+    products for 768 <= n <= 16384 on structured input, FP32 MFMA below (and
+    for general input, whose squarings are not the structured ones)."""
+    for n, want in ((300, "native"), (767, "native"), (768, "int8crt"), (1100, "int8crt")):
+        _say(f"precision auto n={n}")
+        L = laplacian(erdos_renyi(n, 0.4, seed=900 + n))
+        with fx.ExpMatComplex64(n) as e:
+            e.run((-1j * 3.0 * L).astype(np.complex64))
+            assert e.stats()["precision"] == want, (n, e.stats()["precision"])
+            e.set_precision("native")
+            e.run((-1j * 3.0 * L).astype(np.complex64))
+            assert e.stats()["precision"] == "native"
+            e.set_precision("auto")
+            rng = np.random.default_rng(n)
+            g = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))).astype(np.complex64) / n
+            e.run(g)
+            assert not e.stats()["structured"] and e.stats()["precision"] == "native"
+    with fx.ExpMatComplex128(800) as e:
+        e.run((-1j * 3.0 * laplacian(erdos_renyi(800, 0.4, seed=3))).astype(np.complex128))
+        assert e.stats()["precision"] == "native"
 
 
 @pytest.mark.parametrize("slices,scope", [(6, 1), (5, 1), (6, 3), (5, 3)])
