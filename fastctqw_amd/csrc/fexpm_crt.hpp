@@ -182,13 +182,17 @@ __global__ __launch_bounds__(256) void crt_residues_sq(const float* __restrict__
 }
 
 // One product of a batched residue launch: C = A B^T over the lower tiles,
-// written as residues mod m.
+// written as residues mod m.  A launch takes its list by value (kernel
+// argument), so it needs no device buffer or host wait.
 struct CrtProb {
   const int8_t* a;  // operand planes (fragment layout, `rows` rows)
   const int8_t* b;
   uint8_t* out;     // rows x rows bytes, element (row, col) at col * rows + row
   uint32_t m;
   uint32_t magic;
+};
+struct CrtProbs {
+  CrtProb p[CRT_NPROB];
 };
 
 __device__ __forceinline__ void crt_dma16(const int8_t* g, int8_t* l) {
@@ -213,8 +217,7 @@ template <int N> __device__ __forceinline__ void crt_wait_vm() {
 // that barrier.  Past the last stage the copies repeat the last k-block into a
 // buffer no wave reads, so the loop has no branches around them.
 constexpr int CRT_RING = 4;
-__global__ __launch_bounds__(256, 1) void crt_gemm_kernel(const CrtProb* __restrict__ probs, int tiles_per, int T,
-                                                          int G, int KB) {
+__global__ __launch_bounds__(256, 1) void crt_gemm_kernel(CrtProbs probs, int tiles_per, int T, int G, int KB) {
   constexpr int KBS = 2;
   constexpr int KSTAGE = 16384;
   constexpr int SB = KSTAGE * KBS;
@@ -225,7 +228,7 @@ __global__ __launch_bounds__(256, 1) void crt_gemm_kernel(const CrtProb* __restr
   const int p = lin / tiles_per;
   int tm, tn;
   grouped_tri_tile(lin - p * tiles_per, T, tm, tn);
-  const CrtProb P = probs[p];
+  const CrtProb P = probs.p[p];
   const int t = threadIdx.x, lane = t & 63;
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
   const int wm = wave >> 1, wn = wave & 1;
@@ -309,8 +312,9 @@ __global__ __launch_bounds__(256, 1) void crt_gemm_kernel(const CrtProb* __restr
         uint32_t w4 = 0;
 #pragma unroll
         for (int r3 = 0; r3 < 4; ++r3) {
-          // |acc| <= k 127^2 < 2^28 and m 2^21 > 2^28: u in [0, 2^31)
-          const uint32_t u = (uint32_t)(acc[i][j][4 * rq + r3] + (int)(P.m << 21));
+          // |acc| <= k 127^2 <= 2^28.3 for k <= CRT64_KMAX and m 2^22 >= 199 2^22 > 2^29.6:
+          // u = acc + m 2^22 in (0, 2^31)
+          const uint32_t u = (uint32_t)(acc[i][j][4 * rq + r3] + (int)(P.m << 22));
           w4 |= crt_umod(u, P.m, P.magic) << (8 * r3);
         }
         const int row = tm * CRT_TILE + wm * 128 + i * 32 + 8 * rq + 4 * kh;
@@ -409,6 +413,191 @@ __global__ __launch_bounds__(256) void crt_combine_sq(const uint8_t* __restrict_
     if (i >= n || j >= n || i < j) continue;
     cr[(int64_t)i * ld + j] = tre[il2][jl];
     ci[(int64_t)i * ld + j] = tim[il2][jl];
+  }
+}
+
+
+// ---------------------------------------------------------------------------
+// FP64-accurate real products by the same residue arithmetic (the symmetric
+// FP64 products of the Pade stage and of the structured solve).  Thirteen
+// moduli (the nine above and 223, 217, 211, 199; M of 102.2 bits); each row
+// r of an operand is scaled by 2^(beta - e_r) and rounded to an integer of at
+// most 2^beta, beta = min(45, floor((log2 M - 1 - log2 k) / 2)) (43 for k
+// from 2^13.2 to 2^14.9), so that |C| <= k 2^(2 beta) < M / 2.  The reconstruction
+// X / M = frac(sum_m s_m / m) runs in double-double arithmetic (1 / m as a
+// pair of doubles, exact product errors by fma, TwoSum), X = f M to a few
+// integer units, and C = 2^(e_i + f_j - 2 beta) X goes through the FP64
+// GEMM epilogue (alpha, beta, added matrices, diagonal, mirrored lower tiles).
+// Operand rows may be the concatenation s1 x1[r, 0:k1] | s2 x2[r, 0:nk-k1]
+// (s1, s2 = +-1), as for S = [V U][V U]^T.
+constexpr int CRT64_K = 13;
+constexpr int CRT64_KMAX = 20480;   // longest k: k 127^2 < 2^28.3 (epilogue offset above)
+constexpr int CRT64_BETA_MAX = 45;  // integers of at most 2^45 (crt64_sres)
+
+struct CrtTable64 {
+  uint32_t m[CRT64_K];
+  uint32_t magic[CRT64_K];
+  uint32_t inv[CRT64_K];   // ((M / m)^-1) mod m
+  uint32_t c22[CRT64_K];   // 2^22 mod m
+  double rcp_hi[CRT64_K];  // 1 / m = rcp_hi + rcp_lo (double-double)
+  double rcp_lo[CRT64_K];
+  double M_hi, M_lo;       // M = M_hi + M_lo
+  double log2M;
+};
+
+// Concatenated operand row r, value k (k < nk).
+__device__ __forceinline__ double crt64_elem(const double* __restrict__ x1, double s1, const double* __restrict__ x2,
+                                             double s2, int k1, int64_t ld, int r, int k) {
+  return k < k1 ? s1 * x1[(int64_t)r * ld + k] : s2 * x2[(int64_t)r * ld + (k - k1)];
+}
+
+// e[r] = e with max_k |x_rk| < 2^e (0 for a zero row), rows r < nr.  One
+// wave per row, four rows per workgroup.
+__global__ __launch_bounds__(256) void crt64_row_exp(const double* __restrict__ x1, double s1,
+                                                      const double* __restrict__ x2, double s2, int k1, int64_t ld,
+                                                      int nr, int nk, int* __restrict__ e) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r = (int)blockIdx.x * 4 + w;
+  if (r >= nr) return;  // whole waves leave; no workgroup barrier follows
+  double mx = 0.0;
+  for (int k = lane; k < nk; k += 64) mx = fmax(mx, fabs(crt64_elem(x1, s1, x2, s2, k1, ld, r, k)));
+  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+  int ex = 0;
+  if (mx > 0.0) (void)frexp(mx, &ex);
+  if (lane == 0) e[r] = ex;
+}
+
+// Symmetric residue of an integer-valued double |x| <= 2^45 (odd m in
+// [199, 255]): x = h 2^22 + l with h = floor(x 2^-22) in [-2^23, 2^23] and
+// l in [0, 2^22); h mod m from u = h + m 2^16 in (0, 2^25), then
+// (h mod m) 2^22 + l = (h mod m) c22 + l (mod m) with the sum below 2^23.
+__device__ __forceinline__ int crt64_sres(double x, uint32_t m, uint32_t magic, uint32_t c22) {
+  const double hd = floor(x * 0x1p-22);
+  const int h = (int)hd;
+  const uint32_t l = (uint32_t)(x - hd * 0x1p22);
+  const uint32_t rh = crt_umod((uint32_t)(h + (int)(m << 16)), m, magic);
+  const int r = (int)crt_umod(rh * c22 + l, m, magic);
+  return r > (int)(m >> 1) ? r - (int)m : r;
+}
+
+// Operand residues (CRT64_K planes of rows x kpad bytes, fragment layout) of
+// x' = rint(x 2^(beta - e_r)); rows >= nr and k >= nk are zero.  One wave per
+// fragment (k-block kb, row group g), four per workgroup.
+__global__ __launch_bounds__(256) void crt64_residues(const double* __restrict__ x1, double s1,
+                                                       const double* __restrict__ x2, double s2, int k1, int64_t ld,
+                                                       int nr, int nk, const int* __restrict__ e, int beta, int rows,
+                                                       int kpad, CrtTable64 tab, int8_t* __restrict__ out) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int G = rows / 32;
+  const int64_t jobs = (int64_t)G * (kpad / 32);
+  const int64_t job = (int64_t)blockIdx.x * 4 + w;
+  if (job >= jobs) return;
+  const int kb = (int)(job / G), g = (int)(job % G);
+  const int row = 32 * g + (lane & 31), k0 = 32 * kb + 16 * (lane >> 5);
+  double x[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) x[q] = 0.0;
+  if (row < nr) {
+    const double sc = ldexp(1.0, beta - e[row]);  // |x sc| < 2^beta
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int k = k0 + q;
+      if (k < nk) x[q] = rint(crt64_elem(x1, s1, x2, s2, k1, ld, row, k) * sc);
+    }
+  }
+  const int64_t plane = (int64_t)rows * kpad;
+  const int64_t off = ((int64_t)kb * G + g) * 1024 + lane * 16;
+#pragma unroll 1
+  for (int i = 0; i < CRT64_K; ++i) {
+    const uint32_t m = tab.m[i], mg = tab.magic[i], c = tab.c22[i];
+    uint4 v;
+    uint32_t* pv = &v.x;
+#pragma unroll
+    for (int cc = 0; cc < 4; ++cc) {
+      uint32_t wv = 0;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) wv |= ((uint32_t)crt64_sres(x[4 * cc + t], m, mg, c) & 0xFFu) << (8 * t);
+      pv[cc] = wv;
+    }
+    *reinterpret_cast<uint4*>(out + (int64_t)i * plane + off) = v;
+  }
+}
+
+// TwoSum (Knuth): s + err == a + b exactly.  The callers turn off
+// floating-point contraction: an add fused with the product before it would
+// round differently from the product that the error terms describe.
+__device__ __forceinline__ void crt_two_sum(double a, double b, double& s, double& err) {
+#pragma clang fp contract(off)
+  s = a + b;
+  const double bb = s - a;
+  err = (a - (s - bb)) + (b - bb);
+}
+
+// C = A B^T from the CRT64_K residue planes (element (row, col) at byte
+// col * rows + row of plane q), 64 x 64 lower tiles (index blockIdx.x), FP64
+// epilogue of e (real planes; diag on the diagonal), values written at (i, j)
+// and (j, i) for i, j < m.
+__global__ __launch_bounds__(256) void crt64_combine(const uint8_t* __restrict__ res, int rows,
+                                                      const int* __restrict__ ea, const int* __restrict__ eb,
+                                                      int beta, int m, Epilogue<double> ep, CrtTable64 tab) {
+  __shared__ double tv[64][65];
+  int TI, TJ;
+  tri_tile((int)blockIdx.x, TI, TJ);
+  const int i0 = TI * 64, j0 = TJ * 64;
+  const int t = threadIdx.x;
+  const int il = 4 * (t & 15);
+  const int64_t plane = (int64_t)rows * rows;
+#pragma unroll 1
+  for (int c = 0; c < 4; ++c) {
+    const int jl = (t >> 4) + 16 * c;
+    const int j = j0 + jl;
+    const int ib = i0 + il;
+    if (j >= m || ib >= m) continue;
+    uint32_t w[CRT64_K];
+#pragma unroll
+    for (int q = 0; q < CRT64_K; ++q)
+      w[q] = *reinterpret_cast<const uint32_t*>(res + q * plane + (int64_t)j * rows + ib);
+    const int ej = eb[j];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = ib + r;
+      if (i >= m || i < j) continue;
+      double hi = 0.0, lo = 0.0;
+#pragma unroll
+      for (int q = 0; q < CRT64_K; ++q) {
+#pragma clang fp contract(off)
+        const uint32_t d = crt_umod(((w[q] >> (8 * r)) & 0xFFu) * tab.inv[q], tab.m[q], tab.magic[q]);
+        const double sd = (double)d;
+        const double p = sd * tab.rcp_hi[q];                                // the rounded product TwoSum adds
+        const double pe = fma(sd, tab.rcp_hi[q], -p) + sd * tab.rcp_lo[q];  // its exact error, + d rcp_lo
+        double s, err;
+        crt_two_sum(hi, p, s, err);
+        hi = s;
+        lo += err + pe;
+      }
+      const double f = hi - rint(hi);  // exact; the fraction is f + lo
+      const double x = f * tab.M_hi + (f * tab.M_lo + lo * tab.M_hi);
+      const double v = ldexp(x, ea[i] + ej - 2 * beta);
+      const int64_t o = (int64_t)i * ep.ldc + j;
+      double out = ep.alpha * v;
+      if (ep.beta != 0.0) out += ep.beta * ep.cr[o];
+      if (ep.nadd > 0) {
+        const int64_t oa = (int64_t)i * ep.ld_add + j;
+        for (int a = 0; a < ep.nadd; ++a) out += ep.addc[a] * ep.add_r[a][oa];
+      }
+      if (i == j) out += ep.diag;
+      tv[il + r][jl] = out;
+      ep.cr[(int64_t)j * ep.ldc + i] = out;  // (j, i): consecutive i across the threads of a column
+    }
+  }
+  __syncthreads();
+  const int jl = t & 63, j = j0 + jl;
+#pragma unroll 1
+  for (int c = 0; c < 16; ++c) {
+    const int il2 = (t >> 6) + 4 * c;
+    const int i = i0 + il2;
+    if (i >= m || j >= m || i < j) continue;
+    ep.cr[(int64_t)i * ep.ldc + j] = tv[il2][jl];
   }
 }
 

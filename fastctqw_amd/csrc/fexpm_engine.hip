@@ -248,6 +248,41 @@ static const CrtTable& crt_table() {
   return tab;
 }
 
+// Constants of the FP64 residue products: the nine moduli above and 223,
+// 217, 211, 199 (pairwise coprime), M exact in 128-bit arithmetic.
+static const CrtTable64& crt_table64() {
+  static const CrtTable64 tab = [] {
+    static const uint32_t mods[CRT64_K] = {255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199};
+    CrtTable64 t;
+    unsigned __int128 M = 1;
+    for (int i = 0; i < CRT64_K; ++i) M *= mods[i];
+    t.M_hi = (double)M;
+    const __int128 rest = (__int128)M - (__int128)(unsigned __int128)t.M_hi;
+    t.M_lo = (double)rest;
+    t.log2M = std::log2(t.M_hi);
+    for (int i = 0; i < CRT64_K; ++i) {
+      const uint32_t m = mods[i];
+      t.m[i] = m;
+      t.magic[i] = (uint32_t)(((1ull << 39) + m - 1) / m);
+      t.c22[i] = (uint32_t)((1u << 22) % m);
+      t.rcp_hi[i] = 1.0 / (double)m;
+      t.rcp_lo[i] = std::fma(-t.rcp_hi[i], (double)m, 1.0) / (double)m;
+      uint64_t mi = 1;
+      for (int j = 0; j < CRT64_K; ++j)
+        if (j != i) mi = (mi * mods[j]) % m;
+      uint32_t u = 0;
+      for (uint32_t c = 1; c < m; ++c)
+        if ((mi * c) % m == 1) {
+          u = c;
+          break;
+        }
+      t.inv[i] = u;
+    }
+    return t;
+  }();
+  return tab;
+}
+
 template <typename W> struct Plane2 {
   W* r = nullptr;
   W* i = nullptr;
@@ -528,9 +563,12 @@ template <typename W> struct Pipeline : PipelineBase {
   int* crt_e = nullptr;
   int8_t* crt_ops = nullptr;
   uint8_t* crt_res = nullptr;
-  CrtProb* crt_probs = nullptr;
-  int crt_probs_n = -1;
-  std::vector<CrtProb> crt_hprobs;
+  // FP64 residue products (FP64 pipelines; oz_slices == CRT64_K): row
+  // exponents and operand residues of A and B, product residues
+  int* c64_e[2] = {nullptr, nullptr};
+  int8_t* c64_op[2] = {nullptr, nullptr};
+  size_t c64_op_bytes[2] = {0, 0};
+  uint8_t* c64_res = nullptr;
   std::map<int, int*> sq_orders;  // grouped lower-triangular tile orders, per tile count
   hipEvent_t ev[8];
   bool events = false;
@@ -548,7 +586,7 @@ template <typename W> struct Pipeline : PipelineBase {
     // slices (same accuracy as FP64 MFMA, measured); the solve's products
     // too when the plan deflates (automatic scope, see slice_scope)
     if (std::is_same<W, float>::value) {
-      oz_slices = 6;
+      oz_slices = CRT64_K;
       oz_scope = 0;
     }
     npcap = round_up(std::max(ncap, 1), 128);
@@ -653,7 +691,11 @@ template <typename W> struct Pipeline : PipelineBase {
     dfree(crt_e);
     dfree(crt_ops);
     dfree(crt_res);
-    dfree(crt_probs);
+    for (int b = 0; b < 2; ++b) {
+      dfree(c64_e[b]);
+      dfree(c64_op[b]);
+    }
+    dfree(c64_res);
     for (auto& kv : sq_orders) dfree(kv.second);
     for (hipEvent_t e : zev) (void)hipEventDestroy(e);
     if (zst) (void)hipStreamDestroy(zst);
@@ -770,11 +812,68 @@ template <typename W> struct Pipeline : PipelineBase {
     }
     FX_LAUNCH();
   }
+  // Symmetric C (epilogue e, m x m) = A B^T by residue products modulo
+  // CRT64_K moduli (fexpm_crt.hpp; B null: B = A).  Returns the FP64-
+  // equivalent flops of the product.
+  double crt64_product(const OzOp& A, const OzOp* B, int m, int nk, const Epilogue<double>& e, hipStream_t st) {
+    const CrtTable64& tab = crt_table64();
+    const int rows = (int)round_up(m, CRT_TILE), kpad = (int)round_up(nk, CRT_KPAD);
+    // |C| <= k 2^(2 beta) < M / 2; at most 45 bits (the range of crt64_sres)
+    const int beta = std::min(CRT64_BETA_MAX, (int)std::floor((tab.log2M - 1.0 - std::log2((double)kpad)) * 0.5));
+    const size_t op_bytes = (size_t)CRT64_K * rows * kpad;
+    const int nops = B ? 2 : 1;
+    for (int b = 0; b < nops; ++b) {
+      if (!c64_e[b]) c64_e[b] = dalloc<int>(round_up(npcap, CRT_TILE));
+      if (c64_op_bytes[b] < op_bytes) {
+        dfree(c64_op[b]);
+        c64_op[b] = nullptr;
+        c64_op_bytes[b] = 0;
+        c64_op[b] = static_cast<int8_t*>(dmalloc_bytes(op_bytes));
+        c64_op_bytes[b] = op_bytes;
+      }
+    }
+    if (!c64_res) {
+      const size_t rc = (size_t)round_up(npcap, CRT_TILE);
+      c64_res = static_cast<uint8_t*>(dmalloc_bytes((size_t)CRT64_K * rc * rc));
+    }
+    const int G = rows / 32, KB = kpad / 32, T = rows / CRT_TILE, tiles_per = T * (T + 1) / 2;
+    const int64_t jobs = (int64_t)G * KB;
+    for (int b = 0; b < nops; ++b) {
+      const OzOp& x = b == 0 ? A : *B;
+      crt64_row_exp<<<(m + 3) / 4, 256, 0, st>>>(x.p1, x.s1, x.p2, x.s2, x.k1, x.ldx, m, nk, c64_e[b]);
+      FX_LAUNCH();
+      crt64_residues<<<(unsigned)((jobs + 3) / 4), 256, 0, st>>>(x.p1, x.s1, x.p2, x.s2, x.k1, x.ldx, m, nk,
+                                                                  c64_e[b], beta, rows, kpad, tab, c64_op[b]);
+      FX_LAUNCH();
+    }
+    CrtProbs probs;
+    std::memset(&probs, 0, sizeof(probs));
+    const int64_t op_plane = (int64_t)rows * kpad, res_plane = (int64_t)rows * rows;
+    for (int q = 0; q < CRT64_K; ++q) {
+      CrtProb& pq = probs.p[q];
+      pq.a = c64_op[0] + q * op_plane;
+      pq.b = c64_op[nops - 1] + q * op_plane;
+      pq.out = c64_res + q * res_plane;
+      pq.m = tab.m[q];
+      pq.magic = tab.magic[q];
+    }
+    crt_gemm_kernel<<<CRT64_K * tiles_per, 256, 0, st>>>(probs, tiles_per, T, G, KB);
+    FX_LAUNCH();
+    const int T64 = (m + 63) / 64;
+    crt64_combine<<<T64 * (T64 + 1) / 2, 256, 0, st>>>(c64_res, rows, c64_e[0], c64_e[nops - 1], beta, m, e, tab);
+    FX_LAUNCH();
+    return (double)m * m * nk;
+  }
+
   // C (epilogue e) = A B^T from slices (B null: B = A, sliced once).  sym:
   // the product is symmetric (m == n), lower tiles computed and mirrored.
-  // Returns the FP64-equivalent flops of the product.
+  // With oz_slices == CRT64_K the symmetric products take the residue form
+  // (crt64_product) while k fits it, else six slices.  Returns the FP64-
+  // equivalent flops of the product.
   double oz_product(const OzOp& A, const OzOp* B, int m, int nn, int nk, const Epilogue<double>& e, bool sym,
                     hipStream_t st) {
+    if (oz_slices == CRT64_K && sym && m == nn && round_up(nk, CRT_KPAD) <= CRT64_KMAX)
+      return crt64_product(A, B, m, nk, e, st);
     const int kp = (int)round_up(nk, OZ_BK), ra = (int)round_up(m, OZ_BM), rb = (int)round_up(nn, OZ_BM);
     oz_reserve(std::max(ra, rb), kp);
     oz_slice_op(A, m, nk, ra, kp, 0, st);
@@ -1386,24 +1485,19 @@ template <typename W> struct Pipeline : PipelineBase {
         crt_e = dalloc<int>((size_t)npcap);
         crt_ops = static_cast<int8_t*>(dmalloc_bytes((size_t)CRT_NPROB * rows_cap * kpad_cap));
         crt_res = static_cast<uint8_t*>(dmalloc_bytes((size_t)CRT_NPROB * rows_cap * rows_cap));
-        crt_probs = dalloc<CrtProb>(CRT_NPROB);
       }
       const CrtTable& tab = crt_table();
       const int64_t op_plane = (int64_t)rows * kpad, res_plane = (int64_t)rows * rows;
-      if (crt_probs_n != n) {
-        crt_hprobs.assign(CRT_NPROB, CrtProb{});
-        for (int p = 0; p < CRT_NPROB; ++p) {
-          // problem p: operand p / CRT_K (yr', yi', s') modulo moduli p % CRT_K; C = X X^T
-          CrtProb& q = crt_hprobs[p];
-          q.a = crt_ops + p * op_plane;
-          q.b = q.a;
-          q.out = crt_res + p * res_plane;
-          q.m = tab.m[p % CRT_K];
-          q.magic = tab.magic[p % CRT_K];
-        }
-        FX_HIP(hipMemcpyAsync(crt_probs, crt_hprobs.data(), CRT_NPROB * sizeof(CrtProb), hipMemcpyHostToDevice, st));
-        FX_HIP(hipStreamSynchronize(st));
-        crt_probs_n = n;
+      CrtProbs probs;
+      std::memset(&probs, 0, sizeof(probs));
+      for (int p = 0; p < CRT_NPROB; ++p) {
+        // problem p: operand p / CRT_K (yr', yi', s') modulo moduli p % CRT_K; C = X X^T
+        CrtProb& q = probs.p[p];
+        q.a = crt_ops + p * op_plane;
+        q.b = q.a;
+        q.out = crt_res + p * res_plane;
+        q.m = tab.m[p % CRT_K];
+        q.magic = tab.magic[p % CRT_K];
       }
       const int G = rows / 32, KB = kpad / 32, T = rows / CRT_TILE, tiles_per = T * (T + 1) / 2;
       const int T64 = (n + 63) / 64;
@@ -1416,7 +1510,7 @@ template <typename W> struct Pipeline : PipelineBase {
         crt_residues_sq<<<(unsigned)((jobs + 3) / 4), 256, 0, st>>>(cur->r, cur->i, ld, n, crt_e, rows, kpad, tab,
                                                                      crt_ops);
         FX_LAUNCH();
-        crt_gemm_kernel<<<CRT_NPROB * tiles_per, 256, 0, st>>>(crt_probs, tiles_per, T, G, KB);
+        crt_gemm_kernel<<<CRT_NPROB * tiles_per, 256, 0, st>>>(probs, tiles_per, T, G, KB);
         FX_LAUNCH();
         crt_combine_sq<<<T64 * (T64 + 1) / 2, 256, 0, st>>>(crt_res, rows, crt_e, cur->r, cur->i, ld, n, dsq[dc],
                                                              dsq[dc ^ 1], nxt->r, nxt->i, tab);
@@ -2767,8 +2861,8 @@ int fexpm_set_option(fexpm_engine* e, int key, int64_t value) {
       if (e->pipe) e->pipe->oz_scope = (int)value;
       return FEXPM_OK;
     case FEXPM_OPT_FP64_SLICES:
-      if (value != 0 && (value < 4 || value > OZ_MAX_SLICES)) {
-        e->err = "int8 slices must be 0 or in [4, 6]";
+      if (value != 0 && value != CRT64_K && (value < 4 || value > OZ_MAX_SLICES)) {
+        e->err = "int8 slices must be 0, in [4, 6], or 13 (residue products)";
         return FEXPM_EINVAL;
       }
       if (value != 0 && e->dtype != FEXPM_COMPLEX64 && e->dtype != FEXPM_FLOAT32) {
@@ -3645,8 +3739,47 @@ void oz_run_t(int sym, int m, int n, int k, const double* a, const double* b, do
 }
 }  // namespace
 
+// The residue form (slices == CRT64_K): the engine's product routine of an
+// FP64 pipeline of order m, operands A (and B when b != a) on the device.
+static void crt64_run(int m, int k, const double* a, const double* b, double* c) {
+  Pipeline<double> P(m, FEXPM_COMPLEX128, false);
+  double *da = nullptr, *db = nullptr, *dc = nullptr;
+  struct Free {
+    double** p[3];
+    ~Free() {
+      for (double** q : p) (void)hipFree(*q);
+    }
+  } fr{{&da, &db, &dc}};
+  const bool same = a == b;
+  FX_HIP(hipMalloc(&da, (size_t)m * k * sizeof(double)));
+  if (!same) FX_HIP(hipMalloc(&db, (size_t)m * k * sizeof(double)));
+  FX_HIP(hipMalloc(&dc, (size_t)m * m * sizeof(double)));
+  FX_HIP(hipMemcpy(da, a, (size_t)m * k * sizeof(double), hipMemcpyHostToDevice));
+  if (!same) FX_HIP(hipMemcpy(db, b, (size_t)m * k * sizeof(double), hipMemcpyHostToDevice));
+  Epilogue<double> e;
+  std::memset(&e, 0, sizeof(e));
+  e.cr = dc;
+  e.ldc = m;
+  e.ld_add = m;
+  e.alpha = 1.0;
+  using Op = Pipeline<double>::OzOp;
+  const Op A{da, 1.0, nullptr, 0.0, k, k};
+  const Op B{db, 1.0, nullptr, 0.0, k, k};
+  P.crt64_product(A, same ? nullptr : &B, m, k, e, nullptr);
+  FX_HIP(hipDeviceSynchronize());
+  FX_HIP(hipMemcpy(c, dc, (size_t)m * m * sizeof(double), hipMemcpyDeviceToHost));
+}
+
 extern "C" int fexpm_test_ozaki_gemm(int slices, int sym, int m, int n, int k, const double* a, const double* b,
                                      double* c) {
+  if (slices == CRT64_K) {
+    if (!a || !b || !c || !sym || m <= 0 || m != n || k <= 0 || round_up(k, CRT_KPAD) > CRT64_KMAX)
+      return FEXPM_EINVAL;
+    return guarded(nullptr, [&] {
+      check_device();
+      crt64_run(m, k, a, b, c);
+    });
+  }
   if (!a || !b || !c || m <= 0 || n <= 0 || k <= 0 || (sym && m != n) || slices < 3 || slices > OZ_MAX_SLICES)
     return FEXPM_EINVAL;
   return guarded(nullptr, [&] {

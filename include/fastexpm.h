@@ -84,9 +84,12 @@ enum fexpm_option {
   FEXPM_OPT_GRAPH = 10,          /* grouped batches: 1 = capture the program in a hipGraph and replay it
                                     while the plans stay the same (default), 0 = eager launches */
   FEXPM_OPT_FP64_SLICES = 11,    /* complex64 / float32 engines, structured input: the symmetric FP64
-                                    Pade products from S exact int8 slices per value (S = 4..6, about
-                                    7S - 1 bits of each row's largest value) on the int8 MFMA with int32
-                                    accumulation; default 6 (the solve stays on FP64 MFMA); 0 = FP64 MFMA */
+                                    Pade products (and the solve's S and X = Z Z^T where the scope
+                                    allows) on the int8 MFMA: 13 (default) = residue products modulo 13
+                                    coprime moduli (each row rounded to 43 bits of its largest value,
+                                    exact int32 products, Chinese-remainder reconstruction; products
+                                    with k > 20480 take 6 slices); S = 4..6 = S exact int8 slices per
+                                    value (about 7S - 1 bits) with int32 accumulation; 0 = FP64 MFMA */
   FEXPM_OPT_SPECTRAL = 12,       /* structured input (A = iG, G real symmetric, hence normal), degree 13:
                                     1 = s from the spectral-radius bound min_k ||B^k||_1^(1/k), k = 1, 2,
                                     4, 6, of the shifted B (default; single runs), 0 = s from ||B||_1

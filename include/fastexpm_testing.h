@@ -113,7 +113,11 @@ int fexpm_debug_hamiltonian(fexpm_engine* e, double* h);
 /* C = A B^T (FP64, host row-major A m x k, B n x k, C m x n) computed from
  * exact int8 slices of the rows of A and B (`slices` in 3..6) with int32
  * accumulation on the int8 MFMA and FP64 combination; with sym, B must be A
- * and only lower tiles are computed (mirrored). */
+ * and only lower tiles are computed (mirrored).  slices == 13: the residue
+ * form of the engine (rows rounded to beta bits, exact products modulo 13
+ * moduli, Chinese-remainder reconstruction), sym and m == n only; the lower
+ * triangle of A B^T is computed and mirrored (B may differ from A: the
+ * engine uses it for products that are symmetric in exact arithmetic). */
 int fexpm_test_ozaki_gemm(int slices, int sym, int m, int n, int k, const double* a, const double* b, double* c);
 
 /* MFMA flops one FP32 structured (complex symmetric, 3M) squaring of order n

@@ -173,6 +173,54 @@ def test_gemm_int8_slices(lib, slices, shape, sym):
         assert np.array_equal(c, c.T)
 
 
+CRT64_MODULI = (255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199)
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (64, 40), (257, 64), (300, 513), (640, 10016), (300, 20000)])
+@pytest.mark.parametrize("same", [True, False])
+def test_gemm_int8_residues(lib, shape, same):
+    """FP64-accurate product from exact int8 residues (the engine's default
+    for the Pade and solve products of complex64 engines): each row rounded
+    to beta = min(45, floor((log2 M - 1 - log2 kpad) / 2)) bits of its largest
+    element, 13 exact int8 products modulo coprime moduli, Chinese-remainder
+    reconstruction.  The lower triangle of A B^T within the rounding bound
+    k 2^(e_i + f_j - beta) (2^e_i < 2 max_k |a_ik|), the output mirrored;
+    rows of very different scale and a zero row."""
+    import math
+
+    m, k = shape
+    _say(f"int8 residue product m={m} k={k} same={same}")
+    rng = np.random.default_rng(m + 3 * k + int(same))
+    a = rng.standard_normal((m, k)) * np.exp2(rng.integers(-30, 30, size=(m, 1)))
+    a[0, :] = 0.0  # a zero row
+    b = a.copy() if same else rng.standard_normal((m, k)) * np.exp2(rng.integers(-30, 30, size=(m, 1)))
+    a = np.ascontiguousarray(a)
+    b = a if same else np.ascontiguousarray(b)
+    c = np.zeros((m, m))
+    rc = lib.fexpm_test_ozaki_gemm(13, 1, m, m, k, _p(a), _p(b), _p(c))
+    assert rc == 0, lib.fexpm_last_error(None)
+    kpad = (k + 63) // 64 * 64
+    log2m = math.log2(math.prod(CRT64_MODULI))
+    beta = min(45, math.floor((log2m - 1.0 - math.log2(kpad)) / 2.0))
+    ref = a @ b.T
+    bound = k * 4.0 * np.outer(np.abs(a).max(1), np.abs(b).max(1)) * 2.0 ** -beta
+    low = np.tril(np.ones((m, m), dtype=bool))
+    err = np.abs(c - ref)[low]
+    worst = float((err / np.maximum(bound[low], 1e-300)).max())
+    print(f"  beta={beta} worst error / bound = {worst:.3e}", flush=True)
+    assert np.all(err <= bound[low]), worst
+    assert np.array_equal(c, c.T)
+    # the bound is tight to a few bits, not loose: a product without the
+    # rounding (integers well inside beta bits) is exact
+    ai = rng.integers(-1000, 1000, size=(m, k)).astype(np.float64)
+    ci = np.zeros((m, m))
+    assert lib.fexpm_test_ozaki_gemm(13, 1, m, m, k, _p(ai), _p(ai), _p(ci)) == 0
+    if m > 1:
+        # integer entries: |a| < 1024 = 2^10 <= 2^beta, so every value is kept exactly
+        assert np.array_equal(ci, ai @ ai.T)
+    assert lib.fexpm_test_ozaki_gemm(13, 0, m, m, k, _p(a), _p(b), _p(c)) != 0  # residue form: sym only
+
+
 def test_square_issued_plan(lib):
     """Issued MFMA flops of one FP32 structured squaring: 64 x 64 lower tiles
     up to n = 6144; above it 128 x 128 tiles whose last partial wave may run

@@ -406,11 +406,12 @@ def test_precision_auto_default(fx):
         assert e.stats()["precision"] == "native"
 
 
-@pytest.mark.parametrize("slices,scope", [(6, 1), (5, 1), (6, 3), (5, 3)])
-@pytest.mark.parametrize("n", [64, 300, 700])
+@pytest.mark.parametrize("slices,scope", [(13, 1), (13, 3), (6, 1), (5, 1), (6, 3), (5, 3)])
+@pytest.mark.parametrize("n", [64, 300, 700, 1100])
 def test_fp64_slice_products_ctqw(fx, slices, scope, n):
-    """Complex64 engine with symmetric FP64 products from int8 slices (the
-    Pade products by default, scope 3 also S and X = Z Z^T of the solve): U
+    """Complex64 engine with the symmetric FP64 products on the int8 MFMA:
+    residue products modulo 13 moduli (the default) or int8 slices (the Pade
+    products by default, scope 3 also S and X = Z Z^T of the solve): U
     against the eigen oracle within the complex64 tolerance, and close to
     the U of the FP64-MFMA products."""
     from fastctqw_amd.fastexpm import _fastexpm_core as core
@@ -422,7 +423,7 @@ def test_fp64_slice_products_ctqw(fx, slices, scope, n):
     ref = (V * np.exp(1j * lam)[None, :]) @ V.T
     with fx.ExpMatComplex64(n) as e:
         e.run(a)
-        assert e.stats()["fp64_slices"] == 6 and e.stats()["fp64_slices_scope"] == 1  # the default
+        assert e.stats()["fp64_slices"] == 13 and e.stats()["fp64_slices_scope"] == 1  # the default
         e._backend.set_option(core.OPT_FP64_SLICES, 0)
         u64 = e.run(a)
         assert e.stats()["fp64_slices"] == 0
