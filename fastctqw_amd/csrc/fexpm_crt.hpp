@@ -438,7 +438,7 @@ struct CrtTable64 {
   uint32_t m[CRT64_K];
   uint32_t magic[CRT64_K];
   uint32_t inv[CRT64_K];   // ((M / m)^-1) mod m
-  uint32_t c22[CRT64_K];   // 2^22 mod m
+  uint32_t c24[CRT64_K];   // 2^24 mod m
   double rcp_hi[CRT64_K];  // 1 / m = rcp_hi + rcp_lo (double-double)
   double rcp_lo[CRT64_K];
   double M_hi, M_lo;       // M = M_hi + M_lo
@@ -467,26 +467,32 @@ __global__ __launch_bounds__(256) void crt64_row_exp(const double* __restrict__ 
   if (lane == 0) e[r] = ex;
 }
 
-// Symmetric residue of an integer-valued double |x| <= 2^45 (odd m in
-// [199, 255]): x = h 2^22 + l with h = floor(x 2^-22) in [-2^23, 2^23] and
-// l in [0, 2^22); h mod m from u = h + m 2^16 in (0, 2^25), then
-// (h mod m) 2^22 + l = (h mod m) c22 + l (mod m) with the sum below 2^23.
-__device__ __forceinline__ int crt64_sres(double x, uint32_t m, uint32_t magic, uint32_t c22) {
-  const double hd = floor(x * 0x1p-22);
-  const int h = (int)hd;
-  const uint32_t l = (uint32_t)(x - hd * 0x1p22);
-  const uint32_t rh = crt_umod((uint32_t)(h + (int)(m << 16)), m, magic);
-  const int r = (int)crt_umod(rh * c22 + l, m, magic);
+// Split of an integer-valued double |x| <= 2^45: x = h 2^24 + l with
+// h = floor(x 2^-24) in [-2^21, 2^21] and l in [0, 2^24) (both exact).
+__device__ __forceinline__ void crt64_split(double x, int& h, uint32_t& l) {
+  const double hd = floor(x * 0x1p-24);
+  h = (int)hd;
+  l = (uint32_t)(x - hd * 0x1p24);
+}
+
+// Symmetric residue (odd m in [199, 255]) of h 2^24 + l: with c24 = 2^24 mod
+// m, u = h c24 + l + m 2^22 lies in (0, 2^31) (|h c24| < 2^29 <= m 2^22,
+// l < 2^24), so one exact reduction suffices.
+__device__ __forceinline__ int crt64_sres(int h, uint32_t l, uint32_t m, uint32_t magic, uint32_t c24) {
+  const uint32_t u = (uint32_t)(h * (int)c24 + (int)(m << 22)) + l;
+  const int r = (int)crt_umod(u, m, magic);
   return r > (int)(m >> 1) ? r - (int)m : r;
 }
 
 // Operand residues (CRT64_K planes of rows x kpad bytes, fragment layout) of
 // x' = rint(x 2^(beta - e_r)); rows >= nr and k >= nk are zero.  One wave per
-// fragment (k-block kb, row group g), four per workgroup.
+// fragment (k-block kb, row group g), four per workgroup.  vec: both parts
+// start 16-byte aligned, ld is even and k1 % 16 == 0, so a lane's 16 values
+// (k0 % 16 == 0) lie in one part and load as 8 pairs.
 __global__ __launch_bounds__(256) void crt64_residues(const double* __restrict__ x1, double s1,
                                                        const double* __restrict__ x2, double s2, int k1, int64_t ld,
                                                        int nr, int nk, const int* __restrict__ e, int beta, int rows,
-                                                       int kpad, CrtTable64 tab, int8_t* __restrict__ out) {
+                                                       int kpad, int vec, CrtTable64 tab, int8_t* __restrict__ out) {
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int G = rows / 32;
   const int64_t jobs = (int64_t)G * (kpad / 32);
@@ -497,26 +503,43 @@ __global__ __launch_bounds__(256) void crt64_residues(const double* __restrict__
   double x[16];
 #pragma unroll
   for (int q = 0; q < 16; ++q) x[q] = 0.0;
-  if (row < nr) {
+  if (row < nr && k0 < nk) {
     const double sc = ldexp(1.0, beta - e[row]);  // |x sc| < 2^beta
+    if (vec && k0 + 16 <= nk) {
+      const bool first = k0 < k1;
+      const double* src = first ? x1 + (int64_t)row * ld + k0 : x2 + (int64_t)row * ld + (k0 - k1);
+      const double sg = (first ? s1 : s2) * sc;  // +-2^(beta - e): exact products
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int k = k0 + q;
-      if (k < nk) x[q] = rint(crt64_elem(x1, s1, x2, s2, k1, ld, row, k) * sc);
+      for (int c = 0; c < 8; ++c) {
+        const double2 v = reinterpret_cast<const double2*>(src)[c];
+        x[2 * c] = rint(v.x * sg);
+        x[2 * c + 1] = rint(v.y * sg);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int k = k0 + q;
+        if (k < nk) x[q] = rint(crt64_elem(x1, s1, x2, s2, k1, ld, row, k) * sc);
+      }
     }
   }
+  int h[16];
+  uint32_t l[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) crt64_split(x[q], h[q], l[q]);
   const int64_t plane = (int64_t)rows * kpad;
   const int64_t off = ((int64_t)kb * G + g) * 1024 + lane * 16;
 #pragma unroll 1
   for (int i = 0; i < CRT64_K; ++i) {
-    const uint32_t m = tab.m[i], mg = tab.magic[i], c = tab.c22[i];
+    const uint32_t m = tab.m[i], mg = tab.magic[i], c = tab.c24[i];
     uint4 v;
     uint32_t* pv = &v.x;
 #pragma unroll
     for (int cc = 0; cc < 4; ++cc) {
       uint32_t wv = 0;
 #pragma unroll
-      for (int t = 0; t < 4; ++t) wv |= ((uint32_t)crt64_sres(x[4 * cc + t], m, mg, c) & 0xFFu) << (8 * t);
+      for (int t = 0; t < 4; ++t)
+        wv |= ((uint32_t)crt64_sres(h[4 * cc + t], l[4 * cc + t], m, mg, c) & 0xFFu) << (8 * t);
       pv[cc] = wv;
     }
     *reinterpret_cast<uint4*>(out + (int64_t)i * plane + off) = v;

@@ -264,7 +264,7 @@ static const CrtTable64& crt_table64() {
       const uint32_t m = mods[i];
       t.m[i] = m;
       t.magic[i] = (uint32_t)(((1ull << 39) + m - 1) / m);
-      t.c22[i] = (uint32_t)((1u << 22) % m);
+      t.c24[i] = (uint32_t)((1u << 24) % m);
       t.rcp_hi[i] = 1.0 / (double)m;
       t.rcp_lo[i] = std::fma(-t.rcp_hi[i], (double)m, 1.0) / (double)m;
       uint64_t mi = 1;
@@ -842,8 +842,10 @@ template <typename W> struct Pipeline : PipelineBase {
       const OzOp& x = b == 0 ? A : *B;
       crt64_row_exp<<<(m + 3) / 4, 256, 0, st>>>(x.p1, x.s1, x.p2, x.s2, x.k1, x.ldx, m, nk, c64_e[b]);
       FX_LAUNCH();
+      const int vec = (x.k1 % 16 == 0 && x.ldx % 2 == 0 && (uintptr_t)x.p1 % 16 == 0 &&
+                       (!x.p2 || (uintptr_t)x.p2 % 16 == 0)) ? 1 : 0;
       crt64_residues<<<(unsigned)((jobs + 3) / 4), 256, 0, st>>>(x.p1, x.s1, x.p2, x.s2, x.k1, x.ldx, m, nk,
-                                                                  c64_e[b], beta, rows, kpad, tab, c64_op[b]);
+                                                                  c64_e[b], beta, rows, kpad, vec, tab, c64_op[b]);
       FX_LAUNCH();
     }
     CrtProbs probs;
