@@ -97,6 +97,43 @@ def squaring_roofline(n, ms_square, s, two, issued_3m, peak, split_parts=0):
     }
 
 
+INT8_PEAK_TOPS = 5000.0  # MI355X dense int8 MFMA
+
+
+def crt_roofline(n, kernel_ms, launches, ms_square, s):
+    """Roofline fields of the int8 residue squarings: the dominant kernel is
+    the batched residue product (27 int8 products per squaring: yr', yi' and
+    s' = yr' + yi' against themselves, modulo 9 moduli, lower 256 x 256
+    tiles).  Its algorithmic work per launch is 27 n^3 int8 operations (the
+    lower half of each n x n x n product: n^2/2 outputs, n multiply-adds,
+    2 operations each); ``issued`` counts the whole lower tiles and the padded
+    k.  kernel_ms is the launches' own device time (HIP events around each
+    launch on the squaring stream).  The squaring phase is also expressed as
+    the FP32-equivalent rate of the structured algorithm it replaces (3 n^3
+    real flops per squaring), against the FP32 MFMA peak."""
+    launches = max(int(launches), 1)
+    rows = (n + 255) // 256 * 256
+    kpad = (n + 63) // 64 * 64
+    T = rows // 256
+    alg = 27.0 * float(n) ** 3
+    issued = 27.0 * 2.0 * (T * (T + 1) // 2) * 256.0 * 256.0 * kpad
+    t_launch = kernel_ms / launches * 1e-3
+    t_phase = ms_square * 1e-3
+    return {
+        "achieved": alg / t_launch / 1e12, "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
+        "frac": alg / t_launch / 1e12 / INT8_PEAK_TOPS,
+        "algorithmic_ops_per_launch": alg, "issued_ops_per_launch": issued,
+        "issued_tops": issued / t_launch / 1e12, "issued_frac": issued / t_launch / 1e12 / INT8_PEAK_TOPS,
+        "fp32_equivalent": {
+            "flops_per_squaring": 3.0 * float(n) ** 3,
+            "tflops": 3.0 * float(n) ** 3 * max(int(s), 1) / t_phase / 1e12,
+            "frac_of_fp32_peak": 3.0 * float(n) ** 3 * max(int(s), 1) / t_phase / 1e12 / FP32_PEAK_TFLOPS,
+            "note": "whole squaring phase (row exponents, residues, products, reconstruction) as the rate of "
+                    "the FP32 structured algorithm it replaces, whose results it reproduces more accurately",
+        },
+    }
+
+
 def work_flops(n, s, products=6):
     # 4M complex convention: 8 n^3 per complex product/squaring, LU 8/3 n^3, solve 8 n^3
     return (8.0 * (products + s) + 32.0 / 3.0) * float(n) ** 3
@@ -422,7 +459,7 @@ def run_general(args):
 
 
 C3_TIMES = (1.0, 4.5, 20.0)
-C3_PRECISIONS = ("native", "fp16x2", "bf16x3")
+C3_PRECISIONS = ("native", "int8crt", "fp16x2", "bf16x3")
 C3_PROBES = 8
 
 
@@ -471,10 +508,11 @@ def _c3_reference(n, out_path):
 def run_c3(args):
     """Precision study at N=10^4: unweighted and weighted ER(N,0.5)
     Laplacians, t in {1, 4.5, 20}, complex64; squarings in FP32 MFMA
-    (native) versus fp16x2 / bf16x3 split products with FP32 accumulation
-    (Pade products, solve and norm in FP64 in every mode).  Per case: device
-    ms per expm, infidelity of U e_0 and of U psi_rand, and ||U - U_ref||_F /
-    ||U_ref||_F estimated with Gaussian probes."""
+    (native), from exact int8 residue products (int8crt, the default at this
+    order) and as fp16x2 / bf16x3 split products with FP32 accumulation
+    (Pade products, solve and norm FP64-accurate in every mode).  Per case:
+    device ms per expm, infidelity of U e_0 and of U psi_rand, and
+    ||U - U_ref||_F / ||U_ref||_F estimated with Gaussian probes."""
     import subprocess
     import tempfile
 
@@ -515,6 +553,7 @@ def run_c3(args):
                     eng.run_resident()
                     ms.append(eng.stats()["ms_total"])
                 st = eng.stats()
+                last_st = st
                 case = {"graph": "weighted" if weighted else "unweighted", "t": t, "precision": prec,
                         "s": st["s"], "ms": float(np.median(ms)), "ms_square": st["ms_square"]}
                 if ref is not None:
@@ -532,12 +571,12 @@ def run_c3(args):
     native = next(c for c in head if c["precision"] == "native")
     best = min(head, key=lambda c: c["ms"])
     res = {
-        "metric": "s per N=10k complex expm, split-precision squarings (fastest mode at t=4.5, unweighted)",
+        "metric": "s per N=10k complex expm, squaring precision study (fastest mode at t=4.5, unweighted)",
         "value": best["ms"] / 1000.0, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": best["ms"], "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": f"c64 ({best['precision']} squarings, pade products from 6 exact int8 slices, fp64 solve)",
+        "dtype": f"c64 ({_square_phrase(best['precision'])}, {_slice_phrase(last_st)}",
         "data": "synthetic: seeded ER(10000,0.5) Laplacian, unweighted and weighted U(0,1]",
-        "config": {"workload": "C3: expm(-i t L), t in {1, 4.5, 20}, precision native/fp16x2/bf16x3",
+        "config": {"workload": "C3: expm(-i t L), t in {1, 4.5, 20}, precision native/int8crt/fp16x2/bf16x3",
                    "n": n, "native_ms_t4.5": native["ms"], "speedup_vs_native": native["ms"] / best["ms"]},
         "cases": cases,
     }
@@ -993,7 +1032,20 @@ def _slice_phrase(st):
         return "fp64 pade and solve)"
     where = {1: "pade products", 2: "solve products", 3: "pade and solve products"}.get(scope, "products")
     rest = {1: ", fp64 cholesky solve", 2: ", fp64 pade", 3: ", fp64 cholesky factorisation"}.get(scope, "")
-    return f"fp64-accurate {where} from {k} exact int8 slices (int32 accumulation){rest})"
+    if k == 13:
+        how = "exact int8 residue products (13 moduli, rows rounded to 43 bits, chinese-remainder reconstruction)"
+    else:
+        how = f"{k} exact int8 slices (int32 accumulation)"
+    return f"fp64-accurate {where} from {how}{rest})"
+
+
+def _square_phrase(prec):
+    """How the squarings of a complex64 run computed their products."""
+    return {"native": "fp32 mfma squarings",
+            "int8crt": "squarings from exact int8 residue products (9 moduli, rows rounded to 27 bits, exact "
+                       "int32 accumulation, chinese-remainder reconstruction)",
+            "fp16x2": "fp16x2 split squarings (fp32 accumulation)",
+            "bf16x3": "bf16x3 split squarings (fp32 accumulation)"}.get(prec, prec + " squarings")
 
 
 def _infidelity(a, b):
@@ -1034,7 +1086,7 @@ def main():
                     help="c2: N=10^4 single expm (default); c3: precision study at N=10^4; "
                          "c4: 256 uneven walks; c5: 32 times x 128 states, N=8192; general: dense general "
                          "complex64 expm at N (LU route) and the reference's performance inputs at 16..1024")
-    ap.add_argument("--precision", choices=list(C3_PRECISIONS) + ["int8crt"], default="native",
+    ap.add_argument("--precision", choices=list(C3_PRECISIONS) + ["auto"], default="auto",
                     help="c2: squaring precision of the complex64 engine")
     ap.add_argument("--no-split", action="store_true",
                     help="c2: skip the fp16x2 split-precision comparison run")
@@ -1143,7 +1195,7 @@ def main():
     print(f"[bench] input ready in {time.perf_counter() - t_gen:.1f} s", file=sys.stderr, flush=True)
 
     eng = (ExpMatComplex64 if cdt == np.complex64 else ExpMatComplex128)(args.n)
-    if args.precision != "native":
+    if cdt == np.complex64:
         eng.set_precision(args.precision)
     if args.kblock is not None:
         from fastctqw_amd.fastexpm import _fastexpm_core
@@ -1172,13 +1224,25 @@ def main():
     T = (n + 127) // 128
     np_pad = T * 128
     kpad = (n + 15) // 16 * 16
-    split = cdt == np.complex64 and args.precision != "native" and st["structured"]
-    if split:
-        npair = 3 if args.precision == "fp16x2" else 6
+    used = st.get("precision", "native")  # the squaring form the run used (the default resolves by n)
+    kernel_ms = float(np.mean([p.get("ms_square_kernel", 0.0) for p in phase]))
+    kernel_launches = st.get("square_kernel_launches", 0)
+    split = cdt == np.complex64 and used in ("fp16x2", "bf16x3") and st["structured"]
+    crt = cdt == np.complex64 and used == "int8crt" and st["structured"]
+    if crt:
+        roof = crt_roofline(n, kernel_ms, kernel_launches, float(np.mean([p["ms_square"] for p in phase])), s)
+        kname = ("crt_gemm_kernel (27 batched int8 products per squaring modulo 9 moduli on "
+                 "v_mfma_i32_32x32x32_i8: lower 256x256 tiles, 4 waves of 128x128, direct-to-LDS operand "
+                 "copies in a 4-stage ring interleaved with the MFMAs; byte residues out)")
+        traffic_key = "int8crt_squaring_gemm"
+        peak = INT8_PEAK_TOPS
+        issued_tf = roof["issued_tops"]
+    elif split:
+        npair = 3 if used == "fp16x2" else 6
         peak = FP16_PEAK_TFLOPS
         issued = 4 * npair * 2.0 * (T * (T + 1) // 2) * 128 * 128 * kpad
-        kname = f"sq_split_kernel<{args.precision}> (4M complex, 16-bit MFMA, FP32 accumulation)"
-        traffic_key = f"{args.precision}_squaring"
+        kname = f"sq_split_kernel<{used}> (4M complex, 16-bit MFMA, FP32 accumulation)"
+        traffic_key = f"{used}_squaring"
     else:
         peak = FP32_PEAK_TFLOPS if cdt == np.complex64 else FP64_PEAK_TFLOPS
         issued = (_issued_structured_squaring(n, kpad, cdt == np.complex64) if st["structured"]
@@ -1187,12 +1251,16 @@ def main():
                  "N=6144 lower 64x64 tiles of 4 waves, above it 128x128 tiles of 16 waves plus a 64x64 launch "
                  "for a last partial wave of tiles; 8x8 tile groups; each wave 32x32)")
         traffic_key = "fp32_squaring" if cdt == np.complex64 else None
-    roof = squaring_roofline(n, float(np.mean([p["ms_square"] for p in phase])), s, st.get("s_two_product", 0),
-                             issued, peak, split_parts=(3 if args.precision == "fp16x2" else 6) if split else 0)
-    issued_tf = roof["issued_tflops"]
+    if not crt:
+        # the squaring GEMM launches' own time (HIP events), else the phase's
+        sq_time = kernel_ms if kernel_launches == s and kernel_ms > 0 else float(np.mean([p["ms_square"]
+                                                                                          for p in phase]))
+        roof = squaring_roofline(n, sq_time, s, st.get("s_two_product", 0), issued, peak,
+                                 split_parts=(3 if used == "fp16x2" else 6) if split else 0)
+        issued_tf = roof["issued_tflops"]
     traffic = _profiled_traffic(traffic_key) if n == 10000 else None
-    sustained = _sustained_peak("fp16" if split and args.precision == "fp16x2" else
-                                "bf16" if split else "fp32") if cdt == np.complex64 else None
+    sustained = (_sustained_peak("int8" if crt else "fp16" if split and used == "fp16x2" else
+                                 "bf16" if split else "fp32") if cdt == np.complex64 else None)
 
     # accuracy: invariants and, unless disabled, infidelity against an FP64
     # eigendecomposition of the same H computed in a child process
@@ -1236,8 +1304,7 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": (f"c64 ({'fp32 mfma' if args.precision == 'native' else args.precision} squarings, "
-                  f"{_slice_phrase(st)}" if cdt == np.complex64 else "c128 (fp64)"),
+        "dtype": (f"c64 ({_square_phrase(used)}, {_slice_phrase(st)}" if cdt == np.complex64 else "c128 (fp64)"),
         "data": "synthetic: seeded ER(N,p) Laplacian, A=-i*t*L",
         "config": {"workload": f"expm(-i*{args.t}*L), L Laplacian of ER({n},{args.p}) seed {args.seed}, "
                                f"{args.dtype}, full U resident", "n": n, "t": args.t,
@@ -1247,36 +1314,63 @@ def main():
                                           "ms_total")},
         "roofline": {"kernel": kname, "bound": "mfma", **roof,
                      "traffic": traffic, "traffic_source": TRAFFIC_PROFILE if traffic else None,
-                     "avg_launch_ms": sq_ms, "launches_per_step": s,
+                     "avg_launch_ms": (kernel_ms / kernel_launches if kernel_launches else sq_ms),
+                     "squaring_ms": sq_ms, "launches_per_step": s,
                      "sustained_peak": sustained, "sustained_source": MFMA_PROFILE if sustained else None,
                      "issued_frac_of_sustained": issued_tf / sustained if sustained else None,
-                     "note": "achieved = the structured algorithm's flops of the s squarings (3 N^3 per 3M "
-                             "squaring, 2 N^3 per two-product squaring: real symmetric half products) / "
-                             "their device time from HIP events on the squaring stream; issued = MFMA flops "
-                             "the kernels execute (whole lower tiles; equal to the profiler's "
-                             "SQ_INSTS_VALU_MFMA_MOPS x 512); achieved_4m_convention = 8 N^3 per squaring, "
-                             "the general complex convention; traffic = HBM bytes per 3M launch from "
-                             "FETCH_SIZE / WRITE_SIZE scaled by a copy of known size"},
+                     "note": ("achieved = algorithmic int8 operations of one launch of the residue products "
+                              "(27 N^3) / its device time (HIP events around each launch on the squaring "
+                              "stream); issued = the whole lower tiles and padded k; fp32_equivalent = the "
+                              "squaring phase as the rate of the FP32 structured algorithm (3 N^3 per "
+                              "squaring); traffic = HBM bytes per launch from FETCH_SIZE / WRITE_SIZE scaled by "
+                              "a copy of known size" if crt else
+                              "achieved = the structured algorithm's flops of the s squarings (3 N^3 per 3M "
+                              "squaring, 2 N^3 per two-product squaring: real symmetric half products) / "
+                              "their device time from HIP events around the launches on the squaring stream; "
+                              "issued = MFMA flops the kernels execute (whole lower tiles; equal to the "
+                              "profiler's SQ_INSTS_VALU_MFMA_MOPS x 512); achieved_4m_convention = 8 N^3 per "
+                              "squaring, the general complex convention; traffic = HBM bytes per 3M launch "
+                              "from FETCH_SIZE / WRITE_SIZE scaled by a copy of known size")},
         "accuracy": acc,
         "memory": _memory_report(n, st),
     }
-    if cdt == np.complex64 and args.precision == "native" and not args.no_split:
-        # the same expm with fp16x2 split-precision squarings, same input and reference
-        eng.set_precision("fp16x2")
-        for _ in range(args.warmup):
-            eng.run_resident()
-        ms = []
-        for _ in range(args.steps):
-            eng.run_resident()
-            ms.append(eng.stats()["ms_total"])
-        sp = {"precision": "fp16x2", "ms_per_step": float(np.median(ms)),
-              "ms_square": eng.stats()["ms_square"], "speedup_vs_fp32": ms_step / float(np.median(ms))}
-        if ref is not None:
-            psi = eng.apply(np.vstack([np.eye(1, n, dtype=np.complex128), random_state(n, seed=55)[None, :]]))
-            sp["infidelity_e0"] = _infidelity(ref["psi_e0"], psi[0])
-            sp["infidelity_random_state"] = _infidelity(ref["psi_rand"], psi[1])
+    if cdt == np.complex64 and args.precision == "auto" and not args.no_split:
+        # the same expm with other squaring forms, same input and reference:
+        # FP32 MFMA (the plain form the default is measured against) and
+        # fp16x2 split precision
+        def variant(prec):
+            eng.set_precision(prec)
+            for _ in range(args.warmup):
+                eng.run_resident()
+            ms, vst = [], []
+            for _ in range(args.steps):
+                eng.run_resident()
+                vst.append(eng.stats())
+                ms.append(vst[-1]["ms_total"])
+            v = {"precision": vst[-1]["precision"], "ms_per_step": float(np.median(ms)),
+                 "ms_square": float(np.median([q["ms_square"] for q in vst])),
+                 "ms_per_squaring_launch": float(np.median([q["ms_square_kernel"] for q in vst])) /
+                 max(vst[-1]["square_kernel_launches"], 1),
+                 "default_speedup": float(np.median(ms)) / ms_step}
+            if ref is not None:
+                psi = eng.apply(np.vstack([np.eye(1, n, dtype=np.complex128),
+                                           random_state(n, seed=55)[None, :]]))
+                v["infidelity_e0"] = _infidelity(ref["psi_e0"], psi[0])
+                v["infidelity_random_state"] = _infidelity(ref["psi_rand"], psi[1])
+            return v, vst[-1]
+
+        if used != "native":
+            fp32, fst = variant("native")
+            kl = max(fst["square_kernel_launches"], 1)
+            fp32["roofline"] = squaring_roofline(n, fst["ms_square_kernel"], fst["s"], 0,
+                                                 _issued_structured_squaring(n, kpad, True), FP32_PEAK_TFLOPS)
+            fp32["roofline"]["avg_launch_ms"] = fst["ms_square_kernel"] / kl
+            out["fp32_squarings"] = fp32
+        sp, _ = variant("fp16x2")
+        sp["speedup_vs_fp32"] = (out["fp32_squarings"]["ms_per_step"] if "fp32_squarings" in out
+                                 else ms_step) / sp["ms_per_step"]
         out["split_precision"] = sp
-        eng.set_precision("native")
+        eng.set_precision("auto")
     if not args.no_api_timing:
         out["api_timing"] = _api_timing(eng, a, args)
     if cdt == np.complex64 and not args.no_c128:

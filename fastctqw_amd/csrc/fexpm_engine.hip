@@ -572,6 +572,22 @@ template <typename W> struct Pipeline : PipelineBase {
   std::map<int, int*> sq_orders;  // grouped lower-triangular tile orders, per tile count
   hipEvent_t ev[8];
   bool events = false;
+  // events around each squaring's product launch (kernel-level timing of
+  // single runs): pairs, created on first use
+  std::vector<hipEvent_t> kev;
+  bool time_kernels = false;
+  int kev_used = 0;
+  void kernel_mark(int q, bool end, hipStream_t st) {
+    if (!time_kernels) return;
+    const size_t need = 2 * (size_t)q + 2;
+    while (kev.size() < need) {
+      hipEvent_t e;
+      FX_HIP(hipEventCreate(&e));
+      kev.push_back(e);
+    }
+    FX_HIP(hipEventRecord(kev[2 * q + (end ? 1 : 0)], st));
+    if (end) kev_used = std::max(kev_used, q + 1);
+  }
   // result of the last run (for apply)
   Plane2<W> res;
   double ph_re = 1.0, ph_im = 0.0;
@@ -701,6 +717,7 @@ template <typename W> struct Pipeline : PipelineBase {
     if (zst) (void)hipStreamDestroy(zst);
     if (events)
       for (int k = 0; k < 8; ++k) (void)hipEventDestroy(ev[k]);
+    for (hipEvent_t e : kev) (void)hipEventDestroy(e);
   }
 
   bool has_result() const override { return have_result; }
@@ -1463,8 +1480,10 @@ template <typename W> struct Pipeline : PipelineBase {
       a.k = kpad;
       a.kblock = kblock > 0 ? kblock : (1 << 30);
       a.order = sq_order;
+      kernel_mark(q, false, st);
       sq_split_kernel<P><<<dim3(nb), dim3(sq_threads<P>()), 0, st>>>(a);
       FX_LAUNCH();
+      kernel_mark(q, true, st);
       flops += 4.0 * P::NPAIR * 2.0 * nb * (double)SQ_BM * SQ_BM * (double)kpad;
       std::swap(cur, nxt);
       dc ^= 1;
@@ -1512,8 +1531,10 @@ template <typename W> struct Pipeline : PipelineBase {
         crt_residues_sq<<<(unsigned)((jobs + 3) / 4), 256, 0, st>>>(cur->r, cur->i, ld, n, crt_e, rows, kpad, tab,
                                                                      crt_ops);
         FX_LAUNCH();
+        kernel_mark(q, false, st);
         crt_gemm_kernel<<<CRT_NPROB * tiles_per, 256, 0, st>>>(probs, tiles_per, T, G, KB);
         FX_LAUNCH();
+        kernel_mark(q, true, st);
         crt_combine_sq<<<T64 * (T64 + 1) / 2, 256, 0, st>>>(crt_res, rows, crt_e, cur->r, cur->i, ld, n, dsq[dc],
                                                              dsq[dc ^ 1], nxt->r, nxt->i, tab);
         FX_LAUNCH();
@@ -1590,6 +1611,7 @@ template <typename W> struct Pipeline : PipelineBase {
       const bool small = std::is_same<W, float>::value && square_tile(n, sq_tile) == 64;
       // (FCfgSmall runs 32x32x2 FP32 MFMAs; FP64 engines never take this branch)
       using SmallC = typename std::conditional<std::is_same<W, float>::value, FCfgSmall, GemmCfg<W>>::type;
+      kernel_mark(q, false, st);
       if (structured && sq2) {
         if (blk && small) gemm<MODE_SQ2, true, true, SmallC>(g, st);
         else if (blk) square_big_tail<MODE_SQ2>(g, st);
@@ -1603,6 +1625,7 @@ template <typename W> struct Pipeline : PipelineBase {
         else if (blk) gemm<MODE_C3M, false, true>(g, st);
         else gemm<MODE_C3M, false>(g, st);
       }
+      kernel_mark(q, true, st);
       std::swap(cur, nxt);
       dc ^= 1;
     }
@@ -1886,6 +1909,8 @@ template <typename W> struct Pipeline : PipelineBase {
   template <typename Tin, typename Tout>
   void run_typed(const Tin* d_a, Tout* d_u, hipStream_t st, fexpm_stats& S, int opt_structure) {
     const bool timing = events;  // pipelines without events (no-allocation path) report no phase times
+    time_kernels = timing;
+    kev_used = 0;
     if (timing) FX_HIP(hipEventRecord(ev[0], st));
     analyze_enqueue<Tin>(d_a, st, nullptr);
     Scalars hs;
@@ -1901,6 +1926,9 @@ template <typename W> struct Pipeline : PipelineBase {
     fill_stats(P, S);
     if (P.nonfinite || !timing) {
       S.ms_total = S.ms_analyze = S.ms_pade = S.ms_solve = S.ms_square = S.ms_output = 0.0;
+      S.ms_square_kernel = 0.0;
+      S.square_kernel_launches = 0;
+      time_kernels = false;
       if (!P.nonfinite) check_info(info);
       return;
     }
@@ -1916,6 +1944,15 @@ template <typename W> struct Pipeline : PipelineBase {
     S.ms_square = t34;
     S.ms_output = t45;
     S.ms_total = t01 + t12 + t23 + t34 + t45;
+    double kms = 0.0;
+    for (int q = 0; q < kev_used; ++q) {
+      float t = 0.f;
+      FX_HIP(hipEventElapsedTime(&t, kev[2 * q], kev[2 * q + 1]));
+      kms += t;
+    }
+    S.ms_square_kernel = kms;
+    S.square_kernel_launches = kev_used;
+    time_kernels = false;
     check_info(info);
   }
 

@@ -78,6 +78,8 @@ static py::dict stats_dict(const fexpm_stats& s) {
   d["rho_bound"] = s.rho_bound;
   d["s_two_product"] = s.s_two_product;
   d["device_bytes"] = s.device_bytes;
+  d["ms_square_kernel"] = s.ms_square_kernel;
+  d["square_kernel_launches"] = s.square_kernel_launches;
   d["ms_propagate"] = s.ms_propagate;
   return d;
 }

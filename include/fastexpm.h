@@ -142,6 +142,11 @@ typedef struct fexpm_stats {
   int64_t device_bytes;  /* device memory the engine holds now (work planes, slices, I/O buffers, the
                             walk Hamiltonian); the FP64 planes of a complex64 engine are allocated
                             at its first run */
+  double ms_square_kernel;  /* device time of the squarings' product launches alone (HIP events
+                               around them on the squaring stream): the batched int8 residue
+                               products of FEXPM_PREC_INT8_CRT squarings, else the squaring GEMMs */
+  int32_t square_kernel_launches;  /* number of those timed launches (one per squaring) */
+  int32_t reserved2;
 } fexpm_stats;
 
 /* Create an engine for n x n matrices of the given dtype and allocate its

@@ -164,7 +164,7 @@ def test_spectral_scaling(fx, dt, case):
 def test_c3_precision(fx, weighted):
     """The precision study at test scale: the dense ER(n, 0.5) Laplacian
     (unweighted, and weighted with w ~ U(0, 1]), t in {1, 4.5, 20}, complex64
-    with FP32 (native), fp16x2 and bf16x3 squarings.  Per case: m and s as
+    with FP32 (native), int8 residue (int8crt), fp16x2 and bf16x3 squarings.  Per case: m and s as
     the oracle's rule, U against the FP64 eigendecomposition of the same
     rounded input, and the infidelity of U e_0 and of U psi for a random
     state (device apply with FP64 refinement) at most 1e-9."""
@@ -183,7 +183,7 @@ def test_c3_precision(fx, weighted):
             g = a128.imag
             lam, V = np.linalg.eigh(g)
             ref = (V * np.exp(1j * lam)[None, :]) @ V.T
-            for prec in ("native", "fp16x2", "bf16x3"):
+            for prec in ("native", "int8crt", "fp16x2", "bf16x3"):
                 _say(f"C3 n={n} weighted={weighted} t={t} {prec}")
                 e.set_precision(prec)
                 u = e.run(a)
