@@ -97,9 +97,6 @@ def squaring_roofline(n, ms_square, s, two, issued_3m, peak, split_parts=0):
     }
 
 
-INT8_PEAK_TOPS = 5000.0  # MI355X dense int8 MFMA
-
-
 def crt_roofline(n, kernel_ms, launches, ms_square, s):
     """Roofline fields of the int8 residue squarings: the dominant kernel is
     the batched residue product (27 int8 products per squaring: yr', yi' and
