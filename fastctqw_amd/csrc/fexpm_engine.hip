@@ -283,6 +283,30 @@ static const CrtTable64& crt_table64() {
   return tab;
 }
 
+// A device buffer that grows on demand (freeing its previous contents; the
+// device synchronises in hipFree, so no launch still uses them) and that
+// several pipelines may share when they use it one after another on one
+// stream: the residue products of an FP32 pipeline's squarings and of its
+// FP64 companion's products.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { dfree(p); }
+  void* get(size_t need) {
+    if (need > bytes) {
+      dfree(p);
+      p = nullptr;
+      bytes = 0;
+      p = dmalloc_bytes(need);
+      bytes = need;
+    }
+    return p;
+  }
+};
+
 template <typename W> struct Plane2 {
   W* r = nullptr;
   W* i = nullptr;
@@ -561,14 +585,12 @@ template <typename W> struct Pipeline : PipelineBase {
   // (3 CRT_K planes of rows x kpad), product residues (3 CRT_K planes of
   // rows x rows), the launch's problem list for the current order
   int* crt_e = nullptr;
-  int8_t* crt_ops = nullptr;
-  uint8_t* crt_res = nullptr;
   // FP64 residue products (FP64 pipelines; oz_slices == CRT64_K): row
-  // exponents and operand residues of A and B, product residues
+  // exponents of A and B
   int* c64_e[2] = {nullptr, nullptr};
-  int8_t* c64_op[2] = {nullptr, nullptr};
-  size_t c64_op_bytes[2] = {0, 0};
-  uint8_t* c64_res = nullptr;
+  // operand and product residues of both: one buffer, shared with the FP64
+  // companion (companion())
+  std::shared_ptr<DevBuf> crt_buf = std::make_shared<DevBuf>();
   std::map<int, int*> sq_orders;  // grouped lower-triangular tile orders, per tile count
   hipEvent_t ev[8];
   bool events = false;
@@ -705,13 +727,7 @@ template <typename W> struct Pipeline : PipelineBase {
     }
     dfree(sbuf);
     dfree(crt_e);
-    dfree(crt_ops);
-    dfree(crt_res);
-    for (int b = 0; b < 2; ++b) {
-      dfree(c64_e[b]);
-      dfree(c64_op[b]);
-    }
-    dfree(c64_res);
+    for (int b = 0; b < 2; ++b) dfree(c64_e[b]);
     for (auto& kv : sq_orders) dfree(kv.second);
     for (hipEvent_t e : zev) (void)hipEventDestroy(e);
     if (zst) (void)hipStreamDestroy(zst);
@@ -839,20 +855,18 @@ template <typename W> struct Pipeline : PipelineBase {
     const int beta = std::min(CRT64_BETA_MAX, (int)std::floor((tab.log2M - 1.0 - std::log2((double)kpad)) * 0.5));
     const size_t op_bytes = (size_t)CRT64_K * rows * kpad;
     const int nops = B ? 2 : 1;
-    for (int b = 0; b < nops; ++b) {
+    for (int b = 0; b < nops; ++b)
       if (!c64_e[b]) c64_e[b] = dalloc<int>(round_up(npcap, CRT_TILE));
-      if (c64_op_bytes[b] < op_bytes) {
-        dfree(c64_op[b]);
-        c64_op[b] = nullptr;
-        c64_op_bytes[b] = 0;
-        c64_op[b] = static_cast<int8_t*>(dmalloc_bytes(op_bytes));
-        c64_op_bytes[b] = op_bytes;
-      }
-    }
-    if (!c64_res) {
-      const size_t rc = (size_t)round_up(npcap, CRT_TILE);
-      c64_res = static_cast<uint8_t*>(dmalloc_bytes((size_t)CRT64_K * rc * rc));
-    }
+    // operands of A (and B), then the product residues; sized once for the
+    // largest product of the pipeline (one operand with k = 2 npcap, or two
+    // with k = npcap) so that the buffer does not grow from one to the next
+    const size_t rc = (size_t)round_up(npcap, CRT_TILE);
+    const size_t cap_need = (size_t)CRT64_K * rc *
+                            (std::max((size_t)round_up(2 * npcap, CRT_KPAD), 2 * (size_t)round_up(npcap, CRT_KPAD)) + rc);
+    uint8_t* base = static_cast<uint8_t*>(crt_buf->get(std::max(cap_need, nops * op_bytes +
+                                                                 (size_t)CRT64_K * rows * rows)));
+    int8_t* c64_op[2] = {reinterpret_cast<int8_t*>(base), reinterpret_cast<int8_t*>(base + op_bytes)};
+    uint8_t* c64_res = base + nops * op_bytes;
     const int G = rows / 32, KB = kpad / 32, T = rows / CRT_TILE, tiles_per = T * (T + 1) / 2;
     const int64_t jobs = (int64_t)G * KB;
     for (int b = 0; b < nops; ++b) {
@@ -1502,11 +1516,13 @@ template <typename W> struct Pipeline : PipelineBase {
     } else {
       const int rows = (int)round_up(n, CRT_TILE), kpad = (int)round_up(n, CRT_KPAD);
       const int64_t rows_cap = round_up(ncap, CRT_TILE), kpad_cap = round_up(ncap, CRT_KPAD);
-      if (!crt_e) {
-        crt_e = dalloc<int>((size_t)npcap);
-        crt_ops = static_cast<int8_t*>(dmalloc_bytes((size_t)CRT_NPROB * rows_cap * kpad_cap));
-        crt_res = static_cast<uint8_t*>(dmalloc_bytes((size_t)CRT_NPROB * rows_cap * rows_cap));
-      }
+      if (!crt_e) crt_e = dalloc<int>((size_t)npcap);
+      const size_t ops_bytes = (size_t)CRT_NPROB * rows * kpad;
+      uint8_t* base = static_cast<uint8_t*>(
+          crt_buf->get(std::max((size_t)CRT_NPROB * rows_cap * (kpad_cap + rows_cap),
+                                ops_bytes + (size_t)CRT_NPROB * rows * rows)));
+      int8_t* crt_ops = reinterpret_cast<int8_t*>(base);
+      uint8_t* crt_res = base + ops_bytes;
       const CrtTable& tab = crt_table();
       const int64_t op_plane = (int64_t)rows * kpad, res_plane = (int64_t)rows * rows;
       CrtProbs probs;
@@ -1660,6 +1676,7 @@ template <typename W> struct Pipeline : PipelineBase {
   Pipeline<double>* companion() {
     if (!hi) {
       hi.reset(new Pipeline<double>(ncap, FEXPM_COMPLEX128, events));
+      hi->crt_buf = crt_buf;  // its residue products run before this pipeline's squarings
       hi->panel_coop = panel_coop;
       hi->set_size(n, nullptr);
       FX_HIP(hipDeviceSynchronize());
