@@ -1025,6 +1025,12 @@ def _host_info(cores):
 def _slice_phrase(st):
     """How the FP64 stages of a complex64 run computed their symmetric products."""
     k, scope = st.get("fp64_slices", 0), st.get("fp64_slices_scope", 0)
+    deg = st.get("chebyshev", 0)
+    if deg:
+        how = ("exact int8 residue products (13 moduli, rows rounded to 43 bits, chinese-remainder reconstruction)"
+               if k == 13 else (f"{k} exact int8 slices" if k else "fp64 mfma"))
+        return (f"degree-{deg} chebyshev polynomial of exp(iX) in place of pade and solve, fp64-accurate "
+                f"products from {how})")
     if not k:
         return "fp64 pade and solve)"
     where = {1: "pade products", 2: "solve products", 3: "pade and solve products"}.get(scope, "products")
@@ -1305,7 +1311,7 @@ def main():
         "data": "synthetic: seeded ER(N,p) Laplacian, A=-i*t*L",
         "config": {"workload": f"expm(-i*{args.t}*L), L Laplacian of ER({n},{args.p}) seed {args.seed}, "
                                f"{args.dtype}, full U resident", "n": n, "t": args.t,
-                   "m": st["m"], "s": s, "structured": st["structured"],
+                   "m": st["m"], "s": s, "structured": st["structured"], "chebyshev_degree": st.get("chebyshev", 0),
                    "pade_int8_slices": st.get("fp64_slices", 0), "int8_slice_scope": st.get("fp64_slices_scope", 0)},
         "phases_ms": {k: st[k] for k in ("ms_analyze", "ms_pade", "ms_solve", "ms_square", "ms_output",
                                           "ms_total")},

@@ -89,14 +89,16 @@ static void choose_degree(double norm, int& m, int& s) {
 
 // Smallest of the spectral-radius bounds of a real symmetric B:
 //   ||B^k||_1^(1/k), k = 1, 2, 4, 6 (h[0..3], the 1-norms), and
-//   trace(B^2k)^(1/2k), 2k = 4, 6, ..., 12 (h[4..8]; the sum of the 2k-th
-//   powers of the eigenvalues is at least rho^2k),
+//   trace(B^2k)^(1/2k), 2k = 4, 6, ..., 12, and 14, 16 when B^8 was formed
+//   (h[4..10], negative when absent; the sum of the 2k-th powers of the
+//   eigenvalues is at least rho^2k),
 // widened by 1e-9 to cover the rounding of the norms, sums and roots.  The
 // trace bounds are the tight ones when the spectrum has no isolated
-// extreme eigenvalue (about 1.2 rho against 1.6 rho at n = 4000).
-constexpr int kSpecNorms = 9;
+// extreme eigenvalue (ER(5000, 0.5) Laplacian, deflated: 1.12 rho at 2k = 12,
+// 1.05 rho at 2k = 16).
+constexpr int kSpecNorms = 11;
 static double spectral_bound(const double h[kSpecNorms]) {
-  static const double e[kSpecNorms] = {1.0, 2.0, 4.0, 6.0, 4.0, 6.0, 8.0, 10.0, 12.0};
+  static const double e[kSpecNorms] = {1.0, 2.0, 4.0, 6.0, 4.0, 6.0, 8.0, 10.0, 12.0, 14.0, 16.0};
   double rho = INFINITY;
   for (int k = 0; k < kSpecNorms; ++k)
     if (std::isfinite(h[k]) && h[k] >= 0.0) rho = std::min(rho, std::pow(h[k], 1.0 / e[k]));
@@ -110,6 +112,99 @@ static int scaling_for(double rho) {
   int s = std::max(0, (int)std::ceil(std::log2(rho / kTheta[4])));
   while (rho / std::ldexp(1.0, s) > kTheta[4]) ++s;
   return s;
+}
+
+// Polynomial of exp(i x) on [-theta, theta] from the Jacobi-Anger expansion
+//     cos x = J0(theta) + 2 sum_{k>=1} (-1)^k J2k(theta) T2k(x / theta),
+//     sin x = 2 sum_{k>=0} (-1)^k J2k+1(theta) T2k+1(x / theta),
+// truncated at the least even degree dc and odd degree ds whose tails,
+// bounded by |Jm(theta)| <= (theta/2)^m / m!, are each at most 2^-54: for a
+// real symmetric X with spectrum in [-theta, theta] the 2-norm error of the
+// polynomial at X is then at most 2^-53.  Returned in w = x^2 / theta^2 in
+// [0, 1]: cos x = sum_j cw[j] w^j (j <= dc/2) and
+// sin x = (x / theta) sum_j sw[j] w^j (j <= (ds-1)/2), from
+// T2k(x/theta) = Tk(2w - 1) and T2k+1(x/theta) = (x/theta) Wk(w) with W0 = 1,
+// W1 = 4w - 3, Wk+1 = 2 (2w - 1) Wk - Wk-1 (T(n+2) = 2 T2 Tn - T(n-2)).  The
+// Bessel values come from their power series and the basis change runs in
+// long double (80-bit on the host).
+constexpr int kChebMax = 40;  // coefficients per part (dc/2 + 1, (ds-1)/2 + 1)
+struct ChebSeries {
+  int dc = 0, ds = 1;
+  double cw[kChebMax], sw[kChebMax];
+};
+
+static long double bessel_j(int m, long double x) {
+  // sum_j (-1)^j (x/2)^(2j+m) / (j! (j+m)!); 60 terms reach far below the
+  // last bit for x <= 16
+  const long double h = x / 2;
+  long double term = 1.0L;
+  for (int i = 1; i <= m; ++i) term *= h / (long double)i;
+  long double sum = term;
+  for (int j = 1; j < 60; ++j) {
+    term *= -h * h / ((long double)j * (long double)(j + m));
+    sum += term;
+  }
+  return sum;
+}
+
+// 2 sum_{m >= m0, m - m0 even} (theta/2)^m / m!
+static double bessel_tail_bound(double theta, int m0) {
+  const long double h = theta / 2;
+  long double t = 1.0L;
+  for (int i = 1; i <= m0; ++i) t *= h / (long double)i;
+  long double s = 0.0L;
+  for (int m = m0; m < m0 + 80; m += 2) {
+    s += t;
+    t *= h * h / ((long double)(m + 1) * (long double)(m + 2));
+  }
+  return (double)(2.0L * s);
+}
+
+static ChebSeries cheb_series(double theta) {
+  ChebSeries c;
+  const int dmax = 2 * (kChebMax - 1);
+  while (c.dc < dmax && bessel_tail_bound(theta, c.dc + 2) > 0x1p-54) c.dc += 2;
+  while (c.ds < dmax - 1 && bessel_tail_bound(theta, c.ds + 2) > 0x1p-54) c.ds += 2;
+  const long double th = theta;
+  const int nc = c.dc / 2 + 1, ns = (c.ds - 1) / 2 + 1;
+  long double acc[kChebMax] = {}, prev[kChebMax + 1] = {}, cur[kChebMax + 1] = {}, next[kChebMax + 1] = {};
+  // cos: sum_k b_k Tk(2w - 1), b_0 = J0, b_k = 2 (-1)^k J2k
+  prev[0] = 1.0L;                 // T0
+  cur[0] = -1.0L, cur[1] = 2.0L;  // T1(2w - 1)
+  for (int k = 0; k < nc; ++k) {
+    const long double b = k == 0 ? bessel_j(0, th) : 2.0L * (k & 1 ? -1.0L : 1.0L) * bessel_j(2 * k, th);
+    const long double* t = k == 0 ? prev : cur;
+    for (int j = 0; j <= k; ++j) acc[j] += b * t[j];
+    if (k >= 1) {  // T(k+1) = 2 (2w - 1) Tk - T(k-1)
+      for (int j = 0; j <= k + 1; ++j)
+        next[j] = 2.0L * ((j > 0 ? 2.0L * cur[j - 1] : 0.0L) - cur[j]) - prev[j];
+      for (int j = 0; j <= k + 1; ++j) {
+        prev[j] = cur[j];
+        cur[j] = next[j];
+      }
+    }
+  }
+  for (int j = 0; j < nc; ++j) c.cw[j] = (double)acc[j];
+  // sin / (x / theta): sum_k b_k Wk(w), b_k = 2 (-1)^k J2k+1
+  for (int j = 0; j <= kChebMax; ++j) prev[j] = cur[j] = 0.0L;
+  for (int j = 0; j < kChebMax; ++j) acc[j] = 0.0L;
+  prev[0] = 1.0L;                 // W0
+  cur[0] = -3.0L, cur[1] = 4.0L;  // W1
+  for (int k = 0; k < ns; ++k) {
+    const long double b = 2.0L * (k & 1 ? -1.0L : 1.0L) * bessel_j(2 * k + 1, th);
+    const long double* t = k == 0 ? prev : cur;
+    for (int j = 0; j <= k; ++j) acc[j] += b * t[j];
+    if (k >= 1) {
+      for (int j = 0; j <= k + 1; ++j)
+        next[j] = 2.0L * ((j > 0 ? 2.0L * cur[j - 1] : 0.0L) - cur[j]) - prev[j];
+      for (int j = 0; j <= k + 1; ++j) {
+        prev[j] = cur[j];
+        cur[j] = next[j];
+      }
+    }
+  }
+  for (int j = 0; j < ns; ++j) c.sw[j] = (double)acc[j];
+  return c;
 }
 
 static inline int64_t round_up(int64_t x, int64_t r) { return (x + r - 1) / r * r; }
@@ -342,6 +437,7 @@ struct PipelineBase {
   int sq2_lead = 0;        // structured squarings: leading squarings in the two-product form
   int spectral = 1;        // structured inputs: s from a spectral-radius bound (0: the 1-norm rule)
   int deflate = 1;         // spectral scaling: deflate an exact eigenvector 1 (equal row sums)
+  int cheb = 1;            // spectral plans: exp(iX) from its Chebyshev polynomial (0: Pade + solve)
 };
 
 // ---------------------------------------------------------------------------
@@ -640,7 +736,10 @@ template <typename W> struct Pipeline : PipelineBase {
     // (ensure_stage_buffers): an FP32 pipeline runs those stages on its
     // FP64 companion by default, and structured inputs use only the real
     // planes.  A pipeline in an arena holds all of them from the start.
-    if (in_arena) ensure_stage_buffers(true);
+    if (in_arena) {
+      ensure_stage_buffers(true);
+      ensure_system();
+    }
     ipiv = dalloc<int>(npcap);
     gmax = (int)((npcap + PANEL_R - 1) / PANEL_R);
     cand = dalloc<PanelCand>((size_t)2 * gmax);
@@ -664,28 +763,33 @@ template <typename W> struct Pipeline : PipelineBase {
     }
   }
 
-  // The Pade and solve stages of this pipeline use the planes a, p2, p4,
-  // p6, t2, u, v and the FP64 system lr / li with the transposed block rows
-  // tr / ti; structured inputs (A = iG) only the real planes.  They are
-  // allocated (zeroed) here on first use; the squarings need only x0, x1
-  // and t1, which the constructor allocates.
-  bool stage_real = false, stage_cplx = false;
+  // The Pade (or Chebyshev) and solve stages of this pipeline use the planes
+  // a, p2, p4, p6, t2, u, v, and the solves the FP64 system lr / li with the
+  // transposed block rows tr / ti; structured inputs (A = iG) only the real
+  // planes.  They are allocated (zeroed) on first use; the squarings need
+  // only x0, x1 and t1, which the constructor allocates.
+  bool stage_real = false, stage_cplx = false, stage_system = false;
   void ensure_stage_buffers(bool cplx) {
     const size_t cnt = (size_t)npcap * (size_t)ld;
     if (!stage_real) {
       for (Plane2<W>* p : {&a, &p2, &p4, &p6, &t2, &u, &v}) p->r = dalloc<W>(cnt);
-      // lr and li are one allocation (li = lr + npcap*ldl): the structured
-      // solve uses the 4 npcap^2 doubles as one contiguous work area.
-      lr = dalloc<double>((size_t)2 * npcap * ldl);
-      li = lr + (size_t)npcap * ldl;
-      tr = dalloc<double>((size_t)2 * npcap * LU_OUTER);
-      ti = dalloc<double>((size_t)2 * npcap * LU_OUTER);
       stage_real = true;
     }
     if (cplx && !stage_cplx) {
       for (Plane2<W>* p : {&a, &p2, &p4, &p6, &t2, &u, &v}) p->i = dalloc<W>(cnt);
       stage_cplx = true;
     }
+  }
+
+  void ensure_system() {
+    if (stage_system) return;
+    // lr and li are one allocation (li = lr + npcap*ldl): the structured
+    // solve uses the 4 npcap^2 doubles as one contiguous work area.
+    lr = dalloc<double>((size_t)2 * npcap * ldl);
+    li = lr + (size_t)npcap * ldl;
+    tr = dalloc<double>((size_t)2 * npcap * LU_OUTER);
+    ti = dalloc<double>((size_t)2 * npcap * LU_OUTER);
+    stage_system = true;
   }
 
   ~Pipeline() override {
@@ -965,18 +1069,30 @@ template <typename W> struct Pipeline : PipelineBase {
 
   void lincomb(const Plane2<W>& out, bool cplx, std::initializer_list<std::pair<double, const Plane2<W>*>> terms,
                double dg, hipStream_t st) {
+    double c[4];
+    const Plane2<W>* pl[4];
+    int k = 0;
+    for (auto& t : terms) {
+      c[k] = t.first;
+      pl[k] = t.second;
+      ++k;
+    }
+    lincomb_n(out, cplx, k, c, pl, dg, st);
+  }
+
+  // out = sum_k c[k] in[k] + dg I over the n x n region (at most 4 terms).
+  void lincomb_n(const Plane2<W>& out, bool cplx, int nin, const double* c, const Plane2<W>* const* in, double dg,
+                 hipStream_t st) {
     Lincomb<W> L;
     std::memset(&L, 0, sizeof(L));
     L.out_r = out.r;
     L.out_i = cplx ? out.i : nullptr;
-    int k = 0;
-    for (auto& t : terms) {
-      L.c[k] = t.first;
-      L.in_r[k] = t.second->r;
-      L.in_i[k] = t.second->i;
-      ++k;
+    for (int k = 0; k < nin; ++k) {
+      L.c[k] = c[k];
+      L.in_r[k] = in[k]->r;
+      L.in_i[k] = in[k]->i;
     }
-    L.nin = k;
+    L.nin = nin;
     L.diag = dg;
     dim3 grid((n + 255) / 256, n);
     lincomb_kernel<W><<<grid, 256, 0, st>>>(L, n, ld);
@@ -989,21 +1105,33 @@ template <typename W> struct Pipeline : PipelineBase {
     FX_LAUNCH();
   }
 
-  // NT product with symmetric (half-tile) output, real planes.
+  // NT product with symmetric (half-tile) output, real planes:
+  // C = alpha A B^T + sum_k c[k] add[k] + dg I (at most 3 terms).
   void sym_real(const Plane2<W>& A, const Plane2<W>& B, const Plane2<W>& C, hipStream_t st,
                 std::initializer_list<std::pair<double, const Plane2<W>*>> add = {}, double dg = 0.0,
                 double alpha = 1.0) {
+    double c[3];
+    const Plane2<W>* pl[3];
+    int k = 0;
+    for (auto& t : add) {
+      c[k] = t.first;
+      pl[k] = t.second;
+      ++k;
+    }
+    sym_real_n(A, B, C, st, k, c, pl, dg, alpha);
+  }
+
+  void sym_real_n(const Plane2<W>& A, const Plane2<W>& B, const Plane2<W>& C, hipStream_t st, int nadd,
+                  const double* c, const Plane2<W>* const* add, double dg, double alpha) {
     GemmArgs<W> g = args(A, B, ld, ld, n, n, (int)np);
     set_out(g, C);
     g.epi.alpha = alpha;
-    int k = 0;
-    for (auto& t : add) {
-      g.epi.addc[k] = t.first;
-      g.epi.add_r[k] = t.second->r;
-      g.epi.add_i[k] = t.second->i;
-      ++k;
+    for (int k = 0; k < nadd; ++k) {
+      g.epi.addc[k] = c[k];
+      g.epi.add_r[k] = add[k]->r;
+      g.epi.add_i[k] = add[k]->i;
     }
-    g.epi.nadd = k;
+    g.epi.nadd = nadd;
     g.epi.diag = dg;
     if constexpr (std::is_same<W, double>::value) {
       if (oz_slices > 0 && (oz_scope & 1)) {
@@ -1088,6 +1216,69 @@ template <typename W> struct Pipeline : PipelineBase {
     pade_assemble(m, 1.0, st);
   }
 
+  // ---- Chebyshev stage (spectral plans, in place of Pade and solve) ----
+  // exp(i X) for X = B theta / R, B real symmetric in a.r with spectral
+  // radius at most R (the spectral bound), theta = R / 2^s, from the
+  // polynomial of cheb_series(theta):
+  //     cos X = sum_j cw[j] Y^j,  sin X = (B / R) sum_j sw[j] Y^j,  Y = C^2 / R^2,
+  // C^2k the powers of the unscaled B that spectral_front formed (C^2, C^4,
+  // C^6 in p2, p4, p6, C^8 in x1).  Each polynomial in Y by Horner's rule in
+  // Y^4 over blocks of four coefficients (Paterson-Stockmeyer): a block
+  // c0 I + c1 Y + c2 Y^2 + c3 Y^3 is a combination of the powers, and a step
+  // T <- Y^4 T + block one symmetric product (C^8 times T, scaled by R^-8)
+  // with the block in its epilogue; a top block c I makes the first step
+  // a combination too (c Y^4 + block).  The factors commute (all are
+  // polynomials in B), so every product is symmetric.  No solve.  Returns
+  // the planes holding cos X and sin X (real parts).
+  std::pair<Plane2<W>*, Plane2<W>*> cheb_structured(double theta, double R, hipStream_t st) {
+    ensure_stage_buffers(false);
+    const ChebSeries cs = cheb_series(theta);
+    const Plane2<W>* pw[4] = {nullptr, &p2, &p4, &p6};
+    const Plane2<W>& mul = x1;
+    const double rinv = R > 0.0 ? 1.0 / R : 0.0;
+    const double r2 = rinv * rinv;
+    const double r8 = r2 * r2 * r2 * r2;
+    auto series = [&](const double* a, int nc, Plane2<W>* cur, Plane2<W>* nxt) {
+      double c[4];
+      const Plane2<W>* pl[4];
+      // block i: a[4i] I + sum_{0 < j < 4, 4i + j < nc} a[4i + j] R^-2j C^2j
+      auto block = [&](int i) {
+        int q = 0;
+        double rj = 1.0;
+        for (int j = 1; j < 4 && 4 * i + j < nc; ++j) {
+          rj *= r2;
+          c[q] = a[4 * i + j] * rj;
+          pl[q++] = pw[j];
+        }
+        return q;
+      };
+      const int nb = (nc + 3) / 4;
+      int i = nb - 1;
+      if (nb > 1 && nc % 4 == 1) {
+        // top block a[nc - 1] I: Y^4 (a[nc - 1] I) + block(nb - 2) needs no product
+        --i;
+        int q = block(i);
+        c[q] = a[nc - 1] * r8;
+        pl[q++] = &mul;
+        lincomb_n(*cur, false, q, c, pl, a[4 * i], st);
+      } else {
+        const int q = block(i);
+        lincomb_n(*cur, false, q, c, pl, a[4 * i], st);
+      }
+      for (--i; i >= 0; --i) {
+        const int q = block(i);
+        sym_real_n(mul, *cur, *nxt, st, q, c, pl, a[4 * i], r8);
+        std::swap(cur, nxt);
+      }
+      return cur;
+    };
+    Plane2<W>* cx = series(cs.cw, cs.dc / 2 + 1, &t1, &t2);
+    Plane2<W>* sp = series(cs.sw, (cs.ds - 1) / 2 + 1, &u, &v);
+    Plane2<W>* sx = sp == &u ? &v : &u;
+    sym_real(a, *sp, *sx, st, {}, 0.0, rinv);
+    return {cx, sx};
+  }
+
   // ---- spectral scaling (structured inputs) ----
   // A = i G with G real symmetric is normal, so the backward error of the
   // Pade approximant at X = i B / 2^s is that of its eigenvalues: the
@@ -1101,24 +1292,26 @@ template <typename W> struct Pipeline : PipelineBase {
   // The bounds rho(B) <= ||B^k||_1^(1/k), k = 1, 2, 4, 6, and
   // rho(B) <= trace(B^2k)^(1/2k), 2k = 4, ..., 12, use the Pade powers,
   // computed here on the unscaled B and reused by pade_assemble with
-  // scaled coefficients.  For the Laplacian of a dense graph, whose
+  // scaled coefficients; with8 (Chebyshev plans) also forms C^8, which the
+  // Chebyshev stage needs, and adds the tighter 2k = 14, 16.  For the Laplacian of a dense graph, whose
   // eigenvalue 0 lies far from the rest of the spectrum, this removes 3-4 of
   // 13 squarings at N = 10^4.  Returns the smallest bound (FP64 planes only).
   template <typename Tin>
-  double spectral_front(const Tin* d_a, double mu_im, double alpha, hipStream_t st) {
+  double spectral_front(const Tin* d_a, double mu_im, double alpha, bool with8, hipStream_t st) {
     ensure_stage_buffers(false);
     if constexpr (std::is_same<W, double>::value) {
       constexpr int kRows = 64;
       const int nch = (n + kRows - 1) / kRows;
       if (!npart)
         npart = dalloc<double>(std::max((size_t)std::max(1, (ncap + kRows - 1) / kRows) * std::max(ncap, 1),
-                                        (size_t)5 * std::max(ncap, 1)));
+                                        (size_t)kTraceDots * std::max(ncap, 1)));
       if (!d_norms) d_norms = dalloc<double>(kSpecNorms);
       if (!ncmax) ncmax = dalloc<double>((size_t)(ncap + 255) / 256 + 1);
       dim3 grid((n + 255) / 256, n);
       prepare_planar<Tin, W><<<grid, 256, 0, st>>>(d_a, n, a.r, a.i, ld, 0.0, mu_im, 1.0, 1, alpha);
       FX_LAUNCH();
       pade_powers(13, st);
+      if (with8) sym_real(p4, p4, x1, st);  // C^8 in x1, for the Chebyshev stage and two more trace bounds
       const Plane2<W>* pw[4] = {&a, &p2, &p4, &p6};
       for (int k = 0; k < 4; ++k) {
         norm1_part<<<dim3((unsigned)((n + 255) / 256), (unsigned)nch), 256, 0, st>>>(pw[k]->r, ld, n, kRows, npart);
@@ -1129,9 +1322,9 @@ template <typename W> struct Pipeline : PipelineBase {
         norm1_final<<<1, 256, 0, st>>>(ncmax, ncb, d_norms + k);
         FX_LAUNCH();
       }
-      trace_dots_part<<<(unsigned)n, 256, 0, st>>>(p2.r, p4.r, p6.r, ld, n, npart);
+      trace_dots_part<<<(unsigned)n, 256, 0, st>>>(p2.r, p4.r, p6.r, with8 ? x1.r : nullptr, ld, n, npart);
       FX_LAUNCH();
-      trace_dots_final<<<1, 256, 0, st>>>(npart, n, d_norms + 4);
+      trace_dots_final<<<1, 256, 0, st>>>(npart, n, with8 ? 1 : 0, d_norms + 4);
       FX_LAUNCH();
       double h[kSpecNorms];
       FX_HIP(hipMemcpyAsync(h, d_norms, sizeof(h), hipMemcpyDeviceToHost, st));
@@ -1141,6 +1334,7 @@ template <typename W> struct Pipeline : PipelineBase {
       (void)d_a;
       (void)mu_im;
       (void)alpha;
+      (void)with8;
       (void)st;
       throw Error(FEXPM_EINVAL, "spectral scaling runs on FP64 planes");
     }
@@ -1191,6 +1385,7 @@ template <typename W> struct Pipeline : PipelineBase {
   // engine) the first q64 squarings run in FP64 on the free LU buffer first.
   int solve(bool structured, int q64, hipStream_t st) {
     ensure_stage_buffers(!structured);
+    ensure_system();
     FX_HIP(hipMemsetAsync(&scal->info, 0, sizeof(int), st));
     if (structured && spd_solve) return solve_spd(q64, st);
     const int inp = (int)np;
@@ -1710,6 +1905,7 @@ template <typename W> struct Pipeline : PipelineBase {
     double rho;     // spectral-radius bound of B (spectral plans)
     int m, s, s_do, q64;
     int s_onenorm;  // s of the 1-norm rule
+    bool cheb;      // Chebyshev polynomial in place of Pade and solve (spectral plans)
   };
 
   // spectral_ok: the caller can wait for the spectral bound on the host
@@ -1784,8 +1980,9 @@ template <typename W> struct Pipeline : PipelineBase {
     Pipeline<double>* h = fp64_pipe();
     h->oz_slices = oz_slices;
     h->oz_scope = slice_scope(P);
-    P.rho = h->template spectral_front<Tin>(d_a, P.mu_im, P.alpha, st);
+    P.rho = h->template spectral_front<Tin>(d_a, P.mu_im, P.alpha, cheb != 0, st);
     P.s = scaling_for(P.rho);
+    P.cheb = cheb != 0;
     finish_plan(P);
   }
 
@@ -1818,12 +2015,21 @@ template <typename W> struct Pipeline : PipelineBase {
       h->z_overlap = z_overlap;
       h->ch_outer = ch_outer;
       h->z_left = z_left;
-      if (P.spectral)
-        h->pade_assemble(P.m, scale, st);  // powers formed by spectral_plan
-      else
-        h->front_pade<Tin>(d_a, P.structured, P.m, P.mu_re, P.mu_im, scale, st);
-      if (timing) FX_HIP(hipEventRecord(ev[2], st));
-      const int dh = h->solve(P.structured, 0, st);
+      int dh;
+      if (P.cheb) {
+        // Chebyshev polynomial from the powers formed by spectral_plan; no solve
+        auto cs = h->cheb_structured(P.rho * scale, P.rho, st);
+        if (timing) FX_HIP(hipEventRecord(ev[2], st));
+        FX_HIP(hipMemsetAsync(&h->scal->info, 0, sizeof(int), st));
+        dh = h->solution_tail(cs.first->r, cs.second->r, h->ld, nullptr, nullptr, 0, 0, st);
+      } else {
+        if (P.spectral)
+          h->pade_assemble(P.m, scale, st);  // powers formed by spectral_plan
+        else
+          h->front_pade<Tin>(d_a, P.structured, P.m, P.mu_re, P.mu_im, scale, st);
+        if (timing) FX_HIP(hipEventRecord(ev[2], st));
+        dh = h->solve(P.structured, 0, st);
+      }
       auto r = h->square_split(P.q64, P.structured, dh, st);
       dim3 grid((n + 255) / 256, n);
       fp64_to_work<W><<<grid, 256, 0, st>>>(r.first->r, r.first->i, h->ld, x0.r, x0.i, ld, n);
@@ -1831,12 +2037,23 @@ template <typename W> struct Pipeline : PipelineBase {
       FX_HIP(hipMemcpyAsync(dsq[0], h->dsq[r.second], (size_t)n * sizeof(double2), hipMemcpyDeviceToDevice, st));
       flops += h->flops;
     } else {
-      if (P.spectral)
-        pade_assemble(P.m, scale, st);
-      else
-        front_pade<Tin>(d_a, P.structured, P.m, P.mu_re, P.mu_im, scale, st);
-      if (timing) FX_HIP(hipEventRecord(ev[2], st));
-      dc = solve(P.structured, P.q64, st);
+      bool done = false;
+      if constexpr (std::is_same<W, double>::value) {
+        if (P.cheb) {
+          auto cs = cheb_structured(P.rho * scale, P.rho, st);
+          if (timing) FX_HIP(hipEventRecord(ev[2], st));
+          dc = solution_tail(cs.first->r, cs.second->r, ld, nullptr, nullptr, 0, 0, st);
+          done = true;
+        }
+      }
+      if (!done) {
+        if (P.spectral)
+          pade_assemble(P.m, scale, st);
+        else
+          front_pade<Tin>(d_a, P.structured, P.m, P.mu_re, P.mu_im, scale, st);
+        if (timing) FX_HIP(hipEventRecord(ev[2], st));
+        dc = solve(P.structured, P.q64, st);
+      }
     }
     if (timing) FX_HIP(hipEventRecord(ev[3], st));
     Plane2<W>* X = square(P.s_do - P.q64, P.structured, dc, st, P.two);
@@ -1884,7 +2101,7 @@ template <typename W> struct Pipeline : PipelineBase {
       for (int T = 1; T * SQ_BM <= npcap; ++T) tile_order(T, nullptr);
       constexpr int kRows = 64;
       npart = dalloc<double>(std::max((size_t)std::max(1, (ncap + kRows - 1) / kRows) * std::max(ncap, 1),
-                                      (size_t)5 * std::max(ncap, 1)));
+                                      (size_t)kTraceDots * std::max(ncap, 1)));
       d_norms = dalloc<double>(kSpecNorms);
       ncmax = dalloc<double>((size_t)(ncap + 255) / 256 + 1);
     }
@@ -1912,6 +2129,12 @@ template <typename W> struct Pipeline : PipelineBase {
     S.s_onenorm = P.nonfinite ? 0 : P.s_onenorm;
     S.rho_bound = P.spectral ? P.rho : 0.0;
     S.s_two_product = P.nonfinite ? 0 : P.two;
+    if (P.cheb && !P.nonfinite) {
+      const ChebSeries cs = cheb_series(P.rho * std::ldexp(1.0, -P.s));
+      S.chebyshev = std::max(cs.dc, cs.ds);
+    } else {
+      S.chebyshev = 0;
+    }
   }
 
   static void check_info(int info) {
@@ -2940,6 +3163,9 @@ int fexpm_set_option(fexpm_engine* e, int key, int64_t value) {
     case FEXPM_OPT_DEFLATE:
       if (e->pipe) e->pipe->deflate = value ? 1 : 0;
       return FEXPM_OK;
+    case FEXPM_OPT_CHEBYSHEV:
+      if (e->pipe) e->pipe->cheb = value ? 1 : 0;
+      return FEXPM_OK;
     case FEXPM_OPT_FP64_SQUARINGS:
       if (value < 0 || value > 64) {
         e->err = "fp64 squarings must be in [0, 64]";
@@ -3929,6 +4155,19 @@ extern "C" int fexpm_test_grouped_tile_order(int T, int* out) {
     int ti = 0, tj = 0;
     grouped_tri_tile(b, T, ti, tj);
     out[b] = (ti << 16) | tj;
+  }
+  return FEXPM_OK;
+}
+
+// The Chebyshev polynomial of spectral plans, evaluated on the host.
+extern "C" int fexpm_test_cheb_series(double theta, int* dc, int* ds, double* cw, double* sw) {
+  if (!dc || !ds || !cw || !sw || !(theta >= 0.0) || !(theta <= 16.0)) return FEXPM_EINVAL;
+  const ChebSeries c = cheb_series(theta);
+  *dc = c.dc;
+  *ds = c.ds;
+  for (int j = 0; j < kChebMax; ++j) {
+    cw[j] = j <= c.dc / 2 ? c.cw[j] : 0.0;
+    sw[j] = j <= (c.ds - 1) / 2 ? c.sw[j] : 0.0;
   }
   return FEXPM_OK;
 }

@@ -650,16 +650,18 @@ __global__ __launch_bounds__(256) void norm1_final(const double* __restrict__ cm
 }
 
 // Traces of even powers of a real symmetric B from its powers B^2, B^4,
-// B^6 (FP64 planes, n x n region): trace(B^(2j+2k)) = sum_il (B^2j)_il
-// (B^2k)_il.  One workgroup per row writes part[row * 5 + q] for
-// q = 0..4: trace B^4, B^6, B^8, B^10, B^12; trace_dots_final sums the rows
-// in order (deterministic).
+// B^6 and, when b8 is given, B^8 (FP64 planes, n x n region):
+// trace(B^(2j+2k)) = sum_il (B^2j)_il (B^2k)_il.  One workgroup per row
+// writes part[row * kTraceDots + q] for q = 0..6: trace B^4, B^6, B^8, B^10,
+// B^12, B^14, B^16 (the last two 0 without b8); trace_dots_final sums the
+// rows in order (deterministic) and reports -1 for the last two without b8.
+constexpr int kTraceDots = 7;
 __global__ __launch_bounds__(256) void trace_dots_part(const double* __restrict__ b2, const double* __restrict__ b4,
-                                                       const double* __restrict__ b6, int64_t ld, int n,
-                                                       double* __restrict__ part) {
-  __shared__ double sm[5][256];
+                                                       const double* __restrict__ b6, const double* __restrict__ b8,
+                                                       int64_t ld, int n, double* __restrict__ part) {
+  __shared__ double sm[kTraceDots][256];
   const int i = blockIdx.x;
-  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  double acc[kTraceDots] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (int j = threadIdx.x; j < n; j += 256) {
     const int64_t o = (int64_t)i * ld + j;
     const double x2 = b2[o], x4 = b4[o], x6 = b6[o];
@@ -668,36 +670,41 @@ __global__ __launch_bounds__(256) void trace_dots_part(const double* __restrict_
     acc[2] += x4 * x4;
     acc[3] += x4 * x6;
     acc[4] += x6 * x6;
+    if (b8) {
+      const double x8 = b8[o];
+      acc[5] += x6 * x8;
+      acc[6] += x8 * x8;
+    }
   }
 #pragma unroll
-  for (int q = 0; q < 5; ++q) sm[q][threadIdx.x] = acc[q];
+  for (int q = 0; q < kTraceDots; ++q) sm[q][threadIdx.x] = acc[q];
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w)
 #pragma unroll
-      for (int q = 0; q < 5; ++q) sm[q][threadIdx.x] += sm[q][threadIdx.x + w];
+      for (int q = 0; q < kTraceDots; ++q) sm[q][threadIdx.x] += sm[q][threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x < 5) part[(int64_t)i * 5 + threadIdx.x] = sm[threadIdx.x][0];
+  if (threadIdx.x < kTraceDots) part[(int64_t)i * kTraceDots + threadIdx.x] = sm[threadIdx.x][0];
 }
 
-__global__ __launch_bounds__(256) void trace_dots_final(const double* __restrict__ part, int n,
+__global__ __launch_bounds__(256) void trace_dots_final(const double* __restrict__ part, int n, int with8,
                                                         double* __restrict__ out) {
-  __shared__ double sm[5][256];
-  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  __shared__ double sm[kTraceDots][256];
+  double acc[kTraceDots] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (int i = threadIdx.x; i < n; i += 256)
 #pragma unroll
-    for (int q = 0; q < 5; ++q) acc[q] += part[(int64_t)i * 5 + q];
+    for (int q = 0; q < kTraceDots; ++q) acc[q] += part[(int64_t)i * kTraceDots + q];
 #pragma unroll
-  for (int q = 0; q < 5; ++q) sm[q][threadIdx.x] = acc[q];
+  for (int q = 0; q < kTraceDots; ++q) sm[q][threadIdx.x] = acc[q];
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w)
 #pragma unroll
-      for (int q = 0; q < 5; ++q) sm[q][threadIdx.x] += sm[q][threadIdx.x + w];
+      for (int q = 0; q < kTraceDots; ++q) sm[q][threadIdx.x] += sm[q][threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x < 5) out[threadIdx.x] = sm[threadIdx.x][0];
+  if (threadIdx.x < kTraceDots) out[threadIdx.x] = (threadIdx.x >= 5 && !with8) ? -1.0 : sm[threadIdx.x][0];
 }
 
 // Output: out[i][j] = e^mu * (xr + i xi), interleaved complex or real.

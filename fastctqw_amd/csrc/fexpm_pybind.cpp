@@ -77,6 +77,7 @@ static py::dict stats_dict(const fexpm_stats& s) {
   d["s_onenorm"] = s.s_onenorm;
   d["rho_bound"] = s.rho_bound;
   d["s_two_product"] = s.s_two_product;
+  d["chebyshev"] = s.chebyshev;
   d["device_bytes"] = s.device_bytes;
   d["ms_square_kernel"] = s.ms_square_kernel;
   d["square_kernel_launches"] = s.square_kernel_launches;
@@ -528,6 +529,7 @@ PYBIND11_MODULE(_fastexpm_core, m) {
   m.attr("OPT_FAIL_ALLOC") = (int)FEXPM_OPT_FAIL_ALLOC;
   m.attr("OPT_DEFLATE") = (int)FEXPM_OPT_DEFLATE;
   m.attr("OPT_TWO_PRODUCT") = (int)FEXPM_OPT_TWO_PRODUCT;
+  m.attr("OPT_CHEBYSHEV") = (int)FEXPM_OPT_CHEBYSHEV;
   m.attr("PREC_NATIVE") = (int)FEXPM_PREC_NATIVE;
   m.attr("PREC_FP16X2") = (int)FEXPM_PREC_FP16X2;
   m.attr("PREC_BF16X3") = (int)FEXPM_PREC_BF16X3;

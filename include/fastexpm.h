@@ -100,10 +100,19 @@ enum fexpm_option {
                                     rounding errors grow in later squarings.  Default 0 (3M throughout):
                                     at N = 10^4 a two-product squaring takes 23.6 vs 25.4 ms (its
                                     two-MFMA loop is bound by the operand loads) */
-  FEXPM_OPT_DEFLATE = 13         /* with FEXPM_OPT_SPECTRAL: 1 = when every row sum of G is the same c
+  FEXPM_OPT_DEFLATE = 13,        /* with FEXPM_OPT_SPECTRAL: 1 = when every row sum of G is the same c
                                     (double-double sums), deflate the exact eigenvector 1 first:
                                     B = G - mu I + (mu - c)/n J and exp(A) = e^{i mu} (exp(iB) +
                                     (e^{i(c - mu)} - 1) J / n) (default); 0 = no deflation */
+  FEXPM_OPT_CHEBYSHEV = 15       /* spectral plans (s from the bound R of rho(B), theta = R / 2^s <=
+                                    theta_13): 1 = exp(iX), X = B / 2^s, from the Chebyshev (Jacobi-
+                                    Anger) expansion of e^{ix} on [-theta, theta], cos x = J0(theta) +
+                                    2 sum (-1)^k J2k(theta) T2k(x/theta), sin x = 2 sum (-1)^k
+                                    J2k+1(theta) T2k+1(x/theta), each truncated where the bound
+                                    2 sum (theta/2)^m/m! of its tail reaches 2^-54, evaluated in X^2
+                                    by symmetric products only (Paterson-Stockmeyer), in place of the
+                                    degree-13 Pade approximant and its solve (default); 0 = Pade and
+                                    solve.  fexpm_stats.chebyshev reports the degree. */
 };
 
 /* Per-run report (filled by fexpm_get_stats after a run). */
@@ -135,7 +144,8 @@ typedef struct fexpm_stats {
   int32_t spectral;     /* 1: s came from the spectral-radius bound (FEXPM_OPT_SPECTRAL) */
   int32_t deflated;     /* 1: the eigenvector 1 was deflated (FEXPM_OPT_DEFLATE) */
   int32_t s_onenorm;    /* s of the 1-norm rule (equal to s when spectral = 0) */
-  int32_t reserved0;
+  int32_t chebyshev;    /* degree of the Chebyshev polynomial that replaced the Pade approximant
+                           and its solve (FEXPM_OPT_CHEBYSHEV); 0 = Pade (m is then its degree) */
   double rho_bound;     /* spectral-radius bound of the shifted (deflated) B, when spectral = 1 */
   int32_t s_two_product;  /* leading squarings done in the two-product form (FEXPM_OPT_TWO_PRODUCT) */
   int32_t reserved1;

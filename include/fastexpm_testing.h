@@ -125,6 +125,14 @@ int fexpm_test_ozaki_gemm(int slices, int sym, int m, int n, int k, const double
  * figure an issued-rate report divides by the launch time. */
 int fexpm_test_square_issued(int64_t n, double* flops);
 
+/* Polynomial of exp(ix) on [-theta, theta] that spectral plans use in place
+ * of the Pade approximant (FEXPM_OPT_CHEBYSHEV), evaluated on the host,
+ * 0 <= theta <= 16: the degrees dc (even, cos) and ds (odd, sin) and the
+ * coefficients in w = x^2 / theta^2, cos x = sum_j cw[j] w^j (j <= dc/2),
+ * sin x = (x / theta) sum_j sw[j] w^j (j <= (ds-1)/2); cw and sw hold 40
+ * values each (zeros beyond the degree). */
+int fexpm_test_cheb_series(double theta, int* dc, int* ds, double* cw, double* sw);
+
 /* Tile order of the symmetric (lower-triangular) GEMM launches, evaluated on
  * the host: out[b] = (ti << 16) | tj for every b < T(T+1)/2, 1 <= T <= 4096.
  * Groups of 8 x 8 tiles, group rows in order, within one the groups left to

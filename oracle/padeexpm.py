@@ -22,7 +22,10 @@ d_u1, d_u2, d_v1, d_v2, d_mu, d_nrmA`` (include/MatrixExpCalculator.h:52-61)
   The bounds are rho(B) <= ||B^k||_1^(1/k) (Gelfand; k = 1, 2, 4, 6, the
   powers the Pade-13 evaluation needs anyway; cf. the d_k = ||A^k||^(1/k)
   of Al-Mohy & Higham, SIMAX 31(3), 2009) and, B being symmetric,
-  rho(B)^2k <= trace(B^2k) = sum of lambda^2k for 2k = 4, ..., 12.
+  rho(B)^2k <= trace(B^2k) = sum of lambda^2k for 2k = 4, ..., 12, and
+  2k = 14, 16 from B^8 for plans that take the Chebyshev route
+  (``oracle.chebcs``, which needs B^8 anyway; ``cheb=True``, the engine's
+  default).
   When every row sum of G equals the same c, the eigenvector 1 is
   deflated first: B = G - mu I + alpha J
   with alpha = (mu - c)/n has B 1 = 0 and commutes with J = 1 1^T, and
@@ -108,17 +111,21 @@ def scaling_for(rho: float) -> int:
     return s
 
 
-def spectral_bound(b: np.ndarray) -> float:
+def spectral_bound(b: np.ndarray, with8: bool = False) -> float:
     """Upper bound of the spectral radius of a real symmetric b: the least of
     ||b^k||_1^(1/k), k in (1, 2, 4, 6) (Gelfand), and trace(b^2k)^(1/2k),
-    2k in (4, 6, 8, 10, 12) (a sum of the 2k-th powers of the eigenvalues,
-    formed as elementwise products of b^2, b^4, b^6), widened by 1e-9 for
-    rounding."""
+    2k in (4, 6, 8, 10, 12), and (14, 16) with ``with8`` (a sum of the 2k-th
+    powers of the eigenvalues, formed as elementwise products of b^2, b^4,
+    b^6, b^8), widened by 1e-9 for rounding."""
     b2 = b @ b
     b4 = b2 @ b2
     b6 = b4 @ b2
     vals = [onenorm(b), onenorm(b2) ** 0.5, onenorm(b4) ** 0.25, onenorm(b6) ** (1.0 / 6.0)]
-    for (x, y), k in (((b2, b2), 4), ((b2, b4), 6), ((b4, b4), 8), ((b4, b6), 10), ((b6, b6), 12)):
+    pairs = [((b2, b2), 4), ((b2, b4), 6), ((b4, b4), 8), ((b4, b6), 10), ((b6, b6), 12)]
+    if with8:
+        b8 = b4 @ b4
+        pairs += [((b6, b8), 14), ((b8, b8), 16)]
+    for (x, y), k in pairs:
         tr = float(np.sum(x * y))
         if tr >= 0.0:
             vals.append(tr ** (1.0 / k))
@@ -170,10 +177,11 @@ def pade_uv(a: np.ndarray, m: int):
     return a @ odd, even
 
 
-def pade_plan(a: np.ndarray, spectral: bool = False, deflate: bool = True) -> dict:
+def pade_plan(a: np.ndarray, spectral: bool = False, deflate: bool = True, cheb: bool = True) -> dict:
     """The decisions of ``pade_expm`` without the exponential: mu, the
     1-norm, m, s (and, for spectral plans, s_onenorm, rho, deflated, c,
-    alpha and the unscaled B)."""
+    alpha and the unscaled B).  ``cheb``: the spectral bound includes the
+    traces of B^14 and B^16 (the engine's Chebyshev plans, its default)."""
     a = np.asarray(a)
     n = a.shape[0]
     mu, ash = trace_shift(a)
@@ -186,19 +194,22 @@ def pade_plan(a: np.ndarray, spectral: bool = False, deflate: bool = True) -> di
         equal, c = equal_row_sums(g) if deflate else (False, 0.0)
         alpha = (mu.imag - c) / n if equal else 0.0
         b = (g - mu.imag * np.eye(n)) + alpha
-        rho = spectral_bound(b)
+        rho = spectral_bound(b, with8=cheb)
         plan.update({"s": scaling_for(rho), "spectral": True, "deflated": bool(equal), "rho": rho,
                      "c": c if equal else 0.0, "alpha": alpha, "B": b})
     return plan
 
 
-def pade_expm(a: np.ndarray, stages: bool = False, spectral: bool = False, deflate: bool = True):
+def pade_expm(a: np.ndarray, stages: bool = False, spectral: bool = False, deflate: bool = True,
+              cheb: bool = True):
     """exp(a) in FP64 by trace shift + Higham 2005 scaling and squaring.
 
     With ``spectral=True`` (the engine's default for single runs), inputs of
     the form a = iG (G real symmetric) that need degree 13 with s > 0 take
     s from the spectral-radius bound, after deflating the eigenvector 1
-    when all row sums of G are equal (``deflate``).
+    when all row sums of G are equal (``deflate``); ``cheb`` selects the
+    bound of the engine's Chebyshev plans (see ``pade_plan``), while the
+    value stays the Pade approximant's.
 
     With ``stages=True`` returns a dict with mu, norm, m, s, U, V, X (the
     Pade value before squaring) and E (the result); spectral runs add
@@ -211,7 +222,7 @@ def pade_expm(a: np.ndarray, stages: bool = False, spectral: bool = False, defla
     if n == 0:
         out = np.zeros((0, 0), dtype=np.complex128)
         return {"E": out, "m": 0, "s": 0, "mu": 0j, "norm": 0.0} if stages else out
-    plan = pade_plan(a, spectral, deflate)
+    plan = pade_plan(a, spectral, deflate, cheb)
     mu, norm, m, s = plan["mu"], plan["norm"], plan["m"], plan["s"]
     info = {k: plan[k] for k in ("spectral", "deflated", "s_onenorm", "rho", "c", "alpha", "B") if k in plan}
     if plan["spectral"]:

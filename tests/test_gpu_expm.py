@@ -120,7 +120,13 @@ def test_spectral_scaling(fx, dt, case):
     s, deflation and bound against the oracle's same rule; the Pade value
     (no squarings: output e^mu (X + delta J)) against the FP64 stage
     restatement; U against the eigen oracle; the 1-norm rule (option off)
-    against the same oracle with its own, larger s."""
+    against the same oracle with its own, larger s.  The approximant before
+    the squarings is the Chebyshev polynomial of degree
+    max(cheb_degrees(rho / 2^s)) (option OPT_CHEBYSHEV, default); with the
+    option off the Pade approximant and its solve give U to the same
+    tolerance."""
+    from oracle.chebcs import cheb_degrees
+
     core = fx._fastexpm_core
     a, defl = _spectral_case(case)
     n = a.shape[0]
@@ -138,10 +144,18 @@ def test_spectral_scaling(fx, dt, case):
         assert stats["m"] == 13 and _same_s(stats, st), (stats, st["s"], st["rho"])
         assert stats["s_onenorm"] == st["s_onenorm"] and stats["s"] <= stats["s_onenorm"]
         assert abs(stats["rho_bound"] - st["rho"]) <= 1e-9 * st["rho"], (stats["rho_bound"], st["rho"])
+        assert stats["chebyshev"] == max(cheb_degrees(stats["rho_bound"] * 2.0 ** -stats["s"])), stats
         err = rel_fro(ref, u)
         print(f"  s={stats['s']} (1-norm rule {stats['s_onenorm']}) rho={stats['rho_bound']:.2f} "
-              f"relF={err:.3e} tol={tol:.1e}", flush=True)
+              f"Chebyshev degree {stats['chebyshev']} relF={err:.3e} tol={tol:.1e}", flush=True)
         assert err <= tol
+        e._backend.set_option(core.OPT_CHEBYSHEV, 0)
+        up = e.run(a)
+        assert e.stats()["chebyshev"] == 0
+        errp = rel_fro(ref, up)
+        print(f"  Pade and solve relF={errp:.3e}", flush=True)
+        assert errp <= tol
+        e._backend.set_option(core.OPT_CHEBYSHEV, 1)
         # the Pade value before the squarings
         e._backend.set_option(core.OPT_MAX_SQUARINGS, 0)
         x0 = e.run(a)
@@ -158,6 +172,55 @@ def test_spectral_scaling(fx, dt, case):
         err1 = rel_fro(ref, u1)
         print(f"  1-norm rule s={s1['s']} relF={err1:.3e}", flush=True)
         assert err1 <= _tol(dt, s1["s"])
+
+
+@pytest.mark.parametrize("dt,n", [(np.complex64, 300), (np.complex64, 1100), (np.complex128, 300)])
+def test_chebyshev_route(fx, dt, n):
+    """Spectral plans with the Chebyshev polynomial in place of Pade and
+    solve over ER(n, 0.5) Laplacians (s > 0) and complete-graph Laplacians
+    (rho(B) = t: s = 0 and small theta): the degree against cheb_degrees,
+    the value before the squarings against the oracle's Horner evaluation
+    (to the products' rounding: FP64-accurate residue products for
+    complex64) and U against the eigendecomposition oracle and against the
+    Pade route."""
+    from oracle.chebcs import cheb_cs, cheb_degrees
+
+    core = fx._fastexpm_core
+    graphs = {"er": laplacian(erdos_renyi(n, 0.5, seed=21 + n)), "complete": laplacian(complete_graph(n))}
+    seen_s0 = False
+    with _engine(fx, dt, n) as e:
+        for g, t in (("er", 0.05), ("er", 0.15), ("er", 4.5), ("complete", 0.2), ("complete", 0.3),
+                     ("complete", 1.0)):
+            _say(f"Chebyshev route {np.dtype(dt).name} n={n} {g} t={t}")
+            a = np.ascontiguousarray((-1j * t * graphs[g]).astype(dt))
+            a128 = a.astype(np.complex128)
+            st = pade_expm(a128, stages=True, spectral=True)
+            if not st["spectral"]:
+                continue
+            ref = eig_propagator(-a128.imag, 1.0)
+            u = e.run(a)
+            stats = e.stats()
+            theta = stats["rho_bound"] * 2.0 ** -stats["s"]
+            assert stats["spectral"] == 1 and stats["chebyshev"] == max(cheb_degrees(theta)), stats
+            assert _same_s(stats, st), (stats["s"], st["s"])
+            seen_s0 |= stats["s"] == 0
+            err = rel_fro(ref, u)
+            e._backend.set_option(core.OPT_MAX_SQUARINGS, 0)
+            x0 = e.run(a)
+            e._backend.set_option(core.OPT_MAX_SQUARINGS, -1)
+            c, sn, _ = cheb_cs(st["B"], st["rho"], st["s"])
+            xr = c + 1j * sn + ((np.exp(1j * (st["c"] - st["mu"].imag)) - 1.0) / n if st["deflated"] else 0.0)
+            errx = rel_fro(np.exp(st["mu"]) * xr, x0)
+            e._backend.set_option(core.OPT_CHEBYSHEV, 0)
+            up = e.run(a)
+            e._backend.set_option(core.OPT_CHEBYSHEV, 1)
+            diff = rel_fro(up, u)
+            print(f"  s={stats['s']} theta={theta:.3f} degree={stats['chebyshev']} relF={err:.3e} "
+                  f"value relF={errx:.3e} vs Pade {diff:.3e}", flush=True)
+            tol = _tol(dt, stats["s"])
+            assert err <= tol and diff <= tol
+            assert errx <= (1e-6 if dt == np.complex64 else 1e-12)
+    assert seen_s0
 
 
 @pytest.mark.parametrize("weighted", [False, True])
@@ -304,7 +367,7 @@ def test_two_product_squarings(fx, dt, case):
 def test_structured_cholesky_vs_lu(fx, dt, n):
     """The structured solve (real Cholesky of S = V^2 + Uq^2, X = Z Z^T)
     against the pivoted complex LU of the same Pade system (option
-    OPT_SPD_SOLVE = 0) and the eigen oracle."""
+    OPT_SPD_SOLVE = 0) and the eigen oracle (Pade route: OPT_CHEBYSHEV = 0)."""
     L = laplacian(erdos_renyi(n, 0.35, seed=300 + n, weighted=True))
     t = 6.0
     a = np.ascontiguousarray((-1j * t * L).astype(dt))
@@ -313,6 +376,7 @@ def test_structured_cholesky_vs_lu(fx, dt, n):
     for spd in (1, 0):
         _say(f"structured solve spd={spd} n={n} dtype={np.dtype(dt).name}")
         with _engine(fx, dt, n) as e:
+            e._backend.set_option(fx._fastexpm_core.OPT_CHEBYSHEV, 0)
             e._backend.set_option(fx._fastexpm_core.OPT_SPD_SOLVE, spd)
             out[spd] = e.run(a)
             st = e.stats()
@@ -413,7 +477,8 @@ def test_fp64_slice_products_ctqw(fx, slices, scope, n):
     residue products modulo 13 moduli (the default) or int8 slices (the Pade
     products by default, scope 3 also S and X = Z Z^T of the solve): U
     against the eigen oracle within the complex64 tolerance, and close to
-    the U of the FP64-MFMA products."""
+    the U of the FP64-MFMA products.  Pade route (OPT_CHEBYSHEV = 0), so that
+    scope 3 reaches the solve."""
     from fastctqw_amd.fastexpm import _fastexpm_core as core
 
     _say(f"fp64 slices {slices} scope {scope} n={n}")
@@ -422,6 +487,7 @@ def test_fp64_slice_products_ctqw(fx, slices, scope, n):
     lam, V = np.linalg.eigh(np.ascontiguousarray(a.imag, dtype=np.float64))
     ref = (V * np.exp(1j * lam)[None, :]) @ V.T
     with fx.ExpMatComplex64(n) as e:
+        e._backend.set_option(core.OPT_CHEBYSHEV, 0)
         e.run(a)
         assert e.stats()["fp64_slices"] == 13 and e.stats()["fp64_slices_scope"] == 1  # the default
         e._backend.set_option(core.OPT_FP64_SLICES, 0)
@@ -477,13 +543,14 @@ def test_squaring_tail_split(fx, n):
 @pytest.mark.parametrize("dt,n", [(np.complex64, 700), (np.complex128, 330)])
 def test_cholesky_z_overlap(fx, dt, n):
     """Z updates of the Cholesky solve on a second stream give the same U bit
-    for bit as one stream."""
+    for bit as one stream (Pade route: OPT_CHEBYSHEV = 0)."""
     from fastctqw_amd.fastexpm import _fastexpm_core as core
 
     _say(f"z overlap {np.dtype(dt).name} n={n}")
     L = laplacian(erdos_renyi(n, 0.3, seed=70 + n, weighted=True))
     a = (-1j * 6.0 * L).astype(dt)
     with _engine(fx, dt, n) as e:
+        e._backend.set_option(core.OPT_CHEBYSHEV, 0)
         u_two = e.run(a)
         e._backend.set_option(core.OPT_Z_OVERLAP, 0)
         u_one = e.run(a)
@@ -497,7 +564,8 @@ def test_cholesky_outer_width(fx, dt, n):
     """Outer block widths of the Cholesky solve (64 .. 1024, wider than the
     order included) change only the FP64 summation order: every width agrees
     with the default to FP64 rounding carried through the squarings, and
-    with the eigendecomposition oracle; invalid widths are rejected."""
+    with the eigendecomposition oracle; invalid widths are rejected (Pade
+    route: OPT_CHEBYSHEV = 0)."""
     from fastctqw_amd.fastexpm import _fastexpm_core as core
 
     _say(f"cholesky outer width {np.dtype(dt).name} n={n}")
@@ -506,6 +574,7 @@ def test_cholesky_outer_width(fx, dt, n):
     lam, V = np.linalg.eigh(np.ascontiguousarray(a.imag, dtype=np.float64))
     psi = (V * np.exp(1j * lam)[None, :]) @ (V.T[:, :1])
     with _engine(fx, dt, n) as e:
+        e._backend.set_option(core.OPT_CHEBYSHEV, 0)
         u_def = e.run(a)
         s = e.stats()["s"]
         for w in (64, 256, 1024, 512):
@@ -524,7 +593,8 @@ def test_cholesky_left_looking_z(fx, dt, n):
     """Left-looking Z updates (one long-K product per outer block): the same
     values with and without the second stream, agreement with the
     right-looking form to FP64 rounding carried through the squarings, and
-    with the eigendecomposition oracle; small outer blocks give several."""
+    with the eigendecomposition oracle; small outer blocks give several
+    (Pade route: OPT_CHEBYSHEV = 0)."""
     from fastctqw_amd.fastexpm import _fastexpm_core as core
 
     _say(f"left-looking Z {np.dtype(dt).name} n={n}")
@@ -533,6 +603,7 @@ def test_cholesky_left_looking_z(fx, dt, n):
     lam, V = np.linalg.eigh(np.ascontiguousarray(a.imag, dtype=np.float64))
     psi = (V * np.exp(1j * lam)[None, :]) @ (V.T[:, :1])
     with _engine(fx, dt, n) as e:
+        e._backend.set_option(core.OPT_CHEBYSHEV, 0)
         e._backend.set_option(core.OPT_CHOL_OUTER, 128)
         u_right = e.run(a)
         s = e.stats()["s"]

@@ -327,3 +327,31 @@ def test_adjacency_list_and_er_cpu_routes():
     er = Operator.from_erdos_renyi(30, 0.4, 9, weighted=True, laplacian=True)
     dense = Operator.from_numpy_array(erdos_renyi(30, 0.4, 9, weighted=True).astype(np.float32), laplacian=True)
     assert np.array_equal(er._hamiltonian, dense._hamiltonian)
+
+
+@pytest.mark.parametrize("theta", [0.0, 1e-3, 0.3, 1.0, 2.7, 4.08, 5.371920351148152, 12.0])
+def test_cheb_series_host(theta):
+    """The engine's host routine for the Chebyshev polynomial of spectral
+    plans (Bessel power series and basis change in long double) against the
+    oracle's 50-digit coefficients: the same degrees, coefficients to a few
+    units of the float rounding of the largest term, and an out-of-range
+    theta rejected."""
+    pytest.importorskip("mpmath")
+    from oracle.chebcs import cheb_coeffs, cheb_degrees
+
+    if not os.path.exists(LIB):
+        pytest.skip("HIP library not built")
+    lib = ctypes.CDLL(LIB)
+    lib.fexpm_test_cheb_series.argtypes = [ctypes.c_double, ctypes.POINTER(ctypes.c_int),
+                                           ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),
+                                           ctypes.POINTER(ctypes.c_double)]
+    dc, ds = ctypes.c_int(), ctypes.c_int()
+    cw, sw = (ctypes.c_double * 40)(), (ctypes.c_double * 40)()
+    assert lib.fexpm_test_cheb_series(theta, ctypes.byref(dc), ctypes.byref(ds), cw, sw) == 0
+    assert (dc.value, ds.value) == cheb_degrees(theta)
+    rc, rs = cheb_coeffs(theta)
+    scale = max(1.0, max(abs(v) for v in rc + rs))
+    assert max(abs(cw[j] - v) for j, v in enumerate(rc)) <= 8 * 2.0 ** -53 * scale
+    assert max(abs(sw[j] - v) for j, v in enumerate(rs)) <= 8 * 2.0 ** -53 * scale
+    assert all(cw[j] == 0.0 for j in range(len(rc), 40)) and all(sw[j] == 0.0 for j in range(len(rs), 40))
+    assert lib.fexpm_test_cheb_series(17.0, ctypes.byref(dc), ctypes.byref(ds), cw, sw) != 0

@@ -212,12 +212,16 @@ def _spectral_inputs():
 @pytest.mark.parametrize("name,a,defl", list(_spectral_inputs()))
 def test_spectral_bound_is_an_upper_bound(name, a, defl):
     """rho(B) <= ||B^k||_1^(1/k) for every k (Gelfand): the bound is at
-    least the largest |eigenvalue| (LAPACK eigvalsh) and at most ||B||_1."""
+    least the largest |eigenvalue| (LAPACK eigvalsh) and at most ||B||_1;
+    the traces of B^14 and B^16 (the default Chebyshev plans) only tighten
+    the bound of the Pade plans."""
     st = P.pade_expm(a, stages=True, spectral=True)
     assert st["spectral"] and st["deflated"] == defl, name
     b = st["B"]
     rho = np.abs(np.linalg.eigvalsh(b)).max()
     assert rho <= st["rho"] <= P.onenorm(b) * (1 + 1e-9) + 1e-300, (name, rho, st["rho"])
+    assert st["rho"] <= P.spectral_bound(b, with8=False), name
+    assert P.pade_plan(a, spectral=True, cheb=False)["rho"] == P.spectral_bound(b, with8=False)
     assert st["s"] == P.scaling_for(st["rho"]) and st["s"] <= st["s_onenorm"]
     if defl:
         # the deflated matrix annihilates 1 exactly in exact arithmetic
@@ -281,3 +285,153 @@ def test_equal_row_sums_is_exact():
     g[2, 3] = np.nextafter(1.0, 2.0)
     g[3, 2] = g[2, 3]
     assert not P.equal_row_sums(g)[0]
+
+
+# ---- Chebyshev route of spectral plans (oracle.chebcs) ----
+
+THETAS = [0.0, 1e-3, 0.3, 1.0, 2.7, 4.08, P.THETA[13]]
+
+
+@pytest.mark.parametrize("theta", THETAS)
+def test_cheb_degrees_exact(theta):
+    """dc (even) and ds (odd) are the least degrees whose tail bounds
+    2 sum (theta/2)^m / m! are at most 2^-54, in exact rational arithmetic
+    (theta as its double), summed far enough that the rest is negligible."""
+    from fractions import Fraction
+    from oracle.chebcs import cheb_degrees
+
+    dc, ds = cheb_degrees(theta)
+    h = Fraction(theta) / 2
+    u = Fraction(1, 2 ** 54)
+
+    def tail(m0):
+        return 2 * sum(h ** m / math.factorial(m) for m in range(m0, m0 + 120, 2))
+
+    assert dc % 2 == 0 and ds % 2 == 1
+    assert tail(dc + 2) <= u and tail(ds + 2) <= u
+    if dc >= 2:
+        assert tail(dc) > u
+    if ds >= 3:
+        assert tail(ds) > u
+
+
+@pytest.mark.parametrize("theta", THETAS[1:])
+def test_cheb_polynomial_error(theta):
+    """The polynomial against e^{ix} on 401 points of [-theta, theta] in
+    50-digit arithmetic: at most 2^-53 (truncation) plus a few units of
+    rounding of the float coefficients, and two degrees lower (both parts)
+    above 2^-53 / 16, so no term is superfluous and a dropped one shows."""
+    mpmath = pytest.importorskip("mpmath")
+    from oracle.chebcs import cheb_coeffs, cheb_degrees
+
+    dc, ds = cheb_degrees(theta)
+
+    def worst(cw, sw):
+        with mpmath.workdps(50):
+            err = mpmath.mpf(0)
+            for x in np.linspace(-theta, theta, 401):
+                x = mpmath.mpf(x)
+                w = (x / theta) ** 2
+                c = sum(mpmath.mpf(v) * w ** j for j, v in enumerate(cw))
+                sn = x / theta * sum(mpmath.mpf(v) * w ** j for j, v in enumerate(sw))
+                err = max(err, abs(mpmath.mpc(c, sn) - mpmath.expj(x)))
+            return float(err)
+
+    cw, sw = cheb_coeffs(theta)
+    growth = sum(abs(v) for v in cw) + sum(abs(v) for v in sw)
+    assert worst(cw, sw) <= 2.0 ** -53 + 4 * growth * 2.0 ** -53
+    if dc >= 2 and ds >= 3:
+        lo = worst(*cheb_coeffs(theta, dc - 2, ds - 2))
+        assert lo > 2.0 ** -53 / 16, lo
+
+
+@pytest.mark.parametrize("theta", THETAS)
+def test_cheb_coefficients_independent_route(theta):
+    """The 50-digit coefficients against float64 Bessel values
+    (scipy.special.jv) and numpy's Chebyshev-to-power conversion of the same
+    expansion in w (shifted basis Tk(2w - 1))."""
+    from numpy.polynomial import chebyshev as C
+    from numpy.polynomial import polynomial as Pn
+    from scipy.special import jv
+
+    from oracle.chebcs import cheb_coeffs, cheb_degrees
+
+    dc, ds = cheb_degrees(theta)
+    cw, sw = cheb_coeffs(theta)
+    b = [jv(0, theta)] + [2 * (-1) ** k * jv(2 * k, theta) for k in range(1, dc // 2 + 1)]
+    # Tk(2w - 1) in powers of w: compose cheb2poly(e_k) with 2w - 1
+    ref_c = np.zeros(dc // 2 + 1)
+    for k, bk in enumerate(b):
+        pk = C.cheb2poly([0] * k + [1])
+        comp = np.zeros(1)
+        for j, cj in enumerate(pk):
+            comp = Pn.polyadd(comp, cj * Pn.polypow([-1.0, 2.0], j))
+        ref_c[:len(comp)] += bk * comp
+    scale = max(1.0, sum(abs(x) for x in b) * 3.0 ** (dc // 2))
+    assert np.max(np.abs(np.array(cw) - ref_c)) <= 64 * 2.0 ** -53 * scale
+    # sin: Wk(w) = T2k+1(z)/z at z^2 = w
+    bs = [2 * (-1) ** k * jv(2 * k + 1, theta) for k in range((ds - 1) // 2 + 1)]
+    ref_s = np.zeros((ds - 1) // 2 + 1)
+    for k, bk in enumerate(bs):
+        pk = C.cheb2poly([0] * (2 * k + 1) + [1])  # odd powers of z
+        ref_s[:k + 1] += bk * pk[1::2]
+    scale = max(1.0, sum(abs(x) for x in bs) * 3.0 ** (ds // 2))
+    assert np.max(np.abs(np.array(sw) - ref_s)) <= 64 * 2.0 ** -53 * scale
+
+
+def _sym(n, seed, rho):
+    rng = np.random.default_rng(seed)
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    lam = rng.uniform(-rho, rho, n)
+    lam[0] = rho
+    return (q * lam) @ q.T, q, lam
+
+
+@pytest.mark.parametrize("theta", [0.05, 1.0, 4.08, P.THETA[13]])
+def test_cheb_cs_against_eigendecomposition(theta):
+    """cos X + i sin X from the blocked Horner evaluation against
+    V diag(e^{i lambda}) V^T (LAPACK eigh of the same X), with B unscaled
+    (spectral radius R = 900) and X = B / 2^s as the engine uses them; the
+    product count against cheb_products."""
+    from oracle.chebcs import cheb_cs, cheb_degrees, cheb_products
+
+    n = 120
+    s = max(0, math.ceil(math.log2(900.0 / theta)))
+    r = theta * 2.0 ** s
+    b, _, _ = _sym(n, 3, r)
+    c, sn, prods = cheb_cs(b, r, s)
+    assert prods == cheb_products(*cheb_degrees(theta))
+    w, v = np.linalg.eigh(b * 2.0 ** -s)
+    ref = (v * np.exp(1j * w)) @ v.T
+    err = np.linalg.norm(c + 1j * sn - ref, 2)
+    assert err <= 1e-13, err
+    assert np.allclose(c, c.T, atol=1e-15) and np.allclose(sn, sn.T, atol=1e-15)
+
+
+def test_cheb_cs_closed_forms_and_counts():
+    """Diagonal X: cos and sin of its entries; B = 0 (R = 0): (I, 0); the
+    Horner product counts (a top block of one coefficient is free)."""
+    from oracle.chebcs import cheb_cs, horner_products
+
+    d = np.linspace(-5.3, 5.3, 11)
+    c, s, _ = cheb_cs(np.diag(d * 4.0), 5.3 * 4.0, 2)
+    assert np.max(np.abs(np.diag(c) - np.cos(d))) <= 1e-14
+    assert np.max(np.abs(np.diag(s) - np.sin(d))) <= 1e-14
+    assert np.count_nonzero(c - np.diag(np.diag(c))) == 0 and np.count_nonzero(s - np.diag(np.diag(s))) == 0
+    c0, s0, _ = cheb_cs(np.zeros((5, 5)), 0.0, 0)
+    assert np.array_equal(c0, np.eye(5)) and np.array_equal(s0, np.zeros((5, 5)))
+    assert [horner_products(k) for k in (1, 2, 4, 5, 8, 9, 12, 13, 16, 17)] == [0, 0, 0, 0, 1, 1, 2, 2, 3, 3]
+
+
+def test_cheb_route_matches_pade_oracle():
+    """The Chebyshev value at the spectral plan's X = B / 2^s against the
+    Pade oracle's X for the deflated ER Laplacian: both approximate exp(iX)
+    to about 2^-53 in the 2-norm."""
+    from oracle.chebcs import cheb_cs
+
+    n = 200
+    g = -4.5 * laplacian(erdos_renyi(n, 0.5, seed=2))
+    st = P.pade_expm(1j * g, stages=True, spectral=True)
+    assert st["spectral"] and st["deflated"]
+    c, s, _ = cheb_cs(st["B"], st["rho"], st["s"])
+    assert np.linalg.norm(c + 1j * s - st["X"], 2) <= 1e-13
