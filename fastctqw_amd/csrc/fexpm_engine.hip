@@ -949,6 +949,28 @@ template <typename W> struct Pipeline : PipelineBase {
     }
     FX_LAUNCH();
   }
+  // Operand reuse across consecutive residue products: while c64_keep names
+  // a plane, a product whose first operand is that plane alone (scale 1)
+  // computes its row exponents and residues once and the following ones
+  // with the same order, k and buffer reuse them from the front of the
+  // shared residue buffer.  The caller sets c64_keep around such a sequence
+  // (KeepOperand), in which no other use of the buffer may come between.
+  const void* c64_keep = nullptr;
+  bool c64_kept = false;
+  int c64_kept_m = 0, c64_kept_k = 0;
+  const void* c64_kept_base = nullptr;
+  struct KeepOperand {
+    Pipeline* p;
+    KeepOperand(Pipeline* q, const void* plane) : p(q) {
+      p->c64_keep = plane;
+      p->c64_kept = false;
+    }
+    ~KeepOperand() {
+      p->c64_keep = nullptr;
+      p->c64_kept = false;
+    }
+  };
+
   // Symmetric C (epilogue e, m x m) = A B^T by residue products modulo
   // CRT64_K moduli (fexpm_crt.hpp; B null: B = A).  Returns the FP64-
   // equivalent flops of the product.
@@ -973,8 +995,17 @@ template <typename W> struct Pipeline : PipelineBase {
     uint8_t* c64_res = base + nops * op_bytes;
     const int G = rows / 32, KB = kpad / 32, T = rows / CRT_TILE, tiles_per = T * (T + 1) / 2;
     const int64_t jobs = (int64_t)G * KB;
+    const bool keep = c64_keep && A.p1 == c64_keep && !A.p2 && A.s1 == 1.0;
+    if (!keep) c64_kept = false;  // the front of the buffer gets another operand
     for (int b = 0; b < nops; ++b) {
       const OzOp& x = b == 0 ? A : *B;
+      if (b == 0 && keep) {
+        if (c64_kept && c64_kept_m == m && c64_kept_k == nk && c64_kept_base == base) continue;
+        c64_kept = true;
+        c64_kept_m = m;
+        c64_kept_k = nk;
+        c64_kept_base = base;
+      }
       crt64_row_exp<<<(m + 3) / 4, 256, 0, st>>>(x.p1, x.s1, x.p2, x.s2, x.k1, x.ldx, m, nk, c64_e[b]);
       FX_LAUNCH();
       const int vec = (x.k1 % 16 == 0 && x.ldx % 2 == 0 && (uintptr_t)x.p1 % 16 == 0 &&
@@ -1238,6 +1269,7 @@ template <typename W> struct Pipeline : PipelineBase {
     const double rinv = R > 0.0 ? 1.0 / R : 0.0;
     const double r2 = rinv * rinv;
     const double r8 = r2 * r2 * r2 * r2;
+    KeepOperand keep(this, mul.r);  // the Horner steps share their first operand C^8
     auto series = [&](const double* a, int nc, Plane2<W>* cur, Plane2<W>* nxt) {
       double c[4];
       const Plane2<W>* pl[4];
