@@ -527,6 +527,7 @@ PYBIND11_MODULE(_fastexpm_core, m) {
   m.attr("OPT_FP64_SLICES") = (int)FEXPM_OPT_FP64_SLICES;
   m.attr("OPT_SPECTRAL") = (int)FEXPM_OPT_SPECTRAL;
   m.attr("OPT_FAIL_ALLOC") = (int)FEXPM_OPT_FAIL_ALLOC;
+  m.attr("OPT_CRT_MAP") = (int)FEXPM_OPT_CRT_MAP;
   m.attr("OPT_DEFLATE") = (int)FEXPM_OPT_DEFLATE;
   m.attr("OPT_TWO_PRODUCT") = (int)FEXPM_OPT_TWO_PRODUCT;
   m.attr("OPT_CHEBYSHEV") = (int)FEXPM_OPT_CHEBYSHEV;

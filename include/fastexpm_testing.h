@@ -53,7 +53,7 @@ enum { FEXPM_OPT_Z_OVERLAP = 104 };
 enum { FEXPM_OPT_CHOL_OUTER = 105 };
 
 /* Engine option: tile edge of the FP32 structured squarings: 0 (default,
- * automatic: 64 up to n = 12288, else 128 with the tail split), 64 or 128.
+ * automatic: 64 up to n = 6144, else 128 with the tail split), 64 or 128.
  * Every choice gives the same values bit for bit. */
 enum { FEXPM_OPT_SQUARE_TILE = 106 };
 
@@ -72,6 +72,13 @@ enum { FEXPM_OPT_HOST_STAGING = 108 };
  * matrix buffers fail with FEXPM_ENOMEM after the old buffers are released
  * (host-side injection; no device call fails), to test the recovery path. */
 enum { FEXPM_OPT_FAIL_ALLOC = 109 };
+
+/* Engine option: block map of the batched int8 residue launches.  1
+ * (default): the 8 XCDs work on one problem at a time, each on an eighth of
+ * its tiles (at N = 10^4 a squaring launch takes 12.21 ms against 12.61);
+ * 0: the launch's blocks spread over the XCDs in eighths of all tiles of all
+ * problems.  The same values either way. */
+enum { FEXPM_OPT_CRT_MAP = 110 };
 
 /* Internal planes readable by fexpm_debug_plane (working precision, n x n). */
 enum fexpm_plane {

@@ -447,6 +447,25 @@ def test_int8crt_squarings(fx, n):
     assert infidelity(ref @ psi0, u.astype(np.complex128) @ psi0) < 1e-9
 
 
+@pytest.mark.parametrize("n", [1500])
+def test_crt_block_maps_agree(fx, n):
+    """Both block maps of the residue launches (testing option OPT_CRT_MAP)
+    give the same U bit for bit (squarings and FP64-accurate products)."""
+    core = fx._fastexpm_core
+    _say(f"residue block maps n={n}")
+    L = laplacian(erdos_renyi(n, 0.5, seed=77))
+    a = np.ascontiguousarray((-1j * 4.5 * L).astype(np.complex64))
+    with fx.ExpMatComplex64(n) as e:
+        u0 = e.run(a)
+        assert e.stats()["precision"] == "int8crt"
+        e._backend.set_option(core.OPT_CRT_MAP, 1)
+        u1 = e.run(a)
+        e._backend.set_option(core.OPT_CRT_MAP, 0)
+        with pytest.raises(ValueError):
+            e._backend.set_option(core.OPT_CRT_MAP, 2)
+    assert np.array_equal(u0, u1)
+
+
 def test_precision_auto_default(fx):
     """Complex64 engines default to the automatic squaring form: residue
     products for 768 <= n <= 16384 on structured input, FP32 MFMA below (and
