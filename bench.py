@@ -720,7 +720,10 @@ def run_c4(args):
         "metric": f"batched walks/s ({count} uneven walks, N in [64, 2048])",
         "value": count / (dev / 1000.0), "unit": "walks/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": dev, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "c64 (fp32 mfma products, fp64 pade/solve, fp64 psi)",
+        "vs_baseline": None,
+        "dtype": ("c64 (fp32 mfma squarings; " +
+                  ("chebyshev polynomial of each walk at its 1-norm bound from fp64-accurate int8-slice products"
+                   if all(x.get("chebyshev", 0) for x in st) else "fp64 pade/solve") + ", fp64 psi)"),
         "data": "synthetic: seeded d-regular / ER graphs, weights U(0,1], adjacency or Laplacian, t in [0.1,20]",
         "config": {"workload": f"C4: {count} walks, U_i=expm(-i t_i H_i), psi_i=U_i psi0 (uniform)",
                    "form": "lanes" if args.c4_lanes else "grouped launches, hipGraph, autotuned tiles",

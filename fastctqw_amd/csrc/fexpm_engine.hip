@@ -3435,6 +3435,9 @@ int fexpm_batch_set_option(fexpm_batch* b, int key, int64_t value) {
     case FEXPM_OPT_FAIL_ALLOC:
       b->fail_alloc = value ? 1 : 0;
       return FEXPM_OK;
+    case FEXPM_OPT_CHEBYSHEV:
+      if (b->grp) b->grp->cheb = value ? 1 : 0;
+      return FEXPM_OK;
   }
   b->err = "unknown or unsupported batch option";
   return FEXPM_EINVAL;
@@ -3612,6 +3615,10 @@ static void batch_run_grouped(fexpm_batch* b, const void* psi0, void* psi) {
     S.precision = b->dtype == FEXPM_COMPLEX64 ? G.precision : FEXPM_PREC_NATIVE;
     S.fp64_slices = b->dtype == FEXPM_COMPLEX64 ? G.oz_slices : 0;
     S.fp64_slices_scope = S.fp64_slices ? 1 : 0;
+    if (G.cheb) {
+      const ChebSeries cs = cheb_series(P.norm * std::ldexp(1.0, -P.s));
+      S.chebyshev = std::max(cs.dc, cs.ds);
+    }
   }
   if (out_of_range >= 0)
     throw Error(FEXPM_EINVAL, "walk " + std::to_string(out_of_range) +
@@ -4126,9 +4133,10 @@ namespace {
 template <typename Tin, typename W>
 std::string group_program_text(int precision, int fp64_slices, int64_t count, const int64_t* n, const int32_t* m,
                                const int32_t* s, const int32_t* shifted, const double* mu_re, const double* mu_im,
-                               int with_psi) {
+                               const double* norm, int with_psi) {
   GroupRunner<Tin, W> R(true);
   R.precision = precision;
+  R.cheb = norm ? 1 : 0;
   if (fp64_slices >= 0) R.oz_slices = fp64_slices;
   std::vector<int64_t> nv(n, n + count);
   std::vector<size_t> off((size_t)count), voff((size_t)count);
@@ -4153,6 +4161,8 @@ std::string group_program_text(int precision, int fp64_slices, int64_t count, co
     P.shifted = shifted[i] ? 1 : 0;
     P.mu_re = mu_re[i];
     P.mu_im = mu_im[i];
+    P.norm = norm ? norm[i] : 0.0;
+    if (norm && !(norm[i] >= 0.0 && std::isfinite(norm[i]))) throw Error(FEXPM_EINVAL, "bad plan (norm)");
   }
   R.build(plans, vaddr(VR_IN), vaddr(VR_OUT), reinterpret_cast<Scalars*>(vaddr(VR_SCAL)),
           with_psi ? reinterpret_cast<const double2*>(vaddr(VR_PSI0)) : nullptr,
@@ -4163,8 +4173,8 @@ std::string group_program_text(int precision, int fp64_slices, int64_t count, co
 
 extern "C" int fexpm_test_group_program(int dtype, int precision, int fp64_slices, int64_t count, const int64_t* n,
                                         const int32_t* m, const int32_t* s, const int32_t* shifted,
-                                        const double* mu_re, const double* mu_im, int with_psi, char* buf,
-                                        int64_t cap, int64_t* len) {
+                                        const double* mu_re, const double* mu_im, const double* norm, int with_psi,
+                                        char* buf, int64_t cap, int64_t* len) {
   if (count < 1 || !n || !m || !s || !shifted || !mu_re || !mu_im || !len) return FEXPM_EINVAL;
   if (dtype != FEXPM_COMPLEX64 && dtype != FEXPM_COMPLEX128) return FEXPM_EINVAL;
   if (precision < FEXPM_PREC_NATIVE || precision > FEXPM_PREC_BF16X3 ||
@@ -4174,9 +4184,9 @@ extern "C" int fexpm_test_group_program(int dtype, int precision, int fp64_slice
   try {
     const std::string t = dtype == FEXPM_COMPLEX64
                               ? group_program_text<float2, float>(precision, fp64_slices, count, n, m, s, shifted,
-                                                                  mu_re, mu_im, with_psi)
+                                                                  mu_re, mu_im, norm, with_psi)
                               : group_program_text<double2, double>(precision, fp64_slices, count, n, m, s, shifted,
-                                                                    mu_re, mu_im, with_psi);
+                                                                    mu_re, mu_im, norm, with_psi);
     *len = (int64_t)t.size();
     if (!buf || cap < (int64_t)t.size() + 1) return FEXPM_EINVAL;
     std::memcpy(buf, t.c_str(), t.size() + 1);

@@ -885,6 +885,7 @@ struct GroupRunnerBase {
   int forced = -1;
   int precision = FEXPM_PREC_NATIVE;  // squarings of complex64 batches: split 16-bit forms
   int oz_slices = 0;                  // Pade products from int8 slices (complex64 batches: 6), 0 = FP64 MFMA
+  int cheb = 1;                       // Chebyshev polynomial in place of the Pade products and solve
   // report of the last run
   int launches = 0, graph_used = 0, graph_built = 0, tuned = 0;
   int cand[GC_COUNT] = {0, 0, 0};
@@ -1137,7 +1138,8 @@ template <typename Tin, typename W> struct GroupRunner : GroupRunnerBase {
   }
   // C = (slices ba)(slices bb)^T with the Pade epilogue (symmetric product).
   void oz_prod(GroupStage& S, int i, int ba, int bb, double* c,
-               std::initializer_list<std::pair<double, const double*>> add = {}, double dg = 0.0) {
+               std::initializer_list<std::pair<double, const double*>> add = {}, double dg = 0.0,
+               double alpha = 1.0) {
     const int n = wm[i].n;
     OzArgs<GROUP_OZ_SLICES> g;
     std::memset(&g, 0, sizeof(g));
@@ -1153,7 +1155,7 @@ template <typename Tin, typename W> struct GroupRunner : GroupRunnerBase {
     g.epi.cr = c;
     g.epi.ldc = wm[i].np;
     g.epi.ld_add = wm[i].np;
-    g.epi.alpha = 1.0;
+    g.epi.alpha = alpha;
     int k = 0;
     for (auto& t : add) {
       g.epi.addc[k] = t.first;
@@ -1353,6 +1355,210 @@ template <typename Tin, typename W> struct GroupRunner : GroupRunnerBase {
     }
   }
 
+  // Chebyshev polynomial of exp(i X_i) in place of the Pade products and the
+  // solve: X_i = the prepared plane, whose spectral radius is at most
+  // theta_i = ||A_i - mu_i I||_1 / 2^s_i (the 1-norm plan's own bound), and
+  // cheb_series(theta_i) gives cos X = sum_j cw[j] w^j and
+  // sin X = (X / theta) sum_j sw[j] w^j, w = X^2 / theta^2.  Per walk: the
+  // powers X^2, X^4, X^6, X^8 its blocks need (planes 3, 4, 5, 7), the top
+  // blocks as combinations (cos in plane 1, sin in plane 6; a top block of one
+  // coefficient folds the first Horner step in), Horner steps
+  // T <- theta^-8 X^8 T + block as symmetric products with the block in the
+  // epilogue (cos ping-pongs in planes 1 / 2, sin in 6 / 8; the rounds are
+  // aligned so that every chain ends in the last one), X times the sin
+  // polynomial scaled by 1 / theta, and the split form of cos X + i sin X.
+  // complex64 batches take the products from int8 slices (buffer 0: X, then
+  // the cos iterates; 1: X^2, then the sin iterates; 2: X^4, then X^8).
+  template <class F> void build_cheb(const std::vector<GroupPlan>& plans, F& stage) {
+    const bool oz = oz_slices > 0 && std::is_same<W, float>::value;
+    if (oz) ensure_oz();
+    struct Chain {
+      const double* a = nullptr;  // coefficients in w
+      int nc = 0;                 // their count
+      int top = 0;                // block index of the top combination
+      int cur = 0;                // plane of the current iterate
+      int alt = 0;                // the other plane of the ping-pong
+    };
+    struct Walk {
+      ChebSeries cs;
+      Chain c[2];  // 0 cos, 1 sin
+      double r2 = 0.0, r8 = 0.0, rinv = 0.0;
+      int need = 0;  // highest power of X^2 the blocks read (0 .. 3), 4 with Horner steps
+    };
+    std::vector<Walk> wk(plans.size());
+    int H = 0;
+    for (size_t k = 0; k < plans.size(); ++k) {
+      const GroupPlan& P = plans[k];
+      Walk& w = wk[k];
+      const double theta = P.norm * std::ldexp(1.0, -P.s);
+      w.cs = cheb_series(theta);
+      w.rinv = theta > 0.0 ? 1.0 / theta : 0.0;
+      w.r2 = w.rinv * w.rinv;
+      w.r8 = w.r2 * w.r2 * w.r2 * w.r2;
+      for (int q = 0; q < 2; ++q) {
+        Chain& c = w.c[q];
+        c.a = q == 0 ? w.cs.cw : w.cs.sw;
+        c.nc = q == 0 ? w.cs.dc / 2 + 1 : (w.cs.ds - 1) / 2 + 1;
+        const int nb = (c.nc + 3) / 4;
+        c.top = (nb > 1 && c.nc % 4 == 1) ? nb - 2 : nb - 1;
+        c.cur = q == 0 ? 1 : 6;
+        c.alt = q == 0 ? 2 : 8;
+        w.need = std::max(w.need, std::min(3, c.nc - 1));
+        if (nb > 1) w.need = 4;
+        H = std::max(H, c.top);
+      }
+    }
+    auto block = [&](const Walk& w, const Chain& c, int i, const double** pl, double* cf) {
+      int q = 0;
+      double rj = 1.0;
+      for (int j = 1; j < 4 && 4 * i + j < c.nc; ++j) {
+        rj *= w.r2;
+        cf[q] = c.a[4 * i + j] * rj;
+        pl[q++] = nullptr;  // filled by the caller: the power X^2j of the walk
+      }
+      return q;
+    };
+    auto power = [&](int i, int j) -> double* { return fp(i, j == 4 ? 7 : 2 + j); };  // X^2j
+    // powers
+    if (oz) {
+      GroupStage& z0 = stage(GK_OZSLICE);
+      for (const GroupPlan& P : plans) oz_slice_prob(z0, P.idx, fp(P.idx, 0), 0);
+      GroupStage& g2 = stage(GK_OZGEMM, GC_SYM64);
+      for (size_t k = 0; k < plans.size(); ++k)
+        if (wk[k].need >= 1) oz_prod(g2, plans[k].idx, 0, 0, power(plans[k].idx, 1));
+      GroupStage& z1 = stage(GK_OZSLICE);
+      for (size_t k = 0; k < plans.size(); ++k)
+        if (wk[k].need >= 2) oz_slice_prob(z1, plans[k].idx, power(plans[k].idx, 1), 1);
+      GroupStage& g4 = stage(GK_OZGEMM, GC_SYM64);
+      for (size_t k = 0; k < plans.size(); ++k)
+        if (wk[k].need >= 2) oz_prod(g4, plans[k].idx, 1, 1, power(plans[k].idx, 2));
+      GroupStage& z2 = stage(GK_OZSLICE);
+      for (size_t k = 0; k < plans.size(); ++k)
+        if (wk[k].need >= 3) oz_slice_prob(z2, plans[k].idx, power(plans[k].idx, 2), 2);
+      GroupStage& g6 = stage(GK_OZGEMM, GC_SYM64);
+      for (size_t k = 0; k < plans.size(); ++k) {
+        const int i = plans[k].idx;
+        if (wk[k].need >= 3) oz_prod(g6, i, 2, 1, power(i, 3));
+        if (wk[k].need >= 4) oz_prod(g6, i, 2, 2, power(i, 4));
+      }
+    } else {
+      GroupStage& g2 = stage(GK_GEMM, GC_SYM64, MODE_REAL);
+      GroupStage& g4 = stage(GK_GEMM, GC_SYM64, MODE_REAL);
+      GroupStage& g6 = stage(GK_GEMM, GC_SYM64, MODE_REAL);
+      for (size_t k = 0; k < plans.size(); ++k) {
+        const int i = plans[k].idx, n = wm[i].n, np_ = wm[i].np;
+        if (wk[k].need >= 1) g2.add_gemm(g64(fp(i, 0), fp(i, 0), np_, n, n, np_, power(i, 1)), true, i);
+        if (wk[k].need >= 2) g4.add_gemm(g64(power(i, 1), power(i, 1), np_, n, n, np_, power(i, 2)), true, i);
+        if (wk[k].need >= 3) g6.add_gemm(g64(power(i, 2), power(i, 1), np_, n, n, np_, power(i, 3)), true, i);
+        if (wk[k].need >= 4) g6.add_gemm(g64(power(i, 2), power(i, 2), np_, n, n, np_, power(i, 4)), true, i);
+      }
+    }
+    // top blocks
+    {
+      GroupStage& lc = stage(GK_LINCOMB);
+      for (size_t k = 0; k < plans.size(); ++k) {
+        const int i = plans[k].idx;
+        const Walk& w = wk[k];
+        for (int q = 0; q < 2; ++q) {
+          const Chain& c = w.c[q];
+          OpLincomb::Prob L;
+          std::memset(&L, 0, sizeof(L));
+          const double* pl[4];
+          double cf[4];
+          int t = block(w, c, c.top, pl, cf);
+          for (int j = 0; j < t; ++j) pl[j] = power(i, j + 1);
+          if (c.top < (c.nc + 3) / 4 - 1) {  // the folded step: a[nc - 1] theta^-8 X^8
+            cf[t] = c.a[c.nc - 1] * w.r8;
+            pl[t++] = power(i, 4);
+          }
+          L.out = fp(i, c.cur);
+          for (int j = 0; j < t; ++j) {
+            L.in[j] = pl[j];
+            L.c[j] = cf[j];
+          }
+          L.nin = t;
+          L.diag = c.a[4 * c.top];
+          L.n = wm[i].n;
+          L.ld = wm[i].np;
+          lc.add(L, OpLincomb::blocks(L));
+        }
+      }
+    }
+    // Horner rounds
+    if (oz && H > 0) {
+      GroupStage& z = stage(GK_OZSLICE);
+      for (size_t k = 0; k < plans.size(); ++k)
+        if (wk[k].need >= 4) oz_slice_prob(z, plans[k].idx, power(plans[k].idx, 4), 2);
+    }
+    for (int r = 0; r < H; ++r) {
+      GroupStage* z = oz ? &stage(GK_OZSLICE) : nullptr;
+      GroupStage& g = oz ? stage(GK_OZGEMM, GC_SYM64) : stage(GK_GEMM, GC_SYM64, MODE_REAL);
+      for (size_t k = 0; k < plans.size(); ++k) {
+        const int i = plans[k].idx, n = wm[i].n, np_ = wm[i].np;
+        Walk& w = wk[k];
+        for (int q = 0; q < 2; ++q) {
+          Chain& c = w.c[q];
+          if (r < H - c.top) continue;  // this chain's steps end with the last round
+          const int bi = H - 1 - r;     // block added by this step
+          const double* pl[4];
+          double cf[4];
+          const int t = block(w, c, bi, pl, cf);
+          if (oz) {
+            oz_slice_prob(*z, i, fp(i, c.cur), q);
+            switch (t) {
+              case 0: oz_prod(g, i, 2, q, fp(i, c.alt), {}, c.a[4 * bi], w.r8); break;
+              case 1: oz_prod(g, i, 2, q, fp(i, c.alt), {{cf[0], power(i, 1)}}, c.a[4 * bi], w.r8); break;
+              case 2:
+                oz_prod(g, i, 2, q, fp(i, c.alt), {{cf[0], power(i, 1)}, {cf[1], power(i, 2)}}, c.a[4 * bi], w.r8);
+                break;
+              default:
+                oz_prod(g, i, 2, q, fp(i, c.alt), {{cf[0], power(i, 1)}, {cf[1], power(i, 2)}, {cf[2], power(i, 3)}},
+                        c.a[4 * bi], w.r8);
+                break;
+            }
+          } else {
+            GemmArgs<double> a = g64(power(i, 4), fp(i, c.cur), np_, n, n, np_, fp(i, c.alt));
+            a.epi.alpha = w.r8;
+            a.epi.nadd = t;
+            for (int j = 0; j < t; ++j) {
+              a.epi.addc[j] = cf[j];
+              a.epi.add_r[j] = power(i, j + 1);
+            }
+            a.epi.diag = c.a[4 * bi];
+            g.add_gemm(a, true, i);
+          }
+          std::swap(c.cur, c.alt);
+        }
+      }
+    }
+    // sin X = (X / theta) S, into the free sin plane
+    if (oz) {
+      GroupStage& z = stage(GK_OZSLICE);
+      for (size_t k = 0; k < plans.size(); ++k) {
+        oz_slice_prob(z, plans[k].idx, fp(plans[k].idx, 0), 0);
+        oz_slice_prob(z, plans[k].idx, fp(plans[k].idx, wk[k].c[1].cur), 1);
+      }
+      GroupStage& g = stage(GK_OZGEMM, GC_SYM64);
+      for (size_t k = 0; k < plans.size(); ++k)
+        oz_prod(g, plans[k].idx, 0, 1, fp(plans[k].idx, wk[k].c[1].alt), {}, 0.0, wk[k].rinv);
+    } else {
+      GroupStage& g = stage(GK_GEMM, GC_SYM64, MODE_REAL);
+      for (size_t k = 0; k < plans.size(); ++k) {
+        const int i = plans[k].idx, n = wm[i].n, np_ = wm[i].np;
+        GemmArgs<double> a = g64(fp(i, 0), fp(i, wk[k].c[1].cur), np_, n, n, np_, fp(i, wk[k].c[1].alt));
+        a.epi.alpha = wk[k].rinv;
+        g.add_gemm(a, true, i);
+      }
+    }
+    GroupStage& so = stage(GK_SOLUTION);
+    for (size_t k = 0; k < plans.size(); ++k) {
+      const int i = plans[k].idx, np_ = wm[i].np;
+      typename OpSolution<W>::Prob p{fp(i, wk[k].c[0].cur), fp(i, wk[k].c[1].alt), np_, wp(i, 0, 0), wp(i, 0, 1),
+                                     np_, wm[i].n, wm[i].dsq};
+      so.add(p, OpSolution<W>::blocks(p));
+    }
+  }
+
   // Stages are held in a deque: references returned by stage() stay valid
   // while later stages are appended.
   void build(const std::vector<GroupPlan>& plans, const void* d_in, void* d_out, Scalars* d_scal,
@@ -1378,6 +1584,8 @@ template <typename Tin, typename W> struct GroupRunner : GroupRunnerBase {
         S.add(p, OpPrepare<Tin>::blocks(p));
       }
     }
+    if (cheb) build_cheb(plans, stage);
+    else {
     // 2. Pade products (real symmetric, half tiles); A^{2k} = (-1)^k G^{2k}
     const bool oz = oz_slices > 0 && std::is_same<W, float>::value;
     if (oz) build_pade_oz(plans, stage);
@@ -1588,6 +1796,7 @@ template <typename Tin, typename W> struct GroupRunner : GroupRunnerBase {
         so.add(p, OpSolution<W>::blocks(p));
       }
     }
+    }  // Pade route
     // 4. squaring rounds: round r squares every walk with s_i > r
     int smax = 0;
     for (const GroupPlan& P : plans) smax = std::max(smax, P.s);
@@ -1743,7 +1952,9 @@ template <typename Tin, typename W> struct GroupRunner : GroupRunnerBase {
     key.push_back((double)kblock);
     key.push_back((double)precision);
     key.push_back((double)oz_slices);
+    key.push_back((double)cheb);
     for (const GroupPlan& P : plans) {
+      key.push_back(P.norm);
       key.push_back(P.idx);
       key.push_back(P.m);
       key.push_back(P.s);

@@ -152,7 +152,10 @@ int fexpm_test_grouped_tile_order(int T, int* out);
 int fexpm_test_grouped_rect_order(int nbm, int nbn, int* out);
 
 /* Grouped batch program for `count` CTQW walks with the given orders and
- * plans (Pade degree m, squarings s, trace shift), built by the same code as
+ * plans (Pade degree m, squarings s, trace shift, and the 1-norm `norm` of
+ * the shifted matrix: with norm non-null the Chebyshev polynomial of each
+ * walk's X = (A - mu I) / 2^s at theta = norm / 2^s replaces the Pade
+ * products and the solve, as in batch runs by default), built by the same code as
  * a grouped batch run but on a virtual address layout (no device memory is
  * touched): address = region << 40 | byte offset, regions 1 FP64 planes,
  * 2 squaring planes, 3 per-walk scalars and diagonals, 4 split parts,
@@ -164,7 +167,8 @@ int fexpm_test_grouped_rect_order(int nbm, int nbn, int* out);
  * CPU interpreter check the program against the oracle. */
 int fexpm_test_group_program(int dtype, int precision, int fp64_slices, int64_t count, const int64_t* n,
                              const int32_t* m, const int32_t* s, const int32_t* shifted, const double* mu_re,
-                             const double* mu_im, int with_psi, char* buf, int64_t cap, int64_t* len);
+                             const double* mu_im, const double* norm, int with_psi, char* buf, int64_t cap,
+                             int64_t* len);
 
 #ifdef __cplusplus
 }

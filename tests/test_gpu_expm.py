@@ -1030,6 +1030,40 @@ def _eig_ref(a):
 
 
 @pytest.mark.parametrize("dt", [np.complex64, np.complex128])
+def test_batch_chebyshev_route(fx, dt):
+    """Grouped batches take the Chebyshev polynomial of each walk at theta =
+    ||A - mu I||_1 / 2^s (the 1-norm plan's bound) in place of the Pade
+    products and the solve (default); the per-walk degree against
+    oracle.chebcs; U against the eigen oracle and against the Pade route
+    (option OPT_CHEBYSHEV = 0) of the same batch."""
+    from fastctqw_amd.graphs import c4_walks
+    from oracle.chebcs import cheb_degrees
+
+    core = fx._fastexpm_core
+    walks = c4_walks(count=10, seed=4300, nmin=16, nmax=400)
+    ts = [1e-3, 0.05, 0.4, 1.5, 3.0, 6.0, 9.0, 14.0, 20.0, 0.8]
+    mats = [np.ascontiguousarray((-1j * t * w["H"]).astype(dt)) for w, t in zip(walks, ts)]
+    _say(f"batch Chebyshev route {np.dtype(dt).name}: n = {[m.shape[0] for m in mats]}")
+    b = fx.ExpMatBatch(nmax=400, dtype=dt, lanes=1)
+    us = b.run(mats)
+    tol_fn = (lambda s: 1e-11) if dt == np.complex128 else (lambda s: _tol(np.complex64, s))
+    for i, (a, u) in enumerate(zip(mats, us)):
+        st = b.stats(i)
+        ref_st = pade_expm(a.astype(np.complex128), stages=True)
+        assert st["m"] == ref_st["m"] and st["s"] == ref_st["s"], (i, st)
+        assert st["chebyshev"] == max(cheb_degrees(st["norm"] * 2.0 ** -st["s"])), (i, st)
+        err = rel_fro(_eig_ref(a), u)
+        print(f"  walk {i} n={a.shape[0]} s={st['s']} degree={st['chebyshev']} relF={err:.2e}", flush=True)
+        assert err <= tol_fn(st["s"]), (i, err)
+    b._backend.set_option(core.OPT_CHEBYSHEV, 0)
+    up = b.run(mats)
+    for i in range(len(mats)):
+        assert b.stats(i)["chebyshev"] == 0
+        assert rel_fro(up[i], us[i]) <= 2 * tol_fn(b.stats(i)["s"]), i
+    b._backend.set_option(core.OPT_CHEBYSHEV, 1)
+
+
+@pytest.mark.parametrize("dt", [np.complex64, np.complex128])
 def test_batch_grouped(fx, dt):
     """Grouped launches over a batch of uneven walks: every Pade degree
     (t from 1e-4 to 20), orders 1 .. 300 around the 64-wide padding, a

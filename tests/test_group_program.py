@@ -28,7 +28,9 @@ COMPLEX64, COMPLEX128 = 2, 3
 PREC = {"native": 0, "fp16x2": 1, "bf16x3": 2}
 
 
-def _program(dtype, precision, n, m, s, mu, with_psi, slices=-1):
+def _program(dtype, precision, n, m, s, mu, with_psi, slices=-1, norm=None):
+    """Listing of the batch program; with `norm` (1-norms of the shifted
+    inputs) the Chebyshev route, else the Pade route."""
     lib = ctypes.CDLL(LIB)
     cnt = len(n)
     n_ = np.asarray(n, dtype=np.int64)
@@ -38,10 +40,11 @@ def _program(dtype, precision, n, m, s, mu, with_psi, slices=-1):
     mr = np.asarray([complex(x).real for x in mu], dtype=np.float64)
     mi = np.asarray([complex(x).imag for x in mu], dtype=np.float64)
     ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    nrm = None if norm is None else np.asarray(norm, dtype=np.float64)
     ln = ctypes.c_int64(0)
     args = (ctypes.c_int(dtype), ctypes.c_int(precision), ctypes.c_int(slices), ctypes.c_int64(cnt), ptr(n_), ptr(m_),
             ptr(s_), ptr(sh),
-            ptr(mr), ptr(mi), ctypes.c_int(1 if with_psi else 0))
+            ptr(mr), ptr(mi), None if nrm is None else ptr(nrm), ctypes.c_int(1 if with_psi else 0))
     lib.fexpm_test_group_program(*args, None, ctypes.c_int64(0), ctypes.byref(ln))
     buf = ctypes.create_string_buffer(ln.value + 1)
     rc = lib.fexpm_test_group_program(*args, buf, ctypes.c_int64(ln.value + 1), ctypes.byref(ln))
@@ -280,22 +283,27 @@ def _walks(seed):
     return mats
 
 
+@pytest.mark.parametrize("cheb", [False, True])
 @pytest.mark.parametrize("dt,precision,slices", [(COMPLEX128, "native", -1), (COMPLEX64, "native", -1),
                                                  (COMPLEX64, "native", 0), (COMPLEX64, "fp16x2", -1),
                                                  (COMPLEX64, "bf16x3", 0)])
-def test_group_program_matches_oracle(dt, precision, slices):
+def test_group_program_matches_oracle(dt, precision, slices, cheb):
+    """Pade route and Chebyshev route (the batch default): every walk's U
+    and psi against the Pade oracle and U against the eigen oracle."""
     mats = _walks(11)
     plans = []
     for a in mats:
         st = pade_expm(a, stages=True)
         m, s = (st["m"], st["s"]) if st["m"] else (3, 0)
-        plans.append((m, s, st["mu"]))
+        plans.append((m, s, st["mu"], st["norm"]))
     ms = sorted({p[0] for p in plans})
     assert ms == [3, 5, 7, 9, 13], ms  # every Pade degree is exercised
     psi0 = [np.exp(0.3j * np.arange(a.shape[0])) / np.sqrt(a.shape[0]) for a in mats]
     text = _program(dt, PREC[precision], [a.shape[0] for a in mats], [p[0] for p in plans],
-                    [p[1] for p in plans], [p[2] for p in plans], True, slices)
+                    [p[1] for p in plans], [p[2] for p in plans], True, slices,
+                    [p[3] for p in plans] if cheb else None)
     assert ("OZGEMM" in text) == (dt == COMPLEX64 and slices != 0)
+    assert ("CHOL" in text) == (not cheb)
     it = Interp(text, mats, psi0)
     it.run()
     off = 0
@@ -346,16 +354,16 @@ def test_group_program_rejects_bad_plans():
     d = np.zeros(1)
     ln = ctypes.c_int64(0)
     p = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
-    assert lib.fexpm_test_group_program(2, 0, -1, ctypes.c_int64(1), p(n), p(bad_m), p(z), p(z), p(d), p(d), 0,
+    assert lib.fexpm_test_group_program(2, 0, -1, ctypes.c_int64(1), p(n), p(bad_m), p(z), p(z), p(d), p(d), None, 0,
                                         None, ctypes.c_int64(0), ctypes.byref(ln)) == -1
     good_m = np.array([13], np.int32)
     # split precision is for complex64 only
-    assert lib.fexpm_test_group_program(3, 1, -1, ctypes.c_int64(1), p(n), p(good_m), p(z), p(z), p(d), p(d), 0,
+    assert lib.fexpm_test_group_program(3, 1, -1, ctypes.c_int64(1), p(n), p(good_m), p(z), p(z), p(d), p(d), None, 0,
                                         None, ctypes.c_int64(0), ctypes.byref(ln)) == -1
     # int8-slice products: 0 or 6, complex64 only
-    assert lib.fexpm_test_group_program(3, 0, 6, ctypes.c_int64(1), p(n), p(good_m), p(z), p(z), p(d), p(d), 0,
+    assert lib.fexpm_test_group_program(3, 0, 6, ctypes.c_int64(1), p(n), p(good_m), p(z), p(z), p(d), p(d), None, 0,
                                         None, ctypes.c_int64(0), ctypes.byref(ln)) == -1
-    assert lib.fexpm_test_group_program(2, 0, 5, ctypes.c_int64(1), p(n), p(good_m), p(z), p(z), p(d), p(d), 0,
+    assert lib.fexpm_test_group_program(2, 0, 5, ctypes.c_int64(1), p(n), p(good_m), p(z), p(z), p(d), p(d), None, 0,
                                         None, ctypes.c_int64(0), ctypes.byref(ln)) == -1
 
 
@@ -390,4 +398,54 @@ def test_group_program_detects_planted_mistakes():
     k = next(j for j, x in enumerate(f) if x.startswith("diag="))
     f[k] = "diag=" + repr(float(f[k][5:]) * (1 + 1e-9))
     changed = "\n".join(" ".join(f) if i == lc else ln for i, ln in enumerate(lines))
+    assert worst(changed) > 1e-11
+
+
+def test_group_program_chebyshev_structure_and_mistakes():
+    """Chebyshev route: no Cholesky or pack stages; each walk's degree and
+    product count follow oracle.chebcs at theta = norm / 2^s; a dropped
+    Horner product or a coefficient changed by one part in 10^9 is caught."""
+    from oracle.chebcs import cheb_degrees, horner_products
+
+    mats = _walks(14)[3:]
+    plans = [pade_expm(a, stages=True) for a in mats]
+    m = [p["m"] or 3 for p in plans]
+    s = [p["s"] for p in plans]
+    norm = [p["norm"] for p in plans]
+    args = ([a.shape[0] for a in mats], m, s, [p["mu"] for p in plans])
+    text = _program(COMPLEX128, 0, *args, False, -1, norm)
+    kinds = [int(ln.split()[1].split("=")[1]) for ln in text.splitlines() if ln.startswith("STAGE")]
+    assert 9 not in kinds and 5 not in kinds  # no CHOL, no PACK
+    gemm = [ln for ln in text.splitlines() if ln.startswith("GEMM") and "mode=0" in ln]
+    want = 0
+    for i, a in enumerate(mats):
+        dc, ds = cheb_degrees(norm[i] * 2.0 ** -s[i])
+        nc, ns = dc // 2 + 1, (ds - 1) // 2 + 1
+        powers = 0 if max(nc, ns) == 1 else (1 if max(nc, ns) == 2 else (2 if max(nc, ns) == 3 else 3))
+        if nc > 4 or ns > 4:
+            powers = 4
+        want += powers + horner_products(nc) + horner_products(ns) + 1
+    assert len(gemm) == want, (len(gemm), want)
+    lines = text.splitlines()
+
+    def worst(listing):
+        it = Interp(listing, mats, None)
+        it.run()
+        off, err = 0, 0.0
+        for a in mats:
+            n = a.shape[0]
+            err = max(err, rel_fro(pade_expm(a), it.result(off, n)))
+            off += n * n
+        return err
+
+    assert worst(text) < 1e-12
+    horner = [i for i, ln in enumerate(lines) if ln.startswith("GEMM") and "nadd=3" in ln]
+    assert horner
+    dropped = "\n".join(ln for i, ln in enumerate(lines) if i != horner[0])
+    assert worst(dropped) > 1e-6
+    # the constant term of a last Horner step (block 0: cw[0] or sw[0], of order 1)
+    f = lines[horner[-1]].split()
+    k = next(j for j, x in enumerate(f) if x.startswith("diag="))
+    f[k] = "diag=" + repr(float(f[k][5:]) * (1 + 1e-9))
+    changed = "\n".join(" ".join(f) if i == horner[-1] else ln for i, ln in enumerate(lines))
     assert worst(changed) > 1e-11
