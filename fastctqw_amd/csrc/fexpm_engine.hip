@@ -1249,13 +1249,15 @@ template <typename W> struct Pipeline : PipelineBase {
     pade_assemble(m, 1.0, st);
   }
 
-  // ---- Chebyshev stage (spectral plans, in place of Pade and solve) ----
+  // ---- Chebyshev stage (structured input, in place of Pade and solve) ----
   // exp(i X) for X = B theta / R, B real symmetric in a.r with spectral
-  // radius at most R (the spectral bound), theta = R / 2^s, from the
-  // polynomial of cheb_series(theta):
+  // radius at most R, from the polynomial of cheb_series(theta):
   //     cos X = sum_j cw[j] Y^j,  sin X = (B / R) sum_j sw[j] Y^j,  Y = C^2 / R^2,
-  // C^2k the powers of the unscaled B that spectral_front formed (C^2, C^4,
-  // C^6 in p2, p4, p6, C^8 in x1).  Each polynomial in Y by Horner's rule in
+  // C^2k the powers of B (C^2, C^4, C^6 in p2, p4, p6, C^8 in x1, as far as
+  // the polynomial reads them).  Spectral plans: B the unscaled matrix,
+  // R the spectral bound, theta = R / 2^s (spectral_front formed the
+  // powers); 1-norm plans: B = X itself and R = theta = ||A - mu I||_1 / 2^s
+  // (front_cheb).  Each polynomial in Y by Horner's rule in
   // Y^4 over blocks of four coefficients (Paterson-Stockmeyer): a block
   // c0 I + c1 Y + c2 Y^2 + c3 Y^3 is a combination of the powers, and a step
   // T <- Y^4 T + block one symmetric product (C^8 times T, scaled by R^-8)
@@ -1900,6 +1902,24 @@ template <typename W> struct Pipeline : PipelineBase {
       pade_general(m, st);
   }
 
+  // Chebyshev stage of a 1-norm plan (structured input): the scaled, shifted
+  // X = Im(A - mu I) / 2^s and the powers X^2, X^4, X^6, X^8 that the
+  // polynomial of cheb_series(theta) reads (p2, p4, p6, x1).
+  template <typename Tin>
+  void front_cheb(const Tin* d_a, double mu_re, double mu_im, double scale, double theta, hipStream_t st) {
+    ensure_stage_buffers(false);
+    dim3 grid((n + 255) / 256, n);
+    prepare_planar<Tin, W><<<grid, 256, 0, st>>>(d_a, n, a.r, a.i, ld, mu_re, mu_im, scale, 1);
+    FX_LAUNCH();
+    const ChebSeries cs = cheb_series(theta);
+    const int nc = cs.dc / 2 + 1, ns = (cs.ds - 1) / 2 + 1;
+    const int need = (nc > 4 || ns > 4) ? 4 : std::min(3, std::max(nc, ns) - 1);
+    if (need >= 1) sym_real(a, a, p2, st);
+    if (need >= 2) sym_real(p2, p2, p4, st);
+    if (need >= 3) sym_real(p4, p2, p6, st);
+    if (need >= 4) sym_real(p4, p4, x1, st);
+  }
+
   // FP64 companion of an FP32 engine: Pade, solve and leading squarings in
   // FP64 (allocated on first use).
   std::unique_ptr<Pipeline<double>> hi;
@@ -1940,7 +1960,7 @@ template <typename W> struct Pipeline : PipelineBase {
     double rho;     // spectral-radius bound of B (spectral plans)
     int m, s, s_do, q64;
     int s_onenorm;  // s of the 1-norm rule
-    bool cheb;      // Chebyshev polynomial in place of Pade and solve (spectral plans)
+    bool cheb;      // Chebyshev polynomial in place of Pade and solve (structured, FP64 stage)
   };
 
   // spectral_ok: the caller can wait for the spectral bound on the host
@@ -1968,6 +1988,7 @@ template <typename W> struct Pipeline : PipelineBase {
     P.spectral = spectral_ok && spectral && P.structured && P.m == 13 && P.s > 0 && P.s < 120 &&
                  (P.hybrid || std::is_same<W, double>::value);
     P.deflate = P.spectral && deflate && hs.rowsum_equal && std::isfinite(hs.rowsum_c);
+    P.cheb = cheb && P.structured && (P.hybrid || std::is_same<W, double>::value);
     P.c = P.deflate ? hs.rowsum_c : 0.0;
     P.alpha = P.deflate ? (P.mu_im - P.c) / (double)n : 0.0;
     P.rho = 0.0;
@@ -1990,6 +2011,13 @@ template <typename W> struct Pipeline : PipelineBase {
     if (oz_scope != 0) return oz_scope;
     return 1 | ((P.deflate && n >= 4096) ? 2 : 0);
   }
+
+  // Chebyshev plans: theta bounds the spectral radius of X = B / 2^s (the
+  // spectral bound, else the 1-norm of the shifted input), and R bounds that
+  // of the planes whose powers the stage uses (the unscaled B of spectral
+  // plans, the scaled X otherwise).
+  static double cheb_theta(const Plan& P) { return (P.spectral ? P.rho : P.norm) * std::ldexp(1.0, -P.s); }
+  static double cheb_radius(const Plan& P) { return P.spectral ? P.rho : cheb_theta(P); }
 
   // Fields that follow from s.
   void finish_plan(Plan& P) const {
@@ -2018,7 +2046,6 @@ template <typename W> struct Pipeline : PipelineBase {
     h->crt_map = crt_map;
     P.rho = h->template spectral_front<Tin>(d_a, P.mu_im, P.alpha, cheb != 0, st);
     P.s = scaling_for(P.rho);
-    P.cheb = cheb != 0;
     finish_plan(P);
   }
 
@@ -2054,8 +2081,10 @@ template <typename W> struct Pipeline : PipelineBase {
       h->crt_map = crt_map;
       int dh;
       if (P.cheb) {
-        // Chebyshev polynomial from the powers formed by spectral_plan; no solve
-        auto cs = h->cheb_structured(P.rho * scale, P.rho, st);
+        // Chebyshev polynomial from the powers formed by spectral_plan (or
+        // here, for 1-norm plans); no solve
+        if (!P.spectral) h->front_cheb<Tin>(d_a, P.mu_re, P.mu_im, scale, cheb_theta(P), st);
+        auto cs = h->cheb_structured(cheb_theta(P), cheb_radius(P), st);
         if (timing) FX_HIP(hipEventRecord(ev[2], st));
         FX_HIP(hipMemsetAsync(&h->scal->info, 0, sizeof(int), st));
         dh = h->solution_tail(cs.first->r, cs.second->r, h->ld, nullptr, nullptr, 0, 0, st);
@@ -2077,7 +2106,8 @@ template <typename W> struct Pipeline : PipelineBase {
       bool done = false;
       if constexpr (std::is_same<W, double>::value) {
         if (P.cheb) {
-          auto cs = cheb_structured(P.rho * scale, P.rho, st);
+          if (!P.spectral) front_cheb<Tin>(d_a, P.mu_re, P.mu_im, scale, cheb_theta(P), st);
+          auto cs = cheb_structured(cheb_theta(P), cheb_radius(P), st);
           if (timing) FX_HIP(hipEventRecord(ev[2], st));
           dc = solution_tail(cs.first->r, cs.second->r, ld, nullptr, nullptr, 0, 0, st);
           done = true;
@@ -2167,7 +2197,7 @@ template <typename W> struct Pipeline : PipelineBase {
     S.rho_bound = P.spectral ? P.rho : 0.0;
     S.s_two_product = P.nonfinite ? 0 : P.two;
     if (P.cheb && !P.nonfinite) {
-      const ChebSeries cs = cheb_series(P.rho * std::ldexp(1.0, -P.s));
+      const ChebSeries cs = cheb_series(cheb_theta(P));
       S.chebyshev = std::max(cs.dc, cs.ds);
     } else {
       S.chebyshev = 0;

@@ -104,9 +104,11 @@ enum fexpm_option {
                                     (double-double sums), deflate the exact eigenvector 1 first:
                                     B = G - mu I + (mu - c)/n J and exp(A) = e^{i mu} (exp(iB) +
                                     (e^{i(c - mu)} - 1) J / n) (default); 0 = no deflation */
-  FEXPM_OPT_CHEBYSHEV = 15       /* spectral plans (s from the bound R of rho(B), theta = R / 2^s <=
-                                    theta_13): 1 = exp(iX), X = B / 2^s, from the Chebyshev (Jacobi-
-                                    Anger) expansion of e^{ix} on [-theta, theta], cos x = J0(theta) +
+  FEXPM_OPT_CHEBYSHEV = 15       /* structured input with an FP64 stage (complex128, and complex64 with
+                                    FEXPM_OPT_PADE_FP64): 1 = exp(iX), X = B / 2^s, from the Chebyshev
+                                    (Jacobi-Anger) expansion of e^{ix} on [-theta, theta], theta =
+                                    R / 2^s with R the spectral bound of B (spectral plans) or
+                                    ||A - mu I||_1 (1-norm plans, batches), cos x = J0(theta) +
                                     2 sum (-1)^k J2k(theta) T2k(x/theta), sin x = 2 sum (-1)^k
                                     J2k+1(theta) T2k+1(x/theta), each truncated where the bound
                                     2 sum (theta/2)^m/m! of its tail reaches 2^-54, evaluated in X^2

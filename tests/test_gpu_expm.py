@@ -215,10 +215,18 @@ def test_chebyshev_route(fx, dt, n):
             up = e.run(a)
             e._backend.set_option(core.OPT_CHEBYSHEV, 1)
             diff = rel_fro(up, u)
+            # the 1-norm plan (option OPT_SPECTRAL = 0): the polynomial at theta = ||A - mu I||_1 / 2^s
+            e._backend.set_option(core.OPT_SPECTRAL, 0)
+            u1 = e.run(a)
+            s1 = e.stats()
+            e._backend.set_option(core.OPT_SPECTRAL, 1)
+            assert s1["spectral"] == 0 and s1["chebyshev"] == max(cheb_degrees(s1["norm"] * 2.0 ** -s1["s"])), s1
+            err1 = rel_fro(ref, u1)
             print(f"  s={stats['s']} theta={theta:.3f} degree={stats['chebyshev']} relF={err:.3e} "
-                  f"value relF={errx:.3e} vs Pade {diff:.3e}", flush=True)
+                  f"value relF={errx:.3e} vs Pade {diff:.3e}; 1-norm plan s={s1['s']} degree={s1['chebyshev']} "
+                  f"relF={err1:.3e}", flush=True)
             tol = _tol(dt, stats["s"])
-            assert err <= tol and diff <= tol
+            assert err <= tol and diff <= tol and err1 <= _tol(dt, s1["s"])
             assert errx <= (1e-6 if dt == np.complex64 else 1e-12)
     assert seen_s0
 
