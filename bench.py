@@ -97,11 +97,11 @@ def squaring_roofline(n, ms_square, s, two, issued_3m, peak, split_parts=0):
     }
 
 
-def crt_roofline(n, kernel_ms, launches, ms_square, s):
+def crt_roofline(n, kernel_ms, launches, ms_square, s, moduli=9):
     """Roofline fields of the int8 residue squarings: the dominant kernel is
-    the batched residue product (27 int8 products per squaring: yr', yi' and
-    s' = yr' + yi' against themselves, modulo 9 moduli, lower 256 x 256
-    tiles).  Its algorithmic work per launch is 27 n^3 int8 operations (the
+    the batched residue product (3 K int8 products per squaring: yr', yi' and
+    s' = yr' + yi' against themselves, modulo K moduli, lower 256 x 256
+    tiles).  Its algorithmic work per launch is 3 K n^3 int8 operations (the
     lower half of each n x n x n product: n^2/2 outputs, n multiply-adds,
     2 operations each); ``issued`` counts the whole lower tiles and the padded
     k.  kernel_ms is the launches' own device time (HIP events around each
@@ -112,8 +112,9 @@ def crt_roofline(n, kernel_ms, launches, ms_square, s):
     rows = (n + 255) // 256 * 256
     kpad = (n + 63) // 64 * 64
     T = rows // 256
-    alg = 27.0 * float(n) ** 3
-    issued = 27.0 * 2.0 * (T * (T + 1) // 2) * 256.0 * 256.0 * kpad
+    nprob = 3.0 * moduli
+    alg = nprob * float(n) ** 3
+    issued = nprob * 2.0 * (T * (T + 1) // 2) * 256.0 * 256.0 * kpad
     t_launch = kernel_ms / launches * 1e-3
     t_phase = ms_square * 1e-3
     return {
@@ -277,6 +278,7 @@ def run_c5(args):
 
     a, psi0 = _c5_inputs(n, p, seed, dt, nstates)
     eng = ExpMatComplex64(n)
+    _engine_options(eng, args)
     eng.upload(a)
     for _ in range(args.warmup):
         eng.run_resident()
@@ -457,6 +459,8 @@ def run_general(args):
 
 C3_TIMES = (1.0, 4.5, 20.0)
 C3_PRECISIONS = ("native", "int8crt", "fp16x2", "bf16x3")
+# C3 cases: the squaring forms, the residue form with each moduli count
+C3_CASES = ("native", "int8crt:9", "int8crt:8", "int8crt:7", "fp16x2", "bf16x3")
 C3_PROBES = 8
 
 
@@ -513,7 +517,7 @@ def run_c3(args):
     import subprocess
     import tempfile
 
-    from fastctqw_amd.fastexpm import ExpMatComplex64
+    from fastctqw_amd.fastexpm import ExpMatComplex64, _fastexpm_core
     from fastctqw_amd.graphs import random_state
 
     n = args.n
@@ -541,8 +545,11 @@ def run_c3(args):
             eng.upload(a)
             del a
             key = f"{int(weighted)}_{t}"
-            for prec in C3_PRECISIONS:
+            for case_name in C3_CASES:
+                prec, _, kmod = case_name.partition(":")
                 eng.set_precision(prec)
+                if kmod:
+                    eng._backend.set_option(_fastexpm_core.OPT_CRT_MODULI, int(kmod))
                 for _ in range(args.warmup):
                     eng.run_resident()
                 ms = []
@@ -553,6 +560,8 @@ def run_c3(args):
                 last_st = st
                 case = {"graph": "weighted" if weighted else "unweighted", "t": t, "precision": prec,
                         "s": st["s"], "ms": float(np.median(ms)), "ms_square": st["ms_square"]}
+                if prec == "int8crt":
+                    case["moduli"], case["beta"] = st["crt_moduli"], st["crt_beta"]
                 if ref is not None:
                     out = eng.apply(np.vstack([e0, psi_r, Z.T]))
                     uz = out[2:].T
@@ -571,9 +580,10 @@ def run_c3(args):
         "metric": "s per N=10k complex expm, squaring precision study (fastest mode at t=4.5, unweighted)",
         "value": best["ms"] / 1000.0, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": best["ms"], "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": f"c64 ({_square_phrase(best['precision'])}, {_slice_phrase(last_st)}",
+        "dtype": f"c64 ({_square_phrase(best['precision'], last_st)}, {_slice_phrase(last_st)}",
         "data": "synthetic: seeded ER(10000,0.5) Laplacian, unweighted and weighted U(0,1]",
-        "config": {"workload": "C3: expm(-i t L), t in {1, 4.5, 20}, precision native/int8crt/fp16x2/bf16x3",
+        "config": {"workload": "C3: expm(-i t L), t in {1, 4.5, 20}, precision native/int8crt (7, 8, 9 moduli)/"
+                               "fp16x2/bf16x3",
                    "n": n, "native_ms_t4.5": native["ms"], "speedup_vs_native": native["ms"] / best["ms"]},
         "cases": cases,
     }
@@ -1045,11 +1055,24 @@ def _slice_phrase(st):
     return f"fp64-accurate {where} from {how}{rest})"
 
 
-def _square_phrase(prec):
+def _engine_options(eng, args, cdt=np.complex64):
+    """The residue options of the command line on a complex64 engine."""
+    if cdt != np.complex64:
+        return
+    from fastctqw_amd.fastexpm import _fastexpm_core
+    if getattr(args, "crt_moduli", None) is not None:
+        eng._backend.set_option(_fastexpm_core.OPT_CRT_MODULI, args.crt_moduli)
+    if getattr(args, "crt64_moduli", None) is not None:
+        eng._backend.set_option(_fastexpm_core.OPT_CRT64_MODULI, args.crt64_moduli)
+
+
+def _square_phrase(prec, st=None):
     """How the squarings of a complex64 run computed their products."""
+    K = int((st or {}).get("crt_moduli", 0) or 9)
+    beta = int((st or {}).get("crt_beta", 0) or 27)
     return {"native": "fp32 mfma squarings",
-            "int8crt": "squarings from exact int8 residue products (9 moduli, rows rounded to 27 bits, exact "
-                       "int32 accumulation, chinese-remainder reconstruction)",
+            "int8crt": f"squarings from exact int8 residue products ({K} moduli, rows rounded to {beta} bits, "
+                       "exact int32 accumulation, chinese-remainder reconstruction)",
             "fp16x2": "fp16x2 split squarings (fp32 accumulation)",
             "bf16x3": "bf16x3 split squarings (fp32 accumulation)"}.get(prec, prec + " squarings")
 
@@ -1099,6 +1122,11 @@ def main():
     ap.add_argument("--no-c128", action="store_true", help="c2: skip the complex128 run")
     ap.add_argument("--no-perf-inputs", action="store_true",
                     help="general: skip the reference's performance inputs (16..1024) beside scipy")
+    ap.add_argument("--crt-moduli", type=int, default=None, choices=[7, 8, 9],
+                    help="residue squarings: number of moduli (engine default when omitted)")
+    ap.add_argument("--crt64-moduli", type=int, default=None, choices=[0, 10, 11, 12, 13],
+                    help="FP64-accurate residue products: number of moduli (0 = automatic; engine default "
+                         "when omitted)")
     ap.add_argument("--fp64-slices", type=int, default=None,
                     help="c2: FP64 Pade products from this many exact int8 slices (4..6) on the int8 MFMA; "
                          "0 = FP64 MFMA (default: the engine's, 6)")
@@ -1206,6 +1234,7 @@ def main():
     if args.kblock is not None:
         from fastctqw_amd.fastexpm import _fastexpm_core
         eng._backend.set_option(_fastexpm_core.OPT_KBLOCK, args.kblock)
+    _engine_options(eng, args, cdt)
     if args.fp64_slices is not None and cdt == np.complex64:
         from fastctqw_amd.fastexpm import _fastexpm_core
         eng._backend.set_option(_fastexpm_core.OPT_FP64_SLICES, args.fp64_slices)
@@ -1236,8 +1265,11 @@ def main():
     split = cdt == np.complex64 and used in ("fp16x2", "bf16x3") and st["structured"]
     crt = cdt == np.complex64 and used == "int8crt" and st["structured"]
     if crt:
-        roof = crt_roofline(n, kernel_ms, kernel_launches, float(np.mean([p["ms_square"] for p in phase])), s)
-        kname = ("crt_gemm_kernel (27 batched int8 products per squaring modulo 9 moduli on "
+        K = int(st.get("crt_moduli", 9) or 9)
+        roof = crt_roofline(n, kernel_ms, kernel_launches, float(np.mean([p["ms_square"] for p in phase])), s, K)
+        roof["moduli"] = K
+        roof["beta"] = int(st.get("crt_beta", 0))
+        kname = (f"crt_gemm_kernel ({3 * K} batched int8 products per squaring modulo {K} moduli on "
                  "v_mfma_i32_32x32x32_i8: lower 256x256 tiles, 4 waves of 128x128, direct-to-LDS operand "
                  "copies in a 4-stage ring interleaved with the MFMAs; byte residues out)")
         traffic_key = "int8crt_squaring_gemm"
@@ -1310,7 +1342,7 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": (f"c64 ({_square_phrase(used)}, {_slice_phrase(st)}" if cdt == np.complex64 else "c128 (fp64)"),
+        "dtype": (f"c64 ({_square_phrase(used, st)}, {_slice_phrase(st)}" if cdt == np.complex64 else "c128 (fp64)"),
         "data": "synthetic: seeded ER(N,p) Laplacian, A=-i*t*L",
         "config": {"workload": f"expm(-i*{args.t}*L), L Laplacian of ER({n},{args.p}) seed {args.seed}, "
                                f"{args.dtype}, full U resident", "n": n, "t": args.t,
@@ -1325,7 +1357,7 @@ def main():
                      "sustained_peak": sustained, "sustained_source": MFMA_PROFILE if sustained else None,
                      "issued_frac_of_sustained": issued_tf / sustained if sustained else None,
                      "note": ("achieved = algorithmic int8 operations of one launch of the residue products "
-                              "(27 N^3) / its device time (HIP events around each launch on the squaring "
+                              "(3 K N^3, K moduli) / its device time (HIP events around each launch on the squaring "
                               "stream); issued = the whole lower tiles and padded k; fp32_equivalent = the "
                               "squaring phase as the rate of the FP32 structured algorithm (3 N^3 per "
                               "squaring); traffic = HBM bytes per launch from FETCH_SIZE / WRITE_SIZE scaled by "
@@ -1372,6 +1404,21 @@ def main():
                                                  _issued_structured_squaring(n, kpad, True), FP32_PEAK_TFLOPS)
             fp32["roofline"]["avg_launch_ms"] = fst["ms_square_kernel"] / kl
             out["fp32_squarings"] = fp32
+        if used == "int8crt":
+            # the residue squarings with the other moduli counts (rows rounded
+            # to fewer bits, fewer int8 products), same input and reference
+            from fastctqw_amd.fastexpm import _fastexpm_core
+            K0 = int(st.get("crt_moduli", 9))
+            sweep = {}
+            for K in (7, 8, 9):
+                if K == K0:
+                    continue
+                eng._backend.set_option(_fastexpm_core.OPT_CRT_MODULI, K)
+                v, vs = variant("auto")
+                v["beta"] = vs.get("crt_beta", 0)
+                sweep[str(K)] = v
+            eng._backend.set_option(_fastexpm_core.OPT_CRT_MODULI, K0)
+            out["crt_moduli_sweep"] = sweep
         sp, _ = variant("fp16x2")
         sp["speedup_vs_fp32"] = (out["fp32_squarings"]["ms_per_step"] if "fp32_squarings" in out
                                  else ms_step) / sp["ms_per_step"]

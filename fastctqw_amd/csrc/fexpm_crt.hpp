@@ -22,10 +22,13 @@
 //   4. (Y Y^T)_ij = 2^(e_i + e_j - 2 BETA) (Re + i Im), then the diagonal-split
 //      epilogue of the squaring (FP64 diagonal d) and the FP32 planes.
 //
-// The only rounding is that of step 1 (BETA = 27 bits of each row's
-// largest element) and of the FP32 output; the products and their sums are
-// exact, unlike FP32 MFMA accumulation.  Nine moduli of about 2^7.9 (M of
-// 71.2 bits) leave room for BETA = 27 with k <= 16384.
+// The only rounding is that of step 1 (BETA bits of each row's largest
+// element) and of the FP32 output; the products and their sums are exact,
+// unlike FP32 MFMA accumulation.  K of the nine moduli of about 2^7.9 are
+// used (K = 7, 8 or 9: M of 55.5, 63.4 or 71.2 bits); BETA is the largest
+// value <= 27 with k 2^(2 BETA + 1) < M / 2 (crt_beta on the host): 27 for
+// K = 9 and k <= 16384, 24 for K = 8 and 2729 < k <= 10915, 20 for K = 7 and
+// 3051 < k <= 12203.
 //
 // Layouts.  Operand residues ("fragment layout"): for a k-block kb (32 values
 // of k) and a group g of 32 rows, one 1 KB fragment at ((kb * G + g) * 1024)
@@ -43,21 +46,24 @@
 
 namespace fexpm {
 
-constexpr int CRT_K = 9;         // moduli of the squaring products
-constexpr int CRT_BETA = 27;     // bits kept of each row's largest element
+constexpr int CRT_K = 9;         // most moduli of the squaring products (table capacity)
+constexpr int CRT_KMIN = 7;      // fewest moduli of the squaring products
+constexpr int CRT_BETA = 27;     // most bits kept of each row's largest element
 constexpr int CRT_KMAX = 16384;  // longest k (keeps every bound above)
 constexpr int CRT_NMIN = 768;    // automatic mode: smallest order for which the residue form is faster
 constexpr int CRT_TILE = 256;    // workgroup tile edge of the residue products
 constexpr int CRT_KPAD = 64;     // k granularity (one LDS stage: two k-blocks)
 constexpr int CRT_NPROB = 3 * CRT_K;
 
-// Moduli and reconstruction constants (filled on the host by crt_table()).
+// Moduli and reconstruction constants of K moduli (filled on the host by
+// crt_table(K); the first K entries are used).
 struct CrtTable {
   uint32_t m[CRT_K];
   uint32_t magic[CRT_K];  // ceil(2^39 / m): floor(u / m) = mulhi(u, magic) >> 7 for 0 <= u < 2^31
   uint32_t inv[CRT_K];    // ((M / m)^-1) mod m
   double rcp[CRT_K];      // 1 / m
-  double M;               // product of the moduli (rounded)
+  double M;               // product of the K moduli (rounded)
+  int k;                  // number of moduli K
 };
 
 // u mod m for 0 <= u < 2^31 (exact: the error term of the rounded-up
@@ -108,16 +114,18 @@ __global__ __launch_bounds__(256) void crt_row_exp(const float* __restrict__ yr,
   if (lane == 0) e[r] = ex;
 }
 
-// Operand residues of the structured squaring (3 * CRT_K planes of
-// rows x kpad bytes in the fragment layout: plane o * CRT_K + i holds
-// operand o (0: yr', 1: yi', 2: s') mod moduli i).  One wave per fragment
+// Operand residues of the structured squaring with K moduli (3 K planes of
+// rows x kpad bytes in the fragment layout: plane o * K + i holds operand o
+// (0: yr', 1: yi', 2: s') mod moduli i; rows rounded to beta bits).  One wave per fragment
 // (k-block kb, row group g), four per workgroup; rows >= n and k >= n are
 // zero.  rows is a multiple of CRT_TILE, kpad of CRT_KPAD, kpad <= ld, and
 // the padding columns k in [n, ld) of the planes are zero, so a lane reads
 // its 16 values (k0 + 16 <= kpad <= ld) with four 16-byte loads per plane.
+template <int K>
 __global__ __launch_bounds__(256) void crt_residues_sq(const float* __restrict__ yr, const float* __restrict__ yi,
                                                         int64_t ld, int n, const int* __restrict__ e, int rows,
-                                                        int kpad, CrtTable tab, int8_t* __restrict__ out) {
+                                                        int kpad, int beta, CrtTable tab, int8_t* __restrict__ out) {
+  static_assert(K >= CRT_KMIN && K <= CRT_K, "moduli count");
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int G = rows / 32;
   const int64_t jobs = (int64_t)G * (kpad / 32);
@@ -132,7 +140,7 @@ __global__ __launch_bounds__(256) void crt_residues_sq(const float* __restrict__
     xi[q] = 0;
   }
   if (row < n && k0 < n) {
-    const float sc = ldexpf(1.0f, CRT_BETA - e[row]);  // exact power of two; |y sc| < 2^BETA
+    const float sc = ldexpf(1.0f, beta - e[row]);  // exact power of two; |y sc| < 2^beta <= 2^27
     const float4* pr = reinterpret_cast<const float4*>(yr + (int64_t)row * ld + k0);
     const float4* pi = reinterpret_cast<const float4*>(yi + (int64_t)row * ld + k0);
 #pragma unroll
@@ -151,7 +159,7 @@ __global__ __launch_bounds__(256) void crt_residues_sq(const float* __restrict__
   const int64_t plane = (int64_t)rows * kpad;
   const int64_t off = ((int64_t)kb * G + g) * 1024 + lane * 16;
 #pragma unroll
-  for (int i = 0; i < CRT_K; ++i) {
+  for (int i = 0; i < K; ++i) {
     const uint32_t m = tab.m[i], mg = tab.magic[i];
     uint32_t pk[3][4];
 #pragma unroll
@@ -176,7 +184,7 @@ __global__ __launch_bounds__(256) void crt_residues_sq(const float* __restrict__
       v.y = pk[o][1];
       v.z = pk[o][2];
       v.w = pk[o][3];
-      *reinterpret_cast<uint4*>(out + (int64_t)(o * CRT_K + i) * plane + off) = v;
+      *reinterpret_cast<uint4*>(out + (int64_t)(o * K + i) * plane + off) = v;
     }
   }
 }
@@ -352,19 +360,23 @@ __device__ __forceinline__ uint32_t crt_digit(uint32_t r, uint32_t inv, uint32_t
   return crt_umod(r * inv, m, magic);
 }
 
-// X^2 of the split form from the residues of T1, T2, T3 (planes p, CRT_K + p,
-// 2 CRT_K + p for modulus p; each rows x rows bytes at col * rows + row),
+// X^2 of the split form from the residues of T1, T2, T3 modulo K moduli
+// (planes p, K + p, 2 K + p for modulus p; each rows x rows bytes at
+// col * rows + row; operand rows rounded to beta bits),
 // for the 64 x 64 lower tiles (TI >= TJ, lower-triangular index blockIdx.x):
 //   (Y Y^T)_ij = 2^(e_i + e_j - 2 BETA) (Re + i Im) by CRT, then
 //   i != j: X^2_ij = (d_i + d_j) Y_ij + (Y Y^T)_ij (complex),
 //   i == j: d_out_i = d_i^2 + (Y Y^T)_ii and X^2_ii = 0 in the planes,
 // written at (i, j) and (j, i) of cr / ci (rows and columns < n only).
 // Thread t works on rows 4 (t % 16) .. + 3 of column (t / 16) + 16 c.
+template <int K>
 __global__ __launch_bounds__(256) void crt_combine_sq(const uint8_t* __restrict__ res, int rows,
                                                        const int* __restrict__ e, const float* __restrict__ yr,
                                                        const float* __restrict__ yi, int64_t ld, int n,
                                                        const double2* __restrict__ d_in, double2* __restrict__ d_out,
-                                                       float* __restrict__ cr, float* __restrict__ ci, CrtTable tab) {
+                                                       float* __restrict__ cr, float* __restrict__ ci, int beta,
+                                                       CrtTable tab) {
+  static_assert(K >= CRT_KMIN && K <= CRT_K, "moduli count");
   __shared__ float tre[64][65];
   __shared__ float tim[64][65];
   int TI, TJ;
@@ -380,12 +392,12 @@ __global__ __launch_bounds__(256) void crt_combine_sq(const uint8_t* __restrict_
     const int ib = i0 + il;
     if (j >= n || ib >= n) continue;
     // residues of the 4 rows ib .. ib + 3 in column j, per product and modulus
-    uint32_t w[3][CRT_K];
+    uint32_t w[3][K];
 #pragma unroll
     for (int o = 0; o < 3; ++o)
 #pragma unroll
-      for (int q = 0; q < CRT_K; ++q)
-        w[o][q] = *reinterpret_cast<const uint32_t*>(res + (o * CRT_K + q) * plane + (int64_t)j * rows + ib);
+      for (int q = 0; q < K; ++q)
+        w[o][q] = *reinterpret_cast<const uint32_t*>(res + (o * K + q) * plane + (int64_t)j * rows + ib);
     const double2 dj = d_in[j];
     const int ej = e[j];
 #pragma unroll
@@ -394,7 +406,7 @@ __global__ __launch_bounds__(256) void crt_combine_sq(const uint8_t* __restrict_
       if (i >= n || i < j) continue;  // lower part only (diagonal tiles hold both)
       double yre = 0.0, yim = 0.0;
 #pragma unroll
-      for (int q = 0; q < CRT_K; ++q) {
+      for (int q = 0; q < K; ++q) {
         const uint32_t m = tab.m[q], mg = tab.magic[q];
         const uint32_t t1 = (w[0][q] >> (8 * r)) & 0xFFu;
         const uint32_t t2 = (w[1][q] >> (8 * r)) & 0xFFu;
@@ -404,7 +416,7 @@ __global__ __launch_bounds__(256) void crt_combine_sq(const uint8_t* __restrict_
         yre += (double)crt_digit(re, tab.inv[q], m, mg) * tab.rcp[q];
         yim += (double)crt_digit(im, tab.inv[q], m, mg) * tab.rcp[q];
       }
-      const double sc = ldexp(tab.M, e[i] + ej - 2 * CRT_BETA);
+      const double sc = ldexp(tab.M, e[i] + ej - 2 * beta);
       double vr = (yre - rint(yre)) * sc;
       double vi = (yim - rint(yim)) * sc;
       if (i == j) {
@@ -443,8 +455,9 @@ __global__ __launch_bounds__(256) void crt_combine_sq(const uint8_t* __restrict_
 
 // ---------------------------------------------------------------------------
 // FP64-accurate real products by the same residue arithmetic (the symmetric
-// FP64 products of the Pade stage and of the structured solve).  Thirteen
-// moduli (the nine above and 223, 217, 211, 199; M of 102.2 bits); each row
+// FP64 products of the Pade stage and of the structured solve).  Up to
+// thirteen moduli (the nine above and 223, 217, 211, 199; M of 102.2 bits;
+// the first 10, 11 or 12 of them: 79.1, 86.9 or 94.6 bits); each row
 // r of an operand is scaled by 2^(beta - e_r) and rounded to an integer of at
 // most 2^beta, beta = min(45, floor((log2 M - 1 - log2 k) / 2)) (43 for k
 // from 2^13.2 to 2^14.9), so that |C| <= k 2^(2 beta) < M / 2.  The reconstruction
@@ -454,7 +467,8 @@ __global__ __launch_bounds__(256) void crt_combine_sq(const uint8_t* __restrict_
 // GEMM epilogue (alpha, beta, added matrices, diagonal, mirrored lower tiles).
 // Operand rows may be the concatenation s1 x1[r, 0:k1] | s2 x2[r, 0:nk-k1]
 // (s1, s2 = +-1), as for S = [V U][V U]^T.
-constexpr int CRT64_K = 13;
+constexpr int CRT64_K = 13;         // most moduli (table capacity)
+constexpr int CRT64_KMIN = 10;      // fewest moduli (M of 79.1 bits)
 constexpr int CRT64_KMAX = 20480;   // longest k: k 127^2 < 2^28.3 (epilogue offset above)
 constexpr int CRT64_BETA_MAX = 45;  // integers of at most 2^45 (crt64_sres)
 
@@ -467,6 +481,7 @@ struct CrtTable64 {
   double rcp_lo[CRT64_K];
   double M_hi, M_lo;       // M = M_hi + M_lo
   double log2M;
+  int k;                   // moduli used: the first k of the table
 };
 
 // Concatenated operand row r, value k (k < nk).
@@ -508,7 +523,7 @@ __device__ __forceinline__ int crt64_sres(int h, uint32_t l, uint32_t m, uint32_
   return r > (int)(m >> 1) ? r - (int)m : r;
 }
 
-// Operand residues (CRT64_K planes of rows x kpad bytes, fragment layout) of
+// Operand residues (tab.k planes of rows x kpad bytes, fragment layout) of
 // x' = rint(x 2^(beta - e_r)); rows >= nr and k >= nk are zero.  One wave per
 // fragment (k-block kb, row group g), four per workgroup.  vec: both parts
 // start 16-byte aligned, ld is even and k1 % 16 == 0, so a lane's 16 values
@@ -554,7 +569,7 @@ __global__ __launch_bounds__(256) void crt64_residues(const double* __restrict__
   const int64_t plane = (int64_t)rows * kpad;
   const int64_t off = ((int64_t)kb * G + g) * 1024 + lane * 16;
 #pragma unroll 1
-  for (int i = 0; i < CRT64_K; ++i) {
+  for (int i = 0; i < tab.k; ++i) {
     const uint32_t m = tab.m[i], mg = tab.magic[i], c = tab.c24[i];
     uint4 v;
     uint32_t* pv = &v.x;
@@ -580,7 +595,7 @@ __device__ __forceinline__ void crt_two_sum(double a, double b, double& s, doubl
   err = (a - (s - bb)) + (b - bb);
 }
 
-// C = A B^T from the CRT64_K residue planes (element (row, col) at byte
+// C = A B^T from the tab.k residue planes (element (row, col) at byte
 // col * rows + row of plane q), 64 x 64 lower tiles (index blockIdx.x), FP64
 // epilogue of e (real planes; diag on the diagonal), values written at (i, j)
 // and (j, i) for i, j < m.
@@ -603,7 +618,7 @@ __global__ __launch_bounds__(256) void crt64_combine(const uint8_t* __restrict__
     uint32_t w[CRT64_K];
 #pragma unroll
     for (int q = 0; q < CRT64_K; ++q)
-      w[q] = *reinterpret_cast<const uint32_t*>(res + q * plane + (int64_t)j * rows + ib);
+      w[q] = q < tab.k ? *reinterpret_cast<const uint32_t*>(res + q * plane + (int64_t)j * rows + ib) : 0u;
     const int ej = eb[j];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -613,6 +628,7 @@ __global__ __launch_bounds__(256) void crt64_combine(const uint8_t* __restrict__
 #pragma unroll
       for (int q = 0; q < CRT64_K; ++q) {
 #pragma clang fp contract(off)
+        if (q >= tab.k) break;
         const uint32_t d = crt_umod(((w[q] >> (8 * r)) & 0xFFu) * tab.inv[q], tab.m[q], tab.magic[q]);
         const double sd = (double)d;
         const double p = sd * tab.rcp_hi[q];                                // the rounded product TwoSum adds

@@ -313,69 +313,89 @@ static void dfree(void* p) {
   (void)hipFree(p);
 }
 
-// Moduli and reconstruction constants of the residue squarings: the nine
-// largest pairwise coprime odd moduli below 256, u_m = (M / m)^-1 mod m.
-static const CrtTable& crt_table() {
-  static const CrtTable tab = [] {
-    static const uint32_t mods[CRT_K] = {255, 253, 251, 247, 241, 239, 233, 229, 227};
-    CrtTable t;
-    double M = 1.0;
-    for (int i = 0; i < CRT_K; ++i) M *= (double)mods[i];
-    t.M = M;
-    for (int i = 0; i < CRT_K; ++i) {
-      const uint32_t m = mods[i];
-      t.m[i] = m;
-      t.magic[i] = (uint32_t)(((1ull << 39) + m - 1) / m);
-      t.rcp[i] = 1.0 / (double)m;
-      uint64_t mi = 1;  // (M / m) mod m
-      for (int j = 0; j < CRT_K; ++j)
-        if (j != i) mi = (mi * mods[j]) % m;
-      uint32_t u = 0;  // its inverse mod m (m < 256: by search)
-      for (uint32_t c = 1; c < m; ++c)
-        if ((mi * c) % m == 1) {
-          u = c;
-          break;
-        }
-      t.inv[i] = u;
-    }
-    return t;
-  }();
-  return tab;
+// Moduli and reconstruction constants of the residue squarings: the first K
+// (7, 8 or 9) of the nine largest pairwise coprime odd moduli below 256,
+// u_m = (M / m)^-1 mod m with M the product of those K.
+static CrtTable crt_make_table(int K) {
+  static const uint32_t mods[CRT_K] = {255, 253, 251, 247, 241, 239, 233, 229, 227};
+  CrtTable t;
+  std::memset(&t, 0, sizeof(t));
+  t.k = K;
+  double M = 1.0;
+  for (int i = 0; i < K; ++i) M *= (double)mods[i];
+  t.M = M;
+  for (int i = 0; i < K; ++i) {
+    const uint32_t m = mods[i];
+    t.m[i] = m;
+    t.magic[i] = (uint32_t)(((1ull << 39) + m - 1) / m);
+    t.rcp[i] = 1.0 / (double)m;
+    uint64_t mi = 1;  // (M / m) mod m
+    for (int j = 0; j < K; ++j)
+      if (j != i) mi = (mi * mods[j]) % m;
+    uint32_t u = 0;  // its inverse mod m (m < 256: by search)
+    for (uint32_t c = 1; c < m; ++c)
+      if ((mi * c) % m == 1) {
+        u = c;
+        break;
+      }
+    t.inv[i] = u;
+  }
+  return t;
+}
+static const CrtTable& crt_table(int K) {
+  static const CrtTable tabs[CRT_K - CRT_KMIN + 1] = {crt_make_table(7), crt_make_table(8), crt_make_table(9)};
+  static_assert(CRT_KMIN == 7 && CRT_K == 9, "one table per moduli count");
+  return tabs[K - CRT_KMIN];
+}
+// Bits kept of each row's largest element for products of length k with
+// the moduli of tab: the largest beta <= CRT_BETA with k 2^(2 beta + 1) <
+// M / 2, so that |Re|, |Im| <= k 2^(2 beta + 1) are recovered exactly.  The
+// margin 2^-20 covers the rounding of M (relative 2^-50 at most).
+static int crt_beta(const CrtTable& tab, int64_t k) {
+  int beta = CRT_BETA;
+  while (beta > 0 && std::ldexp((double)k, 2 * beta + 1) >= std::ldexp(tab.M, -1) * (1.0 - 0x1p-20)) --beta;
+  return beta;
 }
 
-// Constants of the FP64 residue products: the nine moduli above and 223,
-// 217, 211, 199 (pairwise coprime), M exact in 128-bit arithmetic.
-static const CrtTable64& crt_table64() {
-  static const CrtTable64 tab = [] {
-    static const uint32_t mods[CRT64_K] = {255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199};
-    CrtTable64 t;
-    unsigned __int128 M = 1;
-    for (int i = 0; i < CRT64_K; ++i) M *= mods[i];
-    t.M_hi = (double)M;
-    const __int128 rest = (__int128)M - (__int128)(unsigned __int128)t.M_hi;
-    t.M_lo = (double)rest;
-    t.log2M = std::log2(t.M_hi);
-    for (int i = 0; i < CRT64_K; ++i) {
-      const uint32_t m = mods[i];
-      t.m[i] = m;
-      t.magic[i] = (uint32_t)(((1ull << 39) + m - 1) / m);
-      t.c24[i] = (uint32_t)((1u << 24) % m);
-      t.rcp_hi[i] = 1.0 / (double)m;
-      t.rcp_lo[i] = std::fma(-t.rcp_hi[i], (double)m, 1.0) / (double)m;
-      uint64_t mi = 1;
-      for (int j = 0; j < CRT64_K; ++j)
-        if (j != i) mi = (mi * mods[j]) % m;
-      uint32_t u = 0;
-      for (uint32_t c = 1; c < m; ++c)
-        if ((mi * c) % m == 1) {
-          u = c;
-          break;
-        }
-      t.inv[i] = u;
-    }
-    return t;
-  }();
-  return tab;
+// Constants of the FP64 residue products with the first K (10 .. 13) of the
+// nine moduli above and 223, 217, 211, 199 (pairwise coprime), M exact in
+// 128-bit arithmetic.
+static CrtTable64 crt_make_table64(int K) {
+  static const uint32_t mods[CRT64_K] = {255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199};
+  CrtTable64 t;
+  std::memset(&t, 0, sizeof(t));
+  t.k = K;
+  unsigned __int128 M = 1;
+  for (int i = 0; i < K; ++i) M *= mods[i];
+  t.M_hi = (double)M;
+  const __int128 rest = (__int128)M - (__int128)(unsigned __int128)t.M_hi;
+  t.M_lo = (double)rest;
+  t.log2M = std::log2(t.M_hi);
+  for (int i = 0; i < K; ++i) {
+    const uint32_t m = mods[i];
+    t.m[i] = m;
+    t.magic[i] = (uint32_t)(((1ull << 39) + m - 1) / m);
+    t.c24[i] = (uint32_t)((1u << 24) % m);
+    t.rcp_hi[i] = 1.0 / (double)m;
+    t.rcp_lo[i] = std::fma(-t.rcp_hi[i], (double)m, 1.0) / (double)m;
+    uint64_t mi = 1;
+    for (int j = 0; j < K; ++j)
+      if (j != i) mi = (mi * mods[j]) % m;
+    uint32_t u = 0;
+    for (uint32_t c = 1; c < m; ++c)
+      if ((mi * c) % m == 1) {
+        u = c;
+        break;
+      }
+    t.inv[i] = u;
+  }
+  return t;
+}
+static const CrtTable64& crt_table64(int K = CRT64_K) {
+  static const CrtTable64 tabs[CRT64_K - CRT64_KMIN + 1] = {crt_make_table64(10), crt_make_table64(11),
+                                                             crt_make_table64(12), crt_make_table64(13)};
+  static_assert(CRT64_KMIN == 10 && CRT64_K == 13, "one table per moduli count");
+  return tabs[K - CRT64_KMIN];
 }
 
 // A device buffer that grows on demand (freeing its previous contents; the
@@ -439,6 +459,21 @@ struct PipelineBase {
   int deflate = 1;         // spectral scaling: deflate an exact eigenvector 1 (equal row sums)
   int cheb = 1;            // spectral plans: exp(iX) from its Chebyshev polynomial (0: Pade + solve)
   int crt_map = 1;         // block map of the residue launches (crt_launch_blocks; 1: 185.7 vs 191.3 ms at C2)
+  // moduli of the residue squarings (7, 8 or 9; FEXPM_OPT_CRT_MODULI).  8
+  // (rows to 24 bits at n = 10^4): 2-4x more accurate than the FP32 MFMA
+  // squarings in every case of the N = 10^4 precision study (ER(N, 0.5)
+  // Laplacians, unweighted and weighted, t = 1, 4.5, 20), 6-8% faster per
+  // expm than 9 moduli
+  int crt_moduli = 8;
+  // moduli of the FP64-accurate residue products: crt64_opt (10 .. 13;
+  // FEXPM_OPT_CRT64_MODULI) for every plan, or with crt64_opt = 0 CRT64_K
+  // for plans that keep the eigenvector 1 and crt64_deflated for plans that
+  // deflate it; crt64_moduli is the count of the current run's FP64 pipeline
+  int crt64_opt = 0;
+  int crt64_deflated = CRT64_K;
+  int crt64_moduli = CRT64_K;
+  int crt64_for(bool deflated) const { return crt64_opt ? crt64_opt : (deflated ? crt64_deflated : CRT64_K); }
+  int last_crt_beta = 0;   // bits per row of the last residue squarings
 };
 
 // ---------------------------------------------------------------------------
@@ -958,7 +993,7 @@ template <typename W> struct Pipeline : PipelineBase {
   // (KeepOperand), in which no other use of the buffer may come between.
   const void* c64_keep = nullptr;
   bool c64_kept = false;
-  int c64_kept_m = 0, c64_kept_k = 0;
+  int c64_kept_m = 0, c64_kept_k = 0, c64_kept_moduli = 0;
   const void* c64_kept_base = nullptr;
   struct KeepOperand {
     Pipeline* p;
@@ -976,11 +1011,12 @@ template <typename W> struct Pipeline : PipelineBase {
   // CRT64_K moduli (fexpm_crt.hpp; B null: B = A).  Returns the FP64-
   // equivalent flops of the product.
   double crt64_product(const OzOp& A, const OzOp* B, int m, int nk, const Epilogue<double>& e, hipStream_t st) {
-    const CrtTable64& tab = crt_table64();
+    const CrtTable64& tab = crt_table64(crt64_moduli);
+    const int K = tab.k;
     const int rows = (int)round_up(m, CRT_TILE), kpad = (int)round_up(nk, CRT_KPAD);
     // |C| <= k 2^(2 beta) < M / 2; at most 45 bits (the range of crt64_sres)
     const int beta = std::min(CRT64_BETA_MAX, (int)std::floor((tab.log2M - 1.0 - std::log2((double)kpad)) * 0.5));
-    const size_t op_bytes = (size_t)CRT64_K * rows * kpad;
+    const size_t op_bytes = (size_t)K * rows * kpad;
     const int nops = B ? 2 : 1;
     for (int b = 0; b < nops; ++b)
       if (!c64_e[b]) c64_e[b] = dalloc<int>(round_up(npcap, CRT_TILE));
@@ -1001,10 +1037,12 @@ template <typename W> struct Pipeline : PipelineBase {
     for (int b = 0; b < nops; ++b) {
       const OzOp& x = b == 0 ? A : *B;
       if (b == 0 && keep) {
-        if (c64_kept && c64_kept_m == m && c64_kept_k == nk && c64_kept_base == base) continue;
+        if (c64_kept && c64_kept_m == m && c64_kept_k == nk && c64_kept_moduli == K && c64_kept_base == base)
+          continue;
         c64_kept = true;
         c64_kept_m = m;
         c64_kept_k = nk;
+        c64_kept_moduli = K;
         c64_kept_base = base;
       }
       crt64_row_exp<<<(m + 3) / 4, 256, 0, st>>>(x.p1, x.s1, x.p2, x.s2, x.k1, x.ldx, m, nk, c64_e[b]);
@@ -1018,7 +1056,7 @@ template <typename W> struct Pipeline : PipelineBase {
     CrtProbs probs;
     std::memset(&probs, 0, sizeof(probs));
     const int64_t op_plane = (int64_t)rows * kpad, res_plane = (int64_t)rows * rows;
-    for (int q = 0; q < CRT64_K; ++q) {
+    for (int q = 0; q < K; ++q) {
       CrtProb& pq = probs.p[q];
       pq.a = c64_op[0] + q * op_plane;
       pq.b = c64_op[nops - 1] + q * op_plane;
@@ -1026,8 +1064,7 @@ template <typename W> struct Pipeline : PipelineBase {
       pq.m = tab.m[q];
       pq.magic = tab.magic[q];
     }
-    crt_gemm_kernel<<<crt_launch_blocks(CRT64_K, tiles_per, crt_map), 256, 0, st>>>(probs, tiles_per, T, G, KB,
-                                                                                    crt_map);
+    crt_gemm_kernel<<<crt_launch_blocks(K, tiles_per, crt_map), 256, 0, st>>>(probs, tiles_per, T, G, KB, crt_map);
     FX_LAUNCH();
     const int T64 = (m + 63) / 64;
     crt64_combine<<<T64 * (T64 + 1) / 2, 256, 0, st>>>(c64_res, rows, c64_e[0], c64_e[nops - 1], beta, m, e, tab);
@@ -1754,18 +1791,22 @@ template <typename W> struct Pipeline : PipelineBase {
                                 ops_bytes + (size_t)CRT_NPROB * rows * rows)));
       int8_t* crt_ops = reinterpret_cast<int8_t*>(base);
       uint8_t* crt_res = base + ops_bytes;
-      const CrtTable& tab = crt_table();
+      const int K = crt_moduli;
+      const CrtTable& tab = crt_table(K);
+      const int beta = crt_beta(tab, n);
+      last_crt_beta = beta;
+      const int nprob = 3 * K;
       const int64_t op_plane = (int64_t)rows * kpad, res_plane = (int64_t)rows * rows;
       CrtProbs probs;
       std::memset(&probs, 0, sizeof(probs));
-      for (int p = 0; p < CRT_NPROB; ++p) {
-        // problem p: operand p / CRT_K (yr', yi', s') modulo moduli p % CRT_K; C = X X^T
+      for (int p = 0; p < nprob; ++p) {
+        // problem p: operand p / K (yr', yi', s') modulo moduli p % K; C = X X^T
         CrtProb& q = probs.p[p];
         q.a = crt_ops + p * op_plane;
         q.b = q.a;
         q.out = crt_res + p * res_plane;
-        q.m = tab.m[p % CRT_K];
-        q.magic = tab.magic[p % CRT_K];
+        q.m = tab.m[p % K];
+        q.magic = tab.magic[p % K];
       }
       const int G = rows / 32, KB = kpad / 32, T = rows / CRT_TILE, tiles_per = T * (T + 1) / 2;
       const int T64 = (n + 63) / 64;
@@ -1775,18 +1816,27 @@ template <typename W> struct Pipeline : PipelineBase {
       for (int q = 0; q < s; ++q) {
         crt_row_exp<<<(n + 3) / 4, 256, 0, st>>>(cur->r, cur->i, ld, n, crt_e);
         FX_LAUNCH();
-        crt_residues_sq<<<(unsigned)((jobs + 3) / 4), 256, 0, st>>>(cur->r, cur->i, ld, n, crt_e, rows, kpad, tab,
-                                                                     crt_ops);
+        const unsigned rblocks = (unsigned)((jobs + 3) / 4), cblocks = (unsigned)(T64 * (T64 + 1) / 2);
+        if (K == 7) crt_residues_sq<7><<<rblocks, 256, 0, st>>>(cur->r, cur->i, ld, n, crt_e, rows, kpad, beta, tab, crt_ops);
+        else if (K == 8) crt_residues_sq<8><<<rblocks, 256, 0, st>>>(cur->r, cur->i, ld, n, crt_e, rows, kpad, beta, tab, crt_ops);
+        else crt_residues_sq<9><<<rblocks, 256, 0, st>>>(cur->r, cur->i, ld, n, crt_e, rows, kpad, beta, tab, crt_ops);
         FX_LAUNCH();
         kernel_mark(q, false, st);
-        crt_gemm_kernel<<<crt_launch_blocks(CRT_NPROB, tiles_per, crt_map), 256, 0, st>>>(probs, tiles_per, T, G,
-                                                                                         KB, crt_map);
+        crt_gemm_kernel<<<crt_launch_blocks(nprob, tiles_per, crt_map), 256, 0, st>>>(probs, tiles_per, T, G, KB,
+                                                                                     crt_map);
         FX_LAUNCH();
         kernel_mark(q, true, st);
-        crt_combine_sq<<<T64 * (T64 + 1) / 2, 256, 0, st>>>(crt_res, rows, crt_e, cur->r, cur->i, ld, n, dsq[dc],
-                                                             dsq[dc ^ 1], nxt->r, nxt->i, tab);
+        if (K == 7)
+          crt_combine_sq<7><<<cblocks, 256, 0, st>>>(crt_res, rows, crt_e, cur->r, cur->i, ld, n, dsq[dc], dsq[dc ^ 1],
+                                                      nxt->r, nxt->i, beta, tab);
+        else if (K == 8)
+          crt_combine_sq<8><<<cblocks, 256, 0, st>>>(crt_res, rows, crt_e, cur->r, cur->i, ld, n, dsq[dc], dsq[dc ^ 1],
+                                                      nxt->r, nxt->i, beta, tab);
+        else
+          crt_combine_sq<9><<<cblocks, 256, 0, st>>>(crt_res, rows, crt_e, cur->r, cur->i, ld, n, dsq[dc], dsq[dc ^ 1],
+                                                      nxt->r, nxt->i, beta, tab);
         FX_LAUNCH();
-        flops += 2.0 * CRT_NPROB * (double)tiles_per * CRT_TILE * CRT_TILE * (double)kpad;  // int8 operations
+        flops += 2.0 * nprob * (double)tiles_per * CRT_TILE * CRT_TILE * (double)kpad;  // int8 operations
         std::swap(cur, nxt);
         dc ^= 1;
       }
@@ -2044,6 +2094,7 @@ template <typename W> struct Pipeline : PipelineBase {
     h->oz_slices = oz_slices;
     h->oz_scope = slice_scope(P);
     h->crt_map = crt_map;
+    h->crt64_moduli = crt64_for(P.deflate);
     P.rho = h->template spectral_front<Tin>(d_a, P.mu_im, P.alpha, cheb != 0, st);
     P.s = scaling_for(P.rho);
     finish_plan(P);
@@ -2079,6 +2130,7 @@ template <typename W> struct Pipeline : PipelineBase {
       h->ch_outer = ch_outer;
       h->z_left = z_left;
       h->crt_map = crt_map;
+      h->crt64_moduli = crt64_for(P.deflate);
       int dh;
       if (P.cheb) {
         // Chebyshev polynomial from the powers formed by spectral_plan (or
@@ -2196,6 +2248,9 @@ template <typename W> struct Pipeline : PipelineBase {
     S.s_onenorm = P.nonfinite ? 0 : P.s_onenorm;
     S.rho_bound = P.spectral ? P.rho : 0.0;
     S.s_two_product = P.nonfinite ? 0 : P.two;
+    const bool crt = !P.nonfinite && S.precision == FEXPM_PREC_INT8_CRT;
+    S.crt_moduli = crt ? crt_moduli : 0;
+    S.crt_beta = crt ? last_crt_beta : 0;
     if (P.cheb && !P.nonfinite) {
       const ChebSeries cs = cheb_series(cheb_theta(P));
       S.chebyshev = std::max(cs.dc, cs.ds);
@@ -3191,6 +3246,20 @@ int fexpm_set_option(fexpm_engine* e, int key, int64_t value) {
         return FEXPM_EINVAL;
       }
       if (e->pipe) e->pipe->crt_map = (int)value;
+      return FEXPM_OK;
+    case FEXPM_OPT_CRT64_MODULI:
+      if (value != 0 && (value < CRT64_KMIN || value > CRT64_K)) {
+        e->err = "FP64 residue product moduli must be 0 (automatic) or in [10, 13]";
+        return FEXPM_EINVAL;
+      }
+      if (e->pipe) e->pipe->crt64_opt = (int)value;
+      return FEXPM_OK;
+    case FEXPM_OPT_CRT_MODULI:
+      if (value < CRT_KMIN || value > CRT_K) {
+        e->err = "residue squaring moduli must be 7, 8 or 9";
+        return FEXPM_EINVAL;
+      }
+      if (e->pipe) e->pipe->crt_moduli = (int)value;
       return FEXPM_OK;
     case FEXPM_OPT_SQUARE_TILE:
       if (value != 0 && value != 64 && value != 128) {

@@ -81,6 +81,8 @@ static py::dict stats_dict(const fexpm_stats& s) {
   d["device_bytes"] = s.device_bytes;
   d["ms_square_kernel"] = s.ms_square_kernel;
   d["square_kernel_launches"] = s.square_kernel_launches;
+  d["crt_moduli"] = s.crt_moduli;
+  d["crt_beta"] = s.crt_beta;
   d["ms_propagate"] = s.ms_propagate;
   return d;
 }
@@ -531,6 +533,8 @@ PYBIND11_MODULE(_fastexpm_core, m) {
   m.attr("OPT_DEFLATE") = (int)FEXPM_OPT_DEFLATE;
   m.attr("OPT_TWO_PRODUCT") = (int)FEXPM_OPT_TWO_PRODUCT;
   m.attr("OPT_CHEBYSHEV") = (int)FEXPM_OPT_CHEBYSHEV;
+  m.attr("OPT_CRT_MODULI") = (int)FEXPM_OPT_CRT_MODULI;
+  m.attr("OPT_CRT64_MODULI") = (int)FEXPM_OPT_CRT64_MODULI;
   m.attr("PREC_NATIVE") = (int)FEXPM_PREC_NATIVE;
   m.attr("PREC_FP16X2") = (int)FEXPM_PREC_FP16X2;
   m.attr("PREC_BF16X3") = (int)FEXPM_PREC_BF16X3;

@@ -56,9 +56,10 @@ enum fexpm_precision {
   FEXPM_PREC_BF16X3 = 2, /* complex64 engines, structured squarings: three bf16 parts,
                             6 products on 16-bit MFMA, FP32 accumulation */
   FEXPM_PREC_INT8_CRT = 3, /* complex64 engines, structured squarings of order n <= 16384
-                              (larger orders use FP32 MFMA): each row rounded to 27 bits of its
-                              largest element, exact int8 products modulo 9 coprime moduli,
-                              Chinese-remainder reconstruction (no accumulation rounding) */
+                              (larger orders use FP32 MFMA): each row rounded to beta bits of its
+                              largest element, exact int8 products modulo K coprime moduli
+                              (FEXPM_OPT_CRT_MODULI), Chinese-remainder reconstruction (no
+                              accumulation rounding) */
   FEXPM_PREC_AUTO = 4      /* default of complex64 engines: FEXPM_PREC_INT8_CRT for orders
                               768 <= n <= 16384 (where it is the faster form, and more accurate
                               than FP32 MFMA), FEXPM_PREC_NATIVE otherwise; fexpm_stats.precision
@@ -104,6 +105,21 @@ enum fexpm_option {
                                     (double-double sums), deflate the exact eigenvector 1 first:
                                     B = G - mu I + (mu - c)/n J and exp(A) = e^{i mu} (exp(iB) +
                                     (e^{i(c - mu)} - 1) J / n) (default); 0 = no deflation */
+  FEXPM_OPT_CRT_MODULI = 16,     /* FEXPM_PREC_INT8_CRT squarings: number K of moduli, 7, 8 or 9
+                                    (3 K int8 products per squaring); each row keeps beta bits of its
+                                    largest element, the largest beta <= 27 with
+                                    n 2^(2 beta + 1) < M / 2 (M the product of the moduli):
+                                    K = 9: beta = 27 (n <= 16384); K = 8: beta = 24 for
+                                    2729 < n <= 10915; K = 7: beta = 20 for 3051 < n <= 12203.
+                                    Default 8: at N = 10^4 (ER(N, 0.5) Laplacians, t = 1 .. 20) 2-4x
+                                    more accurate than FP32 MFMA squarings; 9 is 5-13x more accurate
+                                    again at 6-8% more time per expm, 7 is 7-8% faster at 50-100x the
+                                    infidelity of 8.  fexpm_stats reports K and beta */
+  FEXPM_OPT_CRT64_MODULI = 17,   /* FP64-accurate residue products (FEXPM_OPT_FP64_SLICES = 13): number
+                                    of moduli, 10 .. 13 (each row keeps beta = min(45,
+                                    floor((log2 M - 1 - log2 k) / 2)) bits), or 0 = automatic
+                                    (default): 13 for plans that keep the eigenvector 1, and the
+                                    deflated-plan count for plans that deflate it */
   FEXPM_OPT_CHEBYSHEV = 15       /* structured input with an FP64 stage (complex128, and complex64 with
                                     FEXPM_OPT_PADE_FP64): 1 = exp(iX), X = B / 2^s, from the Chebyshev
                                     (Jacobi-Anger) expansion of e^{ix} on [-theta, theta], theta =
@@ -150,7 +166,7 @@ typedef struct fexpm_stats {
                            and its solve (FEXPM_OPT_CHEBYSHEV); 0 = Pade (m is then its degree) */
   double rho_bound;     /* spectral-radius bound of the shifted (deflated) B, when spectral = 1 */
   int32_t s_two_product;  /* leading squarings done in the two-product form (FEXPM_OPT_TWO_PRODUCT) */
-  int32_t reserved1;
+  int32_t crt_moduli;   /* FEXPM_PREC_INT8_CRT squarings: number of moduli (FEXPM_OPT_CRT_MODULI), else 0 */
   int64_t device_bytes;  /* device memory the engine holds now (work planes, slices, I/O buffers, the
                             walk Hamiltonian); the FP64 planes of a complex64 engine are allocated
                             at its first run */
@@ -158,7 +174,7 @@ typedef struct fexpm_stats {
                                around them on the squaring stream): the batched int8 residue
                                products of FEXPM_PREC_INT8_CRT squarings, else the squaring GEMMs */
   int32_t square_kernel_launches;  /* number of those timed launches (one per squaring) */
-  int32_t reserved2;
+  int32_t crt_beta;     /* FEXPM_PREC_INT8_CRT squarings: bits kept of each row's largest element, else 0 */
 } fexpm_stats;
 
 /* Create an engine for n x n matrices of the given dtype and allocate its

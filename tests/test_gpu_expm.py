@@ -424,8 +424,8 @@ def test_split_precision_ctqw(fx, prec, n):
 @pytest.mark.parametrize("n", [2, 17, 100, 255, 256, 257, 700, 1500, 3000])
 def test_int8crt_squarings(fx, n):
     """Structured squarings from exact int8 residue products (each row of Y
-    rounded to 27 bits of its largest element, products modulo 9 coprime
-    moduli, Chinese-remainder reconstruction): the same m and s as the FP32
+    rounded to beta bits of its largest element, products modulo the default
+    8 coprime moduli, Chinese-remainder reconstruction): the same m and s as the FP32
     squarings, U against the eigen oracle within the complex64 tolerance and
     no less accurate than the FP32 squarings (relF at most 1.5x theirs), and
     the infidelity of a random state at most 1e-9.  Orders around the 256-row
@@ -453,6 +453,67 @@ def test_int8crt_squarings(fx, n):
     psi0 = rng.standard_normal(n) + 1j * rng.standard_normal(n)
     psi0 /= np.linalg.norm(psi0)
     assert infidelity(ref @ psi0, u.astype(np.complex128) @ psi0) < 1e-9
+
+
+def _crt_beta(k_moduli, n):
+    """Bits per row of the residue squarings: the largest beta <= 27 with
+    n 2^(2 beta + 1) < M / 2, M the product of the first k_moduli of the
+    nine moduli (exact integer arithmetic)."""
+    mods = (255, 253, 251, 247, 241, 239, 233, 229, 227)
+    M = 1
+    for m in mods[:k_moduli]:
+        M *= m
+    beta = 27
+    while beta > 0 and 2 * n * 2 ** (2 * beta + 1) >= M:
+        beta -= 1
+    return beta
+
+
+@pytest.mark.parametrize("n,weighted,t", [(1500, True, 6.0), (3000, True, 6.0), (3000, False, 20.0)])
+def test_crt_moduli(fx, n, weighted, t):
+    """Residue squarings with 7, 8 (default) and 9 moduli (option
+    OPT_CRT_MODULI): the run reports K and the beta that keeps every product
+    below M / 2; U against the eigen oracle with the accuracy each count was
+    measured at (ratios to the FP32 MFMA squarings' relF: 9 moduli 0.14-0.21,
+    8 moduli 0.32-0.53, 7 moduli 3.5-3.8; infidelity of a random state at most
+    7e-13, 8.2e-12 and 4.4e-10 at these orders; bounds about 10x those), the
+    error growing as moduli are removed, and counts outside [7, 9] refused."""
+    core = fx._fastexpm_core
+    L = laplacian(erdos_renyi(n, 0.5, seed=700 + n, weighted=weighted))
+    a = np.ascontiguousarray((-1j * t * L).astype(np.complex64))
+    lam, V = np.linalg.eigh(np.ascontiguousarray(a.imag, dtype=np.float64))
+    ref = (V * np.exp(1j * lam)[None, :]) @ V.T
+    rng = np.random.default_rng(n)
+    psi0 = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    psi0 /= np.linalg.norm(psi0)
+    _say(f"crt moduli n={n} weighted={weighted} t={t}")
+    errs, infs = {}, {}
+    with fx.ExpMatComplex64(n) as e:
+        e.set_precision("native")
+        u_native = e.run(a)
+        st_native = e.stats()
+        assert st_native["crt_moduli"] == 0 and st_native["crt_beta"] == 0
+        e.set_precision("auto")
+        u = e.run(a)
+        assert e.stats()["crt_moduli"] == 8  # the default
+        for bad in (6, 10):
+            with pytest.raises(ValueError):
+                e._backend.set_option(core.OPT_CRT_MODULI, bad)
+        for K in (9, 8, 7):
+            e._backend.set_option(core.OPT_CRT_MODULI, K)
+            u = e.run(a)
+            st = e.stats()
+            assert st["precision"] == "int8crt" and (st["m"], st["s"]) == (st_native["m"], st_native["s"])
+            assert st["crt_moduli"] == K and st["crt_beta"] == _crt_beta(K, n)
+            errs[K] = rel_fro(ref, u)
+            infs[K] = infidelity(ref @ psi0, u.astype(np.complex128) @ psi0)
+    err_native = rel_fro(ref, u_native)
+    print(f"  native relF={err_native:.3e} " +
+          " ".join(f"K={K}: relF={errs[K]:.3e} inf={infs[K]:.3e}" for K in (9, 8, 7)), flush=True)
+    assert errs[9] < errs[8] < errs[7]
+    assert errs[9] <= 0.5 * err_native and infs[9] < 1e-11
+    assert errs[8] <= err_native and infs[8] < 1e-10
+    assert errs[7] <= 8.0 * err_native and infs[7] < 5e-9
 
 
 @pytest.mark.parametrize("n", [1500])
