@@ -537,15 +537,18 @@ def run_c3(args):
     e0 = np.zeros(n, dtype=np.complex128)
     e0[0] = 1.0
     eng = ExpMatComplex64(n)
+    _engine_options(eng, args)
+    c3_cases = tuple(args.c3_cases.split(",")) if args.c3_cases else C3_CASES
+    c3_graphs = (False, True) if args.c3_graphs == "both" else ((args.c3_graphs == "weighted"),)
     cases = []
-    for weighted in (False, True):
+    for weighted in c3_graphs:
         L = _c3_graph(n, weighted)
         for t in C3_TIMES:
             a = (-1j * t * L).astype(np.complex64)
             eng.upload(a)
             del a
             key = f"{int(weighted)}_{t}"
-            for case_name in C3_CASES:
+            for case_name in c3_cases:
                 prec, _, kmod = case_name.partition(":")
                 eng.set_precision(prec)
                 if kmod:
@@ -573,8 +576,8 @@ def run_c3(args):
                 print(f"[bench] {case}", file=sys.stderr, flush=True)
         del L
     eng.free()
-    head = [c for c in cases if c["graph"] == "unweighted" and c["t"] == 4.5]
-    native = next(c for c in head if c["precision"] == "native")
+    head = [c for c in cases if c["graph"] == "unweighted" and c["t"] == 4.5] or cases
+    native = next((c for c in head if c["precision"] == "native"), None)
     best = min(head, key=lambda c: c["ms"])
     res = {
         "metric": "s per N=10k complex expm, squaring precision study (fastest mode at t=4.5, unweighted)",
@@ -584,7 +587,8 @@ def run_c3(args):
         "data": "synthetic: seeded ER(10000,0.5) Laplacian, unweighted and weighted U(0,1]",
         "config": {"workload": "C3: expm(-i t L), t in {1, 4.5, 20}, precision native/int8crt (7, 8, 9 moduli)/"
                                "fp16x2/bf16x3",
-                   "n": n, "native_ms_t4.5": native["ms"], "speedup_vs_native": native["ms"] / best["ms"]},
+                   "n": n, "native_ms_t4.5": native["ms"] if native else None,
+                   "speedup_vs_native": native["ms"] / best["ms"] if native else None},
         "cases": cases,
     }
     if ref_error:
@@ -1035,13 +1039,26 @@ def _host_info(cores):
             "scipy": scipy.__version__, "blas": blas}
 
 
+def _crt64_beta(moduli, k):
+    """Bits per operand row of the FP64-accurate residue products with the
+    first `moduli` of the thirteen moduli at product length k (padded to 64):
+    min(45, floor((log2 M - 1 - log2 kpad) / 2))."""
+    import math
+    mods = (255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199)
+    kpad = (k + 63) // 64 * 64
+    return min(45, int(math.floor((math.log2(math.prod(mods[:moduli])) - 1.0 - math.log2(kpad)) * 0.5)))
+
+
 def _slice_phrase(st):
     """How the FP64 stages of a complex64 run computed their symmetric products."""
     k, scope = st.get("fp64_slices", 0), st.get("fp64_slices_scope", 0)
     deg = st.get("chebyshev", 0)
+    if k == 13:
+        km = int(st.get("fp64_moduli", 0) or 13)
+        residue = (f"exact int8 residue products ({km} moduli, rows rounded to {_crt64_beta(km, int(st['n']))} "
+                   "bits, chinese-remainder reconstruction)")
     if deg:
-        how = ("exact int8 residue products (13 moduli, rows rounded to 43 bits, chinese-remainder reconstruction)"
-               if k == 13 else (f"{k} exact int8 slices" if k else "fp64 mfma"))
+        how = residue if k == 13 else (f"{k} exact int8 slices" if k else "fp64 mfma")
         return (f"degree-{deg} chebyshev polynomial of exp(iX) in place of pade and solve, fp64-accurate "
                 f"products from {how})")
     if not k:
@@ -1049,7 +1066,7 @@ def _slice_phrase(st):
     where = {1: "pade products", 2: "solve products", 3: "pade and solve products"}.get(scope, "products")
     rest = {1: ", fp64 cholesky solve", 2: ", fp64 pade", 3: ", fp64 cholesky factorisation"}.get(scope, "")
     if k == 13:
-        how = "exact int8 residue products (13 moduli, rows rounded to 43 bits, chinese-remainder reconstruction)"
+        how = residue
     else:
         how = f"{k} exact int8 slices (int32 accumulation)"
     return f"fp64-accurate {where} from {how}{rest})"
@@ -1124,9 +1141,11 @@ def main():
                     help="general: skip the reference's performance inputs (16..1024) beside scipy")
     ap.add_argument("--crt-moduli", type=int, default=None, choices=[7, 8, 9],
                     help="residue squarings: number of moduli (engine default when omitted)")
-    ap.add_argument("--crt64-moduli", type=int, default=None, choices=[0, 10, 11, 12, 13],
+    ap.add_argument("--crt64-moduli", type=int, default=None, choices=[0, 8, 9, 10, 11, 12, 13],
                     help="FP64-accurate residue products: number of moduli (0 = automatic; engine default "
                          "when omitted)")
+    ap.add_argument("--c3-cases", default=None, help="c3: comma-separated cases (default: all of C3_CASES)")
+    ap.add_argument("--c3-graphs", choices=["both", "unweighted", "weighted"], default="both")
     ap.add_argument("--fp64-slices", type=int, default=None,
                     help="c2: FP64 Pade products from this many exact int8 slices (4..6) on the int8 MFMA; "
                          "0 = FP64 MFMA (default: the engine's, 6)")
@@ -1296,7 +1315,8 @@ def main():
         roof = squaring_roofline(n, sq_time, s, st.get("s_two_product", 0), issued, peak,
                                  split_parts=(3 if used == "fp16x2" else 6) if split else 0)
         issued_tf = roof["issued_tflops"]
-    traffic = _profiled_traffic(traffic_key) if n == 10000 else None
+    # the profiled counters are those of the default residue squaring (8 moduli)
+    traffic = _profiled_traffic(traffic_key) if n == 10000 and (not crt or K == 8) else None
     sustained = (_sustained_peak("int8" if crt else "fp16" if split and used == "fp16x2" else
                                  "bf16" if split else "fp32") if cdt == np.complex64 else None)
 

@@ -457,7 +457,7 @@ __global__ __launch_bounds__(256) void crt_combine_sq(const uint8_t* __restrict_
 // FP64-accurate real products by the same residue arithmetic (the symmetric
 // FP64 products of the Pade stage and of the structured solve).  Up to
 // thirteen moduli (the nine above and 223, 217, 211, 199; M of 102.2 bits;
-// the first 10, 11 or 12 of them: 79.1, 86.9 or 94.6 bits); each row
+// the first 8 .. 12 of them: 63.4, 71.2, 79.1, 86.9 or 94.6 bits); each row
 // r of an operand is scaled by 2^(beta - e_r) and rounded to an integer of at
 // most 2^beta, beta = min(45, floor((log2 M - 1 - log2 k) / 2)) (43 for k
 // from 2^13.2 to 2^14.9), so that |C| <= k 2^(2 beta) < M / 2.  The reconstruction
@@ -468,7 +468,7 @@ __global__ __launch_bounds__(256) void crt_combine_sq(const uint8_t* __restrict_
 // Operand rows may be the concatenation s1 x1[r, 0:k1] | s2 x2[r, 0:nk-k1]
 // (s1, s2 = +-1), as for S = [V U][V U]^T.
 constexpr int CRT64_K = 13;         // most moduli (table capacity)
-constexpr int CRT64_KMIN = 10;      // fewest moduli (M of 79.1 bits)
+constexpr int CRT64_KMIN = 8;       // fewest moduli (M of 63.4 bits)
 constexpr int CRT64_KMAX = 20480;   // longest k: k 127^2 < 2^28.3 (epilogue offset above)
 constexpr int CRT64_BETA_MAX = 45;  // integers of at most 2^45 (crt64_sres)
 

@@ -357,7 +357,7 @@ static int crt_beta(const CrtTable& tab, int64_t k) {
   return beta;
 }
 
-// Constants of the FP64 residue products with the first K (10 .. 13) of the
+// Constants of the FP64 residue products with the first K (8 .. 13) of the
 // nine moduli above and 223, 217, 211, 199 (pairwise coprime), M exact in
 // 128-bit arithmetic.
 static CrtTable64 crt_make_table64(int K) {
@@ -392,9 +392,10 @@ static CrtTable64 crt_make_table64(int K) {
   return t;
 }
 static const CrtTable64& crt_table64(int K = CRT64_K) {
-  static const CrtTable64 tabs[CRT64_K - CRT64_KMIN + 1] = {crt_make_table64(10), crt_make_table64(11),
+  static const CrtTable64 tabs[CRT64_K - CRT64_KMIN + 1] = {crt_make_table64(8),  crt_make_table64(9),
+                                                             crt_make_table64(10), crt_make_table64(11),
                                                              crt_make_table64(12), crt_make_table64(13)};
-  static_assert(CRT64_KMIN == 10 && CRT64_K == 13, "one table per moduli count");
+  static_assert(CRT64_KMIN == 8 && CRT64_K == 13, "one table per moduli count");
   return tabs[K - CRT64_KMIN];
 }
 
@@ -465,14 +466,17 @@ struct PipelineBase {
   // Laplacians, unweighted and weighted, t = 1, 4.5, 20), 6-8% faster per
   // expm than 9 moduli
   int crt_moduli = 8;
-  // moduli of the FP64-accurate residue products: crt64_opt (10 .. 13;
-  // FEXPM_OPT_CRT64_MODULI) for every plan, or with crt64_opt = 0 CRT64_K
-  // for plans that keep the eigenvector 1 and crt64_deflated for plans that
-  // deflate it; crt64_moduli is the count of the current run's FP64 pipeline
+  // moduli of the FP64-accurate residue products: crt64_opt (8 .. 13;
+  // FEXPM_OPT_CRT64_MODULI) for every plan, or with crt64_opt = 0 the
+  // automatic count: 11 (rows to 36 bits at k = 10^4) for spectral plans,
+  // 13 for 1-norm plans.  At N = 10^4 (spectral plans: C2, the six
+  // precision-study cases, the 32-time propagation of a weighted ER(8192,
+  // 0.3)) 13, 12, 11 and 10 moduli gave the same infidelity to within 2.3x
+  // (10 moduli, unweighted t = 20) and 9 up to 30x more, with 8-moduli
+  // squarings.  crt64_moduli is the count of the current run's FP64 pipeline
   int crt64_opt = 0;
-  int crt64_deflated = CRT64_K;
   int crt64_moduli = CRT64_K;
-  int crt64_for(bool deflated) const { return crt64_opt ? crt64_opt : (deflated ? crt64_deflated : CRT64_K); }
+  int crt64_for(bool spectral) const { return crt64_opt ? crt64_opt : (spectral ? 11 : CRT64_K); }
   int last_crt_beta = 0;   // bits per row of the last residue squarings
 };
 
@@ -2094,7 +2098,7 @@ template <typename W> struct Pipeline : PipelineBase {
     h->oz_slices = oz_slices;
     h->oz_scope = slice_scope(P);
     h->crt_map = crt_map;
-    h->crt64_moduli = crt64_for(P.deflate);
+    h->crt64_moduli = crt64_for(P.spectral);
     P.rho = h->template spectral_front<Tin>(d_a, P.mu_im, P.alpha, cheb != 0, st);
     P.s = scaling_for(P.rho);
     finish_plan(P);
@@ -2130,7 +2134,7 @@ template <typename W> struct Pipeline : PipelineBase {
       h->ch_outer = ch_outer;
       h->z_left = z_left;
       h->crt_map = crt_map;
-      h->crt64_moduli = crt64_for(P.deflate);
+      h->crt64_moduli = crt64_for(P.spectral);
       int dh;
       if (P.cheb) {
         // Chebyshev polynomial from the powers formed by spectral_plan (or
@@ -2251,6 +2255,8 @@ template <typename W> struct Pipeline : PipelineBase {
     const bool crt = !P.nonfinite && S.precision == FEXPM_PREC_INT8_CRT;
     S.crt_moduli = crt ? crt_moduli : 0;
     S.crt_beta = crt ? last_crt_beta : 0;
+    S.fp64_moduli = S.fp64_slices == CRT64_K ? crt64_for(P.spectral) : 0;
+    S.reserved3 = 0;
     if (P.cheb && !P.nonfinite) {
       const ChebSeries cs = cheb_series(cheb_theta(P));
       S.chebyshev = std::max(cs.dc, cs.ds);
@@ -3249,7 +3255,7 @@ int fexpm_set_option(fexpm_engine* e, int key, int64_t value) {
       return FEXPM_OK;
     case FEXPM_OPT_CRT64_MODULI:
       if (value != 0 && (value < CRT64_KMIN || value > CRT64_K)) {
-        e->err = "FP64 residue product moduli must be 0 (automatic) or in [10, 13]";
+        e->err = "FP64 residue product moduli must be 0 (automatic) or in [8, 13]";
         return FEXPM_EINVAL;
       }
       if (e->pipe) e->pipe->crt64_opt = (int)value;

@@ -83,6 +83,7 @@ static py::dict stats_dict(const fexpm_stats& s) {
   d["square_kernel_launches"] = s.square_kernel_launches;
   d["crt_moduli"] = s.crt_moduli;
   d["crt_beta"] = s.crt_beta;
+  d["fp64_moduli"] = s.fp64_moduli;
   d["ms_propagate"] = s.ms_propagate;
   return d;
 }

@@ -516,6 +516,51 @@ def test_crt_moduli(fx, n, weighted, t):
     assert errs[7] <= 8.0 * err_native and infs[7] < 5e-9
 
 
+@pytest.mark.parametrize("weighted", [False, True])
+def test_crt64_moduli(fx, weighted):
+    """Moduli of the FP64-accurate residue products of the Chebyshev stage
+    (option OPT_CRT64_MODULI): automatic 11 for spectral plans and 13 for
+    1-norm plans (reported as fp64_moduli), explicit counts 8..13 accepted
+    and others refused; with 13, 11 and 10 moduli U agrees with the eigen
+    oracle to the same accuracy (relF within 1.2x of the 13-moduli run's,
+    infidelity of a random state below 1e-10), with 8 moduli (25 bits per
+    row at this order) less so."""
+    core = fx._fastexpm_core
+    n, t = 2000, 4.5
+    L = laplacian(erdos_renyi(n, 0.5, seed=31, weighted=weighted))
+    a = np.ascontiguousarray((-1j * t * L).astype(np.complex64))
+    lam, V = np.linalg.eigh(np.ascontiguousarray(a.imag, dtype=np.float64))
+    ref = (V * np.exp(1j * lam)[None, :]) @ V.T
+    rng = np.random.default_rng(5)
+    psi0 = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    psi0 /= np.linalg.norm(psi0)
+    _say(f"crt64 moduli n={n} weighted={weighted}")
+    errs, infs = {}, {}
+    with fx.ExpMatComplex64(n) as e:
+        u = e.run(a)
+        st = e.stats()
+        assert st["spectral"] and st["chebyshev"] and st["fp64_slices"] == 13 and st["fp64_moduli"] == 11
+        for bad in (7, 14):
+            with pytest.raises(ValueError):
+                e._backend.set_option(core.OPT_CRT64_MODULI, bad)
+        for K in (13, 11, 10, 8):
+            e._backend.set_option(core.OPT_CRT64_MODULI, K)
+            u = e.run(a)
+            assert e.stats()["fp64_moduli"] == K
+            errs[K] = rel_fro(ref, u)
+            infs[K] = infidelity(ref @ psi0, u.astype(np.complex128) @ psi0)
+        e._backend.set_option(core.OPT_CRT64_MODULI, 0)
+        e._backend.set_option(core.OPT_SPECTRAL, 0)
+        e.run(a)
+        st1 = e.stats()
+        assert not st1["spectral"] and st1["fp64_moduli"] == 13
+    print("  " + " ".join(f"K={K}: relF={errs[K]:.3e} inf={infs[K]:.3e}" for K in (13, 11, 10, 8)), flush=True)
+    for K in (11, 10):
+        assert errs[K] <= 1.2 * errs[13] and infs[K] < 1e-10
+    assert infs[13] < 1e-10
+    assert errs[8] > errs[13]
+
+
 @pytest.mark.parametrize("n", [1500])
 def test_crt_block_maps_agree(fx, n):
     """Both block maps of the residue launches (testing option OPT_CRT_MAP)
