@@ -1789,17 +1789,18 @@ template <typename W> struct Pipeline : PipelineBase {
       const int rows = (int)round_up(n, CRT_TILE), kpad = (int)round_up(n, CRT_KPAD);
       const int64_t rows_cap = round_up(ncap, CRT_TILE), kpad_cap = round_up(ncap, CRT_KPAD);
       if (!crt_e) crt_e = dalloc<int>((size_t)npcap);
-      const size_t ops_bytes = (size_t)CRT_NPROB * rows * kpad;
-      uint8_t* base = static_cast<uint8_t*>(
-          crt_buf->get(std::max((size_t)CRT_NPROB * rows_cap * (kpad_cap + rows_cap),
-                                ops_bytes + (size_t)CRT_NPROB * rows * rows)));
+      const int K = crt_moduli;
+      const int nprob = 3 * K;
+      // operand residues then product residues of the 3 K products, the
+      // buffer sized once for the engine's largest order
+      const size_t ops_bytes = (size_t)nprob * rows * kpad;
+      uint8_t* base = static_cast<uint8_t*>(crt_buf->get(
+          std::max((size_t)nprob * rows_cap * (kpad_cap + rows_cap), ops_bytes + (size_t)nprob * rows * rows)));
       int8_t* crt_ops = reinterpret_cast<int8_t*>(base);
       uint8_t* crt_res = base + ops_bytes;
-      const int K = crt_moduli;
       const CrtTable& tab = crt_table(K);
       const int beta = crt_beta(tab, n);
       last_crt_beta = beta;
-      const int nprob = 3 * K;
       const int64_t op_plane = (int64_t)rows * kpad, res_plane = (int64_t)rows * rows;
       CrtProbs probs;
       std::memset(&probs, 0, sizeof(probs));
@@ -3520,7 +3521,7 @@ int fexpm_batch_set_option(fexpm_batch* b, int key, int64_t value) {
       return FEXPM_OK;
     case FEXPM_OPT_FP64_SLICES:
       if (value != 0 && value != GROUP_OZ_SLICES) {
-        b->err = "batches: int8 slices must be 0 or 6";
+        b->err = "batches: int8 slices must be 0 or 5";
         return FEXPM_EINVAL;
       }
       if (value != 0 && b->dtype != FEXPM_COMPLEX64) {

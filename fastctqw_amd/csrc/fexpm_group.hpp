@@ -311,8 +311,11 @@ template <class P> struct OpSplit {
 };
 
 // Exact int8 slices of one FP64 operand (rows of a walk's plane, optionally
-// the concatenation of two planes), four rows per workgroup.
-constexpr int GROUP_OZ_SLICES = 6;
+// the concatenation of two planes), four rows per workgroup.  Five slices
+// (34 bits of each row's largest element): on the 256-walk batch with
+// uneven orders 64..2048 the same worst infidelity as six (2.0e-11 /
+// 2.5e-11) with the FP64 class 67 -> 53 ms; four slices left 2.2e-8.
+constexpr int GROUP_OZ_SLICES = 5;
 struct OpOzSlice {
   static constexpr int NT = 256;
   struct Prob {

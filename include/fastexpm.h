@@ -90,7 +90,9 @@ enum fexpm_option {
                                     coprime moduli (each row rounded to 43 bits of its largest value,
                                     exact int32 products, Chinese-remainder reconstruction; products
                                     with k > 20480 take 6 slices); S = 4..6 = S exact int8 slices per
-                                    value (about 7S - 1 bits) with int32 accumulation; 0 = FP64 MFMA */
+                                    value (about 7S - 1 bits) with int32 accumulation; 0 = FP64 MFMA.
+                                    Batches (fexpm_batch_set_option): 5 (default; grouped products
+                                    from 5 slices) or 0 */
   FEXPM_OPT_SPECTRAL = 12,       /* structured input (A = iG, G real symmetric, hence normal), degree 13:
                                     1 = s from the spectral-radius bound min_k ||B^k||_1^(1/k), k = 1, 2,
                                     4, 6, of the shifted B (default; single runs), 0 = s from ||B||_1
