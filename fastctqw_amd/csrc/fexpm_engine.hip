@@ -160,11 +160,22 @@ static double bessel_tail_bound(double theta, int m0) {
   return (double)(2.0L * s);
 }
 
-static ChebSeries cheb_series(double theta) {
+// Tail bound of the truncated series for a result squared s times: 2^-54
+// for FP64 results; for FP32 results (complex64 / float32 engines and
+// batches) 2^-(24 + s), so that the truncation error of each eigenvalue's
+// factor, multiplied by at most 2^s in the squarings, stays at the FP32
+// unit roundoff of the output (as Higham's single-precision theta_m do for
+// the Pade approximant).  At N = 10^4 (theta = 4.08, s = 8): degree 24 -> 16,
+// two fewer products.
+static double cheb_tail(int s, bool fp32_result) {
+  return fp32_result ? std::max(0x1p-54, std::ldexp(1.0, -(24 + std::max(s, 0)))) : 0x1p-54;
+}
+
+static ChebSeries cheb_series(double theta, double tail = 0x1p-54) {
   ChebSeries c;
   const int dmax = 2 * (kChebMax - 1);
-  while (c.dc < dmax && bessel_tail_bound(theta, c.dc + 2) > 0x1p-54) c.dc += 2;
-  while (c.ds < dmax - 1 && bessel_tail_bound(theta, c.ds + 2) > 0x1p-54) c.ds += 2;
+  while (c.dc < dmax && bessel_tail_bound(theta, c.dc + 2) > tail) c.dc += 2;
+  while (c.ds < dmax - 1 && bessel_tail_bound(theta, c.ds + 2) > tail) c.ds += 2;
   const long double th = theta;
   const int nc = c.dc / 2 + 1, ns = (c.ds - 1) / 2 + 1;
   long double acc[kChebMax] = {}, prev[kChebMax + 1] = {}, cur[kChebMax + 1] = {}, next[kChebMax + 1] = {};
@@ -1306,9 +1317,9 @@ template <typename W> struct Pipeline : PipelineBase {
   // a combination too (c Y^4 + block).  The factors commute (all are
   // polynomials in B), so every product is symmetric.  No solve.  Returns
   // the planes holding cos X and sin X (real parts).
-  std::pair<Plane2<W>*, Plane2<W>*> cheb_structured(double theta, double R, hipStream_t st) {
+  std::pair<Plane2<W>*, Plane2<W>*> cheb_structured(double theta, double R, double tail, hipStream_t st) {
     ensure_stage_buffers(false);
-    const ChebSeries cs = cheb_series(theta);
+    const ChebSeries cs = cheb_series(theta, tail);
     const Plane2<W>* pw[4] = {nullptr, &p2, &p4, &p6};
     const Plane2<W>& mul = x1;
     const double rinv = R > 0.0 ? 1.0 / R : 0.0;
@@ -1961,12 +1972,13 @@ template <typename W> struct Pipeline : PipelineBase {
   // X = Im(A - mu I) / 2^s and the powers X^2, X^4, X^6, X^8 that the
   // polynomial of cheb_series(theta) reads (p2, p4, p6, x1).
   template <typename Tin>
-  void front_cheb(const Tin* d_a, double mu_re, double mu_im, double scale, double theta, hipStream_t st) {
+  void front_cheb(const Tin* d_a, double mu_re, double mu_im, double scale, double theta, double tail,
+                  hipStream_t st) {
     ensure_stage_buffers(false);
     dim3 grid((n + 255) / 256, n);
     prepare_planar<Tin, W><<<grid, 256, 0, st>>>(d_a, n, a.r, a.i, ld, mu_re, mu_im, scale, 1);
     FX_LAUNCH();
-    const ChebSeries cs = cheb_series(theta);
+    const ChebSeries cs = cheb_series(theta, tail);
     const int nc = cs.dc / 2 + 1, ns = (cs.ds - 1) / 2 + 1;
     const int need = (nc > 4 || ns > 4) ? 4 : std::min(3, std::max(nc, ns) - 1);
     if (need >= 1) sym_real(a, a, p2, st);
@@ -2140,8 +2152,9 @@ template <typename W> struct Pipeline : PipelineBase {
       if (P.cheb) {
         // Chebyshev polynomial from the powers formed by spectral_plan (or
         // here, for 1-norm plans); no solve
-        if (!P.spectral) h->front_cheb<Tin>(d_a, P.mu_re, P.mu_im, scale, cheb_theta(P), st);
-        auto cs = h->cheb_structured(cheb_theta(P), cheb_radius(P), st);
+        const double tail = cheb_tail(P.s, std::is_same<W, float>::value);
+        if (!P.spectral) h->front_cheb<Tin>(d_a, P.mu_re, P.mu_im, scale, cheb_theta(P), tail, st);
+        auto cs = h->cheb_structured(cheb_theta(P), cheb_radius(P), tail, st);
         if (timing) FX_HIP(hipEventRecord(ev[2], st));
         FX_HIP(hipMemsetAsync(&h->scal->info, 0, sizeof(int), st));
         dh = h->solution_tail(cs.first->r, cs.second->r, h->ld, nullptr, nullptr, 0, 0, st);
@@ -2163,8 +2176,9 @@ template <typename W> struct Pipeline : PipelineBase {
       bool done = false;
       if constexpr (std::is_same<W, double>::value) {
         if (P.cheb) {
-          if (!P.spectral) front_cheb<Tin>(d_a, P.mu_re, P.mu_im, scale, cheb_theta(P), st);
-          auto cs = cheb_structured(cheb_theta(P), cheb_radius(P), st);
+          const double tail = cheb_tail(P.s, std::is_same<W, float>::value);
+          if (!P.spectral) front_cheb<Tin>(d_a, P.mu_re, P.mu_im, scale, cheb_theta(P), tail, st);
+          auto cs = cheb_structured(cheb_theta(P), cheb_radius(P), tail, st);
           if (timing) FX_HIP(hipEventRecord(ev[2], st));
           dc = solution_tail(cs.first->r, cs.second->r, ld, nullptr, nullptr, 0, 0, st);
           done = true;
@@ -2259,7 +2273,7 @@ template <typename W> struct Pipeline : PipelineBase {
     S.fp64_moduli = S.fp64_slices == CRT64_K ? crt64_for(P.spectral) : 0;
     S.reserved3 = 0;
     if (P.cheb && !P.nonfinite) {
-      const ChebSeries cs = cheb_series(cheb_theta(P));
+      const ChebSeries cs = cheb_series(cheb_theta(P), cheb_tail(P.s, std::is_same<W, float>::value));
       S.chebyshev = std::max(cs.dc, cs.ds);
     } else {
       S.chebyshev = 0;
@@ -3722,7 +3736,7 @@ static void batch_run_grouped(fexpm_batch* b, const void* psi0, void* psi) {
     S.fp64_slices = b->dtype == FEXPM_COMPLEX64 ? G.oz_slices : 0;
     S.fp64_slices_scope = S.fp64_slices ? 1 : 0;
     if (G.cheb) {
-      const ChebSeries cs = cheb_series(P.norm * std::ldexp(1.0, -P.s));
+      const ChebSeries cs = cheb_series(P.norm * std::ldexp(1.0, -P.s), cheb_tail(P.s, b->dtype == FEXPM_COMPLEX64));
       S.chebyshev = std::max(cs.dc, cs.ds);
     }
   }

@@ -1394,7 +1394,7 @@ template <typename Tin, typename W> struct GroupRunner : GroupRunnerBase {
       const GroupPlan& P = plans[k];
       Walk& w = wk[k];
       const double theta = P.norm * std::ldexp(1.0, -P.s);
-      w.cs = cheb_series(theta);
+      w.cs = cheb_series(theta, cheb_tail(P.s, std::is_same<W, float>::value));
       w.rinv = theta > 0.0 ? 1.0 / theta : 0.0;
       w.r2 = w.rinv * w.rinv;
       w.r8 = w.r2 * w.r2 * w.r2 * w.r2;

@@ -62,12 +62,21 @@ def _tail_bound(theta: float, m0: int) -> float:
     return 2.0 * s
 
 
-def cheb_degrees(theta: float):
-    """(dc, ds): least even / odd degrees with tail bounds at most 2^-54."""
+def cheb_tail(s: int, fp32_result: bool) -> float:
+    """Tail bound for a result squared s times: 2^-54 for FP64 results,
+    max(2^-54, 2^-(24 + s)) for FP32 results, so that the truncation error of
+    each eigenvalue's factor times the 2^s of the squarings stays at the FP32
+    unit roundoff 2^-24 (the engine's rule for complex64 / float32)."""
+    return max(TAIL, 2.0 ** -(24 + max(int(s), 0))) if fp32_result else TAIL
+
+
+def cheb_degrees(theta: float, tail: float = TAIL):
+    """(dc, ds): least even / odd degrees with tail bounds at most `tail`
+    (2^-54 by default)."""
     dc, ds = 0, 1
-    while dc < 78 and _tail_bound(theta, dc + 2) > TAIL:
+    while dc < 78 and _tail_bound(theta, dc + 2) > tail:
         dc += 2
-    while ds < 77 and _tail_bound(theta, ds + 2) > TAIL:
+    while ds < 77 and _tail_bound(theta, ds + 2) > tail:
         ds += 2
     return dc, ds
 

@@ -125,7 +125,7 @@ def test_spectral_scaling(fx, dt, case):
     max(cheb_degrees(rho / 2^s)) (option OPT_CHEBYSHEV, default); with the
     option off the Pade approximant and its solve give U to the same
     tolerance."""
-    from oracle.chebcs import cheb_degrees
+    from oracle.chebcs import cheb_degrees, cheb_tail
 
     core = fx._fastexpm_core
     a, defl = _spectral_case(case)
@@ -144,7 +144,8 @@ def test_spectral_scaling(fx, dt, case):
         assert stats["m"] == 13 and _same_s(stats, st), (stats, st["s"], st["rho"])
         assert stats["s_onenorm"] == st["s_onenorm"] and stats["s"] <= stats["s_onenorm"]
         assert abs(stats["rho_bound"] - st["rho"]) <= 1e-9 * st["rho"], (stats["rho_bound"], st["rho"])
-        assert stats["chebyshev"] == max(cheb_degrees(stats["rho_bound"] * 2.0 ** -stats["s"])), stats
+        assert stats["chebyshev"] == max(cheb_degrees(stats["rho_bound"] * 2.0 ** -stats["s"],
+                                                      cheb_tail(stats["s"], dt == np.complex64))), stats
         err = rel_fro(ref, u)
         print(f"  s={stats['s']} (1-norm rule {stats['s_onenorm']}) rho={stats['rho_bound']:.2f} "
               f"Chebyshev degree {stats['chebyshev']} relF={err:.3e} tol={tol:.1e}", flush=True)
@@ -183,7 +184,7 @@ def test_chebyshev_route(fx, dt, n):
     (to the products' rounding: FP64-accurate residue products for
     complex64) and U against the eigendecomposition oracle and against the
     Pade route."""
-    from oracle.chebcs import cheb_cs, cheb_degrees
+    from oracle.chebcs import cheb_cs, cheb_degrees, cheb_tail
 
     core = fx._fastexpm_core
     graphs = {"er": laplacian(erdos_renyi(n, 0.5, seed=21 + n)), "complete": laplacian(complete_graph(n))}
@@ -201,7 +202,8 @@ def test_chebyshev_route(fx, dt, n):
             u = e.run(a)
             stats = e.stats()
             theta = stats["rho_bound"] * 2.0 ** -stats["s"]
-            assert stats["spectral"] == 1 and stats["chebyshev"] == max(cheb_degrees(theta)), stats
+            assert stats["spectral"] == 1, stats
+            assert stats["chebyshev"] == max(cheb_degrees(theta, cheb_tail(stats["s"], dt == np.complex64))), stats
             assert _same_s(stats, st), (stats["s"], st["s"])
             seen_s0 |= stats["s"] == 0
             err = rel_fro(ref, u)
@@ -220,7 +222,9 @@ def test_chebyshev_route(fx, dt, n):
             u1 = e.run(a)
             s1 = e.stats()
             e._backend.set_option(core.OPT_SPECTRAL, 1)
-            assert s1["spectral"] == 0 and s1["chebyshev"] == max(cheb_degrees(s1["norm"] * 2.0 ** -s1["s"])), s1
+            assert s1["spectral"] == 0, s1
+            assert s1["chebyshev"] == max(cheb_degrees(s1["norm"] * 2.0 ** -s1["s"],
+                                                       cheb_tail(s1["s"], dt == np.complex64))), s1
             err1 = rel_fro(ref, u1)
             print(f"  s={stats['s']} theta={theta:.3f} degree={stats['chebyshev']} relF={err:.3e} "
                   f"value relF={errx:.3e} vs Pade {diff:.3e}; 1-norm plan s={s1['s']} degree={s1['chebyshev']} "
@@ -1151,7 +1155,7 @@ def test_batch_chebyshev_route(fx, dt):
     oracle.chebcs; U against the eigen oracle and against the Pade route
     (option OPT_CHEBYSHEV = 0) of the same batch."""
     from fastctqw_amd.graphs import c4_walks
-    from oracle.chebcs import cheb_degrees
+    from oracle.chebcs import cheb_degrees, cheb_tail
 
     core = fx._fastexpm_core
     walks = c4_walks(count=10, seed=4300, nmin=16, nmax=400)
@@ -1165,7 +1169,8 @@ def test_batch_chebyshev_route(fx, dt):
         st = b.stats(i)
         ref_st = pade_expm(a.astype(np.complex128), stages=True)
         assert st["m"] == ref_st["m"] and st["s"] == ref_st["s"], (i, st)
-        assert st["chebyshev"] == max(cheb_degrees(st["norm"] * 2.0 ** -st["s"])), (i, st)
+        assert st["chebyshev"] == max(cheb_degrees(st["norm"] * 2.0 ** -st["s"],
+                                                   cheb_tail(st["s"], dt == np.complex64))), (i, st)
         err = rel_fro(_eig_ref(a), u)
         print(f"  walk {i} n={a.shape[0]} s={st['s']} degree={st['chebyshev']} relF={err:.2e}", flush=True)
         assert err <= tol_fn(st["s"]), (i, err)

@@ -312,11 +312,14 @@ def test_group_program_matches_oracle(dt, precision, slices, cheb):
         n = a.shape[0]
         u = it.result(off, n)
         ref = pade_expm(a)
-        assert rel_fro(ref, u) < 1e-12, (i, n, plans[i], rel_fro(ref, u))
+        # complex64 programs truncate the Chebyshev series at 2^-(24 + s)
+        # (each eigenvalue's factor within 2^-24 after the squarings)
+        tol = 2.0 ** -23 if (cheb and dt == COMPLEX64) else 1e-12
+        assert rel_fro(ref, u) < tol, (i, n, plans[i], rel_fro(ref, u))
         lam, V = np.linalg.eigh(a.imag)
-        assert rel_fro((V * np.exp(1j * lam)) @ V.T, u) < 1e-11, i
+        assert rel_fro((V * np.exp(1j * lam)) @ V.T, u) < max(tol, 1e-11), i
         psi = it.reg[8][0][voff:voff + n]
-        assert np.linalg.norm(psi - ref @ psi0[i]) < 1e-12 * max(1.0, np.linalg.norm(psi)), i
+        assert np.linalg.norm(psi - ref @ psi0[i]) < tol * max(1.0, np.linalg.norm(psi)), i
         off += n * n
         voff += n
 

@@ -435,3 +435,28 @@ def test_cheb_route_matches_pade_oracle():
     assert st["spectral"] and st["deflated"]
     c, s, _ = cheb_cs(st["B"], st["rho"], st["s"])
     assert np.linalg.norm(c + 1j * s - st["X"], 2) <= 1e-13
+
+
+def test_cheb_tail_rule():
+    """The FP32-result tail rule: 2^-(24 + s) (never below 2^-54), so that
+    2^s times the tail is the FP32 unit roundoff; the degrees it gives are
+    the least ones whose tail bounds meet it, and at the N = 10^4 walk's
+    theta = 4.08, s = 8 they are 16 / 17 (24 / 23 at 2^-54)."""
+    from oracle.chebcs import TAIL, _tail_bound, cheb_degrees, cheb_tail
+
+    assert cheb_tail(8, False) == TAIL == 2.0 ** -54
+    assert cheb_tail(8, True) == 2.0 ** -32 and cheb_tail(0, True) == 2.0 ** -24
+    assert cheb_tail(40, True) == TAIL
+    for s in (0, 3, 8, 13, 20):
+        assert 2.0 ** s * cheb_tail(s, True) <= 2.0 ** -24
+    for theta in (0.1, 1.0, 2.5, 4.08, 5.37):
+        for s in (2, 8, 13):
+            tail = cheb_tail(s, True)
+            dc, ds = cheb_degrees(theta, tail)
+            assert _tail_bound(theta, dc + 2) <= tail and _tail_bound(theta, ds + 2) <= tail
+            assert dc == 0 or _tail_bound(theta, dc) > tail
+            assert ds == 1 or _tail_bound(theta, ds) > tail
+            dc0, ds0 = cheb_degrees(theta)
+            assert dc <= dc0 and ds <= ds0
+    assert cheb_degrees(4.08, cheb_tail(8, True)) == (16, 17)
+    assert cheb_degrees(4.08) == (24, 23)
